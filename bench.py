@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the coupled platelet step on one MI355X.
+
+Workload: single platelet in shear, 256x128x128 D3Q19 Couette flow (wall
+velocity 0.05, tau 0.7), one 2048-element SEM platelet with 512 receptors over
+4096 wall ligands, Bell bonds, FP32 storage.  A step is one full coupled step:
+cell lists, membrane sums, bonds, forces, IB spreading, fused LB
+collide-stream, IB interpolation, marker update, force-field clear.  Inputs are
+resident in HBM before the timed region; the timed region is K hipGraph step
+launches between two device synchronisations.
+
+Prints ONE JSON line.  value = coupled-step MLUPS (lattice cells x steps / s).
+roofline: the fused LB kernel (the dominant kernel), algorithmic bytes
+152 B/cell (FP32 storage: 19 x 4 B read + 19 x 4 B write) over its mean
+duration measured with HIP events on the handle's stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {0: 152, 1: 76, 2: 76}
+METRIC = "D3Q19 MLUPS, % HBM peak; coupled steps/s FP16-store vs FP32 at stated tol"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed_run(pfg, scene, store, steps, warmup, fluid_only=False):
+    p = scene.params
+    p.store = store
+    s = pfg.Solver(p)
+    s.load(scene)
+    s.set_fluid_only(fluid_only)
+    s.step(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    return s, dt
+
+
+def read_traffic():
+    """HBM bytes per LB launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "lb_fused_shear256_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split("=")[1].split()[0])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def cpu_baseline(pfg, steps=3):
+    """FP64 single-thread oracle (a port of the same algorithm) on a bounded
+    sample of the same workload: `steps` coupled steps of the 256x128x128
+    shear scene."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    scene = pfg.Scene(pfg.SCENE_SHEAR)
+    o = pyoracle.Oracle(scene.params, scene=scene)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return {"value": round(scene.N * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} coupled steps of the 256x128x128 shear workload, FP64 "
+                      f"oracle, 1 thread ({dt:.1f} s)"}
+
+
+def fp16_accuracy(pfg):
+    """Relative L2 velocity error of FP16 deviation storage against the FP64
+    oracle: 64x32x32 body-force channel, tau 0.8, F = 1e-6, 300 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    p = pfg.default_params(nx=64, ny=32, nz=32, tau=0.8, store=pfg.STORE_F16)
+    p.wall[1] = p.wall[2] = pfg.WALL_BOUNCE
+    p.body_force[0] = 1e-6
+    with pfg.Solver(p) as s:
+        s.set_fluid_only(True)
+        s.step(300)
+        _, ux, uy, uz = s.fields()
+    L = pyoracle.lattice(p)
+    f = pyoracle.init_eq(L)
+    for _ in range(300):
+        f = pyoracle.lb_step(L, f)
+    _, ox, oy, oz = pyoracle.fields(L, f)
+    num = np.sqrt(((ux - ox) ** 2 + (uy - oy) ** 2 + (uz - oz) ** 2).sum())
+    den = np.sqrt((ox ** 2 + oy ** 2 + oz ** 2).sum())
+    return float(num / den)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp16", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    import platelet_fiber_gpu_amd as pfg
+    if pfg.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    K, W = args.steps, args.warmup
+    scene = pfg.Scene(pfg.SCENE_SHEAR)
+    N = scene.N
+    log(f"workload 256x128x128 shear, {scene.n_markers} elements, "
+        f"{scene.rec_tri.size} receptors, {scene.lig_x.size} ligands; K={K} W={W}")
+
+    s32, dt32 = timed_run(pfg, scene, pfg.STORE_F32, K, W)
+    ms_step = dt32 * 1e3 / K
+    mlups = N * K / dt32 / 1e6
+    log(f"fp32 coupled: {ms_step:.4f} ms/step, {mlups:.0f} MLUPS")
+    # LB kernel duration with HIP events around each LB launch (same stream)
+    prof_steps = min(K, 200)
+    tot_ms, lb_ms = s32.profile(prof_steps)
+    lb_avg_ms = lb_ms / prof_steps
+    achieved = BYTES_PER_CELL[0] * N / (lb_avg_ms * 1e-3) / 1e9
+    info = s32.step(0)
+    s32.close()
+    log(f"fp32 LB kernel: {lb_avg_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s")
+
+    # LB alone (no marker pipeline) in both storage formats
+    sl, dtl = timed_run(pfg, scene, pfg.STORE_F32, K, W, fluid_only=True)
+    sl.close()
+    lb32 = N * K / dtl / 1e6
+    res16 = {}
+    if not args.no_fp16:
+        s16, dt16 = timed_run(pfg, scene, pfg.STORE_F16, K, W)
+        _, lb16_ms = s16.profile(prof_steps)
+        s16.close()
+        sl16, dtl16 = timed_run(pfg, scene, pfg.STORE_F16, K, W, fluid_only=True)
+        sl16.close()
+        err16 = fp16_accuracy(pfg)
+        res16 = {"coupled_mlups": round(N * K / dt16 / 1e6, 1),
+                 "coupled_steps_per_s": round(K / dt16, 1),
+                 "lb_only_mlups": round(N * K / dtl16 / 1e6, 1),
+                 "lb_kernel_us": round(lb16_ms / prof_steps * 1e3, 2),
+                 "lb_kernel_gbs": round(BYTES_PER_CELL[1] * N / (lb16_ms / prof_steps * 1e-3) / 1e9, 1),
+                 "speedup_coupled_vs_fp32": round(dt32 / dt16, 3),
+                 "speedup_lb_only_vs_fp32": round(dtl / dtl16, 3),
+                 "rel_l2_u_vs_fp64": err16,
+                 "rel_l2_u_case": "64x32x32 channel, tau 0.8, F 1e-6, 300 steps; tol 1e-3"}
+        log(f"fp16: {res16}")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (FP64 oracle, 1 thread) ...")
+        cpu = cpu_baseline(pfg)
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene)",
+        "config": {"workload": "single platelet in shear, 256x128x128 D3Q19 Couette, "
+                               "2048 elements, 512 receptors, 4096 ligands, FP32 storage",
+                   "cells": N, "store": "f32"},
+        "coupled_steps_per_s": round(K / dt32, 1),
+        "lb_only_mlups_f32": round(lb32, 1),
+        "bonds_at_end": info["total_bonds"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic(),
+                     "kernel": "lb_fused_kernel<F32>", "bytes_per_cell": 152,
+                     "lb_kernel_us": round(lb_avg_ms * 1e3, 2),
+                     "mlups_kernel": round(N / (lb_avg_ms * 1e-3) / 1e6, 1)},
+        "fp16": res16,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,297 @@
+/*
+ * pfg.h -- C ABI of the MI355X coupled platelet-fibrin step.
+ *
+ * One handle owns one lattice (D3Q19, SoA planes in HBM), one marker set
+ * (platelet SEM elements and fibrin nodes), their topology and one HIP stream.
+ * Every call returns a pfg_status; nothing calls exit().  Handles are
+ * thread-compatible (one thread per handle at a time), not thread-safe.
+ *
+ * The entry points replace the reference's CUDA glue layer
+ * (semLB.cu, `extern "C"` block at semLB.cu:48-1036):
+ *
+ *   pfg_create / pfg_destroy          fluid_allocGPUKernel  semLB.cu:111
+ *                                     sem_allocGPUKernel    semLB.cu:749
+ *                                     fiber_allocGPUKernel  semLB.cu:562
+ *                                     *_releaseGPUKernel    semLB.cu:534,708,1005
+ *   pfg_init_equilibrium, pfg_upload_f, pfg_upload_flags
+ *                                     fluid_initGPUKernel(aFlag=1) semLB.cu:154
+ *   pfg_load_model                    sem_initGPUKernel     semLB.cu:865
+ *                                     fiber_init_GPUKernel  semLB.cu:609
+ *   pfg_step                          sem_invokeGPUKernel_Force semLB.cu:441
+ *                                     + fluid_invokeGPUKernel   semLB.cu:303
+ *   pfg_download_*                    fluid_initGPUKernel(aFlag=0) semLB.cu:154
+ *                                     sem_copyGPUKernel     semLB.cu:979
+ *                                     fiber_copy_GPUKernel  semLB.cu:669
+ *
+ * Units: everything on the device is in lattice units (dx = dt = 1).  The
+ * reference's physical-unit parameter block hostSEMparameters[0..19]
+ * (gpuSEM_LB.c:1206-1233) is mapped onto pfg_params by pfg_params_from_sem().
+ *
+ * Array layouts (host side, caller owned):
+ *   lattice scalar field  [nz][ny][nx], x fastest (reference: pitched [z][y][x]
+ *                          planes, semLB.cu:127-135)
+ *   distributions          [19][nz][ny][nx], direction order of LB_kernel.cu:40-47
+ *   markers                SoA arrays of length n_markers
+ */
+#ifndef PFG_H
+#define PFG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pfg_status {
+  PFG_OK = 0,
+  PFG_ERR_ARG = 1,      /* invalid argument (sizes, null pointers, modes)        */
+  PFG_ERR_ALLOC = 2,    /* device or host allocation failed                      */
+  PFG_ERR_DEVICE = 3,   /* no usable HIP device / HIP runtime error              */
+  PFG_ERR_LAUNCH = 4,   /* kernel launch or execution error                      */
+  PFG_ERR_STATE = 5,    /* call out of order (e.g. step before pfg_load_model)   */
+  PFG_ERR_IO = 6        /* file read/write failure (host driver helpers)          */
+} pfg_status;
+
+/* Storage format of the distributions in HBM; collision arithmetic is FP32. */
+typedef enum pfg_store {
+  PFG_STORE_F32 = 0,    /* (f_i - w_i) as float: 76 B per cell per copy          */
+  PFG_STORE_F16 = 1,    /* (f_i - w_i) * 2^scale as IEEE binary16: 38 B/cell     */
+  PFG_STORE_BF16 = 2    /* (f_i - w_i) as bfloat16: 38 B/cell                    */
+} pfg_store;
+
+/* Boundary treatment along one lattice axis. */
+typedef enum pfg_wall {
+  PFG_WALL_PERIODIC = 0, /* periodic wrap (LB_kernel.cu:1347-1352)                */
+  PFG_WALL_BOUNCE = 1    /* halfway bounce-back walls half a cell outside cells 0
+                            and n-1, each wall with its own velocity
+                            (LB_kernel_old_08102011.cu:1155-1224)                 */
+} pfg_wall;
+
+/* Force law of the fibrin links. */
+typedef enum pfg_fiber_law {
+  PFG_FIBER_WLC = 0,     /* worm-like chain, fiber.cpp:710-733 / network.cpp:378-403 */
+  PFG_FIBER_LINEAR = 1   /* Hookean, network.cpp:352-374 (#if 0 branch)            */
+} pfg_fiber_law;
+
+/* How fibrin nodes move. */
+typedef enum pfg_fiber_move {
+  PFG_FIBER_IB = 0,      /* advected with the interpolated fluid velocity, like
+                            platelet elements (sem_kernel.cu:2087-2148)           */
+  PFG_FIBER_KICK = 1     /* damped kick v = e^{-g dt} v + F (1-e^{-g dt})/g, then
+                            x += v dt (network.cpp:839-881); not fluid-coupled    */
+} pfg_fiber_move;
+
+/* Marker kinds. */
+enum {
+  PFG_MARKER_ELEMENT = 0, /* platelet SEM element                                 */
+  PFG_MARKER_FIBER = 1,   /* free fibrin node                                     */
+  PFG_MARKER_FIXED = 2    /* anchored fibrin node (fiber.cpp:350-388): never moves,
+                             never spread                                          */
+};
+
+typedef struct pfg_params {
+  /* ---- lattice ---- */
+  int32_t nx, ny, nz;          /* cells per axis                                  */
+  int32_t store;               /* pfg_store                                       */
+  int32_t f16_scale_log2;      /* PFG_STORE_F16: deviations scaled by 2^this      */
+  int32_t wall[3];             /* pfg_wall per axis x, y, z                       */
+  float wall_u_lo[3][3];       /* [axis][component] velocity of the lower wall    */
+  float wall_u_hi[3][3];       /* [axis][component] velocity of the upper wall    */
+  float tau;                   /* BGK relaxation time (reference OMEGA, [13])     */
+  float body_force[3];         /* uniform body force density, lattice units      */
+  int32_t use_flags;           /* 1: a solid-cell flag array is uploaded          */
+  /* ---- immersed boundary ---- */
+  float ib_force_scale;        /* marker force -> lattice force factor (F0 chain
+                                  of LB_kernel.cu:1158, 1298); 1 in lattice units */
+  float marker_dt;             /* position update dt (reference dt/T0,
+                                  sem_kernel.cu:2111); 1 in lattice units         */
+  /* ---- SEM element mechanics ---- */
+  float morse_u0, morse_rho, morse_req, morse_cutoff;
+  float k_spring;              /* mesh-edge Hookean springs (reference k_spring [2],
+                                  sem_kernel.cu:356-396)                          */
+  float ks, kv;                /* area / volume constraints ([17], [18],
+                                  sem_kernel.cu:1640-1749)                        */
+  float wall_rep_a, wall_rep_kappa, wall_rep_hc; /* wall repulsion on the lowest
+                                  element of each platelet (sem_kernel.cu:409-430):
+                                  f = a*kappa*e^{-kappa h}/(1-e^{-kappa h}), h<=hc  */
+  float wall_z;                /* z of the ligand-carrying wall (bottom wall plane) */
+  /* ---- receptor-ligand bonds (Bell model) ---- */
+  float k_on;                  /* formation rate within reach (reference Kf0 [3])  */
+  float k_off0;                /* unstressed rupture rate (reference Kr0 [4])      */
+  float bond_k, bond_l0;       /* bond spring constant and rest length            */
+  float bond_reach;            /* a free receptor can bind ligands within this    */
+  float bell_xb_kT;            /* Bell force sensitivity x_beta / kBT              */
+  float detach_rate;           /* detach hazard per step when no bond holds
+                                  (reference gamma = SHEAR_RATE*dt, semLB.cu:451)  */
+  uint64_t seed;               /* counter-RNG key (reference RANDOM_SEED [1])      */
+  /* ---- fibrin ---- */
+  int32_t fiber_law;           /* pfg_fiber_law                                    */
+  int32_t fiber_move;          /* pfg_fiber_move                                   */
+  float fiber_k;               /* WLC: kB*T of fiber.cpp:722 (CC = fiber_k / l0, the
+                                  persistence length being l0); linear: Y*A       */
+  float fiber_xmax;            /* WLC strain clamp x=(l-l0)/l0 <= xmax (< 1)        */
+  float fiber_gamma;           /* damping for PFG_FIBER_KICK                       */
+  int32_t fiber_ref_sign;      /* 1: apply link forces with the reference's sign
+                                  (network.cpp:405-411 / fiber.cpp:693-695 push the
+                                  ends of a stretched link apart); 0: restoring    */
+  /* ---- cell lists ---- */
+  int32_t bin_cells;           /* bin edge in lattice cells (>= every cutoff)      */
+  /* ---- reference parameter block, kept for output/provenance ---- */
+  float sem[20];               /* hostSEMparameters[0..19], gpuSEM_LB.c:1206-1233  */
+} pfg_params;
+
+/* Host description of the immersed structures; all arrays caller-owned and only
+ * read during pfg_load_model.  Counts may be zero (then pointers may be NULL). */
+typedef struct pfg_model {
+  int32_t n_markers;
+  const float *x, *y, *z;          /* unwrapped positions                         */
+  const int32_t *kind;             /* PFG_MARKER_*                                 */
+  const int32_t *body;             /* platelet index for elements, -1 otherwise    */
+  /* platelets: elements and triangles of platelet p are contiguous */
+  int32_t n_platelets;
+  const int32_t *plt_elem_start;   /* n_platelets + 1 marker offsets               */
+  const int32_t *plt_tri_start;    /* n_platelets + 1 triangle offsets             */
+  int32_t n_tri;
+  const int32_t *tri;              /* 3*n_tri marker indices, outward-wound        */
+  const double *tri_S0;            /* rest area per triangle                       */
+  const double *plt_S0;            /* rest total area per platelet                 */
+  const double *plt_V0;            /* rest volume per platelet (reference sign)    */
+  int32_t n_edges;
+  const int32_t *edge;             /* 2*n_edges marker indices                     */
+  const float *edge_l0;            /* rest lengths                                 */
+  /* receptors on triangles, position (1-r1)x0 + r1(1-r2)x1 + r1 r2 x2 */
+  int32_t n_receptors;
+  const int32_t *rec_tri;
+  const float *rec_r1, *rec_r2;
+  /* static ligand sites (e.g. vWF on the wall); lig_z NULL means z = wall_z */
+  int32_t n_wall_ligands;
+  const float *lig_x, *lig_y, *lig_z;
+  int32_t fibrin_ligands;          /* 1: free fibrin nodes are ligands as well     */
+  /* fibrin links */
+  int32_t n_links;
+  const int32_t *link;             /* 2*n_links marker indices                     */
+  const float *link_l0;
+} pfg_model;
+
+/* Per-call step report. */
+typedef struct pfg_step_info {
+  int64_t step;                    /* steps done since pfg_load_model               */
+  int32_t total_bonds;             /* bonds held after the last step                */
+  int32_t detached;                /* 1 once the detach test fired                  */
+  int64_t detach_step;             /* step index at which it fired, else -1         */
+  int64_t near_threshold;          /* bond decisions within eps of their threshold  */
+  int64_t bond_events;             /* formations + ruptures so far                  */
+} pfg_step_info;
+
+/* Stages of one coupled step, in execution order (pfg_run_stage). */
+typedef enum pfg_stage {
+  PFG_STAGE_CELLS = 0,     /* marker bins, counting sort, per-bin index order     */
+  PFG_STAGE_MEMBRANE = 1,  /* triangle areas/normals, FP64 area/volume sums       */
+  PFG_STAGE_BONDS = 2,     /* Bell rupture/formation decisions and bond forces    */
+  PFG_STAGE_FORCES = 3,    /* per-marker force gather (Morse, springs, area/volume,
+                              wall, bonds, fibrin links)                          */
+  PFG_STAGE_SPREAD = 4,    /* IB force spreading to the lattice force field       */
+  PFG_STAGE_LB = 5,        /* fused stream + bounce-back + moments + BGK/Guo      */
+  PFG_STAGE_INTERP = 6,    /* IB velocity interpolation to markers                */
+  PFG_STAGE_UPDATE = 7,    /* marker position update                              */
+  PFG_STAGE_CLEAR = 8,     /* clear the force field tiles touched this step       */
+  PFG_STAGE_COUNT = 9
+} pfg_stage;
+
+typedef struct pfg_handle pfg_handle;
+
+/* ---- library ---- */
+const char *pfg_version(void);
+const char *pfg_status_string(pfg_status s);
+pfg_status pfg_device_count(int *count);
+/* sizeof of the ABI structs, for binding checks: 0 pfg_params, 1 pfg_model,
+ * 2 pfg_step_info, 3 pfg_scene_opts; 0 for an unknown id. */
+size_t pfg_struct_size(int which);
+
+/* Default parameters: 64x32x32, tau 0.8, all walls periodic, FP32 storage. */
+void pfg_params_default(pfg_params *p);
+
+/* Map the reference's FP32 parameter block hostSEMparameters[0..19]
+ * (gpuSEM_LB.c:1206-1233; index names sem_kernel.cu:58-76) and lattice
+ * geometry onto p.  Fields the reference does not carry keep their values. */
+pfg_status pfg_params_from_sem(const float *host_sem_parameters, float dx,
+                               float dt, int lx, int ly, int lz, pfg_params *p);
+
+/* ---- lifetime ---- */
+pfg_status pfg_create(const pfg_params *p, int device, pfg_handle **out);
+pfg_status pfg_destroy(pfg_handle *h);
+pfg_status pfg_get_params(const pfg_handle *h, pfg_params *out);
+/* Bytes of device memory the handle holds. */
+pfg_status pfg_device_bytes(const pfg_handle *h, size_t *bytes);
+
+/* ---- lattice state ---- */
+/* f = f_eq(rho, u) per cell (gpuSEM_LB.c:1234-1258).  NULL rho means 1,
+ * NULL velocity means 0. */
+pfg_status pfg_init_equilibrium(pfg_handle *h, const float *rho,
+                                const float *ux, const float *uy,
+                                const float *uz);
+pfg_status pfg_upload_f(pfg_handle *h, const float *f);      /* [19][N] */
+pfg_status pfg_download_f(pfg_handle *h, float *f);          /* [19][N] */
+/* Solid-cell flags, 1 byte per cell, nonzero = solid (needs use_flags). */
+pfg_status pfg_upload_flags(pfg_handle *h, const uint8_t *flags);
+/* rho and u = (sum c f + F/2)/rho from the current distributions, F the body
+ * force (LB_kernel.cu:1108-1199).  Any pointer may be NULL. */
+pfg_status pfg_download_fields(pfg_handle *h, float *rho, float *ux, float *uy,
+                               float *uz);
+/* FP64 totals over fluid cells: out[0] = sum rho, out[1..3] = sum rho u. */
+pfg_status pfg_conservation(pfg_handle *h, double out[4]);
+
+/* ---- immersed structures ---- */
+pfg_status pfg_load_model(pfg_handle *h, const pfg_model *m);
+pfg_status pfg_upload_markers(pfg_handle *h, const float *x, const float *y,
+                              const float *z);
+/* Any pointer may be NULL. */
+pfg_status pfg_download_markers(pfg_handle *h, float *x, float *y, float *z,
+                                float *vx, float *vy, float *vz, float *fx,
+                                float *fy, float *fz);
+/* Host-side marker forces/velocities, for stage-level tests. */
+pfg_status pfg_upload_marker_forces(pfg_handle *h, const float *fx,
+                                    const float *fy, const float *fz);
+pfg_status pfg_upload_marker_velocities(pfg_handle *h, const float *vx,
+                                        const float *vy, const float *vz);
+/* Per-platelet FP64 area and volume sums of the last membrane stage. */
+pfg_status pfg_download_membrane(pfg_handle *h, double *S_all, double *V,
+                                 double *tri_S);
+/* Bond state: ligand index per receptor (-1 free; wall ligands 0..n_wall-1,
+ * fibrin-node ligand m is n_wall + m), owner receptor per ligand slot. */
+pfg_status pfg_download_bonds(pfg_handle *h, int32_t *rec_bond,
+                              int32_t *lig_owner, float *rec_force);
+pfg_status pfg_upload_bonds(pfg_handle *h, const int32_t *rec_bond);
+/* Cell list of the last CELLS stage: bin_start[n_bins+1], bin_items[n_markers]. */
+pfg_status pfg_cell_grid(const pfg_handle *h, int32_t dims[3]);
+pfg_status pfg_download_cells(pfg_handle *h, int32_t *bin_start,
+                              int32_t *bin_items);
+/* Lattice force field [3][N] as seen by the LB stage (valid between SPREAD and
+ * LB) and velocity field written for IB (valid between LB and CLEAR). */
+pfg_status pfg_download_aux(pfg_handle *h, float *a3n);
+
+/* ---- stepping ---- */
+/* Run nsteps coupled steps (one hipGraph launch per step when enabled). */
+pfg_status pfg_step(pfg_handle *h, int nsteps, pfg_step_info *info);
+pfg_status pfg_run_stage(pfg_handle *h, int stage);
+pfg_status pfg_set_step(pfg_handle *h, int64_t step);
+pfg_status pfg_set_graph(pfg_handle *h, int enable);
+/* Skip the marker pipeline (pure LB stepping). */
+pfg_status pfg_set_fluid_only(pfg_handle *h, int fluid_only);
+pfg_status pfg_synchronize(pfg_handle *h);
+
+/* ---- measurement ---- */
+/* Time nsteps steps with HIP events on the handle's stream.  ms_total: the
+ * whole steps; ms_lb: sum of the LB-kernel launches (events around each LB
+ * launch, same stream).  Either pointer may be NULL. */
+pfg_status pfg_profile(pfg_handle *h, int nsteps, double *ms_total,
+                       double *ms_lb);
+/* LB-kernel launch tuning: cells per thread along z and rows per block. */
+pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_thread, int block_y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFG_H */

@@ -1,0 +1,1059 @@
+/*
+ * pfg_oracle.c -- FP64 CPU restatement of the coupled step (test infrastructure).
+ *
+ * What pins each function (tests under tests/ hold the checks):
+ *   D3Q19 tables, orc_init_eq   feq moments (sum feq = rho, sum c feq = rho u,
+ *                               sum cc feq = rho uu + rho/3 I) and rest-state
+ *                               fixed point.
+ *   orc_lb_step                 exact mass conservation with walls; momentum gain
+ *                               of exactly N*F per step in a periodic box; Guo
+ *                               source moments (0, (1-1/2tau)F, ...); Poiseuille
+ *                               between plates and Couette profiles against their
+ *                               closed forms; the reference's tensor form of the
+ *                               Guo term (LB_kernel.cu:1254-1296) is coded here and
+ *                               the device uses the expanded form, so agreement
+ *                               pins the algebra.
+ *   orc_phi / spread / interp   partition of unity and sum phi^2 = 3/8 exactly;
+ *                               spread/interp adjointness; the reference's own
+ *                               node-loops-over-markers form (orc_ib_spread_scan)
+ *                               against the stencil form.
+ *   orc_cell_list               brute-force all-pairs neighbour search.
+ *   membrane (area/volume)      closed forms for the unit cube mesh and a right
+ *                               triangle; ellipsoid volume against 4/3 pi abc.
+ *   forces                      Morse: finite differences of U, zero at r_eq;
+ *                               area/volume forces: finite differences of the
+ *                               constraint energies; WLC / linear links against
+ *                               network::getSpringForce compiled from the
+ *                               reference (oracle/_ref) and Newton's third law.
+ *   bonds                       Philox4x32-10 known-answer vectors
+ *                               (Random123); p = 1 - exp(-k dt) closed form.
+ *   update / kick               closed form; kick against network::solverkick
+ *                               compiled from the reference (oracle/_ref).
+ */
+#define _DEFAULT_SOURCE /* M_PI */
+#include "pfg_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../platelet_fiber_gpu_amd/csrc/pfg_rng.h"
+
+/* ---------------- D3Q19 (LB_kernel.cu:40-47) ---------------- */
+
+static const int CXo[19] = {0, 1, -1, 0, 0, 0, 0, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0};
+static const int CYo[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1};
+static const int CZo[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1};
+static const int BBo[19] = {0, 2, 1, 4, 3, 6, 5, 12, 11, 14, 13, 8, 7, 10, 9, 18, 17, 16, 15};
+static const double Wo[19] = {1.0 / 3.0,  1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 18.0,
+                              1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                              1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0,
+                              1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+
+double orc_w(int d) { return Wo[d]; }
+int orc_c(int d, int axis) { return axis == 0 ? CXo[d] : (axis == 1 ? CYo[d] : CZo[d]); }
+int orc_bb(int d) { return BBo[d]; }
+
+void orc_lattice_from_params(const pfg_params *p, const uint8_t *solid, orc_lattice *L) {
+  L->nx = p->nx;
+  L->ny = p->ny;
+  L->nz = p->nz;
+  for (int a = 0; a < 3; ++a) {
+    L->periodic[a] = p->wall[a] == PFG_WALL_PERIODIC;
+    for (int k = 0; k < 3; ++k) {
+      L->wall_u_lo[a][k] = p->wall_u_lo[a][k];
+      L->wall_u_hi[a][k] = p->wall_u_hi[a][k];
+    }
+    L->g[a] = p->body_force[a];
+  }
+  L->tau = p->tau;
+  L->solid = p->use_flags ? solid : NULL;
+}
+
+static inline int64_t cidx(const orc_lattice *L, int x, int y, int z) {
+  return (int64_t)x + (int64_t)L->nx * ((int64_t)y + (int64_t)L->ny * z);
+}
+
+/* feq_d(rho, u) with C = 1 (LB_kernel.cu:1281) */
+static inline double feq(int d, double rho, double ux, double uy, double uz) {
+  const double cu = 3.0 * (CXo[d] * ux + CYo[d] * uy + CZo[d] * uz);
+  return rho * Wo[d] * (1.0 + cu + 0.5 * cu * cu - 1.5 * (ux * ux + uy * uy + uz * uz));
+}
+
+void orc_init_eq(const orc_lattice *L, const double *rho, const double *ux,
+                 const double *uy, const double *uz, double *f) {
+  const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+  for (int64_t i = 0; i < N; ++i) {
+    const double r = rho ? rho[i] : 1.0;
+    const double a = ux ? ux[i] : 0.0, b = uy ? uy[i] : 0.0, c = uz ? uz[i] : 0.0;
+    for (int d = 0; d < 19; ++d) f[d * N + i] = feq(d, r, a, b, c);
+  }
+}
+
+/* Pull one population for cell (x,y,z): f_d(x) = f*_d(x - c_d), or the
+ * bounced f*_bb(d)(x) + 6 w_d (c_d . u_wall) when x - c_d is a wall
+ * (z wall over y wall over x wall at edges) or a solid cell. */
+static double pull(const orc_lattice *L, const double *fin, int64_t N, int x, int y,
+                   int z, int d) {
+  int s[3] = {x - CXo[d], y - CYo[d], z - CZo[d]};
+  const int n[3] = {L->nx, L->ny, L->nz};
+  int wall = -1, side = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (s[a] < 0) {
+      if (L->periodic[a]) s[a] += n[a];
+      else { wall = a; side = 0; }
+    } else if (s[a] >= n[a]) {
+      if (L->periodic[a]) s[a] -= n[a];
+      else { wall = a; side = 1; }
+    }
+  }
+  const int64_t self = cidx(L, x, y, z);
+  if (d == 0) return fin[self];
+  if (wall >= 0) {
+    const double *u = side == 0 ? L->wall_u_lo[wall] : L->wall_u_hi[wall];
+    const double cu = CXo[d] * u[0] + CYo[d] * u[1] + CZo[d] * u[2];
+    return fin[BBo[d] * N + self] + 6.0 * Wo[d] * cu;
+  }
+  const int64_t src = cidx(L, s[0], s[1], s[2]);
+  if (L->solid && L->solid[src]) return fin[BBo[d] * N + self];
+  return fin[d * N + src];
+}
+
+void orc_lb_step(const orc_lattice *L, const double *fin, double *fout, const double *Fu,
+                 const double *Fs, double *u_out) {
+  const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+  const double tau = L->tau;
+  for (int z = 0; z < L->nz; ++z)
+    for (int y = 0; y < L->ny; ++y)
+      for (int x = 0; x < L->nx; ++x) {
+        const int64_t i = cidx(L, x, y, z);
+        if (L->solid && L->solid[i]) continue;
+        double f[19];
+        for (int d = 0; d < 19; ++d) f[d] = pull(L, fin, N, x, y, z, d);
+        /* moments (LB_kernel.cu:1137-1163), C = 1 */
+        double rho = 0, jx = 0, jy = 0, jz = 0;
+        for (int d = 0; d < 19; ++d) {
+          rho += f[d];
+          jx += CXo[d] * f[d];
+          jy += CYo[d] * f[d];
+          jz += CZo[d] * f[d];
+        }
+        double Fa[3] = {L->g[0], L->g[1], L->g[2]}, Fb[3] = {L->g[0], L->g[1], L->g[2]};
+        if (Fu)
+          for (int k = 0; k < 3; ++k) Fa[k] += Fu[k * N + i];
+        if (Fs)
+          for (int k = 0; k < 3; ++k) Fb[k] += Fs[k * N + i];
+        const double u[3] = {(jx + 0.5 * Fa[0]) / rho, (jy + 0.5 * Fa[1]) / rho,
+                             (jz + 0.5 * Fa[2]) / rho};
+        if (u_out)
+          for (int k = 0; k < 3; ++k) u_out[k * N + i] = u[k];
+        /* Guo source in the reference's tensor form (LB_kernel.cu:1254-1296):
+         * uF = u F + F u, ee = e e - I/3, fdt = w (1 - 1/(2 tau))
+         * (3 e.F + 4.5 uF:ee) */
+        double uF[3][3];
+        for (int l = 0; l < 3; ++l)
+          for (int m = 0; m < 3; ++m) uF[l][m] = u[l] * Fb[m] + Fb[l] * u[m];
+        for (int d = 0; d < 19; ++d) {
+          const double e[3] = {CXo[d], CYo[d], CZo[d]};
+          double sum = 0;
+          for (int l = 0; l < 3; ++l)
+            for (int m = 0; m < 3; ++m) {
+              const double ee = e[l] * e[m] - (l == m ? 1.0 / 3.0 : 0.0);
+              sum += uF[l][m] * ee;
+            }
+          const double fdt = Wo[d] * (1.0 - 0.5 / tau) *
+                             (3.0 * (e[0] * Fb[0] + e[1] * Fb[1] + e[2] * Fb[2]) + 4.5 * sum);
+          const double fe = feq(d, rho, u[0], u[1], u[2]);
+          fout[d * N + i] = f[d] - (f[d] - fe) / tau + fdt;
+        }
+      }
+}
+
+/* rho and u of the pulled distributions with body force and optional extra
+ * force (velocity_density after streaming, LB_kernel.cu:1108-1199). */
+static void pull_moments(const orc_lattice *L, const double *f, const double *Fx3,
+                         double *rho_o, double *ux, double *uy, double *uz) {
+  const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+  for (int z = 0; z < L->nz; ++z)
+    for (int y = 0; y < L->ny; ++y)
+      for (int x = 0; x < L->nx; ++x) {
+        const int64_t i = cidx(L, x, y, z);
+        if (L->solid && L->solid[i]) {
+          if (rho_o) rho_o[i] = 0;
+          if (ux) ux[i] = 0;
+          if (uy) uy[i] = 0;
+          if (uz) uz[i] = 0;
+          continue;
+        }
+        double rho = 0, j[3] = {0, 0, 0};
+        for (int d = 0; d < 19; ++d) {
+          const double v = pull(L, f, N, x, y, z, d);
+          rho += v;
+          j[0] += CXo[d] * v;
+          j[1] += CYo[d] * v;
+          j[2] += CZo[d] * v;
+        }
+        double F[3] = {L->g[0], L->g[1], L->g[2]};
+        if (Fx3)
+          for (int k = 0; k < 3; ++k) F[k] += Fx3[k * N + i];
+        if (rho_o) rho_o[i] = rho;
+        if (ux) ux[i] = (j[0] + 0.5 * F[0]) / rho;
+        if (uy) uy[i] = (j[1] + 0.5 * F[1]) / rho;
+        if (uz) uz[i] = (j[2] + 0.5 * F[2]) / rho;
+      }
+}
+
+void orc_fields(const orc_lattice *L, const double *f, double *rho, double *ux, double *uy,
+                double *uz) {
+  pull_moments(L, f, NULL, rho, ux, uy, uz);
+}
+
+void orc_conservation(const orc_lattice *L, const double *f, double out[4]) {
+  const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+  out[0] = out[1] = out[2] = out[3] = 0;
+  for (int64_t i = 0; i < N; ++i) {
+    if (L->solid && L->solid[i]) continue;
+    for (int d = 0; d < 19; ++d) {
+      const double v = f[d * N + i];
+      out[0] += v;
+      out[1] += CXo[d] * v;
+      out[2] += CYo[d] * v;
+      out[3] += CZo[d] * v;
+    }
+  }
+}
+
+/* ---------------- immersed boundary ---------------- */
+
+double orc_phi(double r) {
+  const double a = fabs(r);
+  if (a >= 2.0) return 0.0;
+  return 0.25 * (1.0 + cos(0.5 * M_PI * r));
+}
+
+static double wrap_pos(double v, int n, int periodic) {
+  if (!periodic) return v;
+  double w = fmod(v, (double)n);
+  if (w < 0) w += n;
+  if (w >= n) w -= n;
+  return w;
+}
+
+static int axis_idx(int i, int n, int periodic) {
+  if (periodic) {
+    i %= n;
+    return i < 0 ? i + n : i;
+  }
+  return (i < 0 || i >= n) ? -1 : i;
+}
+
+/* Visit the 4x4x4 stencil of a marker: weights and lattice indices. */
+typedef struct {
+  int ix[4], iy[4], iz[4];
+  double wx[4], wy[4], wz[4];
+} stencil;
+
+static void make_stencil(const orc_lattice *L, double x, double y, double z, stencil *s) {
+  const double P[3] = {wrap_pos(x, L->nx, L->periodic[0]), wrap_pos(y, L->ny, L->periodic[1]),
+                       wrap_pos(z, L->nz, L->periodic[2])};
+  const int n[3] = {L->nx, L->ny, L->nz};
+  int *I[3] = {s->ix, s->iy, s->iz};
+  double *W[3] = {s->wx, s->wy, s->wz};
+  for (int a = 0; a < 3; ++a) {
+    const int b = (int)floor(P[a]) - 1;
+    for (int k = 0; k < 4; ++k) {
+      I[a][k] = axis_idx(b + k, n[a], L->periodic[a]);
+      W[a][k] = orc_phi(P[a] - (double)(b + k));
+    }
+  }
+}
+
+void orc_ib_spread(const orc_lattice *L, int n, const double *x, const double *y,
+                   const double *z, const double *fx, const double *fy, const double *fz,
+                   const int *coupled, double scale, double *F) {
+  const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+  for (int m = 0; m < n; ++m) {
+    if (coupled && !coupled[m]) continue;
+    stencil s;
+    make_stencil(L, x[m], y[m], z[m], &s);
+    for (int c = 0; c < 4; ++c) {
+      if (s.iz[c] < 0) continue;
+      for (int b = 0; b < 4; ++b) {
+        if (s.iy[b] < 0) continue;
+        for (int a = 0; a < 4; ++a) {
+          if (s.ix[a] < 0) continue;
+          const double w = s.wx[a] * s.wy[b] * s.wz[c];
+          const int64_t i = cidx(L, s.ix[a], s.iy[b], s.iz[c]);
+          F[i] += w * fx[m] * scale;
+          F[N + i] += w * fy[m] * scale;
+          F[2 * N + i] += w * fz[m] * scale;
+        }
+      }
+    }
+  }
+}
+
+/* LB_kernel.cu:1419-1520: each lattice node loops over all markers and takes
+ * those within 2 cells per axis; periodic images are taken on every periodic
+ * axis (the reference takes them in y only). */
+void orc_ib_spread_scan(const orc_lattice *L, int n, const double *x, const double *y,
+                        const double *z, const double *fx, const double *fy,
+                        const double *fz, const int *coupled, double scale, double *F) {
+  const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+  for (int k = 0; k < L->nz; ++k)
+    for (int j = 0; j < L->ny; ++j)
+      for (int i = 0; i < L->nx; ++i) {
+        const int64_t c = cidx(L, i, j, k);
+        for (int m = 0; m < n; ++m) {
+          if (coupled && !coupled[m]) continue;
+          const double P[3] = {wrap_pos(x[m], L->nx, L->periodic[0]),
+                               wrap_pos(y[m], L->ny, L->periodic[1]),
+                               wrap_pos(z[m], L->nz, L->periodic[2])};
+          const int node[3] = {i, j, k};
+          const int nn[3] = {L->nx, L->ny, L->nz};
+          double w = 1.0;
+          for (int a = 0; a < 3 && w != 0.0; ++a) {
+            double r = P[a] - node[a];
+            if (L->periodic[a]) {
+              if (r > 0.5 * nn[a]) r -= nn[a];
+              if (r < -0.5 * nn[a]) r += nn[a];
+            }
+            if (fabs(r) >= 2.0) w = 0.0;
+            else w *= 0.25 * (1.0 + cos(0.5 * M_PI * r));
+          }
+          if (w == 0.0) continue;
+          F[c] += w * fx[m] * scale;
+          F[N + c] += w * fy[m] * scale;
+          F[2 * N + c] += w * fz[m] * scale;
+        }
+      }
+}
+
+void orc_ib_interp(const orc_lattice *L, int n, const double *x, const double *y,
+                   const double *z, const double *u, const int *coupled, double *vx,
+                   double *vy, double *vz) {
+  const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+  for (int m = 0; m < n; ++m) {
+    if (coupled && !coupled[m]) continue;
+    stencil s;
+    make_stencil(L, x[m], y[m], z[m], &s);
+    double v[3] = {0, 0, 0};
+    for (int c = 0; c < 4; ++c) {
+      if (s.iz[c] < 0) continue;
+      for (int b = 0; b < 4; ++b) {
+        if (s.iy[b] < 0) continue;
+        for (int a = 0; a < 4; ++a) {
+          if (s.ix[a] < 0) continue;
+          const double w = s.wx[a] * s.wy[b] * s.wz[c];
+          const int64_t i = cidx(L, s.ix[a], s.iy[b], s.iz[c]);
+          v[0] += w * u[i];
+          v[1] += w * u[N + i];
+          v[2] += w * u[2 * N + i];
+        }
+      }
+    }
+    vx[m] = v[0];
+    vy[m] = v[1];
+    vz[m] = v[2];
+  }
+}
+
+/* ---------------- cell lists ---------------- */
+
+static int cell_of_f(float v, int n, int periodic) {
+  int c = (int)floorf(v);
+  if (periodic) {
+    c %= n;
+    if (c < 0) c += n;
+  } else {
+    c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+  }
+  return c;
+}
+
+static int bin_of_f(float x, float y, float z, int nx, int ny, int nz, const int per[3],
+                    const int nb[3]) {
+  const int bx = cell_of_f(x, nx, per[0]) * nb[0] / nx;
+  const int by = cell_of_f(y, ny, per[1]) * nb[1] / ny;
+  const int bz = cell_of_f(z, nz, per[2]) * nb[2] / nz;
+  return bx + nb[0] * (by + nb[1] * bz);
+}
+
+void orc_cell_list(int n, const float *x, const float *y, const float *z, int nx, int ny,
+                   int nz, const int periodic[3], const int nb[3], int *bin_start,
+                   int *items) {
+  const int64_t nbins = (int64_t)nb[0] * nb[1] * nb[2];
+  memset(bin_start, 0, sizeof(int) * (size_t)(nbins + 1));
+  int *bin = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) {
+    bin[i] = bin_of_f(x[i], y[i], z[i], nx, ny, nz, periodic, nb);
+    bin_start[bin[i] + 1]++;
+  }
+  for (int64_t b = 0; b < nbins; ++b) bin_start[b + 1] += bin_start[b];
+  int *fillp = (int *)malloc(sizeof(int) * (size_t)(nbins > 0 ? nbins : 1));
+  memcpy(fillp, bin_start, sizeof(int) * (size_t)nbins);
+  for (int i = 0; i < n; ++i) items[fillp[bin[i]]++] = i;
+  free(fillp);
+  free(bin);
+}
+
+/* ---------------- full state ---------------- */
+
+struct orc_state {
+  pfg_params p;
+  orc_lattice L;
+  int64_t N;
+  int order;
+  uint8_t *solid;
+  double *f, *f2, *F, *Fprev, *u;
+  int n;
+  double *mk[9]; /* x y z vx vy vz fx fy fz */
+  int *kind, *body, *coupled;
+  int n_plt, n_tri, n_edges, n_rec, n_wlig, fib_lig, n_lig, n_links;
+  int *plt_elem_start, *plt_tri_start, *tri;
+  double *tri_S0, *plt_S0, *plt_V0;
+  double *tri_S, *tri_n, *plt_S, *plt_V;
+  int *lowest;
+  int *edge;
+  double *edge_l0;
+  int *rec_tri;
+  double *rec_r1, *rec_r2;
+  double *lig_x, *lig_y, *lig_z;
+  int *link;
+  double *link_l0;
+  int *rec_bond, *rec_cand, *rec_release, *lig_owner, *lig_claim;
+  double *rec_force, *rec_w;
+  int nb[3];
+  int *bin_start, *items;
+  int64_t step, total_bonds, detached, detach_step, near, events;
+};
+
+static void *xcalloc(size_t n, size_t sz) { return calloc(n > 0 ? n : 1, sz); }
+
+orc_state *orc_create(const pfg_params *p, const pfg_model *m, const uint8_t *flags) {
+  orc_state *s = (orc_state *)calloc(1, sizeof(orc_state));
+  s->p = *p;
+  s->N = (int64_t)p->nx * p->ny * p->nz;
+  if (p->use_flags && flags) {
+    s->solid = (uint8_t *)malloc((size_t)s->N);
+    memcpy(s->solid, flags, (size_t)s->N);
+  }
+  orc_lattice_from_params(p, s->solid, &s->L);
+  s->f = (double *)xcalloc((size_t)19 * s->N, sizeof(double));
+  s->f2 = (double *)xcalloc((size_t)19 * s->N, sizeof(double));
+  s->F = (double *)xcalloc((size_t)3 * s->N, sizeof(double));
+  s->Fprev = (double *)xcalloc((size_t)3 * s->N, sizeof(double));
+  s->u = (double *)xcalloc((size_t)3 * s->N, sizeof(double));
+  orc_init_eq(&s->L, NULL, NULL, NULL, NULL, s->f);
+  const int n = m ? m->n_markers : 0;
+  s->n = n;
+  for (int k = 0; k < 9; ++k) s->mk[k] = (double *)xcalloc((size_t)n, sizeof(double));
+  s->kind = (int *)xcalloc((size_t)n, sizeof(int));
+  s->body = (int *)xcalloc((size_t)n, sizeof(int));
+  s->coupled = (int *)xcalloc((size_t)n, sizeof(int));
+  for (int i = 0; i < n; ++i) {
+    s->mk[0][i] = m->x[i];
+    s->mk[1][i] = m->y[i];
+    s->mk[2][i] = m->z[i];
+    s->kind[i] = m->kind[i];
+    s->body[i] = m->body[i];
+    s->coupled[i] = m->kind[i] == PFG_MARKER_ELEMENT ||
+                    (m->kind[i] == PFG_MARKER_FIBER && p->fiber_move == PFG_FIBER_IB);
+  }
+  s->n_plt = m ? m->n_platelets : 0;
+  s->n_tri = m ? m->n_tri : 0;
+  s->n_edges = m ? m->n_edges : 0;
+  s->n_rec = m ? m->n_receptors : 0;
+  s->n_wlig = m ? m->n_wall_ligands : 0;
+  s->fib_lig = m ? (m->fibrin_ligands != 0) : 0;
+  s->n_lig = s->n_wlig + (s->fib_lig ? n : 0);
+  s->n_links = m ? m->n_links : 0;
+  s->plt_elem_start = (int *)xcalloc((size_t)s->n_plt + 1, sizeof(int));
+  s->plt_tri_start = (int *)xcalloc((size_t)s->n_plt + 1, sizeof(int));
+  for (int q = 0; q <= s->n_plt && s->n_plt > 0; ++q) {
+    s->plt_elem_start[q] = m->plt_elem_start[q];
+    s->plt_tri_start[q] = m->plt_tri_start[q];
+  }
+  s->tri = (int *)xcalloc((size_t)3 * s->n_tri, sizeof(int));
+  s->tri_S0 = (double *)xcalloc((size_t)s->n_tri, sizeof(double));
+  s->tri_S = (double *)xcalloc((size_t)s->n_tri, sizeof(double));
+  s->tri_n = (double *)xcalloc((size_t)3 * s->n_tri, sizeof(double));
+  for (int t = 0; t < s->n_tri; ++t) {
+    for (int k = 0; k < 3; ++k) s->tri[3 * t + k] = m->tri[3 * t + k];
+    s->tri_S0[t] = m->tri_S0[t];
+  }
+  s->plt_S0 = (double *)xcalloc((size_t)s->n_plt, sizeof(double));
+  s->plt_V0 = (double *)xcalloc((size_t)s->n_plt, sizeof(double));
+  s->plt_S = (double *)xcalloc((size_t)s->n_plt, sizeof(double));
+  s->plt_V = (double *)xcalloc((size_t)s->n_plt, sizeof(double));
+  s->lowest = (int *)xcalloc((size_t)s->n_plt, sizeof(int));
+  for (int q = 0; q < s->n_plt; ++q) {
+    s->plt_S0[q] = m->plt_S0[q];
+    s->plt_V0[q] = m->plt_V0[q];
+    s->lowest[q] = -1;
+  }
+  s->edge = (int *)xcalloc((size_t)2 * s->n_edges, sizeof(int));
+  s->edge_l0 = (double *)xcalloc((size_t)s->n_edges, sizeof(double));
+  for (int e = 0; e < s->n_edges; ++e) {
+    s->edge[2 * e] = m->edge[2 * e];
+    s->edge[2 * e + 1] = m->edge[2 * e + 1];
+    s->edge_l0[e] = m->edge_l0[e];
+  }
+  s->rec_tri = (int *)xcalloc((size_t)s->n_rec, sizeof(int));
+  s->rec_r1 = (double *)xcalloc((size_t)s->n_rec, sizeof(double));
+  s->rec_r2 = (double *)xcalloc((size_t)s->n_rec, sizeof(double));
+  s->rec_bond = (int *)xcalloc((size_t)s->n_rec, sizeof(int));
+  s->rec_cand = (int *)xcalloc((size_t)s->n_rec, sizeof(int));
+  s->rec_release = (int *)xcalloc((size_t)s->n_rec, sizeof(int));
+  s->rec_force = (double *)xcalloc((size_t)3 * s->n_rec, sizeof(double));
+  s->rec_w = (double *)xcalloc((size_t)3 * s->n_rec, sizeof(double));
+  for (int r = 0; r < s->n_rec; ++r) {
+    s->rec_tri[r] = m->rec_tri[r];
+    s->rec_r1[r] = m->rec_r1[r];
+    s->rec_r2[r] = m->rec_r2[r];
+    s->rec_bond[r] = -1;
+  }
+  s->lig_x = (double *)xcalloc((size_t)s->n_wlig, sizeof(double));
+  s->lig_y = (double *)xcalloc((size_t)s->n_wlig, sizeof(double));
+  s->lig_z = (double *)xcalloc((size_t)s->n_wlig, sizeof(double));
+  for (int l = 0; l < s->n_wlig; ++l) {
+    s->lig_x[l] = m->lig_x[l];
+    s->lig_y[l] = m->lig_y[l];
+    s->lig_z[l] = m->lig_z ? (double)m->lig_z[l] : (double)p->wall_z;
+  }
+  s->lig_owner = (int *)xcalloc((size_t)s->n_lig, sizeof(int));
+  s->lig_claim = (int *)xcalloc((size_t)s->n_lig, sizeof(int));
+  for (int l = 0; l < s->n_lig; ++l) {
+    s->lig_owner[l] = -1;
+    s->lig_claim[l] = INT_MAX;
+  }
+  s->link = (int *)xcalloc((size_t)2 * s->n_links, sizeof(int));
+  s->link_l0 = (double *)xcalloc((size_t)s->n_links, sizeof(double));
+  for (int l = 0; l < s->n_links; ++l) {
+    s->link[2 * l] = m->link[2 * l];
+    s->link[2 * l + 1] = m->link[2 * l + 1];
+    s->link_l0[l] = m->link_l0[l];
+  }
+  const int nn[3] = {p->nx, p->ny, p->nz};
+  for (int k = 0; k < 3; ++k) {
+    s->nb[k] = nn[k] / p->bin_cells;
+    if (s->nb[k] < 1) s->nb[k] = 1;
+  }
+  s->bin_start = (int *)xcalloc((size_t)s->nb[0] * s->nb[1] * s->nb[2] + 1, sizeof(int));
+  s->items = (int *)xcalloc((size_t)n, sizeof(int));
+  s->detach_step = -1;
+  return s;
+}
+
+void orc_destroy(orc_state *s) {
+  if (!s) return;
+  free(s->solid);
+  free(s->f);
+  free(s->f2);
+  free(s->F);
+  free(s->Fprev);
+  free(s->u);
+  for (int k = 0; k < 9; ++k) free(s->mk[k]);
+  free(s->kind);
+  free(s->body);
+  free(s->coupled);
+  free(s->plt_elem_start);
+  free(s->plt_tri_start);
+  free(s->tri);
+  free(s->tri_S0);
+  free(s->tri_S);
+  free(s->tri_n);
+  free(s->plt_S0);
+  free(s->plt_V0);
+  free(s->plt_S);
+  free(s->plt_V);
+  free(s->lowest);
+  free(s->edge);
+  free(s->edge_l0);
+  free(s->rec_tri);
+  free(s->rec_r1);
+  free(s->rec_r2);
+  free(s->rec_bond);
+  free(s->rec_cand);
+  free(s->rec_release);
+  free(s->rec_force);
+  free(s->rec_w);
+  free(s->lig_x);
+  free(s->lig_y);
+  free(s->lig_z);
+  free(s->lig_owner);
+  free(s->lig_claim);
+  free(s->link);
+  free(s->link_l0);
+  free(s->bin_start);
+  free(s->items);
+  free(s);
+}
+
+void orc_set_order(orc_state *s, int order) { s->order = order; }
+void orc_set_step(orc_state *s, int64_t step) { s->step = step; }
+
+/* ---- membrane: sem_calculate_A_kernel (sem_kernel.cu:724-872) ---- */
+static void stage_membrane(orc_state *s) {
+  const double *X = s->mk[0], *Y = s->mk[1], *Z = s->mk[2];
+  for (int q = 0; q < s->n_plt; ++q) {
+    double S = 0, V = 0;
+    for (int t = s->plt_tri_start[q]; t < s->plt_tri_start[q + 1]; ++t) {
+      const int *nd = s->tri + 3 * t;
+      double c[3][3];
+      for (int k = 0; k < 3; ++k) {
+        c[k][0] = X[nd[k]];
+        c[k][1] = Y[nd[k]];
+        c[k][2] = Z[nd[k]];
+      }
+      double e12[3], e13[3], ne[3];
+      for (int k = 0; k < 3; ++k) {
+        e12[k] = c[1][k] - c[0][k];
+        e13[k] = c[2][k] - c[0][k];
+      }
+      ne[0] = e12[1] * e13[2] - e12[2] * e13[1];
+      ne[1] = e12[2] * e13[0] - e12[0] * e13[2];
+      ne[2] = e12[0] * e13[1] - e12[1] * e13[0];
+      const double nn = sqrt(ne[0] * ne[0] + ne[1] * ne[1] + ne[2] * ne[2]);
+      for (int k = 0; k < 3; ++k) ne[k] /= -nn;
+      const double St = 0.5 * nn;
+      double cen[3];
+      for (int k = 0; k < 3; ++k) cen[k] = (c[0][k] + c[1][k] + c[2][k]) / 3.0;
+      s->tri_S[t] = St;
+      for (int k = 0; k < 3; ++k) s->tri_n[3 * t + k] = ne[k];
+      S += St;
+      V += (cen[0] * ne[0] + cen[1] * ne[1] + cen[2] * ne[2]) * St / 3.0;
+    }
+    s->plt_S[q] = S;
+    s->plt_V[q] = V;
+    int lo = -1;
+    double zmin = INFINITY;
+    for (int e = s->plt_elem_start[q]; e < s->plt_elem_start[q + 1]; ++e) {
+      if ((float)Z[e] < zmin) {
+        zmin = (float)Z[e];
+        lo = e;
+      }
+    }
+    s->lowest[q] = lo;
+  }
+}
+
+/* ---- cells ---- */
+static void stage_cells(orc_state *s) {
+  float *x = (float *)malloc(sizeof(float) * (size_t)(s->n > 0 ? s->n : 1));
+  float *y = (float *)malloc(sizeof(float) * (size_t)(s->n > 0 ? s->n : 1));
+  float *z = (float *)malloc(sizeof(float) * (size_t)(s->n > 0 ? s->n : 1));
+  for (int i = 0; i < s->n; ++i) {
+    x[i] = (float)s->mk[0][i];
+    y[i] = (float)s->mk[1][i];
+    z[i] = (float)s->mk[2][i];
+  }
+  orc_cell_list(s->n, x, y, z, s->p.nx, s->p.ny, s->p.nz, s->L.periodic, s->nb,
+                s->bin_start, s->items);
+  free(x);
+  free(y);
+  free(z);
+  s->total_bonds = 0;
+}
+
+/* ---- bonds: Bell model (sem_platelet_wall_kernel, sem_kernel.cu:1750-2083) ---- */
+static double min_img(double d, int n, int periodic) {
+  if (!periodic) return d;
+  if (d > 0.5 * n) d -= n;
+  else if (d < -0.5 * n) d += n;
+  return d;
+}
+
+static void receptor_pos(const orc_state *s, int r, double rp[3], int nodes[3]) {
+  const int t = s->rec_tri[r];
+  const double r1 = s->rec_r1[r], r2 = s->rec_r2[r];
+  for (int k = 0; k < 3; ++k) nodes[k] = s->tri[3 * t + k];
+  /* sem_kernel.cu:1811, 1840-1841 */
+  const double c0 = 1.0 - r1, c1 = r1 * (1.0 - r2), c2 = r1 * r2;
+  for (int a = 0; a < 3; ++a)
+    rp[a] = c0 * s->mk[a][nodes[0]] + c1 * s->mk[a][nodes[1]] + c2 * s->mk[a][nodes[2]];
+}
+
+static void ligand_vec(const orc_state *s, int l, const double rp[3], double d[3]) {
+  double L3[3];
+  if (l < s->n_wlig) {
+    L3[0] = s->lig_x[l];
+    L3[1] = s->lig_y[l];
+    L3[2] = s->lig_z[l];
+  } else {
+    const int mk = l - s->n_wlig;
+    for (int a = 0; a < 3; ++a) L3[a] = s->mk[a][mk];
+  }
+  const int nn[3] = {s->p.nx, s->p.ny, s->p.nz};
+  for (int a = 0; a < 3; ++a) d[a] = min_img(L3[a] - rp[a], nn[a], s->L.periodic[a]);
+}
+
+static void stage_bonds(orc_state *s) {
+  const double eps_p = 1e-6, eps_r = 1e-4;
+  const pfg_params *p = &s->p;
+  /* decide from the start-of-step state */
+  for (int r = 0; r < s->n_rec; ++r) {
+    double rp[3];
+    int nodes[3];
+    receptor_pos(s, r, rp, nodes);
+    s->rec_cand[r] = -1;
+    s->rec_release[r] = 0;
+    const int b = s->rec_bond[r];
+    if (b >= 0) {
+      double d[3];
+      ligand_vec(s, b, rp, d);
+      const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const double tension = (double)p->bond_k * (dist - (double)p->bond_l0);
+      const double koff = (double)p->k_off0 * exp((double)p->bell_xb_kT * (tension > 0 ? tension : 0));
+      const double p_off = 1.0 - exp(-koff);
+      const double u = pfg_uniform(p->seed, s->step, (uint32_t)r, PFG_RNG_BOND_OFF);
+      if (fabs(u - p_off) < eps_p) s->near++;
+      s->rec_release[r] = u < p_off;
+      continue;
+    }
+    int best = -1;
+    double best_d2 = INFINITY;
+    int near_reach = 0;
+    const double reach = p->bond_reach, reach2 = (double)p->bond_reach * p->bond_reach;
+    for (int l = 0; l < s->n_lig; ++l) {
+      if (l >= s->n_wlig && s->kind[l - s->n_wlig] == PFG_MARKER_ELEMENT) continue;
+      double d[3];
+      ligand_vec(s, l, rp, d);
+      const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (d2 > 4.0 * reach2 + 1.0) continue;
+      if (fabs(sqrt(d2) - reach) < eps_r) near_reach = 1;
+      if (d2 > reach2 || s->lig_owner[l] >= 0) continue;
+      if (d2 < best_d2) {
+        best_d2 = d2;
+        best = l;
+      }
+    }
+    if (near_reach) s->near++;
+    if (best < 0) continue;
+    const double p_on = 1.0 - exp(-(double)p->k_on);
+    const double u = pfg_uniform(p->seed, s->step, (uint32_t)r, PFG_RNG_BOND_ON);
+    if (fabs(u - p_on) < eps_p) s->near++;
+    if (u < p_on) {
+      s->rec_cand[r] = best;
+      if (r < s->lig_claim[best]) s->lig_claim[best] = r;
+    }
+  }
+  /* resolve: ruptures, lowest-index claims, forces */
+  for (int r = 0; r < s->n_rec; ++r) {
+    int b = s->rec_bond[r];
+    if (s->rec_release[r]) {
+      s->lig_owner[b] = -1;
+      b = -1;
+      s->events++;
+    } else if (s->rec_cand[r] >= 0) {
+      const int l = s->rec_cand[r];
+      if (s->lig_claim[l] == r) {
+        s->lig_owner[l] = r;
+        b = l;
+        s->events++;
+      }
+    }
+    s->rec_bond[r] = b;
+    double F[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+    if (b >= 0) {
+      double rp[3];
+      int nodes[3];
+      receptor_pos(s, r, rp, nodes);
+      double d[3];
+      ligand_vec(s, b, rp, d);
+      const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      if (dist > 0) {
+        const double c = (double)p->bond_k * (dist - (double)p->bond_l0) / dist;
+        for (int a = 0; a < 3; ++a) F[a] = c * d[a];
+      }
+      /* inverse-distance split (sem_kernel.cu:2051-2065) */
+      double inv[3], sum = 0;
+      for (int k = 0; k < 3; ++k) {
+        double e2 = 0;
+        for (int a = 0; a < 3; ++a) {
+          const double e = rp[a] - s->mk[a][nodes[k]];
+          e2 += e * e;
+        }
+        inv[k] = e2 > 0 ? 1.0 / sqrt(e2) : 1e30;
+        sum += inv[k];
+      }
+      for (int k = 0; k < 3; ++k) w[k] = inv[k] / sum;
+      s->total_bonds++;
+    }
+    for (int a = 0; a < 3; ++a) {
+      s->rec_force[3 * r + a] = F[a];
+      s->rec_w[3 * r + a] = w[a];
+    }
+  }
+  for (int l = 0; l < s->n_lig; ++l) s->lig_claim[l] = INT_MAX;
+}
+
+/* ---- forces ---- */
+static void stage_forces(orc_state *s) {
+  const pfg_params *p = &s->p;
+  double *X = s->mk[0], *Y = s->mk[1], *Z = s->mk[2];
+  double *FX = s->mk[6], *FY = s->mk[7], *FZ = s->mk[8];
+  /* detach test (semLB.cu:451-455) after this step's bond update */
+  if (s->n_rec > 0 && p->detach_rate > 0 && s->total_bonds == 0 && !s->detached) {
+    const double u = pfg_uniform(p->seed, s->step, 0u, PFG_RNG_DETACH);
+    if (u < (double)p->detach_rate) {
+      s->detached = 1;
+      s->detach_step = s->step;
+    }
+  }
+  for (int i = 0; i < s->n; ++i) FX[i] = FY[i] = FZ[i] = 0;
+  /* Morse over all pairs of elements of one platelet within the cutoff */
+  if (p->morse_u0 != 0) {
+    const double U0 = p->morse_u0, rho = p->morse_rho, re = p->morse_req;
+    const double cut2 = (double)p->morse_cutoff * p->morse_cutoff;
+    for (int q = 0; q < s->n_plt; ++q)
+      for (int i = s->plt_elem_start[q]; i < s->plt_elem_start[q + 1]; ++i)
+        for (int j = s->plt_elem_start[q]; j < s->plt_elem_start[q + 1]; ++j) {
+          if (j == i) continue;
+          const double dx = X[j] - X[i], dy = Y[j] - Y[i], dz = Z[j] - Z[i];
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 >= cut2 || r2 == 0) continue;
+          const double r = sqrt(r2);
+          /* U = U0 (e^{-2 rho (r - re)} - 2 e^{-rho (r - re)}); F_i = -dU/dr (x_i - x_j)/r */
+          const double dUdr = U0 * (-2.0 * rho * exp(-2.0 * rho * (r - re)) +
+                                     2.0 * rho * exp(-rho * (r - re)));
+          FX[i] += dUdr * dx / r;
+          FY[i] += dUdr * dy / r;
+          FZ[i] += dUdr * dz / r;
+        }
+  }
+  /* mesh-edge Hookean springs (sem_kernel.cu:356-396) */
+  if (p->k_spring != 0) {
+    for (int e = 0; e < s->n_edges; ++e) {
+      const int a = s->edge[2 * e], b = s->edge[2 * e + 1];
+      const double dx = X[b] - X[a], dy = Y[b] - Y[a], dz = Z[b] - Z[a];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      if (r == 0) continue;
+      const double c = (double)p->k_spring * (r - s->edge_l0[e]) / r;
+      FX[a] += c * dx;
+      FY[a] += c * dy;
+      FZ[a] += c * dz;
+      FX[b] -= c * dx;
+      FY[b] -= c * dy;
+      FZ[b] -= c * dz;
+    }
+  }
+  /* area / volume constraints, sem_ForceAreaVol_kernel (sem_kernel.cu:1640-1749),
+   * in the reference's expanded form */
+  if (p->ks != 0 || p->kv != 0) {
+    for (int q = 0; q < s->n_plt; ++q) {
+      const double Sall = s->plt_S[q], S0all = s->plt_S0[q], V = s->plt_V[q], V0 = s->plt_V0[q];
+      for (int t = s->plt_tri_start[q]; t < s->plt_tri_start[q + 1]; ++t) {
+        const int *nd = s->tri + 3 * t;
+        double c[3][3], Lm[3][3], cen[3], ne[3];
+        for (int k = 0; k < 3; ++k) {
+          c[k][0] = X[nd[k]];
+          c[k][1] = Y[nd[k]];
+          c[k][2] = Z[nd[k]];
+        }
+        for (int a = 0; a < 3; ++a) {
+          Lm[0][a] = c[1][a] - c[2][a];
+          Lm[1][a] = c[2][a] - c[0][a];
+          Lm[2][a] = c[0][a] - c[1][a];
+          cen[a] = (c[0][a] + c[1][a] + c[2][a]) / 3.0;
+          ne[a] = s->tri_n[3 * t + a];
+        }
+        const double Se = s->tri_S[t], S0e = s->tri_S0[t];
+        for (int i = 0; i < 3; ++i) {
+          double dS[3];
+          dS[0] = -0.5 * (Lm[i][1] * ne[2] - Lm[i][2] * ne[1]);
+          dS[1] = -0.5 * (Lm[i][2] * ne[0] - Lm[i][0] * ne[2]);
+          dS[2] = -0.5 * (Lm[i][0] * ne[1] - Lm[i][1] * ne[0]);
+          const double ca = -(double)p->ks * ((Se - S0e) / S0e + (Sall - S0all) / S0all);
+          FX[nd[i]] += ca * dS[0];
+          FY[nd[i]] += ca * dS[1];
+          FZ[nd[i]] += ca * dS[2];
+        }
+        double dV[3][3];
+        const double L1xL0_0 = Lm[1][1] * Lm[0][2] - Lm[1][2] * Lm[0][1];
+        const double L1xL0_1 = Lm[1][2] * Lm[0][0] - Lm[1][0] * Lm[0][2];
+        const double L1xL0_2 = Lm[1][0] * Lm[0][1] - Lm[1][1] * Lm[0][0];
+        for (int i = 0; i < 3; ++i) {
+          dV[i][0] = -L1xL0_0 / 3.0 - Lm[i][2] * cen[1] + Lm[i][1] * cen[2];
+          dV[i][1] = -L1xL0_1 / 3.0 + Lm[i][2] * cen[0] - Lm[i][0] * cen[2];
+          dV[i][2] = -L1xL0_2 / 3.0 - Lm[i][1] * cen[0] + Lm[i][0] * cen[1];
+          const double cv = -(double)p->kv * (V - V0) / V0 / 6.0;
+          FX[nd[i]] += cv * dV[i][0];
+          FY[nd[i]] += cv * dV[i][1];
+          FZ[nd[i]] += cv * dV[i][2];
+        }
+      }
+    }
+  }
+  /* wall repulsion on the lowest element (sem_kernel.cu:409-430) */
+  if (p->wall_rep_a != 0) {
+    for (int q = 0; q < s->n_plt; ++q) {
+      const int e = s->lowest[q];
+      if (e < 0) continue;
+      double h = Z[e] - p->wall_z;
+      if (h <= p->wall_rep_hc) {
+        if (h < 0.01 * p->wall_rep_hc) h = 0.01 * (double)p->wall_rep_hc;
+        const double ex = exp(-(double)p->wall_rep_kappa * h);
+        FZ[e] += (double)p->wall_rep_a * p->wall_rep_kappa * ex / (1.0 - ex);
+      }
+    }
+  }
+  /* receptor bond forces onto the triangle vertices */
+  for (int r = 0; r < s->n_rec; ++r) {
+    const int t = s->rec_tri[r];
+    for (int k = 0; k < 3; ++k) {
+      const int v = s->tri[3 * t + k];
+      FX[v] += s->rec_w[3 * r + k] * s->rec_force[3 * r];
+      FY[v] += s->rec_w[3 * r + k] * s->rec_force[3 * r + 1];
+      FZ[v] += s->rec_w[3 * r + k] * s->rec_force[3 * r + 2];
+    }
+  }
+  /* fibrin links (fiber.cpp:647-733, network.cpp:378-418) */
+  const double sgn = p->fiber_ref_sign ? -1.0 : 1.0;
+  for (int l = 0; l < s->n_links; ++l) {
+    const int a = s->link[2 * l], b = s->link[2 * l + 1];
+    const double l0 = s->link_l0[l];
+    const double dx = X[b] - X[a], dy = Y[b] - Y[a], dz = Z[b] - Z[a];
+    const double len = sqrt(dx * dx + dy * dy + dz * dz);
+    if (len == 0 || l0 == 0) continue;
+    double tension;
+    if (p->fiber_law == PFG_FIBER_LINEAR) {
+      /* factor = Y A / rl, force factor (ol - rl) (network.cpp:364-373) */
+      tension = (double)p->fiber_k / l0 * (len - l0);
+    } else {
+      /* wormlike_chain_force (fiber.cpp:710-733): PerL = ini_l, CC = En/PerL,
+       * tl = 1 - (cur-ini)/ini, IP1 = 0.25/tl^2 - 0.25 + (cur-ini)/ini */
+      double x = (len - l0) / l0;
+      if (x > p->fiber_xmax) x = p->fiber_xmax;
+      const double tl = 1.0 - x;
+      const double CC = (double)p->fiber_k / l0;
+      tension = CC * (0.25 / (tl * tl) - 0.25 + x);
+    }
+    const double c = sgn * tension / len;
+    FX[a] += c * dx;
+    FY[a] += c * dy;
+    FZ[a] += c * dz;
+    FX[b] -= c * dx;
+    FY[b] -= c * dy;
+    FZ[b] -= c * dz;
+  }
+  /* bond reaction on fibrin-node ligands */
+  if (s->fib_lig) {
+    for (int i = 0; i < s->n; ++i) {
+      if (s->kind[i] == PFG_MARKER_ELEMENT) continue;
+      const int owner = s->lig_owner[s->n_wlig + i];
+      if (owner < 0) continue;
+      FX[i] -= s->rec_force[3 * owner];
+      FY[i] -= s->rec_force[3 * owner + 1];
+      FZ[i] -= s->rec_force[3 * owner + 2];
+    }
+  }
+}
+
+static void stage_spread(orc_state *s) {
+  orc_ib_spread(&s->L, s->n, s->mk[0], s->mk[1], s->mk[2], s->mk[6], s->mk[7], s->mk[8],
+                s->coupled, s->p.ib_force_scale, s->F);
+}
+
+static void stage_lb(orc_state *s) {
+  if (s->order == 0) {
+    orc_lb_step(&s->L, s->f, s->f2, s->F, s->F, s->u);
+  } else {
+    /* reference order: collision velocity from the previous step's force
+     * (semLB.cu:374-424); IB velocity from the streamed state and F(n) */
+    orc_lb_step(&s->L, s->f, s->f2, s->Fprev, s->F, NULL);
+    const int64_t N = s->N;
+    pull_moments(&s->L, s->f2, s->F, NULL, s->u, s->u + N, s->u + 2 * N);
+    memcpy(s->Fprev, s->F, sizeof(double) * 3 * (size_t)N);
+  }
+  double *t = s->f;
+  s->f = s->f2;
+  s->f2 = t;
+}
+
+static void stage_interp(orc_state *s) {
+  orc_ib_interp(&s->L, s->n, s->mk[0], s->mk[1], s->mk[2], s->u, s->coupled, s->mk[3],
+                s->mk[4], s->mk[5]);
+  for (int i = 0; i < s->n; ++i)
+    if (s->kind[i] == PFG_MARKER_FIXED) s->mk[3][i] = s->mk[4][i] = s->mk[5][i] = 0;
+}
+
+static void stage_update(orc_state *s) {
+  const pfg_params *p = &s->p;
+  const double dt = p->marker_dt;
+  for (int i = 0; i < s->n; ++i) {
+    if (s->kind[i] == PFG_MARKER_FIXED) continue;
+    if (s->kind[i] == PFG_MARKER_FIBER && p->fiber_move == PFG_FIBER_KICK) {
+      /* network::solverkick (network.cpp:839-881) then x += v dt */
+      const double g = p->fiber_gamma;
+      const double vdt = exp(-g * dt);
+      const double fdt = g > 0 ? (1.0 - vdt) / g : dt;
+      for (int a = 0; a < 3; ++a) {
+        s->mk[3 + a][i] = vdt * s->mk[3 + a][i] + s->mk[6 + a][i] * fdt;
+        s->mk[a][i] += s->mk[3 + a][i] * dt;
+      }
+      continue;
+    }
+    /* sem_update_position_kernel (sem_kernel.cu:2112-2138) */
+    for (int a = 0; a < 3; ++a) s->mk[a][i] += s->mk[3 + a][i] * dt;
+  }
+}
+
+static void stage_clear(orc_state *s) {
+  memset(s->F, 0, sizeof(double) * 3 * (size_t)s->N);
+  s->step++;
+}
+
+void orc_stage(orc_state *s, int stage) {
+  switch (stage) {
+    case PFG_STAGE_CELLS: stage_cells(s); break;
+    case PFG_STAGE_MEMBRANE: stage_membrane(s); break;
+    case PFG_STAGE_BONDS: stage_bonds(s); break;
+    case PFG_STAGE_FORCES: stage_forces(s); break;
+    case PFG_STAGE_SPREAD: stage_spread(s); break;
+    case PFG_STAGE_LB: stage_lb(s); break;
+    case PFG_STAGE_INTERP: stage_interp(s); break;
+    case PFG_STAGE_UPDATE: stage_update(s); break;
+    case PFG_STAGE_CLEAR: stage_clear(s); break;
+    default: break;
+  }
+}
+
+void orc_step(orc_state *s) {
+  if (s->n == 0) {
+    stage_lb(s);
+    return;
+  }
+  for (int st = 0; st < PFG_STAGE_COUNT; ++st) orc_stage(s, st);
+}
+
+double *orc_f(orc_state *s) { return s->f; }
+double *orc_force_field(orc_state *s) { return s->F; }
+double *orc_vel_field(orc_state *s) { return s->u; }
+double *orc_marker(orc_state *s, int which) { return (which >= 0 && which < 9) ? s->mk[which] : NULL; }
+double *orc_membrane(orc_state *s, int which) {
+  switch (which) {
+    case 0: return s->plt_S;
+    case 1: return s->plt_V;
+    case 2: return s->tri_S;
+    case 3: return s->tri_n;
+  }
+  return NULL;
+}
+int *orc_lowest(orc_state *s) { return s->lowest; }
+int *orc_rec_bond(orc_state *s) { return s->rec_bond; }
+int *orc_lig_owner(orc_state *s) { return s->lig_owner; }
+double *orc_rec_force(orc_state *s) { return s->rec_force; }
+int *orc_cells(orc_state *s, int which, int *nb3) {
+  if (nb3)
+    for (int k = 0; k < 3; ++k) nb3[k] = s->nb[k];
+  return which == 0 ? s->bin_start : s->items;
+}
+void orc_counters(orc_state *s, int64_t out[5]) {
+  out[0] = s->total_bonds;
+  out[1] = s->detached;
+  out[2] = s->detach_step;
+  out[3] = s->near;
+  out[4] = s->events;
+}

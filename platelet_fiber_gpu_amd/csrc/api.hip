@@ -1,0 +1,952 @@
+/*
+ * api.hip -- the C ABI (include/pfg.h): handle lifetime, uploads/downloads,
+ * model loading (host-side CSR construction) and the step driver.
+ *
+ * The reference's glue layer (semLB.cu) allocates pitched FP64 arrays, copies
+ * the whole SEM struct host->device and back every step (semLB.cu:435, 507) and
+ * refills random numbers on the host (semLB.cu:348-365).  Here all state lives
+ * on the device between calls; a step is one hipGraph launch whose kernels read
+ * the step index from device memory, and the host reads back only counters at
+ * the end of a pfg_step call.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "pfg_internal.h"
+
+using namespace pfg;
+
+namespace {
+
+pfg_status from_hip(hipError_t e) {
+  if (e == hipSuccess) return PFG_OK;
+  if (e == hipErrorOutOfMemory) return PFG_ERR_ALLOC;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return PFG_ERR_DEVICE;
+  return PFG_ERR_LAUNCH;
+}
+
+#define PFG_TRY(expr)                         \
+  do {                                        \
+    hipError_t e_ = (expr);                   \
+    if (e_ != hipSuccess) return from_hip(e_); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(pfg_handle *h, T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * count);
+  if (e == hipSuccess) h->dev_bytes += sizeof(T) * count;
+  return e;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) hipFree((void *)p);
+  p = nullptr;
+}
+
+template <typename T>
+hipError_t upload(pfg_handle *h, T *dst, const T *src, size_t count) {
+  if (count == 0) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyHostToDevice,
+                                h->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(h->stream);
+}
+
+template <typename T>
+hipError_t download(pfg_handle *h, T *dst, const T *src, size_t count) {
+  if (count == 0 || dst == nullptr) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost,
+                                h->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(h->stream);
+}
+
+template <typename T>
+hipError_t fill(pfg_handle *h, T *dst, T value, size_t count) {
+  std::vector<T> v(count, value);
+  return upload(h, dst, v.data(), count);
+}
+
+void free_markers(Markers &m) {
+  dfree(m.x); dfree(m.y); dfree(m.z);
+  dfree(m.vx); dfree(m.vy); dfree(m.vz);
+  dfree(m.fx); dfree(m.fy); dfree(m.fz);
+  dfree(m.kind); dfree(m.body);
+  dfree(m.plt_elem_start); dfree(m.plt_tri_start);
+  dfree(m.tri); dfree(m.tri_S0); dfree(m.tri_S); dfree(m.tri_n);
+  dfree(m.plt_S0); dfree(m.plt_V0); dfree(m.plt_S); dfree(m.plt_V);
+  dfree(m.plt_lowest);
+  dfree(m.m2t_start); dfree(m.m2t);
+  dfree(m.m2e_start); dfree(m.m2e_nbr); dfree(m.m2e_l0);
+  dfree(m.rec_tri); dfree(m.rec_r1); dfree(m.rec_r2);
+  dfree(m.rec_bond); dfree(m.rec_cand); dfree(m.rec_release);
+  dfree(m.rec_force); dfree(m.rec_w);
+  dfree(m.m2r_start); dfree(m.m2r);
+  dfree(m.lig_x); dfree(m.lig_y); dfree(m.lig_z); dfree(m.lig_owner); dfree(m.lig_claim);
+  dfree(m.lg_start); dfree(m.lg_items);
+  dfree(m.m2l_start); dfree(m.m2l_nbr); dfree(m.m2l_l0);
+  dfree(m.bin_count); dfree(m.bin_start); dfree(m.bin_items);
+  dfree(m.marker_bin); dfree(m.marker_slot); dfree(m.scan_partials);
+  Markers fresh;
+  fresh.counters = m.counters;
+  fresh.detach_step = m.detach_step;
+  fresh.d_step = m.d_step;
+  m = fresh;
+}
+
+void drop_graph(pfg_handle *h) {
+  if (h->graph_exec) {
+    hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+  }
+  if (h->graph_exec2) {
+    hipGraphExecDestroy(h->graph_exec2);
+    h->graph_exec2 = nullptr;
+  }
+  h->graph_valid = false;
+}
+
+/* Build a CSR (start[n+1], items) from (key, value) pairs; values keep their
+ * input order inside a key. */
+void build_csr(int n, const std::vector<std::pair<int, int>> &kv,
+               std::vector<int32_t> &start, std::vector<int32_t> &items) {
+  start.assign((size_t)n + 1, 0);
+  for (const auto &p : kv) start[(size_t)p.first + 1]++;
+  for (int i = 0; i < n; ++i) start[(size_t)i + 1] += start[(size_t)i];
+  items.assign(kv.size(), 0);
+  std::vector<int32_t> fillp(start.begin(), start.end() - 1);
+  for (const auto &p : kv) items[(size_t)fillp[(size_t)p.first]++] = p.second;
+}
+
+bool is_coupled_kind(int k) {
+  return k == PFG_MARKER_ELEMENT || k == PFG_MARKER_FIBER || k == PFG_MARKER_FIXED;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pfg_version(void) { return "platelet_fiber_gpu_amd 0.1 (gfx950)"; }
+
+const char *pfg_status_string(pfg_status s) {
+  switch (s) {
+    case PFG_OK: return "ok";
+    case PFG_ERR_ARG: return "invalid argument";
+    case PFG_ERR_ALLOC: return "allocation failed";
+    case PFG_ERR_DEVICE: return "no usable HIP device";
+    case PFG_ERR_LAUNCH: return "kernel launch or HIP runtime error";
+    case PFG_ERR_STATE: return "call out of order";
+    case PFG_ERR_IO: return "file I/O error";
+  }
+  return "unknown status";
+}
+
+pfg_status pfg_device_count(int *count) {
+  if (!count) return PFG_ERR_ARG;
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return PFG_ERR_DEVICE;
+  }
+  return PFG_OK;
+}
+
+void pfg_params_default(pfg_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->nx = 64;
+  p->ny = 32;
+  p->nz = 32;
+  p->store = PFG_STORE_F32;
+  p->f16_scale_log2 = 10;
+  p->tau = 0.8f;
+  p->ib_force_scale = 1.0f;
+  p->marker_dt = 1.0f;
+  p->morse_u0 = 0.0f;
+  p->morse_rho = 2.0f;
+  p->morse_req = 0.8f;
+  p->morse_cutoff = 2.5f;
+  p->wall_rep_kappa = 20.0f;
+  p->wall_rep_hc = 0.5f;
+  p->wall_z = -0.5f;
+  p->bond_reach = 1.0f;
+  p->bell_xb_kT = 1.0f;
+  p->seed = 1101;
+  p->fiber_law = PFG_FIBER_WLC;
+  p->fiber_move = PFG_FIBER_IB;
+  p->fiber_xmax = 0.9f;
+  p->fiber_gamma = 1.0f;
+  p->bin_cells = 3;
+}
+
+/* hostSEMparameters index map (sem_kernel.cu:58-76, filled at
+ * gpuSEM_LB.c:1206-1233): [0] receptors [1] seed [2] k_spring [3] Kf0 [4] Kr0
+ * [5] shear rate [6] rho0 [7] L0 [8] T0 [9] M0 [10] F0 [11] UMAX [12] nu
+ * [13] tau [14] P0 [15] C [16] UNIT_FACTOR [17] Ks [18] Kv [19] Kbend. */
+pfg_status pfg_params_from_sem(const float *s, float dx, float dt, int lx, int ly,
+                               int lz, pfg_params *p) {
+  if (!s || !p || dx <= 0.f || dt <= 0.f || lx <= 0 || ly <= 0 || lz <= 0)
+    return PFG_ERR_ARG;
+  for (int i = 0; i < 20; ++i) p->sem[i] = s[i];
+  p->nx = lx;
+  p->ny = ly;
+  p->nz = lz;
+  p->seed = (uint64_t)s[1];
+  p->k_spring = s[2];
+  p->k_on = s[3] * dt;  /* rates per second -> per step */
+  p->k_off0 = s[4] * dt;
+  p->tau = s[13];
+  p->ks = s[17];
+  p->kv = s[18];
+  /* marker force F/F0 -> lattice force density: F0 * UNIT_FACTOR * dt /
+   * (rho0 * UMAX) as in the Guo term (LB_kernel.cu:1298), in units of C */
+  const float F0 = s[10], rho0 = s[6], umax = s[11], C = s[15], uf = s[16];
+  if (rho0 > 0.f && umax > 0.f && C > 0.f)
+    p->ib_force_scale = F0 * uf * dt / (rho0 * umax) / C;
+  if (s[8] > 0.f) p->marker_dt = dt / s[8] / dx;
+  p->detach_rate = s[5] * dt;
+  return PFG_OK;
+}
+
+pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
+  if (!pin || !out) return PFG_ERR_ARG;
+  *out = nullptr;
+  const pfg_params &p = *pin;
+  if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return PFG_ERR_ARG;
+  if ((int64_t)p.nx * p.ny * p.nz > (int64_t)INT_MAX) return PFG_ERR_ARG;
+  if (p.store < PFG_STORE_F32 || p.store > PFG_STORE_BF16) return PFG_ERR_ARG;
+  if (!(p.tau > 0.5f)) return PFG_ERR_ARG;
+  for (int k = 0; k < 3; ++k)
+    if (p.wall[k] != PFG_WALL_PERIODIC && p.wall[k] != PFG_WALL_BOUNCE) return PFG_ERR_ARG;
+  if (p.bin_cells <= 0) return PFG_ERR_ARG;
+  if (p.f16_scale_log2 < 0 || p.f16_scale_log2 > 20) return PFG_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PFG_ERR_DEVICE;
+  if (device < 0 || device >= ndev) return PFG_ERR_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return PFG_ERR_DEVICE;
+
+  pfg_handle *h = new (std::nothrow) pfg_handle();
+  if (!h) return PFG_ERR_ALLOC;
+  h->p = p;
+  h->device = device;
+  pfg_status st = PFG_OK;
+  do {
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+      st = PFG_ERR_DEVICE;
+      break;
+    }
+    h->N = (int64_t)p.nx * p.ny * p.nz;
+    h->plane = (h->N + 255) / 256 * 256;
+    h->esize = p.store == PFG_STORE_F32 ? 4 : 2;
+    for (int k = 0; k < 2; ++k) {
+      if (hipMalloc(&h->f[k], (size_t)h->esize * Q * h->plane) != hipSuccess) {
+        st = PFG_ERR_ALLOC;
+        break;
+      }
+      h->dev_bytes += (size_t)h->esize * Q * h->plane;
+    }
+    if (st != PFG_OK) break;
+    if (dalloc(h, &h->aux, (size_t)3 * h->plane) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (hipMemset(h->aux, 0, sizeof(float) * 3 * h->plane) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    h->tnx = (p.nx + TILE_X - 1) / TILE_X;
+    h->tny = (p.ny + TILE_Y - 1) / TILE_Y;
+    h->tnz = (p.nz + TILE_Z - 1) / TILE_Z;
+    h->ntiles = (int64_t)h->tnx * h->tny * h->tnz;
+    const size_t flag_bytes = (size_t)(h->ntiles + 3) / 4 * 4;
+    if (dalloc(h, &h->tile_flag, flag_bytes) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (hipMemset(h->tile_flag, 0, flag_bytes) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    if (dalloc(h, &h->tile_list, (size_t)h->ntiles + 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (hipMemset(h->tile_list, 0, sizeof(int32_t) * (h->ntiles + 1)) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    if (dalloc(h, &h->tile_done, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (hipMemset(h->tile_done, 0, sizeof(int32_t)) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    h->red_blocks = std::min<int64_t>(2048, (h->N + 255) / 256);
+    if (dalloc(h, &h->red, (size_t)4 * h->red_blocks) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (dalloc(h, &h->m.counters, 8) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (hipMemset(h->m.counters, 0, sizeof(int64_t) * 8) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    if (dalloc(h, &h->m.detach_step, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (dalloc(h, &h->m.d_step, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    const int64_t minus1 = -1, zero = 0;
+    if (hipMemcpy(h->m.detach_step, &minus1, sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->m.d_step, &zero, sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess) {
+      st = PFG_ERR_DEVICE;
+      break;
+    }
+    if (p.use_flags) {
+      if (dalloc(h, &h->solid, (size_t)h->N) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+      if (hipMemset(h->solid, 0, (size_t)h->N) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    }
+    /* f = equilibrium at rest until the caller says otherwise */
+    if (launch_init_eq(h, nullptr, nullptr, nullptr, nullptr) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+      st = PFG_ERR_LAUNCH;
+      break;
+    }
+  } while (0);
+  if (st != PFG_OK) {
+    pfg_destroy(h);
+    return st;
+  }
+  *out = h;
+  return PFG_OK;
+}
+
+pfg_status pfg_destroy(pfg_handle *h) {
+  if (!h) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  drop_graph(h);
+  free_markers(h->m);
+  dfree(h->m.counters);
+  dfree(h->m.detach_step);
+  dfree(h->m.d_step);
+  for (int k = 0; k < 2; ++k) {
+    if (h->f[k]) hipFree(h->f[k]);
+    h->f[k] = nullptr;
+  }
+  dfree(h->aux);
+  dfree(h->tile_flag);
+  dfree(h->tile_list);
+  dfree(h->tile_done);
+  dfree(h->solid);
+  dfree(h->red);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return PFG_OK;
+}
+
+pfg_status pfg_get_params(const pfg_handle *h, pfg_params *out) {
+  if (!h || !out) return PFG_ERR_ARG;
+  *out = h->p;
+  return PFG_OK;
+}
+
+pfg_status pfg_device_bytes(const pfg_handle *h, size_t *bytes) {
+  if (!h || !bytes) return PFG_ERR_ARG;
+  *bytes = h->dev_bytes;
+  return PFG_OK;
+}
+
+/* ---------------- lattice state ---------------- */
+
+pfg_status pfg_init_equilibrium(pfg_handle *h, const float *rho, const float *ux,
+                                const float *uy, const float *uz) {
+  if (!h) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  const float *src[4] = {rho, ux, uy, uz};
+  float *dev[4] = {nullptr, nullptr, nullptr, nullptr};
+  pfg_status st = PFG_OK;
+  for (int k = 0; k < 4 && st == PFG_OK; ++k) {
+    if (!src[k]) continue;
+    if (hipMalloc(&dev[k], sizeof(float) * h->N) != hipSuccess) {
+      st = PFG_ERR_ALLOC;
+      break;
+    }
+    st = from_hip(upload(h, dev[k], src[k], (size_t)h->N));
+  }
+  if (st == PFG_OK) {
+    st = from_hip(launch_init_eq(h, dev[0], dev[1], dev[2], dev[3]));
+    if (st == PFG_OK) st = from_hip(hipStreamSynchronize(h->stream));
+  }
+  for (int k = 0; k < 4; ++k)
+    if (dev[k]) hipFree(dev[k]);
+  return st;
+}
+
+pfg_status pfg_upload_f(pfg_handle *h, const float *f) {
+  if (!h || !f) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  return from_hip(launch_convert_f(h, nullptr, f));
+}
+
+pfg_status pfg_download_f(pfg_handle *h, float *f) {
+  if (!h || !f) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  return from_hip(launch_convert_f(h, f, nullptr));
+}
+
+pfg_status pfg_upload_flags(pfg_handle *h, const uint8_t *flags) {
+  if (!h || !flags) return PFG_ERR_ARG;
+  if (!h->solid) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  return from_hip(upload(h, h->solid, flags, (size_t)h->N));
+}
+
+pfg_status pfg_download_fields(pfg_handle *h, float *rho, float *ux, float *uy,
+                               float *uz) {
+  if (!h) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  float *dev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float *host[4] = {rho, ux, uy, uz};
+  pfg_status st = PFG_OK;
+  for (int k = 0; k < 4; ++k) {
+    if (!host[k]) continue;
+    if (hipMalloc(&dev[k], sizeof(float) * h->N) != hipSuccess) {
+      st = PFG_ERR_ALLOC;
+      break;
+    }
+    hipMemsetAsync(dev[k], 0, sizeof(float) * h->N, h->stream);
+  }
+  if (st == PFG_OK) st = from_hip(launch_lb_fields(h, dev[0], dev[1], dev[2], dev[3]));
+  for (int k = 0; k < 4 && st == PFG_OK; ++k)
+    if (host[k]) st = from_hip(download(h, host[k], dev[k], (size_t)h->N));
+  for (int k = 0; k < 4; ++k)
+    if (dev[k]) hipFree(dev[k]);
+  return st;
+}
+
+pfg_status pfg_conservation(pfg_handle *h, double out[4]) {
+  if (!h || !out) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  return from_hip(launch_conservation(h, out));
+}
+
+/* ---------------- immersed structures ---------------- */
+
+pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
+  if (!h || !md) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  const pfg_model &M = *md;
+  const int n = M.n_markers;
+  /* ---- validation ---- */
+  if (n < 0 || M.n_platelets < 0 || M.n_tri < 0 || M.n_edges < 0 ||
+      M.n_receptors < 0 || M.n_wall_ligands < 0 || M.n_links < 0)
+    return PFG_ERR_ARG;
+  if (n > 0 && (!M.x || !M.y || !M.z || !M.kind || !M.body)) return PFG_ERR_ARG;
+  for (int i = 0; i < n; ++i) {
+    if (!is_coupled_kind(M.kind[i])) return PFG_ERR_ARG;
+    if (M.kind[i] == PFG_MARKER_ELEMENT && (M.body[i] < 0 || M.body[i] >= M.n_platelets))
+      return PFG_ERR_ARG;
+    if (!std::isfinite(M.x[i]) || !std::isfinite(M.y[i]) || !std::isfinite(M.z[i]))
+      return PFG_ERR_ARG;
+  }
+  if (M.n_platelets > 0) {
+    if (!M.plt_elem_start || !M.plt_tri_start || !M.plt_S0 || !M.plt_V0) return PFG_ERR_ARG;
+    if (M.plt_elem_start[0] != 0 || M.plt_tri_start[0] != 0) return PFG_ERR_ARG;
+    for (int q = 0; q < M.n_platelets; ++q) {
+      if (M.plt_elem_start[q + 1] < M.plt_elem_start[q]) return PFG_ERR_ARG;
+      if (M.plt_tri_start[q + 1] < M.plt_tri_start[q]) return PFG_ERR_ARG;
+      for (int e = M.plt_elem_start[q]; e < M.plt_elem_start[q + 1]; ++e)
+        if (e >= n || M.kind[e] != PFG_MARKER_ELEMENT || M.body[e] != q) return PFG_ERR_ARG;
+    }
+    if (M.plt_tri_start[M.n_platelets] != M.n_tri) return PFG_ERR_ARG;
+  } else if (M.n_tri > 0) {
+    return PFG_ERR_ARG;
+  }
+  if (M.n_tri > 0 && (!M.tri || !M.tri_S0)) return PFG_ERR_ARG;
+  for (int q = 0; q < M.n_platelets; ++q)
+    for (int t = M.plt_tri_start[q]; t < M.plt_tri_start[q + 1]; ++t)
+      for (int k = 0; k < 3; ++k) {
+        const int v = M.tri[3 * t + k];
+        if (v < M.plt_elem_start[q] || v >= M.plt_elem_start[q + 1]) return PFG_ERR_ARG;
+      }
+  if (M.n_edges > 0 && (!M.edge || !M.edge_l0)) return PFG_ERR_ARG;
+  for (int e = 0; e < M.n_edges; ++e)
+    for (int k = 0; k < 2; ++k) {
+      const int v = M.edge[2 * e + k];
+      if (v < 0 || v >= n || M.kind[v] != PFG_MARKER_ELEMENT) return PFG_ERR_ARG;
+    }
+  if (M.n_receptors > 0 && (!M.rec_tri || !M.rec_r1 || !M.rec_r2)) return PFG_ERR_ARG;
+  for (int r = 0; r < M.n_receptors; ++r)
+    if (M.rec_tri[r] < 0 || M.rec_tri[r] >= M.n_tri) return PFG_ERR_ARG;
+  if (M.n_wall_ligands > 0 && (!M.lig_x || !M.lig_y)) return PFG_ERR_ARG;
+  if (M.n_links > 0 && (!M.link || !M.link_l0)) return PFG_ERR_ARG;
+  for (int l = 0; l < M.n_links; ++l)
+    for (int k = 0; k < 2; ++k) {
+      const int v = M.link[2 * l + k];
+      if (v < 0 || v >= n || M.kind[v] == PFG_MARKER_ELEMENT) return PFG_ERR_ARG;
+    }
+
+  drop_graph(h);
+  free_markers(h->m);
+  h->model_loaded = false;
+  Markers &m = h->m;
+  const pfg_params &p = h->p;
+  m.n = n;
+  m.n_plt = M.n_platelets;
+  m.n_tri = M.n_tri;
+  m.n_edges = M.n_edges;
+  m.n_rec = M.n_receptors;
+  m.n_wall_lig = M.n_wall_ligands;
+  m.fibrin_lig = M.fibrin_ligands ? 1 : 0;
+  m.n_lig = m.n_wall_lig + (m.fibrin_lig ? n : 0);
+  m.n_links = M.n_links;
+
+  /* ---- host-side CSR construction ---- */
+  std::vector<std::pair<int, int>> kv;
+  std::vector<int32_t> m2t_start, m2t, m2e_start, m2e_nbr, m2r_start, m2r, m2l_start,
+      m2l_nbr;
+  std::vector<float> m2e_l0, m2l_l0;
+  kv.clear();
+  for (int t = 0; t < M.n_tri; ++t)
+    for (int k = 0; k < 3; ++k) kv.push_back({M.tri[3 * t + k], t});
+  build_csr(n, kv, m2t_start, m2t);
+  kv.clear();
+  for (int e = 0; e < M.n_edges; ++e) {
+    kv.push_back({M.edge[2 * e], 2 * e});
+    kv.push_back({M.edge[2 * e + 1], 2 * e + 1});
+  }
+  {
+    std::vector<int32_t> tmp;
+    build_csr(n, kv, m2e_start, tmp);
+    m2e_nbr.resize(tmp.size());
+    m2e_l0.resize(tmp.size());
+    for (size_t k = 0; k < tmp.size(); ++k) {
+      const int e = tmp[k] / 2, side = tmp[k] % 2;
+      m2e_nbr[k] = M.edge[2 * e + (1 - side)];
+      m2e_l0[k] = M.edge_l0[e];
+    }
+  }
+  kv.clear();
+  for (int r = 0; r < M.n_receptors; ++r)
+    for (int k = 0; k < 3; ++k) kv.push_back({M.tri[3 * M.rec_tri[r] + k], 3 * r + k});
+  build_csr(n, kv, m2r_start, m2r);
+  kv.clear();
+  for (int l = 0; l < M.n_links; ++l) {
+    kv.push_back({M.link[2 * l], 2 * l});
+    kv.push_back({M.link[2 * l + 1], 2 * l + 1});
+  }
+  {
+    std::vector<int32_t> tmp;
+    build_csr(n, kv, m2l_start, tmp);
+    m2l_nbr.resize(tmp.size());
+    m2l_l0.resize(tmp.size());
+    for (size_t k = 0; k < tmp.size(); ++k) {
+      const int l = tmp[k] / 2, side = tmp[k] % 2;
+      m2l_nbr[k] = M.link[2 * l + (1 - side)];
+      m2l_l0[k] = M.link_l0[l];
+    }
+  }
+  /* cell-list geometry */
+  m.nb[0] = std::max(1, p.nx / p.bin_cells);
+  m.nb[1] = std::max(1, p.ny / p.bin_cells);
+  m.nb[2] = std::max(1, p.nz / p.bin_cells);
+  m.nbins = (int64_t)m.nb[0] * m.nb[1] * m.nb[2];
+  m.scan_blocks = (m.nbins + 1023) / 1024;
+  /* static ligands on the same bin grid, ascending ligand id per bin */
+  std::vector<int32_t> lg_start, lg_items;
+  std::vector<float> lig_z_host((size_t)M.n_wall_ligands, p.wall_z);
+  {
+    kv.clear();
+    auto cell = [](float v, int nn, int per) {
+      int c = (int)std::floor(v);
+      if (per) {
+        c %= nn;
+        if (c < 0) c += nn;
+      } else {
+        c = std::min(std::max(c, 0), nn - 1);
+      }
+      return c;
+    };
+    for (int l = 0; l < M.n_wall_ligands; ++l) {
+      if (M.lig_z) lig_z_host[(size_t)l] = M.lig_z[l];
+      const float c3[3] = {M.lig_x[l], M.lig_y[l], lig_z_host[(size_t)l]};
+      const int nn[3] = {p.nx, p.ny, p.nz};
+      int b[3];
+      for (int k = 0; k < 3; ++k)
+        b[k] = cell(c3[k], nn[k], p.wall[k] == PFG_WALL_PERIODIC) * m.nb[k] / nn[k];
+      kv.push_back({b[0] + m.nb[0] * (b[1] + m.nb[1] * b[2]), l});
+    }
+    build_csr((int)m.nbins, kv, lg_start, lg_items);
+  }
+
+  /* ---- device allocation + upload ---- */
+  hipError_t e = hipSuccess;
+#define ALLOC_UP(ptr, src, count)                                   \
+  do {                                                              \
+    if (e == hipSuccess) e = dalloc(h, &(ptr), (size_t)(count));    \
+    if (e == hipSuccess && (src) != nullptr) e = upload(h, (ptr), (src), (size_t)(count)); \
+  } while (0)
+#define ALLOC_FILL(ptr, value, count)                               \
+  do {                                                              \
+    if (e == hipSuccess) e = dalloc(h, &(ptr), (size_t)(count));    \
+    if (e == hipSuccess) e = fill(h, (ptr), (value), (size_t)(count)); \
+  } while (0)
+  ALLOC_UP(m.x, M.x, n);
+  ALLOC_UP(m.y, M.y, n);
+  ALLOC_UP(m.z, M.z, n);
+  ALLOC_FILL(m.vx, 0.0f, n);
+  ALLOC_FILL(m.vy, 0.0f, n);
+  ALLOC_FILL(m.vz, 0.0f, n);
+  ALLOC_FILL(m.fx, 0.0f, n);
+  ALLOC_FILL(m.fy, 0.0f, n);
+  ALLOC_FILL(m.fz, 0.0f, n);
+  ALLOC_UP(m.kind, M.kind, n);
+  ALLOC_UP(m.body, M.body, n);
+  if (M.n_platelets > 0) {
+    ALLOC_UP(m.plt_elem_start, M.plt_elem_start, M.n_platelets + 1);
+    ALLOC_UP(m.plt_tri_start, M.plt_tri_start, M.n_platelets + 1);
+  } else {
+    const int32_t zero2[2] = {0, 0};
+    ALLOC_UP(m.plt_elem_start, zero2, 1);
+    ALLOC_UP(m.plt_tri_start, zero2, 1);
+  }
+  ALLOC_UP(m.tri, M.tri, 3 * (size_t)M.n_tri);
+  ALLOC_UP(m.tri_S0, M.tri_S0, M.n_tri);
+  ALLOC_FILL(m.tri_S, 0.0, M.n_tri);
+  ALLOC_FILL(m.tri_n, 0.0f, 3 * (size_t)M.n_tri);
+  ALLOC_UP(m.plt_S0, M.plt_S0, M.n_platelets);
+  ALLOC_UP(m.plt_V0, M.plt_V0, M.n_platelets);
+  ALLOC_FILL(m.plt_S, 0.0, M.n_platelets);
+  ALLOC_FILL(m.plt_V, 0.0, M.n_platelets);
+  ALLOC_FILL(m.plt_lowest, -1, M.n_platelets);
+  ALLOC_UP(m.m2t_start, m2t_start.data(), m2t_start.size());
+  ALLOC_UP(m.m2t, m2t.data(), m2t.size());
+  ALLOC_UP(m.m2e_start, m2e_start.data(), m2e_start.size());
+  ALLOC_UP(m.m2e_nbr, m2e_nbr.data(), m2e_nbr.size());
+  ALLOC_UP(m.m2e_l0, m2e_l0.data(), m2e_l0.size());
+  ALLOC_UP(m.rec_tri, M.rec_tri, M.n_receptors);
+  ALLOC_UP(m.rec_r1, M.rec_r1, M.n_receptors);
+  ALLOC_UP(m.rec_r2, M.rec_r2, M.n_receptors);
+  ALLOC_FILL(m.rec_bond, -1, M.n_receptors);
+  ALLOC_FILL(m.rec_cand, -1, M.n_receptors);
+  ALLOC_FILL(m.rec_release, 0, M.n_receptors);
+  ALLOC_FILL(m.rec_force, 0.0f, 3 * (size_t)M.n_receptors);
+  ALLOC_FILL(m.rec_w, 0.0f, 3 * (size_t)M.n_receptors);
+  ALLOC_UP(m.m2r_start, m2r_start.data(), m2r_start.size());
+  ALLOC_UP(m.m2r, m2r.data(), m2r.size());
+  ALLOC_UP(m.lig_x, M.lig_x, M.n_wall_ligands);
+  ALLOC_UP(m.lig_y, M.lig_y, M.n_wall_ligands);
+  ALLOC_UP(m.lig_z, lig_z_host.data(), M.n_wall_ligands);
+  ALLOC_FILL(m.lig_owner, -1, m.n_lig);
+  ALLOC_FILL(m.lig_claim, (int32_t)INT_MAX, m.n_lig);
+  ALLOC_UP(m.lg_start, lg_start.data(), lg_start.size());
+  ALLOC_UP(m.lg_items, lg_items.data(), lg_items.size());
+  ALLOC_UP(m.m2l_start, m2l_start.data(), m2l_start.size());
+  ALLOC_UP(m.m2l_nbr, m2l_nbr.data(), m2l_nbr.size());
+  ALLOC_UP(m.m2l_l0, m2l_l0.data(), m2l_l0.size());
+  ALLOC_FILL(m.bin_count, 0, m.nbins);
+  ALLOC_FILL(m.bin_start, 0, m.nbins + 1);
+  ALLOC_FILL(m.bin_items, 0, n);
+  ALLOC_FILL(m.marker_bin, 0, n);
+  ALLOC_FILL(m.marker_slot, 0, n);
+  ALLOC_FILL(m.scan_partials, 0, m.scan_blocks);
+#undef ALLOC_UP
+#undef ALLOC_FILL
+  if (e != hipSuccess) {
+    free_markers(h->m);
+    return from_hip(e);
+  }
+  /* fresh counters and step index */
+  const int64_t zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0}, minus1 = -1;
+  PFG_TRY(upload(h, m.counters, zeros, 8));
+  PFG_TRY(upload(h, m.detach_step, &minus1, 1));
+  PFG_TRY(upload(h, m.d_step, zeros, 1));
+  h->h_step = 0;
+  h->model_loaded = true;
+  return PFG_OK;
+}
+
+pfg_status pfg_upload_markers(pfg_handle *h, const float *x, const float *y,
+                              const float *z) {
+  if (!h || !x || !y || !z) return PFG_ERR_ARG;
+  if (!h->model_loaded) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  const size_t n = (size_t)h->m.n;
+  PFG_TRY(upload(h, h->m.x, x, n));
+  PFG_TRY(upload(h, h->m.y, y, n));
+  PFG_TRY(upload(h, h->m.z, z, n));
+  return PFG_OK;
+}
+
+pfg_status pfg_download_markers(pfg_handle *h, float *x, float *y, float *z,
+                                float *vx, float *vy, float *vz, float *fx,
+                                float *fy, float *fz) {
+  if (!h) return PFG_ERR_ARG;
+  if (!h->model_loaded) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  const size_t n = (size_t)h->m.n;
+  PFG_TRY(download(h, x, h->m.x, n));
+  PFG_TRY(download(h, y, h->m.y, n));
+  PFG_TRY(download(h, z, h->m.z, n));
+  PFG_TRY(download(h, vx, h->m.vx, n));
+  PFG_TRY(download(h, vy, h->m.vy, n));
+  PFG_TRY(download(h, vz, h->m.vz, n));
+  PFG_TRY(download(h, fx, h->m.fx, n));
+  PFG_TRY(download(h, fy, h->m.fy, n));
+  PFG_TRY(download(h, fz, h->m.fz, n));
+  return PFG_OK;
+}
+
+pfg_status pfg_upload_marker_forces(pfg_handle *h, const float *fx,
+                                    const float *fy, const float *fz) {
+  if (!h || !fx || !fy || !fz) return PFG_ERR_ARG;
+  if (!h->model_loaded) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  const size_t n = (size_t)h->m.n;
+  PFG_TRY(upload(h, h->m.fx, fx, n));
+  PFG_TRY(upload(h, h->m.fy, fy, n));
+  PFG_TRY(upload(h, h->m.fz, fz, n));
+  return PFG_OK;
+}
+
+pfg_status pfg_upload_marker_velocities(pfg_handle *h, const float *vx,
+                                        const float *vy, const float *vz) {
+  if (!h || !vx || !vy || !vz) return PFG_ERR_ARG;
+  if (!h->model_loaded) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  const size_t n = (size_t)h->m.n;
+  PFG_TRY(upload(h, h->m.vx, vx, n));
+  PFG_TRY(upload(h, h->m.vy, vy, n));
+  PFG_TRY(upload(h, h->m.vz, vz, n));
+  return PFG_OK;
+}
+
+pfg_status pfg_download_membrane(pfg_handle *h, double *S_all, double *V,
+                                 double *tri_S) {
+  if (!h) return PFG_ERR_ARG;
+  if (!h->model_loaded) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  PFG_TRY(download(h, S_all, h->m.plt_S, (size_t)h->m.n_plt));
+  PFG_TRY(download(h, V, h->m.plt_V, (size_t)h->m.n_plt));
+  PFG_TRY(download(h, tri_S, h->m.tri_S, (size_t)h->m.n_tri));
+  return PFG_OK;
+}
+
+pfg_status pfg_download_bonds(pfg_handle *h, int32_t *rec_bond, int32_t *lig_owner,
+                              float *rec_force) {
+  if (!h) return PFG_ERR_ARG;
+  if (!h->model_loaded) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  PFG_TRY(download(h, rec_bond, h->m.rec_bond, (size_t)h->m.n_rec));
+  PFG_TRY(download(h, lig_owner, h->m.lig_owner, (size_t)h->m.n_lig));
+  PFG_TRY(download(h, rec_force, h->m.rec_force, 3 * (size_t)h->m.n_rec));
+  return PFG_OK;
+}
+
+pfg_status pfg_upload_bonds(pfg_handle *h, const int32_t *rec_bond) {
+  if (!h || !rec_bond) return PFG_ERR_ARG;
+  if (!h->model_loaded) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  const Markers &m = h->m;
+  std::vector<int32_t> owner((size_t)m.n_lig, -1);
+  for (int r = 0; r < m.n_rec; ++r) {
+    const int l = rec_bond[r];
+    if (l < -1 || l >= m.n_lig) return PFG_ERR_ARG;
+    if (l >= m.n_wall_lig && l >= 0) {
+      if (!m.fibrin_lig) return PFG_ERR_ARG;
+    }
+    if (l >= 0) {
+      if (owner[(size_t)l] >= 0) return PFG_ERR_ARG;
+      owner[(size_t)l] = r;
+    }
+  }
+  PFG_TRY(upload(h, m.rec_bond, rec_bond, (size_t)m.n_rec));
+  PFG_TRY(upload(h, m.lig_owner, owner.data(), owner.size()));
+  return PFG_OK;
+}
+
+pfg_status pfg_cell_grid(const pfg_handle *h, int32_t dims[3]) {
+  if (!h || !dims) return PFG_ERR_ARG;
+  if (!h->model_loaded) return PFG_ERR_STATE;
+  for (int k = 0; k < 3; ++k) dims[k] = h->m.nb[k];
+  return PFG_OK;
+}
+
+pfg_status pfg_download_cells(pfg_handle *h, int32_t *bin_start, int32_t *bin_items) {
+  if (!h) return PFG_ERR_ARG;
+  if (!h->model_loaded) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  PFG_TRY(download(h, bin_start, h->m.bin_start, (size_t)h->m.nbins + 1));
+  PFG_TRY(download(h, bin_items, h->m.bin_items, (size_t)h->m.n));
+  return PFG_OK;
+}
+
+pfg_status pfg_download_aux(pfg_handle *h, float *a3n) {
+  if (!h || !a3n) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  for (int k = 0; k < 3; ++k)
+    PFG_TRY(download(h, a3n + (size_t)k * h->N, h->aux + (size_t)k * h->plane,
+                     (size_t)h->N));
+  return PFG_OK;
+}
+
+/* ---------------- stepping ---------------- */
+
+static hipError_t run_stage(pfg_handle *h, int s) {
+  switch (s) {
+    case PFG_STAGE_CELLS: return launch_cells(h);
+    case PFG_STAGE_MEMBRANE: return launch_membrane(h);
+    case PFG_STAGE_BONDS: return launch_bonds(h);
+    case PFG_STAGE_FORCES: return launch_forces(h);
+    case PFG_STAGE_SPREAD: return launch_spread(h);
+    case PFG_STAGE_LB: return launch_lb(h);
+    case PFG_STAGE_INTERP: return launch_interp(h);
+    case PFG_STAGE_UPDATE: return launch_update(h);
+    case PFG_STAGE_CLEAR: return launch_clear(h);
+  }
+  return hipErrorInvalidValue;
+}
+
+/* One coupled step, in the order of the reference's per-step calls
+ * (gpuSEM_LB.c:1441-1442 -> semLB.cu:441-531 then semLB.cu:303-438):
+ * forces from X(n), spread, LB with F(n), interpolate, move. */
+static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
+  hipError_t e = hipSuccess;
+  const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
+  if (markers) {
+    for (int s = PFG_STAGE_CELLS; s <= PFG_STAGE_SPREAD && e == hipSuccess; ++s)
+      e = run_stage(h, s);
+  }
+  if (e == hipSuccess && lb0) e = hipEventRecord(lb0, h->stream);
+  if (e == hipSuccess) e = launch_lb(h);
+  if (e == hipSuccess && lb1) e = hipEventRecord(lb1, h->stream);
+  if (markers) {
+    for (int s = PFG_STAGE_INTERP; s <= PFG_STAGE_CLEAR && e == hipSuccess; ++s)
+      e = run_stage(h, s);
+  }
+  return e;
+}
+
+static hipError_t build_graphs(pfg_handle *h) {
+  drop_graph(h);
+  const int cur0 = h->cur;
+  hipError_t e = hipSuccess;
+  for (int g = 0; g < 2 && e == hipSuccess; ++g) {
+    h->cur = g;
+    hipGraph_t graph = nullptr;
+    e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) break;
+    hipError_t e2 = enqueue_step(h, nullptr, nullptr);
+    e = hipStreamEndCapture(h->stream, &graph);
+    if (e2 != hipSuccess) e = e2;
+    if (e == hipSuccess) {
+      hipGraphExec_t ex = nullptr;
+      e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      if (e == hipSuccess) {
+        if (g == 0) h->graph_exec = ex; else h->graph_exec2 = ex;
+      }
+    }
+    if (graph) hipGraphDestroy(graph);
+  }
+  h->cur = cur0;
+  if (e == hipSuccess) h->graph_valid = true;
+  else drop_graph(h);
+  return e;
+}
+
+static pfg_status fill_info(pfg_handle *h, pfg_step_info *info) {
+  if (!info) return PFG_OK;
+  int64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t ds = -1;
+  PFG_TRY(download(h, c, h->m.counters, 8));
+  PFG_TRY(download(h, &ds, h->m.detach_step, 1));
+  info->step = h->h_step;
+  info->total_bonds = (int32_t)c[0];
+  info->detached = (int32_t)c[1];
+  info->detach_step = ds;
+  info->near_threshold = c[2];
+  info->bond_events = c[4];
+  return PFG_OK;
+}
+
+pfg_status pfg_step(pfg_handle *h, int nsteps, pfg_step_info *info) {
+  if (!h || nsteps < 0) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  if (h->use_graph && nsteps > 0 && !h->graph_valid) PFG_TRY(build_graphs(h));
+  for (int s = 0; s < nsteps; ++s) {
+    if (h->use_graph) {
+      PFG_TRY(hipGraphLaunch(h->cur == 0 ? h->graph_exec : h->graph_exec2, h->stream));
+      h->cur = 1 - h->cur;
+    } else {
+      PFG_TRY(enqueue_step(h, nullptr, nullptr));
+    }
+    h->h_step++;
+  }
+  PFG_TRY(hipStreamSynchronize(h->stream));
+  return fill_info(h, info);
+}
+
+pfg_status pfg_run_stage(pfg_handle *h, int stage) {
+  if (!h || stage < 0 || stage >= PFG_STAGE_COUNT) return PFG_ERR_ARG;
+  if (stage != PFG_STAGE_LB && !h->model_loaded) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  PFG_TRY(run_stage(h, stage));
+  PFG_TRY(hipStreamSynchronize(h->stream));
+  return PFG_OK;
+}
+
+pfg_status pfg_set_step(pfg_handle *h, int64_t step) {
+  if (!h || step < 0) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  PFG_TRY(upload(h, h->m.d_step, &step, 1));
+  h->h_step = step;
+  return PFG_OK;
+}
+
+pfg_status pfg_set_graph(pfg_handle *h, int enable) {
+  if (!h) return PFG_ERR_ARG;
+  h->use_graph = enable != 0;
+  drop_graph(h);
+  return PFG_OK;
+}
+
+pfg_status pfg_set_fluid_only(pfg_handle *h, int fluid_only) {
+  if (!h) return PFG_ERR_ARG;
+  h->fluid_only = fluid_only != 0;
+  drop_graph(h);
+  return PFG_OK;
+}
+
+pfg_status pfg_synchronize(pfg_handle *h) {
+  if (!h) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  PFG_TRY(hipStreamSynchronize(h->stream));
+  return PFG_OK;
+}
+
+pfg_status pfg_profile(pfg_handle *h, int nsteps, double *ms_total, double *ms_lb) {
+  if (!h || nsteps <= 0) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  std::vector<hipEvent_t> ev((size_t)2 * nsteps, nullptr);
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipError_t e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  for (size_t k = 0; k < ev.size() && e == hipSuccess; ++k) e = hipEventCreate(&ev[k]);
+  if (e == hipSuccess) e = hipEventRecord(t0, h->stream);
+  for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
+    e = enqueue_step(h, ev[2 * (size_t)s], ev[2 * (size_t)s + 1]);
+    h->h_step++;
+  }
+  if (e == hipSuccess) e = hipEventRecord(t1, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  double lb = 0.0;
+  float ms = 0.f;
+  for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
+    e = hipEventElapsedTime(&ms, ev[2 * (size_t)s], ev[2 * (size_t)s + 1]);
+    lb += ms;
+  }
+  float tot = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&tot, t0, t1);
+  for (auto x : ev)
+    if (x) hipEventDestroy(x);
+  if (t0) hipEventDestroy(t0);
+  if (t1) hipEventDestroy(t1);
+  if (e != hipSuccess) return from_hip(e);
+  if (ms_total) *ms_total = tot;
+  if (ms_lb) *ms_lb = lb;
+  return PFG_OK;
+}
+
+pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_thread, int block_y) {
+  if (!h) return PFG_ERR_ARG;
+  if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4)
+    return PFG_ERR_ARG;
+  if (block_y != 1 && block_y != 2 && block_y != 4) return PFG_ERR_ARG;
+  h->lb_cpt = cells_per_thread;
+  h->lb_by = block_y;
+  drop_graph(h);
+  return PFG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,467 @@
+/*
+ * lb.hip -- fused D3Q19 pull-stream + bounce-back + moments + BGK/Guo collide.
+ *
+ * One pass per step replaces the reference's four full-grid passes
+ * (fluid3d_collision_kernel LB_kernel.cu:1203-1304, fluid3d_stream_kernel
+ * LB_kernel.cu:1312-1359, fluid3d_velocity_density_kernel LB_kernel.cu:1108-1199
+ * and the wall kernels LB_kernel.cu:395-780): each cell pulls its 19
+ * post-collision populations from the upstream neighbours of the previous step
+ * (bounce-back where the upstream cell is a wall or a solid cell), takes its
+ * moments with the current force, relaxes and writes its 19 new populations.
+ * Each population is read once and written once: 152 B per cell update in
+ * FP32 storage, 76 B in 16-bit storage.
+ *
+ * Storage holds deviations f_i - w_i (FP32), (f_i - w_i) * 2^s (FP16) or
+ * f_i - w_i (BF16); arithmetic is FP32 in deviation form throughout
+ * (rho - 1 = sum of deviations), so FP32 rounding acts on the deviations and
+ * not on the O(1) weights.
+ *
+ * Layout: plane d of copy k at f[k] + d*plane, cell index x + nx*(y + ny*z).
+ * A wavefront covers 64 consecutive x: loads of the x-shifted directions are
+ * misaligned by one element but stay within two cache lines that the
+ * neighbouring wavefront of the same block also reads.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "pfg_internal.h"
+
+namespace pfg {
+
+/* ---------------- storage formats ---------------- */
+
+template <int S>
+struct Store;
+
+template <>
+struct Store<PFG_STORE_F32> {
+  using T = float;
+  __device__ static inline float load(const T *p, int64_t i, float) {
+    return p[i];
+  }
+  __device__ static inline void put(T *p, int64_t i, float v, float) {
+    __builtin_nontemporal_store(v, p + i);
+  }
+};
+
+template <>
+struct Store<PFG_STORE_F16> {
+  using T = uint16_t;
+  __device__ static inline float load(const T *p, int64_t i, float inv_scale) {
+    uint16_t b = p[i];
+    return (float)__builtin_bit_cast(_Float16, b) * inv_scale;
+  }
+  __device__ static inline void put(T *p, int64_t i, float v, float scale) {
+    _Float16 hv = (_Float16)(v * scale); /* round to nearest even */
+    __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, hv), p + i);
+  }
+};
+
+template <>
+struct Store<PFG_STORE_BF16> {
+  using T = uint16_t;
+  __device__ static inline float load(const T *p, int64_t i, float) {
+    uint32_t u = ((uint32_t)p[i]) << 16;
+    return __builtin_bit_cast(float, u);
+  }
+  __device__ static inline void put(T *p, int64_t i, float v, float) {
+    uint32_t u = __builtin_bit_cast(uint32_t, v);
+    u += 0x7FFFu + ((u >> 16) & 1u); /* round to nearest even */
+    __builtin_nontemporal_store((uint16_t)(u >> 16), p + i);
+  }
+};
+
+/* ---------------- the fused kernel ---------------- */
+
+enum { LB_MODE_STEP = 0, LB_MODE_FIELDS = 1 };
+
+template <int S, int CPT, int MODE>
+__global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho,
+                                                       float *oux, float *ouy,
+                                                       float *ouz) {
+  using ST = Store<S>;
+  using T = typename ST::T;
+  const T *__restrict__ fin = (const T *)a.fin;
+  T *__restrict__ fout = (T *)a.fout;
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= a.nx || y >= a.ny) return;
+  const int64_t nxy = (int64_t)a.nx * a.ny;
+
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int z = blockIdx.z * CPT + c;
+    if (z >= a.nz) return;
+    const int64_t idx = x + (int64_t)a.nx * y + nxy * z;
+    if (a.solid != nullptr && a.solid[idx]) continue;
+
+    float dev[Q];
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      int sx = x - CX[d], sy = y - CY[d], sz = z - CZ[d];
+      int wall = -1, side = 0;
+      if (CX[d] != 0) {
+        if (sx < 0) {
+          if (a.periodic[0]) sx += a.nx; else { wall = 0; side = 0; }
+        } else if (sx >= a.nx) {
+          if (a.periodic[0]) sx -= a.nx; else { wall = 0; side = 1; }
+        }
+      }
+      if (CY[d] != 0) {
+        if (sy < 0) {
+          if (a.periodic[1]) sy += a.ny; else { wall = 1; side = 0; }
+        } else if (sy >= a.ny) {
+          if (a.periodic[1]) sy -= a.ny; else { wall = 1; side = 1; }
+        }
+      }
+      if (CZ[d] != 0) {
+        if (sz < 0) {
+          if (a.periodic[2]) sz += a.nz; else { wall = 2; side = 0; }
+        } else if (sz >= a.nz) {
+          if (a.periodic[2]) sz -= a.nz; else { wall = 2; side = 1; }
+        }
+      }
+      if (d == 0) {
+        dev[d] = ST::load(fin, idx, a.inv_scale);
+      } else if (wall >= 0) {
+        dev[d] = ST::load(fin + (int64_t)BB[d] * a.plane, idx, a.inv_scale) +
+                 a.wall_term[wall][side][d];
+      } else {
+        const int64_t sidx = sx + (int64_t)a.nx * sy + nxy * sz;
+        if (a.solid != nullptr && a.solid[sidx]) {
+          dev[d] = ST::load(fin + (int64_t)BB[d] * a.plane, idx, a.inv_scale);
+        } else {
+          dev[d] = ST::load(fin + (int64_t)d * a.plane, sidx, a.inv_scale);
+        }
+      }
+    }
+
+    /* moments: rho - 1 and momentum from deviations (sum_i w_i c_i = 0) */
+    float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      drho += dev[d];
+      if (CX[d] > 0) jx += dev[d]; else if (CX[d] < 0) jx -= dev[d];
+      if (CY[d] > 0) jy += dev[d]; else if (CY[d] < 0) jy -= dev[d];
+      if (CZ[d] > 0) jz += dev[d]; else if (CZ[d] < 0) jz -= dev[d];
+    }
+    const float rho = 1.0f + drho;
+    float Fx = a.g[0], Fy = a.g[1], Fz = a.g[2];
+    bool flagged = false;
+    int64_t aidx = 0;
+    if (MODE == LB_MODE_STEP) {
+      const int tile = (x >> TILE_SHIFT_X) +
+                       a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
+      flagged = a.tile_flag[tile] != 0;
+      if (flagged) {
+        aidx = idx;
+        Fx += a.aux[aidx];
+        Fy += a.aux[a.plane + aidx];
+        Fz += a.aux[2 * a.plane + aidx];
+      }
+    }
+    const float inv_rho = 1.0f / rho;
+    const float ux = (jx + 0.5f * Fx) * inv_rho;
+    const float uy = (jy + 0.5f * Fy) * inv_rho;
+    const float uz = (jz + 0.5f * Fz) * inv_rho;
+
+    if (MODE == LB_MODE_FIELDS) {
+      if (orho) orho[idx] = rho;
+      if (oux) oux[idx] = ux;
+      if (ouy) ouy[idx] = uy;
+      if (ouz) ouz[idx] = uz;
+      continue;
+    }
+    if (flagged) { /* velocity for the IB interpolation of this step */
+      a.aux[aidx] = ux;
+      a.aux[a.plane + aidx] = uy;
+      a.aux[2 * a.plane + aidx] = uz;
+    }
+
+    const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+    const float uF = ux * Fx + uy * Fy + uz * Fz;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      const float w = weight(d);
+      float cu = 0.f, cF = 0.f;
+      if (CX[d] > 0) { cu += ux; cF += Fx; } else if (CX[d] < 0) { cu -= ux; cF -= Fx; }
+      if (CY[d] > 0) { cu += uy; cF += Fy; } else if (CY[d] < 0) { cu -= uy; cF -= Fy; }
+      if (CZ[d] > 0) { cu += uz; cF += Fz; } else if (CZ[d] < 0) { cu -= uz; cF -= Fz; }
+      const float feq_dev = w * drho + w * rho * (3.0f * cu + 4.5f * cu * cu - usq15);
+      const float src = a.guo * w * (3.0f * (cF - uF) + 9.0f * cu * cF);
+      const float out = dev[d] + a.omega * (feq_dev - dev[d]) + src;
+      ST::put(fout + (int64_t)d * a.plane, idx, out, a.scale);
+    }
+  }
+}
+
+/* ---------------- auxiliary kernels ---------------- */
+
+template <int S>
+__global__ void init_eq_kernel(LBArgs a, void *dst0, void *dst1,
+                               const float *rho_in, const float *ux_in,
+                               const float *uy_in, const float *uz_in,
+                               int64_t n) {
+  using ST = Store<S>;
+  using T = typename ST::T;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const float rho = rho_in ? rho_in[idx] : 1.0f;
+  const float ux = ux_in ? ux_in[idx] : 0.f;
+  const float uy = uy_in ? uy_in[idx] : 0.f;
+  const float uz = uz_in ? uz_in[idx] : 0.f;
+  const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+  const float drho = rho - 1.0f;
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    const float w = weight(d);
+    const float cu = CX[d] * ux + CY[d] * uy + CZ[d] * uz;
+    const float dev = w * drho + w * rho * (3.0f * cu + 4.5f * cu * cu - usq15);
+    ST::put((T *)dst0 + (int64_t)d * a.plane, idx, dev, a.scale);
+    ST::put((T *)dst1 + (int64_t)d * a.plane, idx, dev, a.scale);
+  }
+}
+
+/* plane d: to_float -> out[i] = w_d + dev; else stored <- in[i] - w_d */
+template <int S>
+__global__ void convert_plane_kernel(LBArgs a, void *stored, float *buf,
+                                     int d, int to_float, int64_t n) {
+  using ST = Store<S>;
+  using T = typename ST::T;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  T *p = (T *)stored + (int64_t)d * a.plane;
+  const float w = weight(d);
+  if (to_float) {
+    buf[idx] = w + ST::load(p, idx, a.inv_scale);
+  } else {
+    ST::put(p, idx, buf[idx] - w, a.scale);
+  }
+}
+
+/* FP64 totals of mass and momentum over fluid cells, one partial per block. */
+template <int S>
+__global__ __launch_bounds__(256) void conservation_kernel(LBArgs a,
+                                                          const void *stored,
+                                                          double *partials,
+                                                          int64_t n) {
+  using ST = Store<S>;
+  using T = typename ST::T;
+  const T *p = (const T *)stored;
+  double m = 0, jx = 0, jy = 0, jz = 0;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    if (a.solid != nullptr && a.solid[idx]) continue;
+    double cm = 1.0, cx = 0, cy = 0, cz = 0;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      const double v = (double)ST::load(p + (int64_t)d * a.plane, idx, a.inv_scale);
+      cm += v;
+      cx += CX[d] * v;
+      cy += CY[d] * v;
+      cz += CZ[d] * v;
+    }
+    m += cm;
+    jx += cx;
+    jy += cy;
+    jz += cz;
+  }
+  __shared__ double sh[4][256];
+  const int t = threadIdx.x;
+  sh[0][t] = m;
+  sh[1][t] = jx;
+  sh[2][t] = jy;
+  sh[3][t] = jz;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      for (int k = 0; k < 4; ++k) sh[k][t] += sh[k][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    for (int k = 0; k < 4; ++k) partials[4 * blockIdx.x + k] = sh[k][0];
+  }
+}
+
+/* ---------------- host launchers ---------------- */
+
+void make_lb_args(const pfg_handle *h, LBArgs *a) {
+  const pfg_params &p = h->p;
+  a->fin = h->f[h->cur];
+  a->fout = h->f[1 - h->cur];
+  a->aux = h->aux;
+  a->tile_flag = h->tile_flag;
+  a->solid = h->solid;
+  a->nx = p.nx;
+  a->ny = p.ny;
+  a->nz = p.nz;
+  a->plane = h->plane;
+  a->tnx = h->tnx;
+  a->tny = h->tny;
+  for (int k = 0; k < 3; ++k) a->periodic[k] = p.wall[k] == PFG_WALL_PERIODIC;
+  a->omega = 1.0f / p.tau;
+  a->guo = 1.0f - 0.5f / p.tau;
+  for (int k = 0; k < 3; ++k) a->g[k] = p.body_force[k];
+  if (p.store == PFG_STORE_F16) {
+    a->scale = ldexpf(1.0f, p.f16_scale_log2);
+    a->inv_scale = ldexpf(1.0f, -p.f16_scale_log2);
+  } else {
+    a->scale = 1.0f;
+    a->inv_scale = 1.0f;
+  }
+  /* halfway bounce-back with a moving wall (Ladd): the population entering
+   * from the wall is f_bb(d) + 6 w_d rho_w (c_d . u_w), rho_w = 1 */
+  for (int ax = 0; ax < 3; ++ax)
+    for (int s = 0; s < 2; ++s)
+      for (int d = 0; d < Q; ++d) {
+        const float *u = s == 0 ? p.wall_u_lo[ax] : p.wall_u_hi[ax];
+        const float cu = CX[d] * u[0] + CY[d] * u[1] + CZ[d] * u[2];
+        a->wall_term[ax][s][d] = 6.0f * weight(d) * cu;
+      }
+}
+
+template <int S, int CPT>
+static hipError_t launch_lb_t(pfg_handle *h, const LBArgs &a, int mode,
+                              float *rho, float *ux, float *uy, float *uz) {
+  const int by = h->lb_by;
+  dim3 block(64, by, 1);
+  dim3 grid((a.nx + 63) / 64, (a.ny + by - 1) / by, (a.nz + CPT - 1) / CPT);
+  if (mode == LB_MODE_STEP)
+    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_STEP>), grid, block, 0,
+                       h->stream, a, nullptr, nullptr, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_FIELDS>), grid, block, 0,
+                       h->stream, a, rho, ux, uy, uz);
+  return hipGetLastError();
+}
+
+template <int S>
+static hipError_t launch_lb_s(pfg_handle *h, const LBArgs &a, int mode,
+                              float *rho, float *ux, float *uy, float *uz) {
+  switch (h->lb_cpt) {
+    case 2: return launch_lb_t<S, 2>(h, a, mode, rho, ux, uy, uz);
+    case 4: return launch_lb_t<S, 4>(h, a, mode, rho, ux, uy, uz);
+    default: return launch_lb_t<S, 1>(h, a, mode, rho, ux, uy, uz);
+  }
+}
+
+static hipError_t dispatch_lb(pfg_handle *h, int mode, float *rho, float *ux,
+                              float *uy, float *uz) {
+  LBArgs a;
+  make_lb_args(h, &a);
+  switch (h->p.store) {
+    case PFG_STORE_F16: return launch_lb_s<PFG_STORE_F16>(h, a, mode, rho, ux, uy, uz);
+    case PFG_STORE_BF16: return launch_lb_s<PFG_STORE_BF16>(h, a, mode, rho, ux, uy, uz);
+    default: return launch_lb_s<PFG_STORE_F32>(h, a, mode, rho, ux, uy, uz);
+  }
+}
+
+hipError_t launch_lb(pfg_handle *h) {
+  hipError_t e = dispatch_lb(h, LB_MODE_STEP, nullptr, nullptr, nullptr, nullptr);
+  if (e == hipSuccess) h->cur = 1 - h->cur;
+  return e;
+}
+
+hipError_t launch_lb_fields(pfg_handle *h, float *rho, float *ux, float *uy,
+                            float *uz) {
+  return dispatch_lb(h, LB_MODE_FIELDS, rho, ux, uy, uz);
+}
+
+hipError_t launch_init_eq(pfg_handle *h, const float *rho, const float *ux,
+                          const float *uy, const float *uz) {
+  LBArgs a;
+  make_lb_args(h, &a);
+  const int64_t n = h->N;
+  dim3 block(256), grid((unsigned)((n + 255) / 256));
+  switch (h->p.store) {
+    case PFG_STORE_F16:
+      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F16>, grid, block, 0, h->stream,
+                         a, h->f[0], h->f[1], rho, ux, uy, uz, n);
+      break;
+    case PFG_STORE_BF16:
+      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_BF16>, grid, block, 0, h->stream,
+                         a, h->f[0], h->f[1], rho, ux, uy, uz, n);
+      break;
+    default:
+      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F32>, grid, block, 0, h->stream,
+                         a, h->f[0], h->f[1], rho, ux, uy, uz, n);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_convert_f(pfg_handle *h, float *f_out, const float *f_in) {
+  LBArgs a;
+  make_lb_args(h, &a);
+  const int64_t n = h->N;
+  float *buf = nullptr;
+  hipError_t e = hipMalloc(&buf, sizeof(float) * (size_t)n);
+  if (e != hipSuccess) return e;
+  dim3 block(256), grid((unsigned)((n + 255) / 256));
+  void *stored = h->f[h->cur];
+  for (int d = 0; d < Q && e == hipSuccess; ++d) {
+    const int to_float = f_out != nullptr;
+    if (!to_float) {
+      e = hipMemcpyAsync(buf, f_in + (int64_t)d * n, sizeof(float) * n,
+                         hipMemcpyHostToDevice, h->stream);
+      if (e != hipSuccess) break;
+    }
+    switch (h->p.store) {
+      case PFG_STORE_F16:
+        hipLaunchKernelGGL(convert_plane_kernel<PFG_STORE_F16>, grid, block, 0,
+                           h->stream, a, stored, buf, d, to_float, n);
+        break;
+      case PFG_STORE_BF16:
+        hipLaunchKernelGGL(convert_plane_kernel<PFG_STORE_BF16>, grid, block, 0,
+                           h->stream, a, stored, buf, d, to_float, n);
+        break;
+      default:
+        hipLaunchKernelGGL(convert_plane_kernel<PFG_STORE_F32>, grid, block, 0,
+                           h->stream, a, stored, buf, d, to_float, n);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) break;
+    if (to_float) {
+      e = hipMemcpyAsync(f_out + (int64_t)d * n, buf, sizeof(float) * n,
+                         hipMemcpyDeviceToHost, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  }
+  hipFree(buf);
+  return e;
+}
+
+hipError_t launch_conservation(pfg_handle *h, double out[4]) {
+  LBArgs a;
+  make_lb_args(h, &a);
+  const int blocks = (int)h->red_blocks;
+  switch (h->p.store) {
+    case PFG_STORE_F16:
+      hipLaunchKernelGGL(conservation_kernel<PFG_STORE_F16>, dim3(blocks), dim3(256),
+                         0, h->stream, a, h->f[h->cur], h->red, h->N);
+      break;
+    case PFG_STORE_BF16:
+      hipLaunchKernelGGL(conservation_kernel<PFG_STORE_BF16>, dim3(blocks), dim3(256),
+                         0, h->stream, a, h->f[h->cur], h->red, h->N);
+      break;
+    default:
+      hipLaunchKernelGGL(conservation_kernel<PFG_STORE_F32>, dim3(blocks), dim3(256),
+                         0, h->stream, a, h->f[h->cur], h->red, h->N);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  std::vector<double> part((size_t)blocks * 4);
+  e = hipMemcpyAsync(part.data(), h->red, sizeof(double) * part.size(),
+                     hipMemcpyDeviceToHost, h->stream);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return e;
+  for (int k = 0; k < 4; ++k) out[k] = 0.0;
+  for (int b = 0; b < blocks; ++b)
+    for (int k = 0; k < 4; ++k) out[k] += part[4 * (size_t)b + k];
+  return hipSuccess;
+}
+
+}  // namespace pfg

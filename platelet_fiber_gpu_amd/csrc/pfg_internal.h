@@ -1,0 +1,157 @@
+/*
+ * pfg_internal.h -- device-side state of a pfg_handle and the launchers each
+ * translation unit provides.  Not part of the C ABI.
+ */
+#ifndef PFG_INTERNAL_H
+#define PFG_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/pfg.h"
+#include "pfg_lattice.h"
+
+namespace pfg {
+
+/* Blocks of the persistent tile-clear kernel. */
+constexpr int CLEAR_BLOCKS = 512;
+
+/* Everything the fused LB kernel needs, passed by value. */
+struct LBArgs {
+  const void *fin;        /* post-collision state of the previous step      */
+  void *fout;             /* post-collision state of this step              */
+  float *aux;             /* [3][plane]: IB force in, IB velocity out       */
+  const uint8_t *tile_flag;
+  const uint8_t *solid;   /* nullptr when there are no solid cells          */
+  int nx, ny, nz;
+  int64_t plane;          /* element stride between direction planes         */
+  int tnx, tny;           /* tiles per axis                                   */
+  int periodic[3];
+  float omega;            /* 1/tau                                           */
+  float guo;              /* 1 - 1/(2 tau)                                   */
+  float g[3];             /* body force                                      */
+  float scale, inv_scale; /* FP16 deviation scale                            */
+  /* 6 w_d (c_d . u_wall) for [axis][0 lower, 1 upper][d] */
+  float wall_term[3][2][Q];
+};
+
+/* Marker-side device arrays. */
+struct Markers {
+  int n = 0;
+  float *x = nullptr, *y = nullptr, *z = nullptr;
+  float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  float *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  int32_t *kind = nullptr, *body = nullptr;
+  /* platelets */
+  int n_plt = 0;
+  int32_t *plt_elem_start = nullptr, *plt_tri_start = nullptr;
+  int n_tri = 0;
+  int32_t *tri = nullptr;        /* 3*n_tri */
+  double *tri_S0 = nullptr, *tri_S = nullptr;
+  float *tri_n = nullptr;        /* 3*n_tri unit normals (reference sign)   */
+  double *plt_S0 = nullptr, *plt_V0 = nullptr, *plt_S = nullptr, *plt_V = nullptr;
+  int32_t *plt_lowest = nullptr; /* lowest element per platelet             */
+  /* CSR marker -> incident triangles (value = triangle index) */
+  int32_t *m2t_start = nullptr, *m2t = nullptr;
+  /* CSR marker -> mesh-edge partners with rest length */
+  int n_edges = 0;
+  int32_t *m2e_start = nullptr, *m2e_nbr = nullptr;
+  float *m2e_l0 = nullptr;
+  /* receptors */
+  int n_rec = 0;
+  int32_t *rec_tri = nullptr;
+  float *rec_r1 = nullptr, *rec_r2 = nullptr;
+  int32_t *rec_bond = nullptr;   /* ligand id or -1                          */
+  int32_t *rec_cand = nullptr;   /* candidate ligand this step or -1         */
+  int32_t *rec_release = nullptr;/* 1 if the bond ruptures this step         */
+  float *rec_force = nullptr;    /* 3*n_rec force on the receptor            */
+  float *rec_w = nullptr;        /* 3*n_rec share of that force per vertex   */
+  int32_t *m2r_start = nullptr, *m2r = nullptr; /* marker -> (receptor*3+slot) */
+  /* ligands */
+  int n_wall_lig = 0;
+  int fibrin_lig = 0;
+  int n_lig = 0;                 /* n_wall_lig + n (fibrin node ids)         */
+  float *lig_x = nullptr, *lig_y = nullptr, *lig_z = nullptr;
+  int32_t *lig_owner = nullptr;  /* receptor or -1                           */
+  int32_t *lig_claim = nullptr;  /* lowest claiming receptor or INT_MAX      */
+  /* static ligands binned on the cell-list grid: lg_start[nbins+1] */
+  int32_t *lg_start = nullptr, *lg_items = nullptr;
+  /* fibrin links, CSR marker -> (partner, rest length) */
+  int n_links = 0;
+  int32_t *m2l_start = nullptr, *m2l_nbr = nullptr;
+  float *m2l_l0 = nullptr;
+  /* cell list */
+  int nb[3] = {1, 1, 1};
+  int64_t nbins = 0;
+  int32_t *bin_count = nullptr, *bin_start = nullptr, *bin_items = nullptr,
+          *marker_bin = nullptr, *marker_slot = nullptr;
+  int32_t *scan_partials = nullptr;
+  int64_t scan_blocks = 0;
+  /* counters: [0] total bonds, [1] detached, [2] near-threshold lo, [3] hi,
+   * [4] events lo, [5] events hi */
+  int64_t *counters = nullptr;
+  int64_t *detach_step = nullptr;
+  int64_t *d_step = nullptr;
+};
+
+}  // namespace pfg
+
+struct pfg_handle {
+  pfg_params p{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t N = 0, plane = 0;
+  int esize = 4;
+  void *f[2] = {nullptr, nullptr};
+  int cur = 0;
+  float *aux = nullptr;
+  uint8_t *tile_flag = nullptr;    /* one byte per tile, padded to 4 bytes     */
+  int32_t *tile_list = nullptr;    /* [0] count, then flagged tile ids         */
+  int32_t *tile_done = nullptr;    /* block counter of the clear kernel        */
+  int tnx = 0, tny = 0, tnz = 0;
+  int64_t ntiles = 0;
+  uint8_t *solid = nullptr;
+  double *red = nullptr;       /* reduction partials                        */
+  int64_t red_blocks = 0;
+  int lb_cpt = 1, lb_by = 4;
+  pfg::Markers m;
+  bool model_loaded = false;
+  bool fluid_only = false;
+  int64_t h_step = 0;
+  bool use_graph = true;
+  hipGraphExec_t graph_exec = nullptr;   /* step starting from f[0]         */
+  hipGraphExec_t graph_exec2 = nullptr;  /* step starting from f[1]         */
+  bool graph_valid = false;
+  size_t dev_bytes = 0;
+};
+
+namespace pfg {
+
+/* lb.hip */
+void make_lb_args(const pfg_handle *h, LBArgs *a);
+hipError_t launch_lb(pfg_handle *h);
+hipError_t launch_lb_fields(pfg_handle *h, float *rho, float *ux, float *uy,
+                            float *uz);
+hipError_t launch_init_eq(pfg_handle *h, const float *rho, const float *ux,
+                          const float *uy, const float *uz);
+hipError_t launch_convert_f(pfg_handle *h, float *f_out_or_null,
+                            const float *f_in_or_null);
+hipError_t launch_conservation(pfg_handle *h, double out[4]);
+
+/* ib.hip */
+hipError_t launch_spread(pfg_handle *h);
+hipError_t launch_interp(pfg_handle *h);
+hipError_t launch_clear(pfg_handle *h);
+
+/* markers.hip */
+hipError_t launch_cells(pfg_handle *h);
+hipError_t launch_membrane(pfg_handle *h);
+hipError_t launch_bonds(pfg_handle *h);
+hipError_t launch_forces(pfg_handle *h);
+hipError_t launch_update(pfg_handle *h);
+
+}  // namespace pfg
+
+#endif

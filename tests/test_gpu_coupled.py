@@ -1,0 +1,77 @@
+"""Whole coupled steps on the device against the FP64 oracle run in the same
+(fused) order from the same seeded scene."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+Q = 19
+
+
+def run_both(pfg, po, sc, steps, graph=True):
+    with pfg.Solver(sc.params) as s:
+        s.load(sc)
+        s.set_graph(graph)
+        info = s.step(steps)
+        f = s.f().astype(np.float64)
+        mk = s.markers()
+        rho, ux, uy, uz = s.fields()
+        bonds = s.bonds()[0] if s.n_rec else None
+    o = po.Oracle(sc.params, scene=sc)
+    o.step(steps)
+    return dict(info=info, f=f, mk=mk, u=(ux, uy, uz), bonds=bonds), o
+
+
+def rel_l2(a, b):
+    return np.sqrt(((a - b) ** 2).sum()) / max(np.sqrt((b ** 2).sum()), 1e-300)
+
+
+def test_channel_platelet_config(gpu, po):
+    """The CPU-runnable configuration in full: 64x32x32 channel, tau 0.8,
+    F = 1e-6, one 256-element Morse platelet, 4-point IB, 500 steps."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_CHANNEL)
+    g, o = run_both(pfg, po, sc, 500)
+    ofx = po.fields(po.lattice(sc.params), o.f())
+    err_u = rel_l2(np.stack(g["u"]), np.stack(ofx[1:]))
+    pos = np.stack([g["mk"][k] for k in ("x", "y", "z")])
+    opos = np.stack([o.marker(k) for k in ("x", "y", "z")])
+    err_x = np.abs(pos - opos).max()
+    print(f"channel config: rel L2 u {err_u:.2e}, max position error {err_x:.2e}")
+    assert err_u < 1e-4
+    assert err_x < 1e-3
+    # the platelet moved with the flow
+    assert pos[0].mean() > sc.x.mean()
+
+
+def test_shear_with_bonds_short(gpu, po):
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_SHEAR, nx=64, ny=32, nz=32, elements=256, receptors=128,
+                   ligands=800)
+    sc.params.k_on = 0.2
+    g, o = run_both(pfg, po, sc, 20)
+    ob = o.bonds()[0]
+    mism = int((g["bonds"] != ob).sum())
+    near = o.counters()["near_threshold"]
+    print(f"shear: bonds gpu {int((g['bonds'] >= 0).sum())} oracle {int((ob >= 0).sum())}, "
+          f"mismatches {mism}, near-threshold {near}")
+    assert mism <= near
+    pos = np.stack([g["mk"][k] for k in ("x", "y", "z")])
+    opos = np.stack([o.marker(k) for k in ("x", "y", "z")])
+    if mism == 0:
+        assert np.abs(pos - opos).max() < 1e-4
+
+
+def test_thrombus_small_flags_fibrin(gpu, po):
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=64, ny=48, nz=48, n_platelets=3, elements=150,
+                   receptors=16, ligands=400, fibrin_nodes=500, fibrin_links=800)
+    g, o = run_both(pfg, po, sc, 15)
+    fluid = sc.flags == 0
+    w = np.array([po.load().orc_w(d) for d in range(Q)])[:, None]
+    d = np.abs((g["f"] - w) - (o.f() - w))[:, fluid]
+    pos = np.stack([g["mk"][k] for k in ("x", "y", "z")])
+    opos = np.stack([o.marker(k) for k in ("x", "y", "z")])
+    print(f"thrombus: max f-dev error {d.max():.2e}, max position error "
+          f"{np.abs(pos - opos).max():.2e}")
+    assert d.max() < 1e-5
+    assert np.abs(pos - opos).max() < 1e-4
