@@ -107,7 +107,10 @@ typedef struct pfg_params {
   float marker_dt;             /* position update dt (reference dt/T0,
                                   sem_kernel.cu:2111); 1 in lattice units         */
   /* ---- SEM element mechanics ---- */
-  float morse_u0, morse_rho, morse_req, morse_cutoff;
+  /* Morse U = U0 (e^{-2 rho (r-req)} - 2 e^{-rho (r-req)}) between elements of
+   * one platelet, multiplied by a smoothstep that falls from 1 at
+   * cutoff - taper to 0 at cutoff (force continuous at the cutoff) */
+  float morse_u0, morse_rho, morse_req, morse_cutoff, morse_taper;
   float k_spring;              /* mesh-edge Hookean springs (reference k_spring [2],
                                   sem_kernel.cu:356-396)                          */
   float ks, kv;                /* area / volume constraints ([17], [18],

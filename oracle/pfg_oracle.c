@@ -815,12 +815,22 @@ static void stage_forces(orc_state *s) {
           const double r2 = dx * dx + dy * dy + dz * dz;
           if (r2 >= cut2 || r2 == 0) continue;
           const double r = sqrt(r2);
-          /* U = U0 (e^{-2 rho (r - re)} - 2 e^{-rho (r - re)}); F_i = -dU/dr (x_i - x_j)/r */
-          const double dUdr = U0 * (-2.0 * rho * exp(-2.0 * rho * (r - re)) +
-                                     2.0 * rho * exp(-rho * (r - re)));
-          FX[i] += dUdr * dx / r;
-          FY[i] += dUdr * dy / r;
-          FZ[i] += dUdr * dz / r;
+          /* U = U0 (e^{-2 rho (r - re)} - 2 e^{-rho (r - re)}) times the
+           * smoothstep S that falls to 0 over [rc - taper, rc];
+           * F_i = -d(U S)/dr (x_i - x_j)/r */
+          const double em = exp(-rho * (r - re));
+          const double U = U0 * (em * em - 2.0 * em);
+          double mdUdr = 2.0 * rho * U0 * (em - em * em); /* -dU/dr */
+          const double r1 = (double)p->morse_cutoff - (double)p->morse_taper;
+          if (p->morse_taper > 0 && r > r1) {
+            const double t = (r - r1) / (double)p->morse_taper;
+            const double S = 1.0 - t * t * (3.0 - 2.0 * t);
+            const double dS = 6.0 * t * (t - 1.0) / (double)p->morse_taper;
+            mdUdr = mdUdr * S + U * dS;
+          }
+          FX[i] += mdUdr * dx / r;
+          FY[i] += mdUdr * dy / r;
+          FZ[i] += mdUdr * dz / r;
         }
   }
   /* mesh-edge Hookean springs (sem_kernel.cu:356-396) */

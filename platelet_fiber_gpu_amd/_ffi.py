@@ -38,7 +38,7 @@ class Params(C.Structure):
         ("tau", C.c_float), ("body_force", C.c_float * 3), ("use_flags", C.c_int32),
         ("ib_force_scale", C.c_float), ("marker_dt", C.c_float),
         ("morse_u0", C.c_float), ("morse_rho", C.c_float), ("morse_req", C.c_float),
-        ("morse_cutoff", C.c_float), ("k_spring", C.c_float), ("ks", C.c_float),
+        ("morse_cutoff", C.c_float), ("morse_taper", C.c_float), ("k_spring", C.c_float), ("ks", C.c_float),
         ("kv", C.c_float), ("wall_rep_a", C.c_float), ("wall_rep_kappa", C.c_float),
         ("wall_rep_hc", C.c_float), ("wall_z", C.c_float),
         ("k_on", C.c_float), ("k_off0", C.c_float), ("bond_k", C.c_float),

@@ -177,6 +177,7 @@ void pfg_params_default(pfg_params *p) {
   p->morse_rho = 2.0f;
   p->morse_req = 0.8f;
   p->morse_cutoff = 2.5f;
+  p->morse_taper = 0.0f;
   p->wall_rep_kappa = 20.0f;
   p->wall_rep_hc = 0.5f;
   p->wall_z = -0.5f;

@@ -389,7 +389,10 @@ void base_params(pfg_params *p, int nx, int ny, int nz) {
   p->morse_u0 = 1.0f;
   p->morse_rho = 2.0f;
   p->morse_req = 0.8f;
+  /* nearest neighbours only (spacing ~0.75): a longer attractive tail
+   * crumples a membrane that has no bending rigidity */
   p->morse_cutoff = 1.2f;
+  p->morse_taper = 0.0f;
   p->bin_cells = 2;
   /* model force unit -> lattice force: keeps the explicit IB coupling of the
    * Morse stiffness 2 rho^2 U0 = 8 inside its stability range */

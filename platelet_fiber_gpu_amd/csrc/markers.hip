@@ -35,7 +35,7 @@ struct MArgs {
   int periodic[3];
   int nb[3];
   /* parameters */
-  float morse_u0, morse_rho, morse_req, morse_cut2;
+  float morse_u0, morse_rho, morse_req, morse_cut2, morse_cut, morse_taper;
   float k_spring, ks, kv;
   float wall_a, wall_kappa, wall_hc, wall_z;
   float k_on, k_off0, bond_k, bond_l0, bond_reach, bell_xb_kT, detach_rate;
@@ -509,8 +509,16 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
               if (r2 >= a.morse_cut2 || r2 == 0.f) continue;
               const float r = sqrtf(r2);
               const float e = expf(-a.morse_rho * (r - a.morse_req));
-              /* -dU/dr along (x_j - x_i)/r, U = U0 (e^2 - 2e) */
-              const float c = two_rho_u0 * (e - e * e) / r;
+              /* -d(U S)/dr along (x_j - x_i)/r, U = U0 (e^2 - 2e) */
+              float c = two_rho_u0 * (e - e * e);
+              const float r1 = a.morse_cut - a.morse_taper;
+              if (a.morse_taper > 0.f && r > r1) {
+                const float t = (r - r1) / a.morse_taper;
+                const float S = 1.0f - t * t * (3.0f - 2.0f * t);
+                const float dS = 6.0f * t * (t - 1.0f) / a.morse_taper;
+                c = c * S + a.morse_u0 * (e * e - 2.0f * e) * dS;
+              }
+              c /= r;
               fx += c * dx;
               fy += c * dy;
               fz += c * dz;
@@ -668,6 +676,8 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->morse_rho = p.morse_rho;
   a->morse_req = p.morse_req;
   a->morse_cut2 = p.morse_cutoff * p.morse_cutoff;
+  a->morse_cut = p.morse_cutoff;
+  a->morse_taper = p.morse_taper;
   a->k_spring = p.k_spring;
   a->ks = p.ks;
   a->kv = p.kv;

@@ -159,14 +159,14 @@ def test_morse_force_is_energy_gradient(pfg, po):
     n = 12
     X = (rng.random((n, 3)) * 2.0 + 5).astype(np.float32).astype(np.float64)
     p = pfg.default_params(nx=16, ny=16, nz=16, morse_u0=1.0, morse_rho=2.0, morse_req=0.8,
-                           morse_cutoff=1.5)
+                           morse_cutoff=1.5, morse_taper=0.4)
     ma = ModelArrays(X[:, 0], X[:, 1], X[:, 2], plt_elem_start=[0, n], plt_tri_start=[0, 0],
                      plt_S0=[1.0], plt_V0=[1.0])
     o = po.Oracle(p, model=ma.model)
     o.stage(pfg.STAGE_FORCES)
     F = np.stack([o.marker(k) for k in ("fx", "fy", "fz")], 1)
     # energy with a cut pair potential (pairs counted once)
-    rho, re, U0, rc = 2.0, np.float32(0.8), 1.0, np.float32(1.5)
+    rho, re, U0, rc, tp = 2.0, np.float32(0.8), 1.0, np.float32(1.5), np.float32(0.4)
 
     def E(Y):
         e = 0.0
@@ -174,7 +174,9 @@ def test_morse_force_is_energy_gradient(pfg, po):
             for j in range(i + 1, n):
                 r = np.linalg.norm(Y[i] - Y[j])
                 if r < rc:
-                    e += U0 * (np.exp(-2 * rho * (r - re)) - 2 * np.exp(-rho * (r - re)))
+                    t = max(0.0, (r - (rc - tp)) / tp)
+                    S = 1 - t * t * (3 - 2 * t)
+                    e += S * U0 * (np.exp(-2 * rho * (r - re)) - 2 * np.exp(-rho * (r - re)))
         return e
 
     h = 1e-6
