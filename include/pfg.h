@@ -290,8 +290,11 @@ pfg_status pfg_synchronize(pfg_handle *h);
  * launch, same stream).  Either pointer may be NULL. */
 pfg_status pfg_profile(pfg_handle *h, int nsteps, double *ms_total,
                        double *ms_lb);
-/* LB-kernel launch tuning: cells per thread along z and rows per block. */
-pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_thread, int block_y);
+/* LB-kernel launch tuning: cells per thread along z (1, 2, 4) and the block
+ * shape block_x (64, 128, 256 cells along x) by block_y (1, 2, 4 rows),
+ * block_x * block_y <= 256. */
+pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_thread, int block_x,
+                             int block_y);
 
 #ifdef __cplusplus
 }

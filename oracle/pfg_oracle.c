@@ -992,6 +992,7 @@ static void stage_interp(orc_state *s) {
 static void stage_update(orc_state *s) {
   const pfg_params *p = &s->p;
   const double dt = p->marker_dt;
+  s->step++; /* the step index advances with the marker update */
   for (int i = 0; i < s->n; ++i) {
     if (s->kind[i] == PFG_MARKER_FIXED) continue;
     if (s->kind[i] == PFG_MARKER_FIBER && p->fiber_move == PFG_FIBER_KICK) {
@@ -1012,7 +1013,6 @@ static void stage_update(orc_state *s) {
 
 static void stage_clear(orc_state *s) {
   memset(s->F, 0, sizeof(double) * 3 * (size_t)s->N);
-  s->step++;
 }
 
 void orc_stage(orc_state *s, int stage) {

@@ -321,8 +321,8 @@ class Solver:
         self._c("pfg_profile", int(n), C.byref(tot), C.byref(lb))
         return tot.value, lb.value
 
-    def set_lb_tuning(self, cells_per_thread, block_y):
-        self._c("pfg_set_lb_tuning", int(cells_per_thread), int(block_y))
+    def set_lb_tuning(self, cells_per_thread, block_x, block_y):
+        self._c("pfg_set_lb_tuning", int(cells_per_thread), int(block_x), int(block_y))
 
     def device_bytes(self):
         b = C.c_size_t()

@@ -83,7 +83,7 @@ void free_markers(Markers &m) {
   dfree(m.fx); dfree(m.fy); dfree(m.fz);
   dfree(m.kind); dfree(m.body);
   dfree(m.plt_elem_start); dfree(m.plt_tri_start);
-  dfree(m.tri); dfree(m.tri_S0); dfree(m.tri_S); dfree(m.tri_n);
+  dfree(m.tri); dfree(m.tri_S0); dfree(m.tri_S); dfree(m.tri_V); dfree(m.tri_n);
   dfree(m.plt_S0); dfree(m.plt_V0); dfree(m.plt_S); dfree(m.plt_V);
   dfree(m.plt_lowest);
   dfree(m.m2t_start); dfree(m.m2t);
@@ -267,12 +267,14 @@ pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
     const size_t flag_bytes = (size_t)(h->ntiles + 3) / 4 * 4;
     if (dalloc(h, &h->tile_flag, flag_bytes) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->tile_flag, 0, flag_bytes) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
-    if (dalloc(h, &h->tile_list, (size_t)h->ntiles + 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
-    if (hipMemset(h->tile_list, 0, sizeof(int32_t) * (h->ntiles + 1)) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
-    if (dalloc(h, &h->tile_done, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
-    if (hipMemset(h->tile_done, 0, sizeof(int32_t)) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     h->red_blocks = std::min<int64_t>(2048, (h->N + 255) / 256);
     if (dalloc(h, &h->red, (size_t)4 * h->red_blocks) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    /* LB launch shape measured fastest on MI355X at 256x128x128: whole 256-cell
+     * rows per block (a row's cache lines stay on one XCD), two z-cells per
+     * thread for 4-byte storage, one for 2-byte storage */
+    h->lb_bx = p.nx >= 256 ? 256 : (p.nx >= 128 ? 128 : 64);
+    h->lb_by = std::min(4, 256 / h->lb_bx);
+    h->lb_cpt = p.store == PFG_STORE_F32 ? 2 : 1;
     if (dalloc(h, &h->m.counters, 8) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->m.counters, 0, sizeof(int64_t) * 8) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     if (dalloc(h, &h->m.detach_step, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
@@ -286,6 +288,8 @@ pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
     if (p.use_flags) {
       if (dalloc(h, &h->solid, (size_t)h->N) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
       if (hipMemset(h->solid, 0, (size_t)h->N) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+      if (dalloc(h, &h->bbmask, (size_t)h->N) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+      if (hipMemset(h->bbmask, 0, sizeof(uint32_t) * h->N) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     }
     /* f = equilibrium at rest until the caller says otherwise */
     if (launch_init_eq(h, nullptr, nullptr, nullptr, nullptr) != hipSuccess ||
@@ -317,9 +321,8 @@ pfg_status pfg_destroy(pfg_handle *h) {
   }
   dfree(h->aux);
   dfree(h->tile_flag);
-  dfree(h->tile_list);
-  dfree(h->tile_done);
   dfree(h->solid);
+  dfree(h->bbmask);
   dfree(h->red);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -380,7 +383,11 @@ pfg_status pfg_upload_flags(pfg_handle *h, const uint8_t *flags) {
   if (!h || !flags) return PFG_ERR_ARG;
   if (!h->solid) return PFG_ERR_STATE;
   hipSetDevice(h->device);
-  return from_hip(upload(h, h->solid, flags, (size_t)h->N));
+  PFG_TRY(upload(h, h->solid, flags, (size_t)h->N));
+  drop_graph(h);
+  PFG_TRY(launch_bbmask(h));
+  PFG_TRY(hipStreamSynchronize(h->stream));
+  return PFG_OK;
 }
 
 pfg_status pfg_download_fields(pfg_handle *h, float *rho, float *ux, float *uy,
@@ -595,6 +602,7 @@ pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
   ALLOC_UP(m.tri, M.tri, 3 * (size_t)M.n_tri);
   ALLOC_UP(m.tri_S0, M.tri_S0, M.n_tri);
   ALLOC_FILL(m.tri_S, 0.0, M.n_tri);
+  ALLOC_FILL(m.tri_V, 0.0, M.n_tri);
   ALLOC_FILL(m.tri_n, 0.0f, 3 * (size_t)M.n_tri);
   ALLOC_UP(m.plt_S0, M.plt_S0, M.n_platelets);
   ALLOC_UP(m.plt_V0, M.plt_V0, M.n_platelets);
@@ -775,37 +783,53 @@ pfg_status pfg_download_aux(pfg_handle *h, float *a3n) {
 /* ---------------- stepping ---------------- */
 
 static hipError_t run_stage(pfg_handle *h, int s) {
+  hipError_t e = hipSuccess;
   switch (s) {
-    case PFG_STAGE_CELLS: return launch_cells(h);
-    case PFG_STAGE_MEMBRANE: return launch_membrane(h);
-    case PFG_STAGE_BONDS: return launch_bonds(h);
+    case PFG_STAGE_CELLS:
+      e = launch_begin(h, false, true);
+      if (e == hipSuccess) e = launch_scan(h);
+      if (e == hipSuccess) e = launch_place(h, true, false);
+      if (e == hipSuccess) e = launch_gather(h, true, false, false);
+      return e;
+    case PFG_STAGE_MEMBRANE:
+      e = launch_place(h, false, true);
+      if (e == hipSuccess) e = launch_gather(h, false, true, false);
+      return e;
+    case PFG_STAGE_BONDS:
+      e = launch_gather(h, false, false, true);
+      if (e == hipSuccess) e = launch_resolve(h);
+      return e;
     case PFG_STAGE_FORCES: return launch_forces(h);
     case PFG_STAGE_SPREAD: return launch_spread(h);
     case PFG_STAGE_LB: return launch_lb(h);
-    case PFG_STAGE_INTERP: return launch_interp(h);
-    case PFG_STAGE_UPDATE: return launch_update(h);
-    case PFG_STAGE_CLEAR: return launch_clear(h);
+    case PFG_STAGE_INTERP: return launch_move(h, true, false);
+    case PFG_STAGE_UPDATE: return launch_move(h, false, true);
+    case PFG_STAGE_CLEAR: return launch_begin(h, true, false);
   }
   return hipErrorInvalidValue;
 }
 
 /* One coupled step, in the order of the reference's per-step calls
  * (gpuSEM_LB.c:1441-1442 -> semLB.cu:441-531 then semLB.cu:303-438):
- * forces from X(n), spread, LB with F(n), interpolate, move. */
+ * forces from X(n), spread, LB with F(n), interpolate, move.  The stages are
+ * fused into nine launches; the previous step's force tiles are cleared by
+ * the first of them. */
 static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
   hipError_t e = hipSuccess;
   const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
   if (markers) {
-    for (int s = PFG_STAGE_CELLS; s <= PFG_STAGE_SPREAD && e == hipSuccess; ++s)
-      e = run_stage(h, s);
+    e = launch_begin(h, true, true);
+    if (e == hipSuccess) e = launch_scan(h);
+    if (e == hipSuccess) e = launch_place(h, true, true);
+    if (e == hipSuccess) e = launch_gather(h, true, true, true);
+    if (e == hipSuccess) e = launch_resolve(h);
+    if (e == hipSuccess) e = launch_forces(h);
+    if (e == hipSuccess) e = launch_spread(h);
   }
   if (e == hipSuccess && lb0) e = hipEventRecord(lb0, h->stream);
   if (e == hipSuccess) e = launch_lb(h);
   if (e == hipSuccess && lb1) e = hipEventRecord(lb1, h->stream);
-  if (markers) {
-    for (int s = PFG_STAGE_INTERP; s <= PFG_STAGE_CLEAR && e == hipSuccess; ++s)
-      e = run_stage(h, s);
-  }
+  if (markers && e == hipSuccess) e = launch_move(h, true, true);
   return e;
 }
 
@@ -939,12 +963,16 @@ pfg_status pfg_profile(pfg_handle *h, int nsteps, double *ms_total, double *ms_l
   return PFG_OK;
 }
 
-pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_thread, int block_y) {
+pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_thread, int block_x,
+                             int block_y) {
   if (!h) return PFG_ERR_ARG;
   if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4)
     return PFG_ERR_ARG;
+  if (block_x != 64 && block_x != 128 && block_x != 256) return PFG_ERR_ARG;
   if (block_y != 1 && block_y != 2 && block_y != 4) return PFG_ERR_ARG;
+  if (block_x * block_y > 256) return PFG_ERR_ARG;
   h->lb_cpt = cells_per_thread;
+  h->lb_bx = block_x;
   h->lb_by = block_y;
   drop_graph(h);
   return PFG_OK;

@@ -85,55 +85,68 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho,
   using T = typename ST::T;
   const T *__restrict__ fin = (const T *)a.fin;
   T *__restrict__ fout = (T *)a.fout;
-  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   if (x >= a.nx || y >= a.ny) return;
   const int64_t nxy = (int64_t)a.nx * a.ny;
+
+  /* upstream x and y (wrapped or flagged as outside a wall), shared by the
+   * CPT cells of this thread */
+  const bool x0 = x == 0, x1 = x == a.nx - 1, y0 = y == 0, y1 = y == a.ny - 1;
+  const int xm = x0 ? (a.periodic[0] ? a.nx - 1 : 0) : x - 1;
+  const int xp = x1 ? (a.periodic[0] ? 0 : x) : x + 1;
+  const bool oxm = x0 && !a.periodic[0], oxp = x1 && !a.periodic[0];
+  const int64_t ym = (int64_t)a.nx * (y0 ? (a.periodic[1] ? a.ny - 1 : 0) : y - 1);
+  const int64_t yp = (int64_t)a.nx * (y1 ? (a.periodic[1] ? 0 : y) : y + 1);
+  const int64_t yc = (int64_t)a.nx * y;
+  const bool oym = y0 && !a.periodic[1], oyp = y1 && !a.periodic[1];
 
 #pragma unroll
   for (int c = 0; c < CPT; ++c) {
     const int z = blockIdx.z * CPT + c;
     if (z >= a.nz) return;
-    const int64_t idx = x + (int64_t)a.nx * y + nxy * z;
-    if (a.solid != nullptr && a.solid[idx]) continue;
+    const int64_t idx = x + yc + nxy * z;
+    /* bit d: the upstream cell of direction d is solid; bit 31: this cell is */
+    uint32_t mask = 0;
+    if (a.bbmask != nullptr) {
+      mask = a.bbmask[idx];
+      if (mask >> 31) continue;
+    }
+    const bool z0 = z == 0, z1 = z == a.nz - 1;
+    const int64_t zm = nxy * (z0 ? (a.periodic[2] ? a.nz - 1 : 0) : z - 1);
+    const int64_t zp = nxy * (z1 ? (a.periodic[2] ? 0 : z) : z + 1);
+    const int64_t zc = nxy * z;
+    const bool ozm = z0 && !a.periodic[2], ozp = z1 && !a.periodic[2];
 
+    /* 19 upstream loads, each from its plane's uniform base plus a 32-bit cell
+     * offset; outside a wall the offset falls back to this cell (a valid
+     * address) and the edge fix-up below replaces the value */
     float dev[Q];
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-      int sx = x - CX[d], sy = y - CY[d], sz = z - CZ[d];
-      int wall = -1, side = 0;
-      if (CX[d] != 0) {
-        if (sx < 0) {
-          if (a.periodic[0]) sx += a.nx; else { wall = 0; side = 0; }
-        } else if (sx >= a.nx) {
-          if (a.periodic[0]) sx -= a.nx; else { wall = 0; side = 1; }
-        }
-      }
-      if (CY[d] != 0) {
-        if (sy < 0) {
-          if (a.periodic[1]) sy += a.ny; else { wall = 1; side = 0; }
-        } else if (sy >= a.ny) {
-          if (a.periodic[1]) sy -= a.ny; else { wall = 1; side = 1; }
-        }
-      }
-      if (CZ[d] != 0) {
-        if (sz < 0) {
-          if (a.periodic[2]) sz += a.nz; else { wall = 2; side = 0; }
-        } else if (sz >= a.nz) {
-          if (a.periodic[2]) sz -= a.nz; else { wall = 2; side = 1; }
-        }
-      }
-      if (d == 0) {
-        dev[d] = ST::load(fin, idx, a.inv_scale);
-      } else if (wall >= 0) {
-        dev[d] = ST::load(fin + (int64_t)BB[d] * a.plane, idx, a.inv_scale) +
-                 a.wall_term[wall][side][d];
-      } else {
-        const int64_t sidx = sx + (int64_t)a.nx * sy + nxy * sz;
-        if (a.solid != nullptr && a.solid[sidx]) {
-          dev[d] = ST::load(fin + (int64_t)BB[d] * a.plane, idx, a.inv_scale);
-        } else {
-          dev[d] = ST::load(fin + (int64_t)d * a.plane, sidx, a.inv_scale);
+      const int sx = CX[d] > 0 ? xm : (CX[d] < 0 ? xp : x);
+      const int64_t sy = CY[d] > 0 ? ym : (CY[d] < 0 ? yp : yc);
+      const int64_t sz = CZ[d] > 0 ? zm : (CZ[d] < 0 ? zp : zc);
+      const uint32_t src = (uint32_t)(sx + sy + sz);
+      dev[d] = ST::load(fin + (int64_t)d * a.plane, src, a.inv_scale);
+    }
+    /* halfway bounce-back where the upstream cell is a wall or solid: the
+     * population leaving this cell towards it returns, plus the moving-wall
+     * momentum of that wall (z over y over x at edges) */
+    const bool edge = oxm || oxp || oym || oyp || ozm || ozp || mask != 0;
+    if (edge) {
+#pragma unroll
+      for (int d = 1; d < Q; ++d) {
+        const bool ox = CX[d] > 0 ? oxm : (CX[d] < 0 ? oxp : false);
+        const bool oy = CY[d] > 0 ? oym : (CY[d] < 0 ? oyp : false);
+        const bool oz = CZ[d] > 0 ? ozm : (CZ[d] < 0 ? ozp : false);
+        if (ox || oy || oz || ((mask >> d) & 1u)) {
+          const int sxs = CX[d] > 0 ? 0 : 1, sys = CY[d] > 0 ? 0 : 1,
+                    szs = CZ[d] > 0 ? 0 : 1;
+          const float term = oz ? a.wall_term[2][szs][d]
+                                : (oy ? a.wall_term[1][sys][d]
+                                      : (ox ? a.wall_term[0][sxs][d] : 0.f));
+          dev[d] = ST::load(fin + (int64_t)BB[d] * a.plane, idx, a.inv_scale) + term;
         }
       }
     }
@@ -198,6 +211,34 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho,
 }
 
 /* ---------------- auxiliary kernels ---------------- */
+
+/* Per-cell bounce mask from the solid flags (built once per flag upload). */
+__global__ void bbmask_kernel(LBArgs a, const uint8_t *solid, uint32_t *mask) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nxy = (int64_t)a.nx * a.ny;
+  if (idx >= nxy * a.nz) return;
+  if (solid[idx]) {
+    mask[idx] = 1u << 31;
+    return;
+  }
+  const int x = (int)(idx % a.nx), y = (int)((idx / a.nx) % a.ny), z = (int)(idx / nxy);
+  uint32_t m = 0;
+#pragma unroll
+  for (int d = 1; d < Q; ++d) {
+    int s[3] = {x - CX[d], y - CY[d], z - CZ[d]};
+    const int n[3] = {a.nx, a.ny, a.nz};
+    bool outside = false;
+    for (int k = 0; k < 3; ++k) {
+      if (s[k] < 0) {
+        if (a.periodic[k]) s[k] += n[k]; else outside = true;
+      } else if (s[k] >= n[k]) {
+        if (a.periodic[k]) s[k] -= n[k]; else outside = true;
+      }
+    }
+    if (!outside && solid[s[0] + (int64_t)a.nx * s[1] + nxy * s[2]]) m |= 1u << d;
+  }
+  mask[idx] = m;
+}
 
 template <int S>
 __global__ void init_eq_kernel(LBArgs a, void *dst0, void *dst1,
@@ -295,6 +336,7 @@ void make_lb_args(const pfg_handle *h, LBArgs *a) {
   a->aux = h->aux;
   a->tile_flag = h->tile_flag;
   a->solid = h->solid;
+  a->bbmask = h->bbmask;
   a->nx = p.nx;
   a->ny = p.ny;
   a->nz = p.nz;
@@ -326,9 +368,9 @@ void make_lb_args(const pfg_handle *h, LBArgs *a) {
 template <int S, int CPT>
 static hipError_t launch_lb_t(pfg_handle *h, const LBArgs &a, int mode,
                               float *rho, float *ux, float *uy, float *uz) {
-  const int by = h->lb_by;
-  dim3 block(64, by, 1);
-  dim3 grid((a.nx + 63) / 64, (a.ny + by - 1) / by, (a.nz + CPT - 1) / CPT);
+  const int bx = h->lb_bx, by = h->lb_by;
+  dim3 block(bx, by, 1);
+  dim3 grid((a.nx + bx - 1) / bx, (a.ny + by - 1) / by, (a.nz + CPT - 1) / CPT);
   if (mode == LB_MODE_STEP)
     hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_STEP>), grid, block, 0,
                        h->stream, a, nullptr, nullptr, nullptr, nullptr);
@@ -368,6 +410,15 @@ hipError_t launch_lb(pfg_handle *h) {
 hipError_t launch_lb_fields(pfg_handle *h, float *rho, float *ux, float *uy,
                             float *uz) {
   return dispatch_lb(h, LB_MODE_FIELDS, rho, ux, uy, uz);
+}
+
+hipError_t launch_bbmask(pfg_handle *h) {
+  if (!h->solid || !h->bbmask) return hipSuccess;
+  LBArgs a;
+  make_lb_args(h, &a);
+  hipLaunchKernelGGL(bbmask_kernel, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0,
+                     h->stream, a, h->solid, h->bbmask);
+  return hipGetLastError();
 }
 
 hipError_t launch_init_eq(pfg_handle *h, const float *rho, const float *ux,

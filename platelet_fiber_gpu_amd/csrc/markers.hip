@@ -1,23 +1,39 @@
 /*
- * markers.hip -- marker-side stages of the coupled step.
+ * markers.hip -- marker-side kernels of the coupled step and the immersed-
+ * boundary coupling.
  *
  *  CELLS     integer cell lists: bin per marker, counting sort with a
  *            wavefront prefix scan, ascending marker order inside each bin.
- *  MEMBRANE  per-platelet triangle areas/normals and FP64 area/volume sums
+ *  MEMBRANE  per-triangle area/normal/volume terms and per-platelet FP64 sums
  *            (sem_calculate_A_kernel, sem_kernel.cu:724-872) plus the lowest
  *            element for the wall repulsion (sem_kernel.cu:409-430).
  *  BONDS     Bell-model receptor-ligand rupture/formation with the counter RNG
  *            (sem_platelet_wall_kernel, sem_kernel.cu:1750-2083).
- *  FORCES    one gather per marker: Morse over the cell list, mesh-edge springs
- *            (sem_Force_kernel, sem_kernel.cu:320-513), area/volume constraints
+ *  FORCES    one wavefront per marker gathers Morse pairs over the 27
+ *            neighbouring bins, mesh-edge springs (sem_Force_kernel,
+ *            sem_kernel.cu:320-513), area/volume constraints
  *            (sem_ForceAreaVol_kernel, sem_kernel.cu:1640-1749), wall repulsion,
- *            receptor bond forces, fibrin link forces (fiber.cpp:647-733,
- *            network.cpp:303-437).
- *  UPDATE    x += V dt (sem_update_position_kernel, sem_kernel.cu:2087-2148) or
- *            the damped kick of network.cpp:839-881 for fibrin nodes.
+ *            receptor bond forces and fibrin links (fiber.cpp:647-733,
+ *            network.cpp:303-437), then reduces across the wavefront.
+ *  SPREAD    IB force spreading (fluid3d_force_distribute_kernel,
+ *            LB_kernel.cu:1419-1520): one lane per (marker, stencil node),
+ *            hardware FP32 atomics into the lattice force field, tile flags.
+ *  INTERP/UPDATE  IB interpolation (fluid3d_velocity_distribute_kernel,
+ *            LB_kernel.cu:1524-1642) as a wavefront gather over the 4x4x4
+ *            stencil, then x += V dt (sem_update_position_kernel,
+ *            sem_kernel.cu:2087-2148) or the damped kick of network.cpp:839-881.
+ *  CLEAR     zero the force/velocity field in the tiles flagged this step.
  *
- * Every per-marker force is a gather in a fixed order (cell-list order, CSR
- * order), so forces are deterministic run to run and need no atomics.
+ * The stages of one step are fused into few launches (a launch costs several
+ * microseconds, more than most of these kernels' work at one platelet): each
+ * fused kernel takes a role mask and splits its grid into role ranges.
+ *
+ * The 4-point cosine kernel is phi(r) = (1 + cos(pi r / 2)) / 4 for |r| < 2
+ * (LB_kernel.cu:1477-1478), applied as a product over the three axes.
+ *
+ * Every force is a gather in a fixed order and every cross-lane reduction is a
+ * fixed butterfly, so forces and velocities are run-to-run deterministic; only
+ * the spread force field sums FP32 atomics in arrival order.
  */
 #include <hip/hip_runtime.h>
 
@@ -31,6 +47,11 @@ namespace pfg {
 
 struct MArgs {
   Markers m;
+  /* lattice side of the IB coupling */
+  float *aux;
+  int64_t plane;
+  uint8_t *tile_flag;
+  int tnx, tny, tnz;
   int nx, ny, nz;
   int periodic[3];
   int nb[3];
@@ -43,11 +64,20 @@ struct MArgs {
   int fiber_law, fiber_move, fiber_ref_sign;
   float fiber_k, fiber_xmax, fiber_gamma;
   float marker_dt;
-  int bin_cells;
+  float force_scale;
+  /* role split of the fused launches */
+  int roles;
+  int blocks0, blocks1, blocks2; /* blocks of role ranges 0, 1, 2 */
 };
 
-constexpr float BOND_EPS_P = 1e-6f;   /* |u - p| below this: near-threshold  */
-constexpr float BOND_EPS_R = 1e-4f;   /* |r - reach| below this: near-threshold */
+enum : int {
+  ROLE_A = 1, /* first role of a fused kernel  */
+  ROLE_B = 2, /* second role                   */
+  ROLE_C = 4  /* third role                    */
+};
+
+constexpr float BOND_EPS_P = 1e-6f; /* |u - p| below this: near-threshold      */
+constexpr float BOND_EPS_R = 1e-4f; /* |r - reach| below this: near-threshold  */
 
 /* ---------------- small helpers ---------------- */
 
@@ -62,17 +92,12 @@ __device__ inline int cell_of(float v, int n, int periodic) {
   return c;
 }
 
-__device__ inline int bin_of(const MArgs &a, float x, float y, float z) {
-  const int cx = cell_of(x, a.nx, a.periodic[0]);
-  const int cy = cell_of(y, a.ny, a.periodic[1]);
-  const int cz = cell_of(z, a.nz, a.periodic[2]);
-  const int bx = cx * a.nb[0] / a.nx;
-  const int by = cy * a.nb[1] / a.ny;
-  const int bz = cz * a.nb[2] / a.nz;
-  return bx + a.nb[0] * (by + a.nb[1] * bz);
+__device__ inline void bin3_of(const MArgs &a, float x, float y, float z, int b[3]) {
+  b[0] = cell_of(x, a.nx, a.periodic[0]) * a.nb[0] / a.nx;
+  b[1] = cell_of(y, a.ny, a.periodic[1]) * a.nb[1] / a.ny;
+  b[2] = cell_of(z, a.nz, a.periodic[2]) * a.nb[2] / a.nz;
 }
 
-/* Minimum-image difference on periodic axes. */
 __device__ inline float min_image(float d, int n, int periodic) {
   if (!periodic) return d;
   const float fn = (float)n;
@@ -99,6 +124,23 @@ __device__ inline int nbr_bins(int b, int nb, int periodic, int out[3]) {
     out[c++] = v;
   }
   return c;
+}
+
+/* The k-th of the (up to 27) bins around bin (b3); -1 when k is past the end. */
+__device__ inline int nth_nbr_bin(const MArgs &a, const int b3[3], int k) {
+  int bx[3], by[3], bz[3];
+  const int nx = nbr_bins(b3[0], a.nb[0], a.periodic[0], bx);
+  const int ny = nbr_bins(b3[1], a.nb[1], a.periodic[1], by);
+  const int nz = nbr_bins(b3[2], a.nb[2], a.periodic[2], bz);
+  if (k >= nx * ny * nz) return -1;
+  const int jx = k % nx, jy = (k / nx) % ny, jz = k / (nx * ny);
+  return bx[jx] + a.nb[0] * (by[jy] + a.nb[1] * bz[jz]);
+}
+
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
 __device__ inline int wave_incl_scan(int v) {
@@ -132,22 +174,98 @@ __device__ inline int block_excl_scan4(int v[4], int *sh_wave) {
   return total;
 }
 
-/* ---------------- CELLS ---------------- */
+__device__ inline void bump(int64_t *c, int k) {
+  atomicAdd((unsigned long long *)(c + k), 1ull);
+}
 
-__global__ void bins_kernel(MArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    /* start of the step: reset the bond counter read by the detach test */
-    a.m.counters[0] = 0;
+__device__ inline bool ib_coupled(int kind, int fiber_move) {
+  return kind == PFG_MARKER_ELEMENT || (kind == PFG_MARKER_FIBER && fiber_move == PFG_FIBER_IB);
+}
+
+__device__ inline float wrap_coord(float v, int n, int periodic) {
+  if (!periodic) return v;
+  const float fn = (float)n;
+  float w = v - fn * floorf(v / fn);
+  if (w >= fn) w -= fn;
+  if (w < 0.f) w += fn;
+  return w;
+}
+
+/* Lattice index along an axis, or -1 when outside a wall-bounded axis. */
+__device__ inline int axis_index(int i, int n, int periodic) {
+  if (periodic) {
+    i %= n;
+    return i < 0 ? i + n : i;
   }
+  return (i < 0 || i >= n) ? -1 : i;
+}
+
+/* Stencil node k (0..3) along one axis for wrapped coordinate X: lattice index
+ * (or -1) and weight phi(X - i). */
+__device__ inline int stencil_node(float X, int k, int n, int periodic, float *w) {
+  const float fl = floorf(X);
+  const float r = X - fl + 1.0f - (float)k; /* X - (fl - 1 + k) */
+  *w = 0.25f * (1.0f + cospif(0.5f * r));
+  return axis_index((int)fl - 1 + k, n, periodic);
+}
+
+/* ---------------- K1: tile clear + bin counting ---------------- */
+
+/* Block `block` scans 256 tile flags; the flagged tiles (16x4x4 = 256 cells
+ * each) are zeroed by the whole block and unflagged. */
+__device__ void clear_tiles(const MArgs &a, int block) {
+  __shared__ int list[256];
+  __shared__ int count;
+  if (threadIdx.x == 0) count = 0;
+  __syncthreads();
+  const int64_t ntiles = (int64_t)a.tnx * a.tny * a.tnz;
+  const int64_t my = (int64_t)block * 256 + threadIdx.x;
+  if (my < ntiles && a.tile_flag[my]) list[atomicAdd(&count, 1)] = (int)my;
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int lx = t & (TILE_X - 1);
+  const int ly = (t >> TILE_SHIFT_X) & (TILE_Y - 1);
+  const int lz = t >> (TILE_SHIFT_X + TILE_SHIFT_Y);
+  for (int k = 0; k < count; ++k) {
+    const int tile = list[k];
+    const int tx = tile % a.tnx;
+    const int ty = (tile / a.tnx) % a.tny;
+    const int tz = tile / (a.tnx * a.tny);
+    const int x = tx * TILE_X + lx, y = ty * TILE_Y + ly, z = tz * TILE_Z + lz;
+    if (x < a.nx && y < a.ny && z < a.nz) {
+      const int64_t c = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
+      a.aux[c] = 0.f;
+      a.aux[a.plane + c] = 0.f;
+      a.aux[2 * a.plane + c] = 0.f;
+    }
+  }
+  if (my < ntiles && a.tile_flag[my]) a.tile_flag[my] = 0;
+}
+
+/* roles: A = clear last step's tiles (blocks0 blocks), B = count bins */
+__global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
+  int blk = blockIdx.x;
+  if (a.roles & ROLE_A) {
+    if (blk < a.blocks0) {
+      clear_tiles(a, blk);
+      return;
+    }
+    blk -= a.blocks0;
+  }
+  if (!(a.roles & ROLE_B)) return;
+  const int i = blk * 256 + threadIdx.x;
+  if (i == 0) a.m.counters[0] = 0; /* bonds held, counted again this step */
   if (i >= a.m.n) return;
-  const int b = bin_of(a, a.m.x[i], a.m.y[i], a.m.z[i]);
+  int b3[3];
+  bin3_of(a, a.m.x[i], a.m.y[i], a.m.z[i], b3);
+  const int b = b3[0] + a.nb[0] * (b3[1] + a.nb[1] * b3[2]);
   a.m.marker_bin[i] = b;
   a.m.marker_slot[i] = atomicAdd(&a.m.bin_count[b], 1);
 }
 
-__global__ __launch_bounds__(256) void scan_reduce_kernel(const int32_t *count,
-                                                          int64_t n,
+/* ---------------- K2..K4: prefix scan of the bin counts ---------------- */
+
+__global__ __launch_bounds__(256) void scan_reduce_kernel(const int32_t *count, int64_t n,
                                                           int32_t *partials) {
   __shared__ int sh[4];
   const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x * 4;
@@ -157,11 +275,8 @@ __global__ __launch_bounds__(256) void scan_reduce_kernel(const int32_t *count,
   if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-/* Exclusive scan of the block totals in one block. */
-__global__ __launch_bounds__(256) void scan_top_kernel(int32_t *partials,
-                                                       int64_t nblocks,
-                                                       int32_t *bin_start,
-                                                       int64_t nbins) {
+__global__ __launch_bounds__(256) void scan_top_kernel(int32_t *partials, int64_t nblocks,
+                                                       int32_t *bin_start, int64_t nbins) {
   __shared__ int sh[4];
   int carry = 0;
   for (int64_t off = 0; off < nblocks; off += 1024) {
@@ -194,70 +309,68 @@ __global__ __launch_bounds__(256) void scan_apply_kernel(int32_t *count, int64_t
   }
 }
 
-__global__ void scatter_kernel(MArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.m.n) return;
-  const int b = a.m.marker_bin[i];
-  a.m.bin_items[a.m.bin_start[b] + a.m.marker_slot[i]] = i;
+/* ---------------- K5: scatter + triangle geometry ---------------- */
+
+__device__ void triangle_geometry(const MArgs &a, int t) {
+  const int n0 = a.m.tri[3 * t], n1 = a.m.tri[3 * t + 1], n2 = a.m.tri[3 * t + 2];
+  const double x0 = a.m.x[n0], y0 = a.m.y[n0], z0 = a.m.z[n0];
+  const double x1 = a.m.x[n1], y1 = a.m.y[n1], z1 = a.m.z[n1];
+  const double x2 = a.m.x[n2], y2 = a.m.y[n2], z2 = a.m.z[n2];
+  const double ax = x1 - x0, ay = y1 - y0, az = z1 - z0;
+  const double bx = x2 - x0, by = y2 - y0, bz = z2 - z0;
+  double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const double nn = sqrt(nx * nx + ny * ny + nz * nz);
+  /* n_elem = -(e12 x e13)/|e12 x e13| (sem_kernel.cu:799-804) */
+  nx /= -nn;
+  ny /= -nn;
+  nz /= -nn;
+  const double St = 0.5 * nn;
+  const double cx = (x0 + x1 + x2) / 3.0, cy = (y0 + y1 + y2) / 3.0, cz = (z0 + z1 + z2) / 3.0;
+  a.m.tri_S[t] = St;
+  a.m.tri_V[t] = (cx * nx + cy * ny + cz * nz) * St / 3.0; /* sem_kernel.cu:806-808 */
+  a.m.tri_n[3 * t] = (float)nx;
+  a.m.tri_n[3 * t + 1] = (float)ny;
+  a.m.tri_n[3 * t + 2] = (float)nz;
 }
 
-/* Atomic slots arrive in any order: sort each bin by marker index. */
-__global__ void sort_bins_kernel(MArgs a) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= a.m.nbins) return;
-  const int s = a.m.bin_start[b], e = a.m.bin_start[b + 1];
-  if (e - s < 2) return;
-  int32_t *it = a.m.bin_items;
-  for (int i = s + 1; i < e; ++i) {
-    const int v = it[i];
-    int j = i - 1;
-    while (j >= s && it[j] > v) {
-      it[j + 1] = it[j];
-      --j;
+/* roles: A = scatter markers into bins (blocks0), B = triangle terms (blocks1) */
+__global__ __launch_bounds__(256) void place_kernel(MArgs a) {
+  int blk = blockIdx.x;
+  if (a.roles & ROLE_A) {
+    if (blk < a.blocks0) {
+      const int i = blk * 256 + threadIdx.x;
+      if (i < a.m.n) {
+        const int b = a.m.marker_bin[i];
+        a.m.bin_items[a.m.bin_start[b] + a.m.marker_slot[i]] = i;
+      }
+      return;
     }
-    it[j + 1] = v;
+    blk -= a.blocks0;
+  }
+  if (a.roles & ROLE_B) {
+    const int t = blk * 256 + threadIdx.x;
+    if (t < a.m.n_tri) triangle_geometry(a, t);
   }
 }
 
-/* ---------------- MEMBRANE ---------------- */
+/* ---------------- K6: sort bins + platelet sums + bond decisions ---------------- */
 
-__global__ __launch_bounds__(256) void membrane_kernel(MArgs a) {
-  const int p = blockIdx.x;
+__device__ void platelet_sums(const MArgs &a, int p) {
   const int t0 = a.m.plt_tri_start[p], t1 = a.m.plt_tri_start[p + 1];
   double S = 0.0, V = 0.0;
-  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-    const int n0 = a.m.tri[3 * t], n1 = a.m.tri[3 * t + 1], n2 = a.m.tri[3 * t + 2];
-    const double x0 = a.m.x[n0], y0 = a.m.y[n0], z0 = a.m.z[n0];
-    const double x1 = a.m.x[n1], y1 = a.m.y[n1], z1 = a.m.z[n1];
-    const double x2 = a.m.x[n2], y2 = a.m.y[n2], z2 = a.m.z[n2];
-    const double ax = x1 - x0, ay = y1 - y0, az = z1 - z0;
-    const double bx = x2 - x0, by = y2 - y0, bz = z2 - z0;
-    double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-    const double nn = sqrt(nx * nx + ny * ny + nz * nz);
-    /* n_elem = -(e12 x e13)/|e12 x e13| (sem_kernel.cu:799-804) */
-    nx /= -nn;
-    ny /= -nn;
-    nz /= -nn;
-    const double St = 0.5 * nn;
-    const double cx = (x0 + x1 + x2) / 3.0, cy = (y0 + y1 + y2) / 3.0,
-                 cz = (z0 + z1 + z2) / 3.0;
-    a.m.tri_S[t] = St;
-    a.m.tri_n[3 * t] = (float)nx;
-    a.m.tri_n[3 * t + 1] = (float)ny;
-    a.m.tri_n[3 * t + 2] = (float)nz;
-    S += St;
-    V += (cx * nx + cy * ny + cz * nz) * St / 3.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += 256) {
+    S += a.m.tri_S[t];
+    V += a.m.tri_V[t];
   }
   __shared__ double shS[256], shV[256];
   __shared__ float shz[256];
   __shared__ int shi[256];
   shS[threadIdx.x] = S;
   shV[threadIdx.x] = V;
-  /* lowest element: minimum z, lowest index on ties */
   const int e0 = a.m.plt_elem_start[p], e1 = a.m.plt_elem_start[p + 1];
   float zmin = INFINITY;
   int imin = INT_MAX;
-  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+  for (int e = e0 + threadIdx.x; e < e1; e += 256) {
     const float z = a.m.z[e];
     if (z < zmin || (z == zmin && e < imin)) {
       zmin = z;
@@ -287,10 +400,7 @@ __global__ __launch_bounds__(256) void membrane_kernel(MArgs a) {
   }
 }
 
-/* ---------------- BONDS ---------------- */
-
-__device__ inline void receptor_pos(const MArgs &a, int r, float out[3],
-                                    int nodes[3]) {
+__device__ inline void receptor_pos(const MArgs &a, int r, float out[3], int nodes[3]) {
   const int t = a.m.rec_tri[r];
   const float r1 = a.m.rec_r1[r], r2 = a.m.rec_r2[r];
   nodes[0] = a.m.tri[3 * t];
@@ -303,9 +413,7 @@ __device__ inline void receptor_pos(const MArgs &a, int r, float out[3],
   out[2] = c0 * a.m.z[nodes[0]] + c1 * a.m.z[nodes[1]] + c2 * a.m.z[nodes[2]];
 }
 
-/* Vector from the receptor at rp to ligand l. */
-__device__ inline void ligand_vec(const MArgs &a, int l, const float rp[3],
-                                  float d[3]) {
+__device__ inline void ligand_vec(const MArgs &a, int l, const float rp[3], float d[3]) {
   if (l < a.m.n_wall_lig) {
     d[0] = min_image(a.m.lig_x[l] - rp[0], a.nx, a.periodic[0]);
     d[1] = min_image(a.m.lig_y[l] - rp[1], a.ny, a.periodic[1]);
@@ -318,89 +426,88 @@ __device__ inline void ligand_vec(const MArgs &a, int l, const float rp[3],
   }
 }
 
-__device__ inline void bump(int64_t *c, int k) {
-  atomicAdd((unsigned long long *)(c + k), 1ull);
-}
-
-/* Decide ruptures of held bonds and formation candidates of free receptors
- * from the state at the start of the step. */
-__global__ void bond_decide_kernel(MArgs a) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.m.n_rec) return;
+/* Ruptures of held bonds and formation candidates of free receptors, from the
+ * state at the start of the step; one wavefront per receptor, a lane per
+ * neighbouring bin. */
+__device__ void bond_decide(const MArgs &a, int r) {
+  const int lane = threadIdx.x & 63;
   const int64_t step = *a.m.d_step;
   float rp[3];
   int nodes[3];
   receptor_pos(a, r, rp, nodes);
   const int b = a.m.rec_bond[r];
-  a.m.rec_cand[r] = -1;
-  a.m.rec_release[r] = 0;
   if (b >= 0) {
-    float d[3];
-    ligand_vec(a, b, rp, d);
-    const float dist = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    const float tension = a.bond_k * (dist - a.bond_l0);
-    /* Bell: k_off = k_off0 exp(x_b f / kBT) for a pulling bond */
-    const float koff = a.k_off0 * expf(a.bell_xb_kT * fmaxf(tension, 0.0f));
-    const float p_off = 1.0f - expf(-koff);
-    const float u = pfg_uniform(a.seed, step, (uint32_t)r, PFG_RNG_BOND_OFF);
-    if (fabsf(u - p_off) < BOND_EPS_P) bump(a.m.counters, 2);
-    a.m.rec_release[r] = u < p_off ? 1 : 0;
+    if (lane == 0) {
+      a.m.rec_cand[r] = -1;
+      float d[3];
+      ligand_vec(a, b, rp, d);
+      const float dist = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const float tension = a.bond_k * (dist - a.bond_l0);
+      /* Bell: k_off = k_off0 exp(x_b f / kBT) for a pulling bond */
+      const float koff = a.k_off0 * expf(a.bell_xb_kT * fmaxf(tension, 0.0f));
+      const float p_off = 1.0f - expf(-koff);
+      const float u = pfg_uniform(a.seed, step, (uint32_t)r, PFG_RNG_BOND_OFF);
+      if (fabsf(u - p_off) < BOND_EPS_P) bump(a.m.counters, 2);
+      a.m.rec_release[r] = u < p_off ? 1 : 0;
+    }
     return;
   }
-  /* nearest free ligand within reach (ties: lower ligand id) */
+  /* nearest free ligand within reach; ties go to the lower ligand id */
   const float reach2 = a.bond_reach * a.bond_reach;
-  int best = -1;
+  int best = INT_MAX;
   float best_d2 = INFINITY;
   bool near_reach = false;
-  const int cx = cell_of(rp[0], a.nx, a.periodic[0]) * a.nb[0] / a.nx;
-  const int cy = cell_of(rp[1], a.ny, a.periodic[1]) * a.nb[1] / a.ny;
-  const int cz = cell_of(rp[2], a.nz, a.periodic[2]) * a.nb[2] / a.nz;
-  int bxs[3], bys[3], bzs[3];
-  const int nbx = nbr_bins(cx, a.nb[0], a.periodic[0], bxs);
-  const int nby = nbr_bins(cy, a.nb[1], a.periodic[1], bys);
-  const int nbz = nbr_bins(cz, a.nb[2], a.periodic[2], bzs);
-  if (a.m.n_wall_lig > 0) {
-    for (int jz = 0; jz < nbz; ++jz)
-      for (int jy = 0; jy < nby; ++jy)
-        for (int jx = 0; jx < nbx; ++jx) {
-          const int bin = bxs[jx] + a.nb[0] * (bys[jy] + a.nb[1] * bzs[jz]);
-          for (int k = a.m.lg_start[bin]; k < a.m.lg_start[bin + 1]; ++k) {
-            const int l = a.m.lg_items[k];
-            float d[3];
-            ligand_vec(a, l, rp, d);
-            const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            if (fabsf(sqrtf(d2) - a.bond_reach) < BOND_EPS_R) near_reach = true;
-            if (d2 > reach2 || a.m.lig_owner[l] >= 0) continue;
-            if (d2 < best_d2 || (d2 == best_d2 && l < best)) {
-              best_d2 = d2;
-              best = l;
-            }
-          }
+  int b3[3];
+  bin3_of(a, rp[0], rp[1], rp[2], b3);
+  const int bin = lane < 27 ? nth_nbr_bin(a, b3, lane) : -1;
+  if (bin >= 0) {
+    if (a.m.n_wall_lig > 0) {
+      for (int k = a.m.lg_start[bin]; k < a.m.lg_start[bin + 1]; ++k) {
+        const int l = a.m.lg_items[k];
+        float d[3];
+        ligand_vec(a, l, rp, d);
+        const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (fabsf(sqrtf(d2) - a.bond_reach) < BOND_EPS_R) near_reach = true;
+        if (d2 > reach2 || a.m.lig_owner[l] >= 0) continue;
+        if (d2 < best_d2 || (d2 == best_d2 && l < best)) {
+          best_d2 = d2;
+          best = l;
         }
-  }
-  if (a.m.fibrin_lig) {
-    for (int jz = 0; jz < nbz; ++jz)
-      for (int jy = 0; jy < nby; ++jy)
-        for (int jx = 0; jx < nbx; ++jx) {
-          const int bin = bxs[jx] + a.nb[0] * (bys[jy] + a.nb[1] * bzs[jz]);
-          for (int k = a.m.bin_start[bin]; k < a.m.bin_start[bin + 1]; ++k) {
-            const int mk = a.m.bin_items[k];
-            if (a.m.kind[mk] == PFG_MARKER_ELEMENT) continue;
-            const int l = a.m.n_wall_lig + mk;
-            float d[3];
-            ligand_vec(a, l, rp, d);
-            const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            if (fabsf(sqrtf(d2) - a.bond_reach) < BOND_EPS_R) near_reach = true;
-            if (d2 > reach2 || a.m.lig_owner[l] >= 0) continue;
-            if (d2 < best_d2 || (d2 == best_d2 && l < best)) {
-              best_d2 = d2;
-              best = l;
-            }
-          }
+      }
+    }
+    if (a.m.fibrin_lig) {
+      for (int k = a.m.bin_start[bin]; k < a.m.bin_start[bin + 1]; ++k) {
+        const int mk = a.m.bin_items[k];
+        if (a.m.kind[mk] == PFG_MARKER_ELEMENT) continue;
+        const int l = a.m.n_wall_lig + mk;
+        float d[3];
+        ligand_vec(a, l, rp, d);
+        const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (fabsf(sqrtf(d2) - a.bond_reach) < BOND_EPS_R) near_reach = true;
+        if (d2 > reach2 || a.m.lig_owner[l] >= 0) continue;
+        if (d2 < best_d2 || (d2 == best_d2 && l < best)) {
+          best_d2 = d2;
+          best = l;
         }
+      }
+    }
   }
-  if (near_reach) bump(a.m.counters, 2);
-  if (best < 0) return;
+  /* wavefront arg-min on (distance^2, ligand id) */
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float od = __shfl_xor(best_d2, o, 64);
+    const int ol = __shfl_xor(best, o, 64);
+    if (od < best_d2 || (od == best_d2 && ol < best)) {
+      best_d2 = od;
+      best = ol;
+    }
+  }
+  const bool any_near = __any(near_reach);
+  if (lane != 0) return;
+  a.m.rec_release[r] = 0;
+  a.m.rec_cand[r] = -1;
+  if (any_near) bump(a.m.counters, 2);
+  if (best == INT_MAX) return;
   const float p_on = 1.0f - expf(-a.k_on);
   const float u = pfg_uniform(a.seed, step, (uint32_t)r, PFG_RNG_BOND_ON);
   if (fabsf(u - p_on) < BOND_EPS_P) bump(a.m.counters, 2);
@@ -409,6 +516,47 @@ __global__ void bond_decide_kernel(MArgs a) {
     atomicMin(&a.m.lig_claim[best], r);
   }
 }
+
+/* roles: A = sort bins (blocks0, one lane per marker: slot-0 markers sort
+ * their bin), B = platelet sums (blocks1 = one block per platelet),
+ * C = bond decisions (blocks2, four receptors per block) */
+__global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
+  int blk = blockIdx.x;
+  if (a.roles & ROLE_A) {
+    if (blk < a.blocks0) {
+      const int i = blk * 256 + threadIdx.x;
+      if (i < a.m.n && a.m.marker_slot[i] == 0) {
+        const int b = a.m.marker_bin[i];
+        const int s = a.m.bin_start[b], e = a.m.bin_start[b + 1];
+        int32_t *it = a.m.bin_items;
+        for (int k = s + 1; k < e; ++k) {
+          const int v = it[k];
+          int j = k - 1;
+          while (j >= s && it[j] > v) {
+            it[j + 1] = it[j];
+            --j;
+          }
+          it[j + 1] = v;
+        }
+      }
+      return;
+    }
+    blk -= a.blocks0;
+  }
+  if (a.roles & ROLE_B) {
+    if (blk < a.blocks1) {
+      platelet_sums(a, blk);
+      return;
+    }
+    blk -= a.blocks1;
+  }
+  if (a.roles & ROLE_C) {
+    const int r = blk * 4 + (threadIdx.x >> 6);
+    if (r < a.m.n_rec) bond_decide(a, r);
+  }
+}
+
+/* ---------------- K7: bond resolution ---------------- */
 
 /* Apply ruptures and the lowest-index claim per ligand, then the bond force
  * k (|d| - l0) d/|d| on the receptor and its inverse-distance split over the
@@ -467,11 +615,83 @@ __global__ void bond_resolve_kernel(MArgs a) {
   a.m.rec_w[3 * r + 2] = w2;
 }
 
-/* ---------------- FORCES ---------------- */
+/* ---------------- K8: forces ---------------- */
 
+__device__ inline void morse_bin(const MArgs &a, int bin, int i, int body, float xi, float yi,
+                                 float zi, float &fx, float &fy, float &fz) {
+  const float two_rho_u0 = 2.0f * a.morse_rho * a.morse_u0;
+  for (int k = a.m.bin_start[bin]; k < a.m.bin_start[bin + 1]; ++k) {
+    const int j = a.m.bin_items[k];
+    if (j == i || a.m.body[j] != body || a.m.kind[j] != PFG_MARKER_ELEMENT) continue;
+    const float dx = a.m.x[j] - xi, dy = a.m.y[j] - yi, dz = a.m.z[j] - zi;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 >= a.morse_cut2 || r2 == 0.f) continue;
+    const float r = sqrtf(r2);
+    const float e = expf(-a.morse_rho * (r - a.morse_req));
+    /* -d(U S)/dr along (x_j - x_i)/r, U = U0 (e^2 - 2e) */
+    float c = two_rho_u0 * (e - e * e);
+    const float r1 = a.morse_cut - a.morse_taper;
+    if (a.morse_taper > 0.f && r > r1) {
+      const float t = (r - r1) / a.morse_taper;
+      const float S = 1.0f - t * t * (3.0f - 2.0f * t);
+      const float dS = 6.0f * t * (t - 1.0f) / a.morse_taper;
+      c = c * S + a.morse_u0 * (e * e - 2.0f * e) * dS;
+    }
+    c /= r;
+    fx += c * dx;
+    fy += c * dy;
+    fz += c * dz;
+  }
+}
+
+/* area and volume constraint force on vertex i from incident triangle t */
+__device__ inline void area_volume_term(const MArgs &a, int t, int i, float xi, float yi,
+                                        float zi, float g_area, float cvol, float &fx,
+                                        float &fy, float &fz) {
+  const int n0 = a.m.tri[3 * t], n1 = a.m.tri[3 * t + 1], n2 = a.m.tri[3 * t + 2];
+  const int s = n0 == i ? 0 : (n1 == i ? 1 : 2);
+  const int s1 = s == 0 ? n1 : (s == 1 ? n2 : n0);
+  const int s2 = s == 0 ? n2 : (s == 1 ? n0 : n1);
+  /* L_s = x_{s+1} - x_{s+2} (sem_kernel.cu:1688-1690) */
+  const float Lx = a.m.x[s1] - a.m.x[s2], Ly = a.m.y[s1] - a.m.y[s2], Lz = a.m.z[s1] - a.m.z[s2];
+  const float nx = a.m.tri_n[3 * t], ny = a.m.tri_n[3 * t + 1], nz = a.m.tri_n[3 * t + 2];
+  /* dS/dx_s = -(L_s x n)/2 with the inward n_elem (sem_kernel.cu:1719-1721) */
+  const float gx = -0.5f * (Ly * nz - Lz * ny);
+  const float gy = -0.5f * (Lz * nx - Lx * nz);
+  const float gz = -0.5f * (Lx * ny - Ly * nx);
+  const double St = a.m.tri_S[t], S0t = a.m.tri_S0[t];
+  const float ca = -a.ks * ((float)((St - S0t) / S0t) + g_area);
+  fx += ca * gx;
+  fy += ca * gy;
+  fz += ca * gz;
+  /* dV/dx_s = x_{s+1} x x_{s+2} (the reference's expansion at
+   * sem_kernel.cu:1723-1731 reduces to this), taken about x_i: the sum over a
+   * closed one-ring is unchanged by the shift and stays accurate in FP32 */
+  const float ax = a.m.x[s1] - xi, ay = a.m.y[s1] - yi, az = a.m.z[s1] - zi;
+  const float bx = a.m.x[s2] - xi, by = a.m.y[s2] - yi, bz = a.m.z[s2] - zi;
+  fx += cvol * (ay * bz - az * by);
+  fy += cvol * (az * bx - ax * bz);
+  fz += cvol * (ax * by - ay * bx);
+}
+
+__device__ inline float link_tension(const MArgs &a, float l, float l0) {
+  if (a.fiber_law == PFG_FIBER_LINEAR) {
+    /* Y A (l - l0)/l0 (network.cpp:364-373) */
+    return a.fiber_k * (l - l0) / l0;
+  }
+  /* WLC (fiber.cpp:721-730, network.cpp:386-405): CC = kB T / Lp with Lp = l0,
+   * tension CC (1/(4(1-x)^2) - 1/4 + x), x = (l - l0)/l0 */
+  const float x = fminf((l - l0) / l0, a.fiber_xmax);
+  const float tl = 1.0f - x;
+  return (a.fiber_k / l0) * (0.25f / (tl * tl) - 0.25f + x);
+}
+
+/* One wavefront per marker: lanes 0..26 take the neighbouring bins (Morse),
+ * all lanes stride over the marker's CSR lists (triangles, edges, receptor
+ * shares, fibrin links); a butterfly reduction sums the lanes. */
 __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0 && a.m.n_rec > 0 && a.detach_rate > 0.f) {
+  const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.m.n_rec > 0 && a.detach_rate > 0.f) {
     /* detach test (semLB.cu:451-455): no bond held after this step's update */
     const int64_t step = *a.m.d_step;
     if (a.m.counters[0] == 0 && a.m.counters[1] == 0) {
@@ -482,98 +702,56 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
       }
     }
   }
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= a.m.n) return;
   const int kind = a.m.kind[i];
   const float xi = a.m.x[i], yi = a.m.y[i], zi = a.m.z[i];
   float fx = 0.f, fy = 0.f, fz = 0.f;
   if (kind == PFG_MARKER_ELEMENT) {
     const int body = a.m.body[i];
-    /* Morse over the cell list, same platelet only */
-    if (a.morse_u0 != 0.f) {
+    if (a.morse_u0 != 0.f && lane < 27) {
       const int b = a.m.marker_bin[i];
-      const int bx = b % a.nb[0], by = (b / a.nb[0]) % a.nb[1], bz = b / (a.nb[0] * a.nb[1]);
-      int bxs[3], bys[3], bzs[3];
-      const int nbx = nbr_bins(bx, a.nb[0], a.periodic[0], bxs);
-      const int nby = nbr_bins(by, a.nb[1], a.periodic[1], bys);
-      const int nbz = nbr_bins(bz, a.nb[2], a.periodic[2], bzs);
-      const float two_rho_u0 = 2.0f * a.morse_rho * a.morse_u0;
-      for (int jz = 0; jz < nbz; ++jz)
-        for (int jy = 0; jy < nby; ++jy)
-          for (int jx = 0; jx < nbx; ++jx) {
-            const int bin = bxs[jx] + a.nb[0] * (bys[jy] + a.nb[1] * bzs[jz]);
-            for (int k = a.m.bin_start[bin]; k < a.m.bin_start[bin + 1]; ++k) {
-              const int j = a.m.bin_items[k];
-              if (j == i || a.m.body[j] != body || a.m.kind[j] != PFG_MARKER_ELEMENT) continue;
-              const float dx = a.m.x[j] - xi, dy = a.m.y[j] - yi, dz = a.m.z[j] - zi;
-              const float r2 = dx * dx + dy * dy + dz * dz;
-              if (r2 >= a.morse_cut2 || r2 == 0.f) continue;
-              const float r = sqrtf(r2);
-              const float e = expf(-a.morse_rho * (r - a.morse_req));
-              /* -d(U S)/dr along (x_j - x_i)/r, U = U0 (e^2 - 2e) */
-              float c = two_rho_u0 * (e - e * e);
-              const float r1 = a.morse_cut - a.morse_taper;
-              if (a.morse_taper > 0.f && r > r1) {
-                const float t = (r - r1) / a.morse_taper;
-                const float S = 1.0f - t * t * (3.0f - 2.0f * t);
-                const float dS = 6.0f * t * (t - 1.0f) / a.morse_taper;
-                c = c * S + a.morse_u0 * (e * e - 2.0f * e) * dS;
-              }
-              c /= r;
-              fx += c * dx;
-              fy += c * dy;
-              fz += c * dz;
-            }
-          }
+      const int b3[3] = {b % a.nb[0], (b / a.nb[0]) % a.nb[1], b / (a.nb[0] * a.nb[1])};
+      const int bin = nth_nbr_bin(a, b3, lane);
+      if (bin >= 0) morse_bin(a, bin, i, body, xi, yi, zi, fx, fy, fz);
     }
-    /* mesh-edge Hookean springs: k (r - L)/r (x_j - x_i) (sem_kernel.cu:374-384) */
-    if (a.k_spring != 0.f) {
-      for (int k = a.m.m2e_start[i]; k < a.m.m2e_start[i + 1]; ++k) {
-        const int j = a.m.m2e_nbr[k];
-        const float dx = a.m.x[j] - xi, dy = a.m.y[j] - yi, dz = a.m.z[j] - zi;
-        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        if (r == 0.f) continue;
-        const float c = a.k_spring * (r - a.m.m2e_l0[k]) / r;
-        fx += c * dx;
-        fy += c * dy;
-        fz += c * dz;
-      }
-    }
-    /* area and volume constraints over the incident triangles */
-    if (a.ks != 0.f || a.kv != 0.f) {
+    const int t0 = a.m.m2t_start[i], nt = (a.ks != 0.f || a.kv != 0.f) ? a.m.m2t_start[i + 1] - t0 : 0;
+    const int e0 = a.m.m2e_start[i], ne = a.k_spring != 0.f ? a.m.m2e_start[i + 1] - e0 : 0;
+    const int r0 = a.m.m2r_start[i], nr = a.m.m2r_start[i + 1] - r0;
+    float g_area = 0.f, cvol = 0.f;
+    if (nt > 0) {
       const double Sall = a.m.plt_S[body], S0all = a.m.plt_S0[body];
       const double V = a.m.plt_V[body], V0 = a.m.plt_V0[body];
-      const float g_area = (float)((Sall - S0all) / S0all);
-      const float cvol = -a.kv * (float)((V - V0) / V0) / 6.0f;
-      for (int k = a.m.m2t_start[i]; k < a.m.m2t_start[i + 1]; ++k) {
-        const int t = a.m.m2t[k];
-        const int n[3] = {a.m.tri[3 * t], a.m.tri[3 * t + 1], a.m.tri[3 * t + 2]};
-        const int s = n[0] == i ? 0 : (n[1] == i ? 1 : 2);
-        const int s1 = n[(s + 1) % 3], s2 = n[(s + 2) % 3];
-        /* L_s = x_{s+1} - x_{s+2} (sem_kernel.cu:1688-1690) */
-        const float Lx = a.m.x[s1] - a.m.x[s2], Ly = a.m.y[s1] - a.m.y[s2],
-                    Lz = a.m.z[s1] - a.m.z[s2];
-        const float nx = a.m.tri_n[3 * t], ny = a.m.tri_n[3 * t + 1], nz = a.m.tri_n[3 * t + 2];
-        /* dS/dx_s = -(L_s x n)/2 with the inward n_elem (sem_kernel.cu:1719-1721) */
-        const float gx = -0.5f * (Ly * nz - Lz * ny);
-        const float gy = -0.5f * (Lz * nx - Lx * nz);
-        const float gz = -0.5f * (Lx * ny - Ly * nx);
-        const double St = a.m.tri_S[t], S0t = a.m.tri_S0[t];
-        const float ca = -a.ks * ((float)((St - S0t) / S0t) + g_area);
-        fx += ca * gx;
-        fy += ca * gy;
-        fz += ca * gz;
-        /* dV/dx_s = x_{s+1} x x_{s+2} (the reference's expansion at
-         * sem_kernel.cu:1723-1731 reduces to this), taken about x_i: the sum over
-         * a closed one-ring is unchanged by the shift and stays exact in FP32 */
-        const float ax = a.m.x[s1] - xi, ay = a.m.y[s1] - yi, az = a.m.z[s1] - zi;
-        const float bx = a.m.x[s2] - xi, by = a.m.y[s2] - yi, bz = a.m.z[s2] - zi;
-        fx += cvol * (ay * bz - az * by);
-        fy += cvol * (az * bx - ax * bz);
-        fz += cvol * (ax * by - ay * bx);
+      g_area = (float)((Sall - S0all) / S0all);
+      cvol = -a.kv * (float)((V - V0) / V0) / 6.0f;
+    }
+    for (int k = lane; k < nt + ne + nr; k += 64) {
+      if (k < nt) {
+        area_volume_term(a, a.m.m2t[t0 + k], i, xi, yi, zi, g_area, cvol, fx, fy, fz);
+      } else if (k < nt + ne) {
+        /* mesh-edge Hookean spring k (r - L)/r (x_j - x_i) (sem_kernel.cu:374-384) */
+        const int q = e0 + k - nt;
+        const int j = a.m.m2e_nbr[q];
+        const float dx = a.m.x[j] - xi, dy = a.m.y[j] - yi, dz = a.m.z[j] - zi;
+        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+        if (r > 0.f) {
+          const float c = a.k_spring * (r - a.m.m2e_l0[q]) / r;
+          fx += c * dx;
+          fy += c * dy;
+          fz += c * dz;
+        }
+      } else {
+        /* receptor bond force, inverse-distance share of this vertex */
+        const int e = a.m.m2r[r0 + k - nt - ne];
+        const int r = e / 3, slot = e % 3;
+        const float w = a.m.rec_w[3 * r + slot];
+        fx += w * a.m.rec_force[3 * r];
+        fy += w * a.m.rec_force[3 * r + 1];
+        fz += w * a.m.rec_force[3 * r + 2];
       }
     }
     /* wall repulsion on the lowest element (sem_kernel.cu:409-430) */
-    if (a.wall_a != 0.f && a.m.plt_lowest[body] == i) {
+    if (lane == 0 && a.wall_a != 0.f && a.m.plt_lowest[body] == i) {
       float h = zi - a.wall_z;
       if (h <= a.wall_hc) {
         h = fmaxf(h, 0.01f * a.wall_hc);
@@ -581,42 +759,23 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
         fz += a.wall_a * a.wall_kappa * e / (1.0f - e);
       }
     }
-    /* receptor bond forces, inverse-distance shares */
-    for (int k = a.m.m2r_start[i]; k < a.m.m2r_start[i + 1]; ++k) {
-      const int e = a.m.m2r[k];
-      const int r = e / 3, slot = e % 3;
-      const float w = a.m.rec_w[3 * r + slot];
-      fx += w * a.m.rec_force[3 * r];
-      fy += w * a.m.rec_force[3 * r + 1];
-      fz += w * a.m.rec_force[3 * r + 2];
-    }
   } else {
     /* fibrin links in node-gather form (fiber.cpp:647-706) */
     const float sgn = a.fiber_ref_sign ? -1.0f : 1.0f;
-    for (int k = a.m.m2l_start[i]; k < a.m.m2l_start[i + 1]; ++k) {
-      const int j = a.m.m2l_nbr[k];
-      const float l0 = a.m.m2l_l0[k];
+    const int l0i = a.m.m2l_start[i], nl = a.m.m2l_start[i + 1] - l0i;
+    for (int k = lane; k < nl; k += 64) {
+      const int j = a.m.m2l_nbr[l0i + k];
+      const float l0 = a.m.m2l_l0[l0i + k];
       const float dx = a.m.x[j] - xi, dy = a.m.y[j] - yi, dz = a.m.z[j] - zi;
       const float l = sqrtf(dx * dx + dy * dy + dz * dz);
       if (l == 0.f || l0 == 0.f) continue;
-      float tension;
-      if (a.fiber_law == PFG_FIBER_LINEAR) {
-        /* Y A (l - l0)/l0 (network.cpp:364-373) */
-        tension = a.fiber_k * (l - l0) / l0;
-      } else {
-        /* WLC (fiber.cpp:721-730, network.cpp:386-405): CC = kB T / Lp with
-         * Lp = l0, tension CC (1/(4(1-x)^2) - 1/4 + x), x = (l - l0)/l0 */
-        const float x = fminf((l - l0) / l0, a.fiber_xmax);
-        const float tl = 1.0f - x;
-        tension = (a.fiber_k / l0) * (0.25f / (tl * tl) - 0.25f + x);
-      }
-      const float c = sgn * tension / l; /* restoring: toward j when stretched */
+      const float c = sgn * link_tension(a, l, l0) / l; /* restoring when sgn = 1 */
       fx += c * dx;
       fy += c * dy;
       fz += c * dz;
     }
     /* reaction of a receptor bond held on this node */
-    if (a.m.fibrin_lig) {
+    if (lane == 0 && a.m.fibrin_lig) {
       const int owner = a.m.lig_owner[a.m.n_wall_lig + i];
       if (owner >= 0) {
         fx -= a.m.rec_force[3 * owner];
@@ -625,39 +784,106 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
       }
     }
   }
-  a.m.fx[i] = fx;
-  a.m.fy[i] = fy;
-  a.m.fz[i] = fz;
+  fx = wave_sum(fx);
+  fy = wave_sum(fy);
+  fz = wave_sum(fz);
+  if (lane == 0) {
+    a.m.fx[i] = fx;
+    a.m.fy[i] = fy;
+    a.m.fz[i] = fz;
+  }
 }
 
-/* ---------------- UPDATE ---------------- */
+/* ---------------- K9: spread ---------------- */
 
-__global__ void update_kernel(MArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+/* One lane per (marker, stencil node): 64 lanes = one marker's 4x4x4 nodes. */
+__global__ __launch_bounds__(256) void spread_kernel(MArgs a) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= a.m.n) return;
+  if (!ib_coupled(a.m.kind[i], a.fiber_move)) return;
+  const float X = wrap_coord(a.m.x[i], a.nx, a.periodic[0]);
+  const float Y = wrap_coord(a.m.y[i], a.ny, a.periodic[1]);
+  const float Z = wrap_coord(a.m.z[i], a.nz, a.periodic[2]);
+  float wx, wy, wz;
+  const int ix = stencil_node(X, lane & 3, a.nx, a.periodic[0], &wx);
+  const int iy = stencil_node(Y, (lane >> 2) & 3, a.ny, a.periodic[1], &wy);
+  const int iz = stencil_node(Z, lane >> 4, a.nz, a.periodic[2], &wz);
+  if (ix < 0 || iy < 0 || iz < 0) return;
+  /* flag the tile of this node (idempotent byte store, read first so that a
+   * flagged tile takes no further stores) */
+  const int tile = (ix >> TILE_SHIFT_X) + a.tnx * ((iy >> TILE_SHIFT_Y) + a.tny * (iz >> TILE_SHIFT_Z));
+  if (a.tile_flag[tile] == 0) a.tile_flag[tile] = 1;
+  const float w = wx * wy * wz * a.force_scale;
+  const int64_t c = ix + (int64_t)a.nx * (iy + (int64_t)a.ny * iz);
+  atomicAdd(a.aux + c, w * a.m.fx[i]);
+  atomicAdd(a.aux + a.plane + c, w * a.m.fy[i]);
+  atomicAdd(a.aux + 2 * a.plane + c, w * a.m.fz[i]);
+}
+
+/* ---------------- K10: interpolation + update ---------------- */
+
+/* roles: A = interpolate V at the markers, B = move the markers and advance
+ * the step counter.  One wavefront per marker, a lane per stencil node. */
+__global__ __launch_bounds__(256) void move_kernel(MArgs a) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if ((a.roles & ROLE_B) && blockIdx.x == 0 && threadIdx.x == 0) *a.m.d_step += 1;
   if (i >= a.m.n) return;
   const int kind = a.m.kind[i];
-  if (kind == PFG_MARKER_FIXED) return;
+  const bool coupled = ib_coupled(kind, a.fiber_move);
+  float vx = 0.f, vy = 0.f, vz = 0.f;
+  if (a.roles & ROLE_A) {
+    if (coupled) {
+      const float X = wrap_coord(a.m.x[i], a.nx, a.periodic[0]);
+      const float Y = wrap_coord(a.m.y[i], a.ny, a.periodic[1]);
+      const float Z = wrap_coord(a.m.z[i], a.nz, a.periodic[2]);
+      float wx, wy, wz;
+      const int ix = stencil_node(X, lane & 3, a.nx, a.periodic[0], &wx);
+      const int iy = stencil_node(Y, (lane >> 2) & 3, a.ny, a.periodic[1], &wy);
+      const int iz = stencil_node(Z, lane >> 4, a.nz, a.periodic[2], &wz);
+      if (ix >= 0 && iy >= 0 && iz >= 0) {
+        const float w = wx * wy * wz;
+        const int64_t c = ix + (int64_t)a.nx * (iy + (int64_t)a.ny * iz);
+        vx = w * a.aux[c];
+        vy = w * a.aux[a.plane + c];
+        vz = w * a.aux[2 * a.plane + c];
+      }
+      vx = wave_sum(vx);
+      vy = wave_sum(vy);
+      vz = wave_sum(vz);
+      if (lane == 0) {
+        a.m.vx[i] = vx;
+        a.m.vy[i] = vy;
+        a.m.vz[i] = vz;
+      }
+    } else if (kind == PFG_MARKER_FIXED && lane == 0) {
+      a.m.vx[i] = 0.f;
+      a.m.vy[i] = 0.f;
+      a.m.vz[i] = 0.f;
+    }
+  } else if (lane == 0) {
+    vx = a.m.vx[i];
+    vy = a.m.vy[i];
+    vz = a.m.vz[i];
+  }
+  if (!(a.roles & ROLE_B) || lane != 0 || kind == PFG_MARKER_FIXED) return;
+  const float dt = a.marker_dt;
   if (kind == PFG_MARKER_FIBER && a.fiber_move == PFG_FIBER_KICK) {
     /* network.cpp:839-881: v <- e^{-g dt} v + F (1 - e^{-g dt})/g */
-    const float dt = a.marker_dt;
     const float vdt = expf(-a.fiber_gamma * dt);
     const float fdt = a.fiber_gamma > 0.f ? (1.0f - vdt) / a.fiber_gamma : dt;
-    const float vx = vdt * a.m.vx[i] + a.m.fx[i] * fdt;
-    const float vy = vdt * a.m.vy[i] + a.m.fy[i] * fdt;
-    const float vz = vdt * a.m.vz[i] + a.m.fz[i] * fdt;
+    vx = vdt * a.m.vx[i] + a.m.fx[i] * fdt;
+    vy = vdt * a.m.vy[i] + a.m.fy[i] * fdt;
+    vz = vdt * a.m.vz[i] + a.m.fz[i] * fdt;
     a.m.vx[i] = vx;
     a.m.vy[i] = vy;
     a.m.vz[i] = vz;
-    a.m.x[i] += vx * dt;
-    a.m.y[i] += vy * dt;
-    a.m.z[i] += vz * dt;
-    return;
   }
-  /* explicit Euler with the interpolated velocity (sem_kernel.cu:2112-2138);
-   * positions stay unwrapped, the wrap is applied where the lattice is indexed */
-  a.m.x[i] += a.m.vx[i] * a.marker_dt;
-  a.m.y[i] += a.m.vy[i] * a.marker_dt;
-  a.m.z[i] += a.m.vz[i] * a.marker_dt;
+  /* explicit Euler (sem_kernel.cu:2112-2138); positions stay unwrapped */
+  a.m.x[i] += vx * dt;
+  a.m.y[i] += vy * dt;
+  a.m.z[i] += vz * dt;
 }
 
 /* ---------------- launchers ---------------- */
@@ -665,6 +891,12 @@ __global__ void update_kernel(MArgs a) {
 static void make_margs(const pfg_handle *h, MArgs *a) {
   const pfg_params &p = h->p;
   a->m = h->m;
+  a->aux = h->aux;
+  a->plane = h->plane;
+  a->tile_flag = h->tile_flag;
+  a->tnx = h->tnx;
+  a->tny = h->tny;
+  a->tnz = h->tnz;
   a->nx = p.nx;
   a->ny = p.ny;
   a->nz = p.nz;
@@ -700,43 +932,74 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->fiber_xmax = p.fiber_xmax;
   a->fiber_gamma = p.fiber_gamma;
   a->marker_dt = p.marker_dt;
-  a->bin_cells = p.bin_cells;
+  a->force_scale = p.ib_force_scale;
+  a->roles = 0;
+  a->blocks0 = a->blocks1 = a->blocks2 = 0;
 }
 
-static inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+static inline int blocks_for(int64_t n, int per_block) {
+  return (int)((n + per_block - 1) / per_block);
+}
 
-hipError_t launch_cells(pfg_handle *h) {
+hipError_t launch_begin(pfg_handle *h, bool clear, bool count) {
+  MArgs a;
+  make_margs(h, &a);
+  const int nb_count = count ? blocks_for(h->m.n, 256) : 0;
+  a.roles = (clear ? ROLE_A : 0) | (count && h->m.n > 0 ? ROLE_B : 0);
+  a.blocks0 = clear ? blocks_for(h->ntiles, 256) : 0;
+  const int grid = a.blocks0 + ((a.roles & ROLE_B) ? nb_count : 0);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(begin_kernel, dim3(grid), dim3(256), 0, h->stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(pfg_handle *h) {
   const Markers &m = h->m;
   if (m.n == 0) return hipSuccess;
-  MArgs a;
-  make_margs(h, &a);
-  hipLaunchKernelGGL(bins_kernel, grid_for(m.n), dim3(256), 0, h->stream, a);
   hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
                      h->stream, m.bin_count, m.nbins, m.scan_partials);
-  hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, h->stream,
-                     m.scan_partials, m.scan_blocks, m.bin_start, m.nbins);
+  hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, h->stream, m.scan_partials,
+                     m.scan_blocks, m.bin_start, m.nbins);
   hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
                      h->stream, m.bin_count, m.nbins, m.scan_partials, m.bin_start);
-  hipLaunchKernelGGL(scatter_kernel, grid_for(m.n), dim3(256), 0, h->stream, a);
-  hipLaunchKernelGGL(sort_bins_kernel, grid_for(m.nbins), dim3(256), 0, h->stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_membrane(pfg_handle *h) {
-  if (h->m.n_plt == 0) return hipSuccess;
+hipError_t launch_place(pfg_handle *h, bool scatter, bool geometry) {
   MArgs a;
   make_margs(h, &a);
-  hipLaunchKernelGGL(membrane_kernel, dim3(h->m.n_plt), dim3(256), 0, h->stream, a);
+  scatter = scatter && h->m.n > 0;
+  geometry = geometry && h->m.n_tri > 0;
+  a.roles = (scatter ? ROLE_A : 0) | (geometry ? ROLE_B : 0);
+  a.blocks0 = scatter ? blocks_for(h->m.n, 256) : 0;
+  a.blocks1 = geometry ? blocks_for(h->m.n_tri, 256) : 0;
+  const int grid = a.blocks0 + a.blocks1;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(place_kernel, dim3(grid), dim3(256), 0, h->stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_bonds(pfg_handle *h) {
+hipError_t launch_gather(pfg_handle *h, bool sort, bool sums, bool decide) {
+  MArgs a;
+  make_margs(h, &a);
+  sort = sort && h->m.n > 0;
+  sums = sums && h->m.n_plt > 0;
+  decide = decide && h->m.n_rec > 0;
+  a.roles = (sort ? ROLE_A : 0) | (sums ? ROLE_B : 0) | (decide ? ROLE_C : 0);
+  a.blocks0 = sort ? blocks_for(h->m.n, 256) : 0;
+  a.blocks1 = sums ? h->m.n_plt : 0;
+  a.blocks2 = decide ? blocks_for(h->m.n_rec, 4) : 0;
+  const int grid = a.blocks0 + a.blocks1 + a.blocks2;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_kernel, dim3(grid), dim3(256), 0, h->stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_resolve(pfg_handle *h) {
   if (h->m.n_rec == 0) return hipSuccess;
   MArgs a;
   make_margs(h, &a);
-  hipLaunchKernelGGL(bond_decide_kernel, grid_for(h->m.n_rec), dim3(256), 0,
-                     h->stream, a);
-  hipLaunchKernelGGL(bond_resolve_kernel, grid_for(h->m.n_rec), dim3(256), 0,
+  hipLaunchKernelGGL(bond_resolve_kernel, dim3(blocks_for(h->m.n_rec, 256)), dim3(256), 0,
                      h->stream, a);
   return hipGetLastError();
 }
@@ -745,15 +1008,24 @@ hipError_t launch_forces(pfg_handle *h) {
   if (h->m.n == 0) return hipSuccess;
   MArgs a;
   make_margs(h, &a);
-  hipLaunchKernelGGL(forces_kernel, grid_for(h->m.n), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(forces_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_update(pfg_handle *h) {
+hipError_t launch_spread(pfg_handle *h) {
   if (h->m.n == 0) return hipSuccess;
   MArgs a;
   make_margs(h, &a);
-  hipLaunchKernelGGL(update_kernel, grid_for(h->m.n), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(spread_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_move(pfg_handle *h, bool interp, bool update) {
+  if (h->m.n == 0) return hipSuccess;
+  MArgs a;
+  make_margs(h, &a);
+  a.roles = (interp ? ROLE_A : 0) | (update ? ROLE_B : 0);
+  hipLaunchKernelGGL(move_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
   return hipGetLastError();
 }
 

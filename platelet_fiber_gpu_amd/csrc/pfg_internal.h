@@ -15,9 +15,6 @@
 
 namespace pfg {
 
-/* Blocks of the persistent tile-clear kernel. */
-constexpr int CLEAR_BLOCKS = 512;
-
 /* Everything the fused LB kernel needs, passed by value. */
 struct LBArgs {
   const void *fin;        /* post-collision state of the previous step      */
@@ -25,6 +22,8 @@ struct LBArgs {
   float *aux;             /* [3][plane]: IB force in, IB velocity out       */
   const uint8_t *tile_flag;
   const uint8_t *solid;   /* nullptr when there are no solid cells          */
+  const uint32_t *bbmask; /* per cell: bit d = upstream of d is solid,
+                             bit 31 = cell is solid; nullptr without flags  */
   int nx, ny, nz;
   int64_t plane;          /* element stride between direction planes         */
   int tnx, tny;           /* tiles per axis                                   */
@@ -49,7 +48,7 @@ struct Markers {
   int32_t *plt_elem_start = nullptr, *plt_tri_start = nullptr;
   int n_tri = 0;
   int32_t *tri = nullptr;        /* 3*n_tri */
-  double *tri_S0 = nullptr, *tri_S = nullptr;
+  double *tri_S0 = nullptr, *tri_S = nullptr, *tri_V = nullptr;
   float *tri_n = nullptr;        /* 3*n_tri unit normals (reference sign)   */
   double *plt_S0 = nullptr, *plt_V0 = nullptr, *plt_S = nullptr, *plt_V = nullptr;
   int32_t *plt_lowest = nullptr; /* lowest element per platelet             */
@@ -107,15 +106,14 @@ struct pfg_handle {
   void *f[2] = {nullptr, nullptr};
   int cur = 0;
   float *aux = nullptr;
-  uint8_t *tile_flag = nullptr;    /* one byte per tile, padded to 4 bytes     */
-  int32_t *tile_list = nullptr;    /* [0] count, then flagged tile ids         */
-  int32_t *tile_done = nullptr;    /* block counter of the clear kernel        */
+  uint8_t *tile_flag = nullptr;    /* one byte per 16x4x4 tile touched by IB   */
   int tnx = 0, tny = 0, tnz = 0;
   int64_t ntiles = 0;
   uint8_t *solid = nullptr;
+  uint32_t *bbmask = nullptr;  /* per-cell bounce mask built from solid        */
   double *red = nullptr;       /* reduction partials                        */
   int64_t red_blocks = 0;
-  int lb_cpt = 1, lb_by = 4;
+  int lb_cpt = 1, lb_bx = 64, lb_by = 1; /* LB launch: cells/thread, block */
   pfg::Markers m;
   bool model_loaded = false;
   bool fluid_only = false;
@@ -132,6 +130,7 @@ namespace pfg {
 /* lb.hip */
 void make_lb_args(const pfg_handle *h, LBArgs *a);
 hipError_t launch_lb(pfg_handle *h);
+hipError_t launch_bbmask(pfg_handle *h);
 hipError_t launch_lb_fields(pfg_handle *h, float *rho, float *ux, float *uy,
                             float *uz);
 hipError_t launch_init_eq(pfg_handle *h, const float *rho, const float *ux,
@@ -140,17 +139,15 @@ hipError_t launch_convert_f(pfg_handle *h, float *f_out_or_null,
                             const float *f_in_or_null);
 hipError_t launch_conservation(pfg_handle *h, double out[4]);
 
-/* ib.hip */
-hipError_t launch_spread(pfg_handle *h);
-hipError_t launch_interp(pfg_handle *h);
-hipError_t launch_clear(pfg_handle *h);
-
-/* markers.hip */
-hipError_t launch_cells(pfg_handle *h);
-hipError_t launch_membrane(pfg_handle *h);
-hipError_t launch_bonds(pfg_handle *h);
+/* markers.hip: fused marker-side launches, each with the roles it runs */
+hipError_t launch_begin(pfg_handle *h, bool clear, bool count);
+hipError_t launch_scan(pfg_handle *h);
+hipError_t launch_place(pfg_handle *h, bool scatter, bool geometry);
+hipError_t launch_gather(pfg_handle *h, bool sort, bool sums, bool decide);
+hipError_t launch_resolve(pfg_handle *h);
 hipError_t launch_forces(pfg_handle *h);
-hipError_t launch_update(pfg_handle *h);
+hipError_t launch_spread(pfg_handle *h);
+hipError_t launch_move(pfg_handle *h, bool interp, bool update);
 
 }  // namespace pfg
 

@@ -144,8 +144,9 @@ def test_graph_and_eager_agree(gpu):
     assert np.array_equal(out[0], out[1])
 
 
-@pytest.mark.parametrize("cpt,by", [(1, 1), (2, 2), (4, 4), (1, 4)])
-def test_launch_shapes_agree(gpu, cpt, by):
+@pytest.mark.parametrize("cpt,bx,by", [(1, 64, 1), (2, 64, 2), (4, 64, 4), (1, 128, 2),
+                                      (2, 256, 1)])
+def test_launch_shapes_agree(gpu, cpt, bx, by):
     pfg = gpu
     p = make_params(pfg, 96, 20, 18, (0, 1, 1), g=(1e-5, 0, 0))
     ref = None
@@ -154,7 +155,7 @@ def test_launch_shapes_agree(gpu, cpt, by):
         s.step(3)
         ref = s.f()
     with pfg.Solver(p) as s:
-        s.set_lb_tuning(cpt, by)
+        s.set_lb_tuning(cpt, bx, by)
         s.set_fluid_only(True)
         s.step(3)
         assert np.array_equal(s.f(), ref)
