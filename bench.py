@@ -5,14 +5,19 @@ Workload: single platelet in shear, 256x128x128 D3Q19 Couette flow (wall
 velocity 0.05, tau 0.7), one 2048-element SEM platelet with 512 receptors over
 4096 wall ligands, Bell bonds, FP32 storage.  A step is one full coupled step:
 cell lists, membrane sums, bonds, forces, IB spreading, fused LB
-collide-stream, IB interpolation, marker update, force-field clear.  Inputs are
-resident in HBM before the timed region; the timed region is K hipGraph step
-launches between two device synchronisations.
+collide-stream, IB interpolation and marker update.  Inputs are resident in
+HBM before the timed region; the timed region is K hipGraph step launches
+between two device synchronisations.
 
 Prints ONE JSON line.  value = coupled-step MLUPS (lattice cells x steps / s).
 roofline: the fused LB kernel (the dominant kernel), algorithmic bytes
 152 B/cell (FP32 storage: 19 x 4 B read + 19 x 4 B write) over its mean
-duration measured with HIP events on the handle's stream.
+duration, measured with HIP events on the solver's stream around each of K
+back-to-back launches of the same kernel on the same lattice.
+
+Other modes (for rocprofv3 runs):
+  --lb-only f32|f16|bf16   K fused LB steps only, no markers
+  --copy-probe             the library's float4 streaming copy (2 x 318 MB)
 """
 import argparse
 import json
@@ -25,19 +30,26 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {0: 152, 1: 76, 2: 76}
+STORES = {"f32": 0, "f16": 1, "bf16": 2}
 METRIC = "D3Q19 MLUPS, % HBM peak; coupled steps/s FP16-store vs FP32 at stated tol"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lb_fused_shear256_traffic.txt")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def timed_run(pfg, scene, store, steps, warmup, fluid_only=False):
+def make_solver(pfg, scene, store, fluid_only=False):
     p = scene.params
     p.store = store
     s = pfg.Solver(p)
     s.load(scene)
     s.set_fluid_only(fluid_only)
+    return s
+
+
+def timed_run(pfg, scene, store, steps, warmup, fluid_only=False):
+    s = make_solver(pfg, scene, store, fluid_only)
     s.step(warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -47,12 +59,21 @@ def timed_run(pfg, scene, store, steps, warmup, fluid_only=False):
     return s, dt
 
 
+def lb_kernel_us(pfg, scene, store, launches):
+    """Mean LB-kernel duration: HIP events on the solver's stream around
+    `launches` back-to-back launches of the LB kernel."""
+    s = make_solver(pfg, scene, store, fluid_only=True)
+    s.step(10)
+    ms = s.time_lb(launches)
+    s.close()
+    return ms * 1e3
+
+
 def read_traffic():
     """HBM bytes per LB launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "lb_fused_shear256_traffic.txt")
-    if not os.path.exists(path):
+    if not os.path.exists(TRAFFIC_FILE):
         return None
-    for line in open(path):
+    for line in open(TRAFFIC_FILE):
         if line.startswith("traffic_bytes_per_launch"):
             try:
                 return float(line.split("=")[1].split()[0])
@@ -108,6 +129,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--lb-only", choices=sorted(STORES), default=None)
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
@@ -118,44 +141,60 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.copy_probe:
+        import ctypes as C
+        g = C.c_double()
+        nbytes = 19 * 4 * 4194304
+        rc = pfg.lib().pfg_copy_probe(0, nbytes, max(1, args.steps), C.byref(g))
+        print(json.dumps({"copy_probe_gbs": round(g.value, 1), "bytes_each_way": nbytes,
+                          "launches": args.steps, "status": rc}), flush=True)
+        return
+
     K, W = args.steps, args.warmup
     scene = pfg.Scene(pfg.SCENE_SHEAR)
     N = scene.N
+
+    if args.lb_only:
+        st = STORES[args.lb_only]
+        s, dt = timed_run(pfg, scene, st, K, W, fluid_only=True)
+        s.close()
+        print(json.dumps({"lb_only": args.lb_only, "steps": K, "us_per_step": dt / K * 1e6,
+                          "mlups": N * K / dt / 1e6}), flush=True)
+        return
+
     log(f"workload 256x128x128 shear, {scene.n_markers} elements, "
         f"{scene.rec_tri.size} receptors, {scene.lig_x.size} ligands; K={K} W={W}")
-
     s32, dt32 = timed_run(pfg, scene, pfg.STORE_F32, K, W)
+    info = s32.step(0)
+    s32.close()
     ms_step = dt32 * 1e3 / K
     mlups = N * K / dt32 / 1e6
     log(f"fp32 coupled: {ms_step:.4f} ms/step, {mlups:.0f} MLUPS")
-    # LB kernel duration with HIP events around each LB launch (same stream)
-    prof_steps = min(K, 200)
-    tot_ms, lb_ms = s32.profile(prof_steps)
-    lb_avg_ms = lb_ms / prof_steps
-    achieved = BYTES_PER_CELL[0] * N / (lb_avg_ms * 1e-3) / 1e9
-    info = s32.step(0)
-    s32.close()
-    log(f"fp32 LB kernel: {lb_avg_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s")
 
-    # LB alone (no marker pipeline) in both storage formats
+    launches = min(K, 500)
+    lb32_us = lb_kernel_us(pfg, scene, pfg.STORE_F32, launches)
+    achieved = BYTES_PER_CELL[0] * N / (lb32_us * 1e-6) / 1e9
+    log(f"fp32 LB kernel: {lb32_us:.1f} us/launch, {achieved:.0f} GB/s")
+
     sl, dtl = timed_run(pfg, scene, pfg.STORE_F32, K, W, fluid_only=True)
     sl.close()
     lb32 = N * K / dtl / 1e6
     res16 = {}
     if not args.no_fp16:
         s16, dt16 = timed_run(pfg, scene, pfg.STORE_F16, K, W)
-        _, lb16_ms = s16.profile(prof_steps)
         s16.close()
         sl16, dtl16 = timed_run(pfg, scene, pfg.STORE_F16, K, W, fluid_only=True)
         sl16.close()
+        lb16_us = lb_kernel_us(pfg, scene, pfg.STORE_F16, launches)
         err16 = fp16_accuracy(pfg)
         res16 = {"coupled_mlups": round(N * K / dt16 / 1e6, 1),
                  "coupled_steps_per_s": round(K / dt16, 1),
                  "lb_only_mlups": round(N * K / dtl16 / 1e6, 1),
-                 "lb_kernel_us": round(lb16_ms / prof_steps * 1e3, 2),
-                 "lb_kernel_gbs": round(BYTES_PER_CELL[1] * N / (lb16_ms / prof_steps * 1e-3) / 1e9, 1),
+                 "lb_kernel_us": round(lb16_us, 2),
+                 "lb_kernel_gbs": round(BYTES_PER_CELL[1] * N / (lb16_us * 1e-6) / 1e9, 1),
                  "speedup_coupled_vs_fp32": round(dt32 / dt16, 3),
                  "speedup_lb_only_vs_fp32": round(dtl / dtl16, 3),
+                 "speedup_lb_kernel_vs_fp32": round(lb32_us / lb16_us, 3),
                  "rel_l2_u_vs_fp64": err16,
                  "rel_l2_u_case": "64x32x32 channel, tau 0.8, F 1e-6, 300 steps; tol 1e-3"}
         log(f"fp16: {res16}")
@@ -188,8 +227,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic(),
                      "kernel": "lb_fused_kernel<F32>", "bytes_per_cell": 152,
-                     "lb_kernel_us": round(lb_avg_ms * 1e3, 2),
-                     "mlups_kernel": round(N / (lb_avg_ms * 1e-3) / 1e6, 1)},
+                     "lb_kernel_us": round(lb32_us, 2),
+                     "mlups_kernel": round(N / (lb32_us * 1e-6) / 1e6, 1)},
         "fp16": res16,
         "cpu_baseline": cpu,
     }

@@ -290,11 +290,20 @@ pfg_status pfg_synchronize(pfg_handle *h);
  * launch, same stream).  Either pointer may be NULL. */
 pfg_status pfg_profile(pfg_handle *h, int nsteps, double *ms_total,
                        double *ms_lb);
-/* LB-kernel launch tuning: cells per thread along z (1, 2, 4) and the block
- * shape block_x (64, 128, 256 cells along x) by block_y (1, 2, 4 rows),
- * block_x * block_y <= 256. */
-pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_thread, int block_x,
-                             int block_y);
+/* Mean duration of the fused LB kernel: nlaunch back-to-back LB launches (no
+ * other kernels) between two HIP events on the handle's stream; advances the
+ * lattice by nlaunch steps. */
+pfg_status pfg_time_lb(pfg_handle *h, int nlaunch, double *ms_per_launch);
+
+/* Streaming-copy probe on `device`: copies `bytes` (rounded down to 16 B)
+ * between two fresh buffers `reps` times with a float4 kernel and reports the
+ * read+write rate in GB/s.  Used to calibrate HBM counters and as the
+ * practical bandwidth reference; allocates its own buffers. */
+pfg_status pfg_copy_probe(int device, size_t bytes, int reps, double *gbs);
+
+/* LB-kernel launch tuning: lattice cells each lane updates along z (1, 2 or
+ * 4; a wavefront updates one 64-cell x-block of that many rows). */
+pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane);
 
 #ifdef __cplusplus
 }

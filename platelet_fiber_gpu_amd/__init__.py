@@ -321,8 +321,14 @@ class Solver:
         self._c("pfg_profile", int(n), C.byref(tot), C.byref(lb))
         return tot.value, lb.value
 
-    def set_lb_tuning(self, cells_per_thread, block_x, block_y):
-        self._c("pfg_set_lb_tuning", int(cells_per_thread), int(block_x), int(block_y))
+    def time_lb(self, n):
+        """Mean LB-kernel duration in ms over n back-to-back launches."""
+        ms = C.c_double()
+        self._c("pfg_time_lb", int(n), C.byref(ms))
+        return ms.value
+
+    def set_lb_tuning(self, cells_per_lane):
+        self._c("pfg_set_lb_tuning", int(cells_per_lane))
 
     def device_bytes(self):
         b = C.c_size_t()

@@ -117,7 +117,9 @@ SIGNATURES = {
     "pfg_set_fluid_only": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_synchronize": (C.c_int, [C.c_void_p]),
     "pfg_profile": (C.c_int, [C.c_void_p, C.c_int, f64p, f64p]),
-    "pfg_set_lb_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "pfg_time_lb": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "pfg_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "pfg_set_lb_tuning": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_scene_opts_init": (None, [C.POINTER(SceneOpts), C.c_int]),
     "pfg_scene_build": (C.c_int, [C.POINTER(SceneOpts), C.POINTER(Params),
                                   C.POINTER(C.c_void_p)]),

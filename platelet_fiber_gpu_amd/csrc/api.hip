@@ -105,14 +105,12 @@ void free_markers(Markers &m) {
 }
 
 void drop_graph(pfg_handle *h) {
-  if (h->graph_exec) {
-    hipGraphExecDestroy(h->graph_exec);
-    h->graph_exec = nullptr;
-  }
-  if (h->graph_exec2) {
-    hipGraphExecDestroy(h->graph_exec2);
-    h->graph_exec2 = nullptr;
-  }
+  for (int g = 0; g < 2; ++g)
+    for (int c = 0; c < 2; ++c)
+      if (h->graphs[g][c]) {
+        hipGraphExecDestroy(h->graphs[g][c]);
+        h->graphs[g][c] = nullptr;
+      }
   h->graph_valid = false;
 }
 
@@ -126,6 +124,14 @@ void build_csr(int n, const std::vector<std::pair<int, int>> &kv,
   items.assign(kv.size(), 0);
   std::vector<int32_t> fillp(start.begin(), start.end() - 1);
   for (const auto &p : kv) items[(size_t)fillp[(size_t)p.first]++] = p.second;
+}
+
+/* one float4 per thread: the streaming pattern that reaches the highest HBM
+ * rate on MI355X */
+__global__ void copy_probe_kernel(const float4 *__restrict__ a, float4 *__restrict__ b,
+                                  size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 bool is_coupled_kind(int k) {
@@ -250,12 +256,14 @@ pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
     h->N = (int64_t)p.nx * p.ny * p.nz;
     h->plane = (h->N + 255) / 256 * 256;
     h->esize = p.store == PFG_STORE_F32 ? 4 : 2;
+    h->nxb = (p.nx + XBLOCK - 1) / XBLOCK;
+    const size_t fbytes = (size_t)h->esize * Q * XBLOCK * h->nxb * p.ny * p.nz;
     for (int k = 0; k < 2; ++k) {
-      if (hipMalloc(&h->f[k], (size_t)h->esize * Q * h->plane) != hipSuccess) {
+      if (hipMalloc(&h->f[k], fbytes) != hipSuccess) {
         st = PFG_ERR_ALLOC;
         break;
       }
-      h->dev_bytes += (size_t)h->esize * Q * h->plane;
+      h->dev_bytes += fbytes;
     }
     if (st != PFG_OK) break;
     if (dalloc(h, &h->aux, (size_t)3 * h->plane) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
@@ -269,12 +277,8 @@ pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
     if (hipMemset(h->tile_flag, 0, flag_bytes) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     h->red_blocks = std::min<int64_t>(2048, (h->N + 255) / 256);
     if (dalloc(h, &h->red, (size_t)4 * h->red_blocks) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
-    /* LB launch shape measured fastest on MI355X at 256x128x128: whole 256-cell
-     * rows per block (a row's cache lines stay on one XCD), two z-cells per
-     * thread for 4-byte storage, one for 2-byte storage */
-    h->lb_bx = p.nx >= 256 ? 256 : (p.nx >= 128 ? 128 : 64);
-    h->lb_by = std::min(4, 256 / h->lb_bx);
-    h->lb_cpt = p.store == PFG_STORE_F32 ? 2 : 1;
+    /* z-cells per lane of the LB kernel */
+    h->lb_cpt = 1;
     if (dalloc(h, &h->m.counters, 8) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->m.counters, 0, sizeof(int64_t) * 8) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     if (dalloc(h, &h->m.detach_step, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
@@ -833,26 +837,31 @@ static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
   return e;
 }
 
+/* Graphs of one step and of GRAPH_CHUNK steps, for each starting copy of the
+ * distributions (the copies alternate every step; a chunk has an even number
+ * of steps and ends on the copy it started from). */
 static hipError_t build_graphs(pfg_handle *h) {
   drop_graph(h);
   const int cur0 = h->cur;
   hipError_t e = hipSuccess;
   for (int g = 0; g < 2 && e == hipSuccess; ++g) {
-    h->cur = g;
-    hipGraph_t graph = nullptr;
-    e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) break;
-    hipError_t e2 = enqueue_step(h, nullptr, nullptr);
-    e = hipStreamEndCapture(h->stream, &graph);
-    if (e2 != hipSuccess) e = e2;
-    if (e == hipSuccess) {
-      hipGraphExec_t ex = nullptr;
-      e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    for (int chunk = 0; chunk < 2 && e == hipSuccess; ++chunk) {
+      h->cur = g;
+      hipGraph_t graph = nullptr;
+      e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) break;
+      hipError_t e2 = hipSuccess;
+      const int n = chunk ? GRAPH_CHUNK : 1;
+      for (int s = 0; s < n && e2 == hipSuccess; ++s) e2 = enqueue_step(h, nullptr, nullptr);
+      e = hipStreamEndCapture(h->stream, &graph);
+      if (e2 != hipSuccess) e = e2;
       if (e == hipSuccess) {
-        if (g == 0) h->graph_exec = ex; else h->graph_exec2 = ex;
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) h->graphs[g][chunk] = ex;
       }
+      if (graph) hipGraphDestroy(graph);
     }
-    if (graph) hipGraphDestroy(graph);
   }
   h->cur = cur0;
   if (e == hipSuccess) h->graph_valid = true;
@@ -879,9 +888,16 @@ pfg_status pfg_step(pfg_handle *h, int nsteps, pfg_step_info *info) {
   if (!h || nsteps < 0) return PFG_ERR_ARG;
   hipSetDevice(h->device);
   if (h->use_graph && nsteps > 0 && !h->graph_valid) PFG_TRY(build_graphs(h));
-  for (int s = 0; s < nsteps; ++s) {
+  int s = 0;
+  if (h->use_graph) {
+    for (; s + GRAPH_CHUNK <= nsteps; s += GRAPH_CHUNK) {
+      PFG_TRY(hipGraphLaunch(h->graphs[h->cur][1], h->stream));
+      h->h_step += GRAPH_CHUNK;
+    }
+  }
+  for (; s < nsteps; ++s) {
     if (h->use_graph) {
-      PFG_TRY(hipGraphLaunch(h->cur == 0 ? h->graph_exec : h->graph_exec2, h->stream));
+      PFG_TRY(hipGraphLaunch(h->graphs[h->cur][0], h->stream));
       h->cur = 1 - h->cur;
     } else {
       PFG_TRY(enqueue_step(h, nullptr, nullptr));
@@ -963,19 +979,68 @@ pfg_status pfg_profile(pfg_handle *h, int nsteps, double *ms_total, double *ms_l
   return PFG_OK;
 }
 
-pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_thread, int block_x,
-                             int block_y) {
+pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane) {
   if (!h) return PFG_ERR_ARG;
-  if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4)
-    return PFG_ERR_ARG;
-  if (block_x != 64 && block_x != 128 && block_x != 256) return PFG_ERR_ARG;
-  if (block_y != 1 && block_y != 2 && block_y != 4) return PFG_ERR_ARG;
-  if (block_x * block_y > 256) return PFG_ERR_ARG;
-  h->lb_cpt = cells_per_thread;
-  h->lb_bx = block_x;
-  h->lb_by = block_y;
+  if (cells_per_lane != 1 && cells_per_lane != 2 && cells_per_lane != 4) return PFG_ERR_ARG;
+  h->lb_cpt = cells_per_lane;
   drop_graph(h);
   return PFG_OK;
+}
+
+pfg_status pfg_time_lb(pfg_handle *h, int nlaunch, double *ms_per_launch) {
+  if (!h || nlaunch <= 0 || !ms_per_launch) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+  for (int k = 0; k < nlaunch && e == hipSuccess; ++k) {
+    e = launch_lb(h);
+    h->h_step++;
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) *ms_per_launch = (double)ms / nlaunch;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  return from_hip(e);
+}
+
+pfg_status pfg_copy_probe(int device, size_t bytes, int reps, double *gbs) {
+  if (!gbs || reps <= 0 || bytes < 16) return PFG_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return PFG_ERR_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return PFG_ERR_DEVICE;
+  const size_t n4 = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, n4 * 16);
+  if (e == hipSuccess) e = hipMalloc(&b, n4 * 16);
+  if (e == hipSuccess) e = hipMemset(a, 0, n4 * 16);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  const dim3 grid((unsigned)((n4 + 255) / 256));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(copy_probe_kernel, grid, dim3(256), 0, nullptr, a, b, n4);
+    e = hipEventRecord(e0, nullptr);
+  }
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    hipLaunchKernelGGL(copy_probe_kernel, grid, dim3(256), 0, nullptr, a, b, n4);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) *gbs = 2.0 * (double)n4 * 16.0 * reps / (ms * 1e-3) / 1e9;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return from_hip(e);
 }
 
 }  // extern "C"

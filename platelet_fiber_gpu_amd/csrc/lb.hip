@@ -12,14 +12,20 @@
  * FP32 storage, 76 B in 16-bit storage.
  *
  * Storage holds deviations f_i - w_i (FP32), (f_i - w_i) * 2^s (FP16) or
- * f_i - w_i (BF16); arithmetic is FP32 in deviation form throughout
+ * f_i - w_i (BF16).  Arithmetic is FP32 in deviation form throughout
  * (rho - 1 = sum of deviations), so FP32 rounding acts on the deviations and
- * not on the O(1) weights.
+ * not on the O(1) weights; FP16 values stay in the scaled space and only the
+ * four moments are unscaled.
  *
- * Layout: plane d of copy k at f[k] + d*plane, cell index x + nx*(y + ny*z).
- * A wavefront covers 64 consecutive x: loads of the x-shifted directions are
- * misaligned by one element but stay within two cache lines that the
- * neighbouring wavefront of the same block also reads.
+ * Layout (blocked SoA): the cells of a lattice row are grouped in x-blocks of
+ * 64 (one wavefront).  Block b = (x/64) + nxb*(y + ny*z) stores its 19
+ * directions as 19 consecutive chunks of 64 values:
+ *     element(d, x, y, z) = (b * 19 + d) * 64 + x % 64.
+ * A wavefront updates one x-block: its 19 stores fill one contiguous
+ * 19*64-value segment, and each of its loads comes from one chunk of an
+ * upstream block (one of nine rows), so DRAM pages are reused across the
+ * directions instead of streaming 38 separate planes.  The chunk base is
+ * wavefront-uniform; the lane offset is one of three values per lane.
  */
 #include <hip/hip_runtime.h>
 
@@ -32,121 +38,140 @@ namespace pfg {
 
 /* ---------------- storage formats ---------------- */
 
+/* load: the stored value as float (FP16 values stay scaled by 2^s);
+ * put: store an already scaled value. */
 template <int S>
 struct Store;
 
 template <>
 struct Store<PFG_STORE_F32> {
   using T = float;
-  __device__ static inline float load(const T *p, int64_t i, float) {
-    return p[i];
-  }
-  __device__ static inline void put(T *p, int64_t i, float v, float) {
-    __builtin_nontemporal_store(v, p + i);
-  }
+  static constexpr bool scaled = false;
+  __device__ static inline float load(const T *p) { return *p; }
+  __device__ static inline void put(T *p, float v) { __builtin_nontemporal_store(v, p); }
 };
 
 template <>
 struct Store<PFG_STORE_F16> {
   using T = uint16_t;
-  __device__ static inline float load(const T *p, int64_t i, float inv_scale) {
-    uint16_t b = p[i];
-    return (float)__builtin_bit_cast(_Float16, b) * inv_scale;
+  static constexpr bool scaled = true;
+  __device__ static inline float load(const T *p) {
+    return (float)__builtin_bit_cast(_Float16, *p);
   }
-  __device__ static inline void put(T *p, int64_t i, float v, float scale) {
-    _Float16 hv = (_Float16)(v * scale); /* round to nearest even */
-    __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, hv), p + i);
+  __device__ static inline void put(T *p, float v) {
+    const _Float16 hv = (_Float16)v; /* round to nearest even */
+    __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, hv), p);
   }
 };
 
 template <>
 struct Store<PFG_STORE_BF16> {
   using T = uint16_t;
-  __device__ static inline float load(const T *p, int64_t i, float) {
-    uint32_t u = ((uint32_t)p[i]) << 16;
-    return __builtin_bit_cast(float, u);
+  static constexpr bool scaled = false;
+  __device__ static inline float load(const T *p) {
+    return __builtin_bit_cast(float, ((uint32_t)*p) << 16);
   }
-  __device__ static inline void put(T *p, int64_t i, float v, float) {
+  __device__ static inline void put(T *p, float v) {
     uint32_t u = __builtin_bit_cast(uint32_t, v);
     u += 0x7FFFu + ((u >> 16) & 1u); /* round to nearest even */
-    __builtin_nontemporal_store((uint16_t)(u >> 16), p + i);
+    __builtin_nontemporal_store((uint16_t)(u >> 16), p);
   }
 };
+
+__device__ __host__ inline int64_t felem(int nxb, int ny, int d, int x, int y, int z) {
+  const int64_t b = (int64_t)(x >> 6) + (int64_t)nxb * (y + (int64_t)ny * z);
+  return (b * Q + d) * XBLOCK + (x & (XBLOCK - 1));
+}
 
 /* ---------------- the fused kernel ---------------- */
 
 enum { LB_MODE_STEP = 0, LB_MODE_FIELDS = 1 };
 
+/* One wavefront per (x-block, y, CPT consecutive z). */
 template <int S, int CPT, int MODE>
-__global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho,
-                                                       float *oux, float *ouy,
-                                                       float *ouz) {
+__global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, float *oux,
+                                                       float *ouy, float *ouz) {
   using ST = Store<S>;
   using T = typename ST::T;
+  constexpr int R = Q * XBLOCK; /* values per x-block */
   const T *__restrict__ fin = (const T *)a.fin;
   T *__restrict__ fout = (T *)a.fout;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  if (x >= a.nx || y >= a.ny) return;
-  const int64_t nxy = (int64_t)a.nx * a.ny;
+  const int lane = threadIdx.x & (XBLOCK - 1);
+  const int64_t task = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nzc = (a.nz + CPT - 1) / CPT;
+  if (task >= (int64_t)a.nxb * a.ny * nzc) return;
+  const int xb = (int)(task % a.nxb);
+  const int y = (int)((task / a.nxb) % a.ny);
+  const int zc = (int)(task / ((int64_t)a.nxb * a.ny));
+  const int x = xb * XBLOCK + lane;
+  if (x >= a.nx) return;
 
-  /* upstream x and y (wrapped or flagged as outside a wall), shared by the
-   * CPT cells of this thread */
-  const bool x0 = x == 0, x1 = x == a.nx - 1, y0 = y == 0, y1 = y == a.ny - 1;
-  const int xm = x0 ? (a.periodic[0] ? a.nx - 1 : 0) : x - 1;
-  const int xp = x1 ? (a.periodic[0] ? 0 : x) : x + 1;
-  const bool oxm = x0 && !a.periodic[0], oxp = x1 && !a.periodic[0];
-  const int64_t ym = (int64_t)a.nx * (y0 ? (a.periodic[1] ? a.ny - 1 : 0) : y - 1);
-  const int64_t yp = (int64_t)a.nx * (y1 ? (a.periodic[1] ? 0 : y) : y + 1);
-  const int64_t yc = (int64_t)a.nx * y;
+  /* lane offsets from the upstream chunk base (placed one block early, so all
+   * offsets are >= 0): centre, upstream x-1, upstream x+1.  Lanes at the ends
+   * of the row take their x-neighbour in the edge pass below. */
+  const bool xlo = x == 0, xhi = x == a.nx - 1;
+  const uint32_t o_c = R + lane;
+  const uint32_t o_m = xlo ? o_c : (lane > 0 ? R + lane - 1 : XBLOCK - 1);
+  const uint32_t o_p = xhi ? o_c : (lane < XBLOCK - 1 ? R + lane + 1 : 2 * R);
+
+  /* upstream rows, wavefront-uniform */
+  const int y0 = y == 0, y1 = y == a.ny - 1;
+  const int ym = y0 ? (a.periodic[1] ? a.ny - 1 : y) : y - 1;
+  const int yp = y1 ? (a.periodic[1] ? 0 : y) : y + 1;
   const bool oym = y0 && !a.periodic[1], oyp = y1 && !a.periodic[1];
+
+  const float sc = ST::scaled ? a.scale : 1.0f;
+  const float isc = ST::scaled ? a.inv_scale : 1.0f;
 
 #pragma unroll
   for (int c = 0; c < CPT; ++c) {
-    const int z = blockIdx.z * CPT + c;
+    const int z = zc * CPT + c;
     if (z >= a.nz) return;
-    const int64_t idx = x + yc + nxy * z;
-    /* bit d: the upstream cell of direction d is solid; bit 31: this cell is */
+    const int64_t idx = x + (int64_t)a.nx * (y + (int64_t)a.ny * z); /* cell index */
     uint32_t mask = 0;
     if (a.bbmask != nullptr) {
       mask = a.bbmask[idx];
-      if (mask >> 31) continue;
+      if (mask >> 31) continue; /* solid cell */
     }
-    const bool z0 = z == 0, z1 = z == a.nz - 1;
-    const int64_t zm = nxy * (z0 ? (a.periodic[2] ? a.nz - 1 : 0) : z - 1);
-    const int64_t zp = nxy * (z1 ? (a.periodic[2] ? 0 : z) : z + 1);
-    const int64_t zc = nxy * z;
+    const int z0 = z == 0, z1 = z == a.nz - 1;
+    const int zm = z0 ? (a.periodic[2] ? a.nz - 1 : z) : z - 1;
+    const int zp = z1 ? (a.periodic[2] ? 0 : z) : z + 1;
     const bool ozm = z0 && !a.periodic[2], ozp = z1 && !a.periodic[2];
+    const int64_t bown = (int64_t)xb + (int64_t)a.nxb * (y + (int64_t)a.ny * z);
 
-    /* 19 upstream loads, each from its plane's uniform base plus a 32-bit cell
-     * offset; outside a wall the offset falls back to this cell (a valid
-     * address) and the edge fix-up below replaces the value */
     float dev[Q];
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-      const int sx = CX[d] > 0 ? xm : (CX[d] < 0 ? xp : x);
-      const int64_t sy = CY[d] > 0 ? ym : (CY[d] < 0 ? yp : yc);
-      const int64_t sz = CZ[d] > 0 ? zm : (CZ[d] < 0 ? zp : zc);
-      const uint32_t src = (uint32_t)(sx + sy + sz);
-      dev[d] = ST::load(fin + (int64_t)d * a.plane, src, a.inv_scale);
+      const int sy = CY[d] > 0 ? ym : (CY[d] < 0 ? yp : y);
+      const int sz = CZ[d] > 0 ? zm : (CZ[d] < 0 ? zp : z);
+      const int64_t bsrc = (int64_t)xb + (int64_t)a.nxb * (sy + (int64_t)a.ny * sz);
+      const T *base = fin + ((bsrc * Q + d) * XBLOCK - R);
+      const uint32_t o = CX[d] > 0 ? o_m : (CX[d] < 0 ? o_p : o_c);
+      dev[d] = ST::load(base + o);
     }
-    /* halfway bounce-back where the upstream cell is a wall or solid: the
-     * population leaving this cell towards it returns, plus the moving-wall
-     * momentum of that wall (z over y over x at edges) */
-    const bool edge = oxm || oxp || oym || oyp || ozm || ozp || mask != 0;
+    /* edge pass: rows next to a wall (wavefront-uniform), row ends, solid
+     * neighbours.  Upstream cell outside a wall or solid: halfway bounce-back,
+     * the population leaving this cell returns plus the moving-wall momentum
+     * 6 w (c . u_wall) (z wall over y wall over x wall); otherwise the
+     * periodic x-neighbour. */
+    const bool edge = xlo || xhi || oym || oyp || ozm || ozp || mask != 0;
     if (edge) {
 #pragma unroll
       for (int d = 1; d < Q; ++d) {
-        const bool ox = CX[d] > 0 ? oxm : (CX[d] < 0 ? oxp : false);
-        const bool oy = CY[d] > 0 ? oym : (CY[d] < 0 ? oyp : false);
         const bool oz = CZ[d] > 0 ? ozm : (CZ[d] < 0 ? ozp : false);
-        if (ox || oy || oz || ((mask >> d) & 1u)) {
-          const int sxs = CX[d] > 0 ? 0 : 1, sys = CY[d] > 0 ? 0 : 1,
-                    szs = CZ[d] > 0 ? 0 : 1;
+        const bool oy = CY[d] > 0 ? oym : (CY[d] < 0 ? oyp : false);
+        const bool xe = CX[d] > 0 ? xlo : (CX[d] < 0 ? xhi : false);
+        const bool ox = xe && !a.periodic[0];
+        if (oz || oy || ox || ((mask >> d) & 1u)) {
+          const int sxs = CX[d] > 0 ? 0 : 1, sys = CY[d] > 0 ? 0 : 1, szs = CZ[d] > 0 ? 0 : 1;
           const float term = oz ? a.wall_term[2][szs][d]
-                                : (oy ? a.wall_term[1][sys][d]
-                                      : (ox ? a.wall_term[0][sxs][d] : 0.f));
-          dev[d] = ST::load(fin + (int64_t)BB[d] * a.plane, idx, a.inv_scale) + term;
+                                : (oy ? a.wall_term[1][sys][d] : (ox ? a.wall_term[0][sxs][d] : 0.f));
+          dev[d] = ST::load(fin + (bown * Q + BB[d]) * XBLOCK + lane) + term;
+        } else if (xe) { /* periodic wrap along x */
+          const int sx = CX[d] > 0 ? a.nx - 1 : 0;
+          const int sy = CY[d] > 0 ? ym : (CY[d] < 0 ? yp : y);
+          const int sz = CZ[d] > 0 ? zm : (CZ[d] < 0 ? zp : z);
+          dev[d] = ST::load(fin + felem(a.nxb, a.ny, d, sx, sy, sz));
         }
       }
     }
@@ -160,19 +185,22 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho,
       if (CY[d] > 0) jy += dev[d]; else if (CY[d] < 0) jy -= dev[d];
       if (CZ[d] > 0) jz += dev[d]; else if (CZ[d] < 0) jz -= dev[d];
     }
+    if (ST::scaled) {
+      drho *= isc;
+      jx *= isc;
+      jy *= isc;
+      jz *= isc;
+    }
     const float rho = 1.0f + drho;
     float Fx = a.g[0], Fy = a.g[1], Fz = a.g[2];
     bool flagged = false;
-    int64_t aidx = 0;
     if (MODE == LB_MODE_STEP) {
-      const int tile = (x >> TILE_SHIFT_X) +
-                       a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
+      const int tile = (x >> TILE_SHIFT_X) + a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
       flagged = a.tile_flag[tile] != 0;
       if (flagged) {
-        aidx = idx;
-        Fx += a.aux[aidx];
-        Fy += a.aux[a.plane + aidx];
-        Fz += a.aux[2 * a.plane + aidx];
+        Fx += a.aux[idx];
+        Fy += a.aux[a.plane + idx];
+        Fz += a.aux[2 * a.plane + idx];
       }
     }
     const float inv_rho = 1.0f / rho;
@@ -188,24 +216,26 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho,
       continue;
     }
     if (flagged) { /* velocity for the IB interpolation of this step */
-      a.aux[aidx] = ux;
-      a.aux[a.plane + aidx] = uy;
-      a.aux[2 * a.plane + aidx] = uz;
+      a.aux[idx] = ux;
+      a.aux[a.plane + idx] = uy;
+      a.aux[2 * a.plane + idx] = uz;
     }
 
     const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
     const float uF = ux * Fx + uy * Fy + uz * Fz;
+    const float keep = 1.0f - a.omega;
+    T *own = fout + bown * R + lane;
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-      const float w = weight(d);
+      const float w = weight(d) * sc; /* scaled weight: feq and source in storage units */
       float cu = 0.f, cF = 0.f;
       if (CX[d] > 0) { cu += ux; cF += Fx; } else if (CX[d] < 0) { cu -= ux; cF -= Fx; }
       if (CY[d] > 0) { cu += uy; cF += Fy; } else if (CY[d] < 0) { cu -= uy; cF -= Fy; }
       if (CZ[d] > 0) { cu += uz; cF += Fz; } else if (CZ[d] < 0) { cu -= uz; cF -= Fz; }
       const float feq_dev = w * drho + w * rho * (3.0f * cu + 4.5f * cu * cu - usq15);
       const float src = a.guo * w * (3.0f * (cF - uF) + 9.0f * cu * cF);
-      const float out = dev[d] + a.omega * (feq_dev - dev[d]) + src;
-      ST::put(fout + (int64_t)d * a.plane, idx, out, a.scale);
+      const float out = keep * dev[d] + a.omega * feq_dev + src;
+      ST::put(own + d * XBLOCK, out);
     }
   }
 }
@@ -240,74 +270,87 @@ __global__ void bbmask_kernel(LBArgs a, const uint8_t *solid, uint32_t *mask) {
   mask[idx] = m;
 }
 
+__device__ inline void cell_xyz(const LBArgs &a, int64_t idx, int &x, int &y, int &z) {
+  x = (int)(idx % a.nx);
+  y = (int)((idx / a.nx) % a.ny);
+  z = (int)(idx / ((int64_t)a.nx * a.ny));
+}
+
 template <int S>
-__global__ void init_eq_kernel(LBArgs a, void *dst0, void *dst1,
-                               const float *rho_in, const float *ux_in,
-                               const float *uy_in, const float *uz_in,
+__global__ void init_eq_kernel(LBArgs a, void *dst0, void *dst1, const float *rho_in,
+                               const float *ux_in, const float *uy_in, const float *uz_in,
                                int64_t n) {
   using ST = Store<S>;
   using T = typename ST::T;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
+  int x, y, z;
+  cell_xyz(a, idx, x, y, z);
   const float rho = rho_in ? rho_in[idx] : 1.0f;
   const float ux = ux_in ? ux_in[idx] : 0.f;
   const float uy = uy_in ? uy_in[idx] : 0.f;
   const float uz = uz_in ? uz_in[idx] : 0.f;
   const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
   const float drho = rho - 1.0f;
+  const float sc = ST::scaled ? a.scale : 1.0f;
 #pragma unroll
   for (int d = 0; d < Q; ++d) {
     const float w = weight(d);
     const float cu = CX[d] * ux + CY[d] * uy + CZ[d] * uz;
     const float dev = w * drho + w * rho * (3.0f * cu + 4.5f * cu * cu - usq15);
-    ST::put((T *)dst0 + (int64_t)d * a.plane, idx, dev, a.scale);
-    ST::put((T *)dst1 + (int64_t)d * a.plane, idx, dev, a.scale);
+    const int64_t e = felem(a.nxb, a.ny, d, x, y, z);
+    ST::put((T *)dst0 + e, dev * sc);
+    ST::put((T *)dst1 + e, dev * sc);
   }
 }
 
-/* plane d: to_float -> out[i] = w_d + dev; else stored <- in[i] - w_d */
+/* direction d: to_float -> buf[i] = w_d + dev; else stored <- buf[i] - w_d */
 template <int S>
-__global__ void convert_plane_kernel(LBArgs a, void *stored, float *buf,
-                                     int d, int to_float, int64_t n) {
+__global__ void convert_plane_kernel(LBArgs a, void *stored, float *buf, int d, int to_float,
+                                     int64_t n) {
   using ST = Store<S>;
   using T = typename ST::T;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
-  T *p = (T *)stored + (int64_t)d * a.plane;
+  int x, y, z;
+  cell_xyz(a, idx, x, y, z);
+  T *p = (T *)stored + felem(a.nxb, a.ny, d, x, y, z);
   const float w = weight(d);
   if (to_float) {
-    buf[idx] = w + ST::load(p, idx, a.inv_scale);
+    buf[idx] = w + ST::load(p) * (ST::scaled ? a.inv_scale : 1.0f);
   } else {
-    ST::put(p, idx, buf[idx] - w, a.scale);
+    ST::put(p, (buf[idx] - w) * (ST::scaled ? a.scale : 1.0f));
   }
 }
 
 /* FP64 totals of mass and momentum over fluid cells, one partial per block. */
 template <int S>
-__global__ __launch_bounds__(256) void conservation_kernel(LBArgs a,
-                                                          const void *stored,
-                                                          double *partials,
-                                                          int64_t n) {
+__global__ __launch_bounds__(256) void conservation_kernel(LBArgs a, const void *stored,
+                                                          double *partials, int64_t n) {
   using ST = Store<S>;
   using T = typename ST::T;
   const T *p = (const T *)stored;
+  const double isc = ST::scaled ? (double)a.inv_scale : 1.0;
   double m = 0, jx = 0, jy = 0, jz = 0;
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (int64_t)gridDim.x * blockDim.x) {
     if (a.solid != nullptr && a.solid[idx]) continue;
-    double cm = 1.0, cx = 0, cy = 0, cz = 0;
+    int x, y, z;
+    cell_xyz(a, idx, x, y, z);
+    const int64_t e0 = felem(a.nxb, a.ny, 0, x, y, z);
+    double cm = 0, cx = 0, cy = 0, cz = 0;
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
-      const double v = (double)ST::load(p + (int64_t)d * a.plane, idx, a.inv_scale);
+      const double v = (double)ST::load(p + e0 + d * XBLOCK);
       cm += v;
       cx += CX[d] * v;
       cy += CY[d] * v;
       cz += CZ[d] * v;
     }
-    m += cm;
-    jx += cx;
-    jy += cy;
-    jz += cz;
+    m += 1.0 + cm * isc;
+    jx += cx * isc;
+    jy += cy * isc;
+    jz += cz * isc;
   }
   __shared__ double sh[4][256];
   const int t = threadIdx.x;
@@ -340,6 +383,7 @@ void make_lb_args(const pfg_handle *h, LBArgs *a) {
   a->nx = p.nx;
   a->ny = p.ny;
   a->nz = p.nz;
+  a->nxb = h->nxb;
   a->plane = h->plane;
   a->tnx = h->tnx;
   a->tny = h->tny;
@@ -355,34 +399,34 @@ void make_lb_args(const pfg_handle *h, LBArgs *a) {
     a->inv_scale = 1.0f;
   }
   /* halfway bounce-back with a moving wall (Ladd): the population entering
-   * from the wall is f_bb(d) + 6 w_d rho_w (c_d . u_w), rho_w = 1 */
+   * from the wall is f_bb(d) + 6 w_d rho_w (c_d . u_w), rho_w = 1; in the
+   * storage units of the format */
   for (int ax = 0; ax < 3; ++ax)
     for (int s = 0; s < 2; ++s)
       for (int d = 0; d < Q; ++d) {
         const float *u = s == 0 ? p.wall_u_lo[ax] : p.wall_u_hi[ax];
         const float cu = CX[d] * u[0] + CY[d] * u[1] + CZ[d] * u[2];
-        a->wall_term[ax][s][d] = 6.0f * weight(d) * cu;
+        a->wall_term[ax][s][d] = 6.0f * weight(d) * cu * a->scale;
       }
 }
 
 template <int S, int CPT>
-static hipError_t launch_lb_t(pfg_handle *h, const LBArgs &a, int mode,
-                              float *rho, float *ux, float *uy, float *uz) {
-  const int bx = h->lb_bx, by = h->lb_by;
-  dim3 block(bx, by, 1);
-  dim3 grid((a.nx + bx - 1) / bx, (a.ny + by - 1) / by, (a.nz + CPT - 1) / CPT);
+static hipError_t launch_lb_t(pfg_handle *h, const LBArgs &a, int mode, float *rho,
+                              float *ux, float *uy, float *uz) {
+  const int64_t waves = (int64_t)a.nxb * a.ny * ((a.nz + CPT - 1) / CPT);
+  const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
   if (mode == LB_MODE_STEP)
-    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_STEP>), grid, block, 0,
-                       h->stream, a, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_STEP>), grid, block, 0, h->stream, a,
+                       nullptr, nullptr, nullptr, nullptr);
   else
-    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_FIELDS>), grid, block, 0,
-                       h->stream, a, rho, ux, uy, uz);
+    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_FIELDS>), grid, block, 0, h->stream, a,
+                       rho, ux, uy, uz);
   return hipGetLastError();
 }
 
 template <int S>
-static hipError_t launch_lb_s(pfg_handle *h, const LBArgs &a, int mode,
-                              float *rho, float *ux, float *uy, float *uz) {
+static hipError_t launch_lb_s(pfg_handle *h, const LBArgs &a, int mode, float *rho, float *ux,
+                              float *uy, float *uz) {
   switch (h->lb_cpt) {
     case 2: return launch_lb_t<S, 2>(h, a, mode, rho, ux, uy, uz);
     case 4: return launch_lb_t<S, 4>(h, a, mode, rho, ux, uy, uz);
@@ -390,8 +434,8 @@ static hipError_t launch_lb_s(pfg_handle *h, const LBArgs &a, int mode,
   }
 }
 
-static hipError_t dispatch_lb(pfg_handle *h, int mode, float *rho, float *ux,
-                              float *uy, float *uz) {
+static hipError_t dispatch_lb(pfg_handle *h, int mode, float *rho, float *ux, float *uy,
+                              float *uz) {
   LBArgs a;
   make_lb_args(h, &a);
   switch (h->p.store) {
@@ -407,8 +451,7 @@ hipError_t launch_lb(pfg_handle *h) {
   return e;
 }
 
-hipError_t launch_lb_fields(pfg_handle *h, float *rho, float *ux, float *uy,
-                            float *uz) {
+hipError_t launch_lb_fields(pfg_handle *h, float *rho, float *ux, float *uy, float *uz) {
   return dispatch_lb(h, LB_MODE_FIELDS, rho, ux, uy, uz);
 }
 
@@ -421,24 +464,24 @@ hipError_t launch_bbmask(pfg_handle *h) {
   return hipGetLastError();
 }
 
-hipError_t launch_init_eq(pfg_handle *h, const float *rho, const float *ux,
-                          const float *uy, const float *uz) {
+hipError_t launch_init_eq(pfg_handle *h, const float *rho, const float *ux, const float *uy,
+                          const float *uz) {
   LBArgs a;
   make_lb_args(h, &a);
   const int64_t n = h->N;
   dim3 block(256), grid((unsigned)((n + 255) / 256));
   switch (h->p.store) {
     case PFG_STORE_F16:
-      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F16>, grid, block, 0, h->stream,
-                         a, h->f[0], h->f[1], rho, ux, uy, uz, n);
+      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F16>, grid, block, 0, h->stream, a, h->f[0],
+                         h->f[1], rho, ux, uy, uz, n);
       break;
     case PFG_STORE_BF16:
-      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_BF16>, grid, block, 0, h->stream,
-                         a, h->f[0], h->f[1], rho, ux, uy, uz, n);
+      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_BF16>, grid, block, 0, h->stream, a, h->f[0],
+                         h->f[1], rho, ux, uy, uz, n);
       break;
     default:
-      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F32>, grid, block, 0, h->stream,
-                         a, h->f[0], h->f[1], rho, ux, uy, uz, n);
+      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F32>, grid, block, 0, h->stream, a, h->f[0],
+                         h->f[1], rho, ux, uy, uz, n);
   }
   return hipGetLastError();
 }
@@ -455,28 +498,28 @@ hipError_t launch_convert_f(pfg_handle *h, float *f_out, const float *f_in) {
   for (int d = 0; d < Q && e == hipSuccess; ++d) {
     const int to_float = f_out != nullptr;
     if (!to_float) {
-      e = hipMemcpyAsync(buf, f_in + (int64_t)d * n, sizeof(float) * n,
-                         hipMemcpyHostToDevice, h->stream);
+      e = hipMemcpyAsync(buf, f_in + (int64_t)d * n, sizeof(float) * n, hipMemcpyHostToDevice,
+                         h->stream);
       if (e != hipSuccess) break;
     }
     switch (h->p.store) {
       case PFG_STORE_F16:
-        hipLaunchKernelGGL(convert_plane_kernel<PFG_STORE_F16>, grid, block, 0,
-                           h->stream, a, stored, buf, d, to_float, n);
+        hipLaunchKernelGGL(convert_plane_kernel<PFG_STORE_F16>, grid, block, 0, h->stream, a,
+                           stored, buf, d, to_float, n);
         break;
       case PFG_STORE_BF16:
-        hipLaunchKernelGGL(convert_plane_kernel<PFG_STORE_BF16>, grid, block, 0,
-                           h->stream, a, stored, buf, d, to_float, n);
+        hipLaunchKernelGGL(convert_plane_kernel<PFG_STORE_BF16>, grid, block, 0, h->stream, a,
+                           stored, buf, d, to_float, n);
         break;
       default:
-        hipLaunchKernelGGL(convert_plane_kernel<PFG_STORE_F32>, grid, block, 0,
-                           h->stream, a, stored, buf, d, to_float, n);
+        hipLaunchKernelGGL(convert_plane_kernel<PFG_STORE_F32>, grid, block, 0, h->stream, a,
+                           stored, buf, d, to_float, n);
     }
     e = hipGetLastError();
     if (e != hipSuccess) break;
     if (to_float) {
-      e = hipMemcpyAsync(f_out + (int64_t)d * n, buf, sizeof(float) * n,
-                         hipMemcpyDeviceToHost, h->stream);
+      e = hipMemcpyAsync(f_out + (int64_t)d * n, buf, sizeof(float) * n, hipMemcpyDeviceToHost,
+                         h->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   }
@@ -490,22 +533,22 @@ hipError_t launch_conservation(pfg_handle *h, double out[4]) {
   const int blocks = (int)h->red_blocks;
   switch (h->p.store) {
     case PFG_STORE_F16:
-      hipLaunchKernelGGL(conservation_kernel<PFG_STORE_F16>, dim3(blocks), dim3(256),
-                         0, h->stream, a, h->f[h->cur], h->red, h->N);
+      hipLaunchKernelGGL(conservation_kernel<PFG_STORE_F16>, dim3(blocks), dim3(256), 0,
+                         h->stream, a, h->f[h->cur], h->red, h->N);
       break;
     case PFG_STORE_BF16:
-      hipLaunchKernelGGL(conservation_kernel<PFG_STORE_BF16>, dim3(blocks), dim3(256),
-                         0, h->stream, a, h->f[h->cur], h->red, h->N);
+      hipLaunchKernelGGL(conservation_kernel<PFG_STORE_BF16>, dim3(blocks), dim3(256), 0,
+                         h->stream, a, h->f[h->cur], h->red, h->N);
       break;
     default:
-      hipLaunchKernelGGL(conservation_kernel<PFG_STORE_F32>, dim3(blocks), dim3(256),
-                         0, h->stream, a, h->f[h->cur], h->red, h->N);
+      hipLaunchKernelGGL(conservation_kernel<PFG_STORE_F32>, dim3(blocks), dim3(256), 0,
+                         h->stream, a, h->f[h->cur], h->red, h->N);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   std::vector<double> part((size_t)blocks * 4);
-  e = hipMemcpyAsync(part.data(), h->red, sizeof(double) * part.size(),
-                     hipMemcpyDeviceToHost, h->stream);
+  e = hipMemcpyAsync(part.data(), h->red, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                     h->stream);
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return e;

@@ -15,6 +15,10 @@
 
 namespace pfg {
 
+/* Steps captured per chunk graph (even: a chunk ends on the copy of f it
+ * started from). */
+constexpr int GRAPH_CHUNK = 8;
+
 /* Everything the fused LB kernel needs, passed by value. */
 struct LBArgs {
   const void *fin;        /* post-collision state of the previous step      */
@@ -25,7 +29,8 @@ struct LBArgs {
   const uint32_t *bbmask; /* per cell: bit d = upstream of d is solid,
                              bit 31 = cell is solid; nullptr without flags  */
   int nx, ny, nz;
-  int64_t plane;          /* element stride between direction planes         */
+  int nxb;                /* x-blocks of XBLOCK cells per row                 */
+  int64_t plane;          /* stride between the components of aux            */
   int tnx, tny;           /* tiles per axis                                   */
   int periodic[3];
   float omega;            /* 1/tau                                           */
@@ -102,7 +107,10 @@ struct pfg_handle {
   int device = 0;
   hipStream_t stream = nullptr;
   int64_t N = 0, plane = 0;
+  int nxb = 0;                 /* x-blocks per row in the population layout */
   int esize = 4;
+  /* populations, blocked: element (d,x,y,z) at
+   * ((x/64 + nxb*(y + ny*z))*19 + d)*64 + x%64 */
   void *f[2] = {nullptr, nullptr};
   int cur = 0;
   float *aux = nullptr;
@@ -113,14 +121,14 @@ struct pfg_handle {
   uint32_t *bbmask = nullptr;  /* per-cell bounce mask built from solid        */
   double *red = nullptr;       /* reduction partials                        */
   int64_t red_blocks = 0;
-  int lb_cpt = 1, lb_bx = 64, lb_by = 1; /* LB launch: cells/thread, block */
+  int lb_cpt = 1;              /* LB launch: z-cells per lane               */
   pfg::Markers m;
   bool model_loaded = false;
   bool fluid_only = false;
   int64_t h_step = 0;
   bool use_graph = true;
-  hipGraphExec_t graph_exec = nullptr;   /* step starting from f[0]         */
-  hipGraphExec_t graph_exec2 = nullptr;  /* step starting from f[1]         */
+  /* [starting copy of f][0: one step, 1: GRAPH_CHUNK steps] */
+  hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   bool graph_valid = false;
   size_t dev_bytes = 0;
 };

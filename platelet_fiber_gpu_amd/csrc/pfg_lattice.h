@@ -32,6 +32,9 @@ PFG_HDI constexpr float weight(int d) {
   return d == 0 ? 1.0f / 3.0f : (d < 7 ? 1.0f / 18.0f : 1.0f / 36.0f);
 }
 
+/* Cells per x-block of the blocked population layout (one wavefront). */
+constexpr int XBLOCK = 64;
+
 /* Force-field tile (cells) used to skip untouched IB force/velocity data. */
 constexpr int TILE_X = 16, TILE_Y = 4, TILE_Z = 4;
 constexpr int TILE_SHIFT_X = 4, TILE_SHIFT_Y = 2, TILE_SHIFT_Z = 2;
