@@ -97,12 +97,15 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
   const T *__restrict__ fin = (const T *)a.fin;
   T *__restrict__ fout = (T *)a.fout;
   const int lane = threadIdx.x & (XBLOCK - 1);
-  const int64_t task = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int nzc = (a.nz + CPT - 1) / CPT;
-  if (task >= (int64_t)a.nxb * a.ny * nzc) return;
-  const int xb = (int)(task % a.nxb);
-  const int y = (int)((task / a.nxb) % a.ny);
-  const int zc = (int)(task / ((int64_t)a.nxb * a.ny));
+  /* wavefront-uniform task decomposition in 32-bit arithmetic (x-blocks per
+   * lattice stay below 2^31) */
+  const uint32_t task = blockIdx.x * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t rows = (uint32_t)a.nxb * (uint32_t)a.ny;
+  const uint32_t zc = task / rows;
+  if (zc * CPT >= (uint32_t)a.nz) return;
+  const uint32_t rem = task - zc * rows;
+  const int y = (int)(rem / (uint32_t)a.nxb);
+  const int xb = (int)(rem - (uint32_t)y * a.nxb);
   const int x = xb * XBLOCK + lane;
   if (x >= a.nx) return;
 
@@ -120,7 +123,6 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
   const int yp = y1 ? (a.periodic[1] ? 0 : y) : y + 1;
   const bool oym = y0 && !a.periodic[1], oyp = y1 && !a.periodic[1];
 
-  const float sc = ST::scaled ? a.scale : 1.0f;
   const float isc = ST::scaled ? a.inv_scale : 1.0f;
 
 #pragma unroll
@@ -137,15 +139,15 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
     const int zm = z0 ? (a.periodic[2] ? a.nz - 1 : z) : z - 1;
     const int zp = z1 ? (a.periodic[2] ? 0 : z) : z + 1;
     const bool ozm = z0 && !a.periodic[2], ozp = z1 && !a.periodic[2];
-    const int64_t bown = (int64_t)xb + (int64_t)a.nxb * (y + (int64_t)a.ny * z);
+    const uint64_t bown = (uint32_t)xb + (uint32_t)a.nxb * (uint32_t)(y + a.ny * z);
 
     float dev[Q];
 #pragma unroll
     for (int d = 0; d < Q; ++d) {
       const int sy = CY[d] > 0 ? ym : (CY[d] < 0 ? yp : y);
       const int sz = CZ[d] > 0 ? zm : (CZ[d] < 0 ? zp : z);
-      const int64_t bsrc = (int64_t)xb + (int64_t)a.nxb * (sy + (int64_t)a.ny * sz);
-      const T *base = fin + ((bsrc * Q + d) * XBLOCK - R);
+      const uint64_t bsrc = (uint32_t)xb + (uint32_t)a.nxb * (uint32_t)(sy + a.ny * sz);
+      const T *base = fin + ((int64_t)(bsrc * Q + d) * XBLOCK - R);
       const uint32_t o = CX[d] > 0 ? o_m : (CX[d] < 0 ? o_p : o_c);
       dev[d] = ST::load(base + o);
     }
@@ -176,14 +178,19 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
       }
     }
 
-    /* moments: rho - 1 and momentum from deviations (sum_i w_i c_i = 0) */
-    float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+    /* moments: rho - 1 and momentum from deviations (sum_i w_i c_i = 0),
+     * over the nine pairs (d, opposite of d): the pair sum feeds the density,
+     * the pair difference the momentum.  -0.0f starts are exact no-ops. */
+    float drho = dev[0], jx = -0.0f, jy = -0.0f, jz = -0.0f;
 #pragma unroll
-    for (int d = 0; d < Q; ++d) {
-      drho += dev[d];
-      if (CX[d] > 0) jx += dev[d]; else if (CX[d] < 0) jx -= dev[d];
-      if (CY[d] > 0) jy += dev[d]; else if (CY[d] < 0) jy -= dev[d];
-      if (CZ[d] > 0) jz += dev[d]; else if (CZ[d] < 0) jz -= dev[d];
+    for (int d = 1; d < Q; ++d) {
+      if (d < BB[d]) {
+        const float s = dev[d] + dev[BB[d]], df = dev[d] - dev[BB[d]];
+        drho += s;
+        if (CX[d] > 0) jx += df; else if (CX[d] < 0) jx -= df;
+        if (CY[d] > 0) jy += df; else if (CY[d] < 0) jy -= df;
+        if (CZ[d] > 0) jz += df; else if (CZ[d] < 0) jz -= df;
+      }
     }
     if (ST::scaled) {
       drho *= isc;
@@ -203,7 +210,7 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
         Fz += a.aux[2 * a.plane + idx];
       }
     }
-    const float inv_rho = 1.0f / rho;
+    const float inv_rho = __builtin_amdgcn_rcpf(rho); /* hardware reciprocal, 1 ulp */
     const float ux = (jx + 0.5f * Fx) * inv_rho;
     const float uy = (jy + 0.5f * Fy) * inv_rho;
     const float uz = (jz + 0.5f * Fz) * inv_rho;
@@ -221,21 +228,33 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
       a.aux[2 * a.plane + idx] = uz;
     }
 
-    const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+    /* BGK with Guo forcing, in deviation form and storage units:
+     *   out_d = (1-omega) dev_d + omega (feq_d - w_d) + guo w_d (3 (c_d-u).F + 9 (c_d.u)(c_d.F))
+     * split into the part even in c_d,
+     *   E = w_d [K0 + cu (4.5 omega rho cu + 9 guo cF)],
+     *   K0 = omega (drho - 1.5 rho u.u) - 3 guo u.F,
+     * and the odd part O = w_d (3 omega rho cu + 3 guo cF); the pair
+     * (d, opposite) takes E + O and E - O. */
+    const float usq = ux * ux + uy * uy + uz * uz;
     const float uF = ux * Fx + uy * Fy + uz * Fz;
-    const float keep = 1.0f - a.omega;
+    const float K0 = fmaf(-a.guo3, uF, a.omega * fmaf(-1.5f * rho, usq, drho));
+    const float Ab = a.om45 * rho, Cp = a.om3 * rho;
     T *own = fout + bown * R + lane;
+    ST::put(own, fmaf(a.keep, dev[0], a.ws[0] * K0));
 #pragma unroll
-    for (int d = 0; d < Q; ++d) {
-      const float w = weight(d) * sc; /* scaled weight: feq and source in storage units */
-      float cu = 0.f, cF = 0.f;
-      if (CX[d] > 0) { cu += ux; cF += Fx; } else if (CX[d] < 0) { cu -= ux; cF -= Fx; }
-      if (CY[d] > 0) { cu += uy; cF += Fy; } else if (CY[d] < 0) { cu -= uy; cF -= Fy; }
-      if (CZ[d] > 0) { cu += uz; cF += Fz; } else if (CZ[d] < 0) { cu -= uz; cF -= Fz; }
-      const float feq_dev = w * drho + w * rho * (3.0f * cu + 4.5f * cu * cu - usq15);
-      const float src = a.guo * w * (3.0f * (cF - uF) + 9.0f * cu * cF);
-      const float out = keep * dev[d] + a.omega * feq_dev + src;
-      ST::put(own + d * XBLOCK, out);
+    for (int d = 1; d < Q; ++d) {
+      if (d < BB[d]) {
+        const int k = d < 7 ? 1 : 2;
+        float cu = -0.0f, cF = -0.0f;
+        if (CX[d] > 0) { cu += ux; cF += Fx; } else if (CX[d] < 0) { cu -= ux; cF -= Fx; }
+        if (CY[d] > 0) { cu += uy; cF += Fy; } else if (CY[d] < 0) { cu -= uy; cF -= Fy; }
+        if (CZ[d] > 0) { cu += uz; cF += Fz; } else if (CZ[d] < 0) { cu -= uz; cF -= Fz; }
+        const float t = fmaf(a.gB[k], cF, (a.ws[k] * Ab) * cu);
+        const float E = fmaf(cu, t, a.ws[k] * K0);
+        const float O = fmaf(a.gD[k], cF, (a.ws[k] * Cp) * cu);
+        ST::put(own + d * XBLOCK, fmaf(a.keep, dev[d], E + O));
+        ST::put(own + BB[d] * XBLOCK, fmaf(a.keep, dev[BB[d]], E - O));
+      }
     }
   }
 }
@@ -397,6 +416,17 @@ void make_lb_args(const pfg_handle *h, LBArgs *a) {
   } else {
     a->scale = 1.0f;
     a->inv_scale = 1.0f;
+  }
+  a->keep = 1.0f - a->omega;
+  a->om15 = 1.5f * a->omega;
+  a->om45 = 4.5f * a->omega;
+  a->om3 = 3.0f * a->omega;
+  a->guo3 = 3.0f * a->guo;
+  const int rep[3] = {0, 1, 7}; /* rest, axis, diagonal */
+  for (int k = 0; k < 3; ++k) {
+    a->ws[k] = weight(rep[k]) * a->scale;
+    a->gB[k] = 9.0f * a->guo * a->ws[k];
+    a->gD[k] = 3.0f * a->guo * a->ws[k];
   }
   /* halfway bounce-back with a moving wall (Ladd): the population entering
    * from the wall is f_bb(d) + 6 w_d rho_w (c_d . u_w), rho_w = 1; in the

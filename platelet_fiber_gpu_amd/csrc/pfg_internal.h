@@ -37,6 +37,11 @@ struct LBArgs {
   float guo;              /* 1 - 1/(2 tau)                                   */
   float g[3];             /* body force                                      */
   float scale, inv_scale; /* FP16 deviation scale                            */
+  /* collision constants for the three weight classes (rest, axis, diagonal),
+   * in storage units: ws = w * scale, gB = 9 guo ws, gD = 3 guo ws */
+  float keep;             /* 1 - omega                                       */
+  float om15, om45, om3, guo3; /* 1.5 omega, 4.5 omega, 3 omega, 3 guo        */
+  float ws[3], gB[3], gD[3];
   /* 6 w_d (c_d . u_wall) for [axis][0 lower, 1 upper][d] */
   float wall_term[3][2][Q];
 };
