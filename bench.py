@@ -74,9 +74,10 @@ def read_traffic():
     if not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
-        if line.startswith("traffic_bytes_per_launch"):
+        key, _, val = line.partition("=")
+        if key.strip() == "traffic_bytes_per_launch":
             try:
-                return float(line.split("=")[1].split()[0])
+                return float(val.split()[0])
             except (IndexError, ValueError):
                 return None
     return None
