@@ -1,7 +1,8 @@
 /*
  * pfg.h -- C ABI of the MI355X coupled platelet-fibrin step.
  *
- * One handle owns one lattice (D3Q19, SoA planes in HBM), one marker set
+ * One handle owns one lattice (D3Q19, populations in HBM as 64-cell x-blocks
+ * of 19 direction chunks), one marker set
  * (platelet SEM elements and fibrin nodes), their topology and one HIP stream.
  * Every call returns a pfg_status; nothing calls exit().  Handles are
  * thread-compatible (one thread per handle at a time), not thread-safe.
@@ -282,6 +283,12 @@ pfg_status pfg_set_step(pfg_handle *h, int64_t step);
 pfg_status pfg_set_graph(pfg_handle *h, int enable);
 /* Skip the marker pipeline (pure LB stepping). */
 pfg_status pfg_set_fluid_only(pfg_handle *h, int fluid_only);
+/* enable = 1: the run ends at detach, as the reference's loop does
+ * (gpuSEM_LB.c:1433-1446 stops when sem_invokeGPUKernel_Force reports done,
+ * semLB.cu:451-455).  Steps launched after the detach test fired change
+ * neither the lattice, the markers nor the bonds, and pfg_step_info.step
+ * reports the steps that ran.  Default 0: stepping continues after detach. */
+pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable);
 pfg_status pfg_synchronize(pfg_handle *h);
 
 /* ---- measurement ---- */

@@ -312,6 +312,11 @@ class Solver:
     def set_fluid_only(self, on):
         self._c("pfg_set_fluid_only", int(bool(on)))
 
+    def set_halt_on_detach(self, on):
+        """End the run at detach: later steps change nothing and step()
+        reports the steps that ran."""
+        self._c("pfg_set_halt_on_detach", int(bool(on)))
+
     def synchronize(self):
         self._c("pfg_synchronize")
 

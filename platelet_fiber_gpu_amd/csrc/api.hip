@@ -905,7 +905,25 @@ pfg_status pfg_step(pfg_handle *h, int nsteps, pfg_step_info *info) {
     h->h_step++;
   }
   PFG_TRY(hipStreamSynchronize(h->stream));
+  if (h->halt_on_detach && h->model_loaded && !h->fluid_only) {
+    /* steps launched after the detach test fired left the lattice in place
+     * while the host alternated the copies: realign the current copy with
+     * the steps that actually ran */
+    int64_t done = 0;
+    PFG_TRY(download(h, &done, h->m.d_step, 1));
+    if (h->h_step > done) {
+      if ((h->h_step - done) & 1) h->cur = 1 - h->cur;
+      h->h_step = done;
+    }
+  }
   return fill_info(h, info);
+}
+
+pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable) {
+  if (!h) return PFG_ERR_ARG;
+  h->halt_on_detach = enable != 0;
+  drop_graph(h);
+  return PFG_OK;
 }
 
 pfg_status pfg_run_stage(pfg_handle *h, int stage) {

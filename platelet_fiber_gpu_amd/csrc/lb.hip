@@ -96,6 +96,7 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
   constexpr int R = Q * XBLOCK; /* values per x-block */
   const T *__restrict__ fin = (const T *)a.fin;
   T *__restrict__ fout = (T *)a.fout;
+  if (MODE == LB_MODE_STEP && a.halt != nullptr && *a.halt != 0) return;
   const int lane = threadIdx.x & (XBLOCK - 1);
   /* wavefront-uniform task decomposition in 32-bit arithmetic (x-blocks per
    * lattice stay below 2^31) */
@@ -399,6 +400,7 @@ void make_lb_args(const pfg_handle *h, LBArgs *a) {
   a->tile_flag = h->tile_flag;
   a->solid = h->solid;
   a->bbmask = h->bbmask;
+  a->halt = h->halt_on_detach ? h->m.counters + 1 : nullptr;
   a->nx = p.nx;
   a->ny = p.ny;
   a->nz = p.nz;

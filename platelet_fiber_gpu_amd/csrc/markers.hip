@@ -67,6 +67,8 @@ struct MArgs {
   float force_scale;
   /* role split of the fused launches */
   int roles;
+  int halt; /* 1: after the detach test fires, leave bonds, force field and
+               markers unchanged (the step loop ends at detach) */
   int blocks0, blocks1, blocks2; /* blocks of role ranges 0, 1, 2 */
 };
 
@@ -80,6 +82,10 @@ constexpr float BOND_EPS_P = 1e-6f; /* |u - p| below this: near-threshold      *
 constexpr float BOND_EPS_R = 1e-4f; /* |r - reach| below this: near-threshold  */
 
 /* ---------------- small helpers ---------------- */
+
+/* The detach test has fired and the run halts there (written by an earlier
+ * launch, so every lane reads the same value). */
+__device__ inline bool halted(const MArgs &a) { return a.halt && a.m.counters[1] != 0; }
 
 __device__ inline int cell_of(float v, int n, int periodic) {
   int c = (int)floorf(v);
@@ -552,7 +558,7 @@ __global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
   }
   if (a.roles & ROLE_C) {
     const int r = blk * 4 + (threadIdx.x >> 6);
-    if (r < a.m.n_rec) bond_decide(a, r);
+    if (r < a.m.n_rec && !halted(a)) bond_decide(a, r);
   }
 }
 
@@ -563,7 +569,7 @@ __global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
  * three vertices (sem_kernel.cu:2039-2065). */
 __global__ void bond_resolve_kernel(MArgs a) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.m.n_rec) return;
+  if (r >= a.m.n_rec || halted(a)) return;
   int b = a.m.rec_bond[r];
   if (a.m.rec_release[r]) {
     a.m.lig_owner[b] = -1;
@@ -800,7 +806,7 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
 __global__ __launch_bounds__(256) void spread_kernel(MArgs a) {
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (i >= a.m.n) return;
+  if (i >= a.m.n || halted(a)) return;
   if (!ib_coupled(a.m.kind[i], a.fiber_move)) return;
   const float X = wrap_coord(a.m.x[i], a.nx, a.periodic[0]);
   const float Y = wrap_coord(a.m.y[i], a.ny, a.periodic[1]);
@@ -828,6 +834,7 @@ __global__ __launch_bounds__(256) void spread_kernel(MArgs a) {
 __global__ __launch_bounds__(256) void move_kernel(MArgs a) {
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  if (halted(a)) return;
   if ((a.roles & ROLE_B) && blockIdx.x == 0 && threadIdx.x == 0) *a.m.d_step += 1;
   if (i >= a.m.n) return;
   const int kind = a.m.kind[i];
@@ -934,6 +941,7 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->marker_dt = p.marker_dt;
   a->force_scale = p.ib_force_scale;
   a->roles = 0;
+  a->halt = h->halt_on_detach ? 1 : 0;
   a->blocks0 = a->blocks1 = a->blocks2 = 0;
 }
 

@@ -28,6 +28,8 @@ struct LBArgs {
   const uint8_t *solid;   /* nullptr when there are no solid cells          */
   const uint32_t *bbmask; /* per cell: bit d = upstream of d is solid,
                              bit 31 = cell is solid; nullptr without flags  */
+  const int64_t *halt;    /* nonzero *halt: leave the lattice unchanged
+                             (run halted at detach); nullptr: never          */
   int nx, ny, nz;
   int nxb;                /* x-blocks of XBLOCK cells per row                 */
   int64_t plane;          /* stride between the components of aux            */
@@ -130,6 +132,7 @@ struct pfg_handle {
   pfg::Markers m;
   bool model_loaded = false;
   bool fluid_only = false;
+  bool halt_on_detach = false; /* steps after the detach test fired change nothing */
   int64_t h_step = 0;
   bool use_graph = true;
   /* [starting copy of f][0: one step, 1: GRAPH_CHUNK steps] */

@@ -96,3 +96,43 @@ def test_thrombus_small_flags_fibrin(gpu, po):
           f"{np.abs(pos - opos).max():.2e}")
     assert d.max() < 1e-5
     assert np.abs(pos - opos).max() < 1e-4
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_halt_on_detach_freezes_the_run(gpu, po, graph):
+    """With no ligands no bond can form, so the detach test runs every step
+    (hazard 0.02; with this scene's seed the first draw below it comes at
+    step 14).  A run halted at detach ends in the state of a plain run of
+    exactly the steps that completed, and the detach step is the oracle's
+    (same counter-RNG draws)."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_SHEAR, nx=64, ny=32, nz=32, elements=256, receptors=32,
+                   ligands=-1)
+    sc.params.detach_rate = 0.02
+    with pfg.Solver(sc.params) as s:
+        s.load(sc)
+        s.set_graph(graph)
+        s.set_halt_on_detach(True)
+        info = s.step(30)
+        assert info["detached"] == 1
+        done = info["step"]
+        assert done > 0
+        assert done == info["detach_step"]
+        f_halt = s.f()
+        mk_halt = s.markers()
+        info2 = s.step(5)  # further steps change nothing
+        assert info2["step"] == done
+        assert np.array_equal(s.f(), f_halt)
+        assert np.array_equal(s.markers()["x"], mk_halt["x"])
+    with pfg.Solver(sc.params) as s:
+        s.load(sc)
+        s.step(done)
+        f_plain = s.f()
+        mk_plain = s.markers()
+    o = po.Oracle(sc.params, scene=sc)
+    o.step(done + 1)
+    print(f"halt: detached after {done} steps (oracle {o.counters()['detach_step']})")
+    assert o.counters()["detach_step"] == done
+    assert np.abs(f_halt - f_plain).max() < 1e-6
+    for k in ("x", "y", "z"):
+        assert np.abs(mk_halt[k] - mk_plain[k]).max() < 1e-5
