@@ -100,6 +100,57 @@ def cpu_baseline(pfg, steps=3):
                       f"oracle, 1 thread ({dt:.1f} s)"}
 
 
+WORKLOADS = {
+    "thrombus": ("SCENE_THROMBUS",
+                 "thrombus in a tube: 512x256x256 lattice, solid-flag tube of radius 120 "
+                 "along x, body force, 500 platelets x 1000 elements, fibrin 20k nodes / "
+                 "30k links, FP32 storage"),
+    "large": ("SCENE_LARGE",
+              "large shear: 1024x512x512 lattice, Couette, 2000 platelets x 1000 "
+              "elements, fibrin 80k nodes / 120k links, FP16 storage"),
+}
+
+
+def other_workload(pfg, name, K, W):
+    """Coupled steps and the LB kernel on a full-size configuration other than
+    the headline one (same JSON schema; value = coupled MLUPS over all
+    lattice cells)."""
+    attr, desc = WORKLOADS[name]
+    t0 = time.perf_counter()
+    scene = pfg.Scene(getattr(pfg, attr))
+    log(f"{name}: scene built in {time.perf_counter() - t0:.1f} s: {scene.N} cells, "
+        f"{scene.n_markers} markers")
+    store = scene.params.store
+    s, dt = timed_run(pfg, scene, store, K, W)
+    info = s.step(0)
+    dev_gb = s.device_bytes() / 1e9
+    s.close()
+    fluid = int((scene.flags == 0).sum()) if scene.flags is not None else scene.N
+    bpc = BYTES_PER_CELL[store]
+    lb_us = lb_kernel_us(pfg, scene, store, min(K, 200))
+    # solid cells are skipped by the kernel: algorithmic bytes count fluid cells
+    achieved = bpc * fluid / (lb_us * 1e-6) / 1e9
+    mlups = scene.N * K / dt / 1e6
+    dtype = {0: "f32", 1: "f16", 2: "bf16"}[store]
+    return {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(dt * 1e3 / K, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic (seeded scene)",
+        "config": {"workload": desc, "cells": scene.N, "fluid_cells": fluid,
+                   "markers": scene.n_markers, "store": dtype},
+        "coupled_steps_per_s": round(K / dt, 2),
+        "fluid_mlups": round(fluid * K / dt / 1e6, 1),
+        "device_gb": round(dev_gb, 2),
+        "bonds_at_end": info["total_bonds"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": f"lb_fused_kernel<{dtype.upper()}>",
+                     "bytes_per_cell": bpc, "lb_kernel_us": round(lb_us, 2)},
+        "cpu_baseline": None,
+    }
+
+
 def fp16_accuracy(pfg):
     """Relative L2 velocity error of FP16 deviation storage against the FP64
     oracle: 64x32x32 body-force channel, tau 0.8, F = 1e-6, 300 steps."""
@@ -132,6 +183,8 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--lb-only", choices=sorted(STORES), default=None)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--workload", choices=["shear"] + sorted(WORKLOADS), default="shear",
+                    help="shear (default) is the headline configuration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
@@ -152,6 +205,9 @@ def main():
         return
 
     K, W = args.steps, args.warmup
+    if args.workload != "shear":
+        print(json.dumps(other_workload(pfg, args.workload, K, W)), flush=True)
+        return
     scene = pfg.Scene(pfg.SCENE_SHEAR)
     N = scene.N
 

@@ -289,6 +289,12 @@ pfg_status pfg_set_fluid_only(pfg_handle *h, int fluid_only);
  * neither the lattice, the markers nor the bonds, and pfg_step_info.step
  * reports the steps that ran.  Default 0: stepping continues after detach. */
 pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable);
+/* IB spreading form (LB_kernel.cu:1419-1520 restated two ways): 0 automatic
+ * (LDS-staged from 32768 markers up), 1 always staged (a workgroup per run
+ * of up to 128 markers of one platelet accumulates them in LDS first),
+ * 2 always direct (FP32 atomics per marker and node).  Same result up to
+ * FP32 summation order. */
+pfg_status pfg_set_ib_staging(pfg_handle *h, int mode);
 pfg_status pfg_synchronize(pfg_handle *h);
 
 /* ---- measurement ---- */

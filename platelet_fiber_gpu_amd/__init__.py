@@ -312,6 +312,10 @@ class Solver:
     def set_fluid_only(self, on):
         self._c("pfg_set_fluid_only", int(bool(on)))
 
+    def set_ib_staging(self, mode):
+        """IB spreading form: 0 automatic, 1 LDS-staged chunks, 2 direct."""
+        self._c("pfg_set_ib_staging", int(mode))
+
     def set_halt_on_detach(self, on):
         """End the run at detach: later steps change nothing and step()
         reports the steps that ran."""

@@ -116,6 +116,7 @@ SIGNATURES = {
     "pfg_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_fluid_only": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_halt_on_detach": (C.c_int, [C.c_void_p, C.c_int]),
+    "pfg_set_ib_staging": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_synchronize": (C.c_int, [C.c_void_p]),
     "pfg_profile": (C.c_int, [C.c_void_p, C.c_int, f64p, f64p]),
     "pfg_time_lb": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),

@@ -97,6 +97,7 @@ void free_markers(Markers &m) {
   dfree(m.m2l_start); dfree(m.m2l_nbr); dfree(m.m2l_l0);
   dfree(m.bin_count); dfree(m.bin_start); dfree(m.bin_items);
   dfree(m.marker_bin); dfree(m.marker_slot); dfree(m.scan_partials);
+  dfree(m.chunk_start);
   Markers fresh;
   fresh.counters = m.counters;
   fresh.detach_step = m.detach_step;
@@ -494,6 +495,19 @@ pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
   m.n_lig = m.n_wall_lig + (m.fibrin_lig ? n : 0);
   m.n_links = M.n_links;
 
+  /* ---- IB chunks: runs of consecutive markers of one body; the length
+   * (16..IB_CHUNK, a power of two) aims at ~2048 chunks so that a single
+   * platelet still spreads over many workgroups ---- */
+  int chunk_len = 16;
+  while (chunk_len < IB_CHUNK && (int64_t)chunk_len * 2 * 2048 <= n) chunk_len *= 2;
+  std::vector<int32_t> chunk_start;
+  for (int i = 0; i < n; ++i) {
+    if (chunk_start.empty() || i - chunk_start.back() == chunk_len || M.body[i] != M.body[i - 1])
+      chunk_start.push_back(i);
+  }
+  chunk_start.push_back(n);
+  m.n_chunks = (int)chunk_start.size() - 1;
+
   /* ---- host-side CSR construction ---- */
   std::vector<std::pair<int, int>> kv;
   std::vector<int32_t> m2t_start, m2t, m2e_start, m2e_nbr, m2r_start, m2r, m2l_start,
@@ -644,6 +658,7 @@ pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
   ALLOC_FILL(m.marker_bin, 0, n);
   ALLOC_FILL(m.marker_slot, 0, n);
   ALLOC_FILL(m.scan_partials, 0, m.scan_blocks);
+  ALLOC_UP(m.chunk_start, chunk_start.data(), chunk_start.size());
 #undef ALLOC_UP
 #undef ALLOC_FILL
   if (e != hipSuccess) {
@@ -917,6 +932,13 @@ pfg_status pfg_step(pfg_handle *h, int nsteps, pfg_step_info *info) {
     }
   }
   return fill_info(h, info);
+}
+
+pfg_status pfg_set_ib_staging(pfg_handle *h, int mode) {
+  if (!h || mode < 0 || mode > 2) return PFG_ERR_ARG;
+  h->ib_staging = mode;
+  drop_graph(h);
+  return PFG_OK;
 }
 
 pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable) {

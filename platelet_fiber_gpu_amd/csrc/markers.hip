@@ -16,8 +16,10 @@
  *            receptor bond forces and fibrin links (fiber.cpp:647-733,
  *            network.cpp:303-437), then reduces across the wavefront.
  *  SPREAD    IB force spreading (fluid3d_force_distribute_kernel,
- *            LB_kernel.cu:1419-1520): one lane per (marker, stencil node),
- *            hardware FP32 atomics into the lattice force field, tile flags.
+ *            LB_kernel.cu:1419-1520) into the lattice force field with FP32
+ *            atomics: per (marker, stencil node), or staged through an LDS
+ *            box per chunk of markers when many markers share the lattice;
+ *            tile flags.
  *  INTERP/UPDATE  IB interpolation (fluid3d_velocity_distribute_kernel,
  *            LB_kernel.cu:1524-1642) as a wavefront gather over the 4x4x4
  *            stencil, then x += V dt (sem_update_position_kernel,
@@ -800,13 +802,24 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
   }
 }
 
-/* ---------------- K9: spread ---------------- */
+/* ---------------- K9/K10: spreading, interpolation, update ---------------- */
 
-/* One lane per (marker, stencil node): 64 lanes = one marker's 4x4x4 nodes. */
-__global__ __launch_bounds__(256) void spread_kernel(MArgs a) {
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (i >= a.m.n || halted(a)) return;
+/* Spreading has two forms.  Direct: one wavefront per marker, a lane per
+ * stencil node, FP32 atomics into the lattice force field.  Staged: a
+ * workgroup per chunk (a run of consecutive markers of one platelet)
+ * accumulates the chunk's stencils in an LDS box of lattice nodes and then
+ * adds the box's nonzero cells to the global field; a platelet's stencils
+ * overlap, so this cuts the device-wide atomics several times over when many
+ * markers share the lattice.  A chunk whose box does not fit in LDS (markers
+ * scattered through the lattice) is spread directly. */
+
+__device__ inline void flag_tile(const MArgs &a, int x, int y, int z) {
+  const int tile = (x >> TILE_SHIFT_X) + a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
+  if (a.tile_flag[tile] == 0) a.tile_flag[tile] = 1;
+}
+
+/* Direct spreading of marker i: lane = stencil node. */
+__device__ void spread_direct(const MArgs &a, int i, int lane) {
   if (!ib_coupled(a.m.kind[i], a.fiber_move)) return;
   const float X = wrap_coord(a.m.x[i], a.nx, a.periodic[0]);
   const float Y = wrap_coord(a.m.y[i], a.ny, a.periodic[1]);
@@ -816,10 +829,7 @@ __global__ __launch_bounds__(256) void spread_kernel(MArgs a) {
   const int iy = stencil_node(Y, (lane >> 2) & 3, a.ny, a.periodic[1], &wy);
   const int iz = stencil_node(Z, lane >> 4, a.nz, a.periodic[2], &wz);
   if (ix < 0 || iy < 0 || iz < 0) return;
-  /* flag the tile of this node (idempotent byte store, read first so that a
-   * flagged tile takes no further stores) */
-  const int tile = (ix >> TILE_SHIFT_X) + a.tnx * ((iy >> TILE_SHIFT_Y) + a.tny * (iz >> TILE_SHIFT_Z));
-  if (a.tile_flag[tile] == 0) a.tile_flag[tile] = 1;
+  flag_tile(a, ix, iy, iz);
   const float w = wx * wy * wz * a.force_scale;
   const int64_t c = ix + (int64_t)a.nx * (iy + (int64_t)a.ny * iz);
   atomicAdd(a.aux + c, w * a.m.fx[i]);
@@ -827,7 +837,188 @@ __global__ __launch_bounds__(256) void spread_kernel(MArgs a) {
   atomicAdd(a.aux + 2 * a.plane + c, w * a.m.fz[i]);
 }
 
-/* ---------------- K10: interpolation + update ---------------- */
+/* One wavefront per marker. */
+__global__ __launch_bounds__(256) void spread_kernel(MArgs a) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= a.m.n || halted(a)) return;
+  spread_direct(a, i, threadIdx.x & 63);
+}
+
+struct ChunkBox {
+  int lo[3];  /* first node, in the chunk's unwrapped node space */
+  int dim[3];
+  int vol;    /* 0: no IB-coupled marker in the chunk */
+  bool lds;   /* box fits in LDS */
+};
+
+/* Wrapped coordinate (gives the kernel weights exactly as the direct form)
+ * and the first stencil node in the periodic image nearest to the chunk's
+ * reference point `ref` (the wrapped position of its first marker), so that
+ * one platelet's stencils stay contiguous across a periodic boundary. */
+__device__ inline void stencil_origin(const MArgs &a, int i, const float ref[3], float W[3],
+                                      int o[3]) {
+  const float v[3] = {a.m.x[i], a.m.y[i], a.m.z[i]};
+  const int n[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    W[k] = wrap_coord(v[k], n[k], a.periodic[k]);
+    const int img = a.periodic[k] ? -(int)rintf((W[k] - ref[k]) / (float)n[k]) * n[k] : 0;
+    o[k] = (int)floorf(W[k]) - 1 + img;
+  }
+}
+
+__device__ inline float stencil_weight(float W, int k) {
+  const float r = W - floorf(W) + 1.0f - (float)k; /* W - (floor(W) - 1 + k) */
+  return 0.25f * (1.0f + cospif(0.5f * r));
+}
+
+/* Block-wide box of the chunk [m0, m0 + cnt) (cnt <= 256); every thread
+ * gets the result. */
+__device__ ChunkBox chunk_box(const MArgs &a, int m0, int cnt, const float ref[3], int *sh) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  if (t < cnt && ib_coupled(a.m.kind[m0 + t], a.fiber_move)) {
+    float W[3];
+    int o[3];
+    stencil_origin(a, m0 + t, ref, W, o);
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = o[k];
+      hi[k] = o[k] + 3;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+      lo[k] = min(lo[k], __shfl_xor(lo[k], s, 64));
+      hi[k] = max(hi[k], __shfl_xor(hi[k], s, 64));
+    }
+  }
+  if (lane == 0)
+    for (int k = 0; k < 3; ++k) {
+      sh[6 * wave + k] = lo[k];
+      sh[6 * wave + 3 + k] = hi[k];
+    }
+  __syncthreads();
+  ChunkBox b;
+  int64_t vol = 1;
+  for (int k = 0; k < 3; ++k) {
+    int l = INT_MAX, h = INT_MIN;
+    for (int w = 0; w < 4; ++w) {
+      l = min(l, sh[6 * w + k]);
+      h = max(h, sh[6 * w + 3 + k]);
+    }
+    b.lo[k] = l;
+    b.dim[k] = l <= h ? h - l + 1 : 0;
+    vol *= b.dim[k];
+  }
+  __syncthreads(); /* sh is free again */
+  b.vol = vol > 0 ? (vol > INT_MAX ? INT_MAX : (int)vol) : 0;
+  b.lds = b.vol > 0 && vol <= IB_BOX_CELLS;
+  return b;
+}
+
+/* One workgroup per chunk; dynamic LDS of 3 * IB_BOX_CELLS floats. */
+__global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
+  extern __shared__ float box[];
+  __shared__ int sh[24];
+  if (halted(a)) return;
+  const int m0 = a.m.chunk_start[blockIdx.x], cnt = a.m.chunk_start[blockIdx.x + 1] - m0;
+  float ref[3];
+  ref[0] = wrap_coord(a.m.x[m0], a.nx, a.periodic[0]);
+  ref[1] = wrap_coord(a.m.y[m0], a.ny, a.periodic[1]);
+  ref[2] = wrap_coord(a.m.z[m0], a.nz, a.periodic[2]);
+  const ChunkBox b = chunk_box(a, m0, cnt, ref, sh);
+  if (b.vol == 0) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (!b.lds) {
+    for (int j = wave; j < cnt; j += 4) spread_direct(a, m0 + j, lane);
+    return;
+  }
+  for (int k = threadIdx.x; k < 3 * b.vol; k += 256) box[k] = 0.f;
+  __syncthreads();
+  const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
+  for (int j = wave; j < cnt; j += 4) {
+    const int i = m0 + j;
+    if (!ib_coupled(a.m.kind[i], a.fiber_move)) continue;
+    float W[3];
+    int o[3];
+    stencil_origin(a, i, ref, W, o);
+    const float w = stencil_weight(W[0], kx) * stencil_weight(W[1], ky) *
+                    stencil_weight(W[2], kz) * a.force_scale;
+    const int cell = (o[0] + kx - b.lo[0]) + b.dim[0] * ((o[1] + ky - b.lo[1]) +
+                                                         b.dim[1] * (o[2] + kz - b.lo[2]));
+    atomicAdd(&box[cell], w * a.m.fx[i]);
+    atomicAdd(&box[b.vol + cell], w * a.m.fy[i]);
+    atomicAdd(&box[2 * b.vol + cell], w * a.m.fz[i]);
+  }
+  __syncthreads();
+  /* flush row by row: the (y, z) of a row is wavefront-uniform */
+  const int rows = b.dim[1] * b.dim[2];
+  for (int r = wave; r < rows; r += 4) {
+    const int by = r % b.dim[1], bz = r / b.dim[1];
+    const int gy = axis_index(b.lo[1] + by, a.ny, a.periodic[1]);
+    const int gz = axis_index(b.lo[2] + bz, a.nz, a.periodic[2]);
+    if (gy < 0 || gz < 0) continue;
+    const int64_t row = (int64_t)a.nx * (gy + (int64_t)a.ny * gz);
+    for (int bx = lane; bx < b.dim[0]; bx += 64) {
+      const int gx = axis_index(b.lo[0] + bx, a.nx, a.periodic[0]);
+      if (gx < 0) continue;
+      flag_tile(a, gx, gy, gz);
+      const int k = bx + b.dim[0] * r;
+      const float fx = box[k], fy = box[b.vol + k], fz = box[2 * b.vol + k];
+      const int64_t c = row + gx;
+      if (fx != 0.f) atomicAdd(a.aux + c, fx);
+      if (fy != 0.f) atomicAdd(a.aux + a.plane + c, fy);
+      if (fz != 0.f) atomicAdd(a.aux + 2 * a.plane + c, fz);
+    }
+  }
+}
+
+/* Interpolated velocity of marker i (lane = stencil node, summed over the
+ * wavefront; nodes outside a wall-bounded axis contribute nothing). */
+__device__ void interp_marker(const MArgs &a, int i, int lane, float *v) {
+  const float X = wrap_coord(a.m.x[i], a.nx, a.periodic[0]);
+  const float Y = wrap_coord(a.m.y[i], a.ny, a.periodic[1]);
+  const float Z = wrap_coord(a.m.z[i], a.nz, a.periodic[2]);
+  float wx, wy, wz, ux = 0.f, uy = 0.f, uz = 0.f;
+  const int ix = stencil_node(X, lane & 3, a.nx, a.periodic[0], &wx);
+  const int iy = stencil_node(Y, (lane >> 2) & 3, a.ny, a.periodic[1], &wy);
+  const int iz = stencil_node(Z, lane >> 4, a.nz, a.periodic[2], &wz);
+  if (ix >= 0 && iy >= 0 && iz >= 0) {
+    const float w = wx * wy * wz;
+    const int64_t c = ix + (int64_t)a.nx * (iy + (int64_t)a.ny * iz);
+    ux = w * a.aux[c];
+    uy = w * a.aux[a.plane + c];
+    uz = w * a.aux[2 * a.plane + c];
+  }
+  v[0] = wave_sum(ux);
+  v[1] = wave_sum(uy);
+  v[2] = wave_sum(uz);
+}
+
+/* Marker update by lane 0: x += V dt (sem_kernel.cu:2112-2138), positions
+ * stay unwrapped; fibrin nodes under PFG_FIBER_KICK take the damped kick of
+ * network.cpp:839-881 first. */
+__device__ void update_marker(const MArgs &a, int i, float vx, float vy, float vz) {
+  const int kind = a.m.kind[i];
+  if (kind == PFG_MARKER_FIXED) return;
+  const float dt = a.marker_dt;
+  if (kind == PFG_MARKER_FIBER && a.fiber_move == PFG_FIBER_KICK) {
+    /* v <- e^{-g dt} v + F (1 - e^{-g dt})/g */
+    const float vdt = expf(-a.fiber_gamma * dt);
+    const float fdt = a.fiber_gamma > 0.f ? (1.0f - vdt) / a.fiber_gamma : dt;
+    vx = vdt * a.m.vx[i] + a.m.fx[i] * fdt;
+    vy = vdt * a.m.vy[i] + a.m.fy[i] * fdt;
+    vz = vdt * a.m.vz[i] + a.m.fz[i] * fdt;
+    a.m.vx[i] = vx;
+    a.m.vy[i] = vy;
+    a.m.vz[i] = vz;
+  }
+  a.m.x[i] += vx * dt;
+  a.m.y[i] += vy * dt;
+  a.m.z[i] += vz * dt;
+}
 
 /* roles: A = interpolate V at the markers, B = move the markers and advance
  * the step counter.  One wavefront per marker, a lane per stencil node. */
@@ -838,59 +1029,20 @@ __global__ __launch_bounds__(256) void move_kernel(MArgs a) {
   if ((a.roles & ROLE_B) && blockIdx.x == 0 && threadIdx.x == 0) *a.m.d_step += 1;
   if (i >= a.m.n) return;
   const int kind = a.m.kind[i];
-  const bool coupled = ib_coupled(kind, a.fiber_move);
-  float vx = 0.f, vy = 0.f, vz = 0.f;
+  float v[3] = {0.f, 0.f, 0.f};
   if (a.roles & ROLE_A) {
-    if (coupled) {
-      const float X = wrap_coord(a.m.x[i], a.nx, a.periodic[0]);
-      const float Y = wrap_coord(a.m.y[i], a.ny, a.periodic[1]);
-      const float Z = wrap_coord(a.m.z[i], a.nz, a.periodic[2]);
-      float wx, wy, wz;
-      const int ix = stencil_node(X, lane & 3, a.nx, a.periodic[0], &wx);
-      const int iy = stencil_node(Y, (lane >> 2) & 3, a.ny, a.periodic[1], &wy);
-      const int iz = stencil_node(Z, lane >> 4, a.nz, a.periodic[2], &wz);
-      if (ix >= 0 && iy >= 0 && iz >= 0) {
-        const float w = wx * wy * wz;
-        const int64_t c = ix + (int64_t)a.nx * (iy + (int64_t)a.ny * iz);
-        vx = w * a.aux[c];
-        vy = w * a.aux[a.plane + c];
-        vz = w * a.aux[2 * a.plane + c];
-      }
-      vx = wave_sum(vx);
-      vy = wave_sum(vy);
-      vz = wave_sum(vz);
-      if (lane == 0) {
-        a.m.vx[i] = vx;
-        a.m.vy[i] = vy;
-        a.m.vz[i] = vz;
-      }
-    } else if (kind == PFG_MARKER_FIXED && lane == 0) {
-      a.m.vx[i] = 0.f;
-      a.m.vy[i] = 0.f;
-      a.m.vz[i] = 0.f;
+    if (ib_coupled(kind, a.fiber_move)) interp_marker(a, i, lane, v);
+    if (lane == 0 && (ib_coupled(kind, a.fiber_move) || kind == PFG_MARKER_FIXED)) {
+      a.m.vx[i] = v[0];
+      a.m.vy[i] = v[1];
+      a.m.vz[i] = v[2];
     }
   } else if (lane == 0) {
-    vx = a.m.vx[i];
-    vy = a.m.vy[i];
-    vz = a.m.vz[i];
+    v[0] = a.m.vx[i];
+    v[1] = a.m.vy[i];
+    v[2] = a.m.vz[i];
   }
-  if (!(a.roles & ROLE_B) || lane != 0 || kind == PFG_MARKER_FIXED) return;
-  const float dt = a.marker_dt;
-  if (kind == PFG_MARKER_FIBER && a.fiber_move == PFG_FIBER_KICK) {
-    /* network.cpp:839-881: v <- e^{-g dt} v + F (1 - e^{-g dt})/g */
-    const float vdt = expf(-a.fiber_gamma * dt);
-    const float fdt = a.fiber_gamma > 0.f ? (1.0f - vdt) / a.fiber_gamma : dt;
-    vx = vdt * a.m.vx[i] + a.m.fx[i] * fdt;
-    vy = vdt * a.m.vy[i] + a.m.fy[i] * fdt;
-    vz = vdt * a.m.vz[i] + a.m.fz[i] * fdt;
-    a.m.vx[i] = vx;
-    a.m.vy[i] = vy;
-    a.m.vz[i] = vz;
-  }
-  /* explicit Euler (sem_kernel.cu:2112-2138); positions stay unwrapped */
-  a.m.x[i] += vx * dt;
-  a.m.y[i] += vy * dt;
-  a.m.z[i] += vz * dt;
+  if ((a.roles & ROLE_B) && lane == 0) update_marker(a, i, v[0], v[1], v[2]);
 }
 
 /* ---------------- launchers ---------------- */
@@ -1024,7 +1176,13 @@ hipError_t launch_spread(pfg_handle *h) {
   if (h->m.n == 0) return hipSuccess;
   MArgs a;
   make_margs(h, &a);
-  hipLaunchKernelGGL(spread_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
+  /* staged through LDS when many markers share the lattice */
+  const bool staged = h->ib_staging == 1 || (h->ib_staging == 0 && h->m.n >= IB_STAGED_MIN);
+  if (staged)
+    hipLaunchKernelGGL(spread_chunk_kernel, dim3(h->m.n_chunks), dim3(256), IB_BOX_BYTES,
+                       h->stream, a);
+  else
+    hipLaunchKernelGGL(spread_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
   return hipGetLastError();
 }
 

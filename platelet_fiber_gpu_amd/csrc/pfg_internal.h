@@ -19,6 +19,15 @@ namespace pfg {
  * started from). */
 constexpr int GRAPH_CHUNK = 8;
 
+/* IB work units: at most IB_CHUNK consecutive markers of one body; their
+ * stencils are staged in an LDS box of at most IB_BOX_CELLS lattice nodes
+ * (3 floats each). */
+constexpr int IB_CHUNK = 128;
+constexpr int IB_BOX_CELLS = 4096;
+constexpr size_t IB_BOX_BYTES = 3 * IB_BOX_CELLS * sizeof(float);
+/* Marker count from which spreading is staged through LDS by default. */
+constexpr int IB_STAGED_MIN = 32768;
+
 /* Everything the fused LB kernel needs, passed by value. */
 struct LBArgs {
   const void *fin;        /* post-collision state of the previous step      */
@@ -100,6 +109,10 @@ struct Markers {
           *marker_bin = nullptr, *marker_slot = nullptr;
   int32_t *scan_partials = nullptr;
   int64_t scan_blocks = 0;
+  /* IB chunks: chunk_start[n_chunks+1], runs of <= IB_CHUNK markers of one
+   * body (platelet, or the markers outside platelets) */
+  int n_chunks = 0;
+  int32_t *chunk_start = nullptr;
   /* counters: [0] total bonds, [1] detached, [2] near-threshold lo, [3] hi,
    * [4] events lo, [5] events hi */
   int64_t *counters = nullptr;
@@ -133,6 +146,7 @@ struct pfg_handle {
   bool model_loaded = false;
   bool fluid_only = false;
   bool halt_on_detach = false; /* steps after the detach test fired change nothing */
+  int ib_staging = 0;          /* spreading: 0 auto, 1 LDS-staged chunks, 2 direct */
   int64_t h_step = 0;
   bool use_graph = true;
   /* [starting copy of f][0: one step, 1: GRAPH_CHUNK steps] */

@@ -7,10 +7,11 @@ pytestmark = pytest.mark.gpu
 Q = 19
 
 
-def run_both(pfg, po, sc, steps, graph=True):
+def run_both(pfg, po, sc, steps, graph=True, staging=0):
     with pfg.Solver(sc.params) as s:
         s.load(sc)
         s.set_graph(graph)
+        s.set_ib_staging(staging)
         info = s.step(steps)
         f = s.f().astype(np.float64)
         mk = s.markers()
@@ -82,11 +83,15 @@ def test_shear_with_bonds_short(gpu, po):
         assert np.abs(pos - opos).max() < 1e-4
 
 
-def test_thrombus_small_flags_fibrin(gpu, po):
+@pytest.mark.parametrize("staging", [1, 2])
+def test_thrombus_small_flags_fibrin(gpu, po, staging):
+    """Tube flags, three platelets, fibrin nodes spread through the lattice
+    (their chunks' boxes exceed LDS and fall back to direct spreading), with
+    both spreading forms."""
     pfg = gpu
     sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=64, ny=48, nz=48, n_platelets=3, elements=150,
                    receptors=16, ligands=400, fibrin_nodes=500, fibrin_links=800)
-    g, o = run_both(pfg, po, sc, 15)
+    g, o = run_both(pfg, po, sc, 15, staging=staging)
     fluid = sc.flags == 0
     w = np.array([po.load().orc_w(d) for d in range(Q)])[:, None]
     d = np.abs((g["f"] - w) - (o.f() - w))[:, fluid]

@@ -104,7 +104,11 @@ def test_forces(gpu, po, what):
     assert np.abs(F - oF).max() < 2e-5 * scale, (np.abs(F - oF).max(), scale)
 
 
-def test_spread_interp_parity(gpu, po):
+@pytest.mark.parametrize("staging", [1, 2])
+def test_spread_interp_parity(gpu, po, staging):
+    """Both spreading forms (1: LDS-staged chunks, 2: direct atomics) against
+    the oracle's stencil form; a seventh of the markers sit one periodic
+    image away, so staged chunks straddle the periodic boundary."""
     pfg = gpu
     sc = scene_small(pfg, receptors=-1, ligands=-1)
     rng = np.random.default_rng(5)
@@ -112,6 +116,7 @@ def test_spread_interp_parity(gpu, po):
     f = (1e-3 * rng.standard_normal((3, n))).astype(np.float32)
     s, o = pair(pfg, po, sc)
     with s:
+        s.set_ib_staging(staging)
         x, y, z = perturbed(sc, 0.1)
         x[::7] += sc.params.nx  # wrap on the periodic axis
         set_pos(s, o, x, y, z)
