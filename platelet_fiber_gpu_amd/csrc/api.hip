@@ -97,7 +97,7 @@ void free_markers(Markers &m) {
   dfree(m.m2l_start); dfree(m.m2l_nbr); dfree(m.m2l_l0);
   dfree(m.bin_count); dfree(m.bin_start); dfree(m.bin_items);
   dfree(m.marker_bin); dfree(m.marker_slot); dfree(m.scan_partials);
-  dfree(m.chunk_start);
+  dfree(m.chunk_start); dfree(m.bin_scatter); dfree(m.sorted_pos);
   Markers fresh;
   fresh.counters = m.counters;
   fresh.detach_step = m.detach_step;
@@ -659,6 +659,8 @@ pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
   ALLOC_FILL(m.marker_slot, 0, n);
   ALLOC_FILL(m.scan_partials, 0, m.scan_blocks);
   ALLOC_UP(m.chunk_start, chunk_start.data(), chunk_start.size());
+  ALLOC_FILL(m.bin_scatter, 0, n);
+  ALLOC_FILL(m.sorted_pos, make_float4(0.f, 0.f, 0.f, 0.f), n);
 #undef ALLOC_UP
 #undef ALLOC_FILL
   if (e != hipSuccess) {

@@ -349,7 +349,7 @@ __global__ __launch_bounds__(256) void place_kernel(MArgs a) {
       const int i = blk * 256 + threadIdx.x;
       if (i < a.m.n) {
         const int b = a.m.marker_bin[i];
-        a.m.bin_items[a.m.bin_start[b] + a.m.marker_slot[i]] = i;
+        a.m.bin_scatter[a.m.bin_start[b] + a.m.marker_slot[i]] = i;
       }
       return;
     }
@@ -485,7 +485,7 @@ __device__ void bond_decide(const MArgs &a, int r) {
     }
     if (a.m.fibrin_lig) {
       for (int k = a.m.bin_start[bin]; k < a.m.bin_start[bin + 1]; ++k) {
-        const int mk = a.m.bin_items[k];
+        const int mk = a.m.bin_scatter[k]; /* order-free: ties go to the lower id */
         if (a.m.kind[mk] == PFG_MARKER_ELEMENT) continue;
         const int l = a.m.n_wall_lig + mk;
         float d[3];
@@ -525,9 +525,17 @@ __device__ void bond_decide(const MArgs &a, int r) {
   }
 }
 
-/* roles: A = sort bins (blocks0, one lane per marker: slot-0 markers sort
- * their bin), B = platelet sums (blocks1 = one block per platelet),
- * C = bond decisions (blocks2, four receptors per block) */
+/* Tag of a marker in the bin-sorted packed positions: its platelet for an
+ * element, -1 for a fibrin node (Morse acts between elements of one body). */
+__device__ inline float morse_tag(const MArgs &a, int j) {
+  return __int_as_float(a.m.kind[j] == PFG_MARKER_ELEMENT ? a.m.body[j] : -1);
+}
+
+/* roles: A = sort bins (blocks0, one lane per marker: the slot-0 marker of a
+ * bin sorts it from the scatter order into bin_items and writes the packed
+ * positions), B = platelet sums (blocks1 = one block per platelet),
+ * C = bond decisions (blocks2, four receptors per block; they read the
+ * scatter order, which no role of this launch writes) */
 __global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
   int blk = blockIdx.x;
   if (a.roles & ROLE_A) {
@@ -537,14 +545,18 @@ __global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
         const int b = a.m.marker_bin[i];
         const int s = a.m.bin_start[b], e = a.m.bin_start[b + 1];
         int32_t *it = a.m.bin_items;
-        for (int k = s + 1; k < e; ++k) {
-          const int v = it[k];
+        for (int k = s; k < e; ++k) { /* insertion sort by marker index */
+          const int v = a.m.bin_scatter[k];
           int j = k - 1;
           while (j >= s && it[j] > v) {
             it[j + 1] = it[j];
             --j;
           }
           it[j + 1] = v;
+        }
+        for (int k = s; k < e; ++k) {
+          const int j = it[k];
+          a.m.sorted_pos[k] = make_float4(a.m.x[j], a.m.y[j], a.m.z[j], morse_tag(a, j));
         }
       }
       return;
@@ -625,13 +637,17 @@ __global__ void bond_resolve_kernel(MArgs a) {
 
 /* ---------------- K8: forces ---------------- */
 
-__device__ inline void morse_bin(const MArgs &a, int bin, int i, int body, float xi, float yi,
+/* Morse pairs of element i with the elements of its body in one bin, read
+ * from the bin-sorted packed positions (one 16-byte load per candidate; the
+ * pair with itself is the r = 0 one and is skipped). */
+__device__ inline void morse_bin(const MArgs &a, int bin, int body, float xi, float yi,
                                  float zi, float &fx, float &fy, float &fz) {
   const float two_rho_u0 = 2.0f * a.morse_rho * a.morse_u0;
+  const float tag = __int_as_float(body);
   for (int k = a.m.bin_start[bin]; k < a.m.bin_start[bin + 1]; ++k) {
-    const int j = a.m.bin_items[k];
-    if (j == i || a.m.body[j] != body || a.m.kind[j] != PFG_MARKER_ELEMENT) continue;
-    const float dx = a.m.x[j] - xi, dy = a.m.y[j] - yi, dz = a.m.z[j] - zi;
+    const float4 p = a.m.sorted_pos[k];
+    if (__float_as_int(p.w) != __float_as_int(tag)) continue;
+    const float dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
     const float r2 = dx * dx + dy * dy + dz * dz;
     if (r2 >= a.morse_cut2 || r2 == 0.f) continue;
     const float r = sqrtf(r2);
@@ -721,7 +737,7 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
       const int b = a.m.marker_bin[i];
       const int b3[3] = {b % a.nb[0], (b / a.nb[0]) % a.nb[1], b / (a.nb[0] * a.nb[1])};
       const int bin = nth_nbr_bin(a, b3, lane);
-      if (bin >= 0) morse_bin(a, bin, i, body, xi, yi, zi, fx, fy, fz);
+      if (bin >= 0) morse_bin(a, bin, body, xi, yi, zi, fx, fy, fz);
     }
     const int t0 = a.m.m2t_start[i], nt = (a.ks != 0.f || a.kv != 0.f) ? a.m.m2t_start[i + 1] - t0 : 0;
     const int e0 = a.m.m2e_start[i], ne = a.k_spring != 0.f ? a.m.m2e_start[i + 1] - e0 : 0;

@@ -107,6 +107,8 @@ struct Markers {
   int64_t nbins = 0;
   int32_t *bin_count = nullptr, *bin_start = nullptr, *bin_items = nullptr,
           *marker_bin = nullptr, *marker_slot = nullptr;
+  int32_t *bin_scatter = nullptr; /* items in scatter (atomic slot) order    */
+  float4 *sorted_pos = nullptr;   /* per bin_items slot: x, y, z, Morse tag  */
   int32_t *scan_partials = nullptr;
   int64_t scan_blocks = 0;
   /* IB chunks: chunk_start[n_chunks+1], runs of <= IB_CHUNK markers of one
