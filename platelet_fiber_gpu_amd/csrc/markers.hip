@@ -888,8 +888,8 @@ __device__ inline float stencil_weight(float W, int k) {
   return 0.25f * (1.0f + cospif(0.5f * r));
 }
 
-/* Block-wide box of the chunk [m0, m0 + cnt) (cnt <= 256); every thread
- * gets the result. */
+/* Block-wide box of the chunk's markers [m0, m0 + cnt) (cnt <= 256); every
+ * thread gets the result. */
 __device__ ChunkBox chunk_box(const MArgs &a, int m0, int cnt, const float ref[3], int *sh) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};

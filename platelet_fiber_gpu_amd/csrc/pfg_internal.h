@@ -23,7 +23,7 @@ constexpr int GRAPH_CHUNK = 8;
  * stencils are staged in an LDS box of at most IB_BOX_CELLS lattice nodes
  * (3 floats each). */
 constexpr int IB_CHUNK = 128;
-constexpr int IB_BOX_CELLS = 4096;
+constexpr int IB_BOX_CELLS = 2048;
 constexpr size_t IB_BOX_BYTES = 3 * IB_BOX_CELLS * sizeof(float);
 /* Marker count from which spreading is staged through LDS by default. */
 constexpr int IB_STAGED_MIN = 32768;
@@ -111,8 +111,9 @@ struct Markers {
   float4 *sorted_pos = nullptr;   /* per bin_items slot: x, y, z, Morse tag  */
   int32_t *scan_partials = nullptr;
   int64_t scan_blocks = 0;
-  /* IB chunks: chunk_start[n_chunks+1], runs of <= IB_CHUNK markers of one
-   * body (platelet, or the markers outside platelets) */
+  /* IB chunks: chunk k holds the markers chunk_start[k] .. chunk_start[k+1]-1,
+   * at most IB_CHUNK consecutive markers of one body (a platelet, or the
+   * markers outside platelets) */
   int n_chunks = 0;
   int32_t *chunk_start = nullptr;
   /* counters: [0] total bonds, [1] detached, [2] near-threshold lo, [3] hi,
