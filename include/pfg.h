@@ -297,6 +297,36 @@ pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable);
 pfg_status pfg_set_ib_staging(pfg_handle *h, int mode);
 pfg_status pfg_synchronize(pfg_handle *h);
 
+/* ---- batches: many lattices stepped together ----
+ * The reference runs one lattice per process (fluid_allocGPUKernel,
+ * semLB.cu:111, a run per seed in gpuSEM_LB.c:1614); a batch holds
+ * n_lattices independent lattices (vessel segments) of their own size, walls,
+ * wall velocities, tau and body force, with their markers, and advances all
+ * of them with one set of launches per step (pfg_step, one hipGraph).
+ * Storage format, marker mechanics, bond and fibrin parameters come from
+ * params[0].  Lattice-state calls of a single-lattice handle
+ * (pfg_init_equilibrium, pfg_upload_f, pfg_download_f, pfg_upload_flags,
+ * pfg_download_fields, pfg_conservation, pfg_load_model) return
+ * PFG_ERR_STATE on a batch; use the pfg_batch_* forms with a lattice index.
+ * Marker arrays of the other calls cover all lattices, lattice after lattice.
+ * Step counters (pfg_step_info) and the detach test are batch-wide, and the
+ * counter RNG is keyed by the batch-wide receptor index. */
+pfg_status pfg_batch_create(const pfg_params *params, int n_lattices, int device,
+                            pfg_handle **out);
+pfg_status pfg_batch_size(const pfg_handle *h, int *n_lattices);
+pfg_status pfg_batch_init_equilibrium(pfg_handle *h, int k, const float *rho,
+                                      const float *ux, const float *uy,
+                                      const float *uz);
+pfg_status pfg_batch_upload_flags(pfg_handle *h, int k, const uint8_t *flags);
+/* models[0..n_lattices-1], positions in each lattice's own coordinates */
+pfg_status pfg_batch_load_models(pfg_handle *h, const pfg_model *models);
+pfg_status pfg_batch_download_fields(pfg_handle *h, int k, float *rho, float *ux,
+                                     float *uy, float *uz);
+pfg_status pfg_batch_download_markers(pfg_handle *h, int k, float *x, float *y,
+                                      float *z, float *vx, float *vy, float *vz,
+                                      float *fx, float *fy, float *fz);
+pfg_status pfg_batch_conservation(pfg_handle *h, int k, double out[4]);
+
 /* ---- measurement ---- */
 /* Time nsteps steps with HIP events on the handle's stream.  ms_total: the
  * whole steps; ms_lb: sum of the LB-kernel launches (events around each LB

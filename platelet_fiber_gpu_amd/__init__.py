@@ -22,7 +22,7 @@ from ._ffi import (FIBER_IB, FIBER_KICK, FIBER_LINEAR, FIBER_WLC, MARKER_ELEMENT
                    WALL_PERIODIC, Model, Params, PfgError, SceneOpts, StepInfo)
 
 __all__ = [
-    "Scene", "Solver", "Params", "Model", "StepInfo", "PfgError", "lib", "default_params",
+    "Scene", "Solver", "Batch", "Params", "Model", "StepInfo", "PfgError", "lib", "default_params",
     "device_count", "STORE_F32", "STORE_F16", "STORE_BF16", "WALL_PERIODIC", "WALL_BOUNCE",
     "MARKER_ELEMENT", "MARKER_FIBER", "MARKER_FIXED", "FIBER_WLC", "FIBER_LINEAR",
     "FIBER_IB", "FIBER_KICK", "SCENE_CHANNEL", "SCENE_SHEAR", "SCENE_THROMBUS",
@@ -343,3 +343,62 @@ class Solver:
         b = C.c_size_t()
         self._c("pfg_device_bytes", C.byref(b))
         return b.value
+
+
+class Batch(Solver):
+    """Many lattices stepped together (pfg_batch_*): lattice k has its own
+    size, walls, tau and body force; storage and marker parameters come from
+    the first.  step(), set_graph(), markers() etc. act on the whole batch."""
+
+    def __init__(self, params_list, device=0):
+        self._lib = lib()
+        self.params_list = list(params_list)
+        self.params = self.params_list[0]
+        arr = (Params * len(self.params_list))(*self.params_list)
+        h = C.c_void_p()
+        _ffi.check("pfg_batch_create", self._lib.pfg_batch_create(
+            arr, len(self.params_list), device, C.byref(h)))
+        self._h = h
+        self.K = len(self.params_list)
+        self.Ns = [p.nx * p.ny * p.nz for p in self.params_list]
+        self.N = sum(self.Ns)
+        self.n_markers = 0
+        self.n_markers_k = [0] * self.K
+        self.n_rec = self.n_lig = self.n_tri = self.n_plt = 0
+
+    def load_scenes(self, scenes):
+        """Models, flags and initial flow of one seeded scene per lattice."""
+        models = (Model * self.K)(*[s.model() for s in scenes])
+        self._c("pfg_batch_load_models", models)
+        self.n_markers_k = [m.n_markers for m in models]
+        self.n_markers = sum(self.n_markers_k)
+        self.n_rec = sum(m.n_receptors for m in models)
+        self.n_tri = sum(m.n_tri for m in models)
+        self.n_plt = sum(m.n_platelets for m in models)
+        for k, s in enumerate(scenes):
+            if s.flags is not None and self.params_list[k].use_flags:
+                a = np.ascontiguousarray(s.flags, dtype=np.uint8).ravel()
+                self._c("pfg_batch_upload_flags", int(k), _ptr(a, C.c_uint8))
+            arrs = [None] + [f32(v).ravel() for v in s.u0]
+            self._c("pfg_batch_init_equilibrium", int(k), *[_ptr(a, C.c_float) for a in arrs])
+
+    def init_lattice(self, k, rho=None, ux=None, uy=None, uz=None):
+        arrs = [None if a is None else f32(a).ravel() for a in (rho, ux, uy, uz)]
+        self._c("pfg_batch_init_equilibrium", int(k), *[_ptr(a, C.c_float) for a in arrs])
+
+    def fields_of(self, k):
+        out = [np.empty(self.Ns[k], np.float32) for _ in range(4)]
+        self._c("pfg_batch_download_fields", int(k), *[_ptr(a, C.c_float) for a in out])
+        return out
+
+    def conservation_of(self, k):
+        out = np.zeros(4, np.float64)
+        self._c("pfg_batch_conservation", int(k), _ptr(out, C.c_double))
+        return out
+
+    def markers_of(self, k):
+        n = self.n_markers_k[k]
+        names = ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz")
+        out = {name: np.empty(n, np.float32) for name in names}
+        self._c("pfg_batch_download_markers", int(k), *[_ptr(out[nm], C.c_float) for nm in names])
+        return out

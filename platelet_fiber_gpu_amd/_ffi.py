@@ -117,6 +117,15 @@ SIGNATURES = {
     "pfg_set_fluid_only": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_halt_on_detach": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_ib_staging": (C.c_int, [C.c_void_p, C.c_int]),
+    "pfg_batch_create": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int,
+                                   C.POINTER(C.c_void_p)]),
+    "pfg_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "pfg_batch_init_equilibrium": (C.c_int, [C.c_void_p, C.c_int, f32p, f32p, f32p, f32p]),
+    "pfg_batch_upload_flags": (C.c_int, [C.c_void_p, C.c_int, u8p]),
+    "pfg_batch_load_models": (C.c_int, [C.c_void_p, C.POINTER(Model)]),
+    "pfg_batch_download_fields": (C.c_int, [C.c_void_p, C.c_int, f32p, f32p, f32p, f32p]),
+    "pfg_batch_download_markers": (C.c_int, [C.c_void_p, C.c_int] + [f32p] * 9),
+    "pfg_batch_conservation": (C.c_int, [C.c_void_p, C.c_int, f64p]),
     "pfg_synchronize": (C.c_int, [C.c_void_p]),
     "pfg_profile": (C.c_int, [C.c_void_p, C.c_int, f64p, f64p]),
     "pfg_time_lb": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),

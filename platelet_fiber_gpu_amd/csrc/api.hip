@@ -78,6 +78,7 @@ hipError_t fill(pfg_handle *h, T *dst, T value, size_t count) {
 }
 
 void free_markers(Markers &m) {
+  dfree(m.minst);
   dfree(m.x); dfree(m.y); dfree(m.z);
   dfree(m.vx); dfree(m.vy); dfree(m.vz);
   dfree(m.fx); dfree(m.fy); dfree(m.fz);
@@ -227,18 +228,47 @@ pfg_status pfg_params_from_sem(const float *s, float dx, float dt, int lx, int l
   return PFG_OK;
 }
 
-pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
-  if (!pin || !out) return PFG_ERR_ARG;
-  *out = nullptr;
-  const pfg_params &p = *pin;
+}  /* extern "C" */
+
+namespace {
+
+pfg_status check_lattice_params(const pfg_params &p) {
   if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0) return PFG_ERR_ARG;
   if ((int64_t)p.nx * p.ny * p.nz > (int64_t)INT_MAX) return PFG_ERR_ARG;
-  if (p.store < PFG_STORE_F32 || p.store > PFG_STORE_BF16) return PFG_ERR_ARG;
   if (!(p.tau > 0.5f)) return PFG_ERR_ARG;
   for (int k = 0; k < 3; ++k)
     if (p.wall[k] != PFG_WALL_PERIODIC && p.wall[k] != PFG_WALL_BOUNCE) return PFG_ERR_ARG;
+  return PFG_OK;
+}
+
+/* Batched LB arguments (both starting copies) and the workgroup table. */
+hipError_t upload_batch_args(pfg_handle *h) {
+  if (h->K <= 1) return hipSuccess;
+  std::vector<LBArgs> args((size_t)h->K);
+  for (int cur = 0; cur < 2; ++cur) {
+    for (int k = 0; k < h->K; ++k) make_lb_args(h, k, cur, &args[(size_t)k]);
+    hipError_t e = upload(h, h->d_lbargs[cur], args.data(), args.size());
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+/* One or K lattices; shared fields (storage, markers, bonds, fibrin) come
+ * from ps[0], lattice and fluid fields from each ps[k]. */
+pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out) {
+  if (!ps || !out || K < 1) return PFG_ERR_ARG;
+  *out = nullptr;
+  const pfg_params &p = ps[0];
+  if (p.store < PFG_STORE_F32 || p.store > PFG_STORE_BF16) return PFG_ERR_ARG;
   if (p.bin_cells <= 0) return PFG_ERR_ARG;
   if (p.f16_scale_log2 < 0 || p.f16_scale_log2 > 20) return PFG_ERR_ARG;
+  int64_t ncells = 0;
+  for (int k = 0; k < K; ++k) {
+    const pfg_status st = check_lattice_params(ps[k]);
+    if (st != PFG_OK) return st;
+    ncells += (int64_t)ps[k].nx * ps[k].ny * ps[k].nz;
+  }
+  if (ncells > (int64_t)INT_MAX) return PFG_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PFG_ERR_DEVICE;
   if (device < 0 || device >= ndev) return PFG_ERR_DEVICE;
@@ -248,19 +278,48 @@ pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
   if (!h) return PFG_ERR_ALLOC;
   h->p = p;
   h->device = device;
+  h->K = K;
+  h->esize = p.store == PFG_STORE_F32 ? 4 : 2;
+  bool flags = false;
+  /* lattice k occupies cells [cell0, cell0 + N), x-blocks [fblk0, ...),
+   * tiles [tile0, ...) and LB workgroups [lb_block0, ...) */
+  h->inst.resize((size_t)K);
+  int64_t cell = 0, fblk = 0, tile = 0, blk = 0;
+  for (int k = 0; k < K; ++k) {
+    Inst &I = h->inst[(size_t)k];
+    I.p = ps[k];
+    I.p.store = p.store;
+    I.p.f16_scale_log2 = p.f16_scale_log2;
+    I.N = (int64_t)I.p.nx * I.p.ny * I.p.nz;
+    I.cell0 = cell;
+    I.nxb = (I.p.nx + XBLOCK - 1) / XBLOCK;
+    I.fblk0 = fblk;
+    I.tnx = (I.p.nx + TILE_X - 1) / TILE_X;
+    I.tny = (I.p.ny + TILE_Y - 1) / TILE_Y;
+    I.tnz = (I.p.nz + TILE_Z - 1) / TILE_Z;
+    I.ntiles = (int64_t)I.tnx * I.tny * I.tnz;
+    I.tile0 = tile;
+    I.lb_blocks = ((int64_t)I.nxb * I.p.ny * I.p.nz + 3) / 4;
+    I.lb_block0 = blk;
+    cell += I.N;
+    fblk += (int64_t)I.nxb * I.p.ny * I.p.nz;
+    tile += I.ntiles;
+    blk += I.lb_blocks;
+    flags = flags || I.p.use_flags;
+  }
+  h->p.use_flags = flags ? 1 : 0;
   pfg_status st = PFG_OK;
   do {
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
       st = PFG_ERR_DEVICE;
       break;
     }
-    h->N = (int64_t)p.nx * p.ny * p.nz;
+    h->N = cell;
     h->plane = (h->N + 255) / 256 * 256;
-    h->esize = p.store == PFG_STORE_F32 ? 4 : 2;
-    h->nxb = (p.nx + XBLOCK - 1) / XBLOCK;
-    const size_t fbytes = (size_t)h->esize * Q * XBLOCK * h->nxb * p.ny * p.nz;
-    for (int k = 0; k < 2; ++k) {
-      if (hipMalloc(&h->f[k], fbytes) != hipSuccess) {
+    h->nxb = h->inst[0].nxb;
+    const size_t fbytes = (size_t)h->esize * Q * XBLOCK * (size_t)fblk;
+    for (int c = 0; c < 2; ++c) {
+      if (hipMalloc(&h->f[c], fbytes) != hipSuccess) {
         st = PFG_ERR_ALLOC;
         break;
       }
@@ -269,14 +328,16 @@ pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
     if (st != PFG_OK) break;
     if (dalloc(h, &h->aux, (size_t)3 * h->plane) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->aux, 0, sizeof(float) * 3 * h->plane) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
-    h->tnx = (p.nx + TILE_X - 1) / TILE_X;
-    h->tny = (p.ny + TILE_Y - 1) / TILE_Y;
-    h->tnz = (p.nz + TILE_Z - 1) / TILE_Z;
-    h->ntiles = (int64_t)h->tnx * h->tny * h->tnz;
+    h->tnx = h->inst[0].tnx;
+    h->tny = h->inst[0].tny;
+    h->tnz = h->inst[0].tnz;
+    h->ntiles = tile;
     const size_t flag_bytes = (size_t)(h->ntiles + 3) / 4 * 4;
     if (dalloc(h, &h->tile_flag, flag_bytes) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->tile_flag, 0, flag_bytes) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
-    h->red_blocks = std::min<int64_t>(2048, (h->N + 255) / 256);
+    int64_t nmax = 0;
+    for (const Inst &I : h->inst) nmax = std::max(nmax, I.N);
+    h->red_blocks = std::min<int64_t>(2048, (nmax + 255) / 256);
     if (dalloc(h, &h->red, (size_t)4 * h->red_blocks) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     /* z-cells per lane of the LB kernel */
     h->lb_cpt = 1;
@@ -290,18 +351,38 @@ pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
       st = PFG_ERR_DEVICE;
       break;
     }
-    if (p.use_flags) {
+    if (flags) {
       if (dalloc(h, &h->solid, (size_t)h->N) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
       if (hipMemset(h->solid, 0, (size_t)h->N) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
       if (dalloc(h, &h->bbmask, (size_t)h->N) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
       if (hipMemset(h->bbmask, 0, sizeof(uint32_t) * h->N) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     }
-    /* f = equilibrium at rest until the caller says otherwise */
-    if (launch_init_eq(h, nullptr, nullptr, nullptr, nullptr) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess) {
-      st = PFG_ERR_LAUNCH;
-      break;
+    if (K > 1) {
+      h->lb_blocks = blk;
+      std::vector<int32_t> table((size_t)blk);
+      std::vector<int64_t> tile0((size_t)K + 1);
+      for (int k = 0; k < K; ++k) {
+        const Inst &I = h->inst[(size_t)k];
+        for (int64_t b = 0; b < I.lb_blocks; ++b) table[(size_t)(I.lb_block0 + b)] = k;
+        tile0[(size_t)k] = I.tile0;
+      }
+      tile0[(size_t)K] = tile;
+      hipError_t e = dalloc(h, &h->lb_block_inst, table.size());
+      if (e == hipSuccess) e = upload(h, h->lb_block_inst, table.data(), table.size());
+      if (e == hipSuccess) e = dalloc(h, &h->d_tile0, tile0.size());
+      if (e == hipSuccess) e = upload(h, h->d_tile0, tile0.data(), tile0.size());
+      for (int c = 0; c < 2 && e == hipSuccess; ++c) e = dalloc(h, &h->d_lbargs[c], (size_t)K);
+      if (e == hipSuccess) e = dalloc(h, &h->d_lats, (size_t)K);
+      if (e == hipSuccess) e = upload_batch_args(h);
+      if (e != hipSuccess) {
+        st = from_hip(e);
+        break;
+      }
     }
+    /* f = equilibrium at rest until the caller says otherwise */
+    for (int k = 0; k < K && st == PFG_OK; ++k)
+      if (launch_init_eq(h, k, nullptr, nullptr, nullptr, nullptr) != hipSuccess) st = PFG_ERR_LAUNCH;
+    if (st == PFG_OK && hipStreamSynchronize(h->stream) != hipSuccess) st = PFG_ERR_LAUNCH;
   } while (0);
   if (st != PFG_OK) {
     pfg_destroy(h);
@@ -309,6 +390,71 @@ pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
   }
   *out = h;
   return PFG_OK;
+}
+
+pfg_status init_eq_impl(pfg_handle *h, int k, const float *rho, const float *ux,
+                        const float *uy, const float *uz) {
+  hipSetDevice(h->device);
+  const int64_t n = h->inst[(size_t)k].N;
+  const float *src[4] = {rho, ux, uy, uz};
+  float *dev[4] = {nullptr, nullptr, nullptr, nullptr};
+  pfg_status st = PFG_OK;
+  for (int c = 0; c < 4 && st == PFG_OK; ++c) {
+    if (!src[c]) continue;
+    if (hipMalloc(&dev[c], sizeof(float) * n) != hipSuccess) {
+      st = PFG_ERR_ALLOC;
+      break;
+    }
+    st = from_hip(upload(h, dev[c], src[c], (size_t)n));
+  }
+  if (st == PFG_OK) {
+    st = from_hip(launch_init_eq(h, k, dev[0], dev[1], dev[2], dev[3]));
+    if (st == PFG_OK) st = from_hip(hipStreamSynchronize(h->stream));
+  }
+  for (int c = 0; c < 4; ++c)
+    if (dev[c]) hipFree(dev[c]);
+  return st;
+}
+
+pfg_status upload_flags_impl(pfg_handle *h, int k, const uint8_t *flags) {
+  if (!h->solid || !h->inst[(size_t)k].p.use_flags) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  const Inst &I = h->inst[(size_t)k];
+  PFG_TRY(upload(h, h->solid + I.cell0, flags, (size_t)I.N));
+  drop_graph(h);
+  PFG_TRY(launch_bbmask(h, k));
+  PFG_TRY(hipStreamSynchronize(h->stream));
+  return PFG_OK;
+}
+
+pfg_status fields_impl(pfg_handle *h, int k, float *rho, float *ux, float *uy, float *uz) {
+  hipSetDevice(h->device);
+  const int64_t n = h->inst[(size_t)k].N;
+  float *dev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float *host[4] = {rho, ux, uy, uz};
+  pfg_status st = PFG_OK;
+  for (int c = 0; c < 4; ++c) {
+    if (!host[c]) continue;
+    if (hipMalloc(&dev[c], sizeof(float) * n) != hipSuccess) {
+      st = PFG_ERR_ALLOC;
+      break;
+    }
+    hipMemsetAsync(dev[c], 0, sizeof(float) * n, h->stream);
+  }
+  if (st == PFG_OK) st = from_hip(launch_lb_fields(h, k, dev[0], dev[1], dev[2], dev[3]));
+  for (int c = 0; c < 4 && st == PFG_OK; ++c)
+    if (host[c]) st = from_hip(download(h, host[c], dev[c], (size_t)n));
+  for (int c = 0; c < 4; ++c)
+    if (dev[c]) hipFree(dev[c]);
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+pfg_status pfg_create(const pfg_params *pin, int device, pfg_handle **out) {
+  return create_impl(pin, 1, device, out);
 }
 
 pfg_status pfg_destroy(pfg_handle *h) {
@@ -323,12 +469,16 @@ pfg_status pfg_destroy(pfg_handle *h) {
   for (int k = 0; k < 2; ++k) {
     if (h->f[k]) hipFree(h->f[k]);
     h->f[k] = nullptr;
+    dfree(h->d_lbargs[k]);
   }
   dfree(h->aux);
   dfree(h->tile_flag);
   dfree(h->solid);
   dfree(h->bbmask);
   dfree(h->red);
+  dfree(h->lb_block_inst);
+  dfree(h->d_lats);
+  dfree(h->d_tile0);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return PFG_OK;
@@ -346,90 +496,60 @@ pfg_status pfg_device_bytes(const pfg_handle *h, size_t *bytes) {
   return PFG_OK;
 }
 
-/* ---------------- lattice state ---------------- */
+/* ---------------- lattice state (single-lattice handles) ---------------- */
 
 pfg_status pfg_init_equilibrium(pfg_handle *h, const float *rho, const float *ux,
                                 const float *uy, const float *uz) {
   if (!h) return PFG_ERR_ARG;
-  hipSetDevice(h->device);
-  const float *src[4] = {rho, ux, uy, uz};
-  float *dev[4] = {nullptr, nullptr, nullptr, nullptr};
-  pfg_status st = PFG_OK;
-  for (int k = 0; k < 4 && st == PFG_OK; ++k) {
-    if (!src[k]) continue;
-    if (hipMalloc(&dev[k], sizeof(float) * h->N) != hipSuccess) {
-      st = PFG_ERR_ALLOC;
-      break;
-    }
-    st = from_hip(upload(h, dev[k], src[k], (size_t)h->N));
-  }
-  if (st == PFG_OK) {
-    st = from_hip(launch_init_eq(h, dev[0], dev[1], dev[2], dev[3]));
-    if (st == PFG_OK) st = from_hip(hipStreamSynchronize(h->stream));
-  }
-  for (int k = 0; k < 4; ++k)
-    if (dev[k]) hipFree(dev[k]);
-  return st;
+  if (h->K != 1) return PFG_ERR_STATE;
+  return init_eq_impl(h, 0, rho, ux, uy, uz);
 }
 
 pfg_status pfg_upload_f(pfg_handle *h, const float *f) {
   if (!h || !f) return PFG_ERR_ARG;
+  if (h->K != 1) return PFG_ERR_STATE;
   hipSetDevice(h->device);
   return from_hip(launch_convert_f(h, nullptr, f));
 }
 
 pfg_status pfg_download_f(pfg_handle *h, float *f) {
   if (!h || !f) return PFG_ERR_ARG;
+  if (h->K != 1) return PFG_ERR_STATE;
   hipSetDevice(h->device);
   return from_hip(launch_convert_f(h, f, nullptr));
 }
 
 pfg_status pfg_upload_flags(pfg_handle *h, const uint8_t *flags) {
   if (!h || !flags) return PFG_ERR_ARG;
-  if (!h->solid) return PFG_ERR_STATE;
-  hipSetDevice(h->device);
-  PFG_TRY(upload(h, h->solid, flags, (size_t)h->N));
-  drop_graph(h);
-  PFG_TRY(launch_bbmask(h));
-  PFG_TRY(hipStreamSynchronize(h->stream));
-  return PFG_OK;
+  if (h->K != 1) return PFG_ERR_STATE;
+  return upload_flags_impl(h, 0, flags);
 }
 
 pfg_status pfg_download_fields(pfg_handle *h, float *rho, float *ux, float *uy,
                                float *uz) {
   if (!h) return PFG_ERR_ARG;
-  hipSetDevice(h->device);
-  float *dev[4] = {nullptr, nullptr, nullptr, nullptr};
-  float *host[4] = {rho, ux, uy, uz};
-  pfg_status st = PFG_OK;
-  for (int k = 0; k < 4; ++k) {
-    if (!host[k]) continue;
-    if (hipMalloc(&dev[k], sizeof(float) * h->N) != hipSuccess) {
-      st = PFG_ERR_ALLOC;
-      break;
-    }
-    hipMemsetAsync(dev[k], 0, sizeof(float) * h->N, h->stream);
-  }
-  if (st == PFG_OK) st = from_hip(launch_lb_fields(h, dev[0], dev[1], dev[2], dev[3]));
-  for (int k = 0; k < 4 && st == PFG_OK; ++k)
-    if (host[k]) st = from_hip(download(h, host[k], dev[k], (size_t)h->N));
-  for (int k = 0; k < 4; ++k)
-    if (dev[k]) hipFree(dev[k]);
-  return st;
+  if (h->K != 1) return PFG_ERR_STATE;
+  return fields_impl(h, 0, rho, ux, uy, uz);
 }
 
 pfg_status pfg_conservation(pfg_handle *h, double out[4]) {
   if (!h || !out) return PFG_ERR_ARG;
+  if (h->K != 1) return PFG_ERR_STATE;
   hipSetDevice(h->device);
-  return from_hip(launch_conservation(h, out));
+  return from_hip(launch_conservation(h, 0, out));
 }
 
 /* ---------------- immersed structures ---------------- */
 
-pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
-  if (!h || !md) return PFG_ERR_ARG;
+}  /* extern "C" */
+
+namespace {
+
+/* Load the markers of all lattices.  minst[i] is the lattice of marker i and
+ * linst[l] that of wall ligand l (both NULL: a single lattice). */
+pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
+                     const int32_t *linst) {
   hipSetDevice(h->device);
-  const pfg_model &M = *md;
   const int n = M.n_markers;
   /* ---- validation ---- */
   if (n < 0 || M.n_platelets < 0 || M.n_tri < 0 || M.n_edges < 0 ||
@@ -502,7 +622,8 @@ pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
   while (chunk_len < IB_CHUNK && (int64_t)chunk_len * 2 * 2048 <= n) chunk_len *= 2;
   std::vector<int32_t> chunk_start;
   for (int i = 0; i < n; ++i) {
-    if (chunk_start.empty() || i - chunk_start.back() == chunk_len || M.body[i] != M.body[i - 1])
+    if (chunk_start.empty() || i - chunk_start.back() == chunk_len || M.body[i] != M.body[i - 1] ||
+        (minst && minst[i] != minst[i - 1]))
       chunk_start.push_back(i);
   }
   chunk_start.push_back(n);
@@ -553,11 +674,20 @@ pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
       m2l_l0[k] = M.link_l0[l];
     }
   }
-  /* cell-list geometry */
-  m.nb[0] = std::max(1, p.nx / p.bin_cells);
-  m.nb[1] = std::max(1, p.ny / p.bin_cells);
-  m.nb[2] = std::max(1, p.nz / p.bin_cells);
-  m.nbins = (int64_t)m.nb[0] * m.nb[1] * m.nb[2];
+  /* cell-list geometry: bins of bin_cells cells per lattice, the lattices'
+   * bins one after the other */
+  int64_t nbins = 0;
+  for (Inst &I : h->inst) {
+    I.nb[0] = std::max(1, I.p.nx / p.bin_cells);
+    I.nb[1] = std::max(1, I.p.ny / p.bin_cells);
+    I.nb[2] = std::max(1, I.p.nz / p.bin_cells);
+    I.nbins = (int64_t)I.nb[0] * I.nb[1] * I.nb[2];
+    I.bin0 = nbins;
+    nbins += I.nbins;
+  }
+  if (nbins > (int64_t)INT_MAX / 2) return PFG_ERR_ARG;
+  for (int k = 0; k < 3; ++k) m.nb[k] = h->inst[0].nb[k];
+  m.nbins = nbins;
   m.scan_blocks = (m.nbins + 1023) / 1024;
   /* static ligands on the same bin grid, ascending ligand id per bin */
   std::vector<int32_t> lg_start, lg_items;
@@ -576,12 +706,13 @@ pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
     };
     for (int l = 0; l < M.n_wall_ligands; ++l) {
       if (M.lig_z) lig_z_host[(size_t)l] = M.lig_z[l];
+      const Inst &I = h->inst[linst ? (size_t)linst[l] : 0];
       const float c3[3] = {M.lig_x[l], M.lig_y[l], lig_z_host[(size_t)l]};
-      const int nn[3] = {p.nx, p.ny, p.nz};
+      const int nn[3] = {I.p.nx, I.p.ny, I.p.nz};
       int b[3];
       for (int k = 0; k < 3; ++k)
-        b[k] = cell(c3[k], nn[k], p.wall[k] == PFG_WALL_PERIODIC) * m.nb[k] / nn[k];
-      kv.push_back({b[0] + m.nb[0] * (b[1] + m.nb[1] * b[2]), l});
+        b[k] = cell(c3[k], nn[k], I.p.wall[k] == PFG_WALL_PERIODIC) * I.nb[k] / nn[k];
+      kv.push_back({(int)I.bin0 + b[0] + I.nb[0] * (b[1] + I.nb[1] * b[2]), l});
     }
     build_csr((int)m.nbins, kv, lg_start, lg_items);
   }
@@ -659,6 +790,7 @@ pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
   ALLOC_FILL(m.marker_slot, 0, n);
   ALLOC_FILL(m.scan_partials, 0, m.scan_blocks);
   ALLOC_UP(m.chunk_start, chunk_start.data(), chunk_start.size());
+  if (minst) ALLOC_UP(m.minst, minst, n);
   ALLOC_FILL(m.bin_scatter, 0, n);
   ALLOC_FILL(m.sorted_pos, make_float4(0.f, 0.f, 0.f, 0.f), n);
 #undef ALLOC_UP
@@ -672,9 +804,183 @@ pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
   PFG_TRY(upload(h, m.counters, zeros, 8));
   PFG_TRY(upload(h, m.detach_step, &minus1, 1));
   PFG_TRY(upload(h, m.d_step, zeros, 1));
+  if (h->K > 1) {
+    std::vector<MLat> lats((size_t)h->K);
+    for (int k = 0; k < h->K; ++k) fill_mlat(h, k, &lats[(size_t)k]);
+    PFG_TRY(upload(h, h->d_lats, lats.data(), lats.size()));
+  }
   h->h_step = 0;
   h->model_loaded = true;
   return PFG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pfg_status pfg_load_model(pfg_handle *h, const pfg_model *md) {
+  if (!h || !md) return PFG_ERR_ARG;
+  if (h->K != 1) return PFG_ERR_STATE;
+  const pfg_status st = load_impl(h, *md, nullptr, nullptr);
+  if (st == PFG_OK) h->marker0 = {0, md->n_markers};
+  return st;
+}
+
+/* ---------------- batches ---------------- */
+
+pfg_status pfg_batch_create(const pfg_params *params, int n_lattices, int device,
+                            pfg_handle **out) {
+  if (!params || n_lattices < 1 || !out) return PFG_ERR_ARG;
+  return create_impl(params, n_lattices, device, out);
+}
+
+pfg_status pfg_batch_size(const pfg_handle *h, int *n_lattices) {
+  if (!h || !n_lattices) return PFG_ERR_ARG;
+  *n_lattices = h->K;
+  return PFG_OK;
+}
+
+pfg_status pfg_batch_init_equilibrium(pfg_handle *h, int k, const float *rho, const float *ux,
+                                      const float *uy, const float *uz) {
+  if (!h || k < 0 || k >= h->K) return PFG_ERR_ARG;
+  return init_eq_impl(h, k, rho, ux, uy, uz);
+}
+
+pfg_status pfg_batch_upload_flags(pfg_handle *h, int k, const uint8_t *flags) {
+  if (!h || !flags || k < 0 || k >= h->K) return PFG_ERR_ARG;
+  return upload_flags_impl(h, k, flags);
+}
+
+pfg_status pfg_batch_download_fields(pfg_handle *h, int k, float *rho, float *ux, float *uy,
+                                     float *uz) {
+  if (!h || k < 0 || k >= h->K) return PFG_ERR_ARG;
+  return fields_impl(h, k, rho, ux, uy, uz);
+}
+
+pfg_status pfg_batch_conservation(pfg_handle *h, int k, double out[4]) {
+  if (!h || !out || k < 0 || k >= h->K) return PFG_ERR_ARG;
+  hipSetDevice(h->device);
+  return from_hip(launch_conservation(h, k, out));
+}
+
+/* The models of all lattices become one marker set: lattice k's markers,
+ * triangles, edges, receptors, ligands and links follow those of k-1, with
+ * their indices shifted. */
+pfg_status pfg_batch_load_models(pfg_handle *h, const pfg_model *models) {
+  if (!h || !models) return PFG_ERR_ARG;
+  const int K = h->K;
+  int64_t n = 0, npl = 0, ntri = 0, ned = 0, nrec = 0, nlig = 0, nlink = 0;
+  int fibrin_lig = -1;
+  for (int k = 0; k < K; ++k) {
+    const pfg_model &M = models[k];
+    if (M.n_markers < 0 || M.n_platelets < 0 || M.n_tri < 0 || M.n_edges < 0 ||
+        M.n_receptors < 0 || M.n_wall_ligands < 0 || M.n_links < 0)
+      return PFG_ERR_ARG;
+    if (M.n_platelets > 0 && (!M.plt_elem_start || !M.plt_tri_start || !M.plt_S0 || !M.plt_V0))
+      return PFG_ERR_ARG;
+    if (M.n_markers > 0 && (!M.x || !M.y || !M.z || !M.kind || !M.body)) return PFG_ERR_ARG;
+    if (M.n_tri > 0 && (!M.tri || !M.tri_S0)) return PFG_ERR_ARG;
+    if (M.n_edges > 0 && (!M.edge || !M.edge_l0)) return PFG_ERR_ARG;
+    if (M.n_receptors > 0 && (!M.rec_tri || !M.rec_r1 || !M.rec_r2)) return PFG_ERR_ARG;
+    if (M.n_wall_ligands > 0 && (!M.lig_x || !M.lig_y)) return PFG_ERR_ARG;
+    if (M.n_links > 0 && (!M.link || !M.link_l0)) return PFG_ERR_ARG;
+    if (fibrin_lig >= 0 && (M.fibrin_ligands != 0) != (fibrin_lig != 0)) return PFG_ERR_ARG;
+    fibrin_lig = M.fibrin_ligands != 0;
+    n += M.n_markers;
+    npl += M.n_platelets;
+    ntri += M.n_tri;
+    ned += M.n_edges;
+    nrec += M.n_receptors;
+    nlig += M.n_wall_ligands;
+    nlink += M.n_links;
+  }
+  if (n > INT_MAX / 4 || ntri > INT_MAX / 4) return PFG_ERR_ARG;
+  std::vector<float> x, y, z, edge_l0, r1, r2, lx, ly, lz, link_l0;
+  std::vector<int32_t> kind, body, pes(1, 0), pts(1, 0), tri, edge, rec_tri, link, minst, linst;
+  std::vector<double> tri_S0, plt_S0, plt_V0;
+  int32_t m0 = 0, p0 = 0, t0 = 0;
+  for (int k = 0; k < K; ++k) {
+    const pfg_model &M = models[k];
+    for (int i = 0; i < M.n_markers; ++i) {
+      x.push_back(M.x[i]);
+      y.push_back(M.y[i]);
+      z.push_back(M.z[i]);
+      kind.push_back(M.kind[i]);
+      body.push_back(M.body[i] >= 0 ? M.body[i] + p0 : M.body[i]);
+      minst.push_back(k);
+    }
+    for (int q = 0; q < M.n_platelets; ++q) {
+      pes.push_back(M.plt_elem_start[q + 1] + m0);
+      pts.push_back(M.plt_tri_start[q + 1] + t0);
+      plt_S0.push_back(M.plt_S0[q]);
+      plt_V0.push_back(M.plt_V0[q]);
+    }
+    for (int t = 0; t < M.n_tri; ++t) {
+      for (int c = 0; c < 3; ++c) tri.push_back(M.tri[3 * t + c] + m0);
+      tri_S0.push_back(M.tri_S0[t]);
+    }
+    for (int e = 0; e < M.n_edges; ++e) {
+      edge.push_back(M.edge[2 * e] + m0);
+      edge.push_back(M.edge[2 * e + 1] + m0);
+      edge_l0.push_back(M.edge_l0[e]);
+    }
+    for (int r = 0; r < M.n_receptors; ++r) {
+      rec_tri.push_back(M.rec_tri[r] + t0);
+      r1.push_back(M.rec_r1[r]);
+      r2.push_back(M.rec_r2[r]);
+    }
+    for (int l = 0; l < M.n_wall_ligands; ++l) {
+      lx.push_back(M.lig_x[l]);
+      ly.push_back(M.lig_y[l]);
+      lz.push_back(M.lig_z ? M.lig_z[l] : h->inst[(size_t)k].p.wall_z);
+      linst.push_back(k);
+    }
+    for (int l = 0; l < M.n_links; ++l) {
+      link.push_back(M.link[2 * l] + m0);
+      link.push_back(M.link[2 * l + 1] + m0);
+      link_l0.push_back(M.link_l0[l]);
+    }
+    m0 += M.n_markers;
+    p0 += M.n_platelets;
+    t0 += M.n_tri;
+  }
+  pfg_model all{};
+  all.n_markers = (int32_t)n;
+  all.x = x.data();
+  all.y = y.data();
+  all.z = z.data();
+  all.kind = kind.data();
+  all.body = body.data();
+  all.n_platelets = (int32_t)npl;
+  all.plt_elem_start = pes.data();
+  all.plt_tri_start = pts.data();
+  all.n_tri = (int32_t)ntri;
+  all.tri = tri.data();
+  all.tri_S0 = tri_S0.data();
+  all.plt_S0 = plt_S0.data();
+  all.plt_V0 = plt_V0.data();
+  all.n_edges = (int32_t)ned;
+  all.edge = edge.data();
+  all.edge_l0 = edge_l0.data();
+  all.n_receptors = (int32_t)nrec;
+  all.rec_tri = rec_tri.data();
+  all.rec_r1 = r1.data();
+  all.rec_r2 = r2.data();
+  all.n_wall_ligands = (int32_t)nlig;
+  all.lig_x = lx.data();
+  all.lig_y = ly.data();
+  all.lig_z = lz.data();
+  all.fibrin_ligands = fibrin_lig > 0 ? 1 : 0;
+  all.n_links = (int32_t)nlink;
+  all.link = link.data();
+  all.link_l0 = link_l0.data();
+  const pfg_status st =
+      load_impl(h, all, K > 1 ? minst.data() : nullptr, K > 1 ? linst.data() : nullptr);
+  if (st == PFG_OK) {
+    h->marker0.assign(1, 0);
+    for (int k = 0; k < K; ++k) h->marker0.push_back(h->marker0.back() + models[k].n_markers);
+  }
+  return st;
 }
 
 pfg_status pfg_upload_markers(pfg_handle *h, const float *x, const float *y,
@@ -705,6 +1011,26 @@ pfg_status pfg_download_markers(pfg_handle *h, float *x, float *y, float *z,
   PFG_TRY(download(h, fx, h->m.fx, n));
   PFG_TRY(download(h, fy, h->m.fy, n));
   PFG_TRY(download(h, fz, h->m.fz, n));
+  return PFG_OK;
+}
+
+pfg_status pfg_batch_download_markers(pfg_handle *h, int k, float *x, float *y, float *z,
+                                      float *vx, float *vy, float *vz, float *fx, float *fy,
+                                      float *fz) {
+  if (!h || k < 0 || k >= h->K) return PFG_ERR_ARG;
+  if (!h->model_loaded || h->marker0.size() != (size_t)h->K + 1) return PFG_ERR_STATE;
+  hipSetDevice(h->device);
+  const size_t m0 = (size_t)h->marker0[(size_t)k];
+  const size_t n = (size_t)h->marker0[(size_t)k + 1] - m0;
+  PFG_TRY(download(h, x, h->m.x + m0, n));
+  PFG_TRY(download(h, y, h->m.y + m0, n));
+  PFG_TRY(download(h, z, h->m.z + m0, n));
+  PFG_TRY(download(h, vx, h->m.vx + m0, n));
+  PFG_TRY(download(h, vy, h->m.vy + m0, n));
+  PFG_TRY(download(h, vz, h->m.vz + m0, n));
+  PFG_TRY(download(h, fx, h->m.fx + m0, n));
+  PFG_TRY(download(h, fy, h->m.fy + m0, n));
+  PFG_TRY(download(h, fz, h->m.fz + m0, n));
   return PFG_OK;
 }
 
@@ -947,6 +1273,8 @@ pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable) {
   if (!h) return PFG_ERR_ARG;
   h->halt_on_detach = enable != 0;
   drop_graph(h);
+  hipSetDevice(h->device);
+  PFG_TRY(upload_batch_args(h)); /* the batched LB arguments carry the flag */
   return PFG_OK;
 }
 

@@ -29,6 +29,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
@@ -87,10 +88,11 @@ __device__ __host__ inline int64_t felem(int nxb, int ny, int d, int x, int y, i
 
 enum { LB_MODE_STEP = 0, LB_MODE_FIELDS = 1 };
 
-/* One wavefront per (x-block, y, CPT consecutive z). */
+/* One wavefront per (x-block, y, CPT consecutive z); workgroup `blk` of the
+ * lattice described by `a` (in memory that the whole workgroup reads). */
 template <int S, int CPT, int MODE>
-__global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, float *oux,
-                                                       float *ouy, float *ouz) {
+__device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *orho,
+                                          float *oux, float *ouy, float *ouz) {
   using ST = Store<S>;
   using T = typename ST::T;
   constexpr int R = Q * XBLOCK; /* values per x-block */
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
   const int lane = threadIdx.x & (XBLOCK - 1);
   /* wavefront-uniform task decomposition in 32-bit arithmetic (x-blocks per
    * lattice stay below 2^31) */
-  const uint32_t task = blockIdx.x * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t task = blk * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t rows = (uint32_t)a.nxb * (uint32_t)a.ny;
   const uint32_t zc = task / rows;
   if (zc * CPT >= (uint32_t)a.nz) return;
@@ -260,6 +262,22 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
   }
 }
 
+template <int S, int CPT, int MODE>
+__global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, float *oux,
+                                                       float *ouy, float *ouz) {
+  lb_update<S, CPT, MODE>(a, blockIdx.x, orho, oux, ouy, ouz);
+}
+
+/* A batch of lattices in one launch: workgroup -> lattice through a table,
+ * each lattice's arguments read from device memory. */
+template <int S, int CPT>
+__global__ __launch_bounds__(256) void lb_batch_kernel(const LBArgs *args,
+                                                       const int32_t *block_inst) {
+  const LBArgs &a = args[block_inst[blockIdx.x]];
+  lb_update<S, CPT, LB_MODE_STEP>(a, (uint32_t)(blockIdx.x - a.block0), nullptr, nullptr,
+                                   nullptr, nullptr);
+}
+
 /* ---------------- auxiliary kernels ---------------- */
 
 /* Per-cell bounce mask from the solid flags (built once per flag upload). */
@@ -392,29 +410,33 @@ __global__ __launch_bounds__(256) void conservation_kernel(LBArgs a, const void 
 
 /* ---------------- host launchers ---------------- */
 
-void make_lb_args(const pfg_handle *h, LBArgs *a) {
-  const pfg_params &p = h->p;
-  a->fin = h->f[h->cur];
-  a->fout = h->f[1 - h->cur];
-  a->aux = h->aux;
-  a->tile_flag = h->tile_flag;
-  a->solid = h->solid;
-  a->bbmask = h->bbmask;
+/* Arguments of lattice k of the handle, starting from copy `cur` of f. */
+void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a) {
+  const Inst &L = h->inst[(size_t)k];
+  const pfg_params &p = L.p;
+  const size_t foff = (size_t)L.fblk0 * Q * XBLOCK * h->esize;
+  a->fin = (const char *)h->f[cur] + foff;
+  a->fout = (char *)h->f[1 - cur] + foff;
+  a->aux = h->aux + L.cell0;
+  a->tile_flag = h->tile_flag + L.tile0;
+  a->solid = h->solid ? h->solid + L.cell0 : nullptr;
+  a->bbmask = h->bbmask ? h->bbmask + L.cell0 : nullptr;
   a->halt = h->halt_on_detach ? h->m.counters + 1 : nullptr;
   a->nx = p.nx;
   a->ny = p.ny;
   a->nz = p.nz;
-  a->nxb = h->nxb;
+  a->nxb = L.nxb;
   a->plane = h->plane;
-  a->tnx = h->tnx;
-  a->tny = h->tny;
-  for (int k = 0; k < 3; ++k) a->periodic[k] = p.wall[k] == PFG_WALL_PERIODIC;
+  a->block0 = L.lb_block0;
+  a->tnx = L.tnx;
+  a->tny = L.tny;
+  for (int ax = 0; ax < 3; ++ax) a->periodic[ax] = p.wall[ax] == PFG_WALL_PERIODIC;
   a->omega = 1.0f / p.tau;
   a->guo = 1.0f - 0.5f / p.tau;
-  for (int k = 0; k < 3; ++k) a->g[k] = p.body_force[k];
-  if (p.store == PFG_STORE_F16) {
-    a->scale = ldexpf(1.0f, p.f16_scale_log2);
-    a->inv_scale = ldexpf(1.0f, -p.f16_scale_log2);
+  for (int ax = 0; ax < 3; ++ax) a->g[ax] = p.body_force[ax];
+  if (h->p.store == PFG_STORE_F16) {
+    a->scale = ldexpf(1.0f, h->p.f16_scale_log2);
+    a->inv_scale = ldexpf(1.0f, -h->p.f16_scale_log2);
   } else {
     a->scale = 1.0f;
     a->inv_scale = 1.0f;
@@ -425,10 +447,10 @@ void make_lb_args(const pfg_handle *h, LBArgs *a) {
   a->om3 = 3.0f * a->omega;
   a->guo3 = 3.0f * a->guo;
   const int rep[3] = {0, 1, 7}; /* rest, axis, diagonal */
-  for (int k = 0; k < 3; ++k) {
-    a->ws[k] = weight(rep[k]) * a->scale;
-    a->gB[k] = 9.0f * a->guo * a->ws[k];
-    a->gD[k] = 3.0f * a->guo * a->ws[k];
+  for (int c = 0; c < 3; ++c) {
+    a->ws[c] = weight(rep[c]) * a->scale;
+    a->gB[c] = 9.0f * a->guo * a->ws[c];
+    a->gD[c] = 3.0f * a->guo * a->ws[c];
   }
   /* halfway bounce-back with a moving wall (Ladd): the population entering
    * from the wall is f_bb(d) + 6 w_d rho_w (c_d . u_w), rho_w = 1; in the
@@ -466,10 +488,10 @@ static hipError_t launch_lb_s(pfg_handle *h, const LBArgs &a, int mode, float *r
   }
 }
 
-static hipError_t dispatch_lb(pfg_handle *h, int mode, float *rho, float *ux, float *uy,
+static hipError_t dispatch_lb(pfg_handle *h, int k, int mode, float *rho, float *ux, float *uy,
                               float *uz) {
   LBArgs a;
-  make_lb_args(h, &a);
+  make_lb_args(h, k, h->cur, &a);
   switch (h->p.store) {
     case PFG_STORE_F16: return launch_lb_s<PFG_STORE_F16>(h, a, mode, rho, ux, uy, uz);
     case PFG_STORE_BF16: return launch_lb_s<PFG_STORE_BF16>(h, a, mode, rho, ux, uy, uz);
@@ -477,50 +499,74 @@ static hipError_t dispatch_lb(pfg_handle *h, int mode, float *rho, float *ux, fl
   }
 }
 
-hipError_t launch_lb(pfg_handle *h) {
-  hipError_t e = dispatch_lb(h, LB_MODE_STEP, nullptr, nullptr, nullptr, nullptr);
-  if (e == hipSuccess) h->cur = 1 - h->cur;
-  return e;
-}
-
-hipError_t launch_lb_fields(pfg_handle *h, float *rho, float *ux, float *uy, float *uz) {
-  return dispatch_lb(h, LB_MODE_FIELDS, rho, ux, uy, uz);
-}
-
-hipError_t launch_bbmask(pfg_handle *h) {
-  if (!h->solid || !h->bbmask) return hipSuccess;
-  LBArgs a;
-  make_lb_args(h, &a);
-  hipLaunchKernelGGL(bbmask_kernel, dim3((unsigned)((h->N + 255) / 256)), dim3(256), 0,
-                     h->stream, a, h->solid, h->bbmask);
-  return hipGetLastError();
-}
-
-hipError_t launch_init_eq(pfg_handle *h, const float *rho, const float *ux, const float *uy,
-                          const float *uz) {
-  LBArgs a;
-  make_lb_args(h, &a);
-  const int64_t n = h->N;
-  dim3 block(256), grid((unsigned)((n + 255) / 256));
+/* All lattices of a batch in one launch (one z-cell per lane). */
+static hipError_t launch_lb_batch(pfg_handle *h) {
+  const dim3 grid((unsigned)h->lb_blocks), block(256);
+  const LBArgs *args = h->d_lbargs[h->cur];
   switch (h->p.store) {
     case PFG_STORE_F16:
-      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F16>, grid, block, 0, h->stream, a, h->f[0],
-                         h->f[1], rho, ux, uy, uz, n);
+      hipLaunchKernelGGL((lb_batch_kernel<PFG_STORE_F16, 1>), grid, block, 0, h->stream, args,
+                         h->lb_block_inst);
       break;
     case PFG_STORE_BF16:
-      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_BF16>, grid, block, 0, h->stream, a, h->f[0],
-                         h->f[1], rho, ux, uy, uz, n);
+      hipLaunchKernelGGL((lb_batch_kernel<PFG_STORE_BF16, 1>), grid, block, 0, h->stream, args,
+                         h->lb_block_inst);
       break;
     default:
-      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F32>, grid, block, 0, h->stream, a, h->f[0],
-                         h->f[1], rho, ux, uy, uz, n);
+      hipLaunchKernelGGL((lb_batch_kernel<PFG_STORE_F32, 1>), grid, block, 0, h->stream, args,
+                         h->lb_block_inst);
   }
   return hipGetLastError();
 }
 
+hipError_t launch_lb(pfg_handle *h) {
+  hipError_t e = h->K > 1 ? launch_lb_batch(h)
+                          : dispatch_lb(h, 0, LB_MODE_STEP, nullptr, nullptr, nullptr, nullptr);
+  if (e == hipSuccess) h->cur = 1 - h->cur;
+  return e;
+}
+
+hipError_t launch_lb_fields(pfg_handle *h, int k, float *rho, float *ux, float *uy, float *uz) {
+  return dispatch_lb(h, k, LB_MODE_FIELDS, rho, ux, uy, uz);
+}
+
+hipError_t launch_bbmask(pfg_handle *h, int k) {
+  if (!h->solid || !h->bbmask) return hipSuccess;
+  LBArgs a;
+  make_lb_args(h, k, h->cur, &a);
+  const Inst &L = h->inst[(size_t)k];
+  hipLaunchKernelGGL(bbmask_kernel, dim3((unsigned)((L.N + 255) / 256)), dim3(256), 0,
+                     h->stream, a, a.solid, h->bbmask + L.cell0);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_eq(pfg_handle *h, int k, const float *rho, const float *ux,
+                          const float *uy, const float *uz) {
+  LBArgs a;
+  make_lb_args(h, k, 0, &a);
+  const int64_t n = h->inst[(size_t)k].N;
+  void *d0 = (void *)a.fin, *d1 = a.fout; /* both copies of lattice k */
+  dim3 block(256), grid((unsigned)((n + 255) / 256));
+  switch (h->p.store) {
+    case PFG_STORE_F16:
+      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F16>, grid, block, 0, h->stream, a, d0, d1,
+                         rho, ux, uy, uz, n);
+      break;
+    case PFG_STORE_BF16:
+      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_BF16>, grid, block, 0, h->stream, a, d0, d1,
+                         rho, ux, uy, uz, n);
+      break;
+    default:
+      hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F32>, grid, block, 0, h->stream, a, d0, d1,
+                         rho, ux, uy, uz, n);
+  }
+  return hipGetLastError();
+}
+
+/* [19][N] full f of a single-lattice handle to or from the current copy. */
 hipError_t launch_convert_f(pfg_handle *h, float *f_out, const float *f_in) {
   LBArgs a;
-  make_lb_args(h, &a);
+  make_lb_args(h, 0, h->cur, &a);
   const int64_t n = h->N;
   float *buf = nullptr;
   hipError_t e = hipMalloc(&buf, sizeof(float) * (size_t)n);
@@ -559,22 +605,23 @@ hipError_t launch_convert_f(pfg_handle *h, float *f_out, const float *f_in) {
   return e;
 }
 
-hipError_t launch_conservation(pfg_handle *h, double out[4]) {
+hipError_t launch_conservation(pfg_handle *h, int k, double out[4]) {
   LBArgs a;
-  make_lb_args(h, &a);
-  const int blocks = (int)h->red_blocks;
+  make_lb_args(h, k, h->cur, &a);
+  const int64_t n = h->inst[(size_t)k].N;
+  const int blocks = (int)std::min<int64_t>(h->red_blocks, (n + 255) / 256);
   switch (h->p.store) {
     case PFG_STORE_F16:
       hipLaunchKernelGGL(conservation_kernel<PFG_STORE_F16>, dim3(blocks), dim3(256), 0,
-                         h->stream, a, h->f[h->cur], h->red, h->N);
+                         h->stream, a, a.fin, h->red, n);
       break;
     case PFG_STORE_BF16:
       hipLaunchKernelGGL(conservation_kernel<PFG_STORE_BF16>, dim3(blocks), dim3(256), 0,
-                         h->stream, a, h->f[h->cur], h->red, h->N);
+                         h->stream, a, a.fin, h->red, n);
       break;
     default:
       hipLaunchKernelGGL(conservation_kernel<PFG_STORE_F32>, dim3(blocks), dim3(256), 0,
-                         h->stream, a, h->f[h->cur], h->red, h->N);
+                         h->stream, a, a.fin, h->red, n);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -584,9 +631,9 @@ hipError_t launch_conservation(pfg_handle *h, double out[4]) {
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(h->stream);
   if (e != hipSuccess) return e;
-  for (int k = 0; k < 4; ++k) out[k] = 0.0;
+  for (int c = 0; c < 4; ++c) out[c] = 0.0;
   for (int b = 0; b < blocks; ++b)
-    for (int k = 0; k < 4; ++k) out[k] += part[4 * (size_t)b + k];
+    for (int c = 0; c < 4; ++c) out[c] += part[4 * (size_t)b + c];
   return hipSuccess;
 }
 

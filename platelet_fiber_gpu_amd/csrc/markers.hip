@@ -50,13 +50,12 @@ namespace pfg {
 struct MArgs {
   Markers m;
   /* lattice side of the IB coupling */
-  float *aux;
-  int64_t plane;
-  uint8_t *tile_flag;
-  int tnx, tny, tnz;
-  int nx, ny, nz;
-  int periodic[3];
-  int nb[3];
+  int64_t plane;          /* stride between the components of the field      */
+  MLat lat;               /* the lattice of a single-lattice handle           */
+  const MLat *lats;       /* batches: [K], selected by the marker's lattice   */
+  const int64_t *tile0s;  /* batches: first global tile per lattice, [K+1]    */
+  int K;
+  int64_t ntiles;         /* tiles over all lattices                          */
   /* parameters */
   float morse_u0, morse_rho, morse_req, morse_cut2, morse_cut, morse_taper;
   float k_spring, ks, kv;
@@ -100,10 +99,15 @@ __device__ inline int cell_of(float v, int n, int periodic) {
   return c;
 }
 
-__device__ inline void bin3_of(const MArgs &a, float x, float y, float z, int b[3]) {
-  b[0] = cell_of(x, a.nx, a.periodic[0]) * a.nb[0] / a.nx;
-  b[1] = cell_of(y, a.ny, a.periodic[1]) * a.nb[1] / a.ny;
-  b[2] = cell_of(z, a.nz, a.periodic[2]) * a.nb[2] / a.nz;
+/* The lattice of marker i. */
+__device__ inline MLat lat_of(const MArgs &a, int i) {
+  return a.lats ? a.lats[a.m.minst[i]] : a.lat;
+}
+
+__device__ inline void bin3_of(const MLat &L, float x, float y, float z, int b[3]) {
+  b[0] = cell_of(x, L.nx, L.periodic[0]) * L.nb[0] / L.nx;
+  b[1] = cell_of(y, L.ny, L.periodic[1]) * L.nb[1] / L.ny;
+  b[2] = cell_of(z, L.nz, L.periodic[2]) * L.nb[2] / L.nz;
 }
 
 __device__ inline float min_image(float d, int n, int periodic) {
@@ -134,15 +138,16 @@ __device__ inline int nbr_bins(int b, int nb, int periodic, int out[3]) {
   return c;
 }
 
-/* The k-th of the (up to 27) bins around bin (b3); -1 when k is past the end. */
-__device__ inline int nth_nbr_bin(const MArgs &a, const int b3[3], int k) {
+/* Global index of the k-th of the (up to 27) bins around bin (b3) of lattice
+ * L; -1 when k is past the end. */
+__device__ inline int nth_nbr_bin(const MLat &L, const int b3[3], int k) {
   int bx[3], by[3], bz[3];
-  const int nx = nbr_bins(b3[0], a.nb[0], a.periodic[0], bx);
-  const int ny = nbr_bins(b3[1], a.nb[1], a.periodic[1], by);
-  const int nz = nbr_bins(b3[2], a.nb[2], a.periodic[2], bz);
+  const int nx = nbr_bins(b3[0], L.nb[0], L.periodic[0], bx);
+  const int ny = nbr_bins(b3[1], L.nb[1], L.periodic[1], by);
+  const int nz = nbr_bins(b3[2], L.nb[2], L.periodic[2], bz);
   if (k >= nx * ny * nz) return -1;
   const int jx = k % nx, jy = (k / nx) % ny, jz = k / (nx * ny);
-  return bx[jx] + a.nb[0] * (by[jy] + a.nb[1] * bz[jz]);
+  return L.bin0 + bx[jx] + L.nb[0] * (by[jy] + L.nb[1] * bz[jz]);
 }
 
 __device__ inline float wave_sum(float v) {
@@ -221,33 +226,45 @@ __device__ inline int stencil_node(float X, int k, int n, int periodic, float *w
 
 /* Block `block` scans 256 tile flags; the flagged tiles (16x4x4 = 256 cells
  * each) are zeroed by the whole block and unflagged. */
+/* Lattice of global tile g of a batch (tile0s ascending). */
+__device__ inline int lat_of_tile(const MArgs &a, int64_t g) {
+  int lo = 0, hi = a.K - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.tile0s[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 __device__ void clear_tiles(const MArgs &a, int block) {
-  __shared__ int list[256];
+  __shared__ int64_t list[256];
   __shared__ int count;
   if (threadIdx.x == 0) count = 0;
   __syncthreads();
-  const int64_t ntiles = (int64_t)a.tnx * a.tny * a.tnz;
+  uint8_t *flags = a.lats ? a.lats[0].tile_flag : a.lat.tile_flag; /* all lattices' tiles */
   const int64_t my = (int64_t)block * 256 + threadIdx.x;
-  if (my < ntiles && a.tile_flag[my]) list[atomicAdd(&count, 1)] = (int)my;
+  if (my < a.ntiles && flags[my]) list[atomicAdd(&count, 1)] = my;
   __syncthreads();
   const int t = threadIdx.x;
   const int lx = t & (TILE_X - 1);
   const int ly = (t >> TILE_SHIFT_X) & (TILE_Y - 1);
   const int lz = t >> (TILE_SHIFT_X + TILE_SHIFT_Y);
   for (int k = 0; k < count; ++k) {
-    const int tile = list[k];
-    const int tx = tile % a.tnx;
-    const int ty = (tile / a.tnx) % a.tny;
-    const int tz = tile / (a.tnx * a.tny);
+    const MLat L = a.lats ? a.lats[lat_of_tile(a, list[k])] : a.lat;
+    const int64_t tile = list[k] - L.tile0;
+    const int tx = (int)(tile % L.tnx);
+    const int ty = (int)((tile / L.tnx) % L.tny);
+    const int tz = (int)(tile / ((int64_t)L.tnx * L.tny));
     const int x = tx * TILE_X + lx, y = ty * TILE_Y + ly, z = tz * TILE_Z + lz;
-    if (x < a.nx && y < a.ny && z < a.nz) {
-      const int64_t c = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
-      a.aux[c] = 0.f;
-      a.aux[a.plane + c] = 0.f;
-      a.aux[2 * a.plane + c] = 0.f;
+    if (x < L.nx && y < L.ny && z < L.nz) {
+      const int64_t c = x + (int64_t)L.nx * (y + (int64_t)L.ny * z);
+      L.aux[c] = 0.f;
+      L.aux[a.plane + c] = 0.f;
+      L.aux[2 * a.plane + c] = 0.f;
     }
   }
-  if (my < ntiles && a.tile_flag[my]) a.tile_flag[my] = 0;
+  if (my < a.ntiles && flags[my]) flags[my] = 0;
 }
 
 /* roles: A = clear last step's tiles (blocks0 blocks), B = count bins */
@@ -264,9 +281,10 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
   const int i = blk * 256 + threadIdx.x;
   if (i == 0) a.m.counters[0] = 0; /* bonds held, counted again this step */
   if (i >= a.m.n) return;
+  const MLat L = lat_of(a, i);
   int b3[3];
-  bin3_of(a, a.m.x[i], a.m.y[i], a.m.z[i], b3);
-  const int b = b3[0] + a.nb[0] * (b3[1] + a.nb[1] * b3[2]);
+  bin3_of(L, a.m.x[i], a.m.y[i], a.m.z[i], b3);
+  const int b = L.bin0 + b3[0] + L.nb[0] * (b3[1] + L.nb[1] * b3[2]);
   a.m.marker_bin[i] = b;
   a.m.marker_slot[i] = atomicAdd(&a.m.bin_count[b], 1);
 }
@@ -421,16 +439,17 @@ __device__ inline void receptor_pos(const MArgs &a, int r, float out[3], int nod
   out[2] = c0 * a.m.z[nodes[0]] + c1 * a.m.z[nodes[1]] + c2 * a.m.z[nodes[2]];
 }
 
-__device__ inline void ligand_vec(const MArgs &a, int l, const float rp[3], float d[3]) {
+__device__ inline void ligand_vec(const MArgs &a, const MLat &L, int l, const float rp[3],
+                                  float d[3]) {
   if (l < a.m.n_wall_lig) {
-    d[0] = min_image(a.m.lig_x[l] - rp[0], a.nx, a.periodic[0]);
-    d[1] = min_image(a.m.lig_y[l] - rp[1], a.ny, a.periodic[1]);
-    d[2] = min_image(a.m.lig_z[l] - rp[2], a.nz, a.periodic[2]);
+    d[0] = min_image(a.m.lig_x[l] - rp[0], L.nx, L.periodic[0]);
+    d[1] = min_image(a.m.lig_y[l] - rp[1], L.ny, L.periodic[1]);
+    d[2] = min_image(a.m.lig_z[l] - rp[2], L.nz, L.periodic[2]);
   } else {
     const int mk = l - a.m.n_wall_lig;
-    d[0] = min_image(a.m.x[mk] - rp[0], a.nx, a.periodic[0]);
-    d[1] = min_image(a.m.y[mk] - rp[1], a.ny, a.periodic[1]);
-    d[2] = min_image(a.m.z[mk] - rp[2], a.nz, a.periodic[2]);
+    d[0] = min_image(a.m.x[mk] - rp[0], L.nx, L.periodic[0]);
+    d[1] = min_image(a.m.y[mk] - rp[1], L.ny, L.periodic[1]);
+    d[2] = min_image(a.m.z[mk] - rp[2], L.nz, L.periodic[2]);
   }
 }
 
@@ -443,12 +462,13 @@ __device__ void bond_decide(const MArgs &a, int r) {
   float rp[3];
   int nodes[3];
   receptor_pos(a, r, rp, nodes);
+  const MLat L = lat_of(a, nodes[0]);
   const int b = a.m.rec_bond[r];
   if (b >= 0) {
     if (lane == 0) {
       a.m.rec_cand[r] = -1;
       float d[3];
-      ligand_vec(a, b, rp, d);
+      ligand_vec(a, L, b, rp, d);
       const float dist = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
       const float tension = a.bond_k * (dist - a.bond_l0);
       /* Bell: k_off = k_off0 exp(x_b f / kBT) for a pulling bond */
@@ -466,14 +486,14 @@ __device__ void bond_decide(const MArgs &a, int r) {
   float best_d2 = INFINITY;
   bool near_reach = false;
   int b3[3];
-  bin3_of(a, rp[0], rp[1], rp[2], b3);
-  const int bin = lane < 27 ? nth_nbr_bin(a, b3, lane) : -1;
+  bin3_of(L, rp[0], rp[1], rp[2], b3);
+  const int bin = lane < 27 ? nth_nbr_bin(L, b3, lane) : -1;
   if (bin >= 0) {
     if (a.m.n_wall_lig > 0) {
       for (int k = a.m.lg_start[bin]; k < a.m.lg_start[bin + 1]; ++k) {
         const int l = a.m.lg_items[k];
         float d[3];
-        ligand_vec(a, l, rp, d);
+        ligand_vec(a, L, l, rp, d);
         const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         if (fabsf(sqrtf(d2) - a.bond_reach) < BOND_EPS_R) near_reach = true;
         if (d2 > reach2 || a.m.lig_owner[l] >= 0) continue;
@@ -489,7 +509,7 @@ __device__ void bond_decide(const MArgs &a, int r) {
         if (a.m.kind[mk] == PFG_MARKER_ELEMENT) continue;
         const int l = a.m.n_wall_lig + mk;
         float d[3];
-        ligand_vec(a, l, rp, d);
+        ligand_vec(a, L, l, rp, d);
         const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         if (fabsf(sqrtf(d2) - a.bond_reach) < BOND_EPS_R) near_reach = true;
         if (d2 > reach2 || a.m.lig_owner[l] >= 0) continue;
@@ -605,7 +625,7 @@ __global__ void bond_resolve_kernel(MArgs a) {
     int nodes[3];
     receptor_pos(a, r, rp, nodes);
     float d[3];
-    ligand_vec(a, b, rp, d);
+    ligand_vec(a, lat_of(a, nodes[0]), b, rp, d);
     const float dist = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     if (dist > 0.f) {
       const float c = a.bond_k * (dist - a.bond_l0) / dist;
@@ -734,9 +754,10 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
   if (kind == PFG_MARKER_ELEMENT) {
     const int body = a.m.body[i];
     if (a.morse_u0 != 0.f && lane < 27) {
-      const int b = a.m.marker_bin[i];
-      const int b3[3] = {b % a.nb[0], (b / a.nb[0]) % a.nb[1], b / (a.nb[0] * a.nb[1])};
-      const int bin = nth_nbr_bin(a, b3, lane);
+      const MLat L = lat_of(a, i);
+      const int b = a.m.marker_bin[i] - L.bin0;
+      const int b3[3] = {b % L.nb[0], (b / L.nb[0]) % L.nb[1], b / (L.nb[0] * L.nb[1])};
+      const int bin = nth_nbr_bin(L, b3, lane);
       if (bin >= 0) morse_bin(a, bin, body, xi, yi, zi, fx, fy, fz);
     }
     const int t0 = a.m.m2t_start[i], nt = (a.ks != 0.f || a.kv != 0.f) ? a.m.m2t_start[i + 1] - t0 : 0;
@@ -829,28 +850,29 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
  * markers share the lattice.  A chunk whose box does not fit in LDS (markers
  * scattered through the lattice) is spread directly. */
 
-__device__ inline void flag_tile(const MArgs &a, int x, int y, int z) {
-  const int tile = (x >> TILE_SHIFT_X) + a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
-  if (a.tile_flag[tile] == 0) a.tile_flag[tile] = 1;
+__device__ inline void flag_tile(const MLat &L, int x, int y, int z) {
+  const int tile = (x >> TILE_SHIFT_X) + L.tnx * ((y >> TILE_SHIFT_Y) + L.tny * (z >> TILE_SHIFT_Z));
+  if (L.tile_flag[tile] == 0) L.tile_flag[tile] = 1;
 }
 
 /* Direct spreading of marker i: lane = stencil node. */
 __device__ void spread_direct(const MArgs &a, int i, int lane) {
   if (!ib_coupled(a.m.kind[i], a.fiber_move)) return;
-  const float X = wrap_coord(a.m.x[i], a.nx, a.periodic[0]);
-  const float Y = wrap_coord(a.m.y[i], a.ny, a.periodic[1]);
-  const float Z = wrap_coord(a.m.z[i], a.nz, a.periodic[2]);
+  const MLat L = lat_of(a, i);
+  const float X = wrap_coord(a.m.x[i], L.nx, L.periodic[0]);
+  const float Y = wrap_coord(a.m.y[i], L.ny, L.periodic[1]);
+  const float Z = wrap_coord(a.m.z[i], L.nz, L.periodic[2]);
   float wx, wy, wz;
-  const int ix = stencil_node(X, lane & 3, a.nx, a.periodic[0], &wx);
-  const int iy = stencil_node(Y, (lane >> 2) & 3, a.ny, a.periodic[1], &wy);
-  const int iz = stencil_node(Z, lane >> 4, a.nz, a.periodic[2], &wz);
+  const int ix = stencil_node(X, lane & 3, L.nx, L.periodic[0], &wx);
+  const int iy = stencil_node(Y, (lane >> 2) & 3, L.ny, L.periodic[1], &wy);
+  const int iz = stencil_node(Z, lane >> 4, L.nz, L.periodic[2], &wz);
   if (ix < 0 || iy < 0 || iz < 0) return;
-  flag_tile(a, ix, iy, iz);
+  flag_tile(L, ix, iy, iz);
   const float w = wx * wy * wz * a.force_scale;
-  const int64_t c = ix + (int64_t)a.nx * (iy + (int64_t)a.ny * iz);
-  atomicAdd(a.aux + c, w * a.m.fx[i]);
-  atomicAdd(a.aux + a.plane + c, w * a.m.fy[i]);
-  atomicAdd(a.aux + 2 * a.plane + c, w * a.m.fz[i]);
+  const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
+  atomicAdd(L.aux + c, w * a.m.fx[i]);
+  atomicAdd(L.aux + a.plane + c, w * a.m.fy[i]);
+  atomicAdd(L.aux + 2 * a.plane + c, w * a.m.fz[i]);
 }
 
 /* One wavefront per marker. */
@@ -871,14 +893,14 @@ struct ChunkBox {
  * and the first stencil node in the periodic image nearest to the chunk's
  * reference point `ref` (the wrapped position of its first marker), so that
  * one platelet's stencils stay contiguous across a periodic boundary. */
-__device__ inline void stencil_origin(const MArgs &a, int i, const float ref[3], float W[3],
-                                      int o[3]) {
+__device__ inline void stencil_origin(const MArgs &a, const MLat &L, int i, const float ref[3],
+                                      float W[3], int o[3]) {
   const float v[3] = {a.m.x[i], a.m.y[i], a.m.z[i]};
-  const int n[3] = {a.nx, a.ny, a.nz};
+  const int n[3] = {L.nx, L.ny, L.nz};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    W[k] = wrap_coord(v[k], n[k], a.periodic[k]);
-    const int img = a.periodic[k] ? -(int)rintf((W[k] - ref[k]) / (float)n[k]) * n[k] : 0;
+    W[k] = wrap_coord(v[k], n[k], L.periodic[k]);
+    const int img = L.periodic[k] ? -(int)rintf((W[k] - ref[k]) / (float)n[k]) * n[k] : 0;
     o[k] = (int)floorf(W[k]) - 1 + img;
   }
 }
@@ -890,13 +912,14 @@ __device__ inline float stencil_weight(float W, int k) {
 
 /* Block-wide box of the chunk's markers [m0, m0 + cnt) (cnt <= 256); every
  * thread gets the result. */
-__device__ ChunkBox chunk_box(const MArgs &a, int m0, int cnt, const float ref[3], int *sh) {
+__device__ ChunkBox chunk_box(const MArgs &a, const MLat &L, int m0, int cnt, const float ref[3],
+                              int *sh) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
   if (t < cnt && ib_coupled(a.m.kind[m0 + t], a.fiber_move)) {
     float W[3];
     int o[3];
-    stencil_origin(a, m0 + t, ref, W, o);
+    stencil_origin(a, L, m0 + t, ref, W, o);
     for (int k = 0; k < 3; ++k) {
       lo[k] = o[k];
       hi[k] = o[k] + 3;
@@ -940,11 +963,12 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
   __shared__ int sh[24];
   if (halted(a)) return;
   const int m0 = a.m.chunk_start[blockIdx.x], cnt = a.m.chunk_start[blockIdx.x + 1] - m0;
+  const MLat L = lat_of(a, m0); /* a chunk never spans two lattices */
   float ref[3];
-  ref[0] = wrap_coord(a.m.x[m0], a.nx, a.periodic[0]);
-  ref[1] = wrap_coord(a.m.y[m0], a.ny, a.periodic[1]);
-  ref[2] = wrap_coord(a.m.z[m0], a.nz, a.periodic[2]);
-  const ChunkBox b = chunk_box(a, m0, cnt, ref, sh);
+  ref[0] = wrap_coord(a.m.x[m0], L.nx, L.periodic[0]);
+  ref[1] = wrap_coord(a.m.y[m0], L.ny, L.periodic[1]);
+  ref[2] = wrap_coord(a.m.z[m0], L.nz, L.periodic[2]);
+  const ChunkBox b = chunk_box(a, L, m0, cnt, ref, sh);
   if (b.vol == 0) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (!b.lds) {
@@ -959,7 +983,7 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
     if (!ib_coupled(a.m.kind[i], a.fiber_move)) continue;
     float W[3];
     int o[3];
-    stencil_origin(a, i, ref, W, o);
+    stencil_origin(a, L, i, ref, W, o);
     const float w = stencil_weight(W[0], kx) * stencil_weight(W[1], ky) *
                     stencil_weight(W[2], kz) * a.force_scale;
     const int cell = (o[0] + kx - b.lo[0]) + b.dim[0] * ((o[1] + ky - b.lo[1]) +
@@ -973,20 +997,20 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
   const int rows = b.dim[1] * b.dim[2];
   for (int r = wave; r < rows; r += 4) {
     const int by = r % b.dim[1], bz = r / b.dim[1];
-    const int gy = axis_index(b.lo[1] + by, a.ny, a.periodic[1]);
-    const int gz = axis_index(b.lo[2] + bz, a.nz, a.periodic[2]);
+    const int gy = axis_index(b.lo[1] + by, L.ny, L.periodic[1]);
+    const int gz = axis_index(b.lo[2] + bz, L.nz, L.periodic[2]);
     if (gy < 0 || gz < 0) continue;
-    const int64_t row = (int64_t)a.nx * (gy + (int64_t)a.ny * gz);
+    const int64_t row = (int64_t)L.nx * (gy + (int64_t)L.ny * gz);
     for (int bx = lane; bx < b.dim[0]; bx += 64) {
-      const int gx = axis_index(b.lo[0] + bx, a.nx, a.periodic[0]);
+      const int gx = axis_index(b.lo[0] + bx, L.nx, L.periodic[0]);
       if (gx < 0) continue;
-      flag_tile(a, gx, gy, gz);
+      flag_tile(L, gx, gy, gz);
       const int k = bx + b.dim[0] * r;
       const float fx = box[k], fy = box[b.vol + k], fz = box[2 * b.vol + k];
       const int64_t c = row + gx;
-      if (fx != 0.f) atomicAdd(a.aux + c, fx);
-      if (fy != 0.f) atomicAdd(a.aux + a.plane + c, fy);
-      if (fz != 0.f) atomicAdd(a.aux + 2 * a.plane + c, fz);
+      if (fx != 0.f) atomicAdd(L.aux + c, fx);
+      if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
+      if (fz != 0.f) atomicAdd(L.aux + 2 * a.plane + c, fz);
     }
   }
 }
@@ -994,19 +1018,20 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
 /* Interpolated velocity of marker i (lane = stencil node, summed over the
  * wavefront; nodes outside a wall-bounded axis contribute nothing). */
 __device__ void interp_marker(const MArgs &a, int i, int lane, float *v) {
-  const float X = wrap_coord(a.m.x[i], a.nx, a.periodic[0]);
-  const float Y = wrap_coord(a.m.y[i], a.ny, a.periodic[1]);
-  const float Z = wrap_coord(a.m.z[i], a.nz, a.periodic[2]);
+  const MLat L = lat_of(a, i);
+  const float X = wrap_coord(a.m.x[i], L.nx, L.periodic[0]);
+  const float Y = wrap_coord(a.m.y[i], L.ny, L.periodic[1]);
+  const float Z = wrap_coord(a.m.z[i], L.nz, L.periodic[2]);
   float wx, wy, wz, ux = 0.f, uy = 0.f, uz = 0.f;
-  const int ix = stencil_node(X, lane & 3, a.nx, a.periodic[0], &wx);
-  const int iy = stencil_node(Y, (lane >> 2) & 3, a.ny, a.periodic[1], &wy);
-  const int iz = stencil_node(Z, lane >> 4, a.nz, a.periodic[2], &wz);
+  const int ix = stencil_node(X, lane & 3, L.nx, L.periodic[0], &wx);
+  const int iy = stencil_node(Y, (lane >> 2) & 3, L.ny, L.periodic[1], &wy);
+  const int iz = stencil_node(Z, lane >> 4, L.nz, L.periodic[2], &wz);
   if (ix >= 0 && iy >= 0 && iz >= 0) {
     const float w = wx * wy * wz;
-    const int64_t c = ix + (int64_t)a.nx * (iy + (int64_t)a.ny * iz);
-    ux = w * a.aux[c];
-    uy = w * a.aux[a.plane + c];
-    uz = w * a.aux[2 * a.plane + c];
+    const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
+    ux = w * L.aux[c];
+    uy = w * L.aux[a.plane + c];
+    uz = w * L.aux[2 * a.plane + c];
   }
   v[0] = wave_sum(ux);
   v[1] = wave_sum(uy);
@@ -1063,22 +1088,33 @@ __global__ __launch_bounds__(256) void move_kernel(MArgs a) {
 
 /* ---------------- launchers ---------------- */
 
+void fill_mlat(const pfg_handle *h, int k, MLat *L) {
+  const Inst &I = h->inst[(size_t)k];
+  L->aux = h->aux + I.cell0;
+  L->tile_flag = h->tile_flag + I.tile0;
+  L->tnx = I.tnx;
+  L->tny = I.tny;
+  L->tnz = I.tnz;
+  L->nx = I.p.nx;
+  L->ny = I.p.ny;
+  L->nz = I.p.nz;
+  for (int ax = 0; ax < 3; ++ax) {
+    L->periodic[ax] = I.p.wall[ax] == PFG_WALL_PERIODIC;
+    L->nb[ax] = I.nb[ax];
+  }
+  L->bin0 = (int)I.bin0;
+  L->tile0 = I.tile0;
+}
+
 static void make_margs(const pfg_handle *h, MArgs *a) {
   const pfg_params &p = h->p;
   a->m = h->m;
-  a->aux = h->aux;
   a->plane = h->plane;
-  a->tile_flag = h->tile_flag;
-  a->tnx = h->tnx;
-  a->tny = h->tny;
-  a->tnz = h->tnz;
-  a->nx = p.nx;
-  a->ny = p.ny;
-  a->nz = p.nz;
-  for (int k = 0; k < 3; ++k) {
-    a->periodic[k] = p.wall[k] == PFG_WALL_PERIODIC;
-    a->nb[k] = h->m.nb[k];
-  }
+  fill_mlat(h, 0, &a->lat);
+  a->lats = h->K > 1 ? h->d_lats : nullptr;
+  a->tile0s = h->K > 1 ? h->d_tile0 : nullptr;
+  a->K = h->K;
+  a->ntiles = h->ntiles;
   a->morse_u0 = p.morse_u0;
   a->morse_rho = p.morse_rho;
   a->morse_req = p.morse_req;

@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/pfg.h"
 #include "pfg_lattice.h"
@@ -42,6 +43,8 @@ struct LBArgs {
   int nx, ny, nz;
   int nxb;                /* x-blocks of XBLOCK cells per row                 */
   int64_t plane;          /* stride between the components of aux            */
+  int64_t block0;         /* first workgroup of this lattice in a batched
+                             launch                                          */
   int tnx, tny;           /* tiles per axis                                   */
   int periodic[3];
   float omega;            /* 1/tau                                           */
@@ -57,9 +60,23 @@ struct LBArgs {
   float wall_term[3][2][Q];
 };
 
+/* Lattice side of the IB coupling for one lattice of a batch (a handle
+ * with one lattice has one). */
+struct MLat {
+  float *aux;             /* the lattice's cells of the [3][plane] field      */
+  uint8_t *tile_flag;     /* the lattice's tiles                              */
+  int tnx, tny, tnz;
+  int nx, ny, nz;
+  int periodic[3];
+  int nb[3];              /* cell-list bins per axis                          */
+  int bin0;               /* first global bin of the lattice                  */
+  int64_t tile0;          /* first global tile of the lattice                 */
+};
+
 /* Marker-side device arrays. */
 struct Markers {
   int n = 0;
+  int32_t *minst = nullptr;      /* lattice of each marker (batches only)    */
   float *x = nullptr, *y = nullptr, *z = nullptr;
   float *vx = nullptr, *vy = nullptr, *vz = nullptr;
   float *fx = nullptr, *fy = nullptr, *fz = nullptr;
@@ -123,6 +140,20 @@ struct Markers {
   int64_t *d_step = nullptr;
 };
 
+/* Host-side description of one lattice of a handle. */
+struct Inst {
+  pfg_params p{};          /* lattice and fluid fields apply per lattice     */
+  int64_t N = 0;           /* cells                                          */
+  int64_t cell0 = 0;       /* first cell in the concatenated cell fields     */
+  int nxb = 0;
+  int64_t fblk0 = 0;       /* first x-block in the population arrays          */
+  int tnx = 0, tny = 0, tnz = 0;
+  int64_t ntiles = 0, tile0 = 0;
+  int nb[3] = {1, 1, 1};
+  int64_t nbins = 0, bin0 = 0;
+  int64_t lb_blocks = 0, lb_block0 = 0; /* workgroups in the batched LB launch */
+};
+
 }  // namespace pfg
 
 struct pfg_handle {
@@ -151,6 +182,15 @@ struct pfg_handle {
   bool halt_on_detach = false; /* steps after the detach test fired change nothing */
   int ib_staging = 0;          /* spreading: 0 auto, 1 LDS-staged chunks, 2 direct */
   int64_t h_step = 0;
+  /* lattices: one, or a batch stepped together */
+  int K = 1;
+  std::vector<pfg::Inst> inst;
+  pfg::LBArgs *d_lbargs[2] = {nullptr, nullptr}; /* per starting copy of f, [K] */
+  int32_t *lb_block_inst = nullptr;              /* batched LB: lattice per block */
+  int64_t lb_blocks = 0;
+  pfg::MLat *d_lats = nullptr;                   /* [K] */
+  int64_t *d_tile0 = nullptr;                    /* [K+1] first tile per lattice */
+  std::vector<int32_t> marker0;                  /* [K+1] first marker per lattice */
   bool use_graph = true;
   /* [starting copy of f][0: one step, 1: GRAPH_CHUNK steps] */
   hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -160,19 +200,20 @@ struct pfg_handle {
 
 namespace pfg {
 
-/* lb.hip */
-void make_lb_args(const pfg_handle *h, LBArgs *a);
+/* lb.hip; k selects a lattice of the handle */
+void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a);
 hipError_t launch_lb(pfg_handle *h);
-hipError_t launch_bbmask(pfg_handle *h);
-hipError_t launch_lb_fields(pfg_handle *h, float *rho, float *ux, float *uy,
+hipError_t launch_bbmask(pfg_handle *h, int k);
+hipError_t launch_lb_fields(pfg_handle *h, int k, float *rho, float *ux, float *uy,
                             float *uz);
-hipError_t launch_init_eq(pfg_handle *h, const float *rho, const float *ux,
+hipError_t launch_init_eq(pfg_handle *h, int k, const float *rho, const float *ux,
                           const float *uy, const float *uz);
 hipError_t launch_convert_f(pfg_handle *h, float *f_out_or_null,
                             const float *f_in_or_null);
-hipError_t launch_conservation(pfg_handle *h, double out[4]);
+hipError_t launch_conservation(pfg_handle *h, int k, double out[4]);
 
 /* markers.hip: fused marker-side launches, each with the roles it runs */
+void fill_mlat(const pfg_handle *h, int k, MLat *L);
 hipError_t launch_begin(pfg_handle *h, bool clear, bool count);
 hipError_t launch_scan(pfg_handle *h);
 hipError_t launch_place(pfg_handle *h, bool scatter, bool geometry);

@@ -1,0 +1,89 @@
+"""Batches (pfg_batch_*): every lattice of a batch advances exactly as the
+same lattice in a handle of its own, which the single-lattice parity tests
+tie to the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def lattice_params(pfg, nx, ny, nz, walls, g=(0.0, 0.0, 0.0), tau=0.8, u_top=0.0, store=0):
+    p = pfg.default_params(nx=nx, ny=ny, nz=nz, tau=tau, store=store)
+    for k in range(3):
+        p.wall[k] = walls[k]
+    for k in range(3):
+        p.body_force[k] = g[k]
+    p.wall_u_hi[2][0] = u_top
+    return p
+
+
+def fluid_set(pfg, store):
+    P, B = pfg.WALL_PERIODIC, pfg.WALL_BOUNCE
+    return [lattice_params(pfg, 40, 24, 20, (P, P, P), g=(2e-5, 0, 1e-5), store=store),
+            lattice_params(pfg, 70, 16, 18, (P, P, B), tau=0.7, u_top=0.04, store=store),
+            lattice_params(pfg, 130, 9, 5, (P, B, B), g=(1e-5, 0, 0), tau=1.1, store=store)]
+
+
+@pytest.mark.parametrize("store", [0, 1])
+def test_batch_fluid_equals_single(gpu, store):
+    """Three lattices of different shapes, walls, tau and forcing: the batched
+    LB launch gives bitwise the fields of three separate handles."""
+    pfg = gpu
+    ps = fluid_set(pfg, store)
+    steps = 25
+    with pfg.Batch(ps) as b:
+        b.set_fluid_only(True)
+        b.step(steps)
+        got = [b.fields_of(k) for k in range(len(ps))]
+        cons = [b.conservation_of(k) for k in range(len(ps))]
+    for k, p in enumerate(ps):
+        with pfg.Solver(p) as s:
+            s.set_fluid_only(True)
+            s.step(steps)
+            want = s.fields()
+            c = s.conservation()
+        for a, w in zip(got[k], want):
+            assert np.array_equal(a, w), k
+        assert np.array_equal(cons[k], c), k
+
+
+def test_batch_single_lattice_calls_are_refused(gpu):
+    pfg = gpu
+    ps = fluid_set(pfg, 0)
+    with pfg.Batch(ps) as b:
+        with pytest.raises(pfg.PfgError) as e:
+            b.fields()
+        assert e.value.code == 5  # PFG_ERR_STATE
+        with pytest.raises(pfg.PfgError):
+            b.init_equilibrium()
+
+
+@pytest.mark.parametrize("staging", [1, 2])
+def test_batch_coupled_equals_single(gpu, staging):
+    """Shear lattices of different sizes with one or two platelets each (no
+    receptors: the bond RNG is keyed by the batch-wide receptor index): each
+    lattice's markers and fields follow its separate run."""
+    pfg = gpu
+    scenes = [pfg.Scene(pfg.SCENE_BATCH, nx=48, ny=32, nz=24, elements=128, receptors=-1,
+                        ligands=-1, seed=11),
+              pfg.Scene(pfg.SCENE_BATCH, nx=40, ny=40, nz=28, n_platelets=2, elements=96,
+                        receptors=-1, ligands=-1, seed=12),
+              pfg.Scene(pfg.SCENE_BATCH, nx=64, ny=24, nz=20, elements=160, receptors=-1,
+                        ligands=-1, seed=13)]
+    steps = 12
+    with pfg.Batch([s.params for s in scenes]) as b:
+        b.set_ib_staging(staging)
+        b.load_scenes(scenes)
+        b.step(steps)
+        got = [(b.markers_of(k), b.fields_of(k)) for k in range(len(scenes))]
+    for k, sc in enumerate(scenes):
+        with pfg.Solver(sc.params) as s:
+            s.set_ib_staging(staging)
+            s.load(sc)
+            s.step(steps)
+            mk = s.markers()
+            fl = s.fields()
+        for name in ("x", "y", "z", "vx", "vy", "vz"):
+            assert np.abs(got[k][0][name] - mk[name]).max() < 1e-5, (k, name)
+        for a, w in zip(got[k][1], fl):
+            assert np.abs(a - w).max() < 1e-6, k
