@@ -151,6 +151,89 @@ def other_workload(pfg, name, K, W):
     }
 
 
+def batch_scenes(pfg, n=256, seed=1101):
+    """The batched workload: n vessel segments, each axis U{32..96} cells, wall
+    shear rate log-uniform in [10, 1000] /s mapped to a top-wall velocity of at
+    most 0.05 (0.05 (rate/1000) (nz/96)), 1-8 platelets of 256 elements (fewer
+    when they do not fit), 32 receptors per platelet, 512 wall ligands."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for k in range(n):
+        nx, ny, nz = (int(v) for v in rng.integers(32, 97, size=3))
+        rate = float(np.exp(rng.uniform(np.log(10.0), np.log(1000.0))))
+        want = int(rng.integers(1, 9))
+        sc = None
+        for npl in range(want, 0, -1):
+            try:
+                sc = pfg.Scene(pfg.SCENE_BATCH, nx=nx, ny=ny, nz=nz, n_platelets=npl,
+                               elements=256, receptors=32, ligands=512, seed=seed + k)
+                break
+            except pfg.PfgError:
+                continue
+        if sc is None:
+            sc = pfg.Scene(pfg.SCENE_BATCH, nx=nx, ny=ny, nz=nz, n_platelets=-1,
+                           receptors=-1, ligands=512, seed=seed + k)
+        uw = 0.05 * (rate / 1000.0) * (nz / 96.0)
+        sc.params.wall_u_hi[2][0] = uw
+        sc.u0[0][:] = uw * (np.arange(sc.N) // (nx * ny) + 0.5) / nz
+        scenes.append(sc)
+    return scenes
+
+
+def batch_workload(pfg, K, W, store):
+    """256 lattices stepped as one batch, against the same 256 lattices as
+    separate handles stepped one after the other."""
+    t0 = time.perf_counter()
+    scenes = batch_scenes(pfg)
+    for sc in scenes:
+        sc.params.store = store
+    ncells = sum(sc.N for sc in scenes)
+    nmark = sum(sc.n_markers for sc in scenes)
+    log(f"batch: {len(scenes)} lattices, {ncells} cells, {nmark} markers "
+        f"(built in {time.perf_counter() - t0:.1f} s)")
+    b = pfg.Batch([sc.params for sc in scenes])
+    b.load_scenes(scenes)
+    b.step(W)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(K)
+    b.synchronize()
+    dt_batch = time.perf_counter() - t0
+    dev_gb = b.device_bytes() / 1e9
+    b.close()
+    solvers = []
+    for sc in scenes:
+        s = pfg.Solver(sc.params)
+        s.load(sc)
+        s.step(W)
+        solvers.append(s)
+    ks = max(1, K // 4)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.step(ks)
+    for s in solvers:
+        s.synchronize()
+    dt_sep = (time.perf_counter() - t0) * K / ks
+    for s in solvers:
+        s.close()
+    dtype = {0: "f32", 1: "f16", 2: "bf16"}[store]
+    return {
+        "metric": METRIC, "value": round(ncells * K / dt_batch / 1e6, 1), "unit": "MLUPS",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(dt_batch * 1e3 / K, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic (seeded scenes)",
+        "config": {"workload": "batch of 256 vessel segments, each axis U{32..96}, wall shear "
+                               "log-U[10,1000]/s, 1-8 platelets x 256 elements, one handle",
+                   "lattices": len(scenes), "cells": ncells, "markers": nmark, "store": dtype},
+        "coupled_steps_per_s": round(K / dt_batch, 2),
+        "separate_handles_ms_per_step": round(dt_sep * 1e3 / K, 4),
+        "speedup_vs_separate": round(dt_sep / dt_batch, 2),
+        "device_gb": round(dev_gb, 2),
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def fp16_accuracy(pfg):
     """Relative L2 velocity error of FP16 deviation storage against the FP64
     oracle: 64x32x32 body-force channel, tau 0.8, F = 1e-6, 300 steps."""
@@ -183,8 +266,10 @@ def main():
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--lb-only", choices=sorted(STORES), default=None)
     ap.add_argument("--copy-probe", action="store_true")
-    ap.add_argument("--workload", choices=["shear"] + sorted(WORKLOADS), default="shear",
-                    help="shear (default) is the headline configuration")
+    ap.add_argument("--workload", choices=["shear", "batch"] + sorted(WORKLOADS),
+                    default="shear", help="shear (default) is the headline configuration")
+    ap.add_argument("--store", choices=sorted(STORES), default="f32",
+                    help="storage of the batch workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
@@ -205,6 +290,9 @@ def main():
         return
 
     K, W = args.steps, args.warmup
+    if args.workload == "batch":
+        print(json.dumps(batch_workload(pfg, K, W, STORES[args.store])), flush=True)
+        return
     if args.workload != "shear":
         print(json.dumps(other_workload(pfg, args.workload, K, W)), flush=True)
         return
