@@ -295,6 +295,13 @@ pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable);
  * 2 always direct (FP32 atomics per marker and node).  Same result up to
  * FP32 summation order. */
 pfg_status pfg_set_ib_staging(pfg_handle *h, int mode);
+/* enable = 1: within a step, the LB update of the lattice rows that no IB
+ * stencil touches runs on a second HIP stream concurrently with the marker
+ * pipeline, and the touched rows update after the spreading (semLB.cu:303-438
+ * runs these phases one after the other); 0 (default): one LB launch after
+ * the spreading.  Same result either way; on MI355X the concurrent LB launch
+ * slows the latency-bound marker kernels as much as it hides them. */
+pfg_status pfg_set_overlap(pfg_handle *h, int enable);
 pfg_status pfg_synchronize(pfg_handle *h);
 
 /* ---- batches: many lattices stepped together ----

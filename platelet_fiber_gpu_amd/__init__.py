@@ -316,6 +316,11 @@ class Solver:
         """IB spreading form: 0 automatic, 1 LDS-staged chunks, 2 direct."""
         self._c("pfg_set_ib_staging", int(mode))
 
+    def set_overlap(self, on):
+        """LB rows away from IB stencils update concurrently with the marker
+        pipeline (default off)."""
+        self._c("pfg_set_overlap", int(bool(on)))
+
     def set_halt_on_detach(self, on):
         """End the run at detach: later steps change nothing and step()
         reports the steps that ran."""

@@ -117,6 +117,7 @@ SIGNATURES = {
     "pfg_set_fluid_only": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_halt_on_detach": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_ib_staging": (C.c_int, [C.c_void_p, C.c_int]),
+    "pfg_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_batch_create": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int,
                                    C.POINTER(C.c_void_p)]),
     "pfg_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),

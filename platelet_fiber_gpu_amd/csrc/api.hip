@@ -310,7 +310,10 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
   h->p.use_flags = flags ? 1 : 0;
   pfg_status st = PFG_OK;
   do {
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
       st = PFG_ERR_DEVICE;
       break;
     }
@@ -327,6 +330,12 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
     }
     if (st != PFG_OK) break;
     if (dalloc(h, &h->aux, (size_t)3 * h->plane) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    /* row stamps start at -1: no row is touched before the first step */
+    if (dalloc(h, &h->row_stamp, (size_t)fblk) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (hipMemset(h->row_stamp, 0xff, sizeof(int32_t) * (size_t)fblk) != hipSuccess) {
+      st = PFG_ERR_DEVICE;
+      break;
+    }
     if (hipMemset(h->aux, 0, sizeof(float) * 3 * h->plane) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     h->tnx = h->inst[0].tnx;
     h->tny = h->inst[0].tny;
@@ -479,6 +488,10 @@ pfg_status pfg_destroy(pfg_handle *h) {
   dfree(h->lb_block_inst);
   dfree(h->d_lats);
   dfree(h->d_tile0);
+  dfree(h->row_stamp);
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
+  if (h->ev_join) hipEventDestroy(h->ev_join);
+  if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return PFG_OK;
@@ -1161,11 +1174,23 @@ static hipError_t run_stage(pfg_handle *h, int s) {
  * forces from X(n), spread, LB with F(n), interpolate, move.  The stages are
  * fused into nine launches; the previous step's force tiles are cleared by
  * the first of them. */
+/* One coupled step.  With overlap, the first marker launch stamps the rows
+ * the IB stencils touch; the LB update of all other rows then runs on
+ * stream2 while the marker pipeline runs on the main stream, and the
+ * stamped rows update after the spreading.  Without overlap (or when the LB
+ * launch is timed by lb0/lb1), one LB launch follows the spreading. */
 static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
   hipError_t e = hipSuccess;
   const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
+  const bool overlap = markers && h->overlap && h->row_stamp && !lb0 && !lb1;
   if (markers) {
     e = launch_begin(h, true, true);
+    if (overlap) {
+      if (e == hipSuccess) e = hipEventRecord(h->ev_fork, h->stream);
+      if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+      if (e == hipSuccess) e = launch_lb_part(h, 1, h->stream2);
+      if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->stream2);
+    }
     if (e == hipSuccess) e = launch_scan(h);
     if (e == hipSuccess) e = launch_place(h, true, true);
     if (e == hipSuccess) e = launch_gather(h, true, true, true);
@@ -1173,9 +1198,14 @@ static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
     if (e == hipSuccess) e = launch_forces(h);
     if (e == hipSuccess) e = launch_spread(h);
   }
-  if (e == hipSuccess && lb0) e = hipEventRecord(lb0, h->stream);
-  if (e == hipSuccess) e = launch_lb(h);
-  if (e == hipSuccess && lb1) e = hipEventRecord(lb1, h->stream);
+  if (overlap) {
+    if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->ev_join, 0);
+    if (e == hipSuccess) e = launch_lb_part(h, 2, h->stream);
+  } else {
+    if (e == hipSuccess && lb0) e = hipEventRecord(lb0, h->stream);
+    if (e == hipSuccess) e = launch_lb(h);
+    if (e == hipSuccess && lb1) e = hipEventRecord(lb1, h->stream);
+  }
   if (markers && e == hipSuccess) e = launch_move(h, true, true);
   return e;
 }
@@ -1260,6 +1290,13 @@ pfg_status pfg_step(pfg_handle *h, int nsteps, pfg_step_info *info) {
     }
   }
   return fill_info(h, info);
+}
+
+pfg_status pfg_set_overlap(pfg_handle *h, int enable) {
+  if (!h) return PFG_ERR_ARG;
+  h->overlap = enable != 0;
+  drop_graph(h);
+  return PFG_OK;
 }
 
 pfg_status pfg_set_ib_staging(pfg_handle *h, int mode) {

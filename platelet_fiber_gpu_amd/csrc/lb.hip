@@ -86,7 +86,11 @@ __device__ __host__ inline int64_t felem(int nxb, int ny, int d, int x, int y, i
 
 /* ---------------- the fused kernel ---------------- */
 
-enum { LB_MODE_STEP = 0, LB_MODE_FIELDS = 1 };
+/* STEP: every row, with the IB force in flagged tiles.  FIELDS: rho and u
+ * only.  BULK / REGION: the rows whose stamp is not / is the current step
+ * (rows that this step's IB stencils touch), without / with the IB force;
+ * one z-row per wavefront. */
+enum { LB_MODE_STEP = 0, LB_MODE_FIELDS = 1, LB_MODE_BULK = 2, LB_MODE_REGION = 3 };
 
 /* One wavefront per (x-block, y, CPT consecutive z); workgroup `blk` of the
  * lattice described by `a` (in memory that the whole workgroup reads). */
@@ -98,7 +102,7 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
   constexpr int R = Q * XBLOCK; /* values per x-block */
   const T *__restrict__ fin = (const T *)a.fin;
   T *__restrict__ fout = (T *)a.fout;
-  if (MODE == LB_MODE_STEP && a.halt != nullptr && *a.halt != 0) return;
+  if (MODE != LB_MODE_FIELDS && a.halt != nullptr && *a.halt != 0) return;
   const int lane = threadIdx.x & (XBLOCK - 1);
   /* wavefront-uniform task decomposition in 32-bit arithmetic (x-blocks per
    * lattice stay below 2^31) */
@@ -106,6 +110,11 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
   const uint32_t rows = (uint32_t)a.nxb * (uint32_t)a.ny;
   const uint32_t zc = task / rows;
   if (zc * CPT >= (uint32_t)a.nz) return;
+  if (MODE == LB_MODE_BULK || MODE == LB_MODE_REGION) {
+    static_assert(MODE < LB_MODE_BULK || CPT == 1, "row modes update one z-row per wave");
+    const bool touched = a.row_stamp[task] == (int32_t)*a.step; /* task = x-block row */
+    if (touched != (MODE == LB_MODE_REGION)) return;
+  }
   const uint32_t rem = task - zc * rows;
   const int y = (int)(rem / (uint32_t)a.nxb);
   const int xb = (int)(rem - (uint32_t)y * a.nxb);
@@ -204,7 +213,7 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
     const float rho = 1.0f + drho;
     float Fx = a.g[0], Fy = a.g[1], Fz = a.g[2];
     bool flagged = false;
-    if (MODE == LB_MODE_STEP) {
+    if (MODE == LB_MODE_STEP || MODE == LB_MODE_REGION) {
       const int tile = (x >> TILE_SHIFT_X) + a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
       flagged = a.tile_flag[tile] != 0;
       if (flagged) {
@@ -270,12 +279,12 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
 
 /* A batch of lattices in one launch: workgroup -> lattice through a table,
  * each lattice's arguments read from device memory. */
-template <int S, int CPT>
+template <int S, int MODE>
 __global__ __launch_bounds__(256) void lb_batch_kernel(const LBArgs *args,
                                                        const int32_t *block_inst) {
   const LBArgs &a = args[block_inst[blockIdx.x]];
-  lb_update<S, CPT, LB_MODE_STEP>(a, (uint32_t)(blockIdx.x - a.block0), nullptr, nullptr,
-                                   nullptr, nullptr);
+  lb_update<S, 1, MODE>(a, (uint32_t)(blockIdx.x - a.block0), nullptr, nullptr, nullptr,
+                        nullptr);
 }
 
 /* ---------------- auxiliary kernels ---------------- */
@@ -422,6 +431,8 @@ void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a) {
   a->solid = h->solid ? h->solid + L.cell0 : nullptr;
   a->bbmask = h->bbmask ? h->bbmask + L.cell0 : nullptr;
   a->halt = h->halt_on_detach ? h->m.counters + 1 : nullptr;
+  a->row_stamp = h->row_stamp ? h->row_stamp + L.fblk0 : nullptr;
+  a->step = h->m.d_step;
   a->nx = p.nx;
   a->ny = p.ny;
   a->nz = p.nz;
@@ -466,68 +477,96 @@ void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a) {
 
 template <int S, int CPT>
 static hipError_t launch_lb_t(pfg_handle *h, const LBArgs &a, int mode, float *rho,
-                              float *ux, float *uy, float *uz) {
+                              float *ux, float *uy, float *uz, hipStream_t st) {
   const int64_t waves = (int64_t)a.nxb * a.ny * ((a.nz + CPT - 1) / CPT);
   const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
   if (mode == LB_MODE_STEP)
-    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_STEP>), grid, block, 0, h->stream, a,
+    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_STEP>), grid, block, 0, st, a,
+                       nullptr, nullptr, nullptr, nullptr);
+  else if (mode == LB_MODE_FIELDS)
+    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_FIELDS>), grid, block, 0, st, a,
+                       rho, ux, uy, uz);
+  else if (CPT != 1)
+    return hipErrorInvalidValue;
+  else if (mode == LB_MODE_BULK)
+    hipLaunchKernelGGL((lb_fused_kernel<S, 1, LB_MODE_BULK>), grid, block, 0, st, a,
                        nullptr, nullptr, nullptr, nullptr);
   else
-    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_FIELDS>), grid, block, 0, h->stream, a,
-                       rho, ux, uy, uz);
+    hipLaunchKernelGGL((lb_fused_kernel<S, 1, LB_MODE_REGION>), grid, block, 0, st, a,
+                       nullptr, nullptr, nullptr, nullptr);
   return hipGetLastError();
 }
 
 template <int S>
 static hipError_t launch_lb_s(pfg_handle *h, const LBArgs &a, int mode, float *rho, float *ux,
-                              float *uy, float *uz) {
-  switch (h->lb_cpt) {
-    case 2: return launch_lb_t<S, 2>(h, a, mode, rho, ux, uy, uz);
-    case 4: return launch_lb_t<S, 4>(h, a, mode, rho, ux, uy, uz);
-    default: return launch_lb_t<S, 1>(h, a, mode, rho, ux, uy, uz);
+                              float *uy, float *uz, hipStream_t st) {
+  const int cpt = mode >= LB_MODE_BULK ? 1 : h->lb_cpt;
+  switch (cpt) {
+    case 2: return launch_lb_t<S, 2>(h, a, mode, rho, ux, uy, uz, st);
+    case 4: return launch_lb_t<S, 4>(h, a, mode, rho, ux, uy, uz, st);
+    default: return launch_lb_t<S, 1>(h, a, mode, rho, ux, uy, uz, st);
   }
 }
 
 static hipError_t dispatch_lb(pfg_handle *h, int k, int mode, float *rho, float *ux, float *uy,
-                              float *uz) {
+                              float *uz, hipStream_t st) {
   LBArgs a;
   make_lb_args(h, k, h->cur, &a);
   switch (h->p.store) {
-    case PFG_STORE_F16: return launch_lb_s<PFG_STORE_F16>(h, a, mode, rho, ux, uy, uz);
-    case PFG_STORE_BF16: return launch_lb_s<PFG_STORE_BF16>(h, a, mode, rho, ux, uy, uz);
-    default: return launch_lb_s<PFG_STORE_F32>(h, a, mode, rho, ux, uy, uz);
+    case PFG_STORE_F16: return launch_lb_s<PFG_STORE_F16>(h, a, mode, rho, ux, uy, uz, st);
+    case PFG_STORE_BF16: return launch_lb_s<PFG_STORE_BF16>(h, a, mode, rho, ux, uy, uz, st);
+    default: return launch_lb_s<PFG_STORE_F32>(h, a, mode, rho, ux, uy, uz, st);
   }
 }
 
-/* All lattices of a batch in one launch (one z-cell per lane). */
-static hipError_t launch_lb_batch(pfg_handle *h) {
+template <int S>
+static hipError_t launch_lb_batch_s(pfg_handle *h, int mode, hipStream_t st) {
   const dim3 grid((unsigned)h->lb_blocks), block(256);
   const LBArgs *args = h->d_lbargs[h->cur];
-  switch (h->p.store) {
-    case PFG_STORE_F16:
-      hipLaunchKernelGGL((lb_batch_kernel<PFG_STORE_F16, 1>), grid, block, 0, h->stream, args,
+  switch (mode) {
+    case LB_MODE_BULK:
+      hipLaunchKernelGGL((lb_batch_kernel<S, LB_MODE_BULK>), grid, block, 0, st, args,
                          h->lb_block_inst);
       break;
-    case PFG_STORE_BF16:
-      hipLaunchKernelGGL((lb_batch_kernel<PFG_STORE_BF16, 1>), grid, block, 0, h->stream, args,
+    case LB_MODE_REGION:
+      hipLaunchKernelGGL((lb_batch_kernel<S, LB_MODE_REGION>), grid, block, 0, st, args,
                          h->lb_block_inst);
       break;
     default:
-      hipLaunchKernelGGL((lb_batch_kernel<PFG_STORE_F32, 1>), grid, block, 0, h->stream, args,
+      hipLaunchKernelGGL((lb_batch_kernel<S, LB_MODE_STEP>), grid, block, 0, st, args,
                          h->lb_block_inst);
   }
   return hipGetLastError();
 }
 
+/* All lattices of a batch in one launch (one z-cell per lane). */
+static hipError_t launch_lb_batch(pfg_handle *h, int mode, hipStream_t st) {
+  switch (h->p.store) {
+    case PFG_STORE_F16: return launch_lb_batch_s<PFG_STORE_F16>(h, mode, st);
+    case PFG_STORE_BF16: return launch_lb_batch_s<PFG_STORE_BF16>(h, mode, st);
+    default: return launch_lb_batch_s<PFG_STORE_F32>(h, mode, st);
+  }
+}
+
 hipError_t launch_lb(pfg_handle *h) {
-  hipError_t e = h->K > 1 ? launch_lb_batch(h)
-                          : dispatch_lb(h, 0, LB_MODE_STEP, nullptr, nullptr, nullptr, nullptr);
+  hipError_t e = h->K > 1 ? launch_lb_batch(h, LB_MODE_STEP, h->stream)
+                          : dispatch_lb(h, 0, LB_MODE_STEP, nullptr, nullptr, nullptr, nullptr,
+                                        h->stream);
   if (e == hipSuccess) h->cur = 1 - h->cur;
   return e;
 }
 
+hipError_t launch_lb_part(pfg_handle *h, int part, hipStream_t st) {
+  if (!h->row_stamp) return hipErrorInvalidValue;
+  const int mode = part == 1 ? LB_MODE_BULK : LB_MODE_REGION;
+  hipError_t e = h->K > 1 ? launch_lb_batch(h, mode, st)
+                          : dispatch_lb(h, 0, mode, nullptr, nullptr, nullptr, nullptr, st);
+  if (e == hipSuccess && part == 2) h->cur = 1 - h->cur;
+  return e;
+}
+
 hipError_t launch_lb_fields(pfg_handle *h, int k, float *rho, float *ux, float *uy, float *uz) {
-  return dispatch_lb(h, k, LB_MODE_FIELDS, rho, ux, uy, uz);
+  return dispatch_lb(h, k, LB_MODE_FIELDS, rho, ux, uy, uz, h->stream);
 }
 
 hipError_t launch_bbmask(pfg_handle *h, int k) {

@@ -70,6 +70,7 @@ struct MArgs {
   int roles;
   int halt; /* 1: after the detach test fires, leave bonds, force field and
                markers unchanged (the step loop ends at detach) */
+  int mark_rows; /* 1: stamp the x-block rows this step's IB stencils touch */
   int blocks0, blocks1, blocks2; /* blocks of role ranges 0, 1, 2 */
 };
 
@@ -287,6 +288,30 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
   const int b = L.bin0 + b3[0] + L.nb[0] * (b3[1] + L.nb[1] * b3[2]);
   a.m.marker_bin[i] = b;
   a.m.marker_slot[i] = atomicAdd(&a.m.bin_count[b], 1);
+  if (a.mark_rows && ib_coupled(a.m.kind[i], a.fiber_move) && !halted(a)) {
+    /* stamp the rows of this step's 4x4x4 stencil (the nodes spreading and
+     * interpolation use), so that the LB update of every other row may run
+     * before the spreading */
+    const int step = (int)*a.m.d_step;
+    const int ox = (int)floorf(wrap_coord(a.m.x[i], L.nx, L.periodic[0])) - 1;
+    const int oy = (int)floorf(wrap_coord(a.m.y[i], L.ny, L.periodic[1])) - 1;
+    const int oz = (int)floorf(wrap_coord(a.m.z[i], L.nz, L.periodic[2])) - 1;
+    for (int kz = 0; kz < 4; ++kz) {
+      const int z = axis_index(oz + kz, L.nz, L.periodic[2]);
+      if (z < 0) continue;
+      for (int ky = 0; ky < 4; ++ky) {
+        const int y = axis_index(oy + ky, L.ny, L.periodic[1]);
+        if (y < 0) continue;
+        int last = -1;
+        for (int kx = 0; kx < 4; ++kx) {
+          const int x = axis_index(ox + kx, L.nx, L.periodic[0]);
+          if (x < 0 || (x >> 6) == last) continue;
+          last = x >> 6;
+          L.row_stamp[last + (int64_t)L.nxb * (y + (int64_t)L.ny * z)] = step;
+        }
+      }
+    }
+  }
 }
 
 /* ---------------- K2..K4: prefix scan of the bin counts ---------------- */
@@ -1104,6 +1129,8 @@ void fill_mlat(const pfg_handle *h, int k, MLat *L) {
   }
   L->bin0 = (int)I.bin0;
   L->tile0 = I.tile0;
+  L->nxb = I.nxb;
+  L->row_stamp = h->row_stamp ? h->row_stamp + I.fblk0 : nullptr;
 }
 
 static void make_margs(const pfg_handle *h, MArgs *a) {
@@ -1146,6 +1173,7 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->force_scale = p.ib_force_scale;
   a->roles = 0;
   a->halt = h->halt_on_detach ? 1 : 0;
+  a->mark_rows = h->overlap && h->row_stamp != nullptr ? 1 : 0;
   a->blocks0 = a->blocks1 = a->blocks2 = 0;
 }
 

@@ -40,6 +40,9 @@ struct LBArgs {
                              bit 31 = cell is solid; nullptr without flags  */
   const int64_t *halt;    /* nonzero *halt: leave the lattice unchanged
                              (run halted at detach); nullptr: never          */
+  const int32_t *row_stamp; /* per x-block row: the last step whose IB
+                             stencils touch the row                          */
+  const int64_t *step;    /* the current step (device counter)              */
   int nx, ny, nz;
   int nxb;                /* x-blocks of XBLOCK cells per row                 */
   int64_t plane;          /* stride between the components of aux            */
@@ -71,6 +74,8 @@ struct MLat {
   int nb[3];              /* cell-list bins per axis                          */
   int bin0;               /* first global bin of the lattice                  */
   int64_t tile0;          /* first global tile of the lattice                 */
+  int nxb;                /* x-blocks per row                                 */
+  int32_t *row_stamp;     /* the lattice's x-block rows (nullptr: unused)     */
 };
 
 /* Marker-side device arrays. */
@@ -181,6 +186,14 @@ struct pfg_handle {
   bool fluid_only = false;
   bool halt_on_detach = false; /* steps after the detach test fired change nothing */
   int ib_staging = 0;          /* spreading: 0 auto, 1 LDS-staged chunks, 2 direct */
+  /* LB rows away from IB stencils update on stream2 while the marker
+   * pipeline runs; the rows the stencils touch update after spreading.  Off
+   * by default: on MI355X the LB launch saturates HBM and the CU slots, and
+   * the latency-bound marker kernels slow down as much as they overlap. */
+  bool overlap = false;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int32_t *row_stamp = nullptr;    /* per x-block row of every lattice      */
   int64_t h_step = 0;
   /* lattices: one, or a batch stepped together */
   int K = 1;
@@ -203,6 +216,9 @@ namespace pfg {
 /* lb.hip; k selects a lattice of the handle */
 void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a);
 hipError_t launch_lb(pfg_handle *h);
+/* part 1: rows no IB stencil touches this step (on stream s); part 2: the
+ * rows they touch, with the IB force (then the copies of f alternate) */
+hipError_t launch_lb_part(pfg_handle *h, int part, hipStream_t s);
 hipError_t launch_bbmask(pfg_handle *h, int k);
 hipError_t launch_lb_fields(pfg_handle *h, int k, float *rho, float *ux, float *uy,
                             float *uz);

@@ -141,3 +141,30 @@ def test_halt_on_detach_freezes_the_run(gpu, po, graph):
     assert np.abs(f_halt - f_plain).max() < 1e-6
     for k in ("x", "y", "z"):
         assert np.abs(mk_halt[k] - mk_plain[k]).max() < 1e-5
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_overlap_equals_serial(gpu, graph):
+    """The LB rows no IB stencil touches update on a second stream while the
+    marker pipeline runs; the result equals the serial step (the spread force
+    field sums atomics in arrival order, hence the FP32 tolerance).  A
+    platelet near a periodic face makes its stencil rows wrap."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_SHEAR, nx=64, ny=32, nz=32, elements=256, receptors=64,
+                   ligands=600)
+    sc.x[:] += 30.0  # across the periodic x face
+    sc.params.k_on = 0.2
+    res = []
+    for overlap in (False, True):
+        with pfg.Solver(sc.params) as s:
+            s.load(sc)
+            s.set_graph(graph)
+            s.set_overlap(overlap)
+            s.step(24)
+            res.append((s.f(), s.markers(), s.bonds()[0]))
+    (f0, m0, b0), (f1, m1, b1) = res
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, np.float32)[:, None]
+    assert np.abs((f0 - w) - (f1 - w)).max() < 1e-6
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.abs(m0[k] - m1[k]).max() < 1e-5, k
+    assert (b0 != b1).sum() <= 1
