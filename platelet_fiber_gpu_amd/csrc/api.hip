@@ -84,7 +84,8 @@ void free_markers(Markers &m) {
   dfree(m.fx); dfree(m.fy); dfree(m.fz);
   dfree(m.kind); dfree(m.body);
   dfree(m.plt_elem_start); dfree(m.plt_tri_start);
-  dfree(m.tri); dfree(m.tri_S0); dfree(m.tri_S); dfree(m.tri_V); dfree(m.tri_n);
+  dfree(m.tri); dfree(m.tri_S0); dfree(m.tri_S); dfree(m.tri_V);
+  dfree(m.tri_g); dfree(m.tri_a2); dfree(m.pos4);
   dfree(m.plt_S0); dfree(m.plt_V0); dfree(m.plt_S); dfree(m.plt_V);
   dfree(m.plt_lowest);
   dfree(m.m2t_start); dfree(m.m2t);
@@ -649,7 +650,7 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   std::vector<float> m2e_l0, m2l_l0;
   kv.clear();
   for (int t = 0; t < M.n_tri; ++t)
-    for (int k = 0; k < 3; ++k) kv.push_back({M.tri[3 * t + k], t});
+    for (int k = 0; k < 3; ++k) kv.push_back({M.tri[3 * t + k], 3 * t + k});
   build_csr(n, kv, m2t_start, m2t);
   kv.clear();
   for (int e = 0; e < M.n_edges; ++e) {
@@ -765,7 +766,9 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   ALLOC_UP(m.tri_S0, M.tri_S0, M.n_tri);
   ALLOC_FILL(m.tri_S, 0.0, M.n_tri);
   ALLOC_FILL(m.tri_V, 0.0, M.n_tri);
-  ALLOC_FILL(m.tri_n, 0.0f, 3 * (size_t)M.n_tri);
+  ALLOC_FILL(m.tri_g, make_float4(0.f, 0.f, 0.f, 0.f), 3 * (size_t)M.n_tri);
+  ALLOC_FILL(m.tri_a2, make_float4(0.f, 0.f, 0.f, 0.f), (size_t)M.n_tri);
+  ALLOC_FILL(m.pos4, make_float4(0.f, 0.f, 0.f, 0.f), (size_t)n);
   ALLOC_UP(m.plt_S0, M.plt_S0, M.n_platelets);
   ALLOC_UP(m.plt_V0, M.plt_V0, M.n_platelets);
   ALLOC_FILL(m.plt_S, 0.0, M.n_platelets);

@@ -268,7 +268,14 @@ __device__ void clear_tiles(const MArgs &a, int block) {
   if (my < a.ntiles && flags[my]) flags[my] = 0;
 }
 
-/* roles: A = clear last step's tiles (blocks0 blocks), B = count bins */
+/* Tag of a marker in the bin-sorted packed positions: its platelet for an
+ * element, -1 for a fibrin node (Morse acts between elements of one body). */
+__device__ inline float morse_tag(const MArgs &a, int j) {
+  return __int_as_float(a.m.kind[j] == PFG_MARKER_ELEMENT ? a.m.body[j] : -1);
+}
+
+/* roles: A = clear last step's tiles (blocks0 blocks), B = count bins and
+ * pack each marker's position with its Morse tag */
 __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
   int blk = blockIdx.x;
   if (a.roles & ROLE_A) {
@@ -283,8 +290,10 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
   if (i == 0) a.m.counters[0] = 0; /* bonds held, counted again this step */
   if (i >= a.m.n) return;
   const MLat L = lat_of(a, i);
+  const float4 p = make_float4(a.m.x[i], a.m.y[i], a.m.z[i], morse_tag(a, i));
+  a.m.pos4[i] = p;
   int b3[3];
-  bin3_of(L, a.m.x[i], a.m.y[i], a.m.z[i], b3);
+  bin3_of(L, p.x, p.y, p.z, b3);
   const int b = L.bin0 + b3[0] + L.nb[0] * (b3[1] + L.nb[1] * b3[2]);
   a.m.marker_bin[i] = b;
   a.m.marker_slot[i] = atomicAdd(&a.m.bin_count[b], 1);
@@ -379,9 +388,23 @@ __device__ void triangle_geometry(const MArgs &a, int t) {
   const double cx = (x0 + x1 + x2) / 3.0, cy = (y0 + y1 + y2) / 3.0, cz = (z0 + z1 + z2) / 3.0;
   a.m.tri_S[t] = St;
   a.m.tri_V[t] = (cx * nx + cy * ny + cz * nz) * St / 3.0; /* sem_kernel.cu:806-808 */
-  a.m.tri_n[3 * t] = (float)nx;
-  a.m.tri_n[3 * t + 1] = (float)ny;
-  a.m.tri_n[3 * t + 2] = (float)nz;
+  /* per vertex slot s: dS/dx_s = -(L_s x n)/2 with L_s = x_{s+1} - x_{s+2}
+   * and the inward n_elem (sem_kernel.cu:1688-1690, 1719-1721), and the
+   * local area strain; the volume gradient at every vertex of the triangle
+   * is (x1 - x0) x (x2 - x0) (the reference's expansion at
+   * sem_kernel.cu:1723-1731 reduces to it) */
+  const double lf = (St - a.m.tri_S0[t]) / a.m.tri_S0[t];
+  const double L[3][3] = {{x1 - x2, y1 - y2, z1 - z2},
+                          {x2 - x0, y2 - y0, z2 - z0},
+                          {x0 - x1, y0 - y1, z0 - z1}};
+  for (int s = 0; s < 3; ++s) {
+    const double gx = -0.5 * (L[s][1] * nz - L[s][2] * ny);
+    const double gy = -0.5 * (L[s][2] * nx - L[s][0] * nz);
+    const double gz = -0.5 * (L[s][0] * ny - L[s][1] * nx);
+    a.m.tri_g[3 * t + s] = make_float4((float)gx, (float)gy, (float)gz, (float)lf);
+  }
+  a.m.tri_a2[t] = make_float4((float)(ay * bz - az * by), (float)(az * bx - ax * bz),
+                              (float)(ax * by - ay * bx), 0.f);
 }
 
 /* roles: A = scatter markers into bins (blocks0), B = triangle terms (blocks1) */
@@ -570,12 +593,6 @@ __device__ void bond_decide(const MArgs &a, int r) {
   }
 }
 
-/* Tag of a marker in the bin-sorted packed positions: its platelet for an
- * element, -1 for a fibrin node (Morse acts between elements of one body). */
-__device__ inline float morse_tag(const MArgs &a, int j) {
-  return __int_as_float(a.m.kind[j] == PFG_MARKER_ELEMENT ? a.m.body[j] : -1);
-}
-
 /* roles: A = sort bins (blocks0, one lane per marker: the slot-0 marker of a
  * bin sorts it from the scatter order into bin_items and writes the packed
  * positions), B = platelet sums (blocks1 = one block per platelet),
@@ -599,10 +616,7 @@ __global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
           }
           it[j + 1] = v;
         }
-        for (int k = s; k < e; ++k) {
-          const int j = it[k];
-          a.m.sorted_pos[k] = make_float4(a.m.x[j], a.m.y[j], a.m.z[j], morse_tag(a, j));
-        }
+        for (int k = s; k < e; ++k) a.m.sorted_pos[k] = a.m.pos4[it[k]];
       }
       return;
     }
@@ -682,65 +696,99 @@ __global__ void bond_resolve_kernel(MArgs a) {
 
 /* ---------------- K8: forces ---------------- */
 
-/* Morse pairs of element i with the elements of its body in one bin, read
- * from the bin-sorted packed positions (one 16-byte load per candidate; the
- * pair with itself is the r = 0 one and is skipped). */
-__device__ inline void morse_bin(const MArgs &a, int bin, int body, float xi, float yi,
-                                 float zi, float &fx, float &fy, float &fz) {
-  const float two_rho_u0 = 2.0f * a.morse_rho * a.morse_u0;
-  const float tag = __int_as_float(body);
-  for (int k = a.m.bin_start[bin]; k < a.m.bin_start[bin + 1]; ++k) {
-    const float4 p = a.m.sorted_pos[k];
-    if (__float_as_int(p.w) != __float_as_int(tag)) continue;
-    const float dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 >= a.morse_cut2 || r2 == 0.f) continue;
-    const float r = sqrtf(r2);
-    const float e = expf(-a.morse_rho * (r - a.morse_req));
-    /* -d(U S)/dr along (x_j - x_i)/r, U = U0 (e^2 - 2e) */
-    float c = two_rho_u0 * (e - e * e);
-    const float r1 = a.morse_cut - a.morse_taper;
-    if (a.morse_taper > 0.f && r > r1) {
-      const float t = (r - r1) / a.morse_taper;
-      const float S = 1.0f - t * t * (3.0f - 2.0f * t);
-      const float dS = 6.0f * t * (t - 1.0f) / a.morse_taper;
-      c = c * S + a.morse_u0 * (e * e - 2.0f * e) * dS;
-    }
-    c /= r;
-    fx += c * dx;
-    fy += c * dy;
-    fz += c * dz;
+/* Morse force on element (xi, yi, zi) from candidate p of the bin-sorted
+ * packed positions: only elements of the same body (tag) within the cutoff;
+ * the pair with itself is the r = 0 one and is skipped. */
+__device__ inline void morse_pair(const MArgs &a, float4 p, float tag, float xi, float yi,
+                                  float zi, float &fx, float &fy, float &fz) {
+  if (__float_as_int(p.w) != __float_as_int(tag)) return;
+  const float dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  if (r2 >= a.morse_cut2 || r2 == 0.f) return;
+  const float r = sqrtf(r2);
+  const float e = expf(-a.morse_rho * (r - a.morse_req));
+  /* -d(U S)/dr along (x_j - x_i)/r, U = U0 (e^2 - 2e) */
+  float c = 2.0f * a.morse_rho * a.morse_u0 * (e - e * e);
+  const float r1 = a.morse_cut - a.morse_taper;
+  if (a.morse_taper > 0.f && r > r1) {
+    const float t = (r - r1) / a.morse_taper;
+    const float S = 1.0f - t * t * (3.0f - 2.0f * t);
+    const float dS = 6.0f * t * (t - 1.0f) / a.morse_taper;
+    c = c * S + a.morse_u0 * (e * e - 2.0f * e) * dS;
   }
+  c /= r;
+  fx += c * dx;
+  fy += c * dy;
+  fz += c * dz;
 }
 
-/* area and volume constraint force on vertex i from incident triangle t */
-__device__ inline void area_volume_term(const MArgs &a, int t, int i, float xi, float yi,
-                                        float zi, float g_area, float cvol, float &fx,
-                                        float &fy, float &fz) {
-  const int n0 = a.m.tri[3 * t], n1 = a.m.tri[3 * t + 1], n2 = a.m.tri[3 * t + 2];
-  const int s = n0 == i ? 0 : (n1 == i ? 1 : 2);
-  const int s1 = s == 0 ? n1 : (s == 1 ? n2 : n0);
-  const int s2 = s == 0 ? n2 : (s == 1 ? n0 : n1);
-  /* L_s = x_{s+1} - x_{s+2} (sem_kernel.cu:1688-1690) */
-  const float Lx = a.m.x[s1] - a.m.x[s2], Ly = a.m.y[s1] - a.m.y[s2], Lz = a.m.z[s1] - a.m.z[s2];
-  const float nx = a.m.tri_n[3 * t], ny = a.m.tri_n[3 * t + 1], nz = a.m.tri_n[3 * t + 2];
-  /* dS/dx_s = -(L_s x n)/2 with the inward n_elem (sem_kernel.cu:1719-1721) */
-  const float gx = -0.5f * (Ly * nz - Lz * ny);
-  const float gy = -0.5f * (Lz * nx - Lx * nz);
-  const float gz = -0.5f * (Lx * ny - Ly * nx);
-  const double St = a.m.tri_S[t], S0t = a.m.tri_S0[t];
-  const float ca = -a.ks * ((float)((St - S0t) / S0t) + g_area);
-  fx += ca * gx;
-  fy += ca * gy;
-  fz += ca * gz;
-  /* dV/dx_s = x_{s+1} x x_{s+2} (the reference's expansion at
-   * sem_kernel.cu:1723-1731 reduces to this), taken about x_i: the sum over a
-   * closed one-ring is unchanged by the shift and stays accurate in FP32 */
-  const float ax = a.m.x[s1] - xi, ay = a.m.y[s1] - yi, az = a.m.z[s1] - zi;
-  const float bx = a.m.x[s2] - xi, by = a.m.y[s2] - yi, bz = a.m.z[s2] - zi;
-  fx += cvol * (ay * bz - az * by);
-  fy += cvol * (az * bx - ax * bz);
-  fz += cvol * (ax * by - ay * bx);
+/* The neighbouring bins of a marker as ranges of the bin-sorted order: bins
+ * are numbered x fastest, so the (up to 3) x-neighbours of one (y, z) pair
+ * are one range unless a periodic wrap splits them in two; at most 2 x 3 x 3
+ * ranges. */
+constexpr int MORSE_RANGES = 18;
+
+/* Neighbour bins of bin b along an axis of nb bins: the j-th of *count
+ * (the same set as nbr_bins, without an indexed array). */
+__device__ inline int nbr_bin_j(int b, int nb, int periodic, int j, int *count) {
+  if (periodic && nb <= 3) {
+    *count = nb;
+    return j;
+  }
+  if (periodic) {
+    *count = 3;
+    const int v = b - 1 + j;
+    return v < 0 ? v + nb : (v >= nb ? v - nb : v);
+  }
+  const int lo = max(b - 1, 0), hi = min(b + 1, nb - 1);
+  *count = hi - lo + 1;
+  return lo + j;
+}
+
+/* Lane r < the returned count gets range r as [*s, *e) of the sorted order. */
+__device__ inline int morse_range(const MArgs &a, const MLat &L, const int b3[3], int r,
+                                  int *s, int *e) {
+  /* x-runs of consecutive bins: two when a periodic wrap splits the three */
+  const int bx = b3[0], nbx = L.nb[0];
+  int runs = 1, f0, l0, f1 = 0, l1 = 0;
+  if (L.periodic[0] && nbx <= 3) {
+    f0 = 0;
+    l0 = nbx - 1;
+  } else if (L.periodic[0] && (bx == 0 || bx == nbx - 1)) {
+    runs = 2;
+    f0 = bx == 0 ? nbx - 1 : nbx - 2;
+    l0 = nbx - 1;
+    l1 = bx == 0 ? 1 : 0;
+  } else {
+    f0 = max(bx - 1, 0);
+    l0 = min(bx + 1, nbx - 1);
+  }
+  int cy, cz;
+  nbr_bin_j(b3[1], L.nb[1], L.periodic[1], 0, &cy);
+  nbr_bin_j(b3[2], L.nb[2], L.periodic[2], 0, &cz);
+  const int n = runs * cy * cz;
+  if (r < n) {
+    const int run = r % runs, jy = (r / runs) % cy, jz = r / (runs * cy);
+    const int y = nbr_bin_j(b3[1], L.nb[1], L.periodic[1], jy, &cy);
+    const int z = nbr_bin_j(b3[2], L.nb[2], L.periodic[2], jz, &cz);
+    const int row = L.bin0 + L.nb[0] * (y + L.nb[1] * z);
+    *s = a.m.bin_start[row + (run == 0 ? f0 : f1)];
+    *e = a.m.bin_start[row + (run == 0 ? l0 : l1) + 1];
+  }
+  return n;
+}
+
+/* Area and volume constraint force on the vertex at slot e % 3 of triangle
+ * e / 3 (sem_ForceAreaVol_kernel, sem_kernel.cu:1640-1749), from the
+ * per-triangle terms of this step. */
+__device__ inline void area_volume_term(const MArgs &a, int e, float g_area, float cvol,
+                                        float &fx, float &fy, float &fz) {
+  const float4 g = a.m.tri_g[e];
+  const float4 v = a.m.tri_a2[e / 3];
+  const float ca = -a.ks * (g.w + g_area);
+  fx += ca * g.x + cvol * v.x;
+  fy += ca * g.y + cvol * v.y;
+  fz += ca * g.z + cvol * v.z;
 }
 
 __device__ inline float link_tension(const MArgs &a, float l, float l0) {
@@ -755,8 +803,9 @@ __device__ inline float link_tension(const MArgs &a, float l, float l0) {
   return (a.fiber_k / l0) * (0.25f / (tl * tl) - 0.25f + x);
 }
 
-/* One wavefront per marker: lanes 0..26 take the neighbouring bins (Morse),
- * all lanes stride over the marker's CSR lists (triangles, edges, receptor
+/* One wavefront per marker: the lanes stride over the Morse candidates of
+ * the neighbouring bins (consecutive lanes read consecutive packed
+ * positions), then over the marker's CSR lists (triangles, edges, receptor
  * shares, fibrin links); a butterfly reduction sums the lanes. */
 __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
   const int lane = threadIdx.x & 63;
@@ -771,19 +820,39 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
       }
     }
   }
+  __shared__ int sh_off[4][MORSE_RANGES], sh_end[4][MORSE_RANGES];
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= a.m.n) return;
   const int kind = a.m.kind[i];
-  const float xi = a.m.x[i], yi = a.m.y[i], zi = a.m.z[i];
+  const float4 pi = a.m.pos4[i];
+  const float xi = pi.x, yi = pi.y, zi = pi.z;
   float fx = 0.f, fy = 0.f, fz = 0.f;
   if (kind == PFG_MARKER_ELEMENT) {
     const int body = a.m.body[i];
-    if (a.morse_u0 != 0.f && lane < 27) {
+    if (a.morse_u0 != 0.f) {
+      /* the candidates of all neighbour ranges as one list over the lanes:
+       * lane r writes range r's offset and end into this wave's LDS table */
       const MLat L = lat_of(a, i);
       const int b = a.m.marker_bin[i] - L.bin0;
       const int b3[3] = {b % L.nb[0], (b / L.nb[0]) % L.nb[1], b / (L.nb[0] * L.nb[1])};
-      const int bin = nth_nbr_bin(L, b3, lane);
-      if (bin >= 0) morse_bin(a, bin, body, xi, yi, zi, fx, fy, fz);
+      int s = 0, e = 0;
+      const int nr = morse_range(a, L, b3, lane, &s, &e);
+      const int end = wave_incl_scan(lane < nr ? e - s : 0);
+      const int total = __shfl(end, 63, 64);
+      int *off = sh_off[threadIdx.x >> 6], *ends = sh_end[threadIdx.x >> 6];
+      if (lane < nr) {
+        off[lane] = s - (end - (e - s)); /* sorted index = candidate + off */
+        ends[lane] = end;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const float tag = pi.w;
+      int r = 0;
+      for (int c = lane; c < total; c += 64) {
+        while (c >= ends[r]) ++r;
+        morse_pair(a, a.m.sorted_pos[c + off[r]], tag, xi, yi, zi, fx, fy, fz);
+      }
     }
     const int t0 = a.m.m2t_start[i], nt = (a.ks != 0.f || a.kv != 0.f) ? a.m.m2t_start[i + 1] - t0 : 0;
     const int e0 = a.m.m2e_start[i], ne = a.k_spring != 0.f ? a.m.m2e_start[i + 1] - e0 : 0;
@@ -797,12 +866,12 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
     }
     for (int k = lane; k < nt + ne + nr; k += 64) {
       if (k < nt) {
-        area_volume_term(a, a.m.m2t[t0 + k], i, xi, yi, zi, g_area, cvol, fx, fy, fz);
+        area_volume_term(a, a.m.m2t[t0 + k], g_area, cvol, fx, fy, fz);
       } else if (k < nt + ne) {
         /* mesh-edge Hookean spring k (r - L)/r (x_j - x_i) (sem_kernel.cu:374-384) */
         const int q = e0 + k - nt;
-        const int j = a.m.m2e_nbr[q];
-        const float dx = a.m.x[j] - xi, dy = a.m.y[j] - yi, dz = a.m.z[j] - zi;
+        const float4 pj = a.m.pos4[a.m.m2e_nbr[q]];
+        const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
         const float r = sqrtf(dx * dx + dy * dy + dz * dz);
         if (r > 0.f) {
           const float c = a.k_spring * (r - a.m.m2e_l0[q]) / r;
@@ -834,9 +903,9 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
     const float sgn = a.fiber_ref_sign ? -1.0f : 1.0f;
     const int l0i = a.m.m2l_start[i], nl = a.m.m2l_start[i + 1] - l0i;
     for (int k = lane; k < nl; k += 64) {
-      const int j = a.m.m2l_nbr[l0i + k];
+      const float4 pj = a.m.pos4[a.m.m2l_nbr[l0i + k]];
       const float l0 = a.m.m2l_l0[l0i + k];
-      const float dx = a.m.x[j] - xi, dy = a.m.y[j] - yi, dz = a.m.z[j] - zi;
+      const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
       const float l = sqrtf(dx * dx + dy * dy + dz * dz);
       if (l == 0.f || l0 == 0.f) continue;
       const float c = sgn * link_tension(a, l, l0) / l; /* restoring when sgn = 1 */

@@ -83,6 +83,7 @@ struct Markers {
   int n = 0;
   int32_t *minst = nullptr;      /* lattice of each marker (batches only)    */
   float *x = nullptr, *y = nullptr, *z = nullptr;
+  float4 *pos4 = nullptr;        /* x, y, z, Morse tag per marker, this step */
   float *vx = nullptr, *vy = nullptr, *vz = nullptr;
   float *fx = nullptr, *fy = nullptr, *fz = nullptr;
   int32_t *kind = nullptr, *body = nullptr;
@@ -92,10 +93,13 @@ struct Markers {
   int n_tri = 0;
   int32_t *tri = nullptr;        /* 3*n_tri */
   double *tri_S0 = nullptr, *tri_S = nullptr, *tri_V = nullptr;
-  float *tri_n = nullptr;        /* 3*n_tri unit normals (reference sign)   */
+  /* per (triangle, vertex slot) 3t+s: area gradient -(L_s x n)/2 and the
+   * local area strain (S - S0)/S0; per triangle: (x1-x0) x (x2-x0), the
+   * volume gradient at any of its vertices (written each step) */
+  float4 *tri_g = nullptr, *tri_a2 = nullptr;
   double *plt_S0 = nullptr, *plt_V0 = nullptr, *plt_S = nullptr, *plt_V = nullptr;
   int32_t *plt_lowest = nullptr; /* lowest element per platelet             */
-  /* CSR marker -> incident triangles (value = triangle index) */
+  /* CSR marker -> incident triangles (value = 3 * triangle + vertex slot) */
   int32_t *m2t_start = nullptr, *m2t = nullptr;
   /* CSR marker -> mesh-edge partners with rest length */
   int n_edges = 0;
