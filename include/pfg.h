@@ -199,7 +199,9 @@ typedef enum pfg_stage {
   PFG_STAGE_LB = 5,        /* fused stream + bounce-back + moments + BGK/Guo      */
   PFG_STAGE_INTERP = 6,    /* IB velocity interpolation to markers                */
   PFG_STAGE_UPDATE = 7,    /* marker position update                              */
-  PFG_STAGE_CLEAR = 8,     /* clear the force field tiles touched this step       */
+  PFG_STAGE_CLEAR = 8,     /* drop a spread no LB stage consumed: zero the force
+                              field tiles this step touched (a coupled step never
+                              needs it: the LB stage zeroes the force it reads)  */
   PFG_STAGE_COUNT = 9
 } pfg_stage;
 
@@ -271,8 +273,8 @@ pfg_status pfg_upload_bonds(pfg_handle *h, const int32_t *rec_bond);
 pfg_status pfg_cell_grid(const pfg_handle *h, int32_t dims[3]);
 pfg_status pfg_download_cells(pfg_handle *h, int32_t *bin_start,
                               int32_t *bin_items);
-/* Lattice force field [3][N] as seen by the LB stage (valid between SPREAD and
- * LB) and velocity field written for IB (valid between LB and CLEAR). */
+/* Lattice force field [3][N] as the LB stage will read it: valid between the
+ * SPREAD and LB stages (the LB stage reads it and leaves zeros). */
 pfg_status pfg_download_aux(pfg_handle *h, float *a3n);
 
 /* ---- stepping ---- */

@@ -95,7 +95,7 @@ void free_markers(Markers &m) {
   dfree(m.rec_force); dfree(m.rec_w);
   dfree(m.m2r_start); dfree(m.m2r);
   dfree(m.lig_x); dfree(m.lig_y); dfree(m.lig_z); dfree(m.lig_owner); dfree(m.lig_claim);
-  dfree(m.lg_start); dfree(m.lg_items);
+  dfree(m.lg_start); dfree(m.lg_items); dfree(m.lg_pos); dfree(m.rec_pos);
   dfree(m.m2l_start); dfree(m.m2l_nbr); dfree(m.m2l_l0);
   dfree(m.bin_count); dfree(m.bin_start); dfree(m.bin_items);
   dfree(m.marker_bin); dfree(m.marker_slot); dfree(m.scan_partials);
@@ -331,20 +331,28 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
     }
     if (st != PFG_OK) break;
     if (dalloc(h, &h->aux, (size_t)3 * h->plane) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (dalloc(h, &h->vel, (size_t)3 * h->plane) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     /* row stamps start at -1: no row is touched before the first step */
     if (dalloc(h, &h->row_stamp, (size_t)fblk) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->row_stamp, 0xff, sizeof(int32_t) * (size_t)fblk) != hipSuccess) {
       st = PFG_ERR_DEVICE;
       break;
     }
-    if (hipMemset(h->aux, 0, sizeof(float) * 3 * h->plane) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    if (hipMemset(h->aux, 0, sizeof(float) * 3 * h->plane) != hipSuccess ||
+        hipMemset(h->vel, 0, sizeof(float) * 3 * h->plane) != hipSuccess) {
+      st = PFG_ERR_DEVICE;
+      break;
+    }
     h->tnx = h->inst[0].tnx;
     h->tny = h->inst[0].tny;
     h->tnz = h->inst[0].tnz;
     h->ntiles = tile;
-    const size_t flag_bytes = (size_t)(h->ntiles + 3) / 4 * 4;
-    if (dalloc(h, &h->tile_flag, flag_bytes) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
-    if (hipMemset(h->tile_flag, 0, flag_bytes) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    /* tile stamps start at -1: no tile is touched before the first step */
+    if (dalloc(h, &h->tile_stamp, (size_t)h->ntiles) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (hipMemset(h->tile_stamp, 0xff, sizeof(int32_t) * (size_t)h->ntiles) != hipSuccess) {
+      st = PFG_ERR_DEVICE;
+      break;
+    }
     int64_t nmax = 0;
     for (const Inst &I : h->inst) nmax = std::max(nmax, I.N);
     h->red_blocks = std::min<int64_t>(2048, (nmax + 255) / 256);
@@ -482,7 +490,8 @@ pfg_status pfg_destroy(pfg_handle *h) {
     dfree(h->d_lbargs[k]);
   }
   dfree(h->aux);
-  dfree(h->tile_flag);
+  dfree(h->vel);
+  dfree(h->tile_stamp);
   dfree(h->solid);
   dfree(h->bbmask);
   dfree(h->red);
@@ -702,7 +711,7 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   if (nbins > (int64_t)INT_MAX / 2) return PFG_ERR_ARG;
   for (int k = 0; k < 3; ++k) m.nb[k] = h->inst[0].nb[k];
   m.nbins = nbins;
-  m.scan_blocks = (m.nbins + 1023) / 1024;
+  m.scan_blocks = (m.nbins + SCAN_SPAN - 1) / SCAN_SPAN;
   /* static ligands on the same bin grid, ascending ligand id per bin */
   std::vector<int32_t> lg_start, lg_items;
   std::vector<float> lig_z_host((size_t)M.n_wall_ligands, p.wall_z);
@@ -796,10 +805,21 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   ALLOC_FILL(m.lig_claim, (int32_t)INT_MAX, m.n_lig);
   ALLOC_UP(m.lg_start, lg_start.data(), lg_start.size());
   ALLOC_UP(m.lg_items, lg_items.data(), lg_items.size());
+  {
+    std::vector<float4> lg_pos(lg_items.size());
+    for (size_t k = 0; k < lg_items.size(); ++k) {
+      const int l = lg_items[k];
+      float tag;
+      std::memcpy(&tag, &l, sizeof(tag)); /* the index as the bits of .w */
+      lg_pos[k] = make_float4(M.lig_x[l], M.lig_y[l], lig_z_host[(size_t)l], tag);
+    }
+    ALLOC_UP(m.lg_pos, lg_pos.data(), lg_pos.size());
+  }
+  ALLOC_FILL(m.rec_pos, make_float4(0.f, 0.f, 0.f, 0.f), m.n_rec);
   ALLOC_UP(m.m2l_start, m2l_start.data(), m2l_start.size());
   ALLOC_UP(m.m2l_nbr, m2l_nbr.data(), m2l_nbr.size());
   ALLOC_UP(m.m2l_l0, m2l_l0.data(), m2l_l0.size());
-  ALLOC_FILL(m.bin_count, 0, m.nbins);
+  ALLOC_FILL(m.bin_count, 0, m.scan_blocks * SCAN_SPAN); /* zero padding to whole scan blocks */
   ALLOC_FILL(m.bin_start, 0, m.nbins + 1);
   ALLOC_FILL(m.bin_items, 0, n);
   ALLOC_FILL(m.marker_bin, 0, n);
@@ -820,6 +840,9 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   PFG_TRY(upload(h, m.counters, zeros, 8));
   PFG_TRY(upload(h, m.detach_step, &minus1, 1));
   PFG_TRY(upload(h, m.d_step, zeros, 1));
+  /* the step index starts again: no tile carries a stamp or a force */
+  PFG_TRY(hipMemset(h->tile_stamp, 0xff, sizeof(int32_t) * (size_t)h->ntiles));
+  PFG_TRY(hipMemset(h->aux, 0, sizeof(float) * 3 * h->plane));
   if (h->K > 1) {
     std::vector<MLat> lats((size_t)h->K);
     for (int k = 0; k < h->K; ++k) fill_mlat(h, k, &lats[(size_t)k]);
@@ -1175,8 +1198,8 @@ static hipError_t run_stage(pfg_handle *h, int s) {
 /* One coupled step, in the order of the reference's per-step calls
  * (gpuSEM_LB.c:1441-1442 -> semLB.cu:441-531 then semLB.cu:303-438):
  * forces from X(n), spread, LB with F(n), interpolate, move.  The stages are
- * fused into nine launches; the previous step's force tiles are cleared by
- * the first of them. */
+ * fused into nine launches; the LB update zeroes the force it reads, so the
+ * next step's spreading adds into zeros without a clearing pass. */
 /* One coupled step.  With overlap, the first marker launch stamps the rows
  * the IB stencils touch; the LB update of all other rows then runs on
  * stream2 while the marker pipeline runs on the main stream, and the
@@ -1187,7 +1210,7 @@ static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
   const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
   const bool overlap = markers && h->overlap && h->row_stamp && !lb0 && !lb1;
   if (markers) {
-    e = launch_begin(h, true, true);
+    e = launch_begin(h, false, true);
     if (overlap) {
       if (e == hipSuccess) e = hipEventRecord(h->ev_fork, h->stream);
       if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->ev_fork, 0);

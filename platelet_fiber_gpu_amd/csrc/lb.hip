@@ -110,9 +110,11 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
   const uint32_t rows = (uint32_t)a.nxb * (uint32_t)a.ny;
   const uint32_t zc = task / rows;
   if (zc * CPT >= (uint32_t)a.nz) return;
+  /* IB force tiles and rows are stamped with the step that touches them */
+  const int32_t step32 = MODE == LB_MODE_FIELDS ? 0 : (int32_t)*a.step;
   if (MODE == LB_MODE_BULK || MODE == LB_MODE_REGION) {
     static_assert(MODE < LB_MODE_BULK || CPT == 1, "row modes update one z-row per wave");
-    const bool touched = a.row_stamp[task] == (int32_t)*a.step; /* task = x-block row */
+    const bool touched = a.row_stamp[task] == step32; /* task = x-block row */
     if (touched != (MODE == LB_MODE_REGION)) return;
   }
   const uint32_t rem = task - zc * rows;
@@ -215,11 +217,14 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
     bool flagged = false;
     if (MODE == LB_MODE_STEP || MODE == LB_MODE_REGION) {
       const int tile = (x >> TILE_SHIFT_X) + a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
-      flagged = a.tile_flag[tile] != 0;
+      flagged = a.tile_stamp[tile] == step32;
       if (flagged) {
         Fx += a.aux[idx];
         Fy += a.aux[a.plane + idx];
         Fz += a.aux[2 * a.plane + idx];
+        a.aux[idx] = 0.f; /* consumed: the next step's spreading adds into zeros */
+        a.aux[a.plane + idx] = 0.f;
+        a.aux[2 * a.plane + idx] = 0.f;
       }
     }
     const float inv_rho = __builtin_amdgcn_rcpf(rho); /* hardware reciprocal, 1 ulp */
@@ -235,9 +240,9 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
       continue;
     }
     if (flagged) { /* velocity for the IB interpolation of this step */
-      a.aux[idx] = ux;
-      a.aux[a.plane + idx] = uy;
-      a.aux[2 * a.plane + idx] = uz;
+      a.vel[idx] = ux;
+      a.vel[a.plane + idx] = uy;
+      a.vel[2 * a.plane + idx] = uz;
     }
 
     /* BGK with Guo forcing, in deviation form and storage units:
@@ -427,7 +432,8 @@ void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a) {
   a->fin = (const char *)h->f[cur] + foff;
   a->fout = (char *)h->f[1 - cur] + foff;
   a->aux = h->aux + L.cell0;
-  a->tile_flag = h->tile_flag + L.tile0;
+  a->vel = h->vel + L.cell0;
+  a->tile_stamp = h->tile_stamp + L.tile0;
   a->solid = h->solid ? h->solid + L.cell0 : nullptr;
   a->bbmask = h->bbmask ? h->bbmask + L.cell0 : nullptr;
   a->halt = h->halt_on_detach ? h->m.counters + 1 : nullptr;

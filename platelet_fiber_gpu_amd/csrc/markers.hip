@@ -238,14 +238,18 @@ __device__ inline int lat_of_tile(const MArgs &a, int64_t g) {
   return lo;
 }
 
+/* Zero the force of the tiles that this step's spreading touched and take
+ * their stamps back (a spread that no LB update will consume). */
 __device__ void clear_tiles(const MArgs &a, int block) {
   __shared__ int64_t list[256];
   __shared__ int count;
   if (threadIdx.x == 0) count = 0;
   __syncthreads();
-  uint8_t *flags = a.lats ? a.lats[0].tile_flag : a.lat.tile_flag; /* all lattices' tiles */
+  int32_t *stamps = a.lats ? a.lats[0].tile_stamp : a.lat.tile_stamp; /* all lattices' tiles */
+  const int32_t step = (int32_t)*a.m.d_step;
   const int64_t my = (int64_t)block * 256 + threadIdx.x;
-  if (my < a.ntiles && flags[my]) list[atomicAdd(&count, 1)] = my;
+  const bool mine = my < a.ntiles && stamps[my] == step;
+  if (mine) list[atomicAdd(&count, 1)] = my;
   __syncthreads();
   const int t = threadIdx.x;
   const int lx = t & (TILE_X - 1);
@@ -265,7 +269,20 @@ __device__ void clear_tiles(const MArgs &a, int block) {
       L.aux[2 * a.plane + c] = 0.f;
     }
   }
-  if (my < a.ntiles && flags[my]) flags[my] = 0;
+  if (mine) stamps[my] = -1;
+}
+
+__device__ inline void receptor_pos(const MArgs &a, int r, float out[3], int nodes[3]) {
+  const int t = a.m.rec_tri[r];
+  const float r1 = a.m.rec_r1[r], r2 = a.m.rec_r2[r];
+  nodes[0] = a.m.tri[3 * t];
+  nodes[1] = a.m.tri[3 * t + 1];
+  nodes[2] = a.m.tri[3 * t + 2];
+  /* (1-r1) x0 + r1 (1-r2) x1 + r1 r2 x2 (sem_kernel.cu:1811, 1840-1841) */
+  const float c0 = 1.0f - r1, c1 = r1 * (1.0f - r2), c2 = r1 * r2;
+  out[0] = c0 * a.m.x[nodes[0]] + c1 * a.m.x[nodes[1]] + c2 * a.m.x[nodes[2]];
+  out[1] = c0 * a.m.y[nodes[0]] + c1 * a.m.y[nodes[1]] + c2 * a.m.y[nodes[2]];
+  out[2] = c0 * a.m.z[nodes[0]] + c1 * a.m.z[nodes[1]] + c2 * a.m.z[nodes[2]];
 }
 
 /* Tag of a marker in the bin-sorted packed positions: its platelet for an
@@ -274,8 +291,9 @@ __device__ inline float morse_tag(const MArgs &a, int j) {
   return __int_as_float(a.m.kind[j] == PFG_MARKER_ELEMENT ? a.m.body[j] : -1);
 }
 
-/* roles: A = clear last step's tiles (blocks0 blocks), B = count bins and
- * pack each marker's position with its Morse tag */
+/* roles: A = clear this step's force tiles (blocks0 blocks, the CLEAR stage
+ * only), B = count bins and pack each marker's position with its Morse tag
+ * (blocks1), C = receptor positions (blocks2) */
 __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
   int blk = blockIdx.x;
   if (a.roles & ROLE_A) {
@@ -284,6 +302,19 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
       return;
     }
     blk -= a.blocks0;
+  }
+  if (a.roles & ROLE_C) {
+    if (!(a.roles & ROLE_B) || blk >= a.blocks1) {
+      const int r = (blk - ((a.roles & ROLE_B) ? a.blocks1 : 0)) * 256 + threadIdx.x;
+      if (r < a.m.n_rec) {
+        float rp[3];
+        int nodes[3];
+        receptor_pos(a, r, rp, nodes);
+        const int lat = a.lats ? a.m.minst[nodes[0]] : 0;
+        a.m.rec_pos[r] = make_float4(rp[0], rp[1], rp[2], __int_as_float(lat));
+      }
+      return;
+    }
   }
   if (!(a.roles & ROLE_B)) return;
   const int i = blk * 256 + threadIdx.x;
@@ -325,48 +356,80 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
 
 /* ---------------- K2..K4: prefix scan of the bin counts ---------------- */
 
-__global__ __launch_bounds__(256) void scan_reduce_kernel(const int32_t *count, int64_t n,
+/* Two launches: block sums of SCAN_SPAN bins each, then per block the sum of
+ * the preceding block sums (at most a few hundred) and the scan of its own
+ * bins.  bin_count is padded with zeros to whole blocks. */
+__device__ inline int block_sum(int v, int *sh_wave) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh_wave[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int t = sh_wave[0] + sh_wave[1] + sh_wave[2] + sh_wave[3];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(256) void scan_reduce_kernel(const int32_t *count,
                                                           int32_t *partials) {
   __shared__ int sh[4];
-  const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x * 4;
-  int v[4];
-  for (int k = 0; k < 4; ++k) v[k] = base + k < n ? count[base + k] : 0;
-  const int total = block_excl_scan4(v, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(256) void scan_top_kernel(int32_t *partials, int64_t nblocks,
-                                                       int32_t *bin_start, int64_t nbins) {
-  __shared__ int sh[4];
-  int carry = 0;
-  for (int64_t off = 0; off < nblocks; off += 1024) {
-    int v[4];
-    const int64_t base = off + threadIdx.x * 4;
-    for (int k = 0; k < 4; ++k) v[k] = base + k < nblocks ? partials[base + k] : 0;
-    const int total = block_excl_scan4(v, sh);
-    for (int k = 0; k < 4; ++k)
-      if (base + k < nblocks) partials[base + k] = v[k] + carry;
-    carry += total;
-    __syncthreads();
+  const int4 *c = reinterpret_cast<const int4 *>(count + (int64_t)blockIdx.x * SCAN_SPAN);
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_SPAN / 1024; ++k) {
+    const int4 v = c[k * 256 + threadIdx.x];
+    s += v.x + v.y + v.z + v.w;
   }
-  if (threadIdx.x == 0) bin_start[nbins] = carry;
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(256) void scan_apply_kernel(int32_t *count, int64_t n,
+/* Each lane scans 32 consecutive bins of the block in registers; one block
+ * scan of the lane totals places them. */
+__global__ __launch_bounds__(256) void scan_apply_kernel(int32_t *count, int64_t nbins,
                                                          const int32_t *partials,
                                                          int32_t *bin_start) {
+  static_assert(SCAN_SPAN == 256 * 32, "32 bins per lane");
   __shared__ int sh[4];
-  const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x * 4;
-  int v[4];
-  for (int k = 0; k < 4; ++k) v[k] = base + k < n ? count[base + k] : 0;
-  block_excl_scan4(v, sh);
-  const int off = partials[blockIdx.x];
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) {
-      bin_start[base + k] = v[k] + off;
-      count[base + k] = 0; /* ready for the next step */
-    }
+  int off = 0;
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += 256) off += partials[k];
+  off = block_sum(off, sh);
+  const int64_t base = (int64_t)blockIdx.x * SCAN_SPAN + threadIdx.x * 32;
+  int4 *c = reinterpret_cast<int4 *>(count + base);
+  int v[32];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int4 q = c[k];
+    v[4 * k] = q.x;
+    v[4 * k + 1] = q.y;
+    v[4 * k + 2] = q.z;
+    v[4 * k + 3] = q.w;
   }
+  int run = 0;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    const int t = v[k];
+    v[k] = run;
+    run += t;
+  }
+  const int incl = wave_incl_scan(run);
+  if ((threadIdx.x & 63) == 63) sh[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  int lane_off = off + incl - run;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) lane_off += sh[w];
+  if (base + 32 <= nbins) {
+    int4 *o = reinterpret_cast<int4 *>(bin_start + base);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      o[k] = make_int4(v[4 * k] + lane_off, v[4 * k + 1] + lane_off, v[4 * k + 2] + lane_off,
+                       v[4 * k + 3] + lane_off);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 32; ++k)
+      if (base + k < nbins) bin_start[base + k] = v[k] + lane_off;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) c[k] = make_int4(0, 0, 0, 0); /* ready for the next step */
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) bin_start[nbins] = lane_off + run;
 }
 
 /* ---------------- K5: scatter + triangle geometry ---------------- */
@@ -429,19 +492,26 @@ __global__ __launch_bounds__(256) void place_kernel(MArgs a) {
 
 /* ---------------- K6: sort bins + platelet sums + bond decisions ---------------- */
 
+/* Platelet p's area and volume (sums of the triangle terms) and its lowest
+ * element (ties to the lower index); one block, the loads of four strides
+ * issued together. */
 __device__ void platelet_sums(const MArgs &a, int p) {
   const int t0 = a.m.plt_tri_start[p], t1 = a.m.plt_tri_start[p + 1];
-  double S = 0.0, V = 0.0;
-  for (int t = t0 + threadIdx.x; t < t1; t += 256) {
-    S += a.m.tri_S[t];
-    V += a.m.tri_V[t];
-  }
-  __shared__ double shS[256], shV[256];
-  __shared__ float shz[256];
-  __shared__ int shi[256];
-  shS[threadIdx.x] = S;
-  shV[threadIdx.x] = V;
   const int e0 = a.m.plt_elem_start[p], e1 = a.m.plt_elem_start[p + 1];
+  double S[4] = {0.0, 0.0, 0.0, 0.0}, V[4] = {0.0, 0.0, 0.0, 0.0};
+  int t = t0 + threadIdx.x;
+  for (; t + 768 < t1; t += 1024) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      S[k] += a.m.tri_S[t + 256 * k];
+      V[k] += a.m.tri_V[t + 256 * k];
+    }
+  }
+  for (; t < t1; t += 256) {
+    S[0] += a.m.tri_S[t];
+    V[0] += a.m.tri_V[t];
+  }
+  double Ssum = (S[0] + S[1]) + (S[2] + S[3]), Vsum = (V[0] + V[1]) + (V[2] + V[3]);
   float zmin = INFINITY;
   int imin = INT_MAX;
   for (int e = e0 + threadIdx.x; e < e1; e += 256) {
@@ -451,40 +521,117 @@ __device__ void platelet_sums(const MArgs &a, int p) {
       imin = e;
     }
   }
-  shz[threadIdx.x] = zmin;
-  shi[threadIdx.x] = imin;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    Ssum += __shfl_xor(Ssum, o, 64);
+    Vsum += __shfl_xor(Vsum, o, 64);
+    const float z2 = __shfl_xor(zmin, o, 64);
+    const int i2 = __shfl_xor(imin, o, 64);
+    if (z2 < zmin || (z2 == zmin && i2 < imin)) {
+      zmin = z2;
+      imin = i2;
+    }
+  }
+  __shared__ double shS[4], shV[4];
+  __shared__ float shz[4];
+  __shared__ int shi[4];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    shS[w] = Ssum;
+    shV[w] = Vsum;
+    shz[w] = zmin;
+    shi[w] = imin;
+  }
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      shS[threadIdx.x] += shS[threadIdx.x + s];
-      shV[threadIdx.x] += shV[threadIdx.x + s];
-      const float z2 = shz[threadIdx.x + s];
-      const int i2 = shi[threadIdx.x + s];
-      if (z2 < shz[threadIdx.x] || (z2 == shz[threadIdx.x] && i2 < shi[threadIdx.x])) {
-        shz[threadIdx.x] = z2;
-        shi[threadIdx.x] = i2;
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      if (shz[k] < shz[0] || (shz[k] == shz[0] && shi[k] < shi[0])) {
+        shz[0] = shz[k];
+        shi[0] = shi[k];
       }
     }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    a.m.plt_S[p] = shS[0];
-    a.m.plt_V[p] = shV[0];
+    a.m.plt_S[p] = (shS[0] + shS[1]) + (shS[2] + shS[3]);
+    a.m.plt_V[p] = (shV[0] + shV[1]) + (shV[2] + shV[3]);
     a.m.plt_lowest[p] = shi[0] == INT_MAX ? -1 : shi[0];
   }
 }
 
-__device__ inline void receptor_pos(const MArgs &a, int r, float out[3], int nodes[3]) {
-  const int t = a.m.rec_tri[r];
-  const float r1 = a.m.rec_r1[r], r2 = a.m.rec_r2[r];
-  nodes[0] = a.m.tri[3 * t];
-  nodes[1] = a.m.tri[3 * t + 1];
-  nodes[2] = a.m.tri[3 * t + 2];
-  /* (1-r1) x0 + r1 (1-r2) x1 + r1 r2 x2 (sem_kernel.cu:1811, 1840-1841) */
-  const float c0 = 1.0f - r1, c1 = r1 * (1.0f - r2), c2 = r1 * r2;
-  out[0] = c0 * a.m.x[nodes[0]] + c1 * a.m.x[nodes[1]] + c2 * a.m.x[nodes[2]];
-  out[1] = c0 * a.m.y[nodes[0]] + c1 * a.m.y[nodes[1]] + c2 * a.m.y[nodes[2]];
-  out[2] = c0 * a.m.z[nodes[0]] + c1 * a.m.z[nodes[1]] + c2 * a.m.z[nodes[2]];
+/* The neighbouring bins of a point as ranges of a bin-ordered array with
+ * per-bin starts: bins are numbered x fastest, so the (up to 3) x-neighbours
+ * of one (y, z) pair are one range unless a periodic wrap splits them in
+ * two; at most 2 x 3 x 3 ranges. */
+constexpr int NBR_RANGES = 18;
+
+/* Neighbour bins of bin b along an axis of nb bins: the j-th of *count
+ * (the same set as nbr_bins, without an indexed array). */
+__device__ inline int nbr_bin_j(int b, int nb, int periodic, int j, int *count) {
+  if (periodic && nb <= 3) {
+    *count = nb;
+    return j;
+  }
+  if (periodic) {
+    *count = 3;
+    const int v = b - 1 + j;
+    return v < 0 ? v + nb : (v >= nb ? v - nb : v);
+  }
+  const int lo = max(b - 1, 0), hi = min(b + 1, nb - 1);
+  *count = hi - lo + 1;
+  return lo + j;
+}
+
+/* Lane r < the returned count gets range r as [*s, *e) of the array whose
+ * bins start at starts[]. */
+__device__ inline int nbr_range(const MLat &L, const int32_t *starts, const int b3[3], int r,
+                                int *s, int *e) {
+  /* x-runs of consecutive bins: two when a periodic wrap splits the three */
+  const int bx = b3[0], nbx = L.nb[0];
+  int runs = 1, f0, l0, f1 = 0, l1 = 0;
+  if (L.periodic[0] && nbx <= 3) {
+    f0 = 0;
+    l0 = nbx - 1;
+  } else if (L.periodic[0] && (bx == 0 || bx == nbx - 1)) {
+    runs = 2;
+    f0 = bx == 0 ? nbx - 1 : nbx - 2;
+    l0 = nbx - 1;
+    l1 = bx == 0 ? 1 : 0;
+  } else {
+    f0 = max(bx - 1, 0);
+    l0 = min(bx + 1, nbx - 1);
+  }
+  int cy, cz;
+  nbr_bin_j(b3[1], L.nb[1], L.periodic[1], 0, &cy);
+  nbr_bin_j(b3[2], L.nb[2], L.periodic[2], 0, &cz);
+  const int n = runs * cy * cz;
+  if (r < n) {
+    const int run = r % runs, jy = (r / runs) % cy, jz = r / (runs * cy);
+    const int y = nbr_bin_j(b3[1], L.nb[1], L.periodic[1], jy, &cy);
+    const int z = nbr_bin_j(b3[2], L.nb[2], L.periodic[2], jz, &cz);
+    const int row = L.bin0 + L.nb[0] * (y + L.nb[1] * z);
+    *s = starts[row + (run == 0 ? f0 : f1)];
+    *e = starts[row + (run == 0 ? l0 : l1) + 1];
+  }
+  return n;
+}
+
+/* Lanes r < nr hold range r as [s, e); the ranges become one list of
+ * candidates c = 0..total-1 over the wavefront, at array index
+ * c + off[r] for the r with ends[r-1] <= c < ends[r] (tables in this
+ * wavefront's LDS).  All 64 lanes call it. */
+__device__ inline int flatten_ranges(int nr, int s, int e, int *off, int *ends) {
+  const int lane = threadIdx.x & 63;
+  const int len = lane < nr ? e - s : 0;
+  const int end = wave_incl_scan(len);
+  const int total = __shfl(end, 63, 64);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (lane < nr) {
+    off[lane] = s - (end - len);
+    ends[lane] = end;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return total;
 }
 
 __device__ inline void ligand_vec(const MArgs &a, const MLat &L, int l, const float rp[3],
@@ -501,16 +648,30 @@ __device__ inline void ligand_vec(const MArgs &a, const MLat &L, int l, const fl
   }
 }
 
+/* Candidate ligand l at offset (dx, dy, dz) from the receptor: the nearest
+ * free one within reach wins, ties to the lower id. */
+__device__ inline void ligand_candidate(const MArgs &a, int l, float dx, float dy, float dz,
+                                        float reach2, float &best_d2, int &best,
+                                        bool &near_reach) {
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  if (fabsf(sqrtf(d2) - a.bond_reach) < BOND_EPS_R) near_reach = true;
+  if (d2 > reach2 || a.m.lig_owner[l] >= 0) return;
+  if (d2 < best_d2 || (d2 == best_d2 && l < best)) {
+    best_d2 = d2;
+    best = l;
+  }
+}
+
 /* Ruptures of held bonds and formation candidates of free receptors, from the
- * state at the start of the step; one wavefront per receptor, a lane per
- * neighbouring bin. */
-__device__ void bond_decide(const MArgs &a, int r) {
+ * state at the start of the step; one wavefront per receptor, the lanes over
+ * the ligands of the neighbouring bins (wall ligands from their bin-ordered
+ * packed positions, fibrin ligands from the scatter order of the markers). */
+__device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
   const int lane = threadIdx.x & 63;
   const int64_t step = *a.m.d_step;
-  float rp[3];
-  int nodes[3];
-  receptor_pos(a, r, rp, nodes);
-  const MLat L = lat_of(a, nodes[0]);
+  const float4 rq = a.m.rec_pos[r];
+  const float rp[3] = {rq.x, rq.y, rq.z};
+  const MLat L = a.lats ? a.lats[__float_as_int(rq.w)] : a.lat;
   const int b = a.m.rec_bond[r];
   if (b >= 0) {
     if (lane == 0) {
@@ -528,44 +689,42 @@ __device__ void bond_decide(const MArgs &a, int r) {
     }
     return;
   }
-  /* nearest free ligand within reach; ties go to the lower ligand id */
   const float reach2 = a.bond_reach * a.bond_reach;
   int best = INT_MAX;
   float best_d2 = INFINITY;
   bool near_reach = false;
   int b3[3];
   bin3_of(L, rp[0], rp[1], rp[2], b3);
-  const int bin = lane < 27 ? nth_nbr_bin(L, b3, lane) : -1;
-  if (bin >= 0) {
-    if (a.m.n_wall_lig > 0) {
-      for (int k = a.m.lg_start[bin]; k < a.m.lg_start[bin + 1]; ++k) {
-        const int l = a.m.lg_items[k];
-        float d[3];
-        ligand_vec(a, L, l, rp, d);
-        const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        if (fabsf(sqrtf(d2) - a.bond_reach) < BOND_EPS_R) near_reach = true;
-        if (d2 > reach2 || a.m.lig_owner[l] >= 0) continue;
-        if (d2 < best_d2 || (d2 == best_d2 && l < best)) {
-          best_d2 = d2;
-          best = l;
-        }
-      }
+  if (a.m.n_wall_lig > 0) {
+    int s = 0, e = 0;
+    const int nr = nbr_range(L, a.m.lg_start, b3, lane, &s, &e);
+    const int total = flatten_ranges(nr, s, e, off, ends);
+    int q = 0;
+    for (int c = lane; c < total; c += 64) {
+      while (c >= ends[q]) ++q;
+      const float4 g = a.m.lg_pos[c + off[q]];
+      ligand_candidate(a, __float_as_int(g.w), min_image(g.x - rp[0], L.nx, L.periodic[0]),
+                       min_image(g.y - rp[1], L.ny, L.periodic[1]),
+                       min_image(g.z - rp[2], L.nz, L.periodic[2]), reach2, best_d2, best,
+                       near_reach);
     }
-    if (a.m.fibrin_lig) {
-      for (int k = a.m.bin_start[bin]; k < a.m.bin_start[bin + 1]; ++k) {
-        const int mk = a.m.bin_scatter[k]; /* order-free: ties go to the lower id */
-        if (a.m.kind[mk] == PFG_MARKER_ELEMENT) continue;
-        const int l = a.m.n_wall_lig + mk;
-        float d[3];
-        ligand_vec(a, L, l, rp, d);
-        const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        if (fabsf(sqrtf(d2) - a.bond_reach) < BOND_EPS_R) near_reach = true;
-        if (d2 > reach2 || a.m.lig_owner[l] >= 0) continue;
-        if (d2 < best_d2 || (d2 == best_d2 && l < best)) {
-          best_d2 = d2;
-          best = l;
-        }
-      }
+  }
+  if (a.m.fibrin_lig) {
+    /* markers other than elements are ligands (index n_wall_lig + marker);
+     * the scatter order is final before this launch and order-free here */
+    int s = 0, e = 0;
+    const int nr = nbr_range(L, a.m.bin_start, b3, lane, &s, &e);
+    const int total = flatten_ranges(nr, s, e, off, ends);
+    int q = 0;
+    for (int c = lane; c < total; c += 64) {
+      while (c >= ends[q]) ++q;
+      const int mk = a.m.bin_scatter[c + off[q]];
+      const float4 g = a.m.pos4[mk];
+      if (__float_as_int(g.w) != -1) continue; /* an element */
+      ligand_candidate(a, a.m.n_wall_lig + mk, min_image(g.x - rp[0], L.nx, L.periodic[0]),
+                       min_image(g.y - rp[1], L.ny, L.periodic[1]),
+                       min_image(g.z - rp[2], L.nz, L.periodic[2]), reach2, best_d2, best,
+                       near_reach);
     }
   }
   /* wavefront arg-min on (distance^2, ligand id) */
@@ -593,9 +752,9 @@ __device__ void bond_decide(const MArgs &a, int r) {
   }
 }
 
-/* roles: A = sort bins (blocks0, one lane per marker: the slot-0 marker of a
- * bin sorts it from the scatter order into bin_items and writes the packed
- * positions), B = platelet sums (blocks1 = one block per platelet),
+/* roles: A = sort bins (blocks0, one lane per marker: each marker takes its
+ * rank by index among the scatter order of its bin, and writes its index and
+ * packed position there), B = platelet sums (blocks1 = one block per platelet),
  * C = bond decisions (blocks2, four receptors per block; they read the
  * scatter order, which no role of this launch writes) */
 __global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
@@ -603,20 +762,14 @@ __global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
   if (a.roles & ROLE_A) {
     if (blk < a.blocks0) {
       const int i = blk * 256 + threadIdx.x;
-      if (i < a.m.n && a.m.marker_slot[i] == 0) {
+      if (i < a.m.n) {
+        /* rank of i among its bin's markers by index */
         const int b = a.m.marker_bin[i];
         const int s = a.m.bin_start[b], e = a.m.bin_start[b + 1];
-        int32_t *it = a.m.bin_items;
-        for (int k = s; k < e; ++k) { /* insertion sort by marker index */
-          const int v = a.m.bin_scatter[k];
-          int j = k - 1;
-          while (j >= s && it[j] > v) {
-            it[j + 1] = it[j];
-            --j;
-          }
-          it[j + 1] = v;
-        }
-        for (int k = s; k < e; ++k) a.m.sorted_pos[k] = a.m.pos4[it[k]];
+        int k = s;
+        for (int q = s; q < e; ++q) k += a.m.bin_scatter[q] < i;
+        a.m.bin_items[k] = i;
+        a.m.sorted_pos[k] = a.m.pos4[i];
       }
       return;
     }
@@ -630,8 +783,10 @@ __global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
     blk -= a.blocks1;
   }
   if (a.roles & ROLE_C) {
-    const int r = blk * 4 + (threadIdx.x >> 6);
-    if (r < a.m.n_rec && !halted(a)) bond_decide(a, r);
+    __shared__ int sh_off[4][NBR_RANGES], sh_end[4][NBR_RANGES];
+    const int w = threadIdx.x >> 6;
+    const int r = blk * 4 + w;
+    if (r < a.m.n_rec && !halted(a)) bond_decide(a, r, sh_off[w], sh_end[w]);
   }
 }
 
@@ -722,62 +877,6 @@ __device__ inline void morse_pair(const MArgs &a, float4 p, float tag, float xi,
   fz += c * dz;
 }
 
-/* The neighbouring bins of a marker as ranges of the bin-sorted order: bins
- * are numbered x fastest, so the (up to 3) x-neighbours of one (y, z) pair
- * are one range unless a periodic wrap splits them in two; at most 2 x 3 x 3
- * ranges. */
-constexpr int MORSE_RANGES = 18;
-
-/* Neighbour bins of bin b along an axis of nb bins: the j-th of *count
- * (the same set as nbr_bins, without an indexed array). */
-__device__ inline int nbr_bin_j(int b, int nb, int periodic, int j, int *count) {
-  if (periodic && nb <= 3) {
-    *count = nb;
-    return j;
-  }
-  if (periodic) {
-    *count = 3;
-    const int v = b - 1 + j;
-    return v < 0 ? v + nb : (v >= nb ? v - nb : v);
-  }
-  const int lo = max(b - 1, 0), hi = min(b + 1, nb - 1);
-  *count = hi - lo + 1;
-  return lo + j;
-}
-
-/* Lane r < the returned count gets range r as [*s, *e) of the sorted order. */
-__device__ inline int morse_range(const MArgs &a, const MLat &L, const int b3[3], int r,
-                                  int *s, int *e) {
-  /* x-runs of consecutive bins: two when a periodic wrap splits the three */
-  const int bx = b3[0], nbx = L.nb[0];
-  int runs = 1, f0, l0, f1 = 0, l1 = 0;
-  if (L.periodic[0] && nbx <= 3) {
-    f0 = 0;
-    l0 = nbx - 1;
-  } else if (L.periodic[0] && (bx == 0 || bx == nbx - 1)) {
-    runs = 2;
-    f0 = bx == 0 ? nbx - 1 : nbx - 2;
-    l0 = nbx - 1;
-    l1 = bx == 0 ? 1 : 0;
-  } else {
-    f0 = max(bx - 1, 0);
-    l0 = min(bx + 1, nbx - 1);
-  }
-  int cy, cz;
-  nbr_bin_j(b3[1], L.nb[1], L.periodic[1], 0, &cy);
-  nbr_bin_j(b3[2], L.nb[2], L.periodic[2], 0, &cz);
-  const int n = runs * cy * cz;
-  if (r < n) {
-    const int run = r % runs, jy = (r / runs) % cy, jz = r / (runs * cy);
-    const int y = nbr_bin_j(b3[1], L.nb[1], L.periodic[1], jy, &cy);
-    const int z = nbr_bin_j(b3[2], L.nb[2], L.periodic[2], jz, &cz);
-    const int row = L.bin0 + L.nb[0] * (y + L.nb[1] * z);
-    *s = a.m.bin_start[row + (run == 0 ? f0 : f1)];
-    *e = a.m.bin_start[row + (run == 0 ? l0 : l1) + 1];
-  }
-  return n;
-}
-
 /* Area and volume constraint force on the vertex at slot e % 3 of triangle
  * e / 3 (sem_ForceAreaVol_kernel, sem_kernel.cu:1640-1749), from the
  * per-triangle terms of this step. */
@@ -820,7 +919,7 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
       }
     }
   }
-  __shared__ int sh_off[4][MORSE_RANGES], sh_end[4][MORSE_RANGES];
+  __shared__ int sh_off[4][NBR_RANGES], sh_end[4][NBR_RANGES];
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= a.m.n) return;
   const int kind = a.m.kind[i];
@@ -836,17 +935,9 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
       const int b = a.m.marker_bin[i] - L.bin0;
       const int b3[3] = {b % L.nb[0], (b / L.nb[0]) % L.nb[1], b / (L.nb[0] * L.nb[1])};
       int s = 0, e = 0;
-      const int nr = morse_range(a, L, b3, lane, &s, &e);
-      const int end = wave_incl_scan(lane < nr ? e - s : 0);
-      const int total = __shfl(end, 63, 64);
+      const int nr = nbr_range(L, a.m.bin_start, b3, lane, &s, &e);
       int *off = sh_off[threadIdx.x >> 6], *ends = sh_end[threadIdx.x >> 6];
-      if (lane < nr) {
-        off[lane] = s - (end - (e - s)); /* sorted index = candidate + off */
-        ends[lane] = end;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int total = flatten_ranges(nr, s, e, off, ends);
       const float tag = pi.w;
       int r = 0;
       for (int c = lane; c < total; c += 64) {
@@ -944,9 +1035,11 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
  * markers share the lattice.  A chunk whose box does not fit in LDS (markers
  * scattered through the lattice) is spread directly. */
 
-__device__ inline void flag_tile(const MLat &L, int x, int y, int z) {
+/* Stamp the tile of node (x, y, z) with this step: the LB update reads the
+ * force of stamped tiles only. */
+__device__ inline void flag_tile(const MLat &L, int x, int y, int z, int32_t step) {
   const int tile = (x >> TILE_SHIFT_X) + L.tnx * ((y >> TILE_SHIFT_Y) + L.tny * (z >> TILE_SHIFT_Z));
-  if (L.tile_flag[tile] == 0) L.tile_flag[tile] = 1;
+  if (L.tile_stamp[tile] != step) L.tile_stamp[tile] = step;
 }
 
 /* Direct spreading of marker i: lane = stencil node. */
@@ -961,7 +1054,7 @@ __device__ void spread_direct(const MArgs &a, int i, int lane) {
   const int iy = stencil_node(Y, (lane >> 2) & 3, L.ny, L.periodic[1], &wy);
   const int iz = stencil_node(Z, lane >> 4, L.nz, L.periodic[2], &wz);
   if (ix < 0 || iy < 0 || iz < 0) return;
-  flag_tile(L, ix, iy, iz);
+  flag_tile(L, ix, iy, iz, (int32_t)*a.m.d_step);
   const float w = wx * wy * wz * a.force_scale;
   const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
   atomicAdd(L.aux + c, w * a.m.fx[i]);
@@ -1088,6 +1181,7 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
   }
   __syncthreads();
   /* flush row by row: the (y, z) of a row is wavefront-uniform */
+  const int32_t step = (int32_t)*a.m.d_step;
   const int rows = b.dim[1] * b.dim[2];
   for (int r = wave; r < rows; r += 4) {
     const int by = r % b.dim[1], bz = r / b.dim[1];
@@ -1098,7 +1192,7 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
     for (int bx = lane; bx < b.dim[0]; bx += 64) {
       const int gx = axis_index(b.lo[0] + bx, L.nx, L.periodic[0]);
       if (gx < 0) continue;
-      flag_tile(L, gx, gy, gz);
+      flag_tile(L, gx, gy, gz, step);
       const int k = bx + b.dim[0] * r;
       const float fx = box[k], fy = box[b.vol + k], fz = box[2 * b.vol + k];
       const int64_t c = row + gx;
@@ -1123,9 +1217,9 @@ __device__ void interp_marker(const MArgs &a, int i, int lane, float *v) {
   if (ix >= 0 && iy >= 0 && iz >= 0) {
     const float w = wx * wy * wz;
     const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
-    ux = w * L.aux[c];
-    uy = w * L.aux[a.plane + c];
-    uz = w * L.aux[2 * a.plane + c];
+    ux = w * L.vel[c];
+    uy = w * L.vel[a.plane + c];
+    uz = w * L.vel[2 * a.plane + c];
   }
   v[0] = wave_sum(ux);
   v[1] = wave_sum(uy);
@@ -1185,7 +1279,8 @@ __global__ __launch_bounds__(256) void move_kernel(MArgs a) {
 void fill_mlat(const pfg_handle *h, int k, MLat *L) {
   const Inst &I = h->inst[(size_t)k];
   L->aux = h->aux + I.cell0;
-  L->tile_flag = h->tile_flag + I.tile0;
+  L->vel = h->vel + I.cell0;
+  L->tile_stamp = h->tile_stamp + I.tile0;
   L->tnx = I.tnx;
   L->tny = I.tny;
   L->tnz = I.tnz;
@@ -1253,10 +1348,13 @@ static inline int blocks_for(int64_t n, int per_block) {
 hipError_t launch_begin(pfg_handle *h, bool clear, bool count) {
   MArgs a;
   make_margs(h, &a);
-  const int nb_count = count ? blocks_for(h->m.n, 256) : 0;
-  a.roles = (clear ? ROLE_A : 0) | (count && h->m.n > 0 ? ROLE_B : 0);
+  const bool recs = count && h->m.n_rec > 0;
+  count = count && h->m.n > 0;
+  a.roles = (clear ? ROLE_A : 0) | (count ? ROLE_B : 0) | (recs ? ROLE_C : 0);
   a.blocks0 = clear ? blocks_for(h->ntiles, 256) : 0;
-  const int grid = a.blocks0 + ((a.roles & ROLE_B) ? nb_count : 0);
+  a.blocks1 = count ? blocks_for(h->m.n, 256) : 0;
+  a.blocks2 = recs ? blocks_for(h->m.n_rec, 256) : 0;
+  const int grid = a.blocks0 + a.blocks1 + a.blocks2;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(begin_kernel, dim3(grid), dim3(256), 0, h->stream, a);
   return hipGetLastError();
@@ -1266,9 +1364,7 @@ hipError_t launch_scan(pfg_handle *h) {
   const Markers &m = h->m;
   if (m.n == 0) return hipSuccess;
   hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
-                     h->stream, m.bin_count, m.nbins, m.scan_partials);
-  hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(256), 0, h->stream, m.scan_partials,
-                     m.scan_blocks, m.bin_start, m.nbins);
+                     h->stream, m.bin_count, m.scan_partials);
   hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
                      h->stream, m.bin_count, m.nbins, m.scan_partials, m.bin_start);
   return hipGetLastError();

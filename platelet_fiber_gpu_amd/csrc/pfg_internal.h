@@ -23,6 +23,7 @@ constexpr int GRAPH_CHUNK = 8;
 /* IB work units: at most IB_CHUNK consecutive markers of one body; their
  * stencils are staged in an LDS box of at most IB_BOX_CELLS lattice nodes
  * (3 floats each). */
+constexpr int SCAN_SPAN = 8192; /* bins per block of the bin-count scan */
 constexpr int IB_CHUNK = 128;
 constexpr int IB_BOX_CELLS = 2048;
 constexpr size_t IB_BOX_BYTES = 3 * IB_BOX_CELLS * sizeof(float);
@@ -33,8 +34,9 @@ constexpr int IB_STAGED_MIN = 32768;
 struct LBArgs {
   const void *fin;        /* post-collision state of the previous step      */
   void *fout;             /* post-collision state of this step              */
-  float *aux;             /* [3][plane]: IB force in, IB velocity out       */
-  const uint8_t *tile_flag;
+  float *aux;             /* [3][plane]: IB force in, zeroed once read      */
+  float *vel;             /* [3][plane]: IB velocity out                    */
+  const int32_t *tile_stamp; /* per tile: the last step that spread into it */
   const uint8_t *solid;   /* nullptr when there are no solid cells          */
   const uint32_t *bbmask; /* per cell: bit d = upstream of d is solid,
                              bit 31 = cell is solid; nullptr without flags  */
@@ -66,8 +68,9 @@ struct LBArgs {
 /* Lattice side of the IB coupling for one lattice of a batch (a handle
  * with one lattice has one). */
 struct MLat {
-  float *aux;             /* the lattice's cells of the [3][plane] field      */
-  uint8_t *tile_flag;     /* the lattice's tiles                              */
+  float *aux;             /* the lattice's cells of the [3][plane] force      */
+  const float *vel;       /* the lattice's cells of the [3][plane] velocity   */
+  int32_t *tile_stamp;    /* the lattice's tiles                              */
   int tnx, tny, tnz;
   int nx, ny, nz;
   int periodic[3];
@@ -124,6 +127,9 @@ struct Markers {
   int32_t *lig_claim = nullptr;  /* lowest claiming receptor or INT_MAX      */
   /* static ligands binned on the cell-list grid: lg_start[nbins+1] */
   int32_t *lg_start = nullptr, *lg_items = nullptr;
+  float4 *lg_pos = nullptr;       /* per lg_items slot: x, y, z, ligand index */
+  float4 *rec_pos = nullptr;      /* receptor positions of this step, lattice
+                                     index in .w (int bits)                   */
   /* fibrin links, CSR marker -> (partner, rest length) */
   int n_links = 0;
   int32_t *m2l_start = nullptr, *m2l_nbr = nullptr;
@@ -135,7 +141,7 @@ struct Markers {
           *marker_bin = nullptr, *marker_slot = nullptr;
   int32_t *bin_scatter = nullptr; /* items in scatter (atomic slot) order    */
   float4 *sorted_pos = nullptr;   /* per bin_items slot: x, y, z, Morse tag  */
-  int32_t *scan_partials = nullptr;
+  int32_t *scan_partials = nullptr; /* sums of SCAN_SPAN bins */
   int64_t scan_blocks = 0;
   /* IB chunks: chunk k holds the markers chunk_start[k] .. chunk_start[k+1]-1,
    * at most IB_CHUNK consecutive markers of one body (a platelet, or the
@@ -176,8 +182,13 @@ struct pfg_handle {
    * ((x/64 + nxb*(y + ny*z))*19 + d)*64 + x%64 */
   void *f[2] = {nullptr, nullptr};
   int cur = 0;
+  /* IB coupling fields: spreading adds the force of a step into aux and
+   * stamps the 16x4x4 tiles it touches with the step; the LB update reads
+   * the force of stamped tiles, zeroes it there for the next step and
+   * writes the velocity the interpolation reads into vel */
   float *aux = nullptr;
-  uint8_t *tile_flag = nullptr;    /* one byte per 16x4x4 tile touched by IB   */
+  float *vel = nullptr;
+  int32_t *tile_stamp = nullptr;
   int tnx = 0, tny = 0, tnz = 0;
   int64_t ntiles = 0;
   uint8_t *solid = nullptr;
