@@ -1042,7 +1042,8 @@ __device__ inline void flag_tile(const MLat &L, int x, int y, int z, int32_t ste
   if (L.tile_stamp[tile] != step) L.tile_stamp[tile] = step;
 }
 
-/* Direct spreading of marker i: lane = stencil node. */
+/* Direct spreading of marker i: lane = stencil node (consecutive lanes take
+ * consecutive x, so one atomic instruction covers 16 runs of 4 nodes). */
 __device__ void spread_direct(const MArgs &a, int i, int lane) {
   if (!ib_coupled(a.m.kind[i], a.fiber_move)) return;
   const MLat L = lat_of(a, i);
@@ -1074,6 +1075,7 @@ struct ChunkBox {
   int dim[3];
   int vol;    /* 0: no IB-coupled marker in the chunk */
   bool lds;   /* box fits in LDS */
+  float amax; /* bound of |force| x |force scale| over the chunk */
 };
 
 /* Wrapped coordinate (gives the kernel weights exactly as the direct form)
@@ -1097,36 +1099,38 @@ __device__ inline float stencil_weight(float W, int k) {
   return 0.25f * (1.0f + cospif(0.5f * r));
 }
 
-/* Block-wide box of the chunk's markers [m0, m0 + cnt) (cnt <= 256); every
- * thread gets the result. */
-__device__ ChunkBox chunk_box(const MArgs &a, const MLat &L, int m0, int cnt, const float ref[3],
-                              int *sh) {
+/* Block-wide box of the stencils whose first nodes o[] the threads with
+ * `has` hold, and the largest of the threads' `amax`; every thread gets the
+ * result. */
+__device__ ChunkBox chunk_box(bool has, const int o[3], float amax, int *sh) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-  if (t < cnt && ib_coupled(a.m.kind[m0 + t], a.fiber_move)) {
-    float W[3];
-    int o[3];
-    stencil_origin(a, L, m0 + t, ref, W, o);
+  if (has) {
     for (int k = 0; k < 3; ++k) {
       lo[k] = o[k];
       hi[k] = o[k] + 3;
     }
   }
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int s = 32; s > 0; s >>= 1) {
 #pragma unroll
-    for (int s = 32; s > 0; s >>= 1) {
+    for (int k = 0; k < 3; ++k) {
       lo[k] = min(lo[k], __shfl_xor(lo[k], s, 64));
       hi[k] = max(hi[k], __shfl_xor(hi[k], s, 64));
     }
+    amax = fmaxf(amax, __shfl_xor(amax, s, 64));
   }
-  if (lane == 0)
+  if (lane == 0) {
     for (int k = 0; k < 3; ++k) {
       sh[6 * wave + k] = lo[k];
       sh[6 * wave + 3 + k] = hi[k];
     }
+    sh[24 + wave] = __float_as_int(amax);
+  }
   __syncthreads();
   ChunkBox b;
+  b.amax = fmaxf(fmaxf(__int_as_float(sh[24]), __int_as_float(sh[25])),
+                 fmaxf(__int_as_float(sh[26]), __int_as_float(sh[27])));
   int64_t vol = 1;
   for (int k = 0; k < 3; ++k) {
     int l = INT_MAX, h = INT_MIN;
@@ -1144,10 +1148,23 @@ __device__ ChunkBox chunk_box(const MArgs &a, const MLat &L, int m0, int cnt, co
   return b;
 }
 
-/* One workgroup per chunk; dynamic LDS of 3 * IB_BOX_CELLS floats. */
+/* One workgroup per chunk; dynamic LDS of 3 * IB_BOX_CELLS int64 sums.
+ * Thread t first takes marker t of the chunk: its first stencil node, its
+ * 3 x 4 kernel weights and its force go to LDS, so that the accumulation
+ * loop reads no global memory.  The box sums in fixed point: each FP32
+ * contribution w f is scaled by a power of two 2^q chosen from the chunk's
+ * force bound so that every sum fits in 63 bits, converted to an integer
+ * (exact above 2^24, rounded below) and added with an integer LDS atomic
+ * (the FP32 LDS atomic runs at a small fraction of its rate); the flush
+ * converts each sum back with one rounding.  The box is thus independent of
+ * the order of the additions. */
 __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
-  extern __shared__ float box[];
-  __shared__ int sh[24];
+  static_assert(IB_CHUNK <= 128, "one thread per marker; sums below 2^62 need <= 128 markers");
+  extern __shared__ unsigned long long box[];
+  __shared__ int sh[28];
+  __shared__ int so[IB_CHUNK][3];    /* first stencil node; INT_MIN: not coupled */
+  __shared__ float sw[IB_CHUNK][12]; /* weights along x (0..3), y (4..7), z (8..11) */
+  __shared__ float sf[IB_CHUNK][3];
   if (halted(a)) return;
   const int m0 = a.m.chunk_start[blockIdx.x], cnt = a.m.chunk_start[blockIdx.x + 1] - m0;
   const MLat L = lat_of(a, m0); /* a chunk never spans two lattices */
@@ -1155,29 +1172,68 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
   ref[0] = wrap_coord(a.m.x[m0], L.nx, L.periodic[0]);
   ref[1] = wrap_coord(a.m.y[m0], L.ny, L.periodic[1]);
   ref[2] = wrap_coord(a.m.z[m0], L.nz, L.periodic[2]);
-  const ChunkBox b = chunk_box(a, L, m0, cnt, ref, sh);
+  const int t = threadIdx.x;
+  bool has = false;
+  int o[3] = {0, 0, 0};
+  float amax = 0.f;
+  if (t < cnt) {
+    const int i = m0 + t;
+    has = ib_coupled(a.m.kind[i], a.fiber_move);
+    if (has) {
+      float W[3];
+      stencil_origin(a, L, i, ref, W, o);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        sw[t][k] = stencil_weight(W[0], k);
+        sw[t][4 + k] = stencil_weight(W[1], k);
+        sw[t][8 + k] = stencil_weight(W[2], k);
+      }
+      sf[t][0] = a.m.fx[i];
+      sf[t][1] = a.m.fy[i];
+      sf[t][2] = a.m.fz[i];
+      amax = fmaxf(fmaxf(fabsf(sf[t][0]), fabsf(sf[t][1])), fabsf(sf[t][2])) *
+             fabsf(a.force_scale);
+    }
+    so[t][0] = has ? o[0] : INT_MIN;
+    so[t][1] = o[1];
+    so[t][2] = o[2];
+  }
+  const ChunkBox b = chunk_box(has, o, amax, sh); /* its barriers publish so, sw, sf */
   if (b.vol == 0) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (!b.lds) {
-    for (int j = wave; j < cnt; j += 4) spread_direct(a, m0 + j, lane);
+  const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
+  if (!b.lds) { /* scattered markers: device-wide atomics per node */
+    const int32_t step = (int32_t)*a.m.d_step;
+    for (int j = wave; j < cnt; j += 4) {
+      if (so[j][0] == INT_MIN) continue;
+      const int ix = axis_index(so[j][0] + kx, L.nx, L.periodic[0]);
+      const int iy = axis_index(so[j][1] + ky, L.ny, L.periodic[1]);
+      const int iz = axis_index(so[j][2] + kz, L.nz, L.periodic[2]);
+      if (ix < 0 || iy < 0 || iz < 0) continue;
+      flag_tile(L, ix, iy, iz, step);
+      const float w = sw[j][kx] * sw[j][4 + ky] * sw[j][8 + kz] * a.force_scale;
+      const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
+      atomicAdd(L.aux + c, w * sf[j][0]);
+      atomicAdd(L.aux + a.plane + c, w * sf[j][1]);
+      atomicAdd(L.aux + 2 * a.plane + c, w * sf[j][2]);
+    }
     return;
   }
-  for (int k = threadIdx.x; k < 3 * b.vol; k += 256) box[k] = 0.f;
+  for (int k = threadIdx.x; k < 3 * b.vol; k += 256) box[k] = 0ull;
+  /* |w f| <= amax / 8 < 2^(e - 3) with amax < 2^e, so with q = 58 - e a
+   * contribution stays below 2^55 and a sum of <= 128 below 2^62 */
+  int e;
+  frexpf(b.amax, &e);
+  const int q = 58 - e;
   __syncthreads();
-  const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
   for (int j = wave; j < cnt; j += 4) {
-    const int i = m0 + j;
-    if (!ib_coupled(a.m.kind[i], a.fiber_move)) continue;
-    float W[3];
-    int o[3];
-    stencil_origin(a, L, i, ref, W, o);
-    const float w = stencil_weight(W[0], kx) * stencil_weight(W[1], ky) *
-                    stencil_weight(W[2], kz) * a.force_scale;
-    const int cell = (o[0] + kx - b.lo[0]) + b.dim[0] * ((o[1] + ky - b.lo[1]) +
-                                                         b.dim[1] * (o[2] + kz - b.lo[2]));
-    atomicAdd(&box[cell], w * a.m.fx[i]);
-    atomicAdd(&box[b.vol + cell], w * a.m.fy[i]);
-    atomicAdd(&box[2 * b.vol + cell], w * a.m.fz[i]);
+    if (so[j][0] == INT_MIN) continue;
+    const float w = sw[j][kx] * sw[j][4 + ky] * sw[j][8 + kz] * a.force_scale;
+    const int cell = (so[j][0] + kx - b.lo[0]) +
+                     b.dim[0] * ((so[j][1] + ky - b.lo[1]) + b.dim[1] * (so[j][2] + kz - b.lo[2]));
+    atomicAdd(&box[cell], (unsigned long long)__float2ll_rn(ldexpf(w * sf[j][0], q)));
+    atomicAdd(&box[b.vol + cell], (unsigned long long)__float2ll_rn(ldexpf(w * sf[j][1], q)));
+    atomicAdd(&box[2 * b.vol + cell], (unsigned long long)__float2ll_rn(ldexpf(w * sf[j][2], q)));
   }
   __syncthreads();
   /* flush row by row: the (y, z) of a row is wavefront-uniform */
@@ -1194,7 +1250,9 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
       if (gx < 0) continue;
       flag_tile(L, gx, gy, gz, step);
       const int k = bx + b.dim[0] * r;
-      const float fx = box[k], fy = box[b.vol + k], fz = box[2 * b.vol + k];
+      const float fx = ldexpf((float)(long long)box[k], -q);
+      const float fy = ldexpf((float)(long long)box[b.vol + k], -q);
+      const float fz = ldexpf((float)(long long)box[2 * b.vol + k], -q);
       const int64_t c = row + gx;
       if (fx != 0.f) atomicAdd(L.aux + c, fx);
       if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
@@ -1203,32 +1261,46 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
   }
 }
 
-/* Interpolated velocity of marker i (lane = stencil node, summed over the
- * wavefront; nodes outside a wall-bounded axis contribute nothing). */
-__device__ void interp_marker(const MArgs &a, int i, int lane, float *v) {
+/* Interpolated velocity of marker i by a group of 16 lanes: lane j takes
+ * node (x, y) = (j & 3, j >> 2) of the four z-layers of the stencil, so one
+ * load instruction of the wavefront covers 4 markers x 4 rows of 4
+ * consecutive nodes; summed over the group (nodes outside a wall-bounded
+ * axis contribute nothing). */
+__device__ void interp_marker(const MArgs &a, int i, int j, float *v) {
   const MLat L = lat_of(a, i);
   const float X = wrap_coord(a.m.x[i], L.nx, L.periodic[0]);
   const float Y = wrap_coord(a.m.y[i], L.ny, L.periodic[1]);
   const float Z = wrap_coord(a.m.z[i], L.nz, L.periodic[2]);
-  float wx, wy, wz, ux = 0.f, uy = 0.f, uz = 0.f;
-  const int ix = stencil_node(X, lane & 3, L.nx, L.periodic[0], &wx);
-  const int iy = stencil_node(Y, (lane >> 2) & 3, L.ny, L.periodic[1], &wy);
-  const int iz = stencil_node(Z, lane >> 4, L.nz, L.periodic[2], &wz);
-  if (ix >= 0 && iy >= 0 && iz >= 0) {
-    const float w = wx * wy * wz;
-    const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
-    ux = w * L.vel[c];
-    uy = w * L.vel[a.plane + c];
-    uz = w * L.vel[2 * a.plane + c];
+  float wx, wy, ux = 0.f, uy = 0.f, uz = 0.f;
+  const int ix = stencil_node(X, j & 3, L.nx, L.periodic[0], &wx);
+  const int iy = stencil_node(Y, j >> 2, L.ny, L.periodic[1], &wy);
+  if (ix >= 0 && iy >= 0) {
+    const int64_t col = ix + (int64_t)L.nx * iy;
+    const int64_t layer = (int64_t)L.nx * L.ny;
+    const float wxy = wx * wy;
+#pragma unroll
+    for (int kz = 0; kz < 4; ++kz) {
+      float wz;
+      const int iz = stencil_node(Z, kz, L.nz, L.periodic[2], &wz);
+      if (iz < 0) continue;
+      const float w = wxy * wz;
+      const int64_t c = col + layer * iz;
+      ux += w * L.vel[c];
+      uy += w * L.vel[a.plane + c];
+      uz += w * L.vel[2 * a.plane + c];
+    }
   }
-  v[0] = wave_sum(ux);
-  v[1] = wave_sum(uy);
-  v[2] = wave_sum(uz);
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    ux += __shfl_xor(ux, o, 16);
+    uy += __shfl_xor(uy, o, 16);
+    uz += __shfl_xor(uz, o, 16);
+  }
+  v[0] = ux;
+  v[1] = uy;
+  v[2] = uz;
 }
 
-/* Marker update by lane 0: x += V dt (sem_kernel.cu:2112-2138), positions
- * stay unwrapped; fibrin nodes under PFG_FIBER_KICK take the damped kick of
- * network.cpp:839-881 first. */
 __device__ void update_marker(const MArgs &a, int i, float vx, float vy, float vz) {
   const int kind = a.m.kind[i];
   if (kind == PFG_MARKER_FIXED) return;
@@ -1250,10 +1322,10 @@ __device__ void update_marker(const MArgs &a, int i, float vx, float vy, float v
 }
 
 /* roles: A = interpolate V at the markers, B = move the markers and advance
- * the step counter.  One wavefront per marker, a lane per stencil node. */
+ * the step counter.  Four markers per wavefront, 16 lanes each. */
 __global__ __launch_bounds__(256) void move_kernel(MArgs a) {
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+  const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int lane = threadIdx.x & 15;
   if (halted(a)) return;
   if ((a.roles & ROLE_B) && blockIdx.x == 0 && threadIdx.x == 0) *a.m.d_step += 1;
   if (i >= a.m.n) return;
@@ -1436,7 +1508,7 @@ hipError_t launch_move(pfg_handle *h, bool interp, bool update) {
   MArgs a;
   make_margs(h, &a);
   a.roles = (interp ? ROLE_A : 0) | (update ? ROLE_B : 0);
-  hipLaunchKernelGGL(move_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
+  hipLaunchKernelGGL(move_kernel, dim3(blocks_for(h->m.n, 16)), dim3(256), 0, h->stream, a);
   return hipGetLastError();
 }
 

@@ -22,11 +22,11 @@ constexpr int GRAPH_CHUNK = 8;
 
 /* IB work units: at most IB_CHUNK consecutive markers of one body; their
  * stencils are staged in an LDS box of at most IB_BOX_CELLS lattice nodes
- * (3 floats each). */
+ * (3 fixed-point int64 sums each). */
 constexpr int SCAN_SPAN = 8192; /* bins per block of the bin-count scan */
 constexpr int IB_CHUNK = 128;
 constexpr int IB_BOX_CELLS = 2048;
-constexpr size_t IB_BOX_BYTES = 3 * IB_BOX_CELLS * sizeof(float);
+constexpr size_t IB_BOX_BYTES = 3 * IB_BOX_CELLS * sizeof(long long);
 /* Marker count from which spreading is staged through LDS by default. */
 constexpr int IB_STAGED_MIN = 32768;
 
