@@ -297,6 +297,12 @@ pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable);
  * 2 always direct (FP32 atomics per marker and node).  Same result up to
  * FP32 summation order. */
 pfg_status pfg_set_ib_staging(pfg_handle *h, int mode);
+/* Lanes per marker in the SEM/fibrin force pass (sem_Force_kernel,
+ * sem_kernel.cu:320-513, with the Morse pairs over the cell lists): 0
+ * automatic (64 below 32768 markers, else 4), or 1, 4, 16 or 64.  With
+ * fewer than 64 the markers are taken in cell-list order.  Same result up
+ * to FP32 summation order. */
+pfg_status pfg_set_forces_lanes(pfg_handle *h, int lanes);
 /* enable = 1: within a step, the LB update of the lattice rows that no IB
  * stencil touches runs on a second HIP stream concurrently with the marker
  * pipeline, and the touched rows update after the spreading (semLB.cu:303-438

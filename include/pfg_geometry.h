@@ -41,6 +41,12 @@ typedef struct pfg_scene_opts {
   float tube_radius;               /* 0: config default                            */
   int32_t store;                   /* pfg_store, -1: config default                */
   uint64_t seed;                   /* 0: 1101 (gpuSEM_LB.c:1614)                   */
+  const char *mesh_path;           /* NULL: generated ellipsoids; else every
+                                      platelet is this Gambit .neu mesh, read as
+                                      NEUReader reads it (gpuSEM_LB.c:346-424),
+                                      centred on its placement and scaled       */
+  float mesh_scale;                /* lattice units per mesh unit; 0: the scale
+                                      that makes the mean mesh edge 0.75        */
 } pfg_scene_opts;
 
 typedef struct pfg_scene pfg_scene;
@@ -69,6 +75,17 @@ int64_t pfg_scene_fluid_cells(const pfg_scene *s);
 pfg_status pfg_geom_ellipsoid(int n, float a, float b, float c, float cx, float cy,
                               float cz, float jitter, uint64_t seed, float *x,
                               float *y, float *z, int32_t *tri, int32_t *n_tri);
+
+/* Gambit .neu surface mesh as NEUReader reads it (gpuSEM_LB.c:346-424): the
+ * node and triangle counts, then one line per node "index x y z" and one
+ * per triangle "index type nodes n1 n2 n3", indices from 1.  With x == NULL
+ * only the counts are returned; otherwise the buffers must hold max_nodes
+ * nodes and max_tri triangles (3 ints each, from 0).  PFG_ERR_IO: the file
+ * cannot be read; PFG_ERR_ARG: a count, an index or a value is missing or
+ * out of range, or a buffer is too small. */
+pfg_status pfg_geom_read_neu(const char *path, int max_nodes, int max_tri, float *x,
+                             float *y, float *z, int32_t *tri, int32_t *n_nodes,
+                             int32_t *n_tri);
 
 /* Triangle areas, total area and the reference-signed volume
  * V = sum (c . n_elem) S / 3 with n_elem = -(e01 x e02)/|e01 x e02|

@@ -10,6 +10,7 @@ the seeded inputs that runGPUSimulation assembles (gpuSEM_LB.c:649-1258),
 allocate (semLB.cu:111, 749) and runs the per-step calls of gpuSEM_LB.c:1441-1442.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -28,7 +29,7 @@ __all__ = [
     "FIBER_IB", "FIBER_KICK", "SCENE_CHANNEL", "SCENE_SHEAR", "SCENE_THROMBUS",
     "SCENE_LARGE", "SCENE_BATCH", "STAGE_CELLS", "STAGE_MEMBRANE", "STAGE_BONDS",
     "STAGE_FORCES", "STAGE_SPREAD", "STAGE_LB", "STAGE_INTERP", "STAGE_UPDATE",
-    "STAGE_CLEAR",
+    "STAGE_CLEAR", "read_neu",
 ]
 
 Q = 19
@@ -54,6 +55,22 @@ def default_params(**kw):
     return p
 
 
+def read_neu(path):
+    """A Gambit .neu surface mesh as the library reads it (NEUReader,
+    gpuSEM_LB.c:346-424): x, y, z (float32) and triangles (int32, n x 3, from 0)."""
+    L = lib()
+    nn, nt = C.c_int32(0), C.c_int32(0)
+    p = os.fsencode(path)
+    _ffi.check("pfg_geom_read_neu", L.pfg_geom_read_neu(p, 0, 0, None, None, None, None,
+                                                        C.byref(nn), C.byref(nt)))
+    x, y, z = (np.empty(nn.value, np.float32) for _ in range(3))
+    tri = np.empty((nt.value, 3), np.int32)
+    _ffi.check("pfg_geom_read_neu", L.pfg_geom_read_neu(
+        p, nn.value, nt.value, _ptr(x, C.c_float), _ptr(y, C.c_float), _ptr(z, C.c_float),
+        _ptr(tri, C.c_int32), C.byref(nn), C.byref(nt)))
+    return x, y, z, tri
+
+
 def device_count():
     n = C.c_int(0)
     lib().pfg_device_count(C.byref(n))
@@ -71,7 +88,10 @@ class Scene:
 
     def __init__(self, config, seed=0, nx=0, ny=0, nz=0, n_platelets=0, elements=0,
                  receptors=0, ligands=0, fibrin_nodes=0, fibrin_links=0, tube_radius=0.0,
-                 store=-1):
+                 store=-1, mesh_path=None, mesh_scale=0.0):
+        """mesh_path: a Gambit .neu surface mesh used for every platelet
+        (scaled by mesh_scale lattice units per mesh unit, 0: mean edge 0.75)
+        instead of the generated ellipsoids."""
         L = lib()
         o = SceneOpts()
         L.pfg_scene_opts_init(C.byref(o), config)
@@ -79,6 +99,8 @@ class Scene:
         o.n_platelets, o.elements, o.receptors, o.ligands = n_platelets, elements, receptors, ligands
         o.fibrin_nodes, o.fibrin_links = fibrin_nodes, fibrin_links
         o.tube_radius, o.store = tube_radius, store
+        o.mesh_path = os.fsencode(mesh_path) if mesh_path else None
+        o.mesh_scale = mesh_scale
         self.params = Params()
         h = C.c_void_p()
         _ffi.check("pfg_scene_build", L.pfg_scene_build(C.byref(o), C.byref(self.params),
@@ -315,6 +337,10 @@ class Solver:
     def set_ib_staging(self, mode):
         """IB spreading form: 0 automatic, 1 LDS-staged chunks, 2 direct."""
         self._c("pfg_set_ib_staging", int(mode))
+
+    def set_forces_lanes(self, lanes):
+        """Lanes per marker in the force pass: 0 automatic, or 1, 4, 16, 64."""
+        self._c("pfg_set_forces_lanes", int(lanes))
 
     def set_overlap(self, on):
         """LB rows away from IB stencils update concurrently with the marker

@@ -77,7 +77,8 @@ class SceneOpts(C.Structure):
                 ("nz", C.c_int32), ("n_platelets", C.c_int32), ("elements", C.c_int32),
                 ("receptors", C.c_int32), ("ligands", C.c_int32),
                 ("fibrin_nodes", C.c_int32), ("fibrin_links", C.c_int32),
-                ("tube_radius", C.c_float), ("store", C.c_int32), ("seed", C.c_uint64)]
+                ("tube_radius", C.c_float), ("store", C.c_int32), ("seed", C.c_uint64),
+                ("mesh_path", C.c_char_p), ("mesh_scale", C.c_float)]
 
 
 # name -> (restype, argtypes); every entry point declared in include/*.h
@@ -117,6 +118,7 @@ SIGNATURES = {
     "pfg_set_fluid_only": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_halt_on_detach": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_ib_staging": (C.c_int, [C.c_void_p, C.c_int]),
+    "pfg_set_forces_lanes": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_batch_create": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int,
                                    C.POINTER(C.c_void_p)]),
@@ -144,6 +146,8 @@ SIGNATURES = {
                                      C.c_float, C.c_float, C.c_float, C.c_uint64, f32p,
                                      f32p, f32p, i32p, i32p]),
     "pfg_geom_area_volume": (C.c_int, [C.c_int, i32p, f32p, f32p, f32p, f64p, f64p, f64p]),
+    "pfg_geom_read_neu": (C.c_int, [C.c_char_p, C.c_int, C.c_int, f32p, f32p, f32p, i32p,
+                                    i32p, i32p]),
     "pfg_geom_fibrin": (C.c_int, [C.c_int, f32p, f32p, C.c_float, C.c_int, C.c_uint64,
                                   f32p, f32p, f32p, i32p, f32p, i32p]),
 }

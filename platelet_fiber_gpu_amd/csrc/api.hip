@@ -1332,6 +1332,14 @@ pfg_status pfg_set_ib_staging(pfg_handle *h, int mode) {
   return PFG_OK;
 }
 
+pfg_status pfg_set_forces_lanes(pfg_handle *h, int lanes) {
+  if (!h || !(lanes == 0 || lanes == 1 || lanes == 4 || lanes == 16 || lanes == 64))
+    return PFG_ERR_ARG;
+  h->forces_lanes = lanes;
+  drop_graph(h);
+  return PFG_OK;
+}
+
 pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable) {
   if (!h) return PFG_ERR_ARG;
   h->halt_on_detach = enable != 0;

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <unordered_map>
@@ -184,6 +185,58 @@ pfg_status pfg_geom_ellipsoid(int n, float a, float b, float c, float cx, float 
   return PFG_OK;
 }
 
+/* NEUReader (gpuSEM_LB.c:346-424): counts, then "index x y z" per node and
+ * "index type nodes n1 n2 n3" per triangle, 1-based.  Unlike the reference,
+ * which writes through whatever indices it reads, every index is checked and
+ * every node and triangle must appear. */
+pfg_status pfg_geom_read_neu(const char *path, int max_nodes, int max_tri, float *x,
+                             float *y, float *z, int32_t *tri, int32_t *n_nodes,
+                             int32_t *n_tri) {
+  if (!path || !n_nodes || !n_tri) return PFG_ERR_ARG;
+  FILE *in = std::fopen(path, "r");
+  if (!in) return PFG_ERR_IO;
+  pfg_status st = PFG_OK;
+  int nn = 0, nt = 0;
+  if (std::fscanf(in, "%d %d", &nn, &nt) != 2 || nn < 3 || nt < 1) st = PFG_ERR_ARG;
+  if (st == PFG_OK) {
+    *n_nodes = nn;
+    *n_tri = nt;
+  }
+  if (st == PFG_OK && x) {
+    if (!y || !z || !tri || nn > max_nodes || nt > max_tri) st = PFG_ERR_ARG;
+    std::vector<char> seen(st == PFG_OK ? (size_t)nn : 0, 0);
+    for (int i = 0; i < nn && st == PFG_OK; ++i) {
+      int idx;
+      double a, b, c;
+      if (std::fscanf(in, "%d %lf %lf %lf", &idx, &a, &b, &c) != 4 || idx < 1 || idx > nn ||
+          seen[(size_t)idx - 1] || !std::isfinite(a) || !std::isfinite(b) || !std::isfinite(c)) {
+        st = PFG_ERR_ARG;
+        break;
+      }
+      seen[(size_t)idx - 1] = 1;
+      x[idx - 1] = (float)a;
+      y[idx - 1] = (float)b;
+      z[idx - 1] = (float)c;
+    }
+    std::vector<char> tseen(st == PFG_OK ? (size_t)nt : 0, 0);
+    for (int t = 0; t < nt && st == PFG_OK; ++t) {
+      int idx, type, nodes, v[3];
+      if (std::fscanf(in, "%d %d %d %d %d %d", &idx, &type, &nodes, &v[0], &v[1], &v[2]) != 6 ||
+          idx < 1 || idx > nt || tseen[(size_t)idx - 1]) {
+        st = PFG_ERR_ARG;
+        break;
+      }
+      tseen[(size_t)idx - 1] = 1;
+      for (int k = 0; k < 3 && st == PFG_OK; ++k) {
+        if (v[k] < 1 || v[k] > nn) st = PFG_ERR_ARG;
+        else tri[3 * (idx - 1) + k] = v[k] - 1;
+      }
+    }
+  }
+  std::fclose(in);
+  return st;
+}
+
 pfg_status pfg_geom_area_volume(int n_tri, const int32_t *tri, const float *x,
                                 const float *y, const float *z, double *tri_S,
                                 double *S_all, double *V) {
@@ -310,17 +363,81 @@ Shape platelet_shape(int elements) {
   return {elements, 4.0f * s, 4.0f * s, 1.5f * s};
 }
 
-/* Append one platelet (mesh, rest area/volume, mesh edges, receptors). */
-pfg_status add_platelet(pfg_scene *S, const Shape &sh, float cx, float cy, float cz,
-                        float jitter, uint64_t seed, int receptors, uint32_t pid) {
+/* A platelet surface read from a file: nodes relative to their centroid,
+ * already scaled to lattice units. */
+struct Mesh {
+  std::vector<float> x, y, z;
+  std::vector<int32_t> tri;
+};
+
+pfg_status load_mesh(const char *path, float scale, Mesh *m, Shape *sh) {
+  int32_t nn = 0, nt = 0;
+  pfg_status st = pfg_geom_read_neu(path, 0, 0, nullptr, nullptr, nullptr, nullptr, &nn, &nt);
+  if (st != PFG_OK) return st;
+  m->x.resize((size_t)nn);
+  m->y.resize((size_t)nn);
+  m->z.resize((size_t)nn);
+  m->tri.resize((size_t)3 * nt);
+  st = pfg_geom_read_neu(path, nn, nt, m->x.data(), m->y.data(), m->z.data(), m->tri.data(),
+                         &nn, &nt);
+  if (st != PFG_OK) return st;
+  double c[3] = {0, 0, 0};
+  for (int i = 0; i < nn; ++i) {
+    c[0] += m->x[(size_t)i];
+    c[1] += m->y[(size_t)i];
+    c[2] += m->z[(size_t)i];
+  }
+  for (double &v : c) v /= nn;
+  if (!(scale > 0)) { /* mean edge 0.75, the spacing of the generated platelets */
+    double len = 0;
+    for (int t = 0; t < nt; ++t)
+      for (int k = 0; k < 3; ++k) {
+        const int a = m->tri[(size_t)3 * t + k], b = m->tri[(size_t)3 * t + (k + 1) % 3];
+        const double dx = (double)m->x[(size_t)a] - m->x[(size_t)b];
+        const double dy = (double)m->y[(size_t)a] - m->y[(size_t)b];
+        const double dz = (double)m->z[(size_t)a] - m->z[(size_t)b];
+        len += std::sqrt(dx * dx + dy * dy + dz * dz);
+      }
+    len /= 3.0 * nt;
+    if (!(len > 0)) return PFG_ERR_ARG;
+    scale = (float)(0.75 / len);
+  }
+  float ext[3] = {0, 0, 0};
+  for (int i = 0; i < nn; ++i) {
+    m->x[(size_t)i] = (float)((m->x[(size_t)i] - c[0]) * scale);
+    m->y[(size_t)i] = (float)((m->y[(size_t)i] - c[1]) * scale);
+    m->z[(size_t)i] = (float)((m->z[(size_t)i] - c[2]) * scale);
+    ext[0] = std::max(ext[0], std::fabs(m->x[(size_t)i]));
+    ext[1] = std::max(ext[1], std::fabs(m->y[(size_t)i]));
+    ext[2] = std::max(ext[2], std::fabs(m->z[(size_t)i]));
+  }
+  *sh = {nn, ext[0], ext[1], ext[2]};
+  return PFG_OK;
+}
+
+/* Append one platelet (mesh, rest area/volume, mesh edges, receptors): the
+ * file mesh when given, else a generated ellipsoid. */
+pfg_status add_platelet(pfg_scene *S, const Shape &sh, const Mesh *mesh, float cx, float cy,
+                        float cz, float jitter, uint64_t seed, int receptors, uint32_t pid) {
   const int n = sh.elements;
   std::vector<float> x((size_t)n), y((size_t)n), z((size_t)n);
-  std::vector<int32_t> tri((size_t)3 * (2 * n - 4));
+  std::vector<int32_t> tri;
   int32_t nt = 0;
-  pfg_status st = pfg_geom_ellipsoid(n, sh.a, sh.b, sh.c, cx, cy, cz, jitter,
-                                     seed + 7919ull * pid, x.data(), y.data(),
-                                     z.data(), tri.data(), &nt);
-  if (st != PFG_OK) return st;
+  if (mesh) {
+    for (int i = 0; i < n; ++i) {
+      x[(size_t)i] = cx + mesh->x[(size_t)i];
+      y[(size_t)i] = cy + mesh->y[(size_t)i];
+      z[(size_t)i] = cz + mesh->z[(size_t)i];
+    }
+    tri = mesh->tri;
+    nt = (int32_t)(tri.size() / 3);
+  } else {
+    tri.resize((size_t)3 * (2 * n - 4));
+    pfg_status st = pfg_geom_ellipsoid(n, sh.a, sh.b, sh.c, cx, cy, cz, jitter,
+                                       seed + 7919ull * pid, x.data(), y.data(),
+                                       z.data(), tri.data(), &nt);
+    if (st != PFG_OK) return st;
+  }
   const int base = (int)S->x.size();
   const int tbase = (int)(S->tri.size() / 3);
   const int body = (int)S->plt_S0.size();
@@ -548,8 +665,17 @@ pfg_status pfg_scene_build(const pfg_scene_opts *o, pfg_params *params, pfg_scen
       }
   }
   /* platelets */
+  Mesh mesh;
+  Shape sh = platelet_shape(elems);
+  if (nplt > 0 && o->mesh_path) {
+    st = load_mesh(o->mesh_path, o->mesh_scale, &mesh, &sh);
+    if (st != PFG_OK) {
+      delete S;
+      return st;
+    }
+  }
+  const Mesh *pm = o->mesh_path ? &mesh : nullptr;
   if (nplt > 0) {
-    const Shape sh = platelet_shape(elems);
     std::vector<V3> centres;
     if (o->config == PFG_SCENE_CHANNEL) {
       centres.push_back({0.5 * p.nx, tcy, tcz});
@@ -557,7 +683,7 @@ pfg_status pfg_scene_build(const pfg_scene_opts *o, pfg_params *params, pfg_scen
       /* lowest element half a cell above the ligand wall */
       centres.push_back({0.25 * p.nx, tcy, p.wall_z + sh.c + 0.5});
     } else {
-      const float rad = sh.a + 0.5f;
+      const float rad = std::max(std::max(sh.a, sh.b), sh.c) + 0.5f;
       const float lo[3] = {0.f, tube_r > 0 ? tcy - tube_r : rad + 1.f,
                            tube_r > 0 ? tcz - tube_r : rad + 1.f};
       const float hi[3] = {(float)p.nx, tube_r > 0 ? tcy + tube_r : p.ny - rad - 1.f,
@@ -566,7 +692,7 @@ pfg_status pfg_scene_build(const pfg_scene_opts *o, pfg_params *params, pfg_scen
       if ((int)centres.size() < nplt) st = PFG_ERR_ARG;
     }
     for (size_t q = 0; q < centres.size() && st == PFG_OK; ++q)
-      st = add_platelet(S, sh, (float)centres[q].x, (float)centres[q].y,
+      st = add_platelet(S, sh, pm, (float)centres[q].x, (float)centres[q].y,
                         (float)centres[q].z, 0.05f, seed, recs, (uint32_t)q);
   }
   if (S->plt_elem_start.empty()) {

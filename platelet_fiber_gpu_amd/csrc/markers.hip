@@ -902,12 +902,57 @@ __device__ inline float link_tension(const MArgs &a, float l, float l0) {
   return (a.fiber_k / l0) * (0.25f / (tl * tl) - 0.25f + x);
 }
 
-/* One wavefront per marker: the lanes stride over the Morse candidates of
- * the neighbouring bins (consecutive lanes read consecutive packed
- * positions), then over the marker's CSR lists (triangles, edges, receptor
- * shares, fibrin links); a butterfly reduction sums the lanes. */
+/* Calls f(s, e) for each neighbour range of bin b3 in the array whose bins
+ * start at starts[] (the ranges of nbr_range, without an indexed array). */
+template <typename F>
+__device__ inline void for_nbr_ranges(const MLat &L, const int32_t *starts, const int b3[3],
+                                      F f) {
+  const int bx = b3[0], nbx = L.nb[0];
+  int runs = 1, f0, l0, f1 = 0, l1 = 0;
+  if (L.periodic[0] && nbx <= 3) {
+    f0 = 0;
+    l0 = nbx - 1;
+  } else if (L.periodic[0] && (bx == 0 || bx == nbx - 1)) {
+    runs = 2;
+    f0 = bx == 0 ? nbx - 1 : nbx - 2;
+    l0 = nbx - 1;
+    l1 = bx == 0 ? 1 : 0;
+  } else {
+    f0 = max(bx - 1, 0);
+    l0 = min(bx + 1, nbx - 1);
+  }
+  int cy, cz;
+  nbr_bin_j(b3[1], L.nb[1], L.periodic[1], 0, &cy);
+  nbr_bin_j(b3[2], L.nb[2], L.periodic[2], 0, &cz);
+  for (int jz = 0; jz < cz; ++jz) {
+    const int z = nbr_bin_j(b3[2], L.nb[2], L.periodic[2], jz, &cz);
+    for (int jy = 0; jy < cy; ++jy) {
+      const int y = nbr_bin_j(b3[1], L.nb[1], L.periodic[1], jy, &cy);
+      const int row = L.bin0 + L.nb[0] * (y + L.nb[1] * z);
+      f(starts[row + f0], starts[row + l0 + 1]);
+      if (runs == 2) f(starts[row + f1], starts[row + l1 + 1]);
+    }
+  }
+}
+
+/* Sum over an aligned group of G lanes. */
+template <int G>
+__device__ inline float group_sum(float v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+/* A group of G lanes per marker (G = 64: one wavefront, markers in index
+ * order, the Morse candidates of all neighbour ranges flattened over the
+ * lanes; G < 64: markers in bin-sorted order, so that the groups of a
+ * wavefront mostly share their neighbour bins and read the same packed
+ * positions, each group striding through the ranges).  The lanes of a group
+ * then stride over the marker's CSR lists (triangles, edges, receptor
+ * shares, fibrin links) and a butterfly reduction sums them. */
+template <int G>
 __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & (G - 1);
   if (blockIdx.x == 0 && threadIdx.x == 0 && a.m.n_rec > 0 && a.detach_rate > 0.f) {
     /* detach test (semLB.cu:451-455): no bond held after this step's update */
     const int64_t step = *a.m.d_step;
@@ -920,8 +965,9 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
     }
   }
   __shared__ int sh_off[4][NBR_RANGES], sh_end[4][NBR_RANGES];
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= a.m.n) return;
+  const int k = (blockIdx.x * 256 + threadIdx.x) / G;
+  if (k >= a.m.n) return;
+  const int i = G == 64 ? k : a.m.bin_items[k];
   const int kind = a.m.kind[i];
   const float4 pi = a.m.pos4[i];
   const float xi = pi.x, yi = pi.y, zi = pi.z;
@@ -929,20 +975,26 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
   if (kind == PFG_MARKER_ELEMENT) {
     const int body = a.m.body[i];
     if (a.morse_u0 != 0.f) {
-      /* the candidates of all neighbour ranges as one list over the lanes:
-       * lane r writes range r's offset and end into this wave's LDS table */
       const MLat L = lat_of(a, i);
       const int b = a.m.marker_bin[i] - L.bin0;
       const int b3[3] = {b % L.nb[0], (b / L.nb[0]) % L.nb[1], b / (L.nb[0] * L.nb[1])};
-      int s = 0, e = 0;
-      const int nr = nbr_range(L, a.m.bin_start, b3, lane, &s, &e);
-      int *off = sh_off[threadIdx.x >> 6], *ends = sh_end[threadIdx.x >> 6];
-      const int total = flatten_ranges(nr, s, e, off, ends);
       const float tag = pi.w;
-      int r = 0;
-      for (int c = lane; c < total; c += 64) {
-        while (c >= ends[r]) ++r;
-        morse_pair(a, a.m.sorted_pos[c + off[r]], tag, xi, yi, zi, fx, fy, fz);
+      if constexpr (G == 64) {
+        /* lane r writes range r's offset and end into this wave's LDS table */
+        int s = 0, e = 0;
+        const int nr = nbr_range(L, a.m.bin_start, b3, lane, &s, &e);
+        int *off = sh_off[threadIdx.x >> 6], *ends = sh_end[threadIdx.x >> 6];
+        const int total = flatten_ranges(nr, s, e, off, ends);
+        int r = 0;
+        for (int c = lane; c < total; c += 64) {
+          while (c >= ends[r]) ++r;
+          morse_pair(a, a.m.sorted_pos[c + off[r]], tag, xi, yi, zi, fx, fy, fz);
+        }
+      } else {
+        for_nbr_ranges(L, a.m.bin_start, b3, [&](int s, int e) {
+          for (int c = s + lane; c < e; c += G)
+            morse_pair(a, a.m.sorted_pos[c], tag, xi, yi, zi, fx, fy, fz);
+        });
       }
     }
     const int t0 = a.m.m2t_start[i], nt = (a.ks != 0.f || a.kv != 0.f) ? a.m.m2t_start[i + 1] - t0 : 0;
@@ -955,7 +1007,7 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
       g_area = (float)((Sall - S0all) / S0all);
       cvol = -a.kv * (float)((V - V0) / V0) / 6.0f;
     }
-    for (int k = lane; k < nt + ne + nr; k += 64) {
+    for (int k = lane; k < nt + ne + nr; k += G) {
       if (k < nt) {
         area_volume_term(a, a.m.m2t[t0 + k], g_area, cvol, fx, fy, fz);
       } else if (k < nt + ne) {
@@ -993,7 +1045,7 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
     /* fibrin links in node-gather form (fiber.cpp:647-706) */
     const float sgn = a.fiber_ref_sign ? -1.0f : 1.0f;
     const int l0i = a.m.m2l_start[i], nl = a.m.m2l_start[i + 1] - l0i;
-    for (int k = lane; k < nl; k += 64) {
+    for (int k = lane; k < nl; k += G) {
       const float4 pj = a.m.pos4[a.m.m2l_nbr[l0i + k]];
       const float l0 = a.m.m2l_l0[l0i + k];
       const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
@@ -1014,9 +1066,9 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
       }
     }
   }
-  fx = wave_sum(fx);
-  fy = wave_sum(fy);
-  fz = wave_sum(fz);
+  fx = group_sum<G>(fx);
+  fy = group_sum<G>(fy);
+  fz = group_sum<G>(fz);
   if (lane == 0) {
     a.m.fx[i] = fx;
     a.m.fy[i] = fy;
@@ -1485,7 +1537,16 @@ hipError_t launch_forces(pfg_handle *h) {
   if (h->m.n == 0) return hipSuccess;
   MArgs a;
   make_margs(h, &a);
-  hipLaunchKernelGGL(forces_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
+  /* few markers: a wavefront each, for parallelism; many: 4 lanes each in
+   * bin order, for fewer instructions per marker */
+  const int G = h->forces_lanes ? h->forces_lanes : (h->m.n >= FORCES_SORTED_MIN ? 4 : 64);
+  const unsigned grid = (unsigned)blocks_for((int64_t)h->m.n * G, 256);
+  switch (G) {
+    case 1: hipLaunchKernelGGL(forces_kernel<1>, dim3(grid), dim3(256), 0, h->stream, a); break;
+    case 4: hipLaunchKernelGGL(forces_kernel<4>, dim3(grid), dim3(256), 0, h->stream, a); break;
+    case 16: hipLaunchKernelGGL(forces_kernel<16>, dim3(grid), dim3(256), 0, h->stream, a); break;
+    default: hipLaunchKernelGGL(forces_kernel<64>, dim3(grid), dim3(256), 0, h->stream, a); break;
+  }
   return hipGetLastError();
 }
 

@@ -29,6 +29,9 @@ constexpr int IB_BOX_CELLS = 2048;
 constexpr size_t IB_BOX_BYTES = 3 * IB_BOX_CELLS * sizeof(long long);
 /* Marker count from which spreading is staged through LDS by default. */
 constexpr int IB_STAGED_MIN = 32768;
+/* Marker count from which the forces take 4 lanes per marker in bin order
+ * (below it, a wavefront per marker). */
+constexpr int FORCES_SORTED_MIN = 32768;
 
 /* Everything the fused LB kernel needs, passed by value. */
 struct LBArgs {
@@ -201,6 +204,7 @@ struct pfg_handle {
   bool fluid_only = false;
   bool halt_on_detach = false; /* steps after the detach test fired change nothing */
   int ib_staging = 0;          /* spreading: 0 auto, 1 LDS-staged chunks, 2 direct */
+  int forces_lanes = 0;        /* lanes per marker in the forces: 0 auto, 1, 4, 16, 64 */
   /* LB rows away from IB stencils update on stream2 while the marker
    * pipeline runs; the rows the stencils touch update after spreading.  Off
    * by default: on MI355X the LB launch saturates HBM and the CU slots, and

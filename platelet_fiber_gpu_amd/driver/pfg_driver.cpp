@@ -14,13 +14,18 @@
  *   Output/cells{Pos,Vel,Force}ASCII.dat  the same as text, every frame
  *   Output/cellsPauseTime.dat  appended "pause time, seed" at detach
  *
- * The platelet geometry comes from the seeded scene generator (the reference
- * reads a NEU mesh); positions, velocities and forces are written in lattice
- * units; the pause time is steps x DELTA_T.
+ * The platelets are the NEU mesh named by --mesh, else by the file's
+ * NEU_PATH when that file exists (as written, next to the parameter file or
+ * in its parent directory), read as NEUReader reads it (gpuSEM_LB.c:346-424)
+ * and scaled by 1 / DELTA_X (mesh units to lattice units; --mesh-scale
+ * overrides); without a mesh file they come from the seeded generator.
+ * Positions, velocities and forces are written in lattice units; the pause
+ * time is steps x DELTA_T.
  *
  * Usage: pfg_driver [-d device] [-p parameter file] [-o output dir]
  *                   [--scene shear|channel|thrombus|large] [--lattice nx ny nz]
  *                   [--store f32|f16|bf16] [--seeds first last] [--max-steps n]
+ *                   [--mesh file.neu] [--mesh-scale s] [--no-mesh]
  *                   [--rates-from-file] [--scene-counts] [--parse-only]
  * Exit status: 0 done, 1 parameter or usage error, 2 I/O error, 11 device
  * error (the reference's exit code for an invalid device), 3 other failure.
@@ -143,6 +148,9 @@ struct Options {
   bool rates_from_file = false;
   bool scene_counts = false;
   bool parse_only = false;
+  std::string mesh;      /* --mesh */
+  float mesh_scale = 0;  /* --mesh-scale; 0: 1 / DELTA_X */
+  bool no_mesh = false;  /* --no-mesh: generated platelets even when NEU_PATH exists */
 };
 
 int usage(const char *msg) {
@@ -151,6 +159,7 @@ int usage(const char *msg) {
                "usage: pfg_driver [-d device] [-p parameter file] [-o output dir]\n"
                "       [--scene shear|channel|thrombus|large] [--lattice nx ny nz]\n"
                "       [--store f32|f16|bf16] [--seeds first last] [--max-steps n]\n"
+               "       [--mesh file.neu] [--mesh-scale s] [--no-mesh]\n"
                "       [--rates-from-file] [--scene-counts] [--parse-only]\n");
   return 1;
 }
@@ -219,6 +228,18 @@ bool parse_args(int argc, char **argv, Options *o, std::string *err) {
         *err = "--max-steps must be positive";
         return false;
       }
+    } else if (a == "--mesh") {
+      if (!need(1)) return false;
+      o->mesh = argv[++i];
+    } else if (a == "--mesh-scale") {
+      if (!need(1)) return false;
+      o->mesh_scale = (float)std::atof(argv[++i]);
+      if (!(o->mesh_scale > 0)) {
+        *err = "--mesh-scale must be positive";
+        return false;
+      }
+    } else if (a == "--no-mesh") {
+      o->no_mesh = true;
     } else if (a == "--rates-from-file") {
       o->rates_from_file = true;
     } else if (a == "--scene-counts") {
@@ -313,10 +334,38 @@ double mean_edge_length(const pfg_model &m) {
   return s / m.n_edges;
 }
 
+bool readable(const std::string &path) {
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) return false;
+  std::fclose(f);
+  return true;
+}
+
+/* The mesh file of the run: --mesh, else NEU_PATH as written, next to the
+ * parameter file, or in its parent directory; "" when there is none. */
+std::string mesh_file(const Options &o, const ParFile &pf) {
+  if (o.no_mesh) return "";
+  if (!o.mesh.empty()) return o.mesh;
+  if (pf.neu.empty()) return "";
+  if (readable(pf.neu)) return pf.neu;
+  const size_t cut = o.par.find_last_of('/');
+  const std::string dir = cut == std::string::npos ? "." : o.par.substr(0, cut);
+  const size_t b = pf.neu.find_last_of('/');
+  const std::string base = b == std::string::npos ? pf.neu : pf.neu.substr(b + 1);
+  if (readable(dir + "/" + base)) return dir + "/" + base;
+  if (readable(dir + "/../" + pf.neu)) return dir + "/../" + pf.neu;
+  return "";
+}
+
 int build_scene(const Options &o, ParFile *pf, long long seed, SceneGuard *sg, pfg_params *p,
                 pfg_model *model) {
   pfg_scene_opts so;
   pfg_scene_opts_init(&so, o.scene);
+  const std::string mesh = mesh_file(o, *pf);
+  if (!mesh.empty()) {
+    so.mesh_path = mesh.c_str();
+    so.mesh_scale = o.mesh_scale > 0 ? o.mesh_scale : (pf->dx > 0 ? 1.0f / pf->dx : 1.0f);
+  }
   so.nx = o.nx;
   so.ny = o.ny;
   so.nz = o.nz;
@@ -329,10 +378,21 @@ int build_scene(const Options &o, ParFile *pf, long long seed, SceneGuard *sg, p
     so.n_platelets = pf->max_cell > 0 ? pf->max_cell : -1;
     if (pf->max_elem >= 4) so.elements = pf->max_elem;
     const int per_tri = pf->surf_elem > 0 ? (int)(pf->host[0] / pf->surf_elem / 2) : 0;
-    const int ntri = 2 * (so.elements > 0 ? so.elements : pf->max_elem) - 4;
+    int ntri = 2 * (so.elements > 0 ? so.elements : pf->max_elem) - 4;
+    if (!mesh.empty()) {
+      int32_t nn = 0, nt = 0;
+      if (pfg_geom_read_neu(mesh.c_str(), 0, 0, nullptr, nullptr, nullptr, nullptr, &nn, &nt) !=
+          PFG_OK) {
+        std::fprintf(stderr, "pfg_driver: cannot read mesh %s\n", mesh.c_str());
+        return 2;
+      }
+      ntri = nt;
+    }
     so.receptors = per_tri > 0 ? per_tri * ntri : -1;
   }
-  if (pfg_scene_build(&so, p, &sg->s) != PFG_OK) return 3;
+  const pfg_status bs = pfg_scene_build(&so, p, &sg->s);
+  if (bs == PFG_ERR_IO) return 2;
+  if (bs != PFG_OK) return 3;
   if (pfg_scene_model(sg->s, model) != PFG_OK) return 3;
   fill_derived(pf, p->nz, mean_edge_length(*model) * pf->dx);
   for (int k = 0; k < 20; ++k) p->sem[k] = pf->host[k];
@@ -439,8 +499,11 @@ int main(int argc, char **argv) {
     SceneGuard sg;
     pfg_params p;
     pfg_model model;
-    if (build_scene(o, &pf, first, &sg, &p, &model)) return 3;
-    std::printf("{\"max_steps\": %lld, \"out_freq\": %d, \"seeds\": [%lld, %lld], "
+    const int rc = build_scene(o, &pf, first, &sg, &p, &model);
+    if (rc) return rc;
+    std::printf("{\"mesh\": \"%s\", \"triangles\": %d, ", mesh_file(o, pf).c_str(),
+                model.n_tri);
+    std::printf("\"max_steps\": %lld, \"out_freq\": %d, \"seeds\": [%lld, %lld], "
                 "\"lattice\": [%d, %d, %d], \"platelets\": %d, \"markers\": %d, "
                 "\"receptors\": %d, \"tau\": %.9g, \"k_on\": %.9g, \"k_off0\": %.9g, "
                 "\"detach_rate\": %.9g, \"seed\": %llu, \"sem\": [",

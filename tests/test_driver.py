@@ -108,3 +108,23 @@ def test_missing_file_and_bad_arguments(tmp_path):
     assert r.returncode == 1 and "unknown argument" in r.stderr
     r = run("-p", FIXTURE, "--seeds", "5", "2")
     assert r.returncode == 1
+
+
+def test_mesh_from_parameter_file():
+    """NEU_PATH (Input/Platelet_Mesh_958.neu) resolves next to the parameter
+    file: 958 elements, 1912 triangles and RECEPTOR / SURF_ELEM / 2 = 2
+    receptors per triangle (gpuSEM_LB.c:723), as the reference sets up."""
+    r = run("-p", FIXTURE, "--parse-only")
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout)
+    assert out["mesh"].endswith("Platelet_Mesh_958.neu")
+    assert out["markers"] == 958 and out["triangles"] == 1912
+    assert out["receptors"] == 2 * 1912
+    r = run("-p", FIXTURE, "--parse-only", "--no-mesh")
+    assert json.loads(r.stdout)["mesh"] == ""
+
+
+def test_mesh_missing_file_is_an_io_error(tmp_path):
+    r = run("-p", FIXTURE, "--parse-only", "--mesh", str(tmp_path / "none.neu"))
+    assert r.returncode == 2
+    assert "cannot read mesh" in r.stderr

@@ -65,6 +65,28 @@ def test_channel_platelet_config(gpu, po):
     assert pos[0].mean() > sc.x.mean()  # the platelet moved with the flow
 
 
+def test_reference_mesh_platelet(gpu, po):
+    """The reference's own platelet mesh (Platelet_Mesh_958.neu, 958 nodes,
+    1912 triangles) in the channel, with mesh springs and the area and
+    volume constraints on, 50 steps against the oracle."""
+    import os
+    pfg = gpu
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fixtures",
+                        "Platelet_Mesh_958.neu")
+    sc = pfg.Scene(pfg.SCENE_CHANNEL, mesh_path=path)
+    sc.params.k_spring, sc.params.ks, sc.params.kv = 0.3, 0.5, 0.4
+    g, o = run_both(pfg, po, sc, 50)
+    ofx = po.fields(po.lattice(sc.params), o.f())
+    err_u = rel_l2(np.stack(g["u"]), np.stack(ofx[1:]))
+    pos = np.stack([g["mk"][k] for k in ("x", "y", "z")])
+    opos = np.stack([o.marker(k) for k in ("x", "y", "z")])
+    err_x = np.abs(pos - opos).max()
+    print(f"mesh platelet 50 steps: rel L2 u {err_u:.2e}, max position error {err_x:.2e}")
+    assert sc.n_markers == 958
+    assert err_u < 1e-3
+    assert err_x < 1e-4
+
+
 def test_shear_with_bonds_short(gpu, po):
     pfg = gpu
     sc = pfg.Scene(pfg.SCENE_SHEAR, nx=64, ny=32, nz=32, elements=256, receptors=128,

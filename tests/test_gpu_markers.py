@@ -73,9 +73,13 @@ def test_membrane_sums_fp64(gpu, po):
     assert np.allclose(tS, otS, rtol=1e-12)
 
 
+@pytest.mark.parametrize("lanes", [0, 4, 1])
 @pytest.mark.parametrize("what", ["morse", "springs_area_volume", "wall", "fibrin_wlc",
                                   "fibrin_linear_refsign"])
-def test_forces(gpu, po, what):
+def test_forces(gpu, po, what, lanes):
+    """Marker forces against the oracle with a wavefront per marker (0: the
+    automatic choice at this size) and with 4 or 1 lanes per marker in
+    cell-list order."""
     pfg = gpu
     if what.startswith("fibrin"):
         sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=64, ny=48, nz=48, n_platelets=2, elements=150,
@@ -92,6 +96,7 @@ def test_forces(gpu, po, what):
         p.fiber_law, p.fiber_ref_sign, p.fiber_k = pfg.FIBER_LINEAR, 1, 0.3
     s, o = pair(pfg, po, sc, p)
     with s:
+        s.set_forces_lanes(lanes)
         set_pos(s, o, *perturbed(sc, 0.1))
         for st in (pfg.STAGE_CELLS, pfg.STAGE_MEMBRANE, pfg.STAGE_BONDS, pfg.STAGE_FORCES):
             s.stage(st)
