@@ -143,6 +143,12 @@ typedef struct pfg_params {
   int32_t bin_cells;           /* bin edge in lattice cells (>= every cutoff)      */
   /* ---- reference parameter block, kept for output/provenance ---- */
   float sem[20];               /* hostSEMparameters[0..19], gpuSEM_LB.c:1206-1233  */
+  /* ---- membrane bending ---- */
+  float kbend;                 /* Kbend / F0 of the curvature chain
+                                  (sem_kernel.cu:874-1128, 1499-1638): each
+                                  element gets kbend (1/A_i) (Laplacian of the
+                                  mean curvature) n_i; 0: off. Needs closed,
+                                  consistently wound platelet meshes.          */
 } pfg_params;
 
 /* Host description of the immersed structures; all arrays caller-owned and only

@@ -427,9 +427,134 @@ struct orc_state {
   int nb[3];
   int *bin_start, *items;
   int64_t step, total_bonds, detached, detach_step, near, events;
+  /* bending chain: 6-node triangles (3 vertices, 3 edge-midpoint nodes
+   * numbered n + edge), triangles per node, ordered vertex rings */
+  int n_enode, bend_ok;
+  int *tri6, *node_cnt, *ring_start, *ring, *ring_ok;
+  double *node_n, *kapa, *km, *fb;
 };
 
 static void *xcalloc(size_t n, size_t sz) { return calloc(n > 0 ? n : 1, sz); }
+
+/* ---- bending topology (gpuSEM_LB.c:846-936) ----
+ * Edge-midpoint nodes: the edge of local vertices (0,1) goes to slot 3,
+ * (0,2) to slot 4, (1,2) to slot 5 (the switch on the sum of the local
+ * indices, gpuSEM_LB.c:859-871).  node_cnt: triangles touching a node
+ * (node_nbrElemNum).  Vertex rings (node_nbrNodes): start at the vertex after
+ * i in its first triangle, then repeatedly take, among the triangles of i
+ * not yet used, the first one holding the current neighbour and step to the
+ * vertex before it there (gpuSEM_LB.c:900-936).  Edge ids in order of the
+ * sorted vertex pairs.  ring_ok[i] = 0 when the ring of i does not close (a
+ * vertex on the border of an open mesh): it gets no bending force. */
+static int edge_key_cmp(const void *a, const void *b) {
+  const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+static void bend_topology(orc_state *s) {
+  const int n = s->n, T = s->n_tri;
+  s->bend_ok = T > 0;
+  if (T == 0) return;
+  /* edges: sort (key, first appearance) to give each edge an id */
+  int64_t *keys = (int64_t *)xcalloc((size_t)3 * T, 2 * sizeof(int64_t));
+  static const int sl[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int t = 0; t < T; ++t)
+    for (int k = 0; k < 3; ++k) {
+      int a = s->tri[3 * t + sl[k][0]], b = s->tri[3 * t + sl[k][1]];
+      if (a > b) { const int c = a; a = b; b = c; }
+      keys[2 * (3 * t + k)] = (int64_t)a * n + b;
+      keys[2 * (3 * t + k) + 1] = 3 * t + k;
+    }
+  int64_t (*kv)[2] = (int64_t (*)[2])keys;
+  qsort(kv, (size_t)3 * T, 2 * sizeof(int64_t), edge_key_cmp);
+  int ne = 0;
+  int *slot_edge = (int *)xcalloc((size_t)3 * T, sizeof(int));
+  for (int i = 0; i < 3 * T;) {
+    int j = i;
+    while (j < 3 * T && kv[j][0] == kv[i][0]) ++j;
+    for (int a = i; a < j; ++a) slot_edge[kv[a][1]] = ne;
+    ++ne;
+    i = j;
+  }
+  s->n_enode = ne;
+  s->tri6 = (int *)xcalloc((size_t)6 * T, sizeof(int));
+  s->node_cnt = (int *)xcalloc((size_t)n + ne, sizeof(int));
+  for (int t = 0; t < T; ++t) {
+    for (int k = 0; k < 3; ++k) s->tri6[6 * t + k] = s->tri[3 * t + k];
+    for (int k = 0; k < 3; ++k) s->tri6[6 * t + 3 + k] = n + slot_edge[3 * t + k];
+    for (int k = 0; k < 6; ++k) s->node_cnt[s->tri6[6 * t + k]]++;
+  }
+  /* triangles of each vertex in ascending order (node_share_Elem) */
+  int *vstart = (int *)xcalloc((size_t)n + 1, sizeof(int));
+  for (int t = 0; t < T; ++t)
+    for (int k = 0; k < 3; ++k) vstart[s->tri[3 * t + k] + 1]++;
+  for (int i = 0; i < n; ++i) vstart[i + 1] += vstart[i];
+  int *vtri = (int *)xcalloc((size_t)vstart[n], sizeof(int));
+  int *fill = (int *)xcalloc((size_t)n, sizeof(int));
+  for (int t = 0; t < T; ++t)
+    for (int k = 0; k < 3; ++k) {
+      const int v = s->tri[3 * t + k];
+      vtri[vstart[v] + fill[v]++] = t;
+    }
+  s->ring_start = vstart;
+  s->ring = (int *)xcalloc((size_t)vstart[n], sizeof(int));
+  s->ring_ok = (int *)xcalloc((size_t)n, sizeof(int));
+  for (int i = 0; i < n; ++i) {
+    const int c = vstart[i + 1] - vstart[i];
+    if (c == 0) continue;
+    int *el = vtri + vstart[i];
+    int *rg = s->ring + vstart[i];
+    int cur = -1, ok = 1;
+    const int e0 = el[0];
+    for (int j = 0; j < 3; ++j)
+      if (s->tri[3 * e0 + j] == i) cur = s->tri[3 * e0 + (j == 2 ? 0 : j + 1)];
+    rg[0] = cur;
+    for (int j = 1; j < c; ++j) {
+      int found = 0;
+      for (int k = j; k < c && !found; ++k) {
+        const int e = el[k];
+        for (int l = 0; l < 3; ++l) {
+          if (s->tri[3 * e + l] == cur) {
+            found = 1;
+            cur = s->tri[3 * e + (l == 0 ? 2 : l - 1)];
+            rg[j] = cur;
+            const int tmp = el[j];
+            el[j] = el[k];
+            el[k] = tmp;
+            break;
+          }
+        }
+      }
+      if (!found) {
+        ok = 0;
+        rg[j] = i;
+      }
+    }
+    /* closed: the last neighbour and the first share a triangle of i */
+    if (ok && c > 2) {
+      int shared = 0;
+      for (int k = 0; k < c && !shared; ++k) {
+        const int e = el[k];
+        int hl = 0, hf = 0;
+        for (int l = 0; l < 3; ++l) {
+          hl |= s->tri[3 * e + l] == rg[c - 1];
+          hf |= s->tri[3 * e + l] == rg[0];
+        }
+        shared = hl && hf;
+      }
+      ok = shared;
+    }
+    s->ring_ok[i] = ok && c > 2;
+  }
+  s->node_n = (double *)xcalloc((size_t)3 * (n + ne), sizeof(double));
+  s->kapa = (double *)xcalloc((size_t)9 * (n + ne), sizeof(double));
+  s->km = (double *)xcalloc((size_t)n + ne, sizeof(double));
+  s->fb = (double *)xcalloc((size_t)3 * n, sizeof(double));
+  free(keys);
+  free(slot_edge);
+  free(vtri);
+  free(fill);
+}
 
 orc_state *orc_create(const pfg_params *p, const pfg_model *m, const uint8_t *flags) {
   orc_state *s = (orc_state *)calloc(1, sizeof(orc_state));
@@ -543,6 +668,7 @@ orc_state *orc_create(const pfg_params *p, const pfg_model *m, const uint8_t *fl
   s->bin_start = (int *)xcalloc((size_t)s->nb[0] * s->nb[1] * s->nb[2] + 1, sizeof(int));
   s->items = (int *)xcalloc((size_t)n, sizeof(int));
   s->detach_step = -1;
+  bend_topology(s);
   return s;
 }
 
@@ -588,6 +714,15 @@ void orc_destroy(orc_state *s) {
   free(s->link_l0);
   free(s->bin_start);
   free(s->items);
+  free(s->tri6);
+  free(s->node_cnt);
+  free(s->ring_start);
+  free(s->ring);
+  free(s->ring_ok);
+  free(s->node_n);
+  free(s->kapa);
+  free(s->km);
+  free(s->fb);
   free(s);
 }
 
@@ -790,6 +925,8 @@ static void stage_bonds(orc_state *s) {
 }
 
 /* ---- forces ---- */
+static void bend_chain(orc_state *s);
+
 static void stage_forces(orc_state *s) {
   const pfg_params *p = &s->p;
   double *X = s->mk[0], *Y = s->mk[1], *Z = s->mk[2];
@@ -948,6 +1085,15 @@ static void stage_forces(orc_state *s) {
     FY[b] -= c * dy;
     FZ[b] -= c * dz;
   }
+  /* membrane bending (sem_Laplace_Kapa_kernel after the area/volume forces) */
+  if (p->kbend != 0 && s->bend_ok) {
+    bend_chain(s);
+    for (int i = 0; i < s->n; ++i) {
+      FX[i] += s->fb[3 * i];
+      FY[i] += s->fb[3 * i + 1];
+      FZ[i] += s->fb[3 * i + 2];
+    }
+  }
   /* bond reaction on fibrin-node ligands */
   if (s->fib_lig) {
     for (int i = 0; i < s->n; ++i) {
@@ -959,6 +1105,250 @@ static void stage_forces(orc_state *s) {
       FZ[i] -= s->rec_force[3 * owner + 2];
     }
   }
+}
+
+/* ---- membrane bending chain ----
+ * calculate_A (sem_kernel.cu:724-820): unit triangle normals
+ *   n_t = -(e12 x e13)/|e12 x e13| summed onto the 6 nodes of the triangle;
+ * surface_tension (874-900): node normal = normalize(sum / count);
+ * calculate_Kapa (950-1055): at each node of the quadratic 6-node triangle
+ *   (midpoints at the vertex averages) solve K dx/dxi = dn/dxi,
+ *   K dx/deta = dn/deta, K n = 0 row by row and add K^T to the node;
+ * calculate_m (1057-1100): K / count, symmetrised, km = trace / 2;
+ * Laplace_Kapa (1499-1638): at each vertex, over its ring, the cotangent sum
+ *   0.5 (cot a + cot b)(km_j - km_i) and the area of the region bounded by
+ *   edge midpoints and circumcentres (an obtuse triangle's long-edge
+ *   midpoint instead), both sides by Heron's formula; force
+ *   kbend (sum / A) n on the element. */
+static const double QN[6][6] = {{2, 2, 4, -3, -3, 1}, {2, 0, 0, -1, 0, 0},
+                                {0, 2, 0, 0, -1, 0},  {-4, 0, -4, 4, 0, 0},
+                                {0, -4, -4, 0, 4, 0}, {0, 0, 4, 0, 0, 0}};
+static const double QXE[6][2] = {{0, 0}, {1, 0}, {0, 1}, {0.5, 0}, {0, 0.5}, {0.5, 0.5}};
+
+/* Forward elimination with partial pivoting and unit pivots on an m x n
+ * augmented matrix (gaussian_elimination, sem_kernel.cu:249-284). */
+static void gauss_elim(double *g, int m, int n) {
+  int i = 0, j = 0;
+  while (i < m && j < n - 1) {
+    int maxi = i;
+    for (int k = i + 1; k < m; ++k)
+      if (fabs(g[k * n + j]) > fabs(g[maxi * n + j])) maxi = k;
+    if (g[maxi * n + j] != 0) {
+      double initial = 0;
+      for (int k = j; k < n; ++k) {
+        const double b = g[i * n + k];
+        g[i * n + k] = g[maxi * n + k];
+        g[maxi * n + k] = b;
+        if (k == j) initial = g[i * n + j];
+        g[i * n + k] /= initial;
+      }
+      for (int k = i + 1; k < m; ++k) {
+        const double f = g[k * n + j];
+        for (int l = j; l < n; ++l) g[k * n + l] -= f * g[i * n + l];
+      }
+      ++i;
+    }
+    ++j;
+  }
+}
+
+/* Rows of K from [A1; A3; A5] K[r] = [A2[r]; A4[r]; A6[r]]
+ * (solve_linear_eqns, sem_kernel.cu:286-305). */
+static void solve_rows(double K[3][3], const double A1[3], const double A2[3],
+                       const double A3[3], const double A4[3], const double A5[3],
+                       const double A6[3]) {
+  for (int r = 0; r < 3; ++r) {
+    double g[12];
+    for (int c = 0; c < 3; ++c) {
+      g[c] = A1[c];
+      g[4 + c] = A3[c];
+      g[8 + c] = A5[c];
+    }
+    g[3] = A2[r];
+    g[7] = A4[r];
+    g[11] = A6[r];
+    gauss_elim(g, 3, 4);
+    K[r][2] = g[11];
+    K[r][1] = g[7] - K[r][2] * g[6];
+    K[r][0] = g[3] - K[r][2] * g[2] - K[r][1] * g[1];
+  }
+}
+
+static double dist3(const double a[3], const double b[3]) {
+  return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) +
+              (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+static double dot3(const double a[3], const double b[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* Ring sums of vertex i for the per-vertex values f (sem_kernel.cu:1533-1628):
+ * *sum = sum 0.5 (cot a + cot b)(f_j - f_i), *area = A_i. */
+static void ring_laplace(const orc_state *s, int i, const double *f, double *sum,
+                         double *area) {
+  const double *X = s->mk[0], *Y = s->mk[1], *Z = s->mk[2];
+  const int r0 = s->ring_start[i], c = s->ring_start[i + 1] - r0;
+  const double xi[3] = {X[i], Y[i], Z[i]};
+  double Ai = 0, sm = 0;
+  for (int q = 0; q < c; ++q) {
+    const int j = s->ring[r0 + q];
+    const int v1 = s->ring[r0 + (q == 0 ? c - 1 : q - 1)];
+    const int v2 = s->ring[r0 + (q == c - 1 ? 0 : q + 1)];
+    const double xj[3] = {X[j], Y[j], Z[j]}, x1[3] = {X[v1], Y[v1], Z[v1]},
+                 x2[3] = {X[v2], Y[v2], Z[v2]};
+    double ri1[3], r1j[3], rij[3], ri2[3], r2j[3], cij[3], ci2[3], c2j[3], ce[3];
+    for (int k = 0; k < 3; ++k) {
+      ri1[k] = xi[k] - x1[k];
+      r1j[k] = x1[k] - xj[k];
+      rij[k] = xi[k] - xj[k];
+      ri2[k] = xi[k] - x2[k];
+      r2j[k] = x2[k] - xj[k];
+      cij[k] = 0.5 * (xi[k] + xj[k]);
+      ci2[k] = 0.5 * (xi[k] + x2[k]);
+      c2j[k] = 0.5 * (x2[k] + xj[k]);
+    }
+    const double ai = dot3(rij, ri2), a2 = -dot3(ri2, r2j), aj = dot3(rij, r2j),
+                 a1 = -dot3(r1j, ri1);
+    if (ai < 0) {
+      for (int k = 0; k < 3; ++k) ce[k] = c2j[k];
+    } else if (a2 < 0) {
+      for (int k = 0; k < 3; ++k) ce[k] = cij[k];
+    } else if (aj < 0) {
+      for (int k = 0; k < 3; ++k) ce[k] = ci2[k];
+    } else { /* circumcentre of (xi, xj, x2): two bisector planes and the plane */
+      double g[12] = {0};
+      for (int k = 0; k < 3; ++k) {
+        g[k] = rij[k];
+        g[4 + k] = ri2[k];
+        g[3] += cij[k] * rij[k];
+        g[7] += ci2[k] * ri2[k];
+      }
+      g[8] = (xj[1] - xi[1]) * (x2[2] - xi[2]) - (xj[2] - xi[2]) * (x2[1] - xi[1]);
+      g[9] = (xj[2] - xi[2]) * (x2[0] - xi[0]) - (xj[0] - xi[0]) * (x2[2] - xi[2]);
+      g[10] = (xj[0] - xi[0]) * (x2[1] - xi[1]) - (xj[1] - xi[1]) * (x2[0] - xi[0]);
+      g[11] = g[8] * xi[0] + g[9] * xi[1] + g[10] * xi[2];
+      gauss_elim(g, 3, 4);
+      ce[2] = g[11];
+      ce[1] = g[7] - ce[2] * g[6];
+      ce[0] = g[3] - ce[2] * g[2] - ce[1] * g[1];
+    }
+    double L1 = dist3(xi, cij), L2 = dist3(ce, cij);
+    const double L3 = dist3(xi, ce);
+    double P = 0.5 * (L1 + L2 + L3);
+    Ai += sqrt(P * (P - L1) * (P - L2) * (P - L3));
+    L1 = dist3(xi, ci2);
+    L2 = dist3(ce, ci2);
+    P = 0.5 * (L1 + L2 + L3);
+    Ai += sqrt(P * (P - L1) * (P - L2) * (P - L3));
+    const double alpha = 1.0 / tan(acos(a1 / (dist3(xi, x1) * dist3(x1, xj))));
+    const double beta = 1.0 / tan(acos(a2 / (dist3(xi, x2) * dist3(x2, xj))));
+    sm += 0.5 * (alpha + beta) * (f[j] - f[i]);
+  }
+  *sum = sm;
+  *area = Ai;
+}
+
+/* Node normals, mean curvature km at every node and the bending force fb of
+ * every element vertex from the current positions. */
+static void bend_chain(orc_state *s) {
+  const int n = s->n, T = s->n_tri, nodes = n + s->n_enode;
+  const double *X = s->mk[0], *Y = s->mk[1], *Z = s->mk[2];
+  memset(s->node_n, 0, sizeof(double) * 3 * nodes);
+  memset(s->kapa, 0, sizeof(double) * 9 * nodes);
+  for (int t = 0; t < T; ++t) {
+    const int a = s->tri[3 * t], b = s->tri[3 * t + 1], c = s->tri[3 * t + 2];
+    const double e12[3] = {X[b] - X[a], Y[b] - Y[a], Z[b] - Z[a]};
+    const double e13[3] = {X[c] - X[a], Y[c] - Y[a], Z[c] - Z[a]};
+    double nt[3] = {e12[1] * e13[2] - e12[2] * e13[1], e12[2] * e13[0] - e12[0] * e13[2],
+                    e12[0] * e13[1] - e12[1] * e13[0]};
+    const double nn = sqrt(dot3(nt, nt));
+    for (int k = 0; k < 3; ++k) nt[k] /= -nn;
+    for (int k = 0; k < 6; ++k)
+      for (int d = 0; d < 3; ++d) s->node_n[3 * s->tri6[6 * t + k] + d] += nt[d];
+  }
+  for (int v = 0; v < nodes; ++v) {
+    if (s->node_cnt[v] == 0) continue;
+    double *r = s->node_n + 3 * v;
+    for (int d = 0; d < 3; ++d) r[d] /= s->node_cnt[v];
+    const double nn = sqrt(dot3(r, r));
+    for (int d = 0; d < 3; ++d) r[d] /= nn;
+  }
+  for (int t = 0; t < T; ++t) {
+    double xc[6][3], nc[6][3];
+    for (int k = 0; k < 3; ++k) {
+      const int v = s->tri[3 * t + k];
+      xc[k][0] = X[v];
+      xc[k][1] = Y[v];
+      xc[k][2] = Z[v];
+    }
+    for (int d = 0; d < 3; ++d) {
+      xc[3][d] = 0.5 * (xc[0][d] + xc[1][d]);
+      xc[4][d] = 0.5 * (xc[0][d] + xc[2][d]);
+      xc[5][d] = 0.5 * (xc[1][d] + xc[2][d]);
+    }
+    for (int k = 0; k < 6; ++k)
+      for (int d = 0; d < 3; ++d) nc[k][d] = s->node_n[3 * s->tri6[6 * t + k] + d];
+    for (int i = 0; i < 6; ++i) {
+      const double xi = QXE[i][0], et = QXE[i][1];
+      double A1[3] = {0}, A2[3] = {0}, A3[3] = {0}, A4[3] = {0}, A5[3], A6[3] = {0, 0, 0};
+      for (int k = 0; k < 6; ++k) {
+        const double dxi = 2 * QN[k][0] * xi + QN[k][2] * et + QN[k][3];
+        const double det = 2 * QN[k][1] * et + QN[k][2] * xi + QN[k][4];
+        for (int d = 0; d < 3; ++d) {
+          A1[d] += xc[k][d] * dxi;
+          A2[d] += nc[k][d] * dxi;
+          A3[d] += xc[k][d] * det;
+          A4[d] += nc[k][d] * det;
+        }
+      }
+      for (int d = 0; d < 3; ++d) A5[d] = nc[i][d];
+      double K[3][3];
+      solve_rows(K, A1, A2, A3, A4, A5, A6);
+      double *M = s->kapa + 9 * s->tri6[6 * t + i];
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) M[3 * k + l] += K[l][k];
+    }
+  }
+  for (int v = 0; v < nodes; ++v) {
+    if (s->node_cnt[v] == 0) {
+      s->km[v] = 0;
+      continue;
+    }
+    double *M = s->kapa + 9 * v, Kv[3][3];
+    for (int k = 0; k < 9; ++k) M[k] /= s->node_cnt[v];
+    for (int k = 0; k < 3; ++k)
+      for (int l = 0; l < 3; ++l) Kv[k][l] = 0.5 * (M[3 * k + l] + M[3 * l + k]);
+    s->km[v] = 0.5 * (Kv[0][0] + Kv[1][1] + Kv[2][2]);
+  }
+  for (int i = 0; i < n; ++i) {
+    s->fb[3 * i] = s->fb[3 * i + 1] = s->fb[3 * i + 2] = 0;
+    if (!s->ring_ok[i]) continue;
+    double sum, Ai;
+    ring_laplace(s, i, s->km, &sum, &Ai);
+    const double c = (double)s->p.kbend / Ai * sum;
+    for (int d = 0; d < 3; ++d) s->fb[3 * i + d] = c * s->node_n[3 * i + d];
+  }
+}
+
+int orc_bend(orc_state *s, double *km, double *fb, double *normals) {
+  if (!s->bend_ok) return 0;
+  bend_chain(s);
+  if (km) memcpy(km, s->km, sizeof(double) * s->n);
+  if (fb) memcpy(fb, s->fb, sizeof(double) * 3 * s->n);
+  if (normals) memcpy(normals, s->node_n, sizeof(double) * 3 * s->n);
+  return 1;
+}
+
+int orc_ring_laplacian(orc_state *s, const double *f, double *out, double *area) {
+  if (!s->bend_ok) return 0;
+  for (int i = 0; i < s->n; ++i) {
+    double sum = 0, Ai = 0;
+    if (s->ring_ok[i]) ring_laplace(s, i, f, &sum, &Ai);
+    out[i] = Ai > 0 ? sum / Ai : 0;
+    if (area) area[i] = Ai;
+  }
+  return 1;
 }
 
 static void stage_spread(orc_state *s) {

@@ -109,6 +109,12 @@ double *orc_rec_force(orc_state *s);
 int *orc_cells(orc_state *s, int which, int *nb3); /* 0 bin_start 1 items */
 /* counters: [0] bonds [1] detached [2] detach step [3] near-threshold [4] events */
 void orc_counters(orc_state *s, int64_t out[5]);
+/* Bending chain on the current positions: mean curvature km (n), bending
+ * force fb (3n, with the kbend of the parameters) and node normals (3n) of
+ * the vertices; 0 when the meshes are not closed (then nothing is written). */
+int orc_bend(orc_state *s, double *km, double *fb, double *normals);
+/* Cotangent ring Laplacian sum/A of per-vertex values f and the areas A. */
+int orc_ring_laplacian(orc_state *s, const double *f, double *out, double *area);
 
 #ifdef __cplusplus
 }

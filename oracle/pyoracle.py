@@ -77,6 +77,8 @@ def load():
         "orc_rec_force": (f64p, [C.c_void_p]),
         "orc_cells": (i32p, [C.c_void_p, C.c_int, i32p]),
         "orc_counters": (None, [C.c_void_p, C.POINTER(C.c_int64)]),
+        "orc_bend": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
+        "orc_ring_laplacian": (C.c_int, [C.c_void_p, f64p, f64p, f64p]),
     }
     for k, (r, a) in sig.items():
         fn = getattr(L, k)
@@ -238,6 +240,24 @@ class Oracle:
         V = self._arr(self._lib.orc_membrane(self._h, 1), self.n_plt).copy()
         tS = self._arr(self._lib.orc_membrane(self._h, 2), self.n_tri).copy()
         return S, V, tS
+
+    def bend(self):
+        """Mean curvature, bending force (n x 3) and node normals (n x 3) of
+        the vertices from the current positions; None for open meshes."""
+        km = np.zeros(self.n)
+        fb = np.zeros((self.n, 3))
+        nrm = np.zeros((self.n, 3))
+        if not self._lib.orc_bend(self._h, P(km), P(fb), P(nrm)):
+            return None
+        return km, fb, nrm
+
+    def ring_laplacian(self, f):
+        f = d64(f)
+        out = np.zeros(self.n)
+        area = np.zeros(self.n)
+        if not self._lib.orc_ring_laplacian(self._h, P(f), P(out), P(area)):
+            return None
+        return out, area
 
     def lowest(self):
         return self._arr(self._lib.orc_lowest(self._h), self.n_plt).copy()

@@ -47,7 +47,7 @@ class Params(C.Structure):
         ("fiber_law", C.c_int32), ("fiber_move", C.c_int32), ("fiber_k", C.c_float),
         ("fiber_xmax", C.c_float), ("fiber_gamma", C.c_float),
         ("fiber_ref_sign", C.c_int32), ("bin_cells", C.c_int32),
-        ("sem", C.c_float * 20),
+        ("sem", C.c_float * 20), ("kbend", C.c_float),
     ]
 
 

@@ -89,6 +89,8 @@ void free_markers(Markers &m) {
   dfree(m.plt_S0); dfree(m.plt_V0); dfree(m.plt_S); dfree(m.plt_V);
   dfree(m.plt_lowest);
   dfree(m.m2t_start); dfree(m.m2t);
+  dfree(m.tri_edge); dfree(m.edge_tri); dfree(m.ring_start); dfree(m.ring);
+  dfree(m.vn); dfree(m.en); dfree(m.km);
   dfree(m.m2e_start); dfree(m.m2e_nbr); dfree(m.m2e_l0);
   dfree(m.rec_tri); dfree(m.rec_r1); dfree(m.rec_r2);
   dfree(m.rec_bond); dfree(m.rec_cand); dfree(m.rec_release);
@@ -226,6 +228,7 @@ pfg_status pfg_params_from_sem(const float *s, float dx, float dt, int lx, int l
     p->ib_force_scale = F0 * uf * dt / (rho0 * umax) / C;
   if (s[8] > 0.f) p->marker_dt = dt / s[8] / dx;
   p->detach_rate = s[5] * dt;
+  p->kbend = F0 > 0.f ? s[19] / F0 : 0.f; /* Kbend ... / F0 (sem_kernel.cu:1630-1635) */
   return PFG_OK;
 }
 
@@ -661,6 +664,80 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   for (int t = 0; t < M.n_tri; ++t)
     for (int k = 0; k < 3; ++k) kv.push_back({M.tri[3 * t + k], 3 * t + k});
   build_csr(n, kv, m2t_start, m2t);
+  /* bending topology (gpuSEM_LB.c:846-936): midpoint node of each triangle
+   * edge (slots v0v1, v0v2, v1v2), and per vertex the ring of neighbours
+   * walked as the reference walks it: the vertex after v in its first
+   * triangle, then, in the first unused triangle of v holding the current
+   * neighbour, the vertex before it.  A ring that does not close (a vertex
+   * on the border of an open mesh) is left empty. */
+  std::vector<int32_t> tri_edge((size_t)3 * M.n_tri), edge_tri, ring_start((size_t)n + 1, 0),
+      ring;
+  {
+    static const int sl[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    std::vector<std::pair<int64_t, int32_t>> ek;
+    ek.reserve((size_t)3 * M.n_tri);
+    for (int t = 0; t < M.n_tri; ++t)
+      for (int k = 0; k < 3; ++k) {
+        int a = M.tri[3 * t + sl[k][0]], b = M.tri[3 * t + sl[k][1]];
+        if (a > b) std::swap(a, b);
+        ek.push_back({(int64_t)a * n + b, 3 * t + k});
+      }
+    std::sort(ek.begin(), ek.end());
+    int ne = 0;
+    for (size_t i = 0; i < ek.size();) {
+      size_t j = i;
+      while (j < ek.size() && ek[j].first == ek[i].first) ++j;
+      edge_tri.push_back(ek[i].second);
+      edge_tri.push_back(j - i > 1 ? ek[i + 1].second : -1);
+      for (size_t q = i; q < j; ++q) tri_edge[(size_t)ek[q].second] = ne;
+      ++ne;
+      i = j;
+    }
+    m.n_tedges = ne;
+    for (int v = 0; v < n; ++v) {
+      const int c = m2t_start[(size_t)v + 1] - m2t_start[(size_t)v];
+      std::vector<int> el((size_t)c), rg((size_t)c);
+      for (int k = 0; k < c; ++k) el[(size_t)k] = m2t[(size_t)m2t_start[(size_t)v] + k] / 3;
+      bool ok = c > 2;
+      if (ok) {
+        int cur = -1;
+        for (int j = 0; j < 3; ++j)
+          if (M.tri[3 * el[0] + j] == v) cur = M.tri[3 * el[0] + (j == 2 ? 0 : j + 1)];
+        rg[0] = cur;
+        for (int j = 1; j < c && ok; ++j) {
+          bool found = false;
+          for (int k = j; k < c && !found; ++k) {
+            const int e = el[(size_t)k];
+            for (int l = 0; l < 3; ++l)
+              if (M.tri[3 * e + l] == cur) {
+                found = true;
+                cur = M.tri[3 * e + (l == 0 ? 2 : l - 1)];
+                rg[(size_t)j] = cur;
+                std::swap(el[(size_t)j], el[(size_t)k]);
+                break;
+              }
+          }
+          ok = found;
+        }
+        if (ok) { /* closed: a triangle of v holds the last and the first */
+          bool shared = false;
+          for (int k = 0; k < c && !shared; ++k) {
+            const int e = el[(size_t)k];
+            bool hl = false, hf = false;
+            for (int l = 0; l < 3; ++l) {
+              hl |= M.tri[3 * e + l] == rg[(size_t)c - 1];
+              hf |= M.tri[3 * e + l] == rg[0];
+            }
+            shared = hl && hf;
+          }
+          ok = shared;
+        }
+      }
+      if (ok) ring.insert(ring.end(), rg.begin(), rg.end());
+      ring_start[(size_t)v + 1] = (int32_t)ring.size();
+    }
+    m.bend = p.kbend != 0.f && M.n_tri > 0;
+  }
   kv.clear();
   for (int e = 0; e < M.n_edges; ++e) {
     kv.push_back({M.edge[2 * e], 2 * e});
@@ -785,6 +862,13 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   ALLOC_FILL(m.plt_lowest, -1, M.n_platelets);
   ALLOC_UP(m.m2t_start, m2t_start.data(), m2t_start.size());
   ALLOC_UP(m.m2t, m2t.data(), m2t.size());
+  ALLOC_UP(m.tri_edge, tri_edge.data(), tri_edge.size());
+  ALLOC_UP(m.edge_tri, edge_tri.data(), edge_tri.size());
+  ALLOC_UP(m.ring_start, ring_start.data(), ring_start.size());
+  ALLOC_UP(m.ring, ring.data(), ring.size());
+  ALLOC_FILL(m.vn, 0.0, m.bend ? (size_t)3 * n : 0);
+  ALLOC_FILL(m.en, 0.0, m.bend ? (size_t)3 * m.n_tedges : 0);
+  ALLOC_FILL(m.km, 0.0, m.bend ? (size_t)n : 0);
   ALLOC_UP(m.m2e_start, m2e_start.data(), m2e_start.size());
   ALLOC_UP(m.m2e_nbr, m2e_nbr.data(), m2e_nbr.size());
   ALLOC_UP(m.m2e_l0, m2e_l0.data(), m2e_l0.size());

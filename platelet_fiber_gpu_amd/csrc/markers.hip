@@ -66,18 +66,20 @@ struct MArgs {
   float fiber_k, fiber_xmax, fiber_gamma;
   float marker_dt;
   float force_scale;
+  float kbend;
   /* role split of the fused launches */
   int roles;
   int halt; /* 1: after the detach test fires, leave bonds, force field and
                markers unchanged (the step loop ends at detach) */
   int mark_rows; /* 1: stamp the x-block rows this step's IB stencils touch */
-  int blocks0, blocks1, blocks2; /* blocks of role ranges 0, 1, 2 */
+  int blocks0, blocks1, blocks2, blocks3; /* blocks of role ranges 0..3 */
 };
 
 enum : int {
   ROLE_A = 1, /* first role of a fused kernel  */
   ROLE_B = 2, /* second role                   */
-  ROLE_C = 4  /* third role                    */
+  ROLE_C = 4, /* third role                    */
+  ROLE_D = 8  /* fourth role                   */
 };
 
 constexpr float BOND_EPS_P = 1e-6f; /* |u - p| below this: near-threshold      */
@@ -470,7 +472,237 @@ __device__ void triangle_geometry(const MArgs &a, int t) {
                               (float)(ax * by - ay * bx), 0.f);
 }
 
-/* roles: A = scatter markers into bins (blocks0), B = triangle terms (blocks1) */
+/* ---------------- membrane bending ---------------- */
+
+/* Quadratic 6-node triangle: N_k = a xi^2 + b eta^2 + c xi eta + d xi + e eta + f
+ * with rows {a, b, c, d, e, f} of the standard quadratic Lagrange triangle
+ * (nodes at the vertices, then the midpoints of v0v1, v0v2, v1v2), and the
+ * natural coordinates of the 6 nodes (sem_kernel.cu:94-107). */
+__constant__ double QN[6][6] = {{2, 2, 4, -3, -3, 1}, {2, 0, 0, -1, 0, 0},
+                                {0, 2, 0, 0, -1, 0},  {-4, 0, -4, 4, 0, 0},
+                                {0, -4, -4, 0, 4, 0}, {0, 0, 4, 0, 0, 0}};
+__constant__ double QXE[6][2] = {{0, 0}, {1, 0}, {0, 1}, {0.5, 0}, {0, 0.5}, {0.5, 0.5}};
+
+/* Forward elimination with partial pivoting and unit pivots on a 3 x 4
+ * augmented matrix (gaussian_elimination, sem_kernel.cu:249-284). */
+__device__ inline void gauss3x4(double g[12]) {
+  int i = 0;
+  for (int j = 0; j < 3 && i < 3; ++j) {
+    int maxi = i;
+    for (int k = i + 1; k < 3; ++k)
+      if (fabs(g[k * 4 + j]) > fabs(g[maxi * 4 + j])) maxi = k;
+    if (g[maxi * 4 + j] == 0.0) continue;
+    double initial = 0.0;
+    for (int k = j; k < 4; ++k) {
+      const double b = g[i * 4 + k];
+      g[i * 4 + k] = g[maxi * 4 + k];
+      g[maxi * 4 + k] = b;
+      if (k == j) initial = g[i * 4 + j];
+      g[i * 4 + k] /= initial;
+    }
+    for (int k = i + 1; k < 3; ++k) {
+      const double f = g[k * 4 + j];
+      for (int l = j; l < 4; ++l) g[k * 4 + l] -= f * g[i * 4 + l];
+    }
+    ++i;
+  }
+}
+
+__device__ inline void pos_d(const MArgs &a, int i, double x[3]) {
+  const float4 p = a.m.pos4[i];
+  x[0] = p.x;
+  x[1] = p.y;
+  x[2] = p.z;
+}
+
+/* Unit normal -(e12 x e13)/|e12 x e13| of triangle t (sem_kernel.cu:793-803). */
+__device__ inline void tri_unit_normal(const MArgs &a, int t, double n[3]) {
+  double p0[3], p1[3], p2[3];
+  pos_d(a, a.m.tri[3 * t], p0);
+  pos_d(a, a.m.tri[3 * t + 1], p1);
+  pos_d(a, a.m.tri[3 * t + 2], p2);
+  const double e12[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+  const double e13[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+  n[0] = e12[1] * e13[2] - e12[2] * e13[1];
+  n[1] = e12[2] * e13[0] - e12[0] * e13[2];
+  n[2] = e12[0] * e13[1] - e12[1] * e13[0];
+  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  n[0] /= -nn;
+  n[1] /= -nn;
+  n[2] /= -nn;
+}
+
+/* Node normal k (k < n: vertex k; else the midpoint of triangle edge k - n):
+ * mean of the unit normals of its triangles, normalised
+ * (sem_calculate_A_kernel and sem_surface_tension_kernel,
+ * sem_kernel.cu:810-820, 874-900). */
+__device__ void bend_normal(const MArgs &a, int k) {
+  double s[3] = {0.0, 0.0, 0.0}, nt[3];
+  int cnt = 0;
+  double *out;
+  if (k < a.m.n) {
+    for (int q = a.m.m2t_start[k]; q < a.m.m2t_start[k + 1]; ++q, ++cnt) {
+      tri_unit_normal(a, a.m.m2t[q] / 3, nt);
+      s[0] += nt[0];
+      s[1] += nt[1];
+      s[2] += nt[2];
+    }
+    out = a.m.vn + 3 * (int64_t)k;
+  } else {
+    const int e = k - a.m.n;
+    for (int j = 0; j < 2; ++j) {
+      const int q = a.m.edge_tri[2 * e + j];
+      if (q < 0) continue;
+      tri_unit_normal(a, q / 3, nt);
+      s[0] += nt[0];
+      s[1] += nt[1];
+      s[2] += nt[2];
+      ++cnt;
+    }
+    out = a.m.en + 3 * (int64_t)e;
+  }
+  if (cnt == 0) return;
+  for (int d = 0; d < 3; ++d) s[d] /= cnt;
+  const double nn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+  for (int d = 0; d < 3; ++d) out[d] = s[d] / nn;
+}
+
+/* Mean curvature of vertex v: at its node of every incident 6-node triangle
+ * (midpoints at the vertex averages) solve K x_xi = n_xi, K x_eta = n_eta,
+ * K n = 0 row by row (solve_linear_eqns); km = trace of the mean K / 2
+ * (sem_calculate_Kapa_kernel and sem_calculate_m_kernel,
+ * sem_kernel.cu:950-1100; the symmetrisation there keeps the trace). */
+__device__ void bend_km(const MArgs &a, int v) {
+  const int q0 = a.m.m2t_start[v], cnt = a.m.m2t_start[v + 1] - q0;
+  if (cnt == 0) return;
+  double tr = 0.0;
+  for (int q = q0; q < q0 + cnt; ++q) {
+    const int t = a.m.m2t[q] / 3, s = a.m.m2t[q] % 3;
+    double xc[6][3], nc[6][3];
+    for (int k = 0; k < 3; ++k) {
+      const int vk = a.m.tri[3 * t + k];
+      pos_d(a, vk, xc[k]);
+      for (int d = 0; d < 3; ++d) nc[k][d] = a.m.vn[3 * (int64_t)vk + d];
+      const int e = a.m.tri_edge[3 * t + k];
+      for (int d = 0; d < 3; ++d) nc[3 + k][d] = a.m.en[3 * (int64_t)e + d];
+    }
+    for (int d = 0; d < 3; ++d) {
+      xc[3][d] = 0.5 * (xc[0][d] + xc[1][d]);
+      xc[4][d] = 0.5 * (xc[0][d] + xc[2][d]);
+      xc[5][d] = 0.5 * (xc[1][d] + xc[2][d]);
+    }
+    const double xi = QXE[s][0], et = QXE[s][1];
+    double A1[3] = {0, 0, 0}, A2[3] = {0, 0, 0}, A3[3] = {0, 0, 0}, A4[3] = {0, 0, 0};
+    for (int k = 0; k < 6; ++k) {
+      const double dxi = 2.0 * QN[k][0] * xi + QN[k][2] * et + QN[k][3];
+      const double det = 2.0 * QN[k][1] * et + QN[k][2] * xi + QN[k][4];
+      for (int d = 0; d < 3; ++d) {
+        A1[d] += xc[k][d] * dxi;
+        A2[d] += nc[k][d] * dxi;
+        A3[d] += xc[k][d] * det;
+        A4[d] += nc[k][d] * det;
+      }
+    }
+    for (int r = 0; r < 3; ++r) { /* row r of K; only K[r][r] enters km */
+      double g[12];
+      for (int c = 0; c < 3; ++c) {
+        g[c] = A1[c];
+        g[4 + c] = A3[c];
+        g[8 + c] = nc[s][c];
+      }
+      g[3] = A2[r];
+      g[7] = A4[r];
+      g[11] = 0.0;
+      gauss3x4(g);
+      const double k2 = g[11];
+      const double k1 = g[7] - k2 * g[6];
+      const double k0 = g[3] - k2 * g[2] - k1 * g[1];
+      tr += r == 0 ? k0 : (r == 1 ? k1 : k2);
+    }
+  }
+  a.m.km[v] = 0.5 * tr / cnt;
+}
+
+__device__ inline double dist_d(const double a[3], const double b[3]) {
+  return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) +
+              (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+/* Ring entry q of vertex i (neighbour j, previous x1, next x2): the cotangent
+ * term 0.5 (cot a + cot b)(km_j - km_i) and the area of vertex i's region in
+ * triangle (i, j, x2) (sem_Laplace_Kapa_kernel, sem_kernel.cu:1533-1628). */
+__device__ void ring_term(const MArgs &a, int i, int r0, int c, int q, double *sum,
+                          double *area) {
+  const int j = a.m.ring[r0 + q];
+  const int v1 = a.m.ring[r0 + (q == 0 ? c - 1 : q - 1)];
+  const int v2 = a.m.ring[r0 + (q == c - 1 ? 0 : q + 1)];
+  double xi[3], xj[3], x1[3], x2[3];
+  pos_d(a, i, xi);
+  pos_d(a, j, xj);
+  pos_d(a, v1, x1);
+  pos_d(a, v2, x2);
+  double ri1[3], r1j[3], rij[3], ri2[3], r2j[3], cij[3], ci2[3], c2j[3], ce[3];
+  for (int k = 0; k < 3; ++k) {
+    ri1[k] = xi[k] - x1[k];
+    r1j[k] = x1[k] - xj[k];
+    rij[k] = xi[k] - xj[k];
+    ri2[k] = xi[k] - x2[k];
+    r2j[k] = x2[k] - xj[k];
+    cij[k] = 0.5 * (xi[k] + xj[k]);
+    ci2[k] = 0.5 * (xi[k] + x2[k]);
+    c2j[k] = 0.5 * (x2[k] + xj[k]);
+  }
+  auto dot = [](const double u[3], const double w[3]) {
+    return u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+  };
+  const double ai = dot(rij, ri2), a2 = -dot(ri2, r2j), aj = dot(rij, r2j),
+               a1 = -dot(r1j, ri1);
+  if (ai < 0) {
+    for (int k = 0; k < 3; ++k) ce[k] = c2j[k];
+  } else if (a2 < 0) {
+    for (int k = 0; k < 3; ++k) ce[k] = cij[k];
+  } else if (aj < 0) {
+    for (int k = 0; k < 3; ++k) ce[k] = ci2[k];
+  } else { /* circumcentre: two bisector planes and the triangle's plane */
+    double g[12] = {0};
+    for (int k = 0; k < 3; ++k) {
+      g[k] = rij[k];
+      g[4 + k] = ri2[k];
+      g[3] += cij[k] * rij[k];
+      g[7] += ci2[k] * ri2[k];
+    }
+    g[8] = (xj[1] - xi[1]) * (x2[2] - xi[2]) - (xj[2] - xi[2]) * (x2[1] - xi[1]);
+    g[9] = (xj[2] - xi[2]) * (x2[0] - xi[0]) - (xj[0] - xi[0]) * (x2[2] - xi[2]);
+    g[10] = (xj[0] - xi[0]) * (x2[1] - xi[1]) - (xj[1] - xi[1]) * (x2[0] - xi[0]);
+    g[11] = g[8] * xi[0] + g[9] * xi[1] + g[10] * xi[2];
+    gauss3x4(g);
+    ce[2] = g[11];
+    ce[1] = g[7] - ce[2] * g[6];
+    ce[0] = g[3] - ce[2] * g[2] - ce[1] * g[1];
+  }
+  double L1 = dist_d(xi, cij), L2 = dist_d(ce, cij);
+  const double L3 = dist_d(xi, ce);
+  double P = 0.5 * (L1 + L2 + L3);
+  double A = sqrt(P * (P - L1) * (P - L2) * (P - L3));
+  L1 = dist_d(xi, ci2);
+  L2 = dist_d(ce, ci2);
+  P = 0.5 * (L1 + L2 + L3);
+  A += sqrt(P * (P - L1) * (P - L2) * (P - L3));
+  const double alpha = 1.0 / tan(acos(a1 / (dist_d(xi, x1) * dist_d(x1, xj))));
+  const double beta = 1.0 / tan(acos(a2 / (dist_d(xi, x2) * dist_d(x2, xj))));
+  *sum = 0.5 * (alpha + beta) * (a.m.km[j] - a.m.km[i]);
+  *area = A;
+}
+
+template <int G>
+__device__ inline double group_sum_d(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+/* roles: A = scatter markers into bins (blocks0), B = triangle terms (blocks1),
+ * C = bending node normals, vertices then edge midpoints (blocks2) */
 __global__ __launch_bounds__(256) void place_kernel(MArgs a) {
   int blk = blockIdx.x;
   if (a.roles & ROLE_A) {
@@ -487,6 +719,45 @@ __global__ __launch_bounds__(256) void place_kernel(MArgs a) {
   if (a.roles & ROLE_B) {
     const int t = blk * 256 + threadIdx.x;
     if (t < a.m.n_tri) triangle_geometry(a, t);
+  }
+}
+
+/* Membrane bending, launched only when it is on (separate kernels keep the
+ * FP64 solves out of the register budget of the force pass):
+ * bend_normal_kernel after the triangle pass, bend_km_kernel after it,
+ * bend_force_kernel after the force pass (it adds to the marker forces). */
+__global__ __launch_bounds__(256) void bend_normal_kernel(MArgs a) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k < a.m.n + a.m.n_tedges) bend_normal(a, k);
+}
+
+__global__ __launch_bounds__(256) void bend_km_kernel(MArgs a) {
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v < a.m.n) bend_km(a, v);
+}
+
+/* Four lanes per vertex over its ring (sem_Laplace_Kapa_kernel,
+ * sem_kernel.cu:1499-1638); lane 0 adds kbend (sum / A) n_i. */
+__global__ __launch_bounds__(256) void bend_force_kernel(MArgs a) {
+  constexpr int G = 4;
+  const int i = (blockIdx.x * 256 + threadIdx.x) / G, lane = threadIdx.x & (G - 1);
+  if (i >= a.m.n) return;
+  const int r0 = a.m.ring_start[i], c = a.m.ring_start[i + 1] - r0;
+  if (c == 0) return; /* the whole group leaves together */
+  double sum = 0.0, area = 0.0;
+  for (int q = lane; q < c; q += G) {
+    double sq, aq;
+    ring_term(a, i, r0, c, q, &sq, &aq);
+    sum += sq;
+    area += aq;
+  }
+  sum = group_sum_d<G>(sum);
+  area = group_sum_d<G>(area);
+  if (lane == 0) {
+    const double coef = (double)a.kbend / area * sum;
+    a.m.fx[i] += (float)(coef * a.m.vn[3 * (int64_t)i]);
+    a.m.fy[i] += (float)(coef * a.m.vn[3 * (int64_t)i + 1]);
+    a.m.fz[i] += (float)(coef * a.m.vn[3 * (int64_t)i + 2]);
   }
 }
 
@@ -1459,10 +1730,11 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->fiber_gamma = p.fiber_gamma;
   a->marker_dt = p.marker_dt;
   a->force_scale = p.ib_force_scale;
+  a->kbend = p.kbend;
   a->roles = 0;
   a->halt = h->halt_on_detach ? 1 : 0;
   a->mark_rows = h->overlap && h->row_stamp != nullptr ? 1 : 0;
-  a->blocks0 = a->blocks1 = a->blocks2 = 0;
+  a->blocks0 = a->blocks1 = a->blocks2 = a->blocks3 = 0;
 }
 
 static inline int blocks_for(int64_t n, int per_block) {
@@ -1505,6 +1777,13 @@ hipError_t launch_place(pfg_handle *h, bool scatter, bool geometry) {
   const int grid = a.blocks0 + a.blocks1;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(place_kernel, dim3(grid), dim3(256), 0, h->stream, a);
+  if (geometry && h->m.bend) { /* node normals, then the mean curvature */
+    hipLaunchKernelGGL(bend_normal_kernel,
+                       dim3(blocks_for((int64_t)h->m.n + h->m.n_tedges, 256)), dim3(256), 0,
+                       h->stream, a);
+    hipLaunchKernelGGL(bend_km_kernel, dim3(blocks_for(h->m.n, 256)), dim3(256), 0,
+                       h->stream, a);
+  }
   return hipGetLastError();
 }
 
@@ -1547,6 +1826,9 @@ hipError_t launch_forces(pfg_handle *h) {
     case 16: hipLaunchKernelGGL(forces_kernel<16>, dim3(grid), dim3(256), 0, h->stream, a); break;
     default: hipLaunchKernelGGL(forces_kernel<64>, dim3(grid), dim3(256), 0, h->stream, a); break;
   }
+  if (h->m.bend)
+    hipLaunchKernelGGL(bend_force_kernel, dim3(blocks_for((int64_t)h->m.n * 4, 256)),
+                       dim3(256), 0, h->stream, a);
   return hipGetLastError();
 }
 

@@ -107,6 +107,15 @@ struct Markers {
   int32_t *plt_lowest = nullptr; /* lowest element per platelet             */
   /* CSR marker -> incident triangles (value = 3 * triangle + vertex slot) */
   int32_t *m2t_start = nullptr, *m2t = nullptr;
+  /* membrane bending: triangles as 6-node elements (3 vertices, 3 edge
+   * midpoints), ordered vertex rings, node normals and mean curvature */
+  int bend = 0;                  /* the chain runs each step                 */
+  int n_tedges = 0;              /* edges of the triangles (midpoint nodes)  */
+  int32_t *tri_edge = nullptr;   /* 3 per triangle: edges v0v1, v0v2, v1v2   */
+  int32_t *edge_tri = nullptr;   /* 2 per edge: 3 * triangle + slot, or -1   */
+  int32_t *ring_start = nullptr, *ring = nullptr; /* empty ring: open vertex */
+  double *vn = nullptr, *en = nullptr; /* unit normals of vertices, edges     */
+  double *km = nullptr;          /* mean curvature per vertex                */
   /* CSR marker -> mesh-edge partners with rest length */
   int n_edges = 0;
   int32_t *m2e_start = nullptr, *m2e_nbr = nullptr;

@@ -65,16 +65,19 @@ def test_channel_platelet_config(gpu, po):
     assert pos[0].mean() > sc.x.mean()  # the platelet moved with the flow
 
 
-def test_reference_mesh_platelet(gpu, po):
+@pytest.mark.parametrize("kbend", [0.0, 0.5])
+def test_reference_mesh_platelet(gpu, po, kbend):
     """The reference's own platelet mesh (Platelet_Mesh_958.neu, 958 nodes,
     1912 triangles) in the channel, with mesh springs and the area and
-    volume constraints on, 50 steps against the oracle."""
+    volume constraints on, without and with membrane bending, 50 steps
+    against the oracle."""
     import os
     pfg = gpu
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fixtures",
                         "Platelet_Mesh_958.neu")
     sc = pfg.Scene(pfg.SCENE_CHANNEL, mesh_path=path)
     sc.params.k_spring, sc.params.ks, sc.params.kv = 0.3, 0.5, 0.4
+    sc.params.kbend = kbend
     g, o = run_both(pfg, po, sc, 50)
     ofx = po.fields(po.lattice(sc.params), o.f())
     err_u = rel_l2(np.stack(g["u"]), np.stack(ofx[1:]))

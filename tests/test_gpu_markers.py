@@ -1,6 +1,8 @@
 """Stage-level parity of the marker kernels with the FP64 oracle on identical
 inputs: cell lists (exact), membrane sums (FP64), forces, IB spreading and
 interpolation, bond decisions (exact except counted near-threshold cases)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -75,7 +77,7 @@ def test_membrane_sums_fp64(gpu, po):
 
 @pytest.mark.parametrize("lanes", [0, 4, 1])
 @pytest.mark.parametrize("what", ["morse", "springs_area_volume", "wall", "fibrin_wlc",
-                                  "fibrin_linear_refsign"])
+                                  "fibrin_linear_refsign", "bending", "bending_mesh"])
 def test_forces(gpu, po, what, lanes):
     """Marker forces against the oracle with a wavefront per marker (0: the
     automatic choice at this size) and with 4 or 1 lanes per marker in
@@ -84,6 +86,11 @@ def test_forces(gpu, po, what, lanes):
     if what.startswith("fibrin"):
         sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=64, ny=48, nz=48, n_platelets=2, elements=150,
                        receptors=-1, ligands=-1, fibrin_nodes=600, fibrin_links=900)
+    elif what == "bending_mesh":  # the reference's platelet mesh, two copies
+        sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=96, ny=64, nz=64, n_platelets=2, receptors=-1,
+                       ligands=-1, fibrin_nodes=-1, mesh_path=os.path.join(
+                           os.path.dirname(os.path.abspath(__file__)), "fixtures",
+                           "Platelet_Mesh_958.neu"))
     else:
         sc = scene_small(pfg, receptors=-1, ligands=-1)
     p = sc.params
@@ -94,6 +101,8 @@ def test_forces(gpu, po, what, lanes):
         p.wall_rep_a, p.wall_rep_kappa, p.wall_rep_hc = 0.01, 10.0, 3.0
     if what == "fibrin_linear_refsign":
         p.fiber_law, p.fiber_ref_sign, p.fiber_k = pfg.FIBER_LINEAR, 1, 0.3
+    if what.startswith("bending"):  # the curvature chain alone (FP64 on both sides)
+        p.kbend = 2.0
     s, o = pair(pfg, po, sc, p)
     with s:
         s.set_forces_lanes(lanes)
