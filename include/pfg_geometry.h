@@ -47,6 +47,13 @@ typedef struct pfg_scene_opts {
                                       centred on its placement and scaled       */
   float mesh_scale;                /* lattice units per mesh unit; 0: the scale
                                       that makes the mean mesh edge 0.75        */
+  const char *fibrin_path;         /* NULL: generated network (when the scene
+                                      has one); else the network of this file
+                                      (pfg_geom_read_fibrin), centred in the
+                                      scene's fibrin box and scaled, replacing
+                                      fibrin_nodes / fibrin_links              */
+  float fibrin_scale;              /* lattice units per file unit; 0: fit the
+                                      file's bounding box into the fibrin box   */
 } pfg_scene_opts;
 
 typedef struct pfg_scene pfg_scene;
@@ -86,6 +93,20 @@ pfg_status pfg_geom_ellipsoid(int n, float a, float b, float c, float cx, float 
 pfg_status pfg_geom_read_neu(const char *path, int max_nodes, int max_tri, float *x,
                              float *y, float *z, int32_t *tri, int32_t *n_nodes,
                              int32_t *n_tri);
+
+/* Fibrin network text file as init_fibrin_network reads it
+ * (fiber.cpp:145-217): one line per node, "index x y z n c_1 .. c_n
+ * t_1 .. t_n" with indices from 1, neighbours c_k and fibre thicknesses t_k.
+ * Links are the pairs (i, c) with c > i of each node's list, repeats kept
+ * (the reference counts fibres so, fiber.cpp:288-306); self links are
+ * dropped (fiber.cpp:190-196).  link holds 2 ints per link (from 0),
+ * thickness one float.  With x == NULL only the counts are returned.
+ * PFG_ERR_IO: the file cannot be read; PFG_ERR_ARG: a malformed line, an
+ * index out of range, a node listed twice or missing, a neighbour list that
+ * is not symmetric, or a buffer too small. */
+pfg_status pfg_geom_read_fibrin(const char *path, int max_nodes, int max_links, float *x,
+                                float *y, float *z, int32_t *link, float *thickness,
+                                int32_t *n_nodes, int32_t *n_links);
 
 /* Triangle areas, total area and the reference-signed volume
  * V = sum (c . n_elem) S / 3 with n_elem = -(e01 x e02)/|e01 x e02|

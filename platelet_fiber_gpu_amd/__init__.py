@@ -29,7 +29,7 @@ __all__ = [
     "FIBER_IB", "FIBER_KICK", "SCENE_CHANNEL", "SCENE_SHEAR", "SCENE_THROMBUS",
     "SCENE_LARGE", "SCENE_BATCH", "STAGE_CELLS", "STAGE_MEMBRANE", "STAGE_BONDS",
     "STAGE_FORCES", "STAGE_SPREAD", "STAGE_LB", "STAGE_INTERP", "STAGE_UPDATE",
-    "STAGE_CLEAR", "read_neu",
+    "STAGE_CLEAR", "read_neu", "read_fibrin",
 ]
 
 Q = 19
@@ -71,6 +71,24 @@ def read_neu(path):
     return x, y, z, tri
 
 
+def read_fibrin(path):
+    """A fibrin network text file as the library reads it (init_fibrin_network,
+    fiber.cpp:145-217): node positions x, y, z (float32), links (int32, m x 2,
+    from 0, the pairs i < j of each node's list) and their thicknesses."""
+    L = lib()
+    nn, nl = C.c_int32(0), C.c_int32(0)
+    p = os.fsencode(path)
+    _ffi.check("pfg_geom_read_fibrin", L.pfg_geom_read_fibrin(
+        p, 0, 0, None, None, None, None, None, C.byref(nn), C.byref(nl)))
+    x, y, z = (np.empty(nn.value, np.float32) for _ in range(3))
+    link = np.empty((nl.value, 2), np.int32)
+    th = np.empty(nl.value, np.float32)
+    _ffi.check("pfg_geom_read_fibrin", L.pfg_geom_read_fibrin(
+        p, nn.value, nl.value, _ptr(x, C.c_float), _ptr(y, C.c_float), _ptr(z, C.c_float),
+        _ptr(link, C.c_int32), _ptr(th, C.c_float), C.byref(nn), C.byref(nl)))
+    return x, y, z, link, th
+
+
 def device_count():
     n = C.c_int(0)
     lib().pfg_device_count(C.byref(n))
@@ -88,10 +106,13 @@ class Scene:
 
     def __init__(self, config, seed=0, nx=0, ny=0, nz=0, n_platelets=0, elements=0,
                  receptors=0, ligands=0, fibrin_nodes=0, fibrin_links=0, tube_radius=0.0,
-                 store=-1, mesh_path=None, mesh_scale=0.0):
+                 store=-1, mesh_path=None, mesh_scale=0.0, fibrin_path=None,
+                 fibrin_scale=0.0):
         """mesh_path: a Gambit .neu surface mesh used for every platelet
         (scaled by mesh_scale lattice units per mesh unit, 0: mean edge 0.75)
-        instead of the generated ellipsoids."""
+        instead of the generated ellipsoids.  fibrin_path: a fibrin network
+        text file (see read_fibrin) placed in the scene's fibrin box (scaled by
+        fibrin_scale, 0: fitted) instead of a generated network."""
         L = lib()
         o = SceneOpts()
         L.pfg_scene_opts_init(C.byref(o), config)
@@ -101,6 +122,8 @@ class Scene:
         o.tube_radius, o.store = tube_radius, store
         o.mesh_path = os.fsencode(mesh_path) if mesh_path else None
         o.mesh_scale = mesh_scale
+        o.fibrin_path = os.fsencode(fibrin_path) if fibrin_path else None
+        o.fibrin_scale = fibrin_scale
         self.params = Params()
         h = C.c_void_p()
         _ffi.check("pfg_scene_build", L.pfg_scene_build(C.byref(o), C.byref(self.params),

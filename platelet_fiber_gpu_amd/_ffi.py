@@ -78,7 +78,8 @@ class SceneOpts(C.Structure):
                 ("receptors", C.c_int32), ("ligands", C.c_int32),
                 ("fibrin_nodes", C.c_int32), ("fibrin_links", C.c_int32),
                 ("tube_radius", C.c_float), ("store", C.c_int32), ("seed", C.c_uint64),
-                ("mesh_path", C.c_char_p), ("mesh_scale", C.c_float)]
+                ("mesh_path", C.c_char_p), ("mesh_scale", C.c_float),
+                ("fibrin_path", C.c_char_p), ("fibrin_scale", C.c_float)]
 
 
 # name -> (restype, argtypes); every entry point declared in include/*.h
@@ -148,6 +149,8 @@ SIGNATURES = {
     "pfg_geom_area_volume": (C.c_int, [C.c_int, i32p, f32p, f32p, f32p, f64p, f64p, f64p]),
     "pfg_geom_read_neu": (C.c_int, [C.c_char_p, C.c_int, C.c_int, f32p, f32p, f32p, i32p,
                                     i32p, i32p]),
+    "pfg_geom_read_fibrin": (C.c_int, [C.c_char_p, C.c_int, C.c_int, f32p, f32p, f32p, i32p,
+                                       f32p, i32p, i32p]),
     "pfg_geom_fibrin": (C.c_int, [C.c_int, f32p, f32p, C.c_float, C.c_int, C.c_uint64,
                                   f32p, f32p, f32p, i32p, f32p, i32p]),
 }

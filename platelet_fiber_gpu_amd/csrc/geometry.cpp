@@ -14,7 +14,11 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <new>
+#include <sstream>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -235,6 +239,90 @@ pfg_status pfg_geom_read_neu(const char *path, int max_nodes, int max_tri, float
   }
   std::fclose(in);
   return st;
+}
+
+/* init_fibrin_network's reader (fiber.cpp:145-217), checked: see the header. */
+pfg_status pfg_geom_read_fibrin(const char *path, int max_nodes, int max_links, float *x,
+                                float *y, float *z, int32_t *link, float *thickness,
+                                int32_t *n_nodes, int32_t *n_links) {
+  if (!path || !n_nodes || !n_links) return PFG_ERR_ARG;
+  std::ifstream in(path);
+  if (!in.is_open()) return PFG_ERR_IO;
+  struct Node {
+    double p[3];
+    std::vector<int> nbr;
+    std::vector<double> th;
+  };
+  std::vector<Node> nodes;
+  std::vector<int> index;
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream iss(line);
+    std::vector<double> dat;
+    double v;
+    while (iss >> v) dat.push_back(v);
+    if (dat.empty()) continue;
+    if (dat.size() < 5) return PFG_ERR_ARG;
+    const double nc = dat[4];
+    if (nc < 0 || nc != std::floor(nc) || dat.size() != 5 + 2 * (size_t)nc) return PFG_ERR_ARG;
+    Node nd;
+    for (int k = 0; k < 3; ++k) {
+      if (!std::isfinite(dat[(size_t)1 + k])) return PFG_ERR_ARG;
+      nd.p[k] = dat[(size_t)1 + k];
+    }
+    for (size_t k = 0; k < (size_t)nc; ++k) {
+      const double c = dat[5 + k];
+      if (c != std::floor(c)) return PFG_ERR_ARG;
+      nd.nbr.push_back((int)c);
+      nd.th.push_back(dat[5 + (size_t)nc + k]);
+    }
+    if (dat[0] != std::floor(dat[0])) return PFG_ERR_ARG;
+    index.push_back((int)dat[0]);
+    nodes.push_back(std::move(nd));
+  }
+  const int nn = (int)nodes.size();
+  if (nn < 1) return PFG_ERR_ARG;
+  std::vector<int> slot((size_t)nn, -1); /* file index - 1 -> line */
+  for (int l = 0; l < nn; ++l) {
+    const int id = index[(size_t)l];
+    if (id < 1 || id > nn || slot[(size_t)id - 1] >= 0) return PFG_ERR_ARG;
+    slot[(size_t)id - 1] = l;
+  }
+  /* symmetric neighbour lists (as multisets, self links aside) */
+  std::map<std::pair<int, int>, int> count;
+  for (int i = 0; i < nn; ++i)
+    for (int c : nodes[(size_t)slot[(size_t)i]].nbr) {
+      if (c < 1 || c > nn) return PFG_ERR_ARG;
+      if (c - 1 != i) count[{i, c - 1}]++;
+    }
+  for (const auto &kv : count) {
+    auto it = count.find({kv.first.second, kv.first.first});
+    if (it == count.end() || it->second != kv.second) return PFG_ERR_ARG;
+  }
+  int nl = 0;
+  for (int i = 0; i < nn; ++i)
+    for (int c : nodes[(size_t)slot[(size_t)i]].nbr)
+      if (c - 1 > i) ++nl;
+  *n_nodes = nn;
+  *n_links = nl;
+  if (!x) return PFG_OK;
+  if (!y || !z || (nl > 0 && (!link || !thickness)) || nn > max_nodes || nl > max_links)
+    return PFG_ERR_ARG;
+  int l = 0;
+  for (int i = 0; i < nn; ++i) {
+    const Node &nd = nodes[(size_t)slot[(size_t)i]];
+    x[i] = (float)nd.p[0];
+    y[i] = (float)nd.p[1];
+    z[i] = (float)nd.p[2];
+    for (size_t k = 0; k < nd.nbr.size(); ++k)
+      if (nd.nbr[k] - 1 > i) {
+        link[2 * l] = i;
+        link[2 * l + 1] = nd.nbr[k] - 1;
+        thickness[l] = (float)nd.th[k];
+        ++l;
+      }
+  }
+  return PFG_OK;
 }
 
 pfg_status pfg_geom_area_volume(int n_tri, const int32_t *tri, const float *x,
@@ -714,13 +802,18 @@ pfg_status pfg_scene_build(const pfg_scene_opts *o, pfg_params *params, pfg_scen
       S->lig_z.push_back(p.wall_z);
     }
   }
-  /* fibrin network in a box inscribed in the flow domain */
-  if (fnodes > 0 && st == PFG_OK) {
+  /* fibrin network in a box inscribed in the flow domain: generated, or read
+   * from a file, centred in the box and scaled */
+  const bool ffile = o->fibrin_path != nullptr;
+  if ((fnodes > 0 || ffile) && st == PFG_OK) {
     float lo[3], hi[3];
     const float half_yz = tube_r > 0 ? tube_r / std::sqrt(2.0f) - 5.0f : 0.4f * p.ny;
-    const double pairs_per_volume = 0.5 * (double)fnodes * fnodes *
-                                    (4.0 / 3.0 * M_PI * 216.0);
-    double len = pairs_per_volume / std::max(1, flinks) / (4.0 * half_yz * half_yz);
+    double len = 0.8 * p.nx;
+    if (!ffile) {
+      const double pairs_per_volume = 0.5 * (double)fnodes * fnodes *
+                                      (4.0 / 3.0 * M_PI * 216.0);
+      len = pairs_per_volume / std::max(1, flinks) / (4.0 * half_yz * half_yz);
+    }
     len = std::min(len, 0.9 * p.nx);
     lo[0] = (float)(0.5 * p.nx - 0.5 * len);
     hi[0] = (float)(0.5 * p.nx + 0.5 * len);
@@ -730,12 +823,65 @@ pfg_status pfg_scene_build(const pfg_scene_opts *o, pfg_params *params, pfg_scen
     const float half_z = tube_r > 0 ? half_yz : std::min(half_yz, 0.4f * p.nz);
     lo[2] = zc - half_z;
     hi[2] = zc + half_z;
-    std::vector<float> fx((size_t)fnodes), fy((size_t)fnodes), fz((size_t)fnodes);
-    std::vector<int32_t> lk((size_t)2 * flinks);
-    std::vector<float> l0((size_t)flinks);
+    std::vector<float> fx, fy, fz, l0;
+    std::vector<int32_t> lk;
     int32_t nl = 0;
-    st = pfg_geom_fibrin(fnodes, lo, hi, 6.0f, flinks, seed, fx.data(), fy.data(),
-                         fz.data(), lk.data(), l0.data(), &nl);
+    if (ffile) {
+      int32_t nn = 0;
+      st = pfg_geom_read_fibrin(o->fibrin_path, 0, 0, nullptr, nullptr, nullptr, nullptr,
+                                nullptr, &nn, &nl);
+      if (st == PFG_OK) {
+        fnodes = nn;
+        fx.resize((size_t)nn);
+        fy.resize((size_t)nn);
+        fz.resize((size_t)nn);
+        lk.resize((size_t)2 * nl);
+        l0.resize((size_t)nl);
+        std::vector<float> thick((size_t)nl);
+        st = pfg_geom_read_fibrin(o->fibrin_path, nn, nl, fx.data(), fy.data(), fz.data(),
+                                  lk.data(), thick.data(), &nn, &nl);
+      }
+      if (st == PFG_OK) {
+        float flo[3] = {fx[0], fy[0], fz[0]}, fhi[3] = {fx[0], fy[0], fz[0]};
+        for (int i = 0; i < fnodes; ++i) {
+          const float q[3] = {fx[(size_t)i], fy[(size_t)i], fz[(size_t)i]};
+          for (int k = 0; k < 3; ++k) {
+            flo[k] = std::min(flo[k], q[k]);
+            fhi[k] = std::max(fhi[k], q[k]);
+          }
+        }
+        double s = o->fibrin_scale;
+        if (!(s > 0)) { /* fit into the box */
+          s = 1e30;
+          for (int k = 0; k < 3; ++k)
+            if (fhi[k] > flo[k]) s = std::min(s, (double)(hi[k] - lo[k]) / (fhi[k] - flo[k]));
+          if (!(s < 1e30)) s = 1.0;
+        }
+        for (int i = 0; i < fnodes; ++i) {
+          float *q[3] = {&fx[(size_t)i], &fy[(size_t)i], &fz[(size_t)i]};
+          for (int k = 0; k < 3; ++k)
+            *q[k] = (float)(0.5 * (lo[k] + hi[k]) + s * (*q[k] - 0.5 * (flo[k] + fhi[k])));
+        }
+        for (int k = 0; k < 3; ++k) { /* the box the network now fills */
+          lo[k] = (float)(0.5 * (lo[k] + hi[k]) - 0.5 * s * (fhi[k] - flo[k]));
+        }
+        for (int l = 0; l < nl; ++l) { /* rest lengths as placed */
+          const int a = lk[(size_t)2 * l], b = lk[(size_t)2 * l + 1];
+          const double dx = (double)fx[(size_t)b] - fx[(size_t)a];
+          const double dy = (double)fy[(size_t)b] - fy[(size_t)a];
+          const double dz = (double)fz[(size_t)b] - fz[(size_t)a];
+          l0[(size_t)l] = (float)std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+      }
+    } else {
+      fx.resize((size_t)fnodes);
+      fy.resize((size_t)fnodes);
+      fz.resize((size_t)fnodes);
+      lk.resize((size_t)2 * flinks);
+      l0.resize((size_t)flinks);
+      st = pfg_geom_fibrin(fnodes, lo, hi, 6.0f, flinks, seed, fx.data(), fy.data(),
+                           fz.data(), lk.data(), l0.data(), &nl);
+    }
     const int base = (int)S->x.size();
     for (int i = 0; i < fnodes && st == PFG_OK; ++i) {
       S->x.push_back(fx[(size_t)i]);

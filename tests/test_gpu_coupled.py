@@ -128,6 +128,24 @@ def test_thrombus_small_flags_fibrin(gpu, po, staging):
     assert np.abs(pos - opos).max() < 1e-4
 
 
+def test_thrombus_fibrin_from_file(gpu, po):
+    """The reference's fibrin network (HemoThickInputCor.txt, 1175 nodes, 1422
+    links) fitted into the tube's fibrin box with two platelets, 15 steps."""
+    import os
+    pfg = gpu
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fixtures",
+                        "HemoThickInputCor.txt")
+    sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=96, ny=64, nz=64, n_platelets=2, elements=150,
+                   receptors=16, ligands=400, fibrin_path=path)
+    assert sc.link.size // 2 == 1422
+    g, o = run_both(pfg, po, sc, 15)
+    pos = np.stack([g["mk"][k] for k in ("x", "y", "z")])
+    opos = np.stack([o.marker(k) for k in ("x", "y", "z")])
+    err = np.abs(pos - opos).max()
+    print(f"fibrin file: max position error {err:.2e}")
+    assert err < 1e-4
+
+
 @pytest.mark.parametrize("graph", [True, False])
 def test_halt_on_detach_freezes_the_run(gpu, po, graph):
     """With no ligands no bond can form, so the detach test runs every step

@@ -129,3 +129,83 @@ def test_mesh_scene(pfg):
     # an explicit scale
     sc2 = pfg.Scene(pfg.SCENE_CHANNEL, mesh_path=path, mesh_scale=6.0)
     assert np.allclose(sc2.plt_S0, S * 36.0, rtol=1e-5)
+
+
+# ---- fibrin network files (init_fibrin_network, fiber.cpp:145-217) ----
+# HemoThickInputCor.txt is the reference's own network input (its Input/
+# directory), copied as data.
+
+def parse_fibrin_np(path):
+    """Independent restatement of the line format for the checks below."""
+    rows = {}
+    for line in open(path):
+        v = line.split()
+        if not v:
+            continue
+        i, n = int(v[0]), int(float(v[4]))
+        rows[i] = ([float(q) for q in v[1:4]], [int(float(q)) for q in v[5:5 + n]],
+                   [float(q) for q in v[5 + n:5 + 2 * n]])
+    return rows
+
+
+def test_fibrin_file(pfg):
+    path = os.path.join(FIX, "HemoThickInputCor.txt")
+    x, y, z, link, th = pfg.read_fibrin(path)
+    rows = parse_fibrin_np(path)
+    assert x.size == len(rows) == 1175
+    for i, (p, nb, t) in rows.items():
+        assert (x[i - 1], y[i - 1], z[i - 1]) == tuple(np.float32(p))
+    want = [(i - 1, c - 1, rows[i][2][k]) for i in sorted(rows)
+            for k, c in enumerate(rows[i][1]) if c > i]
+    assert len(want) == len(link) == 1422
+    assert link.tolist() == [[a, b] for a, b, _ in want]
+    assert np.allclose(th, [t for _, _, t in want], rtol=1e-6)
+    assert np.all(link[:, 0] < link[:, 1])
+
+
+@pytest.mark.parametrize("text", [
+    "1 0 0 0 1 2 5.0\n2 1 0 0 0\n",                # 1 lists 2, 2 does not list 1
+    "1 0 0 0 1 2 5.0\n2 1 0 0 1 1\n",              # thickness missing
+    "1 0 0 0 1 3 5.0\n2 1 0 0 1 1 5.0\n",          # neighbour past the node count
+    "1 0 0 0 1 2 5.0\n1 1 0 0 1 1 5.0\n",          # node 1 twice, node 2 missing
+    "1 0 0 0\n",                                   # no connection count
+])
+def test_fibrin_file_rejects_malformed(pfg, tmp_path, text):
+    p = tmp_path / "bad.txt"
+    p.write_text(text)
+    with pytest.raises(pfg.PfgError) as e:
+        pfg.read_fibrin(str(p))
+    assert e.value.code == 1
+
+
+def test_fibrin_file_self_links_and_repeats(pfg, tmp_path):
+    """Self links are dropped; a repeated neighbour is a repeated link."""
+    p = tmp_path / "net.txt"
+    p.write_text("1 0 0 0 3 1 2 2 9 4 4\n2 3 0 0 2 1 1 4 4\n")
+    x, y, z, link, th = pfg.read_fibrin(str(p))
+    assert link.tolist() == [[0, 1], [0, 1]]
+    assert th.tolist() == [4.0, 4.0]
+
+
+def test_fibrin_file_scene(pfg):
+    path = os.path.join(FIX, "HemoThickInputCor.txt")
+    x, y, z, link, th = pfg.read_fibrin(path)
+    sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=128, ny=96, nz=96, n_platelets=2, elements=256,
+                   receptors=8, ligands=64, fibrin_path=path)
+    nf = x.size
+    fx, fy, fz = sc.x[-nf:], sc.y[-nf:], sc.z[-nf:]
+    assert sc.link.size // 2 == len(link)
+    base = sc.n_markers - nf
+    assert np.array_equal(sc.link.reshape(-1, 2) - base, link)
+    P = np.stack([fx, fy, fz], 1).astype(np.float64)
+    d = np.linalg.norm(P[link[:, 1]] - P[link[:, 0]], axis=1)
+    assert np.allclose(sc.link_l0, d, rtol=1e-6)
+    # one uniform scale: distances scale alike
+    Q = np.stack([x, y, z], 1).astype(np.float64)
+    dq = np.linalg.norm(Q[link[:, 1]] - Q[link[:, 0]], axis=1)
+    ok = dq > 0
+    s = d[ok] / dq[ok]
+    assert np.allclose(s, s[0], rtol=1e-4)
+    kinds = sc.kind[-nf:]
+    assert set(np.unique(kinds)) <= {pfg.MARKER_FIBER, pfg.MARKER_FIXED}
+    assert (kinds == pfg.MARKER_FIXED).any()
