@@ -26,7 +26,10 @@
  *                   [--scene shear|channel|thrombus|large] [--lattice nx ny nz]
  *                   [--store f32|f16|bf16] [--seeds first last] [--max-steps n]
  *                   [--mesh file.neu] [--mesh-scale s] [--no-mesh]
+ *                   [--fibrin network.txt] [--fibrin-scale s]
  *                   [--rates-from-file] [--scene-counts] [--parse-only]
+ * --fibrin places the network of a fibrin text file (the format
+ * init_fibrin_network reads, fiber.cpp:145-217) in the scene's fibrin box.
  * Exit status: 0 done, 1 parameter or usage error, 2 I/O error, 11 device
  * error (the reference's exit code for an invalid device), 3 other failure.
  */
@@ -151,6 +154,8 @@ struct Options {
   std::string mesh;      /* --mesh */
   float mesh_scale = 0;  /* --mesh-scale; 0: 1 / DELTA_X */
   bool no_mesh = false;  /* --no-mesh: generated platelets even when NEU_PATH exists */
+  std::string fibrin;    /* --fibrin */
+  float fibrin_scale = 0;
 };
 
 int usage(const char *msg) {
@@ -160,6 +165,7 @@ int usage(const char *msg) {
                "       [--scene shear|channel|thrombus|large] [--lattice nx ny nz]\n"
                "       [--store f32|f16|bf16] [--seeds first last] [--max-steps n]\n"
                "       [--mesh file.neu] [--mesh-scale s] [--no-mesh]\n"
+               "       [--fibrin network.txt] [--fibrin-scale s]\n"
                "       [--rates-from-file] [--scene-counts] [--parse-only]\n");
   return 1;
 }
@@ -240,6 +246,16 @@ bool parse_args(int argc, char **argv, Options *o, std::string *err) {
       }
     } else if (a == "--no-mesh") {
       o->no_mesh = true;
+    } else if (a == "--fibrin") {
+      if (!need(1)) return false;
+      o->fibrin = argv[++i];
+    } else if (a == "--fibrin-scale") {
+      if (!need(1)) return false;
+      o->fibrin_scale = (float)std::atof(argv[++i]);
+      if (!(o->fibrin_scale > 0)) {
+        *err = "--fibrin-scale must be positive";
+        return false;
+      }
     } else if (a == "--rates-from-file") {
       o->rates_from_file = true;
     } else if (a == "--scene-counts") {
@@ -365,6 +381,10 @@ int build_scene(const Options &o, ParFile *pf, long long seed, SceneGuard *sg, p
   if (!mesh.empty()) {
     so.mesh_path = mesh.c_str();
     so.mesh_scale = o.mesh_scale > 0 ? o.mesh_scale : (pf->dx > 0 ? 1.0f / pf->dx : 1.0f);
+  }
+  if (!o.fibrin.empty()) {
+    so.fibrin_path = o.fibrin.c_str();
+    so.fibrin_scale = o.fibrin_scale;
   }
   so.nx = o.nx;
   so.ny = o.ny;
@@ -501,8 +521,8 @@ int main(int argc, char **argv) {
     pfg_model model;
     const int rc = build_scene(o, &pf, first, &sg, &p, &model);
     if (rc) return rc;
-    std::printf("{\"mesh\": \"%s\", \"triangles\": %d, ", mesh_file(o, pf).c_str(),
-                model.n_tri);
+    std::printf("{\"mesh\": \"%s\", \"triangles\": %d, \"links\": %d, ",
+                mesh_file(o, pf).c_str(), model.n_tri, model.n_links);
     std::printf("\"max_steps\": %lld, \"out_freq\": %d, \"seeds\": [%lld, %lld], "
                 "\"lattice\": [%d, %d, %d], \"platelets\": %d, \"markers\": %d, "
                 "\"receptors\": %d, \"tau\": %.9g, \"k_on\": %.9g, \"k_off0\": %.9g, "

@@ -128,3 +128,14 @@ def test_mesh_missing_file_is_an_io_error(tmp_path):
     r = run("-p", FIXTURE, "--parse-only", "--mesh", str(tmp_path / "none.neu"))
     assert r.returncode == 2
     assert "cannot read mesh" in r.stderr
+
+
+def test_fibrin_network_from_file(tmp_path):
+    """--fibrin places the reference's network file (1422 links) in the scene."""
+    net = os.path.join(ROOT, "tests", "fixtures", "HemoThickInputCor.txt")
+    r = run("-p", FIXTURE, "--parse-only", "--scene", "thrombus", "--lattice", "128", "96",
+            "96", "--fibrin", net)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert json.loads(r.stdout)["links"] == 1422
+    r = run("-p", FIXTURE, "--parse-only", "--fibrin", str(tmp_path / "none.txt"))
+    assert r.returncode == 2
