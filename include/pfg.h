@@ -149,6 +149,21 @@ typedef struct pfg_params {
                                   element gets kbend (1/A_i) (Laplacian of the
                                   mean curvature) n_i; 0: off. Needs closed,
                                   consistently wound platelet meshes.          */
+  /* ---- platelet-platelet contact ---- */
+  float contact_a, contact_kappa; /* between every two platelets, the closest pair
+                                  of elements (lowest indices on ties) repels
+                                  with a kappa e^{-kappa h} / (1 - e^{-kappa h}),
+                                  h = max(distance, contact_hc / 100), when the
+                                  distance is below contact_hc (the non-bonded
+                                  term of sem_Force_kernel, sem_kernel.cu:436-509,
+                                  on the cell lists); contact_a = 0: off       */
+  float contact_hc;            /* <= bin_cells                                   */
+  /* ---- thermal noise ---- */
+  float kick_temp;             /* fibrin nodes moved by the kick: BBK Langevin
+                                  noise xi sqrt(2 T gamma (1 - e^{-2 gamma dt})
+                                  / (2 gamma)) per velocity component and step
+                                  (sem_kernel.cu.orig:251-266, unit mass), from
+                                  the counter RNG; 0: off                       */
 } pfg_params;
 
 /* Host description of the immersed structures; all arrays caller-owned and only

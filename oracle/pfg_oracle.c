@@ -1085,6 +1085,39 @@ static void stage_forces(orc_state *s) {
     FY[b] -= c * dy;
     FZ[b] -= c * dz;
   }
+  /* platelet-platelet contact (sem_kernel.cu:436-509): for every two
+   * platelets the closest element pair, the first in (i, j) order on ties,
+   * repels with a kappa e^{-kappa h}/(1 - e^{-kappa h}), h = max(d, hc/100),
+   * below the cutoff hc */
+  if (p->contact_a != 0) {
+    const double hc = p->contact_hc, ka = p->contact_kappa, A = p->contact_a;
+    for (int q1 = 0; q1 < s->n_plt; ++q1)
+      for (int q2 = q1 + 1; q2 < s->n_plt; ++q2) {
+        double best = INFINITY;
+        int bi = -1, bj = -1;
+        for (int i = s->plt_elem_start[q1]; i < s->plt_elem_start[q1 + 1]; ++i)
+          for (int j = s->plt_elem_start[q2]; j < s->plt_elem_start[q2 + 1]; ++j) {
+            const double dx = X[i] - X[j], dy = Y[i] - Y[j], dz = Z[i] - Z[j];
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            if (d < best) {
+              best = d;
+              bi = i;
+              bj = j;
+            }
+          }
+        if (bi < 0 || !(best < hc) || !(best > 0)) continue;
+        const double h = best > 0.01 * hc ? best : 0.01 * hc;
+        const double e = exp(-ka * h);
+        const double F = A * ka * e / (1.0 - e) / best;
+        const double dx = X[bi] - X[bj], dy = Y[bi] - Y[bj], dz = Z[bi] - Z[bj];
+        FX[bi] += F * dx;
+        FY[bi] += F * dy;
+        FZ[bi] += F * dz;
+        FX[bj] -= F * dx;
+        FY[bj] -= F * dy;
+        FZ[bj] -= F * dz;
+      }
+  }
   /* membrane bending (sem_Laplace_Kapa_kernel after the area/volume forces) */
   if (p->kbend != 0 && s->bend_ok) {
     bend_chain(s);
@@ -1390,8 +1423,15 @@ static void stage_update(orc_state *s) {
       const double g = p->fiber_gamma;
       const double vdt = exp(-g * dt);
       const double fdt = g > 0 ? (1.0 - vdt) / g : dt;
+      /* thermal noise of the exact Ornstein-Uhlenbeck step, unit mass (the
+       * BBK form of sem_kernel.cu.orig:251-266), drawn for the step being
+       * completed */
+      double z[3] = {0, 0, 0};
+      const double sig = p->kick_temp > 0 ? sqrt((double)p->kick_temp * (1.0 - exp(-2.0 * g * dt)))
+                                          : 0.0;
+      if (sig > 0) pfg_normal3(p->seed, s->step - 1, (uint32_t)i, PFG_RNG_KICK, z);
       for (int a = 0; a < 3; ++a) {
-        s->mk[3 + a][i] = vdt * s->mk[3 + a][i] + s->mk[6 + a][i] * fdt;
+        s->mk[3 + a][i] = vdt * s->mk[3 + a][i] + s->mk[6 + a][i] * fdt + sig * z[a];
         s->mk[a][i] += s->mk[3 + a][i] * dt;
       }
       continue;

@@ -48,6 +48,8 @@ class Params(C.Structure):
         ("fiber_xmax", C.c_float), ("fiber_gamma", C.c_float),
         ("fiber_ref_sign", C.c_int32), ("bin_cells", C.c_int32),
         ("sem", C.c_float * 20), ("kbend", C.c_float),
+        ("contact_a", C.c_float), ("contact_kappa", C.c_float), ("contact_hc", C.c_float),
+        ("kick_temp", C.c_float),
     ]
 
 

@@ -91,6 +91,7 @@ void free_markers(Markers &m) {
   dfree(m.m2t_start); dfree(m.m2t);
   dfree(m.tri_edge); dfree(m.edge_tri); dfree(m.ring_start); dfree(m.ring);
   dfree(m.vn); dfree(m.en); dfree(m.km);
+  dfree(m.contact_slot); dfree(m.contact_list);
   dfree(m.m2e_start); dfree(m.m2e_nbr); dfree(m.m2e_l0);
   dfree(m.rec_tri); dfree(m.rec_r1); dfree(m.rec_r2);
   dfree(m.rec_bond); dfree(m.rec_cand); dfree(m.rec_release);
@@ -266,6 +267,11 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
   if (p.store < PFG_STORE_F32 || p.store > PFG_STORE_BF16) return PFG_ERR_ARG;
   if (p.bin_cells <= 0) return PFG_ERR_ARG;
   if (p.f16_scale_log2 < 0 || p.f16_scale_log2 > 20) return PFG_ERR_ARG;
+  /* the contact search covers the neighbouring bins only */
+  if (p.contact_a != 0.f &&
+      !(p.contact_hc > 0.f && p.contact_hc <= (float)p.bin_cells && p.contact_kappa > 0.f))
+    return PFG_ERR_ARG;
+  if (p.kick_temp < 0.f) return PFG_ERR_ARG;
   int64_t ncells = 0;
   for (int k = 0; k < K; ++k) {
     const pfg_status st = check_lattice_params(ps[k]);
@@ -869,6 +875,12 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   ALLOC_FILL(m.vn, 0.0, m.bend ? (size_t)3 * n : 0);
   ALLOC_FILL(m.en, 0.0, m.bend ? (size_t)3 * m.n_tedges : 0);
   ALLOC_FILL(m.km, 0.0, m.bend ? (size_t)n : 0);
+  {
+    const bool contact = p.contact_a != 0.f && M.n_platelets >= 2;
+    const size_t np = (size_t)M.n_platelets;
+    ALLOC_FILL(m.contact_slot, ~(uint64_t)0, contact ? np * np : 0);
+    ALLOC_FILL(m.contact_list, 0, contact ? np * (np - 1) / 2 : 0);
+  }
   ALLOC_UP(m.m2e_start, m2e_start.data(), m2e_start.size());
   ALLOC_UP(m.m2e_nbr, m2e_nbr.data(), m2e_nbr.size());
   ALLOC_UP(m.m2e_l0, m2e_l0.data(), m2e_l0.size());

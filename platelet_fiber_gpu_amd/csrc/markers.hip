@@ -67,6 +67,8 @@ struct MArgs {
   float marker_dt;
   float force_scale;
   float kbend;
+  float contact_a, contact_kappa, contact_hc;
+  float kick_temp;
   /* role split of the fused launches */
   int roles;
   int halt; /* 1: after the detach test fires, leave bonds, force field and
@@ -320,7 +322,12 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
   }
   if (!(a.roles & ROLE_B)) return;
   const int i = blk * 256 + threadIdx.x;
-  if (i == 0) a.m.counters[0] = 0; /* bonds held, counted again this step */
+  if (i == 0) {
+    a.m.counters[0] = 0;          /* bonds held, counted again this step      */
+    a.m.counters[6] = *a.m.d_step; /* this step's index, for the later launches
+                                     (the last one advances d_step)            */
+    a.m.counters[7] = 0;          /* platelet pairs in contact, found again    */
+  }
   if (i >= a.m.n) return;
   const MLat L = lat_of(a, i);
   const float4 p = make_float4(a.m.x[i], a.m.y[i], a.m.z[i], morse_tag(a, i));
@@ -1347,6 +1354,123 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
   }
 }
 
+/* ---------------- platelet-platelet contact ---------------- */
+
+/* The non-bonded term of sem_Force_kernel (sem_kernel.cu:436-509): for every
+ * two platelets the closest pair of elements (lowest element of the lower
+ * platelet, then lowest partner, on ties) repels with
+ * a kappa e^{-kappa h} / (1 - e^{-kappa h}), h = max(distance, hc / 100),
+ * here only below the cutoff hc and found on the cell lists.  First launch:
+ * each element, for each other platelet with an element within hc (the
+ * first four such platelets), proposes the distance to its nearest element
+ * there; the key (distance bits, element of the lower platelet) goes to the
+ * pair's slot by atomicMin (distances >= 0 order as their bits), and the
+ * first proposal of a pair appends the pair to the list. */
+constexpr uint64_t CONTACT_EMPTY = ~(uint64_t)0;
+
+__global__ __launch_bounds__(256) void contact_pairs_kernel(MArgs a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.m.n || a.m.kind[i] != PFG_MARKER_ELEMENT) return;
+  const int p = a.m.body[i];
+  const float4 pi = a.m.pos4[i];
+  const MLat L = lat_of(a, i);
+  const int b = a.m.marker_bin[i] - L.bin0;
+  const int b3[3] = {b % L.nb[0], (b / L.nb[0]) % L.nb[1], b / (L.nb[0] * L.nb[1])};
+  const float hc2 = a.contact_hc * a.contact_hc;
+  int qb[4] = {-1, -1, -1, -1}, qj[4] = {0, 0, 0, 0};
+  float qd[4] = {0.f, 0.f, 0.f, 0.f};
+  for_nbr_ranges(L, a.m.bin_start, b3, [&](int s, int e) {
+    for (int c = s; c < e; ++c) {
+      const float4 q = a.m.sorted_pos[c];
+      const int tag = __float_as_int(q.w);
+      if (tag < 0 || tag == p) continue;
+      const float dx = q.x - pi.x, dy = q.y - pi.y, dz = q.z - pi.z;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 >= hc2) continue;
+      const int j = a.m.bin_items[c];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (qb[k] == tag) {
+          if (d2 < qd[k] || (d2 == qd[k] && j < qj[k])) {
+            qd[k] = d2;
+            qj[k] = j;
+          }
+          break;
+        }
+        if (qb[k] < 0) {
+          qb[k] = tag;
+          qd[k] = d2;
+          qj[k] = j;
+          break;
+        }
+      }
+    }
+  });
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (qb[k] < 0) continue;
+    const int q = qb[k], lo = min(p, q), hi = max(p, q);
+    const uint32_t elem_lo = (uint32_t)(p < q ? i : qj[k]);
+    const uint64_t key = ((uint64_t)__float_as_uint(sqrtf(qd[k])) << 32) | elem_lo;
+    const int64_t pair = (int64_t)lo * a.m.n_plt + hi;
+    const uint64_t old = atomicMin((unsigned long long *)&a.m.contact_slot[pair],
+                                   (unsigned long long)key);
+    if (old == CONTACT_EMPTY) {
+      const int idx = (int)atomicAdd((unsigned long long *)&a.m.counters[7], 1ull);
+      a.m.contact_list[idx] = (int32_t)pair;
+    }
+  }
+}
+
+/* Second launch, after the force pass: each listed pair takes its winning
+ * element e of the lower platelet and e's nearest element j of the higher
+ * one (lowest index on ties), adds the repulsion to both and empties the
+ * slot.  A fixed grid strides over the list. */
+__global__ __launch_bounds__(256) void contact_force_kernel(MArgs a) {
+  const int64_t cnt = a.m.counters[7];
+  for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < cnt;
+       k += (int64_t)gridDim.x * 256) {
+    const int pair = a.m.contact_list[k];
+    const int hi = pair % a.m.n_plt;
+    const uint64_t key = a.m.contact_slot[pair];
+    a.m.contact_slot[pair] = CONTACT_EMPTY;
+    const int e = (int)(uint32_t)(key & 0xffffffffu);
+    const float4 pe = a.m.pos4[e];
+    const MLat L = lat_of(a, e);
+    const int b = a.m.marker_bin[e] - L.bin0;
+    const int b3[3] = {b % L.nb[0], (b / L.nb[0]) % L.nb[1], b / (L.nb[0] * L.nb[1])};
+    float best = INFINITY;
+    int j = -1;
+    for_nbr_ranges(L, a.m.bin_start, b3, [&](int s, int t) {
+      for (int c = s; c < t; ++c) {
+        const float4 q = a.m.sorted_pos[c];
+        if (__float_as_int(q.w) != hi) continue;
+        const float dx = q.x - pe.x, dy = q.y - pe.y, dz = q.z - pe.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        const int jj = a.m.bin_items[c];
+        if (d2 < best || (d2 == best && jj < j)) {
+          best = d2;
+          j = jj;
+        }
+      }
+    });
+    if (j < 0) continue;
+    const float4 pj = a.m.pos4[j];
+    const float dx = pe.x - pj.x, dy = pe.y - pj.y, dz = pe.z - pj.z;
+    const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (!(d > 0.f)) continue;
+    const float h = fmaxf(d, 0.01f * a.contact_hc);
+    const float ex = expf(-a.contact_kappa * h);
+    const float F = a.contact_a * a.contact_kappa * ex / (1.0f - ex) / d;
+    atomicAdd(&a.m.fx[e], F * dx);
+    atomicAdd(&a.m.fy[e], F * dy);
+    atomicAdd(&a.m.fz[e], F * dz);
+    atomicAdd(&a.m.fx[j], -F * dx);
+    atomicAdd(&a.m.fy[j], -F * dy);
+    atomicAdd(&a.m.fz[j], -F * dz);
+  }
+}
+
 /* ---------------- K9/K10: spreading, interpolation, update ---------------- */
 
 /* Spreading has two forms.  Direct: one wavefront per marker, a lane per
@@ -1635,6 +1759,18 @@ __device__ void update_marker(const MArgs &a, int i, float vx, float vy, float v
     vx = vdt * a.m.vx[i] + a.m.fx[i] * fdt;
     vy = vdt * a.m.vy[i] + a.m.fy[i] * fdt;
     vz = vdt * a.m.vz[i] + a.m.fz[i] * fdt;
+    if (a.kick_temp > 0.f) {
+      /* thermal noise of the exact Ornstein-Uhlenbeck step, unit mass (the
+       * BBK form of sem_kernel.cu.orig:251-266), drawn for this step (the
+       * step index the first launch of the step recorded) */
+      const double g = a.fiber_gamma, sig = sqrt((double)a.kick_temp *
+                                                 (1.0 - exp(-2.0 * g * (double)dt)));
+      double z[3];
+      pfg_normal3(a.seed, a.m.counters[6], (uint32_t)i, PFG_RNG_KICK, z);
+      vx += (float)(sig * z[0]);
+      vy += (float)(sig * z[1]);
+      vz += (float)(sig * z[2]);
+    }
     a.m.vx[i] = vx;
     a.m.vy[i] = vy;
     a.m.vz[i] = vz;
@@ -1731,6 +1867,10 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->marker_dt = p.marker_dt;
   a->force_scale = p.ib_force_scale;
   a->kbend = p.kbend;
+  a->contact_a = p.contact_a;
+  a->contact_kappa = p.contact_kappa;
+  a->contact_hc = p.contact_hc;
+  a->kick_temp = p.kick_temp;
   a->roles = 0;
   a->halt = h->halt_on_detach ? 1 : 0;
   a->mark_rows = h->overlap && h->row_stamp != nullptr ? 1 : 0;
@@ -1829,6 +1969,11 @@ hipError_t launch_forces(pfg_handle *h) {
   if (h->m.bend)
     hipLaunchKernelGGL(bend_force_kernel, dim3(blocks_for((int64_t)h->m.n * 4, 256)),
                        dim3(256), 0, h->stream, a);
+  if (h->m.contact_slot && h->p.contact_a != 0.f && h->m.n_plt >= 2) {
+    hipLaunchKernelGGL(contact_pairs_kernel, dim3(blocks_for(h->m.n, 256)), dim3(256), 0,
+                       h->stream, a);
+    hipLaunchKernelGGL(contact_force_kernel, dim3(64), dim3(256), 0, h->stream, a);
+  }
   return hipGetLastError();
 }
 

@@ -160,8 +160,14 @@ struct Markers {
    * markers outside platelets) */
   int n_chunks = 0;
   int32_t *chunk_start = nullptr;
+  /* platelet-platelet contact: per platelet pair (lo * n_plt + hi) the best
+   * proposal (distance bits << 32 | element of the lower platelet), and the
+   * pairs proposed this step */
+  uint64_t *contact_slot = nullptr;
+  int32_t *contact_list = nullptr;
   /* counters: [0] total bonds, [1] detached, [2] near-threshold lo, [3] hi,
-   * [4] events lo, [5] events hi */
+   * [4] events lo, [5] events hi, [6] the step index recorded by the first
+   * launch of the step, [7] platelet pairs in contact this step */
   int64_t *counters = nullptr;
   int64_t *detach_step = nullptr;
   int64_t *d_step = nullptr;

@@ -15,6 +15,7 @@
 #ifndef PFG_RNG_H
 #define PFG_RNG_H
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -69,7 +70,7 @@ PFG_HD float pfg_u01_from_u32(uint32_t r) {
 }
 
 /* Streams drawn per (step, item). */
-enum { PFG_RNG_BOND_OFF = 0, PFG_RNG_BOND_ON = 1, PFG_RNG_DETACH = 2 };
+enum { PFG_RNG_BOND_OFF = 0, PFG_RNG_BOND_ON = 1, PFG_RNG_DETACH = 2, PFG_RNG_KICK = 3 };
 
 /* One uniform for (seed, step, item, stream). */
 PFG_HD float pfg_uniform(uint64_t seed, int64_t step, uint32_t item,
@@ -79,6 +80,25 @@ PFG_HD float pfg_uniform(uint64_t seed, int64_t step, uint32_t item,
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   pfg_u32x4 r = pfg_philox4x32_10(ctr, key);
   return pfg_u01_from_u32(r.v[0]);
+}
+
+/* Three standard normals for (seed, step, item, stream): the Box-Muller
+ * transform (Box and Muller, Ann. Math. Stat. 29, 1958) of the four 24-bit
+ * uniforms of one Philox block, u in (0, 1] for the logarithm; FP64 on both
+ * the device and the host. */
+PFG_HD void pfg_normal3(uint64_t seed, int64_t step, uint32_t item, uint32_t stream,
+                        double out[3]) {
+  uint32_t ctr[4] = {(uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32),
+                     item, stream};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  pfg_u32x4 r = pfg_philox4x32_10(ctr, key);
+  const double s24 = 1.0 / 16777216.0, two_pi = 6.283185307179586;
+  const double u0 = ((double)(r.v[0] >> 8) + 1.0) * s24, u1 = (double)(r.v[1] >> 8) * s24;
+  const double u2 = ((double)(r.v[2] >> 8) + 1.0) * s24, u3 = (double)(r.v[3] >> 8) * s24;
+  const double m0 = sqrt(-2.0 * log(u0)), m1 = sqrt(-2.0 * log(u2));
+  out[0] = m0 * cos(two_pi * u1);
+  out[1] = m0 * sin(two_pi * u1);
+  out[2] = m1 * cos(two_pi * u3);
 }
 
 #endif /* PFG_RNG_H */

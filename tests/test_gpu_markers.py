@@ -187,3 +187,67 @@ def test_bond_decisions(gpu, po):
     print(f"bond decision mismatches {mism}, near-threshold cases {near}")
     assert mism <= near
     assert (orb >= 0).sum() > 10
+
+
+@pytest.mark.parametrize("gap", [0.35, 0.7])
+def test_contact_parity(gpu, po, gap):
+    """Platelet-platelet contact on the cell lists against the oracle's brute
+    force: the second of two platelets is moved so that the closest element
+    pair is `gap` apart; only that pair feels the repulsion."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=64, ny=48, nz=48, n_platelets=2, elements=200,
+                   receptors=-1, ligands=-1, fibrin_nodes=-1)
+    p = sc.params
+    p.morse_u0 = 0.0
+    p.contact_a, p.contact_kappa, p.contact_hc = 0.01, 10.0, 1.0
+    n0 = int(sc.plt_elem_start[1])
+    P = np.stack([sc.x, sc.y, sc.z], 1).astype(np.float64)
+    A, B = P[:n0], P[n0:]
+    # put B next to A along x, then set the closest distance to `gap`
+    B = B - B.mean(0) + A.mean(0) + np.array([A[:, 0].max() - A[:, 0].min() + 1.0, 0.3, 0.2])
+    D = np.linalg.norm(A[:, None] - B[None], axis=2)
+    B[:, 0] += gap - D.min()
+    x, y, z = (np.concatenate([A[:, k], B[:, k]]).astype(np.float32) for k in range(3))
+    s, o = pair(pfg, po, sc, p)
+    with s:
+        set_pos(s, o, x, y, z)
+        for st in (pfg.STAGE_CELLS, pfg.STAGE_MEMBRANE, pfg.STAGE_BONDS, pfg.STAGE_FORCES):
+            s.stage(st)
+            o.stage(st)
+        mk = s.markers()
+        # a second evaluation finds the same pair (the slots were emptied)
+        s.stage(pfg.STAGE_CELLS)
+        s.stage(pfg.STAGE_FORCES)
+        mk2 = s.markers()
+    F = np.stack([mk["fx"], mk["fy"], mk["fz"]], 1)
+    oF = np.stack([o.marker(k) for k in ("fx", "fy", "fz")], 1)
+    nz = np.flatnonzero(np.abs(oF).sum(1))
+    assert len(nz) == 2
+    assert np.flatnonzero(np.abs(F).sum(1)).tolist() == nz.tolist()
+    assert np.abs(F - oF).max() < 1e-5 * np.abs(oF).max()
+    F2 = np.stack([mk2["fx"], mk2["fy"], mk2["fz"]], 1)
+    assert np.array_equal(F2, F)
+
+
+def test_kick_noise_parity(gpu, po):
+    """Fibrin nodes moved by the kick with thermal noise: the device draws
+    the oracle's normals (same counter RNG), two steps of the update."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=64, ny=48, nz=48, n_platelets=1, elements=150,
+                   receptors=-1, ligands=-1, fibrin_nodes=600, fibrin_links=900)
+    p = sc.params
+    p.fiber_move, p.fiber_gamma, p.kick_temp = pfg.FIBER_KICK, 0.2, 1e-4
+    s, o = pair(pfg, po, sc, p)
+    with s:
+        for _ in range(2):
+            for st in (pfg.STAGE_CELLS, pfg.STAGE_MEMBRANE, pfg.STAGE_BONDS,
+                       pfg.STAGE_FORCES, pfg.STAGE_SPREAD, pfg.STAGE_LB, pfg.STAGE_INTERP,
+                       pfg.STAGE_UPDATE):
+                s.stage(st)
+                o.stage(st)
+        mk = s.markers()
+    fib = sc.kind == pfg.MARKER_FIBER
+    v = np.stack([mk["vx"], mk["vy"], mk["vz"]], 1)[fib]
+    ov = np.stack([o.marker(k) for k in ("vx", "vy", "vz")], 1)[fib]
+    assert np.abs(ov).max() > 0
+    assert np.abs(v - ov).max() < 1e-4 * np.abs(ov).max()
