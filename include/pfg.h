@@ -64,9 +64,17 @@ typedef enum pfg_store {
 /* Boundary treatment along one lattice axis. */
 typedef enum pfg_wall {
   PFG_WALL_PERIODIC = 0, /* periodic wrap (LB_kernel.cu:1347-1352)                */
-  PFG_WALL_BOUNCE = 1    /* halfway bounce-back walls half a cell outside cells 0
+  PFG_WALL_BOUNCE = 1,   /* halfway bounce-back walls half a cell outside cells 0
                             and n-1, each wall with its own velocity
                             (LB_kernel_old_08102011.cu:1155-1224)                 */
+  PFG_WALL_VELOCITY = 2  /* z only: inflow/outflow faces z = 0 and z = nz-1 with
+                            the velocity wall_u_lo[2] / wall_u_hi[2] times the
+                            duct profile 4s(L-s)/L^2 (s = i + 1/2) over each
+                            bounded transverse axis, imposed by Zou/He on the
+                            face cells; face cells next to a transverse wall
+                            are set to equilibrium (rho 1) at that velocity
+                            (fluid3d_in_out_flow_boundary_kernel,
+                            LB_kernel_old_08102011.cu:586-1090)                   */
 } pfg_wall;
 
 /* Force law of the fibrin links. */

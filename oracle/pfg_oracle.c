@@ -69,6 +69,7 @@ void orc_lattice_from_params(const pfg_params *p, const uint8_t *solid, orc_latt
   }
   L->tau = p->tau;
   L->solid = p->use_flags ? solid : NULL;
+  L->zvel = p->wall[2] == PFG_WALL_VELOCITY;
 }
 
 static inline int64_t cidx(const orc_lattice *L, int x, int y, int z) {
@@ -120,6 +121,59 @@ static double pull(const orc_lattice *L, const double *fin, int64_t N, int x, in
   return fin[d * N + src];
 }
 
+/* Velocity face z = 0 (s = +1) or z = nz-1 (s = -1)
+ * (fluid3d_in_out_flow_boundary_kernel, LB_kernel_old_08102011.cu:586-1090,
+ * in the current direction order and lattice units): the face velocity is
+ * the wall velocity times the duct profile 16 s1(L1-s1) s2(L2-s2)/(L1 L2)^2
+ * over the bounded transverse axes; a cell next to a transverse wall is set
+ * to feq(1, u) (the reference's fix for those cells); otherwise Zou/He:
+ *   rho = (sum_{cz=0} f + 2 sum_{cz=-s} f) / (1 - s uz),
+ *   f_{s z} = f_{-s z} + s rho uz / 3,
+ *   f_{s z, sig t} = f_{-s z, -sig t} + rho (s uz + sig u_t) / 6 - sig N_t,
+ *   N_t = 1/2 sum_{cz=0} c_t f - rho u_t / 3. */
+static void velocity_face(const orc_lattice *L, int x, int y, int z, double f[19]) {
+  const int s = z == 0 ? 1 : -1;
+  const double *U = s > 0 ? L->wall_u_lo[2] : L->wall_u_hi[2];
+  double shape = 1.0;
+  if (!L->periodic[0]) {
+    const double q = x + 0.5;
+    shape *= 4.0 * q * (L->nx - q) / ((double)L->nx * L->nx);
+  }
+  if (!L->periodic[1]) {
+    const double q = y + 0.5;
+    shape *= 4.0 * q * (L->ny - q) / ((double)L->ny * L->ny);
+  }
+  const double u[3] = {U[0] * shape, U[1] * shape, U[2] * shape};
+  if ((!L->periodic[0] && (x == 0 || x == L->nx - 1)) ||
+      (!L->periodic[1] && (y == 0 || y == L->ny - 1))) {
+    for (int d = 0; d < 19; ++d) f[d] = feq(d, 1.0, u[0], u[1], u[2]);
+    return;
+  }
+  double S0 = 0, Sm = 0, Nx = 0, Ny = 0;
+  for (int d = 0; d < 19; ++d) {
+    if (CZo[d] == 0) {
+      S0 += f[d];
+      Nx += CXo[d] * f[d];
+      Ny += CYo[d] * f[d];
+    } else if (CZo[d] == -s) {
+      Sm += f[d];
+    }
+  }
+  const double rho = (S0 + 2.0 * Sm) / (1.0 - s * u[2]);
+  Nx = 0.5 * Nx - rho * u[0] / 3.0;
+  Ny = 0.5 * Ny - rho * u[1] / 3.0;
+  for (int d = 0; d < 19; ++d) {
+    if (CZo[d] != s) continue;
+    const int o = BBo[d];
+    if (CXo[d] != 0)
+      f[d] = f[o] + rho * (s * u[2] + CXo[d] * u[0]) / 6.0 - CXo[d] * Nx;
+    else if (CYo[d] != 0)
+      f[d] = f[o] + rho * (s * u[2] + CYo[d] * u[1]) / 6.0 - CYo[d] * Ny;
+    else
+      f[d] = f[o] + s * rho * u[2] / 3.0;
+  }
+}
+
 void orc_lb_step(const orc_lattice *L, const double *fin, double *fout, const double *Fu,
                  const double *Fs, double *u_out) {
   const int64_t N = (int64_t)L->nx * L->ny * L->nz;
@@ -131,6 +185,7 @@ void orc_lb_step(const orc_lattice *L, const double *fin, double *fout, const do
         if (L->solid && L->solid[i]) continue;
         double f[19];
         for (int d = 0; d < 19; ++d) f[d] = pull(L, fin, N, x, y, z, d);
+        if (L->zvel && (z == 0 || z == L->nz - 1)) velocity_face(L, x, y, z, f);
         /* moments (LB_kernel.cu:1137-1163), C = 1 */
         double rho = 0, jx = 0, jy = 0, jz = 0;
         for (int d = 0; d < 19; ++d) {
@@ -186,9 +241,11 @@ static void pull_moments(const orc_lattice *L, const double *f, const double *Fx
           if (uz) uz[i] = 0;
           continue;
         }
-        double rho = 0, j[3] = {0, 0, 0};
+        double rho = 0, j[3] = {0, 0, 0}, fp[19];
+        for (int d = 0; d < 19; ++d) fp[d] = pull(L, f, N, x, y, z, d);
+        if (L->zvel && (z == 0 || z == L->nz - 1)) velocity_face(L, x, y, z, fp);
         for (int d = 0; d < 19; ++d) {
-          const double v = pull(L, f, N, x, y, z, d);
+          const double v = fp[d];
           rho += v;
           j[0] += CXo[d] * v;
           j[1] += CYo[d] * v;

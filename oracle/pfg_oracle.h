@@ -29,6 +29,7 @@ typedef struct orc_lattice {
   double tau;
   double g[3];
   const uint8_t *solid; /* NULL: no solid cells */
+  int zvel;             /* z faces are Zou/He velocity faces */
 } orc_lattice;
 
 void orc_lattice_from_params(const pfg_params *p, const uint8_t *solid,

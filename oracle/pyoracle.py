@@ -25,7 +25,7 @@ class Lattice(C.Structure):
                 ("periodic", C.c_int * 3),
                 ("wall_u_lo", (C.c_double * 3) * 3), ("wall_u_hi", (C.c_double * 3) * 3),
                 ("tau", C.c_double), ("g", C.c_double * 3),
-                ("solid", C.POINTER(C.c_uint8))]
+                ("solid", C.POINTER(C.c_uint8)), ("zvel", C.c_int)]
 
 
 _lib = None

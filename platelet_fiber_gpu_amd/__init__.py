@@ -19,12 +19,13 @@ from ._ffi import (FIBER_IB, FIBER_KICK, FIBER_LINEAR, FIBER_WLC, MARKER_ELEMENT
                    MARKER_FIBER, MARKER_FIXED, SCENE_BATCH, SCENE_CHANNEL, SCENE_LARGE,
                    SCENE_SHEAR, SCENE_THROMBUS, STAGE_BONDS, STAGE_CELLS, STAGE_CLEAR,
                    STAGE_FORCES, STAGE_INTERP, STAGE_LB, STAGE_MEMBRANE, STAGE_SPREAD,
-                   STAGE_UPDATE, STORE_BF16, STORE_F16, STORE_F32, WALL_BOUNCE,
+                   STAGE_UPDATE, STORE_BF16, STORE_F16, STORE_F32, WALL_BOUNCE, WALL_VELOCITY,
                    WALL_PERIODIC, Model, Params, PfgError, SceneOpts, StepInfo)
 
 __all__ = [
     "Scene", "Solver", "Batch", "Params", "Model", "StepInfo", "PfgError", "lib", "default_params",
     "device_count", "STORE_F32", "STORE_F16", "STORE_BF16", "WALL_PERIODIC", "WALL_BOUNCE",
+    "WALL_VELOCITY",
     "MARKER_ELEMENT", "MARKER_FIBER", "MARKER_FIXED", "FIBER_WLC", "FIBER_LINEAR",
     "FIBER_IB", "FIBER_KICK", "SCENE_CHANNEL", "SCENE_SHEAR", "SCENE_THROMBUS",
     "SCENE_LARGE", "SCENE_BATCH", "STAGE_CELLS", "STAGE_MEMBRANE", "STAGE_BONDS",

@@ -15,7 +15,7 @@ STATUS = {0: "ok", 1: "invalid argument", 2: "allocation failed", 3: "no usable 
           4: "kernel launch or HIP runtime error", 5: "call out of order", 6: "file I/O error"}
 
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
-WALL_PERIODIC, WALL_BOUNCE = 0, 1
+WALL_PERIODIC, WALL_BOUNCE, WALL_VELOCITY = 0, 1, 2
 FIBER_WLC, FIBER_LINEAR = 0, 1
 FIBER_IB, FIBER_KICK = 0, 1
 MARKER_ELEMENT, MARKER_FIBER, MARKER_FIXED = 0, 1, 2

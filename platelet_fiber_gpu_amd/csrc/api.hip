@@ -242,7 +242,9 @@ pfg_status check_lattice_params(const pfg_params &p) {
   if ((int64_t)p.nx * p.ny * p.nz > (int64_t)INT_MAX) return PFG_ERR_ARG;
   if (!(p.tau > 0.5f)) return PFG_ERR_ARG;
   for (int k = 0; k < 3; ++k)
-    if (p.wall[k] != PFG_WALL_PERIODIC && p.wall[k] != PFG_WALL_BOUNCE) return PFG_ERR_ARG;
+    if (p.wall[k] != PFG_WALL_PERIODIC && p.wall[k] != PFG_WALL_BOUNCE &&
+        !(k == 2 && p.wall[k] == PFG_WALL_VELOCITY))
+      return PFG_ERR_ARG;
   return PFG_OK;
 }
 

@@ -92,6 +92,68 @@ __device__ __host__ inline int64_t felem(int nxb, int ny, int d, int x, int y, i
  * one z-row per wavefront. */
 enum { LB_MODE_STEP = 0, LB_MODE_FIELDS = 1, LB_MODE_BULK = 2, LB_MODE_REGION = 3 };
 
+/* Velocity face z = 0 (s = +1) or z = nz-1 (s = -1) on the pulled
+ * deviations (fluid3d_in_out_flow_boundary_kernel,
+ * LB_kernel_old_08102011.cu:586-1090): face velocity = wall velocity x the
+ * duct profile 4 q (L - q) / L^2, q = i + 1/2, over each bounded transverse
+ * axis; a cell next to a transverse wall takes feq(1, u); the others
+ * Zou/He on the populations entering from outside,
+ *   rho = (1 + sum_{cz=0} dev + 2 sum_{cz=-s} dev) / (1 - s uz),
+ *   dev_{s z} = dev_{-s z} + s rho uz / 3,
+ *   dev_{s z, t} = dev_{opp} + rho (s uz + c_t u_t) / 6 - c_t N_t,
+ *   N_t = 1/2 sum_{cz=0} c_t dev - rho u_t / 3
+ * (the weights cancel between opposite directions, so deviations take the
+ * place of populations except in rho).  dev in storage units. */
+template <class ST>
+__device__ __forceinline__ void velocity_face(const LBArgs &a, int x, int y, int s,
+                                              float dev[Q]) {
+  const float *U = s > 0 ? a.zvel_u[0] : a.zvel_u[1];
+  float shape = 1.0f;
+  if (!a.periodic[0]) {
+    const float q = (float)x + 0.5f;
+    shape *= 4.0f * q * ((float)a.nx - q) / ((float)a.nx * (float)a.nx);
+  }
+  if (!a.periodic[1]) {
+    const float q = (float)y + 0.5f;
+    shape *= 4.0f * q * ((float)a.ny - q) / ((float)a.ny * (float)a.ny);
+  }
+  const float ux = U[0] * shape, uy = U[1] * shape, uz = U[2] * shape;
+  const float sc = ST::scaled ? a.scale : 1.0f, isc = ST::scaled ? a.inv_scale : 1.0f;
+  if ((!a.periodic[0] && (x == 0 || x == a.nx - 1)) ||
+      (!a.periodic[1] && (y == 0 || y == a.ny - 1))) {
+    const float usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int d = 0; d < Q; ++d) {
+      const float cu = 3.0f * (CX[d] * ux + CY[d] * uy + CZ[d] * uz);
+      dev[d] = sc * weight(d) * (cu + 0.5f * cu * cu - 1.5f * usq);
+    }
+    return;
+  }
+  float S0 = 0.f, Sm = 0.f, Nx = 0.f, Ny = 0.f;
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    if (CZ[d] == 0) {
+      S0 += dev[d];
+      Nx += CX[d] * dev[d];
+      Ny += CY[d] * dev[d];
+    } else if (CZ[d] == -s) {
+      Sm += dev[d];
+    }
+  }
+  const float rho = (1.0f + (S0 + 2.0f * Sm) * isc) / (1.0f - (float)s * uz);
+  Nx = 0.5f * Nx * isc - rho * ux / 3.0f;
+  Ny = 0.5f * Ny * isc - rho * uy / 3.0f;
+#pragma unroll
+  for (int d = 0; d < Q; ++d) {
+    if (CZ[d] != s) continue;
+    float t;
+    if (CX[d] != 0) t = rho * ((float)s * uz + CX[d] * ux) / 6.0f - CX[d] * Nx;
+    else if (CY[d] != 0) t = rho * ((float)s * uz + CY[d] * uy) / 6.0f - CY[d] * Ny;
+    else t = (float)s * rho * uz / 3.0f;
+    dev[d] = dev[BB[d]] + sc * t;
+  }
+}
+
 /* One wavefront per (x-block, y, CPT consecutive z); workgroup `blk` of the
  * lattice described by `a` (in memory that the whole workgroup reads). */
 template <int S, int CPT, int MODE>
@@ -191,6 +253,7 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
         }
       }
     }
+    if (a.zvel && (z0 || z1)) velocity_face<ST>(a, x, y, z0 ? 1 : -1, dev);
 
     /* moments: rho - 1 and momentum from deviations (sum_i w_i c_i = 0),
      * over the nine pairs (d, opposite of d): the pair sum feeds the density,
@@ -479,6 +542,11 @@ void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a) {
         const float cu = CX[d] * u[0] + CY[d] * u[1] + CZ[d] * u[2];
         a->wall_term[ax][s][d] = 6.0f * weight(d) * cu * a->scale;
       }
+  a->zvel = p.wall[2] == PFG_WALL_VELOCITY ? 1 : 0;
+  for (int k = 0; k < 3; ++k) {
+    a->zvel_u[0][k] = p.wall_u_lo[2][k];
+    a->zvel_u[1][k] = p.wall_u_hi[2][k];
+  }
 }
 
 template <int S, int CPT>

@@ -66,6 +66,8 @@ struct LBArgs {
   float ws[3], gB[3], gD[3];
   /* 6 w_d (c_d . u_wall) for [axis][0 lower, 1 upper][d] */
   float wall_term[3][2][Q];
+  int zvel;               /* z faces are Zou/He velocity faces                */
+  float zvel_u[2][3];     /* their velocities (z = 0, z = nz-1)               */
 };
 
 /* Lattice side of the IB coupling for one lattice of a batch (a handle

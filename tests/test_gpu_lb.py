@@ -28,6 +28,11 @@ CASES = [
     dict(name="channel_yz", walls=(0, 1, 1), g=(1e-5, 2e-6, 0)),
     dict(name="couette_z", walls=(0, 0, 1), u_hi=(0.05, 0.01, 0)),
     dict(name="box_flags", walls=(1, 1, 1), flags=True),
+    # inflow/outflow faces: duct profile over bounded x and y, one over y only
+    dict(name="velocity_faces_duct", walls=(1, 1, 2), u_hi=(0.0, 0.0, 0.03),
+         u_lo=(0.01, -0.005, 0.04)),
+    dict(name="velocity_faces_slab", walls=(0, 1, 2), u_hi=(0.0, 0.0, 0.02),
+         u_lo=(0.0, 0.0, 0.02), g=(0, 0, 1e-6)),
 ]
 
 
@@ -42,6 +47,9 @@ def test_single_step_parity(gpu, po, case):
         flags = (rng.random(N) < 0.1).astype(np.uint8)
     p = make_params(pfg, nx, ny, nz, case["walls"], g=case.get("g", (0, 0, 0)),
                     u_hi=case.get("u_hi"), use_flags=int(flags is not None))
+    if case.get("u_lo") is not None:
+        for k in range(3):
+            p.wall_u_lo[2][k] = case["u_lo"][k]
     w = w_col(po)
     f = (w * (1 + 0.05 * rng.standard_normal((Q, N)))).astype(np.float32)
     with pfg.Solver(p) as s:
@@ -162,3 +170,27 @@ def test_launch_shapes_agree(gpu, cpt, store):
         s.set_fluid_only(True)
         s.step(3)
         assert np.array_equal(s.f(), ref)
+
+
+@pytest.mark.parametrize("store,tol", [(0, 2e-6), (1, 2e-4)])
+def test_velocity_faces_multistep(gpu, po, store, tol):
+    """Inflow/outflow channel (bounce-back in y, periodic x), 300 steps from
+    rest against the oracle, FP32 and FP16 storage."""
+    pfg = gpu
+    nx, ny, nz = 8, 16, 24
+    p = make_params(pfg, nx, ny, nz, (0, 1, 2), store=store, tau=1.0, u_hi=(0, 0, 0.02))
+    for k in range(3):
+        p.wall_u_lo[2][k] = (0.0, 0.0, 0.02)[k]
+    with pfg.Solver(p) as s:
+        s.init_equilibrium()
+        s.set_fluid_only(True)
+        s.step(300)
+        rho_g, ux_g, uy_g, uz_g = s.fields()
+    L = po.lattice(p)
+    f = po.init_eq(L)
+    for _ in range(300):
+        f = po.lb_step(L, f)
+    rho_o, ux_o, uy_o, uz_o = po.fields(L, f)
+    assert np.abs(uz_o).max() > 0.01
+    assert np.abs(uz_g - uz_o).max() < tol
+    assert np.abs(rho_g - rho_o).max() < 10 * tol
