@@ -70,13 +70,16 @@ def test_reference_mesh_platelet(gpu, po, kbend):
     """The reference's own platelet mesh (Platelet_Mesh_958.neu, 958 nodes,
     1912 triangles) in the channel, with mesh springs and the area and
     volume constraints on, without and with membrane bending, 50 steps
-    against the oracle."""
+    against the oracle.  Morse is off: its cutoff switches pair forces on
+    and off on rounding, which the element-wise comparison here would see
+    (the channel tests above cover it through centroids)."""
     import os
     pfg = gpu
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fixtures",
                         "Platelet_Mesh_958.neu")
     sc = pfg.Scene(pfg.SCENE_CHANNEL, mesh_path=path)
     sc.params.k_spring, sc.params.ks, sc.params.kv = 0.3, 0.5, 0.4
+    sc.params.morse_u0 = 0.0
     sc.params.kbend = kbend
     g, o = run_both(pfg, po, sc, 50)
     ofx = po.fields(po.lattice(sc.params), o.f())
