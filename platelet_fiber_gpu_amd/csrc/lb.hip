@@ -91,6 +91,10 @@ __device__ __host__ inline int64_t felem(int nxb, int ny, int d, int x, int y, i
  * (rows that this step's IB stencils touch), without / with the IB force;
  * one z-row per wavefront. */
 enum { LB_MODE_STEP = 0, LB_MODE_FIELDS = 1, LB_MODE_BULK = 2, LB_MODE_REGION = 3 };
+/* Flag bit of MODE: the lattice has velocity faces on z (compiled only into
+ * the variant that needs it, which keeps the scalar registers and the
+ * occupancy of the default kernels). */
+enum { LB_ZVEL = 4 };
 
 /* Velocity face z = 0 (s = +1) or z = nz-1 (s = -1) on the pulled
  * deviations (fluid3d_in_out_flow_boundary_kernel,
@@ -159,12 +163,14 @@ __device__ __forceinline__ void velocity_face(const LBArgs &a, int x, int y, int
 template <int S, int CPT, int MODE>
 __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *orho,
                                           float *oux, float *ouy, float *ouz) {
+  constexpr int M = MODE & 3;
+  constexpr bool ZV = (MODE & LB_ZVEL) != 0;
   using ST = Store<S>;
   using T = typename ST::T;
   constexpr int R = Q * XBLOCK; /* values per x-block */
   const T *__restrict__ fin = (const T *)a.fin;
   T *__restrict__ fout = (T *)a.fout;
-  if (MODE != LB_MODE_FIELDS && a.halt != nullptr && *a.halt != 0) return;
+  if (M != LB_MODE_FIELDS && a.halt != nullptr && *a.halt != 0) return;
   const int lane = threadIdx.x & (XBLOCK - 1);
   /* wavefront-uniform task decomposition in 32-bit arithmetic (x-blocks per
    * lattice stay below 2^31) */
@@ -173,11 +179,11 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
   const uint32_t zc = task / rows;
   if (zc * CPT >= (uint32_t)a.nz) return;
   /* IB force tiles and rows are stamped with the step that touches them */
-  const int32_t step32 = MODE == LB_MODE_FIELDS ? 0 : (int32_t)*a.step;
-  if (MODE == LB_MODE_BULK || MODE == LB_MODE_REGION) {
-    static_assert(MODE < LB_MODE_BULK || CPT == 1, "row modes update one z-row per wave");
+  const int32_t step32 = M == LB_MODE_FIELDS ? 0 : (int32_t)*a.step;
+  if (M == LB_MODE_BULK || M == LB_MODE_REGION) {
+    static_assert(M < LB_MODE_BULK || CPT == 1, "row modes update one z-row per wave");
     const bool touched = a.row_stamp[task] == step32; /* task = x-block row */
-    if (touched != (MODE == LB_MODE_REGION)) return;
+    if (touched != (M == LB_MODE_REGION)) return;
   }
   const uint32_t rem = task - zc * rows;
   const int y = (int)(rem / (uint32_t)a.nxb);
@@ -253,7 +259,7 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
         }
       }
     }
-    if (a.zvel && (z0 || z1)) velocity_face<ST>(a, x, y, z0 ? 1 : -1, dev);
+    if (ZV && (z0 || z1)) velocity_face<ST>(a, x, y, z0 ? 1 : -1, dev);
 
     /* moments: rho - 1 and momentum from deviations (sum_i w_i c_i = 0),
      * over the nine pairs (d, opposite of d): the pair sum feeds the density,
@@ -278,7 +284,7 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
     const float rho = 1.0f + drho;
     float Fx = a.g[0], Fy = a.g[1], Fz = a.g[2];
     bool flagged = false;
-    if (MODE == LB_MODE_STEP || MODE == LB_MODE_REGION) {
+    if (M == LB_MODE_STEP || M == LB_MODE_REGION) {
       const int tile = (x >> TILE_SHIFT_X) + a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
       flagged = a.tile_stamp[tile] == step32;
       if (flagged) {
@@ -295,7 +301,7 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
     const float uy = (jy + 0.5f * Fy) * inv_rho;
     const float uz = (jz + 0.5f * Fz) * inv_rho;
 
-    if (MODE == LB_MODE_FIELDS) {
+    if (M == LB_MODE_FIELDS) {
       if (orho) orho[idx] = rho;
       if (oux) oux[idx] = ux;
       if (ouy) ouy[idx] = uy;
@@ -549,25 +555,32 @@ void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a) {
   }
 }
 
+template <int S, int CPT, int MODE>
+static void launch_fused(const LBArgs &a, dim3 grid, float *rho, float *ux, float *uy,
+                         float *uz, hipStream_t st) {
+  if (a.zvel)
+    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, MODE | LB_ZVEL>), grid, dim3(256), 0, st, a,
+                       rho, ux, uy, uz);
+  else
+    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, MODE>), grid, dim3(256), 0, st, a, rho, ux,
+                       uy, uz);
+}
+
 template <int S, int CPT>
 static hipError_t launch_lb_t(pfg_handle *h, const LBArgs &a, int mode, float *rho,
                               float *ux, float *uy, float *uz, hipStream_t st) {
   const int64_t waves = (int64_t)a.nxb * a.ny * ((a.nz + CPT - 1) / CPT);
-  const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+  const dim3 grid((unsigned)((waves + 3) / 4));
   if (mode == LB_MODE_STEP)
-    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_STEP>), grid, block, 0, st, a,
-                       nullptr, nullptr, nullptr, nullptr);
+    launch_fused<S, CPT, LB_MODE_STEP>(a, grid, nullptr, nullptr, nullptr, nullptr, st);
   else if (mode == LB_MODE_FIELDS)
-    hipLaunchKernelGGL((lb_fused_kernel<S, CPT, LB_MODE_FIELDS>), grid, block, 0, st, a,
-                       rho, ux, uy, uz);
+    launch_fused<S, CPT, LB_MODE_FIELDS>(a, grid, rho, ux, uy, uz, st);
   else if (CPT != 1)
     return hipErrorInvalidValue;
   else if (mode == LB_MODE_BULK)
-    hipLaunchKernelGGL((lb_fused_kernel<S, 1, LB_MODE_BULK>), grid, block, 0, st, a,
-                       nullptr, nullptr, nullptr, nullptr);
+    launch_fused<S, 1, LB_MODE_BULK>(a, grid, nullptr, nullptr, nullptr, nullptr, st);
   else
-    hipLaunchKernelGGL((lb_fused_kernel<S, 1, LB_MODE_REGION>), grid, block, 0, st, a,
-                       nullptr, nullptr, nullptr, nullptr);
+    launch_fused<S, 1, LB_MODE_REGION>(a, grid, nullptr, nullptr, nullptr, nullptr, st);
   return hipGetLastError();
 }
 
@@ -593,22 +606,24 @@ static hipError_t dispatch_lb(pfg_handle *h, int k, int mode, float *rho, float 
   }
 }
 
-template <int S>
-static hipError_t launch_lb_batch_s(pfg_handle *h, int mode, hipStream_t st) {
+template <int S, int MODE>
+static void launch_batch(pfg_handle *h, hipStream_t st) {
   const dim3 grid((unsigned)h->lb_blocks), block(256);
   const LBArgs *args = h->d_lbargs[h->cur];
+  if (h->p.wall[2] == PFG_WALL_VELOCITY) /* the lattices of a batch share the walls */
+    hipLaunchKernelGGL((lb_batch_kernel<S, MODE | LB_ZVEL>), grid, block, 0, st, args,
+                       h->lb_block_inst);
+  else
+    hipLaunchKernelGGL((lb_batch_kernel<S, MODE>), grid, block, 0, st, args,
+                       h->lb_block_inst);
+}
+
+template <int S>
+static hipError_t launch_lb_batch_s(pfg_handle *h, int mode, hipStream_t st) {
   switch (mode) {
-    case LB_MODE_BULK:
-      hipLaunchKernelGGL((lb_batch_kernel<S, LB_MODE_BULK>), grid, block, 0, st, args,
-                         h->lb_block_inst);
-      break;
-    case LB_MODE_REGION:
-      hipLaunchKernelGGL((lb_batch_kernel<S, LB_MODE_REGION>), grid, block, 0, st, args,
-                         h->lb_block_inst);
-      break;
-    default:
-      hipLaunchKernelGGL((lb_batch_kernel<S, LB_MODE_STEP>), grid, block, 0, st, args,
-                         h->lb_block_inst);
+    case LB_MODE_BULK: launch_batch<S, LB_MODE_BULK>(h, st); break;
+    case LB_MODE_REGION: launch_batch<S, LB_MODE_REGION>(h, st); break;
+    default: launch_batch<S, LB_MODE_STEP>(h, st);
   }
   return hipGetLastError();
 }
