@@ -83,21 +83,144 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(pfg, steps=3):
-    """FP64 single-thread oracle (a port of the same algorithm) on a bounded
-    sample of the same workload: `steps` coupled steps of the 256x128x128
-    shear scene."""
+def oracle_threads():
+    """Host threads for the FP64 oracle: the process's CPU share, at most 16
+    (OMP_NUM_THREADS when it is set)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def load_oracle():
+    """The FP64 oracle, used only in the CPU leg below: timed as the CPU
+    baseline and, as the checker, for the accuracy figures of the GPU runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
+    threads = pyoracle.set_threads(oracle_threads())
+    return pyoracle, threads
+
+
+def time_oracle(o, cells, budget_s=10.0, min_steps=2, max_steps=200):
+    """Coupled oracle steps until about `budget_s` of wall time: MLUPS, steps,
+    seconds."""
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < min_steps or (time.perf_counter() - t0 < budget_s and steps < max_steps):
+        o.step(1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return cells * steps / dt / 1e6, steps, dt
+
+
+def cpu_leg_shear(pfg):
+    """CPU leg of the headline workload.  Baseline: the FP64 oracle (the same
+    algorithm, OpenMP over cells) on a bounded sample of the same workload,
+    about 10 s of coupled steps of the 256x128x128 shear scene.  Checker: the
+    FP16-store velocity error against the oracle on a 64x32x32 body-force
+    channel, tau 0.8, F = 1e-6, 300 steps."""
+    import numpy as np
+    po, threads = load_oracle()
     scene = pfg.Scene(pfg.SCENE_SHEAR)
-    o = pyoracle.Oracle(scene.params, scene=scene)
+    o = po.Oracle(scene.params, scene=scene)
+    mlups, steps, dt = time_oracle(o, scene.N)
+    o.close()
+    cpu = {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+           "sample": f"{steps} coupled steps of the 256x128x128 shear workload, FP64 oracle, "
+                     f"{threads} OpenMP threads ({dt:.1f} s)"}
+    p = pfg.default_params(nx=64, ny=32, nz=32, tau=0.8, store=pfg.STORE_F16)
+    p.wall[1] = p.wall[2] = pfg.WALL_BOUNCE
+    p.body_force[0] = 1e-6
+    with pfg.Solver(p) as s:
+        s.set_fluid_only(True)
+        s.step(300)
+        _, ux, uy, uz = s.fields()
+    L = po.lattice(p)
+    f = po.init_eq(L)
+    for _ in range(300):
+        f = po.lb_step(L, f)
+    _, ox, oy, oz = po.fields(L, f)
+    num = np.sqrt(((ux - ox) ** 2 + (uy - oy) ** 2 + (uz - oz) ** 2).sum())
+    den = np.sqrt((ox ** 2 + oy ** 2 + oz ** 2).sum())
+    return cpu, float(num / den)
+
+
+def subbox_scene(pfg, n=128):
+    """The large configuration's physics on an n^3 sub-box with the same
+    per-cell parameters: tau 0.7, Couette with the shear rate of a 0.05 wall
+    velocity over 512 cells, FP16 deviation storage, the same densities of
+    platelets (2000 x 1000 elements per 1024x512x512), fibrin (80k nodes,
+    120k links) and wall ligands (65536 per 1024x512)."""
+    import numpy as np
+    frac = n ** 3 / (1024 * 512 * 512)
+    sc = pfg.Scene(pfg.SCENE_LARGE, nx=n, ny=n, nz=n,
+                   n_platelets=max(1, round(2000 * frac)),
+                   fibrin_nodes=max(8, round(80000 * frac)),
+                   fibrin_links=max(8, round(120000 * frac)),
+                   ligands=max(64, round(65536 * n * n / (1024 * 512))))
+    uw = 0.05 * n / 512
+    sc.params.wall_u_hi[2][0] = uw
+    sc.params.detach_rate = 0.05 / 512
+    sc.u0[0][:] = uw * (np.arange(sc.N) // (n * n) + 0.5) / n
+    return sc
+
+
+def cpu_leg_large(pfg, steps=200):
+    """CPU leg of the large workload.  Checker: velocity and density L2 and
+    platelet-centroid errors of the FP16-store device run against the FP64
+    oracle on a 128^3 sub-box of the same physics (`steps` coupled steps),
+    with an FP32-store device run of the same sub-box alongside: the Morse
+    cutoff makes element trajectories sensitive to rounding, so the FP32
+    figures show how much of the difference is not the storage format.
+    Baseline: the oracle's rate on that same run."""
+    import numpy as np
+    po, threads = load_oracle()
+    sc = subbox_scene(pfg)
+    runs = {}
+    for store in (pfg.STORE_F16, pfg.STORE_F32):
+        sc.params.store = store
+        with pfg.Solver(sc.params) as s:
+            s.load(sc)
+            s.step(steps)
+            runs[store] = (s.fields(), s.markers())
+    sc.params.store = pfg.STORE_F16
+    o = po.Oracle(sc.params, scene=sc)
     t0 = time.perf_counter()
     o.step(steps)
     dt = time.perf_counter() - t0
-    return {"value": round(scene.N * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": 1,
-            "kind": "port",
-            "sample": f"{steps} coupled steps of the 256x128x128 shear workload, FP64 "
-                      f"oracle, 1 thread ({dt:.1f} s)"}
+    orho, ox, oy, oz = po.fields(po.lattice(sc.params), o.f())
+    OP = np.stack([o.marker(k) for k in ("x", "y", "z")], 1)
+    o.close()
+    body = sc.body
+    npl = int(body.max()) + 1 if (body >= 0).any() else 0
+    ocen = np.array([OP[body == b].mean(0) for b in range(npl)])
+    X0 = np.stack([sc.x, sc.y, sc.z], 1).astype(np.float64)
+    disp = np.array([np.linalg.norm(ocen[b] - X0[body == b].mean(0)) for b in range(npl)])
+    den = np.sqrt((ox ** 2 + oy ** 2 + oz ** 2).sum())
+    acc = {"case": f"128^3 sub-box of the large configuration, {npl} platelets, "
+                   f"{sc.n_markers} markers, {steps} coupled steps, device vs FP64 oracle "
+                   f"(FP32-store run of the same sub-box alongside)",
+           "centroid_disp_mean": float(disp.mean()) if npl else None}
+    for store, tag in ((pfg.STORE_F16, "f16"), (pfg.STORE_F32, "f32")):
+        (rho, ux, uy, uz), mk = runs[store]
+        P = np.stack([mk["x"], mk["y"], mk["z"]], 1).astype(np.float64)
+        cen = np.array([P[body == b].mean(0) for b in range(npl)])
+        dcen = np.linalg.norm(cen - ocen, axis=1)
+        num = np.sqrt(((ux - ox) ** 2 + (uy - oy) ** 2 + (uz - oz) ** 2).sum())
+        acc[tag] = {"rel_l2_u": float(num / den),
+                    "rel_l2_rho": float(np.sqrt(((rho - orho) ** 2).sum() / (orho ** 2).sum())),
+                    "max_abs_rho": float(np.abs(rho - orho).max()),
+                    "centroid_err_max": float(dcen.max()) if npl else None,
+                    "centroid_err_mean": float(dcen.mean()) if npl else None}
+    cpu = {"value": round(sc.N * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads,
+           "kind": "port",
+           "sample": f"{steps} coupled steps of the 128^3 sub-box of the large workload, FP64 "
+                     f"oracle, {threads} OpenMP threads ({dt:.1f} s)"}
+    return cpu, acc
 
 
 WORKLOADS = {
@@ -111,7 +234,7 @@ WORKLOADS = {
 }
 
 
-def other_workload(pfg, name, K, W):
+def other_workload(pfg, name, K, W, cpu_leg=True):
     """Coupled steps and the LB kernel on a full-size configuration other than
     the headline one (same JSON schema; value = coupled MLUPS over all
     lattice cells)."""
@@ -132,6 +255,10 @@ def other_workload(pfg, name, K, W):
     achieved = bpc * fluid / (lb_us * 1e-6) / 1e9
     mlups = scene.N * K / dt / 1e6
     dtype = {0: "f32", 1: "f16", 2: "bf16"}[store]
+    cpu, acc = None, None
+    if cpu_leg and name == "large":
+        log("large: 128^3 sub-box against the FP64 oracle ...")
+        cpu, acc = cpu_leg_large(pfg)
     return {
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": round(dt * 1e3 / K, 4),
@@ -147,7 +274,8 @@ def other_workload(pfg, name, K, W):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": f"lb_fused_kernel<{dtype.upper()}>",
                      "bytes_per_cell": bpc, "lb_kernel_us": round(lb_us, 2)},
-        "cpu_baseline": None,
+        "accuracy_vs_fp64": acc,
+        "cpu_baseline": cpu,
     }
 
 
@@ -234,35 +362,13 @@ def batch_workload(pfg, K, W, store):
     }
 
 
-def fp16_accuracy(pfg):
-    """Relative L2 velocity error of FP16 deviation storage against the FP64
-    oracle: 64x32x32 body-force channel, tau 0.8, F = 1e-6, 300 steps."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-    import pyoracle
-    p = pfg.default_params(nx=64, ny=32, nz=32, tau=0.8, store=pfg.STORE_F16)
-    p.wall[1] = p.wall[2] = pfg.WALL_BOUNCE
-    p.body_force[0] = 1e-6
-    with pfg.Solver(p) as s:
-        s.set_fluid_only(True)
-        s.step(300)
-        _, ux, uy, uz = s.fields()
-    L = pyoracle.lattice(p)
-    f = pyoracle.init_eq(L)
-    for _ in range(300):
-        f = pyoracle.lb_step(L, f)
-    _, ox, oy, oz = pyoracle.fields(L, f)
-    num = np.sqrt(((ux - ox) ** 2 + (uy - oy) ** 2 + (uz - oz) ** 2).sum())
-    den = np.sqrt((ox ** 2 + oy ** 2 + oz ** 2).sum())
-    return float(num / den)
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU leg (FP64 oracle baseline and accuracy checks)")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--lb-only", choices=sorted(STORES), default=None)
     ap.add_argument("--copy-probe", action="store_true")
@@ -294,7 +400,8 @@ def main():
         print(json.dumps(batch_workload(pfg, K, W, STORES[args.store])), flush=True)
         return
     if args.workload != "shear":
-        print(json.dumps(other_workload(pfg, args.workload, K, W)), flush=True)
+        print(json.dumps(other_workload(pfg, args.workload, K, W, not args.no_cpu)),
+              flush=True)
         return
     scene = pfg.Scene(pfg.SCENE_SHEAR)
     N = scene.N
@@ -331,7 +438,6 @@ def main():
         sl16, dtl16 = timed_run(pfg, scene, pfg.STORE_F16, K, W, fluid_only=True)
         sl16.close()
         lb16_us = lb_kernel_us(pfg, scene, pfg.STORE_F16, launches)
-        err16 = fp16_accuracy(pfg)
         res16 = {"coupled_mlups": round(N * K / dt16 / 1e6, 1),
                  "coupled_steps_per_s": round(K / dt16, 1),
                  "lb_only_mlups": round(N * K / dtl16 / 1e6, 1),
@@ -340,14 +446,16 @@ def main():
                  "speedup_coupled_vs_fp32": round(dt32 / dt16, 3),
                  "speedup_lb_only_vs_fp32": round(dtl / dtl16, 3),
                  "speedup_lb_kernel_vs_fp32": round(lb32_us / lb16_us, 3),
-                 "rel_l2_u_vs_fp64": err16,
+                 "rel_l2_u_vs_fp64": None,
                  "rel_l2_u_case": "64x32x32 channel, tau 0.8, F 1e-6, 300 steps; tol 1e-3"}
         log(f"fp16: {res16}")
 
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (FP64 oracle, 1 thread) ...")
-        cpu = cpu_baseline(pfg)
+        log("cpu leg: FP64 oracle baseline and FP16 accuracy ...")
+        cpu, err16 = cpu_leg_shear(pfg)
+        if res16:
+            res16["rel_l2_u_vs_fp64"] = err16
 
     out = {
         "metric": METRIC,

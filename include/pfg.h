@@ -241,7 +241,7 @@ const char *pfg_version(void);
 const char *pfg_status_string(pfg_status s);
 pfg_status pfg_device_count(int *count);
 /* sizeof of the ABI structs, for binding checks: 0 pfg_params, 1 pfg_model,
- * 2 pfg_step_info, 3 pfg_scene_opts; 0 for an unknown id. */
+ * 2 pfg_step_info, 3 pfg_scene_opts, 4 pfg_tuning; 0 for an unknown id. */
 size_t pfg_struct_size(int which);
 
 /* Default parameters: 64x32x32, tau 0.8, all walls periodic, FP32 storage. */
@@ -391,6 +391,28 @@ pfg_status pfg_copy_probe(int device, size_t bytes, int reps, double *gbs);
 /* LB-kernel launch tuning: lattice cells each lane updates along z (1, 2 or
  * 4; a wavefront updates one 64-cell x-block of that many rows). */
 pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane);
+
+/* Launch shapes chosen by pfg_autotune. */
+typedef struct pfg_tuning {
+  int lb_cells_per_lane; /* pfg_set_lb_tuning (1 for batches)               */
+  int forces_lanes;      /* pfg_set_forces_lanes: 0 automatic, 1, 4, 16, 64  */
+  int ib_staging;        /* pfg_set_ib_staging: 0 automatic, 1, 2            */
+  int overlap;           /* pfg_set_overlap                                  */
+  double ms_per_step;    /* the chosen shapes                                */
+  double ms_default;     /* the automatic shapes                             */
+  int shapes_timed;
+} pfg_tuning;
+/* Times `steps` coupled steps of the handle's own lattices and markers for
+ * each launch shape, one knob at a time from the automatic choices (LB cells
+ * per lane, lanes per marker in the force pass, the spreading form, the LB
+ * overlap), keeps the fastest value of each and sets it on the handle.  The
+ * state is saved on the device before and restored after (the handle
+ * continues from where it was; the trial steps leave no trace), at the cost
+ * of one extra copy of the populations and of the marker state while it
+ * runs.  The reference has no counterpart (its launch shapes are fixed,
+ * semLB.cu:56-78, sem_kernel.cu:131-193).  PFG_ERR_STATE with halt at
+ * detach on. */
+pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out);
 
 #ifdef __cplusplus
 }

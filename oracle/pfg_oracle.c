@@ -29,6 +29,21 @@
  *                               (Random123); p = 1 - exp(-k dt) closed form.
  *   update / kick               closed form; kick against network::solverkick
  *                               compiled from the reference (oracle/_ref).
+ *   bending chain               mean curvature of spheres and ellipsoids against
+ *                               their closed forms; the ring Laplacian exact on
+ *                               linear and quadratic functions over planar
+ *                               patches; the hexagonal Voronoi area; the force
+ *                               independent of the triangle winding.
+ *   contact                     the closest element pair by brute force and the
+ *                               closed-form magnitude; zero beyond the cutoff.
+ *   kick noise                  Box-Muller moments (mean, variance, kurtosis);
+ *                               equipartition of the Ornstein-Uhlenbeck step.
+ *   velocity faces              exact face velocities after a step; plane
+ *                               Poiseuille profile and pressure gradient.
+ *
+ * The lattice loops (init, LB step, moments) and the IB interpolation run
+ * under OpenMP when it is enabled: every cell and marker is computed on its
+ * own, so the results do not depend on the thread count.
  */
 #define _DEFAULT_SOURCE /* M_PI */
 #include "pfg_oracle.h"
@@ -39,6 +54,10 @@
 #include <string.h>
 
 #include "../platelet_fiber_gpu_amd/csrc/pfg_rng.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ---------------- D3Q19 (LB_kernel.cu:40-47) ---------------- */
 
@@ -85,6 +104,7 @@ static inline double feq(int d, double rho, double ux, double uy, double uz) {
 void orc_init_eq(const orc_lattice *L, const double *rho, const double *ux,
                  const double *uy, const double *uz, double *f) {
   const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < N; ++i) {
     const double r = rho ? rho[i] : 1.0;
     const double a = ux ? ux[i] : 0.0, b = uy ? uy[i] : 0.0, c = uz ? uz[i] : 0.0;
@@ -178,6 +198,7 @@ void orc_lb_step(const orc_lattice *L, const double *fin, double *fout, const do
                  const double *Fs, double *u_out) {
   const int64_t N = (int64_t)L->nx * L->ny * L->nz;
   const double tau = L->tau;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int z = 0; z < L->nz; ++z)
     for (int y = 0; y < L->ny; ++y)
       for (int x = 0; x < L->nx; ++x) {
@@ -230,6 +251,7 @@ void orc_lb_step(const orc_lattice *L, const double *fin, double *fout, const do
 static void pull_moments(const orc_lattice *L, const double *f, const double *Fx3,
                          double *rho_o, double *ux, double *uy, double *uz) {
   const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int z = 0; z < L->nz; ++z)
     for (int y = 0; y < L->ny; ++y)
       for (int x = 0; x < L->nx; ++x) {
@@ -391,6 +413,7 @@ void orc_ib_interp(const orc_lattice *L, int n, const double *x, const double *y
                    const double *z, const double *u, const int *coupled, double *vx,
                    double *vy, double *vz) {
   const int64_t N = (int64_t)L->nx * L->ny * L->nz;
+#pragma omp parallel for schedule(static)
   for (int m = 0; m < n; ++m) {
     if (coupled && !coupled[m]) continue;
     stencil s;
@@ -784,6 +807,16 @@ void orc_destroy(orc_state *s) {
 }
 
 void orc_set_order(orc_state *s, int order) { s->order = order; }
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 void orc_set_step(orc_state *s, int64_t step) { s->step = step; }
 
 /* ---- membrane: sem_calculate_A_kernel (sem_kernel.cu:724-872) ---- */

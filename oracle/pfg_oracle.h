@@ -92,6 +92,9 @@ orc_state *orc_create(const pfg_params *p, const pfg_model *m, const uint8_t *fl
 void orc_destroy(orc_state *s);
 /* order 0: fused (device order), 1: reference order (see orc_lb_step). */
 void orc_set_order(orc_state *s, int order);
+/* OpenMP threads of the lattice loops (n <= 0 keeps the current count);
+ * returns the count in use, 1 without OpenMP. */
+int orc_set_threads(int n);
 void orc_step(orc_state *s);
 /* One stage, same numbering as pfg_stage. */
 void orc_stage(orc_state *s, int stage);

@@ -63,6 +63,7 @@ def load():
         "orc_create": (C.c_void_p, [C.POINTER(Params), C.POINTER(Model), C.POINTER(C.c_uint8)]),
         "orc_destroy": (None, [C.c_void_p]),
         "orc_set_order": (None, [C.c_void_p, C.c_int]),
+        "orc_set_threads": (C.c_int, [C.c_int]),
         "orc_step": (None, [C.c_void_p]),
         "orc_stage": (None, [C.c_void_p, C.c_int]),
         "orc_set_step": (None, [C.c_void_p, C.c_int64]),
@@ -86,6 +87,12 @@ def load():
         fn.argtypes = a
     _lib = L
     return L
+
+
+def set_threads(n=0):
+    """OpenMP threads of the oracle's lattice loops (0 keeps the current
+    count); returns the count in use."""
+    return int(load().orc_set_threads(int(n)))
 
 
 def P(a, ct=C.c_double):

@@ -394,6 +394,13 @@ class Solver:
     def set_lb_tuning(self, cells_per_lane):
         self._c("pfg_set_lb_tuning", int(cells_per_lane))
 
+    def autotune(self, steps=32):
+        """Time `steps` steps per launch shape, keep the fastest on the
+        handle (the state is restored afterwards); returns the choice."""
+        t = _ffi.Tuning()
+        self._c("pfg_autotune", int(steps), C.byref(t))
+        return {k: getattr(t, k) for k, _ in _ffi.Tuning._fields_}
+
     def device_bytes(self):
         b = C.c_size_t()
         self._c("pfg_device_bytes", C.byref(b))

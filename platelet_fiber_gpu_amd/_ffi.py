@@ -74,6 +74,13 @@ class StepInfo(C.Structure):
                 ("bond_events", C.c_int64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("lb_cells_per_lane", C.c_int32), ("forces_lanes", C.c_int32),
+                ("ib_staging", C.c_int32), ("overlap", C.c_int32),
+                ("ms_per_step", C.c_double), ("ms_default", C.c_double),
+                ("shapes_timed", C.c_int32)]
+
+
 class SceneOpts(C.Structure):
     _fields_ = [("config", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
                 ("nz", C.c_int32), ("n_platelets", C.c_int32), ("elements", C.c_int32),
@@ -137,6 +144,7 @@ SIGNATURES = {
     "pfg_time_lb": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "pfg_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "pfg_set_lb_tuning": (C.c_int, [C.c_void_p, C.c_int]),
+    "pfg_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tuning)]),
     "pfg_scene_opts_init": (None, [C.POINTER(SceneOpts), C.c_int]),
     "pfg_scene_build": (C.c_int, [C.POINTER(SceneOpts), C.POINTER(Params),
                                   C.POINTER(C.c_void_p)]),

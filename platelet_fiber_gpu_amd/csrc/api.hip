@@ -1526,6 +1526,200 @@ pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane) {
   return PFG_OK;
 }
 
+/* ---- autotuning of the launch shapes ---- */
+
+namespace {
+
+/* The state a step carries into the next one: the current population copy,
+ * marker positions and velocities, bonds and ligand ownership, the counters,
+ * the detach step and the step index.  Everything else is rebuilt inside a
+ * step (cell lists, membrane sums, forces) or reset here (the force field,
+ * the tile and row stamps). */
+struct Snapshot {
+  void *f = nullptr;
+  size_t f_bytes = 0;
+  float *mk = nullptr;   /* x, y, z, vx, vy, vz */
+  int32_t *bonds = nullptr; /* rec_bond, lig_owner, lig_claim */
+  int64_t *ctr = nullptr;   /* counters[8], detach_step, d_step */
+  int cur = 0;
+  int64_t h_step = 0;
+};
+
+size_t population_bytes(const pfg_handle *h) {
+  size_t blocks = 0;
+  for (const Inst &I : h->inst) blocks += (size_t)I.nxb * I.p.ny * I.p.nz;
+  return (size_t)h->esize * Q * XBLOCK * blocks;
+}
+
+void snapshot_free(Snapshot &S) {
+  if (S.f) hipFree(S.f);
+  if (S.mk) hipFree(S.mk);
+  if (S.bonds) hipFree(S.bonds);
+  if (S.ctr) hipFree(S.ctr);
+  S = Snapshot{};
+}
+
+/* copies between the handle and the snapshot on the handle's stream */
+hipError_t snapshot_copy(pfg_handle *h, Snapshot &S, bool save) {
+  const Markers &m = h->m;
+  auto cp = [&](void *hv, void *sv, size_t bytes) {
+    if (bytes == 0) return hipSuccess;
+    return save ? hipMemcpyAsync(sv, hv, bytes, hipMemcpyDeviceToDevice, h->stream)
+                : hipMemcpyAsync(hv, sv, bytes, hipMemcpyDeviceToDevice, h->stream);
+  };
+  const int cur = save ? h->cur : S.cur;
+  hipError_t e = cp(h->f[cur], S.f, S.f_bytes);
+  if (h->model_loaded) {
+    const size_t n = (size_t)m.n;
+    float *const arr[6] = {m.x, m.y, m.z, m.vx, m.vy, m.vz};
+    for (int k = 0; k < 6 && e == hipSuccess; ++k) e = cp(arr[k], S.mk + k * n, 4 * n);
+    const size_t nr = (size_t)m.n_rec, nl = (size_t)m.n_lig;
+    if (e == hipSuccess) e = cp(m.rec_bond, S.bonds, 4 * nr);
+    if (e == hipSuccess) e = cp(m.lig_owner, S.bonds + nr, 4 * nl);
+    if (e == hipSuccess) e = cp(m.lig_claim, S.bonds + nr + nl, 4 * nl);
+  }
+  if (e == hipSuccess) e = cp(h->m.counters, S.ctr, 8 * sizeof(int64_t));
+  if (e == hipSuccess) e = cp(h->m.detach_step, S.ctr + 8, sizeof(int64_t));
+  if (e == hipSuccess) e = cp(h->m.d_step, S.ctr + 9, sizeof(int64_t));
+  if (!save) {
+    /* no tile or row carries a stamp or a force of the steps being undone */
+    if (e == hipSuccess) e = hipMemsetAsync(h->tile_stamp, 0xff, 4 * (size_t)h->ntiles, h->stream);
+    if (e == hipSuccess && h->row_stamp) {
+      size_t rows = 0;
+      for (const Inst &I : h->inst) rows += (size_t)I.nxb * I.p.ny * I.p.nz;
+      e = hipMemsetAsync(h->row_stamp, 0xff, 4 * rows, h->stream);
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(h->aux, 0, sizeof(float) * 3 * (size_t)h->plane, h->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && save) {
+    S.cur = h->cur;
+    S.h_step = h->h_step;
+  }
+  if (e == hipSuccess && !save) {
+    h->cur = S.cur;
+    h->h_step = S.h_step;
+  }
+  return e;
+}
+
+hipError_t snapshot_take(pfg_handle *h, Snapshot &S) {
+  S.f_bytes = population_bytes(h);
+  const size_t n = h->model_loaded ? (size_t)h->m.n : 0;
+  const size_t nb = h->model_loaded ? (size_t)h->m.n_rec + 2 * (size_t)h->m.n_lig : 0;
+  hipError_t e = hipMalloc(&S.f, S.f_bytes);
+  if (e == hipSuccess) e = hipMalloc((void **)&S.mk, 6 * 4 * (n > 0 ? n : 1));
+  if (e == hipSuccess) e = hipMalloc((void **)&S.bonds, 4 * (nb > 0 ? nb : 1));
+  if (e == hipSuccess) e = hipMalloc((void **)&S.ctr, 10 * sizeof(int64_t));
+  if (e == hipSuccess) e = snapshot_copy(h, S, true);
+  return e;
+}
+
+struct Shape {
+  int cpt, lanes, staging, overlap;
+};
+
+void apply_shape(pfg_handle *h, const Shape &s) {
+  h->lb_cpt = s.cpt;
+  h->forces_lanes = s.lanes;
+  h->ib_staging = s.staging;
+  h->overlap = s.overlap != 0;
+  drop_graph(h);
+}
+
+/* ms per step of `steps` steps from the snapshot (after `warm` steps from the
+ * snapshot that build the graphs and warm the caches) */
+pfg_status time_shape(pfg_handle *h, Snapshot &S, int steps, int warm, double *ms) {
+  PFG_TRY(snapshot_copy(h, S, false));
+  if (warm > 0) {
+    pfg_status st = pfg_step(h, warm, nullptr);
+    if (st != PFG_OK) return st;
+    PFG_TRY(snapshot_copy(h, S, false));
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+  pfg_status st = PFG_OK;
+  if (e == hipSuccess) st = pfg_step(h, steps, nullptr);
+  if (e == hipSuccess && st == PFG_OK) e = hipEventRecord(e1, h->stream);
+  if (e == hipSuccess && st == PFG_OK) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess && st == PFG_OK) e = hipEventElapsedTime(&t, e0, e1);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st != PFG_OK) return st;
+  if (e != hipSuccess) return from_hip(e);
+  *ms = (double)t / steps;
+  return PFG_OK;
+}
+
+}  // namespace
+
+pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
+  if (!h || steps <= 0 || !out) return PFG_ERR_ARG;
+  if (h->halt_on_detach) return PFG_ERR_STATE; /* a detach inside a trial ends it early */
+  hipSetDevice(h->device);
+  const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
+  const Shape start{h->lb_cpt, h->forces_lanes, h->ib_staging, h->overlap ? 1 : 0};
+  Snapshot S;
+  hipError_t e = snapshot_take(h, S);
+  if (e != hipSuccess) {
+    snapshot_free(S);
+    return from_hip(e);
+  }
+  /* knobs, tried one at a time from the automatic shape, each kept at its
+   * fastest value before the next is tried (values 0 are the automatic
+   * choices); a value replaces the best only when it is 1 % faster */
+  const int cpts[3] = {1, 2, 4}, lanes[4] = {64, 16, 4, 1}, stag[2] = {1, 2}, ovl[2] = {0, 1};
+  Shape best{1, 0, 0, 0};
+  const int warm = GRAPH_CHUNK;
+  pfg_status st = PFG_OK;
+  double best_ms = 0.0;
+  int tried = 0;
+  apply_shape(h, best);
+  st = time_shape(h, S, steps, warm, &best_ms);
+  out->ms_default = best_ms;
+  tried = 1;
+  for (int knob = 0; knob < 4 && st == PFG_OK; ++knob) {
+    const int *vals = knob == 0 ? cpts : knob == 1 ? lanes : knob == 2 ? stag : ovl;
+    const int nv = knob == 0 ? 3 : knob == 1 ? 4 : 2;
+    if (knob > 0 && !markers) break;
+    if (knob == 0 && h->K > 1) continue; /* the batched LB launch takes one row per wave */
+    for (int v = 0; v < nv && st == PFG_OK; ++v) {
+      Shape s = best;
+      int *slot = knob == 0 ? &s.cpt : knob == 1 ? &s.lanes : knob == 2 ? &s.staging : &s.overlap;
+      if (*slot == vals[v]) continue;
+      *slot = vals[v];
+      apply_shape(h, s);
+      double ms = 0.0;
+      st = time_shape(h, S, steps, warm, &ms);
+      ++tried;
+      if (st == PFG_OK && ms < 0.99 * best_ms) {
+        best = s;
+        best_ms = ms;
+      }
+    }
+  }
+  if (st == PFG_OK) {
+    apply_shape(h, best);
+    e = snapshot_copy(h, S, false);
+    if (e != hipSuccess) st = from_hip(e);
+  } else {
+    apply_shape(h, start);
+    snapshot_copy(h, S, false);
+  }
+  snapshot_free(S);
+  if (st != PFG_OK) return st;
+  out->lb_cells_per_lane = best.cpt;
+  out->forces_lanes = best.lanes;
+  out->ib_staging = best.staging;
+  out->overlap = best.overlap;
+  out->ms_per_step = best_ms;
+  out->shapes_timed = tried;
+  return PFG_OK;
+}
+
 pfg_status pfg_time_lb(pfg_handle *h, int nlaunch, double *ms_per_launch) {
   if (!h || nlaunch <= 0 || !ms_per_launch) return PFG_ERR_ARG;
   hipSetDevice(h->device);

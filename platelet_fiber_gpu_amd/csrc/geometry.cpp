@@ -649,6 +649,7 @@ size_t pfg_struct_size(int which) {
     case 1: return sizeof(pfg_model);
     case 2: return sizeof(pfg_step_info);
     case 3: return sizeof(pfg_scene_opts);
+    case 4: return sizeof(pfg_tuning);
   }
   return 0;
 }

@@ -38,6 +38,7 @@ def test_struct_sizes_match(pfg):
     assert lib.pfg_struct_size(1) == C.sizeof(_ffi.Model)
     assert lib.pfg_struct_size(2) == C.sizeof(_ffi.StepInfo)
     assert lib.pfg_struct_size(3) == C.sizeof(_ffi.SceneOpts)
+    assert lib.pfg_struct_size(4) == C.sizeof(_ffi.Tuning)
     assert lib.pfg_struct_size(99) == 0
 
 

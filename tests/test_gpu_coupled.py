@@ -19,7 +19,7 @@ def run_both(pfg, po, sc, steps, graph=True, staging=0):
         bonds = s.bonds()[0] if s.n_rec else None
     o = po.Oracle(sc.params, scene=sc)
     o.step(steps)
-    return dict(info=info, f=f, mk=mk, u=(ux, uy, uz), bonds=bonds), o
+    return dict(info=info, f=f, mk=mk, rho=rho, u=(ux, uy, uz), bonds=bonds), o
 
 
 def rel_l2(a, b):
@@ -147,6 +147,86 @@ def test_thrombus_fibrin_from_file(gpu, po):
     err = np.abs(pos - opos).max()
     print(f"fibrin file: max position error {err:.2e}")
     assert err < 1e-4
+
+
+def test_large_physics_subbox_fp16(gpu, po):
+    """The large configuration's physics (Couette at the shear rate of 0.05
+    over 512 cells, tau 0.7, FP16 deviation storage, platelets, fibrin and
+    wall ligands at its densities) on a 64^3 sub-box, 100 steps against the
+    FP64 oracle: velocity within the FP16 tolerance 1e-3 (relative L2),
+    density, and platelet centroids.  The Morse force is tapered to zero at
+    its cutoff here: with the hard cutoff a pair that crosses it on rounding
+    switches its force on or off, and the FP32 run alone then differs from
+    the FP64 one by about 5e-4 in velocity, which would hide the storage
+    error this test measures (an FP16-rounded oracle run gives 4e-4 with the
+    hard cutoff)."""
+    pfg = gpu
+    n = 64
+    sc = pfg.Scene(pfg.SCENE_LARGE, nx=n, ny=n, nz=n, n_platelets=2, fibrin_nodes=80,
+                   fibrin_links=120, ligands=512)
+    assert sc.params.store == pfg.STORE_F16
+    sc.params.morse_taper = 0.4
+    uw = 0.05 * n / 512
+    sc.params.wall_u_hi[2][0] = uw
+    sc.u0[0][:] = uw * (np.arange(sc.N) // (n * n) + 0.5) / n
+    g, o = run_both(pfg, po, sc, 100)
+    orho, ox, oy, oz = po.fields(po.lattice(sc.params), o.f())
+    err_u = rel_l2(np.stack(g["u"]), np.stack([ox, oy, oz]))
+    err_rho = np.abs(g["rho"] - orho).max()
+    body = sc.body
+    pos = np.stack([g["mk"][k] for k in ("x", "y", "z")], 1)
+    opos = np.stack([o.marker(k) for k in ("x", "y", "z")], 1)
+    x0 = np.stack([sc.x, sc.y, sc.z], 1).astype(np.float64)
+    dcen, disp = [], []
+    for b in range(2):
+        sel = body == b
+        dcen.append(np.linalg.norm(pos[sel].mean(0) - opos[sel].mean(0)))
+        disp.append(np.linalg.norm(opos[sel].mean(0) - x0[sel].mean(0)))
+    print(f"64^3 large-physics sub-box FP16, 100 steps: rel L2 u {err_u:.2e}, max |drho| "
+          f"{err_rho:.2e}, centroid errors {dcen} (displacements {disp})")
+    assert err_u < 1e-3
+    assert err_rho < 2e-4
+    assert max(dcen) < 1e-2 * max(max(disp), 0.1)
+
+
+def test_autotune_restores_the_state(gpu):
+    """Coupled shear scene with bonds: the autotuner times every shape from a
+    device snapshot and restores it, so the run continues from where it was
+    (same step index, bonds and trajectory as an untuned run, up to the FP32
+    summation order that the chosen shapes may change)."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_SHEAR, nx=64, ny=32, nz=32, elements=256, receptors=128,
+                   ligands=800)
+    sc.params.k_on = 0.2
+    with pfg.Solver(sc.params) as s:
+        s.load(sc)
+        ref_info = s.step(20)
+        ref = s.markers()
+        ref_bonds = s.bonds()[0]
+        _, ref_u, _, _ = s.fields()
+    with pfg.Solver(sc.params) as s:
+        s.load(sc)
+        s.step(10)
+        before = s.markers()
+        t = s.autotune(8)
+        print(f"autotune: {t}")
+        assert t["shapes_timed"] >= 7
+        assert t["forces_lanes"] in (0, 1, 4, 16, 64) and t["ib_staging"] in (0, 1, 2)
+        after = s.markers()
+        for k in ("x", "y", "z"):
+            assert np.array_equal(after[k], before[k])
+        info = s.step(10)
+        assert info["step"] == 20
+        got = s.markers()
+        _, u, _, _ = s.fields()
+        mism = int((s.bonds()[0] != ref_bonds).sum())
+    # a bond decision within epsilon of its threshold may flip with the
+    # summation order; those are counted
+    assert mism <= max(info["near_threshold"], ref_info["near_threshold"])
+    if mism == 0:
+        err = max(np.abs(got[k] - ref[k]).max() for k in ("x", "y", "z"))
+        assert err < 1e-4
+        assert np.abs(u - ref_u).max() < 1e-6
 
 
 @pytest.mark.parametrize("graph", [True, False])

@@ -172,6 +172,25 @@ def test_launch_shapes_agree(gpu, cpt, store):
         assert np.array_equal(s.f(), ref)
 
 
+def test_autotune_leaves_the_lattice_unchanged(gpu):
+    """The autotuner's trial steps are undone: after it, the lattice steps on
+    exactly as an untuned one (cells per lane change only the launch)."""
+    pfg = gpu
+    p = make_params(pfg, 96, 20, 18, (0, 1, 1), g=(1e-5, 0, 0))
+    with pfg.Solver(p) as s:
+        s.set_fluid_only(True)
+        s.step(5)
+        ref = s.f()
+    with pfg.Solver(p) as s:
+        s.set_fluid_only(True)
+        s.step(2)
+        t = s.autotune(8)
+        assert t["lb_cells_per_lane"] in (1, 2, 4)
+        assert t["shapes_timed"] == 3 and t["ms_per_step"] > 0
+        assert s.step(3)["step"] == 5
+        assert np.array_equal(s.f(), ref)
+
+
 @pytest.mark.parametrize("store,tol", [(0, 2e-6), (1, 2e-4)])
 def test_velocity_faces_multistep(gpu, po, store, tol):
     """Inflow/outflow channel (bounce-back in y, periodic x), 300 steps from
