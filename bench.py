@@ -48,8 +48,17 @@ def make_solver(pfg, scene, store, fluid_only=False):
     return s
 
 
-def timed_run(pfg, scene, store, steps, warmup, fluid_only=False):
+TUNING = {}
+
+
+def timed_run(pfg, scene, store, steps, warmup, fluid_only=False, tune=None):
+    """Warm-up steps, then `steps` timed steps.  With `tune` (a key), the
+    launch shapes are picked first by the library's autotuner, outside the
+    timed region, and its choice is kept in TUNING[tune]."""
     s = make_solver(pfg, scene, store, fluid_only)
+    if tune is not None:
+        TUNING[tune] = s.autotune(32)
+        log(f"autotune {tune}: {TUNING[tune]}")
     s.step(warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -244,7 +253,7 @@ def other_workload(pfg, name, K, W, cpu_leg=True):
     log(f"{name}: scene built in {time.perf_counter() - t0:.1f} s: {scene.N} cells, "
         f"{scene.n_markers} markers")
     store = scene.params.store
-    s, dt = timed_run(pfg, scene, store, K, W)
+    s, dt = timed_run(pfg, scene, store, K, W, tune=name)
     info = s.step(0)
     dev_gb = s.device_bytes() / 1e9
     s.close()
@@ -274,6 +283,7 @@ def other_workload(pfg, name, K, W, cpu_leg=True):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": f"lb_fused_kernel<{dtype.upper()}>",
                      "bytes_per_cell": bpc, "lb_kernel_us": round(lb_us, 2)},
+        "tuning": TUNING.get(name),
         "accuracy_vs_fp64": acc,
         "cpu_baseline": cpu,
     }
@@ -322,6 +332,8 @@ def batch_workload(pfg, K, W, store):
         f"(built in {time.perf_counter() - t0:.1f} s)")
     b = pfg.Batch([sc.params for sc in scenes])
     b.load_scenes(scenes)
+    TUNING["batch"] = b.autotune(16)
+    log(f"autotune batch: {TUNING['batch']}")
     b.step(W)
     b.synchronize()
     t0 = time.perf_counter()
@@ -358,6 +370,7 @@ def batch_workload(pfg, K, W, store):
         "separate_handles_ms_per_step": round(dt_sep * 1e3 / K, 4),
         "speedup_vs_separate": round(dt_sep / dt_batch, 2),
         "device_gb": round(dev_gb, 2),
+        "tuning": TUNING.get("batch"),
         "roofline": None, "cpu_baseline": None,
     }
 
@@ -416,7 +429,7 @@ def main():
 
     log(f"workload 256x128x128 shear, {scene.n_markers} elements, "
         f"{scene.rec_tri.size} receptors, {scene.lig_x.size} ligands; K={K} W={W}")
-    s32, dt32 = timed_run(pfg, scene, pfg.STORE_F32, K, W)
+    s32, dt32 = timed_run(pfg, scene, pfg.STORE_F32, K, W, tune="f32")
     info = s32.step(0)
     s32.close()
     ms_step = dt32 * 1e3 / K
@@ -433,7 +446,7 @@ def main():
     lb32 = N * K / dtl / 1e6
     res16 = {}
     if not args.no_fp16:
-        s16, dt16 = timed_run(pfg, scene, pfg.STORE_F16, K, W)
+        s16, dt16 = timed_run(pfg, scene, pfg.STORE_F16, K, W, tune="f16")
         s16.close()
         sl16, dtl16 = timed_run(pfg, scene, pfg.STORE_F16, K, W, fluid_only=True)
         sl16.close()
@@ -483,6 +496,7 @@ def main():
                      "lb_kernel_us": round(lb32_us, 2),
                      "mlups_kernel": round(N / (lb32_us * 1e-6) / 1e6, 1)},
         "fp16": res16,
+        "tuning": TUNING,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

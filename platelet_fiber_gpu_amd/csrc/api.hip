@@ -102,7 +102,7 @@ void free_markers(Markers &m) {
   dfree(m.m2l_start); dfree(m.m2l_nbr); dfree(m.m2l_l0);
   dfree(m.bin_count); dfree(m.bin_start); dfree(m.bin_items);
   dfree(m.marker_bin); dfree(m.marker_slot); dfree(m.scan_partials);
-  dfree(m.chunk_start); dfree(m.bin_scatter); dfree(m.sorted_pos);
+  dfree(m.chunk_start); dfree(m.chunk_items); dfree(m.bin_scatter); dfree(m.sorted_pos);
   Markers fresh;
   fresh.counters = m.counters;
   fresh.detach_step = m.detach_step;
@@ -649,19 +649,81 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   m.n_lig = m.n_wall_lig + (m.fibrin_lig ? n : 0);
   m.n_links = M.n_links;
 
-  /* ---- IB chunks: runs of consecutive markers of one body; the length
-   * (16..IB_CHUNK, a power of two) aims at ~2048 chunks so that a single
-   * platelet still spreads over many workgroups ---- */
+  /* ---- IB chunks.  Markers are grouped by (lattice, body) over runs of
+   * consecutive indices; each group is cut by recursive bisection of its
+   * starting positions (along the longest extent, at a multiple of the
+   * chunk length) into compact chunks, so that a chunk covers a patch of a
+   * membrane (or a cluster of fibrin nodes) and its stencils a small box.
+   * Chunks whose box already exceeds IB_BOX_CELLS nodes at load (markers
+   * far apart, as sparse fibrin nodes are) are listed for direct spreading
+   * instead.  The chunk length (16..IB_CHUNK, a power of two) aims at ~2048
+   * chunks so that a single platelet still spreads over many workgroups ---- */
   int chunk_len = 16;
   while (chunk_len < IB_CHUNK && (int64_t)chunk_len * 2 * 2048 <= n) chunk_len *= 2;
-  std::vector<int32_t> chunk_start;
-  for (int i = 0; i < n; ++i) {
-    if (chunk_start.empty() || i - chunk_start.back() == chunk_len || M.body[i] != M.body[i - 1] ||
-        (minst && minst[i] != minst[i - 1]))
-      chunk_start.push_back(i);
+  std::vector<int32_t> chunk_start, chunk_items, direct_items;
+  {
+    const float *P[3] = {M.x, M.y, M.z};
+    std::vector<int32_t> idx((size_t)n);
+    for (int i = 0; i < n; ++i) idx[(size_t)i] = i;
+    auto leaf = [&](int32_t *b, int cnt) {
+      std::sort(b, b + cnt);
+      int64_t vol = 1;
+      for (int k = 0; k < 3; ++k) {
+        float lo = P[k][b[0]], hi = lo;
+        for (int j = 1; j < cnt; ++j) {
+          lo = std::min(lo, P[k][b[j]]);
+          hi = std::max(hi, P[k][b[j]]);
+        }
+        vol *= (int64_t)(std::floor(hi) - std::floor(lo)) + 4;
+      }
+      if (vol <= IB_BOX_CELLS) {
+        chunk_start.push_back((int32_t)chunk_items.size());
+        chunk_items.insert(chunk_items.end(), b, b + cnt);
+      } else {
+        direct_items.insert(direct_items.end(), b, b + cnt);
+      }
+    };
+    std::vector<std::pair<int32_t *, int>> todo;
+    for (int g0 = 0; g0 < n;) {
+      int g1 = g0 + 1;
+      while (g1 < n && M.body[g1] == M.body[g0] && !(minst && minst[g1] != minst[g0])) ++g1;
+      todo.push_back({idx.data() + g0, g1 - g0});
+      while (!todo.empty()) {
+        int32_t *b = todo.back().first;
+        const int cnt = todo.back().second;
+        todo.pop_back();
+        if (cnt <= chunk_len) {
+          leaf(b, cnt);
+          continue;
+        }
+        int axis = 0;
+        float ext = -1.f;
+        for (int k = 0; k < 3; ++k) {
+          float lo = P[k][b[0]], hi = lo;
+          for (int j = 1; j < cnt; ++j) {
+            lo = std::min(lo, P[k][b[j]]);
+            hi = std::max(hi, P[k][b[j]]);
+          }
+          if (hi - lo > ext) {
+            ext = hi - lo;
+            axis = k;
+          }
+        }
+        const int left = ((cnt + chunk_len - 1) / chunk_len / 2) * chunk_len;
+        const float *c = P[axis];
+        std::nth_element(b, b + left, b + cnt, [c](int32_t u, int32_t v) {
+          return c[u] < c[v] || (c[u] == c[v] && u < v);
+        });
+        todo.push_back({b + left, cnt - left}); /* the left half is cut first */
+        todo.push_back({b, left});
+      }
+      g0 = g1;
+    }
   }
-  chunk_start.push_back(n);
-  m.n_chunks = (int)chunk_start.size() - 1;
+  m.n_chunks = (int)chunk_start.size();
+  m.n_direct = (int)direct_items.size();
+  chunk_start.push_back((int32_t)chunk_items.size());
+  chunk_items.insert(chunk_items.end(), direct_items.begin(), direct_items.end());
 
   /* ---- host-side CSR construction ---- */
   std::vector<std::pair<int, int>> kv;
@@ -924,6 +986,7 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   ALLOC_FILL(m.marker_slot, 0, n);
   ALLOC_FILL(m.scan_partials, 0, m.scan_blocks);
   ALLOC_UP(m.chunk_start, chunk_start.data(), chunk_start.size());
+  ALLOC_UP(m.chunk_items, chunk_items.data(), chunk_items.size());
   if (minst) ALLOC_UP(m.minst, minst, n);
   ALLOC_FILL(m.bin_scatter, 0, n);
   ALLOC_FILL(m.sorted_pos, make_float4(0.f, 0.f, 0.f, 0.f), n);

@@ -1475,12 +1475,12 @@ __global__ __launch_bounds__(256) void contact_force_kernel(MArgs a) {
 
 /* Spreading has two forms.  Direct: one wavefront per marker, a lane per
  * stencil node, FP32 atomics into the lattice force field.  Staged: a
- * workgroup per chunk (a run of consecutive markers of one platelet)
- * accumulates the chunk's stencils in an LDS box of lattice nodes and then
- * adds the box's nonzero cells to the global field; a platelet's stencils
- * overlap, so this cuts the device-wide atomics several times over when many
- * markers share the lattice.  A chunk whose box does not fit in LDS (markers
- * scattered through the lattice) is spread directly. */
+ * workgroup per chunk (a run of consecutive markers of one platelet) stages
+ * the chunk's stencils in LDS, sums them node by node over the box of nodes
+ * they cover, and adds the box's nonzero nodes to the global field; a
+ * platelet's stencils overlap, so this cuts the device-wide atomics several
+ * times over when many markers share the lattice.  A chunk whose box is
+ * large (markers scattered through the lattice) is spread directly. */
 
 /* Stamp the tile of node (x, y, z) with this step: the LB update reads the
  * force of stamped tiles only. */
@@ -1517,14 +1517,6 @@ __global__ __launch_bounds__(256) void spread_kernel(MArgs a) {
   spread_direct(a, i, threadIdx.x & 63);
 }
 
-struct ChunkBox {
-  int lo[3];  /* first node, in the chunk's unwrapped node space */
-  int dim[3];
-  int vol;    /* 0: no IB-coupled marker in the chunk */
-  bool lds;   /* box fits in LDS */
-  float amax; /* bound of |force| x |force scale| over the chunk */
-};
-
 /* Wrapped coordinate (gives the kernel weights exactly as the direct form)
  * and the first stencil node in the periodic image nearest to the chunk's
  * reference point `ref` (the wrapped position of its first marker), so that
@@ -1546,86 +1538,46 @@ __device__ inline float stencil_weight(float W, int k) {
   return 0.25f * (1.0f + cospif(0.5f * r));
 }
 
-/* Block-wide box of the stencils whose first nodes o[] the threads with
- * `has` hold, and the largest of the threads' `amax`; every thread gets the
- * result. */
-__device__ ChunkBox chunk_box(bool has, const int o[3], float amax, int *sh) {
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-  if (has) {
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = o[k];
-      hi[k] = o[k] + 3;
-    }
-  }
-#pragma unroll
-  for (int s = 32; s > 0; s >>= 1) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = min(lo[k], __shfl_xor(lo[k], s, 64));
-      hi[k] = max(hi[k], __shfl_xor(hi[k], s, 64));
-    }
-    amax = fmaxf(amax, __shfl_xor(amax, s, 64));
-  }
-  if (lane == 0) {
-    for (int k = 0; k < 3; ++k) {
-      sh[6 * wave + k] = lo[k];
-      sh[6 * wave + 3 + k] = hi[k];
-    }
-    sh[24 + wave] = __float_as_int(amax);
-  }
-  __syncthreads();
-  ChunkBox b;
-  b.amax = fmaxf(fmaxf(__int_as_float(sh[24]), __int_as_float(sh[25])),
-                 fmaxf(__int_as_float(sh[26]), __int_as_float(sh[27])));
-  int64_t vol = 1;
-  for (int k = 0; k < 3; ++k) {
-    int l = INT_MAX, h = INT_MIN;
-    for (int w = 0; w < 4; ++w) {
-      l = min(l, sh[6 * w + k]);
-      h = max(h, sh[6 * w + 3 + k]);
-    }
-    b.lo[k] = l;
-    b.dim[k] = l <= h ? h - l + 1 : 0;
-    vol *= b.dim[k];
-  }
-  __syncthreads(); /* sh is free again */
-  b.vol = vol > 0 ? (vol > INT_MAX ? INT_MAX : (int)vol) : 0;
-  b.lds = b.vol > 0 && vol <= IB_BOX_CELLS;
-  return b;
-}
-
-/* One workgroup per chunk; dynamic LDS of 3 * IB_BOX_CELLS int64 sums.
- * Thread t first takes marker t of the chunk: its first stencil node, its
- * 3 x 4 kernel weights and its force go to LDS, so that the accumulation
- * loop reads no global memory.  The box sums in fixed point: each FP32
- * contribution w f is scaled by a power of two 2^q chosen from the chunk's
- * force bound so that every sum fits in 63 bits, converted to an integer
- * (exact above 2^24, rounded below) and added with an integer LDS atomic
- * (the FP32 LDS atomic runs at a small fraction of its rate); the flush
- * converts each sum back with one rounding.  The box is thus independent of
- * the order of the additions. */
-__global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
-  static_assert(IB_CHUNK <= 128, "one thread per marker; sums below 2^62 need <= 128 markers");
-  extern __shared__ unsigned long long box[];
-  __shared__ int sh[28];
-  __shared__ int so[IB_CHUNK][3];    /* first stencil node; INT_MIN: not coupled */
+/* One wavefront per chunk (at most IB_CHUNK markers of one body).
+ * The wavefront first stages the chunk in LDS: per marker its first
+ * stencil node, its 3 x 4 kernel weights and its force.  When the chunk's
+ * stencils span at most IB_BOX_CELLS nodes, it then adds the markers one
+ * after the other into an FP32 box of those nodes in LDS, lane l taking
+ * node (l & 3, (l >> 2) & 3, l >> 4) of the marker's 4 x 4 x 4 stencil: the
+ * 64 nodes of one marker are distinct, and a wavefront's LDS accesses are
+ * carried out in the order it issues them, so the read-add-write of each
+ * node needs no atomic and the sums follow marker order whatever the
+ * timing.  The box is then added to the lattice field with one device-wide
+ * atomic per nonzero node and component.  A scattered chunk (fibrin nodes
+ * far apart) adds each marker's 64 nodes with device-wide atomics, as the
+ * direct form does.  Contributions are the direct form's bit for bit; only
+ * the order in which they are added differs. */
+__global__ __launch_bounds__(64) void spread_chunk_kernel(MArgs a) {
+  static_assert(IB_CHUNK <= 128, "two markers per lane in the staging pass");
+  __shared__ int4 so[IB_CHUNK];      /* first stencil node; x = INT_MIN: not coupled */
   __shared__ float sw[IB_CHUNK][12]; /* weights along x (0..3), y (4..7), z (8..11) */
   __shared__ float sf[IB_CHUNK][3];
+  __shared__ float box[3 * IB_BOX_CELLS];
   if (halted(a)) return;
-  const int m0 = a.m.chunk_start[blockIdx.x], cnt = a.m.chunk_start[blockIdx.x + 1] - m0;
+  if ((int)blockIdx.x >= a.m.n_chunks) { /* the markers listed for direct spreading */
+    const int k = a.m.chunk_start[a.m.n_chunks] + (int)blockIdx.x - a.m.n_chunks;
+    spread_direct(a, a.m.chunk_items[k], threadIdx.x);
+    return;
+  }
+  const int c0 = a.m.chunk_start[blockIdx.x], cnt = a.m.chunk_start[blockIdx.x + 1] - c0;
+  const int *items = a.m.chunk_items + c0;
+  const int m0 = items[0];
   const MLat L = lat_of(a, m0); /* a chunk never spans two lattices */
   float ref[3];
   ref[0] = wrap_coord(a.m.x[m0], L.nx, L.periodic[0]);
   ref[1] = wrap_coord(a.m.y[m0], L.ny, L.periodic[1]);
   ref[2] = wrap_coord(a.m.z[m0], L.nz, L.periodic[2]);
-  const int t = threadIdx.x;
-  bool has = false;
-  int o[3] = {0, 0, 0};
-  float amax = 0.f;
-  if (t < cnt) {
-    const int i = m0 + t;
-    has = ib_coupled(a.m.kind[i], a.fiber_move);
+  const int lane = threadIdx.x;
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  for (int t = lane; t < cnt; t += 64) {
+    const int i = items[t];
+    const bool has = ib_coupled(a.m.kind[i], a.fiber_move);
+    int o[3] = {0, 0, 0};
     if (has) {
       float W[3];
       stencil_origin(a, L, i, ref, W, o);
@@ -1638,24 +1590,34 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
       sf[t][0] = a.m.fx[i];
       sf[t][1] = a.m.fy[i];
       sf[t][2] = a.m.fz[i];
-      amax = fmaxf(fmaxf(fabsf(sf[t][0]), fabsf(sf[t][1])), fabsf(sf[t][2])) *
-             fabsf(a.force_scale);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = min(lo[k], o[k]);
+        hi[k] = max(hi[k], o[k] + 3);
+      }
     }
-    so[t][0] = has ? o[0] : INT_MIN;
-    so[t][1] = o[1];
-    so[t][2] = o[2];
+    so[t] = make_int4(has ? o[0] : INT_MIN, o[1], o[2], 0);
   }
-  const ChunkBox b = chunk_box(has, o, amax, sh); /* its barriers publish so, sw, sf */
-  if (b.vol == 0) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int s = 32; s > 0; s >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = min(lo[k], __shfl_xor(lo[k], s, 64));
+      hi[k] = max(hi[k], __shfl_xor(hi[k], s, 64));
+    }
+  }
+  if (lo[0] > hi[0]) return; /* no IB-coupled marker in the chunk */
+  const int dim0 = hi[0] - lo[0] + 1, dim1 = hi[1] - lo[1] + 1, dim2 = hi[2] - lo[2] + 1;
+  const int64_t vol64 = (int64_t)dim0 * dim1 * dim2;
+  const int32_t step = (int32_t)*a.m.d_step;
   const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
-  if (!b.lds) { /* scattered markers: device-wide atomics per node */
-    const int32_t step = (int32_t)*a.m.d_step;
-    for (int j = wave; j < cnt; j += 4) {
-      if (so[j][0] == INT_MIN) continue;
-      const int ix = axis_index(so[j][0] + kx, L.nx, L.periodic[0]);
-      const int iy = axis_index(so[j][1] + ky, L.ny, L.periodic[1]);
-      const int iz = axis_index(so[j][2] + kz, L.nz, L.periodic[2]);
+  __syncthreads(); /* the staged chunk is visible to every lane */
+  if (vol64 > IB_BOX_CELLS) { /* scattered markers: device-wide atomics per node */
+    for (int j = 0; j < cnt; ++j) {
+      if (so[j].x == INT_MIN) continue;
+      const int ix = axis_index(so[j].x + kx, L.nx, L.periodic[0]);
+      const int iy = axis_index(so[j].y + ky, L.ny, L.periodic[1]);
+      const int iz = axis_index(so[j].z + kz, L.nz, L.periodic[2]);
       if (ix < 0 || iy < 0 || iz < 0) continue;
       flag_tile(L, ix, iy, iz, step);
       const float w = sw[j][kx] * sw[j][4 + ky] * sw[j][8 + kz] * a.force_scale;
@@ -1666,45 +1628,48 @@ __global__ __launch_bounds__(256) void spread_chunk_kernel(MArgs a) {
     }
     return;
   }
-  for (int k = threadIdx.x; k < 3 * b.vol; k += 256) box[k] = 0ull;
-  /* |w f| <= amax / 8 < 2^(e - 3) with amax < 2^e, so with q = 58 - e a
-   * contribution stays below 2^55 and a sum of <= 128 below 2^62 */
-  int e;
-  frexpf(b.amax, &e);
-  const int q = 58 - e;
+  const int vol = (int)vol64;
+  for (int k = lane; k < 3 * vol; k += 64) box[k] = 0.f;
+  /* each marker's first box node (-1: not coupled), and each lane's offset
+   * of its stencil node from it */
+  for (int t = lane; t < cnt; t += 64) {
+    const int4 oj = so[t];
+    so[t].w = oj.x == INT_MIN ? -1 : (oj.x - lo[0]) + dim0 * ((oj.y - lo[1]) + dim1 * (oj.z - lo[2]));
+  }
+  const int off = kx + dim0 * (ky + dim1 * kz);
   __syncthreads();
-  for (int j = wave; j < cnt; j += 4) {
-    if (so[j][0] == INT_MIN) continue;
+#pragma unroll 4
+  for (int j = 0; j < cnt; ++j) {
+    const int base = so[j].w;
+    if (base < 0) continue;
     const float w = sw[j][kx] * sw[j][4 + ky] * sw[j][8 + kz] * a.force_scale;
-    const int cell = (so[j][0] + kx - b.lo[0]) +
-                     b.dim[0] * ((so[j][1] + ky - b.lo[1]) + b.dim[1] * (so[j][2] + kz - b.lo[2]));
-    atomicAdd(&box[cell], (unsigned long long)__float2ll_rn(ldexpf(w * sf[j][0], q)));
-    atomicAdd(&box[b.vol + cell], (unsigned long long)__float2ll_rn(ldexpf(w * sf[j][1], q)));
-    atomicAdd(&box[2 * b.vol + cell], (unsigned long long)__float2ll_rn(ldexpf(w * sf[j][2], q)));
+    const int c = base + off;
+    box[c] += w * sf[j][0];
+    box[vol + c] += w * sf[j][1];
+    box[2 * vol + c] += w * sf[j][2];
   }
   __syncthreads();
-  /* flush row by row: the (y, z) of a row is wavefront-uniform */
-  const int32_t step = (int32_t)*a.m.d_step;
-  const int rows = b.dim[1] * b.dim[2];
-  for (int r = wave; r < rows; r += 4) {
-    const int by = r % b.dim[1], bz = r / b.dim[1];
-    const int gy = axis_index(b.lo[1] + by, L.ny, L.periodic[1]);
-    const int gz = axis_index(b.lo[2] + bz, L.nz, L.periodic[2]);
-    if (gy < 0 || gz < 0) continue;
-    const int64_t row = (int64_t)L.nx * (gy + (int64_t)L.ny * gz);
-    for (int bx = lane; bx < b.dim[0]; bx += 64) {
-      const int gx = axis_index(b.lo[0] + bx, L.nx, L.periodic[0]);
-      if (gx < 0) continue;
-      flag_tile(L, gx, gy, gz, step);
-      const int k = bx + b.dim[0] * r;
-      const float fx = ldexpf((float)(long long)box[k], -q);
-      const float fy = ldexpf((float)(long long)box[b.vol + k], -q);
-      const float fz = ldexpf((float)(long long)box[2 * b.vol + k], -q);
-      const int64_t c = row + gx;
-      if (fx != 0.f) atomicAdd(L.aux + c, fx);
-      if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
-      if (fz != 0.f) atomicAdd(L.aux + 2 * a.plane + c, fz);
-    }
+  /* Every tile the box meets is stamped (a superset of the stencils' tiles:
+   * interpolation reads the velocity of all their nodes).  Along each axis
+   * the box nodes inside one tile form runs that start at the box's first
+   * node or at a tile boundary (a wrap lands on node 0, and a wall-bounded
+   * axis resumes at node 0), so the corner nodes of those runs reach every
+   * tile while only a few lanes read and write the stamps. */
+  for (int k = lane; k < vol; k += 64) {
+    const int r = k / dim0, bx = k - r * dim0;
+    const int bz = r / dim1, by = r - bz * dim1;
+    const int gx = axis_index(lo[0] + bx, L.nx, L.periodic[0]);
+    const int gy = axis_index(lo[1] + by, L.ny, L.periodic[1]);
+    const int gz = axis_index(lo[2] + bz, L.nz, L.periodic[2]);
+    if (gx < 0 || gy < 0 || gz < 0) continue;
+    if ((bx == 0 || (gx & (TILE_X - 1)) == 0) && (by == 0 || (gy & (TILE_Y - 1)) == 0) &&
+        (bz == 0 || (gz & (TILE_Z - 1)) == 0)) /* few per chunk: stored without a read */
+      L.tile_stamp[(gx >> TILE_SHIFT_X) + L.tnx * ((gy >> TILE_SHIFT_Y) + L.tny * (gz >> TILE_SHIFT_Z))] = step;
+    const int64_t c = gx + (int64_t)L.nx * (gy + (int64_t)L.ny * gz);
+    const float fx = box[k], fy = box[vol + k], fz = box[2 * vol + k];
+    if (fx != 0.f) atomicAdd(L.aux + c, fx);
+    if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
+    if (fz != 0.f) atomicAdd(L.aux + 2 * a.plane + c, fz);
   }
 }
 
@@ -1983,11 +1948,14 @@ hipError_t launch_spread(pfg_handle *h) {
   make_margs(h, &a);
   /* staged through LDS when many markers share the lattice */
   const bool staged = h->ib_staging == 1 || (h->ib_staging == 0 && h->m.n >= IB_STAGED_MIN);
-  if (staged)
-    hipLaunchKernelGGL(spread_chunk_kernel, dim3(h->m.n_chunks), dim3(256), IB_BOX_BYTES,
-                       h->stream, a);
-  else
+  if (!staged) {
     hipLaunchKernelGGL(spread_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
+    return hipGetLastError();
+  }
+  /* one workgroup per chunk, then one per marker listed for direct
+   * spreading, in a single launch */
+  hipLaunchKernelGGL(spread_chunk_kernel, dim3(h->m.n_chunks + h->m.n_direct), dim3(64), 0,
+                     h->stream, a);
   return hipGetLastError();
 }
 

@@ -20,13 +20,12 @@ namespace pfg {
  * started from). */
 constexpr int GRAPH_CHUNK = 8;
 
-/* IB work units: at most IB_CHUNK consecutive markers of one body; their
- * stencils are staged in an LDS box of at most IB_BOX_CELLS lattice nodes
- * (3 fixed-point int64 sums each). */
+/* IB work units: at most IB_CHUNK consecutive markers of one body, staged
+ * in LDS; when their stencils span at most IB_BOX_CELLS lattice nodes the
+ * chunk is summed node by node. */
 constexpr int SCAN_SPAN = 8192; /* bins per block of the bin-count scan */
 constexpr int IB_CHUNK = 128;
-constexpr int IB_BOX_CELLS = 2048;
-constexpr size_t IB_BOX_BYTES = 3 * IB_BOX_CELLS * sizeof(long long);
+constexpr int IB_BOX_CELLS = 1024;
 /* Marker count from which spreading is staged through LDS by default. */
 constexpr int IB_STAGED_MIN = 32768;
 /* Marker count from which the forces take 4 lanes per marker in bin order
@@ -157,11 +156,14 @@ struct Markers {
   float4 *sorted_pos = nullptr;   /* per bin_items slot: x, y, z, Morse tag  */
   int32_t *scan_partials = nullptr; /* sums of SCAN_SPAN bins */
   int64_t scan_blocks = 0;
-  /* IB chunks: chunk k holds the markers chunk_start[k] .. chunk_start[k+1]-1,
-   * at most IB_CHUNK consecutive markers of one body (a platelet, or the
-   * markers outside platelets) */
-  int n_chunks = 0;
+  /* IB chunks: chunk k holds the markers chunk_items[chunk_start[k] ..
+   * chunk_start[k+1]-1], at most IB_CHUNK markers of one body (a platelet,
+   * or the markers outside platelets) of one lattice that lay close together
+   * at load time; the n_direct markers chunk_items[chunk_start[n_chunks] ..]
+   * were too far apart and are spread directly */
+  int n_chunks = 0, n_direct = 0;
   int32_t *chunk_start = nullptr;
+  int32_t *chunk_items = nullptr;
   /* platelet-platelet contact: per platelet pair (lo * n_plt + hi) the best
    * proposal (distance bits << 32 | element of the lower platelet), and the
    * pairs proposed this step */
