@@ -49,6 +49,7 @@ def make_solver(pfg, scene, store, fluid_only=False):
 
 
 TUNING = {}
+NO_TUNE = False  # --no-tune: keep the automatic launch shapes (profiling runs)
 
 
 def timed_run(pfg, scene, store, steps, warmup, fluid_only=False, tune=None):
@@ -56,7 +57,7 @@ def timed_run(pfg, scene, store, steps, warmup, fluid_only=False, tune=None):
     launch shapes are picked first by the library's autotuner, outside the
     timed region, and its choice is kept in TUNING[tune]."""
     s = make_solver(pfg, scene, store, fluid_only)
-    if tune is not None:
+    if tune is not None and not NO_TUNE:
         TUNING[tune] = s.autotune(32)
         log(f"autotune {tune}: {TUNING[tune]}")
     s.step(warmup)
@@ -332,8 +333,9 @@ def batch_workload(pfg, K, W, store):
         f"(built in {time.perf_counter() - t0:.1f} s)")
     b = pfg.Batch([sc.params for sc in scenes])
     b.load_scenes(scenes)
-    TUNING["batch"] = b.autotune(16)
-    log(f"autotune batch: {TUNING['batch']}")
+    if not NO_TUNE:
+        TUNING["batch"] = b.autotune(16)
+        log(f"autotune batch: {TUNING['batch']}")
     b.step(W)
     b.synchronize()
     t0 = time.perf_counter()
@@ -389,7 +391,11 @@ def main():
                     default="shear", help="shear (default) is the headline configuration")
     ap.add_argument("--store", choices=sorted(STORES), default="f32",
                     help="storage of the batch workload")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-shape autotuner (keeps the automatic shapes)")
     args = ap.parse_args()
+    global NO_TUNE
+    NO_TUNE = args.no_tune
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)

@@ -1382,8 +1382,9 @@ static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
     if (e == hipSuccess) e = launch_place(h, true, true);
     if (e == hipSuccess) e = launch_gather(h, true, true, true);
     if (e == hipSuccess) e = launch_resolve(h);
-    if (e == hipSuccess) e = launch_forces(h);
-    if (e == hipSuccess) e = launch_spread(h);
+    const bool fused = forces_can_spread(h);
+    if (e == hipSuccess) e = launch_forces(h, fused);
+    if (e == hipSuccess && !fused) e = launch_spread(h);
   }
   if (overlap) {
     if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->ev_join, 0);

@@ -93,6 +93,8 @@ constexpr float BOND_EPS_R = 1e-4f; /* |r - reach| below this: near-threshold  *
  * launch, so every lane reads the same value). */
 __device__ inline bool halted(const MArgs &a) { return a.halt && a.m.counters[1] != 0; }
 
+__device__ void spread_direct(const MArgs &a, int i, int lane, float Fx, float Fy, float Fz);
+
 __device__ inline int cell_of(float v, int n, int periodic) {
   int c = (int)floorf(v);
   if (periodic) {
@@ -1228,8 +1230,9 @@ __device__ inline float group_sum(float v) {
  * positions, each group striding through the ranges).  The lanes of a group
  * then stride over the marker's CSR lists (triangles, edges, receptor
  * shares, fibrin links) and a butterfly reduction sums them. */
-template <int G>
+template <int G, bool SPREAD = false>
 __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
+  static_assert(!SPREAD || G == 64, "spreading takes a lane per stencil node");
   const int lane = threadIdx.x & (G - 1);
   if (blockIdx.x == 0 && threadIdx.x == 0 && a.m.n_rec > 0 && a.detach_rate > 0.f) {
     /* detach test (semLB.cu:451-455): no bond held after this step's update */
@@ -1352,6 +1355,9 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
     a.m.fy[i] = fy;
     a.m.fz[i] = fz;
   }
+  /* the wavefront holds the marker's whole force: spread it at once, a
+   * lane per stencil node (the direct form, without another launch) */
+  if constexpr (SPREAD) spread_direct(a, i, lane, fx, fy, fz);
 }
 
 /* ---------------- platelet-platelet contact ---------------- */
@@ -1491,7 +1497,7 @@ __device__ inline void flag_tile(const MLat &L, int x, int y, int z, int32_t ste
 
 /* Direct spreading of marker i: lane = stencil node (consecutive lanes take
  * consecutive x, so one atomic instruction covers 16 runs of 4 nodes). */
-__device__ void spread_direct(const MArgs &a, int i, int lane) {
+__device__ void spread_direct(const MArgs &a, int i, int lane, float Fx, float Fy, float Fz) {
   if (!ib_coupled(a.m.kind[i], a.fiber_move)) return;
   const MLat L = lat_of(a, i);
   const float X = wrap_coord(a.m.x[i], L.nx, L.periodic[0]);
@@ -1505,9 +1511,13 @@ __device__ void spread_direct(const MArgs &a, int i, int lane) {
   flag_tile(L, ix, iy, iz, (int32_t)*a.m.d_step);
   const float w = wx * wy * wz * a.force_scale;
   const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
-  atomicAdd(L.aux + c, w * a.m.fx[i]);
-  atomicAdd(L.aux + a.plane + c, w * a.m.fy[i]);
-  atomicAdd(L.aux + 2 * a.plane + c, w * a.m.fz[i]);
+  atomicAdd(L.aux + c, w * Fx);
+  atomicAdd(L.aux + a.plane + c, w * Fy);
+  atomicAdd(L.aux + 2 * a.plane + c, w * Fz);
+}
+
+__device__ inline void spread_direct(const MArgs &a, int i, int lane) {
+  spread_direct(a, i, lane, a.m.fx[i], a.m.fy[i], a.m.fz[i]);
 }
 
 /* One wavefront per marker. */
@@ -1917,14 +1927,39 @@ hipError_t launch_resolve(pfg_handle *h) {
   return hipGetLastError();
 }
 
-hipError_t launch_forces(pfg_handle *h) {
+static bool staged_spreading(const pfg_handle *h) {
+  return h->ib_staging == 1 || (h->ib_staging == 0 && h->m.n >= IB_STAGED_MIN);
+}
+
+static int forces_lanes(const pfg_handle *h) {
+  /* few markers: a wavefront each, for parallelism; many: 4 lanes each in
+   * bin order, for fewer instructions per marker */
+  return h->forces_lanes ? h->forces_lanes : (h->m.n >= FORCES_SORTED_MIN ? 4 : 64);
+}
+
+static bool contact_on(const pfg_handle *h) {
+  return h->m.contact_slot && h->p.contact_a != 0.f && h->m.n_plt >= 2;
+}
+
+/* The force pass can spread each marker's force itself when it has a
+ * wavefront per marker, spreading is direct, no later pass adds to the
+ * forces (bending, contact) and no detach halt can stop the step midway. */
+bool forces_can_spread(const pfg_handle *h) {
+  return h->m.n > 0 && forces_lanes(h) == 64 && !staged_spreading(h) && !h->m.bend &&
+         !contact_on(h) && !h->halt_on_detach;
+}
+
+hipError_t launch_forces(pfg_handle *h, bool spread) {
   if (h->m.n == 0) return hipSuccess;
   MArgs a;
   make_margs(h, &a);
-  /* few markers: a wavefront each, for parallelism; many: 4 lanes each in
-   * bin order, for fewer instructions per marker */
-  const int G = h->forces_lanes ? h->forces_lanes : (h->m.n >= FORCES_SORTED_MIN ? 4 : 64);
+  const int G = forces_lanes(h);
   const unsigned grid = (unsigned)blocks_for((int64_t)h->m.n * G, 256);
+  if (spread) {
+    if (!forces_can_spread(h)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((forces_kernel<64, true>), dim3(grid), dim3(256), 0, h->stream, a);
+    return hipGetLastError();
+  }
   switch (G) {
     case 1: hipLaunchKernelGGL(forces_kernel<1>, dim3(grid), dim3(256), 0, h->stream, a); break;
     case 4: hipLaunchKernelGGL(forces_kernel<4>, dim3(grid), dim3(256), 0, h->stream, a); break;
@@ -1934,7 +1969,7 @@ hipError_t launch_forces(pfg_handle *h) {
   if (h->m.bend)
     hipLaunchKernelGGL(bend_force_kernel, dim3(blocks_for((int64_t)h->m.n * 4, 256)),
                        dim3(256), 0, h->stream, a);
-  if (h->m.contact_slot && h->p.contact_a != 0.f && h->m.n_plt >= 2) {
+  if (contact_on(h)) {
     hipLaunchKernelGGL(contact_pairs_kernel, dim3(blocks_for(h->m.n, 256)), dim3(256), 0,
                        h->stream, a);
     hipLaunchKernelGGL(contact_force_kernel, dim3(64), dim3(256), 0, h->stream, a);
@@ -1947,7 +1982,7 @@ hipError_t launch_spread(pfg_handle *h) {
   MArgs a;
   make_margs(h, &a);
   /* staged through LDS when many markers share the lattice */
-  const bool staged = h->ib_staging == 1 || (h->ib_staging == 0 && h->m.n >= IB_STAGED_MIN);
+  const bool staged = staged_spreading(h);
   if (!staged) {
     hipLaunchKernelGGL(spread_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
     return hipGetLastError();

@@ -273,7 +273,9 @@ hipError_t launch_scan(pfg_handle *h);
 hipError_t launch_place(pfg_handle *h, bool scatter, bool geometry);
 hipError_t launch_gather(pfg_handle *h, bool sort, bool sums, bool decide);
 hipError_t launch_resolve(pfg_handle *h);
-hipError_t launch_forces(pfg_handle *h);
+/* spread = true: the force pass also spreads (only when forces_can_spread) */
+hipError_t launch_forces(pfg_handle *h, bool spread = false);
+bool forces_can_spread(const pfg_handle *h);
 hipError_t launch_spread(pfg_handle *h);
 hipError_t launch_move(pfg_handle *h, bool interp, bool update);
 

@@ -189,6 +189,27 @@ def test_large_physics_subbox_fp16(gpu, po):
     assert max(dcen) < 1e-2 * max(max(disp), 0.1)
 
 
+def test_forces_with_spreading_equals_separate_passes(gpu):
+    """With a wavefront per marker and direct spreading the force pass also
+    spreads; halting at detach turns that off (the test never fires here:
+    detach rate 0).  Same steps both ways, up to the FP32 order of the
+    force-field atomics."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_SHEAR, nx=64, ny=32, nz=32, elements=256, receptors=64,
+                   ligands=400)
+    sc.params.detach_rate = 0.0
+    out = []
+    for halt in (False, True):
+        with pfg.Solver(sc.params) as s:
+            s.load(sc)
+            s.set_halt_on_detach(halt)
+            s.step(20)
+            mk = s.markers()
+            out.append((np.stack([mk[k] for k in ("x", "y", "z")]), s.fields()[1]))
+    assert np.abs(out[0][0] - out[1][0]).max() < 1e-5
+    assert np.abs(out[0][1] - out[1][1]).max() < 1e-6
+
+
 def test_autotune_restores_the_state(gpu):
     """Coupled shear scene with bonds: the autotuner times every shape from a
     device snapshot and restores it, so the run continues from where it was
