@@ -342,7 +342,7 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
     }
     if (st != PFG_OK) break;
     if (dalloc(h, &h->aux, (size_t)3 * h->plane) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
-    if (dalloc(h, &h->vel, (size_t)3 * h->plane) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (dalloc(h, &h->vel, (size_t)h->plane) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     /* row stamps start at -1: no row is touched before the first step */
     if (dalloc(h, &h->row_stamp, (size_t)fblk) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->row_stamp, 0xff, sizeof(int32_t) * (size_t)fblk) != hipSuccess) {
@@ -350,7 +350,7 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
       break;
     }
     if (hipMemset(h->aux, 0, sizeof(float) * 3 * h->plane) != hipSuccess ||
-        hipMemset(h->vel, 0, sizeof(float) * 3 * h->plane) != hipSuccess) {
+        hipMemset(h->vel, 0, sizeof(float4) * h->plane) != hipSuccess) {
       st = PFG_ERR_DEVICE;
       break;
     }

@@ -309,9 +309,7 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
       continue;
     }
     if (flagged) { /* velocity for the IB interpolation of this step */
-      a.vel[idx] = ux;
-      a.vel[a.plane + idx] = uy;
-      a.vel[2 * a.plane + idx] = uz;
+      a.vel[idx] = make_float4(ux, uy, uz, 0.f);
     }
 
     /* BGK with Guo forcing, in deviation form and storage units:
@@ -501,6 +499,7 @@ void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a) {
   a->fin = (const char *)h->f[cur] + foff;
   a->fout = (char *)h->f[1 - cur] + foff;
   a->aux = h->aux + L.cell0;
+  a->plane = h->plane;
   a->vel = h->vel + L.cell0;
   a->tile_stamp = h->tile_stamp + L.tile0;
   a->solid = h->solid ? h->solid + L.cell0 : nullptr;
@@ -512,7 +511,6 @@ void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a) {
   a->ny = p.ny;
   a->nz = p.nz;
   a->nxb = L.nxb;
-  a->plane = h->plane;
   a->block0 = L.lb_block0;
   a->tnx = L.tnx;
   a->tny = L.tny;

@@ -50,12 +50,12 @@ namespace pfg {
 struct MArgs {
   Markers m;
   /* lattice side of the IB coupling */
-  int64_t plane;          /* stride between the components of the field      */
   MLat lat;               /* the lattice of a single-lattice handle           */
   const MLat *lats;       /* batches: [K], selected by the marker's lattice   */
   const int64_t *tile0s;  /* batches: first global tile per lattice, [K+1]    */
   int K;
   int64_t ntiles;         /* tiles over all lattices                          */
+  int64_t plane;          /* stride between the components of the force field */
   /* parameters */
   float morse_u0, morse_rho, morse_req, morse_cut2, morse_cut, morse_taper;
   float k_spring, ks, kv;
@@ -1707,9 +1707,10 @@ __device__ void interp_marker(const MArgs &a, int i, int j, float *v) {
       if (iz < 0) continue;
       const float w = wxy * wz;
       const int64_t c = col + layer * iz;
-      ux += w * L.vel[c];
-      uy += w * L.vel[a.plane + c];
-      uz += w * L.vel[2 * a.plane + c];
+      const float4 v = L.vel[c];
+      ux += w * v.x;
+      uy += w * v.y;
+      uz += w * v.z;
     }
   }
 #pragma unroll
@@ -1806,7 +1807,6 @@ void fill_mlat(const pfg_handle *h, int k, MLat *L) {
 static void make_margs(const pfg_handle *h, MArgs *a) {
   const pfg_params &p = h->p;
   a->m = h->m;
-  a->plane = h->plane;
   fill_mlat(h, 0, &a->lat);
   a->lats = h->K > 1 ? h->d_lats : nullptr;
   a->tile0s = h->K > 1 ? h->d_tile0 : nullptr;
@@ -1841,6 +1841,7 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->fiber_gamma = p.fiber_gamma;
   a->marker_dt = p.marker_dt;
   a->force_scale = p.ib_force_scale;
+  a->plane = h->plane;
   a->kbend = p.kbend;
   a->contact_a = p.contact_a;
   a->contact_kappa = p.contact_kappa;

@@ -37,7 +37,8 @@ struct LBArgs {
   const void *fin;        /* post-collision state of the previous step      */
   void *fout;             /* post-collision state of this step              */
   float *aux;             /* [3][plane]: IB force in, zeroed once read      */
-  float *vel;             /* [3][plane]: IB velocity out                    */
+  float4 *vel;            /* per cell (ux, uy, uz, 0): IB velocity out       */
+  int64_t plane;          /* stride between the components of aux            */
   const int32_t *tile_stamp; /* per tile: the last step that spread into it */
   const uint8_t *solid;   /* nullptr when there are no solid cells          */
   const uint32_t *bbmask; /* per cell: bit d = upstream of d is solid,
@@ -49,7 +50,6 @@ struct LBArgs {
   const int64_t *step;    /* the current step (device counter)              */
   int nx, ny, nz;
   int nxb;                /* x-blocks of XBLOCK cells per row                 */
-  int64_t plane;          /* stride between the components of aux            */
   int64_t block0;         /* first workgroup of this lattice in a batched
                              launch                                          */
   int tnx, tny;           /* tiles per axis                                   */
@@ -73,7 +73,7 @@ struct LBArgs {
  * with one lattice has one). */
 struct MLat {
   float *aux;             /* the lattice's cells of the [3][plane] force      */
-  const float *vel;       /* the lattice's cells of the [3][plane] velocity   */
+  const float4 *vel;      /* the lattice's cells of the velocity field        */
   int32_t *tile_stamp;    /* the lattice's tiles                              */
   int tnx, tny, tnz;
   int nx, ny, nz;
@@ -207,9 +207,12 @@ struct pfg_handle {
   /* IB coupling fields: spreading adds the force of a step into aux and
    * stamps the 16x4x4 tiles it touches with the step; the LB update reads
    * the force of stamped tiles, zeroes it there for the next step and
-   * writes the velocity the interpolation reads into vel */
+   * writes the velocity the interpolation reads into vel.  The force is
+   * kept as three planes, so the three atomic adds of a node go to three
+   * cache lines (atomics to one line are serialised); the velocity is one
+   * float4 per cell, so a gather of a node reads one line */
   float *aux = nullptr;
-  float *vel = nullptr;
+  float4 *vel = nullptr;
   int32_t *tile_stamp = nullptr;
   int tnx = 0, tny = 0, tnz = 0;
   int64_t ntiles = 0;
