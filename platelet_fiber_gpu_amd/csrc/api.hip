@@ -105,6 +105,7 @@ void free_markers(Markers &m) {
   dfree(m.chunk_start); dfree(m.chunk_items); dfree(m.bin_scatter); dfree(m.sorted_pos);
   Markers fresh;
   fresh.counters = m.counters;
+  fresh.ticket = m.ticket;
   fresh.detach_step = m.detach_step;
   fresh.d_step = m.d_step;
   m = fresh;
@@ -372,6 +373,8 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
     h->lb_cpt = 1;
     if (dalloc(h, &h->m.counters, 8) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->m.counters, 0, sizeof(int64_t) * 8) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    if (dalloc(h, &h->m.ticket, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (hipMemset(h->m.ticket, 0, sizeof(int32_t)) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     if (dalloc(h, &h->m.detach_step, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (dalloc(h, &h->m.d_step, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     const int64_t minus1 = -1, zero = 0;
@@ -493,6 +496,7 @@ pfg_status pfg_destroy(pfg_handle *h) {
   drop_graph(h);
   free_markers(h->m);
   dfree(h->m.counters);
+  dfree(h->m.ticket);
   dfree(h->m.detach_step);
   dfree(h->m.d_step);
   for (int k = 0; k < 2; ++k) {
@@ -1380,8 +1384,9 @@ static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
     }
     if (e == hipSuccess) e = launch_scan(h);
     if (e == hipSuccess) e = launch_place(h, true, true);
-    if (e == hipSuccess) e = launch_gather(h, true, true, true);
-    if (e == hipSuccess) e = launch_resolve(h);
+    bool resolved = false;
+    if (e == hipSuccess) e = launch_gather(h, true, true, true, &resolved);
+    if (e == hipSuccess && !resolved) e = launch_resolve(h);
     const bool fused = forces_can_spread(h);
     if (e == hipSuccess) e = launch_forces(h, fused);
     if (e == hipSuccess && !fused) e = launch_spread(h);

@@ -74,6 +74,7 @@ struct MArgs {
   int halt; /* 1: after the detach test fires, leave bonds, force field and
                markers unchanged (the step loop ends at detach) */
   int mark_rows; /* 1: stamp the x-block rows this step's IB stencils touch */
+  int resolve_last; /* 1: the gather launch's last workgroup resolves the bonds */
   int blocks0, blocks1, blocks2, blocks3; /* blocks of role ranges 0..3 */
 };
 
@@ -339,6 +340,20 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
   const int b = L.bin0 + b3[0] + L.nb[0] * (b3[1] + L.nb[1] * b3[2]);
   a.m.marker_bin[i] = b;
   a.m.marker_slot[i] = atomicAdd(&a.m.bin_count[b], 1);
+  {
+    /* the scan's per-span totals, one atomic per span a wavefront touches
+     * (its markers are consecutive, so mostly one); the scan then needs no
+     * reduction pass, and the place kernel clears them for the next step */
+    const int span = b / SCAN_SPAN, lane = threadIdx.x & 63;
+    uint64_t todo = __ballot(1);
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const int ls = __shfl(span, leader, 64);
+      const uint64_t same = __ballot(span == ls) & todo;
+      if (lane == leader) atomicAdd(&a.m.scan_partials[ls], (int)__popcll(same));
+      todo &= ~same;
+    }
+  }
   if (a.mark_rows && ib_coupled(a.m.kind[i], a.fiber_move) && !halted(a)) {
     /* stamp the rows of this step's 4x4x4 stencil (the nodes spreading and
      * interpolation use), so that the LB update of every other row may run
@@ -378,20 +393,6 @@ __device__ inline int block_sum(int v, int *sh_wave) {
   const int t = sh_wave[0] + sh_wave[1] + sh_wave[2] + sh_wave[3];
   __syncthreads();
   return t;
-}
-
-__global__ __launch_bounds__(256) void scan_reduce_kernel(const int32_t *count,
-                                                          int32_t *partials) {
-  __shared__ int sh[4];
-  const int4 *c = reinterpret_cast<const int4 *>(count + (int64_t)blockIdx.x * SCAN_SPAN);
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_SPAN / 1024; ++k) {
-    const int4 v = c[k * 256 + threadIdx.x];
-    s += v.x + v.y + v.z + v.w;
-  }
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 /* Each lane scans 32 consecutive bins of the block in registers; one block
@@ -721,6 +722,9 @@ __global__ __launch_bounds__(256) void place_kernel(MArgs a) {
         const int b = a.m.marker_bin[i];
         a.m.bin_scatter[a.m.bin_start[b] + a.m.marker_slot[i]] = i;
       }
+      /* the scan has read the span totals: clear them for the next count */
+      for (int64_t k = i; k < a.m.scan_blocks; k += (int64_t)a.blocks0 * 256)
+        a.m.scan_partials[k] = 0;
       return;
     }
     blk -= a.blocks0;
@@ -1037,7 +1041,7 @@ __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
  * packed position there), B = platelet sums (blocks1 = one block per platelet),
  * C = bond decisions (blocks2, four receptors per block; they read the
  * scatter order, which no role of this launch writes) */
-__global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
+__device__ void gather_body(const MArgs &a) {
   int blk = blockIdx.x;
   if (a.roles & ROLE_A) {
     if (blk < a.blocks0) {
@@ -1075,9 +1079,8 @@ __global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
 /* Apply ruptures and the lowest-index claim per ligand, then the bond force
  * k (|d| - l0) d/|d| on the receptor and its inverse-distance split over the
  * three vertices (sem_kernel.cu:2039-2065). */
-__global__ void bond_resolve_kernel(MArgs a) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.m.n_rec || halted(a)) return;
+__device__ void resolve_one(const MArgs &a, int r) {
+  if (halted(a)) return;
   int b = a.m.rec_bond[r];
   if (a.m.rec_release[r]) {
     a.m.lig_owner[b] = -1;
@@ -1127,6 +1130,32 @@ __global__ void bond_resolve_kernel(MArgs a) {
   a.m.rec_w[3 * r] = w0;
   a.m.rec_w[3 * r + 1] = w1;
   a.m.rec_w[3 * r + 2] = w2;
+}
+
+__global__ void bond_resolve_kernel(MArgs a) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < a.m.n_rec) resolve_one(a, r);
+}
+
+/* Cell-list sort, platelet sums and bond decisions; with resolve_last the
+ * workgroup that finishes last also resolves the bonds (few receptors: one
+ * launch fewer).  Each workgroup publishes its writes with a device-scope
+ * fence before it takes its ticket; the last one fences again before it
+ * reads them, and clears the ticket for the next launch. */
+__global__ __launch_bounds__(256) void gather_kernel(MArgs a) {
+  gather_body(a);
+  if (!a.resolve_last) return;
+  __shared__ int last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(a.m.ticket, 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int r = threadIdx.x; r < a.m.n_rec; r += 256) resolve_one(a, r);
+  if (threadIdx.x == 0) *a.m.ticket = 0;
 }
 
 /* ---------------- K8: forces ---------------- */
@@ -1875,8 +1904,6 @@ hipError_t launch_begin(pfg_handle *h, bool clear, bool count) {
 hipError_t launch_scan(pfg_handle *h) {
   const Markers &m = h->m;
   if (m.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
-                     h->stream, m.bin_count, m.scan_partials);
   hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
                      h->stream, m.bin_count, m.nbins, m.scan_partials, m.bin_start);
   return hipGetLastError();
@@ -1903,13 +1930,16 @@ hipError_t launch_place(pfg_handle *h, bool scatter, bool geometry) {
   return hipGetLastError();
 }
 
-hipError_t launch_gather(pfg_handle *h, bool sort, bool sums, bool decide) {
+hipError_t launch_gather(pfg_handle *h, bool sort, bool sums, bool decide, bool *resolved) {
   MArgs a;
   make_margs(h, &a);
   sort = sort && h->m.n > 0;
   sums = sums && h->m.n_plt > 0;
   decide = decide && h->m.n_rec > 0;
   a.roles = (sort ? ROLE_A : 0) | (sums ? ROLE_B : 0) | (decide ? ROLE_C : 0);
+  /* the bonds of a few receptors are resolved by the last workgroup */
+  a.resolve_last = resolved && decide && h->m.n_rec <= RESOLVE_IN_GATHER_MAX;
+  if (resolved) *resolved = a.resolve_last != 0;
   a.blocks0 = sort ? blocks_for(h->m.n, 256) : 0;
   a.blocks1 = sums ? h->m.n_plt : 0;
   a.blocks2 = decide ? blocks_for(h->m.n_rec, 4) : 0;

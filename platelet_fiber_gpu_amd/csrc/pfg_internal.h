@@ -31,6 +31,9 @@ constexpr int IB_STAGED_MIN = 32768;
 /* Marker count from which the forces take 4 lanes per marker in bin order
  * (below it, a wavefront per marker). */
 constexpr int FORCES_SORTED_MIN = 32768;
+/* Receptor count up to which the cell-list/bond launch resolves the bonds
+ * itself (its last workgroup takes them all). */
+constexpr int RESOLVE_IN_GATHER_MAX = 2048;
 
 /* Everything the fused LB kernel needs, passed by value. */
 struct LBArgs {
@@ -173,6 +176,7 @@ struct Markers {
    * [4] events lo, [5] events hi, [6] the step index recorded by the first
    * launch of the step, [7] platelet pairs in contact this step */
   int64_t *counters = nullptr;
+  int32_t *ticket = nullptr;      /* workgroups of the gather launch done       */
   int64_t *detach_step = nullptr;
   int64_t *d_step = nullptr;
 };
@@ -274,7 +278,10 @@ void fill_mlat(const pfg_handle *h, int k, MLat *L);
 hipError_t launch_begin(pfg_handle *h, bool clear, bool count);
 hipError_t launch_scan(pfg_handle *h);
 hipError_t launch_place(pfg_handle *h, bool scatter, bool geometry);
-hipError_t launch_gather(pfg_handle *h, bool sort, bool sums, bool decide);
+/* resolved != nullptr: resolve the bonds in the same launch when there are
+ * few receptors (*resolved says whether it did) */
+hipError_t launch_gather(pfg_handle *h, bool sort, bool sums, bool decide,
+                         bool *resolved = nullptr);
 hipError_t launch_resolve(pfg_handle *h);
 /* spread = true: the force pass also spreads (only when forces_can_spread) */
 hipError_t launch_forces(pfg_handle *h, bool spread = false);
