@@ -389,7 +389,8 @@ pfg_status pfg_time_lb(pfg_handle *h, int nlaunch, double *ms_per_launch);
 pfg_status pfg_copy_probe(int device, size_t bytes, int reps, double *gbs);
 
 /* LB-kernel launch tuning: lattice cells each lane updates along z (1, 2 or
- * 4; a wavefront updates one 64-cell x-block of that many rows). */
+ * 4; a wavefront updates one 64-cell x-block of that many rows).  Applies to
+ * batches too; the overlap mode of a batch runs with 1 only. */
 pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane);
 
 /* Launch shapes chosen by pfg_autotune. */

@@ -249,6 +249,29 @@ pfg_status check_lattice_params(const pfg_params &p) {
   return PFG_OK;
 }
 
+/* Batched LB launch layout for h->lb_cpt z-cells per lane: each lattice's
+ * workgroups (four wavefronts, one x-block row of lb_cpt z-cells each) and
+ * the workgroup -> lattice table. */
+hipError_t batch_layout(pfg_handle *h) {
+  int64_t blk = 0;
+  for (Inst &I : h->inst) {
+    const int64_t zrows = (I.p.nz + h->lb_cpt - 1) / h->lb_cpt;
+    I.lb_blocks = ((int64_t)I.nxb * I.p.ny * zrows + 3) / 4;
+    I.lb_block0 = blk;
+    blk += I.lb_blocks;
+  }
+  h->lb_blocks = blk;
+  std::vector<int32_t> table((size_t)blk);
+  for (int k = 0; k < h->K; ++k) {
+    const Inst &I = h->inst[(size_t)k];
+    for (int64_t b = 0; b < I.lb_blocks; ++b) table[(size_t)(I.lb_block0 + b)] = k;
+  }
+  dfree(h->lb_block_inst);
+  hipError_t e = dalloc(h, &h->lb_block_inst, table.size());
+  if (e == hipSuccess) e = upload(h, h->lb_block_inst, table.data(), table.size());
+  return e;
+}
+
 /* Batched LB arguments (both starting copies) and the workgroup table. */
 hipError_t upload_batch_args(pfg_handle *h) {
   if (h->K <= 1) return hipSuccess;
@@ -297,7 +320,7 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
   /* lattice k occupies cells [cell0, cell0 + N), x-blocks [fblk0, ...),
    * tiles [tile0, ...) and LB workgroups [lb_block0, ...) */
   h->inst.resize((size_t)K);
-  int64_t cell = 0, fblk = 0, tile = 0, blk = 0;
+  int64_t cell = 0, fblk = 0, tile = 0;
   for (int k = 0; k < K; ++k) {
     Inst &I = h->inst[(size_t)k];
     I.p = ps[k];
@@ -312,12 +335,9 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
     I.tnz = (I.p.nz + TILE_Z - 1) / TILE_Z;
     I.ntiles = (int64_t)I.tnx * I.tny * I.tnz;
     I.tile0 = tile;
-    I.lb_blocks = ((int64_t)I.nxb * I.p.ny * I.p.nz + 3) / 4;
-    I.lb_block0 = blk;
     cell += I.N;
     fblk += (int64_t)I.nxb * I.p.ny * I.p.nz;
     tile += I.ntiles;
-    blk += I.lb_blocks;
     flags = flags || I.p.use_flags;
   }
   h->p.use_flags = flags ? 1 : 0;
@@ -390,17 +410,10 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
       if (hipMemset(h->bbmask, 0, sizeof(uint32_t) * h->N) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     }
     if (K > 1) {
-      h->lb_blocks = blk;
-      std::vector<int32_t> table((size_t)blk);
       std::vector<int64_t> tile0((size_t)K + 1);
-      for (int k = 0; k < K; ++k) {
-        const Inst &I = h->inst[(size_t)k];
-        for (int64_t b = 0; b < I.lb_blocks; ++b) table[(size_t)(I.lb_block0 + b)] = k;
-        tile0[(size_t)k] = I.tile0;
-      }
+      for (int k = 0; k < K; ++k) tile0[(size_t)k] = h->inst[(size_t)k].tile0;
       tile0[(size_t)K] = tile;
-      hipError_t e = dalloc(h, &h->lb_block_inst, table.size());
-      if (e == hipSuccess) e = upload(h, h->lb_block_inst, table.data(), table.size());
+      hipError_t e = batch_layout(h);
       if (e == hipSuccess) e = dalloc(h, &h->d_tile0, tile0.size());
       if (e == hipSuccess) e = upload(h, h->d_tile0, tile0.data(), tile0.size());
       for (int c = 0; c < 2 && e == hipSuccess; ++c) e = dalloc(h, &h->d_lbargs[c], (size_t)K);
@@ -1373,7 +1386,9 @@ static hipError_t run_stage(pfg_handle *h, int s) {
 static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
   hipError_t e = hipSuccess;
   const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
-  const bool overlap = markers && h->overlap && h->row_stamp && !lb0 && !lb1;
+  /* the row-split LB passes take one z-row per wavefront */
+  const bool overlap = markers && h->overlap && h->row_stamp && !lb0 && !lb1 &&
+                       (h->K == 1 || h->lb_cpt == 1);
   if (markers) {
     e = launch_begin(h, false, true);
     if (overlap) {
@@ -1590,8 +1605,15 @@ pfg_status pfg_profile(pfg_handle *h, int nsteps, double *ms_total, double *ms_l
 pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane) {
   if (!h) return PFG_ERR_ARG;
   if (cells_per_lane != 1 && cells_per_lane != 2 && cells_per_lane != 4) return PFG_ERR_ARG;
+  if (cells_per_lane == h->lb_cpt) return PFG_OK;
+  hipSetDevice(h->device);
   h->lb_cpt = cells_per_lane;
   drop_graph(h);
+  if (h->K > 1) { /* the batched launch's workgroups depend on it */
+    PFG_TRY(hipStreamSynchronize(h->stream));
+    PFG_TRY(batch_layout(h));
+    PFG_TRY(upload_batch_args(h));
+  }
   return PFG_OK;
 }
 
@@ -1754,7 +1776,6 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
     const int *vals = knob == 0 ? cpts : knob == 1 ? lanes : knob == 2 ? stag : ovl;
     const int nv = knob == 0 ? 3 : knob == 1 ? 4 : 2;
     if (knob > 0 && !markers) break;
-    if (knob == 0 && h->K > 1) continue; /* the batched LB launch takes one row per wave */
     for (int v = 0; v < nv && st == PFG_OK; ++v) {
       Shape s = best;
       int *slot = knob == 0 ? &s.cpt : knob == 1 ? &s.lanes : knob == 2 ? &s.staging : &s.overlap;

@@ -351,12 +351,12 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
 
 /* A batch of lattices in one launch: workgroup -> lattice through a table,
  * each lattice's arguments read from device memory. */
-template <int S, int MODE>
+template <int S, int CPT, int MODE>
 __global__ __launch_bounds__(256) void lb_batch_kernel(const LBArgs *args,
                                                        const int32_t *block_inst) {
   const LBArgs &a = args[block_inst[blockIdx.x]];
-  lb_update<S, 1, MODE>(a, (uint32_t)(blockIdx.x - a.block0), nullptr, nullptr, nullptr,
-                        nullptr);
+  lb_update<S, CPT, MODE>(a, (uint32_t)(blockIdx.x - a.block0), nullptr, nullptr, nullptr,
+                          nullptr);
 }
 
 /* ---------------- auxiliary kernels ---------------- */
@@ -604,24 +604,32 @@ static hipError_t dispatch_lb(pfg_handle *h, int k, int mode, float *rho, float 
   }
 }
 
-template <int S, int MODE>
+template <int S, int CPT, int MODE>
 static void launch_batch(pfg_handle *h, hipStream_t st) {
   const dim3 grid((unsigned)h->lb_blocks), block(256);
   const LBArgs *args = h->d_lbargs[h->cur];
   if (h->p.wall[2] == PFG_WALL_VELOCITY) /* the lattices of a batch share the walls */
-    hipLaunchKernelGGL((lb_batch_kernel<S, MODE | LB_ZVEL>), grid, block, 0, st, args,
+    hipLaunchKernelGGL((lb_batch_kernel<S, CPT, MODE | LB_ZVEL>), grid, block, 0, st, args,
                        h->lb_block_inst);
   else
-    hipLaunchKernelGGL((lb_batch_kernel<S, MODE>), grid, block, 0, st, args,
+    hipLaunchKernelGGL((lb_batch_kernel<S, CPT, MODE>), grid, block, 0, st, args,
                        h->lb_block_inst);
 }
 
+/* The workgroup table follows h->lb_cpt; the row-split passes take one
+ * z-row per wavefront. */
 template <int S>
 static hipError_t launch_lb_batch_s(pfg_handle *h, int mode, hipStream_t st) {
+  if (mode != LB_MODE_STEP && h->lb_cpt != 1) return hipErrorInvalidValue;
   switch (mode) {
-    case LB_MODE_BULK: launch_batch<S, LB_MODE_BULK>(h, st); break;
-    case LB_MODE_REGION: launch_batch<S, LB_MODE_REGION>(h, st); break;
-    default: launch_batch<S, LB_MODE_STEP>(h, st);
+    case LB_MODE_BULK: launch_batch<S, 1, LB_MODE_BULK>(h, st); break;
+    case LB_MODE_REGION: launch_batch<S, 1, LB_MODE_REGION>(h, st); break;
+    default:
+      switch (h->lb_cpt) {
+        case 2: launch_batch<S, 2, LB_MODE_STEP>(h, st); break;
+        case 4: launch_batch<S, 4, LB_MODE_STEP>(h, st); break;
+        default: launch_batch<S, 1, LB_MODE_STEP>(h, st);
+      }
   }
   return hipGetLastError();
 }

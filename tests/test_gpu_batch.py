@@ -24,15 +24,18 @@ def fluid_set(pfg, store):
             lattice_params(pfg, 130, 9, 5, (P, B, B), g=(1e-5, 0, 0), tau=1.1, store=store)]
 
 
+@pytest.mark.parametrize("cpt", [1, 2, 4])
 @pytest.mark.parametrize("store", [0, 1])
-def test_batch_fluid_equals_single(gpu, store):
+def test_batch_fluid_equals_single(gpu, store, cpt):
     """Three lattices of different shapes, walls, tau and forcing: the batched
-    LB launch gives bitwise the fields of three separate handles."""
+    LB launch gives bitwise the fields of three separate handles, with one,
+    two or four z-cells per lane (the shape changes only the launch)."""
     pfg = gpu
     ps = fluid_set(pfg, store)
     steps = 25
     with pfg.Batch(ps) as b:
         b.set_fluid_only(True)
+        b.set_lb_tuning(cpt)
         b.step(steps)
         got = [b.fields_of(k) for k in range(len(ps))]
         cons = [b.conservation_of(k) for k in range(len(ps))]
