@@ -1577,30 +1577,34 @@ __device__ inline float stencil_weight(float W, int k) {
   return 0.25f * (1.0f + cospif(0.5f * r));
 }
 
-/* One wavefront per chunk (at most IB_CHUNK markers of one body).
- * The wavefront first stages the chunk in LDS: per marker its first
- * stencil node, its 3 x 4 kernel weights and its force.  When the chunk's
- * stencils span at most IB_BOX_CELLS nodes, it then adds the markers one
- * after the other into an FP32 box of those nodes in LDS, lane l taking
- * node (l & 3, (l >> 2) & 3, l >> 4) of the marker's 4 x 4 x 4 stencil: the
- * 64 nodes of one marker are distinct, and a wavefront's LDS accesses are
- * carried out in the order it issues them, so the read-add-write of each
- * node needs no atomic and the sums follow marker order whatever the
- * timing.  The box is then added to the lattice field with one device-wide
- * atomic per nonzero node and component.  A scattered chunk (fibrin nodes
- * far apart) adds each marker's 64 nodes with device-wide atomics, as the
- * direct form does.  Contributions are the direct form's bit for bit; only
+/* Two wavefronts per chunk (at most IB_CHUNK markers of one body).  The
+ * workgroup first stages the chunk in LDS: per marker its first stencil
+ * node, its 3 x 4 kernel weights and its force.  When the chunk's stencils
+ * span at most IB_BOX_CELLS nodes, each wavefront then adds its half of the
+ * markers, one after the other, into its own FP32 box of those nodes in
+ * LDS, lane l taking node (l & 3, (l >> 2) & 3, l >> 4) of the marker's
+ * 4 x 4 x 4 stencil: the 64 nodes of one marker are distinct, and a
+ * wavefront's LDS accesses are carried out in the order it issues them, so
+ * the read-add-write of each node needs no atomic and the sums follow
+ * marker order whatever the timing.  The two boxes are added (first half,
+ * then second) and go to the lattice field with one device-wide atomic per
+ * nonzero node and component.  A scattered chunk (fibrin nodes far apart)
+ * adds each marker's 64 nodes with device-wide atomics, as the direct form
+ * does; so do the markers listed for direct spreading, two per workgroup
+ * after the chunks.  Contributions are the direct form's bit for bit; only
  * the order in which they are added differs. */
-__global__ __launch_bounds__(64) void spread_chunk_kernel(MArgs a) {
-  static_assert(IB_CHUNK <= 128, "two markers per lane in the staging pass");
+__global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
+  static_assert(IB_CHUNK <= 128, "a thread per marker in the staging pass");
   __shared__ int4 so[IB_CHUNK];      /* first stencil node; x = INT_MIN: not coupled */
   __shared__ float sw[IB_CHUNK][12]; /* weights along x (0..3), y (4..7), z (8..11) */
   __shared__ float sf[IB_CHUNK][3];
-  __shared__ float box[3 * IB_BOX_CELLS];
+  __shared__ float box[2][3 * IB_BOX_CELLS];
+  __shared__ int sh[2][6];
   if (halted(a)) return;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   if ((int)blockIdx.x >= a.m.n_chunks) { /* the markers listed for direct spreading */
-    const int k = a.m.chunk_start[a.m.n_chunks] + (int)blockIdx.x - a.m.n_chunks;
-    spread_direct(a, a.m.chunk_items[k], threadIdx.x);
+    const int k = 2 * ((int)blockIdx.x - a.m.n_chunks) + wave;
+    if (k < a.m.n_direct) spread_direct(a, a.m.chunk_items[a.m.chunk_start[a.m.n_chunks] + k], lane);
     return;
   }
   const int c0 = a.m.chunk_start[blockIdx.x], cnt = a.m.chunk_start[blockIdx.x + 1] - c0;
@@ -1611,9 +1615,8 @@ __global__ __launch_bounds__(64) void spread_chunk_kernel(MArgs a) {
   ref[0] = wrap_coord(a.m.x[m0], L.nx, L.periodic[0]);
   ref[1] = wrap_coord(a.m.y[m0], L.ny, L.periodic[1]);
   ref[2] = wrap_coord(a.m.z[m0], L.nz, L.periodic[2]);
-  const int lane = threadIdx.x;
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-  for (int t = lane; t < cnt; t += 64) {
+  if (t < cnt) {
     const int i = items[t];
     const bool has = ib_coupled(a.m.kind[i], a.fiber_move);
     int o[3] = {0, 0, 0};
@@ -1631,28 +1634,40 @@ __global__ __launch_bounds__(64) void spread_chunk_kernel(MArgs a) {
       sf[t][2] = a.m.fz[i];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        lo[k] = min(lo[k], o[k]);
-        hi[k] = max(hi[k], o[k] + 3);
+        lo[k] = o[k];
+        hi[k] = o[k] + 3;
       }
     }
     so[t] = make_int4(has ? o[0] : INT_MIN, o[1], o[2], 0);
   }
 #pragma unroll
-  for (int s = 32; s > 0; s >>= 1) {
+  for (int s2 = 32; s2 > 0; s2 >>= 1) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      lo[k] = min(lo[k], __shfl_xor(lo[k], s, 64));
-      hi[k] = max(hi[k], __shfl_xor(hi[k], s, 64));
+      lo[k] = min(lo[k], __shfl_xor(lo[k], s2, 64));
+      hi[k] = max(hi[k], __shfl_xor(hi[k], s2, 64));
     }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      sh[wave][k] = lo[k];
+      sh[wave][3 + k] = hi[k];
+    }
+  }
+  __syncthreads(); /* the staged chunk and both halves' bounds */
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = min(sh[0][k], sh[1][k]);
+    hi[k] = max(sh[0][3 + k], sh[1][3 + k]);
   }
   if (lo[0] > hi[0]) return; /* no IB-coupled marker in the chunk */
   const int dim0 = hi[0] - lo[0] + 1, dim1 = hi[1] - lo[1] + 1, dim2 = hi[2] - lo[2] + 1;
   const int64_t vol64 = (int64_t)dim0 * dim1 * dim2;
   const int32_t step = (int32_t)*a.m.d_step;
   const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
-  __syncthreads(); /* the staged chunk is visible to every lane */
   if (vol64 > IB_BOX_CELLS) { /* scattered markers: device-wide atomics per node */
-    for (int j = 0; j < cnt; ++j) {
+    for (int j = wave; j < cnt; j += 2) {
       if (so[j].x == INT_MIN) continue;
       const int ix = axis_index(so[j].x + kx, L.nx, L.periodic[0]);
       const int iy = axis_index(so[j].y + ky, L.ny, L.periodic[1]);
@@ -1668,24 +1683,29 @@ __global__ __launch_bounds__(64) void spread_chunk_kernel(MArgs a) {
     return;
   }
   const int vol = (int)vol64;
-  for (int k = lane; k < 3 * vol; k += 64) box[k] = 0.f;
+  for (int k = t; k < 3 * vol; k += 128) {
+    box[0][k] = 0.f;
+    box[1][k] = 0.f;
+  }
   /* each marker's first box node (-1: not coupled), and each lane's offset
    * of its stencil node from it */
-  for (int t = lane; t < cnt; t += 64) {
+  if (t < cnt) {
     const int4 oj = so[t];
     so[t].w = oj.x == INT_MIN ? -1 : (oj.x - lo[0]) + dim0 * ((oj.y - lo[1]) + dim1 * (oj.z - lo[2]));
   }
   const int off = kx + dim0 * (ky + dim1 * kz);
   __syncthreads();
+  float *mybox = box[wave];
+  const int j1 = min(cnt, 64 * (wave + 1));
 #pragma unroll 4
-  for (int j = 0; j < cnt; ++j) {
+  for (int j = 64 * wave; j < j1; ++j) {
     const int base = so[j].w;
     if (base < 0) continue;
     const float w = sw[j][kx] * sw[j][4 + ky] * sw[j][8 + kz] * a.force_scale;
     const int c = base + off;
-    box[c] += w * sf[j][0];
-    box[vol + c] += w * sf[j][1];
-    box[2 * vol + c] += w * sf[j][2];
+    mybox[c] += w * sf[j][0];
+    mybox[vol + c] += w * sf[j][1];
+    mybox[2 * vol + c] += w * sf[j][2];
   }
   __syncthreads();
   /* Every tile the box meets is stamped (a superset of the stencils' tiles:
@@ -1694,7 +1714,7 @@ __global__ __launch_bounds__(64) void spread_chunk_kernel(MArgs a) {
    * node or at a tile boundary (a wrap lands on node 0, and a wall-bounded
    * axis resumes at node 0), so the corner nodes of those runs reach every
    * tile while only a few lanes read and write the stamps. */
-  for (int k = lane; k < vol; k += 64) {
+  for (int k = t; k < vol; k += 128) {
     const int r = k / dim0, bx = k - r * dim0;
     const int bz = r / dim1, by = r - bz * dim1;
     const int gx = axis_index(lo[0] + bx, L.nx, L.periodic[0]);
@@ -1705,7 +1725,8 @@ __global__ __launch_bounds__(64) void spread_chunk_kernel(MArgs a) {
         (bz == 0 || (gz & (TILE_Z - 1)) == 0)) /* few per chunk: stored without a read */
       L.tile_stamp[(gx >> TILE_SHIFT_X) + L.tnx * ((gy >> TILE_SHIFT_Y) + L.tny * (gz >> TILE_SHIFT_Z))] = step;
     const int64_t c = gx + (int64_t)L.nx * (gy + (int64_t)L.ny * gz);
-    const float fx = box[k], fy = box[vol + k], fz = box[2 * vol + k];
+    const float fx = box[0][k] + box[1][k], fy = box[0][vol + k] + box[1][vol + k],
+                fz = box[0][2 * vol + k] + box[1][2 * vol + k];
     if (fx != 0.f) atomicAdd(L.aux + c, fx);
     if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
     if (fz != 0.f) atomicAdd(L.aux + 2 * a.plane + c, fz);
@@ -2018,10 +2039,10 @@ hipError_t launch_spread(pfg_handle *h) {
     hipLaunchKernelGGL(spread_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
     return hipGetLastError();
   }
-  /* one workgroup per chunk, then one per marker listed for direct
+  /* one workgroup per chunk, then one per two markers listed for direct
    * spreading, in a single launch */
-  hipLaunchKernelGGL(spread_chunk_kernel, dim3(h->m.n_chunks + h->m.n_direct), dim3(64), 0,
-                     h->stream, a);
+  hipLaunchKernelGGL(spread_chunk_kernel, dim3(h->m.n_chunks + (h->m.n_direct + 1) / 2),
+                     dim3(128), 0, h->stream, a);
   return hipGetLastError();
 }
 
