@@ -1710,12 +1710,18 @@ struct Shape {
   int cpt, lanes, staging, overlap;
 };
 
-void apply_shape(pfg_handle *h, const Shape &s) {
-  h->lb_cpt = s.cpt;
+hipError_t apply_shape(pfg_handle *h, const Shape &s) {
+  drop_graph(h);
   h->forces_lanes = s.lanes;
   h->ib_staging = s.staging;
   h->overlap = s.overlap != 0;
-  drop_graph(h);
+  if (h->lb_cpt == s.cpt) return hipSuccess;
+  h->lb_cpt = s.cpt;
+  if (h->K == 1) return hipSuccess;
+  /* the batched LB launch's workgroups follow the cells per lane */
+  hipError_t e = batch_layout(h);
+  if (e == hipSuccess) e = upload_batch_args(h);
+  return e;
 }
 
 /* ms per step of `steps` steps from the snapshot (after `warm` steps from the
@@ -1768,8 +1774,8 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   pfg_status st = PFG_OK;
   double best_ms = 0.0;
   int tried = 0;
-  apply_shape(h, best);
-  st = time_shape(h, S, steps, warm, &best_ms);
+  e = apply_shape(h, best);
+  st = e == hipSuccess ? time_shape(h, S, steps, warm, &best_ms) : from_hip(e);
   out->ms_default = best_ms;
   tried = 1;
   for (int knob = 0; knob < 4 && st == PFG_OK; ++knob) {
@@ -1781,9 +1787,9 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
       int *slot = knob == 0 ? &s.cpt : knob == 1 ? &s.lanes : knob == 2 ? &s.staging : &s.overlap;
       if (*slot == vals[v]) continue;
       *slot = vals[v];
-      apply_shape(h, s);
       double ms = 0.0;
-      st = time_shape(h, S, steps, warm, &ms);
+      e = apply_shape(h, s);
+      st = e == hipSuccess ? time_shape(h, S, steps, warm, &ms) : from_hip(e);
       ++tried;
       if (st == PFG_OK && ms < 0.99 * best_ms) {
         best = s;
@@ -1792,8 +1798,8 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
     }
   }
   if (st == PFG_OK) {
-    apply_shape(h, best);
-    e = snapshot_copy(h, S, false);
+    e = apply_shape(h, best);
+    if (e == hipSuccess) e = snapshot_copy(h, S, false);
     if (e != hipSuccess) st = from_hip(e);
   } else {
     apply_shape(h, start);

@@ -90,3 +90,31 @@ def test_batch_coupled_equals_single(gpu, staging):
             assert np.abs(got[k][0][name] - mk[name]).max() < 1e-5, (k, name)
         for a, w in zip(got[k][1], fl):
             assert np.abs(a - w).max() < 1e-6, k
+
+
+def test_batch_autotune_restores_the_state(gpu):
+    """The autotuner on a batch (it also rebuilds the batched LB launch for
+    each cells-per-lane shape) restores the lattices and markers it stepped:
+    the batch continues as an untuned one, up to the FP32 order of the
+    force-field atomics."""
+    pfg = gpu
+    scenes = [pfg.Scene(pfg.SCENE_BATCH, nx=48, ny=32, nz=24, elements=128, receptors=-1,
+                        ligands=-1, seed=11),
+              pfg.Scene(pfg.SCENE_BATCH, nx=64, ny=24, nz=20, elements=160, receptors=-1,
+                        ligands=-1, seed=13)]
+    runs = []
+    for tune in (False, True):
+        with pfg.Batch([s.params for s in scenes]) as b:
+            b.load_scenes(scenes)
+            b.step(6)
+            if tune:
+                t = b.autotune(4)
+                assert t["shapes_timed"] >= 7 and t["lb_cells_per_lane"] in (1, 2, 4)
+            info = b.step(6)
+            assert info["step"] == 12
+            runs.append([(b.markers_of(k), b.fields_of(k)) for k in range(len(scenes))])
+    for k in range(len(scenes)):
+        for name in ("x", "y", "z"):
+            assert np.abs(runs[0][k][0][name] - runs[1][k][0][name]).max() < 1e-5, (k, name)
+        for a, w in zip(runs[0][k][1], runs[1][k][1]):
+            assert np.abs(a - w).max() < 1e-6, k
