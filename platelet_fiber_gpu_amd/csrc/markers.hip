@@ -1900,6 +1900,7 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->roles = 0;
   a->halt = h->halt_on_detach ? 1 : 0;
   a->mark_rows = h->overlap && h->row_stamp != nullptr ? 1 : 0;
+  a->resolve_last = 0;
   a->blocks0 = a->blocks1 = a->blocks2 = a->blocks3 = 0;
 }
 
