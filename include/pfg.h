@@ -328,7 +328,7 @@ pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable);
 pfg_status pfg_set_ib_staging(pfg_handle *h, int mode);
 /* Lanes per marker in the SEM/fibrin force pass (sem_Force_kernel,
  * sem_kernel.cu:320-513, with the Morse pairs over the cell lists): 0
- * automatic (64 below 32768 markers, else 4), or 1, 4, 16 or 64.  With
+ * automatic (64 below 32768 markers, else 4), or 1, 2, 4, 8, 16, 32 or 64.  With
  * fewer than 64 the markers are taken in cell-list order.  Same result up
  * to FP32 summation order. */
 pfg_status pfg_set_forces_lanes(pfg_handle *h, int lanes);
@@ -396,7 +396,7 @@ pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane);
 /* Launch shapes chosen by pfg_autotune. */
 typedef struct pfg_tuning {
   int lb_cells_per_lane; /* pfg_set_lb_tuning (1 for batches)               */
-  int forces_lanes;      /* pfg_set_forces_lanes: 0 automatic, 1, 4, 16, 64  */
+  int forces_lanes;      /* pfg_set_forces_lanes: 0 automatic, or 1 .. 64     */
   int ib_staging;        /* pfg_set_ib_staging: 0 automatic, 1, 2            */
   int overlap;           /* pfg_set_overlap                                  */
   double ms_per_step;    /* the chosen shapes                                */

@@ -1515,7 +1515,8 @@ pfg_status pfg_set_ib_staging(pfg_handle *h, int mode) {
 }
 
 pfg_status pfg_set_forces_lanes(pfg_handle *h, int lanes) {
-  if (!h || !(lanes == 0 || lanes == 1 || lanes == 4 || lanes == 16 || lanes == 64))
+  if (!h || !(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 ||
+              lanes == 32 || lanes == 64))
     return PFG_ERR_ARG;
   h->forces_lanes = lanes;
   drop_graph(h);
@@ -1768,7 +1769,8 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   /* knobs, tried one at a time from the automatic shape, each kept at its
    * fastest value before the next is tried (values 0 are the automatic
    * choices); a value replaces the best only when it is 1 % faster */
-  const int cpts[3] = {1, 2, 4}, lanes[4] = {64, 16, 4, 1}, stag[2] = {1, 2}, ovl[2] = {0, 1};
+  const int cpts[3] = {1, 2, 4}, lanes[7] = {64, 32, 16, 8, 4, 2, 1}, stag[2] = {1, 2},
+            ovl[2] = {0, 1};
   Shape best{1, 0, 0, 0};
   const int warm = GRAPH_CHUNK;
   pfg_status st = PFG_OK;
@@ -1780,7 +1782,7 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   tried = 1;
   for (int knob = 0; knob < 4 && st == PFG_OK; ++knob) {
     const int *vals = knob == 0 ? cpts : knob == 1 ? lanes : knob == 2 ? stag : ovl;
-    const int nv = knob == 0 ? 3 : knob == 1 ? 4 : 2;
+    const int nv = knob == 0 ? 3 : knob == 1 ? 7 : 2;
     if (knob > 0 && !markers) break;
     for (int v = 0; v < nv && st == PFG_OK; ++v) {
       Shape s = best;

@@ -2016,7 +2016,10 @@ hipError_t launch_forces(pfg_handle *h, bool spread) {
   switch (G) {
     case 1: hipLaunchKernelGGL(forces_kernel<1>, dim3(grid), dim3(256), 0, h->stream, a); break;
     case 4: hipLaunchKernelGGL(forces_kernel<4>, dim3(grid), dim3(256), 0, h->stream, a); break;
+    case 2: hipLaunchKernelGGL(forces_kernel<2>, dim3(grid), dim3(256), 0, h->stream, a); break;
+    case 8: hipLaunchKernelGGL(forces_kernel<8>, dim3(grid), dim3(256), 0, h->stream, a); break;
     case 16: hipLaunchKernelGGL(forces_kernel<16>, dim3(grid), dim3(256), 0, h->stream, a); break;
+    case 32: hipLaunchKernelGGL(forces_kernel<32>, dim3(grid), dim3(256), 0, h->stream, a); break;
     default: hipLaunchKernelGGL(forces_kernel<64>, dim3(grid), dim3(256), 0, h->stream, a); break;
   }
   if (h->m.bend)

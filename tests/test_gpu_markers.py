@@ -75,12 +75,12 @@ def test_membrane_sums_fp64(gpu, po):
     assert np.allclose(tS, otS, rtol=1e-12)
 
 
-@pytest.mark.parametrize("lanes", [0, 4, 1])
+@pytest.mark.parametrize("lanes", [0, 8, 4, 1])
 @pytest.mark.parametrize("what", ["morse", "springs_area_volume", "wall", "fibrin_wlc",
                                   "fibrin_linear_refsign", "bending", "bending_mesh"])
 def test_forces(gpu, po, what, lanes):
     """Marker forces against the oracle with a wavefront per marker (0: the
-    automatic choice at this size) and with 4 or 1 lanes per marker in
+    automatic choice at this size) and with 8, 4 or 1 lanes per marker in
     cell-list order."""
     pfg = gpu
     if what.startswith("fibrin"):
