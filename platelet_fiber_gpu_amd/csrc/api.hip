@@ -1376,13 +1376,16 @@ static hipError_t run_stage(pfg_handle *h, int s) {
 /* One coupled step, in the order of the reference's per-step calls
  * (gpuSEM_LB.c:1441-1442 -> semLB.cu:441-531 then semLB.cu:303-438):
  * forces from X(n), spread, LB with F(n), interpolate, move.  The stages are
- * fused into nine launches; the LB update zeroes the force it reads, so the
- * next step's spreading adds into zeros without a clearing pass. */
-/* One coupled step.  With overlap, the first marker launch stamps the rows
- * the IB stencils touch; the LB update of all other rows then runs on
- * stream2 while the marker pipeline runs on the main stream, and the
- * stamped rows update after the spreading.  Without overlap (or when the LB
- * launch is timed by lb0/lb1), one LB launch follows the spreading. */
+ * fused into seven to ten launches (cell lists, scan, scatter, sort + sums +
+ * bond decisions, bond resolution, forces, spreading, LB, move; resolution
+ * joins the sort launch with few receptors, spreading joins the forces with
+ * a wavefront per marker); the LB update zeroes the force it reads, so the
+ * next step's spreading adds into zeros without a clearing pass.
+ * With overlap, the first marker launch stamps the rows the IB stencils
+ * touch; the LB update of all other rows then runs on stream2 while the
+ * marker pipeline runs on the main stream, and the stamped rows update
+ * after the spreading.  Without overlap (or when the LB launch is timed by
+ * lb0/lb1), one LB launch follows the spreading. */
 static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
   hipError_t e = hipSuccess;
   const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
