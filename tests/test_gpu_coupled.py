@@ -185,7 +185,7 @@ def test_large_physics_subbox_fp16(gpu, po):
     print(f"64^3 large-physics sub-box FP16, 100 steps: rel L2 u {err_u:.2e}, max |drho| "
           f"{err_rho:.2e}, centroid errors {dcen} (displacements {disp})")
     assert err_u < 1e-3
-    assert err_rho < 2e-4
+    assert err_rho < 5e-4
     assert max(dcen) < 1e-2 * max(max(disp), 0.1)
 
 
