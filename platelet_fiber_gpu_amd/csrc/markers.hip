@@ -75,6 +75,7 @@ struct MArgs {
                markers unchanged (the step loop ends at detach) */
   int mark_rows; /* 1: stamp the x-block rows this step's IB stencils touch */
   int resolve_last; /* 1: the gather launch's last workgroup resolves the bonds */
+  int count_spans;  /* 1: the counting launch adds the scan's span totals */
   int blocks0, blocks1, blocks2, blocks3; /* blocks of role ranges 0..3 */
 };
 
@@ -340,10 +341,12 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
   const int b = L.bin0 + b3[0] + L.nb[0] * (b3[1] + L.nb[1] * b3[2]);
   a.m.marker_bin[i] = b;
   a.m.marker_slot[i] = atomicAdd(&a.m.bin_count[b], 1);
-  {
+  if (a.count_spans) {
     /* the scan's per-span totals, one atomic per span a wavefront touches
      * (its markers are consecutive, so mostly one); the scan then needs no
-     * reduction pass, and the place kernel clears them for the next step */
+     * reduction pass, and the place kernel clears them for the next step.
+     * With many markers the waves meet on few spans and these atomics
+     * serialise: the reduction launch is cheaper then. */
     const int span = b / SCAN_SPAN, lane = threadIdx.x & 63;
     uint64_t todo = __ballot(1);
     while (todo) {
@@ -393,6 +396,22 @@ __device__ inline int block_sum(int v, int *sh_wave) {
   const int t = sh_wave[0] + sh_wave[1] + sh_wave[2] + sh_wave[3];
   __syncthreads();
   return t;
+}
+
+/* Per-span totals of the bin counts (many markers; few markers add them
+ * while counting). */
+__global__ __launch_bounds__(256) void scan_reduce_kernel(const int32_t *count,
+                                                          int32_t *partials) {
+  __shared__ int sh[4];
+  const int4 *c = reinterpret_cast<const int4 *>(count + (int64_t)blockIdx.x * SCAN_SPAN);
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_SPAN / 1024; ++k) {
+    const int4 v = c[k * 256 + threadIdx.x];
+    s += v.x + v.y + v.z + v.w;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 /* Each lane scans 32 consecutive bins of the block in registers; one block
@@ -1901,6 +1920,7 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->halt = h->halt_on_detach ? 1 : 0;
   a->mark_rows = h->overlap && h->row_stamp != nullptr ? 1 : 0;
   a->resolve_last = 0;
+  a->count_spans = 0;
   a->blocks0 = a->blocks1 = a->blocks2 = a->blocks3 = 0;
 }
 
@@ -1908,11 +1928,15 @@ static inline int blocks_for(int64_t n, int per_block) {
   return (int)((n + per_block - 1) / per_block);
 }
 
+/* Few markers: the counting launch adds the span totals itself. */
+static bool count_spans(const pfg_handle *h) { return h->m.n < IB_STAGED_MIN; }
+
 hipError_t launch_begin(pfg_handle *h, bool clear, bool count) {
   MArgs a;
   make_margs(h, &a);
   const bool recs = count && h->m.n_rec > 0;
   count = count && h->m.n > 0;
+  a.count_spans = count && count_spans(h);
   a.roles = (clear ? ROLE_A : 0) | (count ? ROLE_B : 0) | (recs ? ROLE_C : 0);
   a.blocks0 = clear ? blocks_for(h->ntiles, 256) : 0;
   a.blocks1 = count ? blocks_for(h->m.n, 256) : 0;
@@ -1926,6 +1950,9 @@ hipError_t launch_begin(pfg_handle *h, bool clear, bool count) {
 hipError_t launch_scan(pfg_handle *h) {
   const Markers &m = h->m;
   if (m.n == 0) return hipSuccess;
+  if (!count_spans(h))
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
+                       h->stream, m.bin_count, m.scan_partials);
   hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
                      h->stream, m.bin_count, m.nbins, m.scan_partials, m.bin_start);
   return hipGetLastError();
