@@ -131,8 +131,8 @@ def cpu_leg_shear(pfg):
     """CPU leg of the headline workload.  Baseline: the FP64 oracle (the same
     algorithm, OpenMP over cells) on a bounded sample of the same workload,
     about 10 s of coupled steps of the 256x128x128 shear scene.  Checker: the
-    FP16-store velocity error against the oracle on a 64x32x32 body-force
-    channel, tau 0.8, F = 1e-6, 300 steps."""
+    FP16-store (and BF16-store) velocity error against the oracle on a
+    64x32x32 body-force channel, tau 0.8, F = 1e-6, 300 steps."""
     import numpy as np
     po, threads = load_oracle()
     scene = pfg.Scene(pfg.SCENE_SHEAR)
@@ -145,18 +145,22 @@ def cpu_leg_shear(pfg):
     p = pfg.default_params(nx=64, ny=32, nz=32, tau=0.8, store=pfg.STORE_F16)
     p.wall[1] = p.wall[2] = pfg.WALL_BOUNCE
     p.body_force[0] = 1e-6
-    with pfg.Solver(p) as s:
-        s.set_fluid_only(True)
-        s.step(300)
-        _, ux, uy, uz = s.fields()
     L = po.lattice(p)
     f = po.init_eq(L)
     for _ in range(300):
         f = po.lb_step(L, f)
     _, ox, oy, oz = po.fields(L, f)
-    num = np.sqrt(((ux - ox) ** 2 + (uy - oy) ** 2 + (uz - oz) ** 2).sum())
     den = np.sqrt((ox ** 2 + oy ** 2 + oz ** 2).sum())
-    return cpu, float(num / den)
+    err = {}
+    for store in (pfg.STORE_F16, pfg.STORE_BF16):
+        p.store = store
+        with pfg.Solver(p) as s:
+            s.set_fluid_only(True)
+            s.step(300)
+            _, ux, uy, uz = s.fields()
+        num = np.sqrt(((ux - ox) ** 2 + (uy - oy) ** 2 + (uz - oz) ** 2).sum())
+        err[store] = float(num / den)
+    return cpu, err[pfg.STORE_F16], err[pfg.STORE_BF16]
 
 
 def subbox_scene(pfg, n=128):
@@ -472,9 +476,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu leg: FP64 oracle baseline and FP16 accuracy ...")
-        cpu, err16 = cpu_leg_shear(pfg)
+        cpu, err16, errb16 = cpu_leg_shear(pfg)
         if res16:
             res16["rel_l2_u_vs_fp64"] = err16
+            res16["rel_l2_u_bf16_store_vs_fp64"] = errb16
 
     out = {
         "metric": METRIC,
