@@ -1929,7 +1929,7 @@ static inline int blocks_for(int64_t n, int per_block) {
 }
 
 /* Few markers: the counting launch adds the span totals itself. */
-static bool count_spans(const pfg_handle *h) { return h->m.n < IB_STAGED_MIN; }
+static bool count_spans(const pfg_handle *h) { return h->m.n < SPAN_COUNT_MAX; }
 
 hipError_t launch_begin(pfg_handle *h, bool clear, bool count) {
   MArgs a;

@@ -34,6 +34,10 @@ constexpr int FORCES_SORTED_MIN = 32768;
 /* Receptor count up to which the cell-list/bond launch resolves the bonds
  * itself (its last workgroup takes them all). */
 constexpr int RESOLVE_IN_GATHER_MAX = 2048;
+/* Marker count below which the counting launch adds the scan's span totals
+ * (above it the counting wavefronts meet on few spans and a reduction
+ * launch is cheaper). */
+constexpr int SPAN_COUNT_MAX = 32768;
 
 /* Everything the fused LB kernel needs, passed by value. */
 struct LBArgs {
