@@ -42,6 +42,31 @@ def test_channel_platelet_short_horizon(gpu, po):
     assert err_x < 1e-4
 
 
+def test_headline_config_full_size_short_horizon(gpu, po):
+    """The headline configuration at full size (256x128x128 Couette, tau 0.7,
+    wall velocity 0.05, one 2048-element platelet with 512 receptors over
+    4096 wall ligands, FP32 storage), 5 coupled steps against the FP64
+    oracle: fields, every marker and every bond."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_SHEAR)
+    assert sc.N == 256 * 128 * 128 and sc.n_markers == 2048
+    g, o = run_both(pfg, po, sc, 5)
+    ofx = po.fields(po.lattice(sc.params), o.f())
+    err_u = rel_l2(np.stack(g["u"]), np.stack(ofx[1:]))
+    pos = np.stack([g["mk"][k] for k in ("x", "y", "z")])
+    opos = np.stack([o.marker(k) for k in ("x", "y", "z")])
+    err_x = np.abs(pos - opos).max()
+    ob = o.bonds()[0]
+    mism = int((g["bonds"] != ob).sum())
+    near = o.counters()["near_threshold"]
+    print(f"headline config, 5 steps: rel L2 u {err_u:.2e}, max position error {err_x:.2e}, "
+          f"bonds {int((ob >= 0).sum())}, mismatches {mism} (near threshold {near})")
+    assert err_u < 1e-5
+    assert mism <= near
+    if mism == 0:  # coordinates near 100: a few FP32 ulps
+        assert err_x < 1e-4
+
+
 def test_channel_platelet_config(gpu, po):
     """The CPU-runnable configuration in full: 64x32x32 channel, tau 0.8,
     F = 1e-6, one 256-element Morse platelet, 4-point IB, 500 steps.
