@@ -118,3 +118,37 @@ def test_batch_autotune_restores_the_state(gpu):
             assert np.abs(runs[0][k][0][name] - runs[1][k][0][name]).max() < 1e-5, (k, name)
         for a, w in zip(runs[0][k][1], runs[1][k][1]):
             assert np.abs(a - w).max() < 1e-6, k
+
+
+def test_full_batch_workload_equals_separate_handles(gpu):
+    """The benchmark's batch at full size (256 vessel segments of uneven
+    size and shear, 1-8 platelets of 256 elements each), 4 coupled steps:
+    every lattice's markers and fields follow the same lattice stepped in a
+    handle of its own."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    pfg = gpu
+    scenes = bench.batch_scenes(pfg)
+    assert len(scenes) == 256
+    steps = 4
+    with pfg.Batch([sc.params for sc in scenes]) as b:
+        b.load_scenes(scenes)
+        b.step(steps)
+        got = [(b.markers_of(k), b.fields_of(k)) for k in range(len(scenes))]
+    worst_x = worst_u = 0.0
+    for k, sc in enumerate(scenes):
+        with pfg.Solver(sc.params) as s:
+            s.load(sc)
+            s.step(steps)
+            mk = s.markers()
+            fl = s.fields()
+        for name in ("x", "y", "z"):
+            worst_x = max(worst_x, float(np.abs(got[k][0][name] - mk[name]).max()))
+        for a, w in zip(got[k][1], fl):
+            worst_u = max(worst_u, float(np.abs(a - w).max()))
+    print(f"full batch, {steps} steps: max position difference {worst_x:.2e}, "
+          f"max field difference {worst_u:.2e}")
+    assert worst_x < 3e-5  # coordinates up to 96: a few FP32 ulps
+    assert worst_u < 1e-6
