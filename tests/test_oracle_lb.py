@@ -166,3 +166,24 @@ def test_reference_order_differs_only_through_collision_velocity(pfg, po):
     assert np.allclose(a.sum(0), b.sum(0), atol=1e-15)
     assert np.allclose(c.T @ a - c.T @ b, (F - Fprev) / (2 * tau), atol=1e-15)
     assert np.array_equal(po.lb_step(L, f, Fu=F, Fs=F), a)
+
+
+def test_results_do_not_depend_on_the_thread_count(pfg, po):
+    """The oracle's lattice loops and interpolation run under OpenMP; every
+    cell and marker is computed on its own, so a coupled run gives the same
+    bits on one thread and on several."""
+    sc = pfg.Scene(pfg.SCENE_CHANNEL)
+    before = po.set_threads(0)
+    out = []
+    try:
+        for threads in (1, 4):
+            po.set_threads(threads)
+            o = po.Oracle(sc.params, scene=sc)
+            o.step(3)
+            out.append((o.f().copy(), o.markers()))
+            o.close()
+    finally:
+        po.set_threads(before)
+    assert np.array_equal(out[0][0], out[1][0])
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(out[0][1][k], out[1][1][k])
