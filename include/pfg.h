@@ -392,10 +392,16 @@ pfg_status pfg_copy_probe(int device, size_t bytes, int reps, double *gbs);
  * 4; a wavefront updates one 64-cell x-block of that many rows).  Applies to
  * batches too; the overlap mode of a batch runs with 1 only. */
 pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane);
+/* LB-kernel workgroup order: 0 (default) as the hardware deals workgroups
+ * round-robin to the 8 XCDs, 1 each XCD a contiguous range of rows, so that
+ * neighbouring rows share its L2.  Same results either way.  Single-lattice
+ * handles only (a batch's launch follows its table). */
+pfg_status pfg_set_lb_xcd_rows(pfg_handle *h, int enable);
 
 /* Launch shapes chosen by pfg_autotune. */
 typedef struct pfg_tuning {
-  int lb_cells_per_lane; /* pfg_set_lb_tuning (1 for batches)               */
+  int lb_cells_per_lane; /* pfg_set_lb_tuning                                */
+  int lb_xcd_rows;       /* pfg_set_lb_xcd_rows (0 for batches)              */
   int forces_lanes;      /* pfg_set_forces_lanes: 0 automatic, or 1 .. 64     */
   int ib_staging;        /* pfg_set_ib_staging: 0 automatic, 1, 2            */
   int overlap;           /* pfg_set_overlap                                  */
@@ -405,8 +411,8 @@ typedef struct pfg_tuning {
 } pfg_tuning;
 /* Times `steps` coupled steps of the handle's own lattices and markers for
  * each launch shape, one knob at a time from the automatic choices (LB cells
- * per lane, lanes per marker in the force pass, the spreading form, the LB
- * overlap), keeps the fastest value of each and sets it on the handle.  The
+ * per lane, LB workgroup order, lanes per marker in the force pass, the
+ * spreading form, the LB overlap), keeps the fastest value of each and sets it on the handle.  The
  * state is saved on the device before and restored after (the handle
  * continues from where it was; the trial steps leave no trace), at the cost
  * of one extra copy of the populations and of the marker state while it

@@ -394,6 +394,9 @@ class Solver:
     def set_lb_tuning(self, cells_per_lane):
         self._c("pfg_set_lb_tuning", int(cells_per_lane))
 
+    def set_lb_xcd_rows(self, on):
+        self._c("pfg_set_lb_xcd_rows", int(bool(on)))
+
     def autotune(self, steps=32):
         """Time `steps` steps per launch shape, keep the fastest on the
         handle (the state is restored afterwards); returns the choice."""

@@ -75,7 +75,8 @@ class StepInfo(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("lb_cells_per_lane", C.c_int32), ("forces_lanes", C.c_int32),
+    _fields_ = [("lb_cells_per_lane", C.c_int32), ("lb_xcd_rows", C.c_int32),
+                ("forces_lanes", C.c_int32),
                 ("ib_staging", C.c_int32), ("overlap", C.c_int32),
                 ("ms_per_step", C.c_double), ("ms_default", C.c_double),
                 ("shapes_timed", C.c_int32)]
@@ -145,6 +146,7 @@ SIGNATURES = {
     "pfg_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "pfg_set_lb_tuning": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tuning)]),
+    "pfg_set_lb_xcd_rows": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_scene_opts_init": (None, [C.POINTER(SceneOpts), C.c_int]),
     "pfg_scene_build": (C.c_int, [C.POINTER(SceneOpts), C.POINTER(Params),
                                   C.POINTER(C.c_void_p)]),

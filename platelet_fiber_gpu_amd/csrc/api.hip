@@ -1711,11 +1711,12 @@ hipError_t snapshot_take(pfg_handle *h, Snapshot &S) {
 }
 
 struct Shape {
-  int cpt, lanes, staging, overlap;
+  int cpt, lanes, staging, overlap, xcd;
 };
 
 hipError_t apply_shape(pfg_handle *h, const Shape &s) {
   drop_graph(h);
+  h->lb_xcd_rows = s.xcd;
   h->forces_lanes = s.lanes;
   h->ib_staging = s.staging;
   h->overlap = s.overlap != 0;
@@ -1762,7 +1763,8 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   if (h->halt_on_detach) return PFG_ERR_STATE; /* a detach inside a trial ends it early */
   hipSetDevice(h->device);
   const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
-  const Shape start{h->lb_cpt, h->forces_lanes, h->ib_staging, h->overlap ? 1 : 0};
+  const Shape start{h->lb_cpt, h->forces_lanes, h->ib_staging, h->overlap ? 1 : 0,
+                    h->lb_xcd_rows};
   Snapshot S;
   hipError_t e = snapshot_take(h, S);
   if (e != hipSuccess) {
@@ -1773,8 +1775,8 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
    * fastest value before the next is tried (values 0 are the automatic
    * choices); a value replaces the best only when it is 1 % faster */
   const int cpts[3] = {1, 2, 4}, lanes[7] = {64, 32, 16, 8, 4, 2, 1}, stag[2] = {1, 2},
-            ovl[2] = {0, 1};
-  Shape best{1, 0, 0, 0};
+            ovl[2] = {0, 1}, xcds[2] = {0, 1};
+  Shape best{1, 0, 0, 0, 0};
   const int warm = GRAPH_CHUNK;
   pfg_status st = PFG_OK;
   double best_ms = 0.0;
@@ -1783,13 +1785,18 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   st = e == hipSuccess ? time_shape(h, S, steps, warm, &best_ms) : from_hip(e);
   out->ms_default = best_ms;
   tried = 1;
-  for (int knob = 0; knob < 4 && st == PFG_OK; ++knob) {
-    const int *vals = knob == 0 ? cpts : knob == 1 ? lanes : knob == 2 ? stag : ovl;
-    const int nv = knob == 0 ? 3 : knob == 1 ? 7 : 2;
-    if (knob > 0 && !markers) break;
+  /* knobs: 0 LB cells per lane, 1 LB row order (single lattice), 2 force
+   * lanes, 3 spreading form, 4 overlap */
+  for (int knob = 0; knob < 5 && st == PFG_OK; ++knob) {
+    const int *vals = knob == 0 ? cpts : knob == 1 ? xcds : knob == 2 ? lanes
+                                       : knob == 3 ? stag : ovl;
+    const int nv = knob == 0 ? 3 : knob == 2 ? 7 : 2;
+    if (knob == 1 && h->K > 1) continue; /* the batched launch keeps its table order */
+    if (knob > 1 && !markers) break;
     for (int v = 0; v < nv && st == PFG_OK; ++v) {
       Shape s = best;
-      int *slot = knob == 0 ? &s.cpt : knob == 1 ? &s.lanes : knob == 2 ? &s.staging : &s.overlap;
+      int *slot = knob == 0 ? &s.cpt : knob == 1 ? &s.xcd : knob == 2 ? &s.lanes
+                                     : knob == 3 ? &s.staging : &s.overlap;
       if (*slot == vals[v]) continue;
       *slot = vals[v];
       double ms = 0.0;
@@ -1813,11 +1820,19 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   snapshot_free(S);
   if (st != PFG_OK) return st;
   out->lb_cells_per_lane = best.cpt;
+  out->lb_xcd_rows = best.xcd;
   out->forces_lanes = best.lanes;
   out->ib_staging = best.staging;
   out->overlap = best.overlap;
   out->ms_per_step = best_ms;
   out->shapes_timed = tried;
+  return PFG_OK;
+}
+
+pfg_status pfg_set_lb_xcd_rows(pfg_handle *h, int enable) {
+  if (!h) return PFG_ERR_ARG;
+  h->lb_xcd_rows = enable != 0;
+  drop_graph(h);
   return PFG_OK;
 }
 

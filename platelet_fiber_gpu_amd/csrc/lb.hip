@@ -346,7 +346,16 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
 template <int S, int CPT, int MODE>
 __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, float *oux,
                                                        float *ouy, float *ouz) {
-  lb_update<S, CPT, MODE>(a, blockIdx.x, orho, oux, ouy, ouz);
+  /* Workgroups are dealt round-robin to the 8 XCDs.  With xcd_rows each
+   * XCD takes a contiguous range of rows instead, so that neighbouring rows
+   * (which read each other's populations) share its L2; the autotuner picks
+   * the order (faster on large lattices, slower on small ones). */
+  uint32_t blk = blockIdx.x;
+  if (a.xcd_rows) {
+    const uint32_t q = gridDim.x / 8u;
+    if (blk < 8u * q) blk = (blk % 8u) * q + blk / 8u;
+  }
+  lb_update<S, CPT, MODE>(a, blk, orho, oux, ouy, ouz);
 }
 
 /* A batch of lattices in one launch: workgroup -> lattice through a table,
@@ -547,6 +556,7 @@ void make_lb_args(const pfg_handle *h, int k, int cur, LBArgs *a) {
         a->wall_term[ax][s][d] = 6.0f * weight(d) * cu * a->scale;
       }
   a->zvel = p.wall[2] == PFG_WALL_VELOCITY ? 1 : 0;
+  a->xcd_rows = h->lb_xcd_rows;
   for (int k = 0; k < 3; ++k) {
     a->zvel_u[0][k] = p.wall_u_lo[2][k];
     a->zvel_u[1][k] = p.wall_u_hi[2][k];

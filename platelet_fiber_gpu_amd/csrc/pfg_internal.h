@@ -73,6 +73,7 @@ struct LBArgs {
   /* 6 w_d (c_d . u_wall) for [axis][0 lower, 1 upper][d] */
   float wall_term[3][2][Q];
   int zvel;               /* z faces are Zou/He velocity faces                */
+  int xcd_rows;           /* 1: each XCD takes a contiguous range of rows     */
   float zvel_u[2][3];     /* their velocities (z = 0, z = nz-1)               */
 };
 
@@ -229,6 +230,7 @@ struct pfg_handle {
   double *red = nullptr;       /* reduction partials                        */
   int64_t red_blocks = 0;
   int lb_cpt = 1;              /* LB launch: z-cells per lane               */
+  int lb_xcd_rows = 0;         /* LB launch: contiguous rows per XCD        */
   pfg::Markers m;
   bool model_loaded = false;
   bool fluid_only = false;

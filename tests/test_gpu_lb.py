@@ -152,11 +152,12 @@ def test_graph_and_eager_agree(gpu):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("xcd", [0, 1])
 @pytest.mark.parametrize("cpt", [1, 2, 4])
 @pytest.mark.parametrize("store", [0, 1])
-def test_launch_shapes_agree(gpu, cpt, store):
-    """z-cells per lane change only the launch, never the result (96 cells
-    along x: one full and one partial 64-cell block)."""
+def test_launch_shapes_agree(gpu, cpt, store, xcd):
+    """z-cells per lane and the workgroup order change only the launch, never
+    the result (96 cells along x: one full and one partial 64-cell block)."""
     pfg = gpu
     p = make_params(pfg, 96, 20, 18, (0, 1, 1), g=(1e-5, 0, 0))
     p.store = store
@@ -167,6 +168,7 @@ def test_launch_shapes_agree(gpu, cpt, store):
         ref = s.f()
     with pfg.Solver(p) as s:
         s.set_lb_tuning(cpt)
+        s.set_lb_xcd_rows(xcd)
         s.set_fluid_only(True)
         s.step(3)
         assert np.array_equal(s.f(), ref)
@@ -186,7 +188,8 @@ def test_autotune_leaves_the_lattice_unchanged(gpu):
         s.step(2)
         t = s.autotune(8)
         assert t["lb_cells_per_lane"] in (1, 2, 4)
-        assert t["shapes_timed"] == 3 and t["ms_per_step"] > 0
+        assert t["shapes_timed"] == 4 and t["ms_per_step"] > 0
+        assert t["lb_xcd_rows"] in (0, 1)
         assert s.step(3)["step"] == 5
         assert np.array_equal(s.f(), ref)
 
