@@ -392,10 +392,11 @@ pfg_status pfg_copy_probe(int device, size_t bytes, int reps, double *gbs);
  * 4; a wavefront updates one 64-cell x-block of that many rows).  Applies to
  * batches too; the overlap mode of a batch runs with 1 only. */
 pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane);
-/* LB-kernel workgroup order: 0 (default) as the hardware deals workgroups
- * round-robin to the 8 XCDs, 1 each XCD a contiguous range of rows, so that
- * neighbouring rows share its L2.  Same results either way.  Single-lattice
- * handles only (a batch's launch follows its table). */
+/* LB-kernel workgroup order: 0 as the hardware deals workgroups round-robin
+ * to the 8 XCDs, 1 each XCD a contiguous range of rows, so that
+ * neighbouring rows share its L2.  Same results either way.  Default: 1
+ * from 2^27 lattice cells up, else 0.  Single-lattice handles only (a
+ * batch's launch follows its table). */
 pfg_status pfg_set_lb_xcd_rows(pfg_handle *h, int enable);
 
 /* Launch shapes chosen by pfg_autotune. */

@@ -391,6 +391,7 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
     if (dalloc(h, &h->red, (size_t)4 * h->red_blocks) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     /* z-cells per lane of the LB kernel */
     h->lb_cpt = 1;
+    h->lb_xcd_rows = (K == 1 && h->N >= XCD_ROWS_MIN_CELLS) ? 1 : 0;
     if (dalloc(h, &h->m.counters, 8) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->m.counters, 0, sizeof(int64_t) * 8) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     if (dalloc(h, &h->m.ticket, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
@@ -1776,7 +1777,7 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
    * choices); a value replaces the best only when it is 1 % faster */
   const int cpts[3] = {1, 2, 4}, lanes[7] = {64, 32, 16, 8, 4, 2, 1}, stag[2] = {1, 2},
             ovl[2] = {0, 1}, xcds[2] = {0, 1};
-  Shape best{1, 0, 0, 0, 0};
+  Shape best{1, 0, 0, 0, (h->K == 1 && h->N >= XCD_ROWS_MIN_CELLS) ? 1 : 0};
   const int warm = GRAPH_CHUNK;
   pfg_status st = PFG_OK;
   double best_ms = 0.0;

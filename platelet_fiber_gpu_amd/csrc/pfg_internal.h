@@ -38,6 +38,10 @@ constexpr int RESOLVE_IN_GATHER_MAX = 2048;
  * (above it the counting wavefronts meet on few spans and a reduction
  * launch is cheaper). */
 constexpr int SPAN_COUNT_MAX = 32768;
+/* Lattice cells from which a single-lattice LB launch gives each XCD a
+ * contiguous range of rows by default (faster at 268 M cells, slower at
+ * 4 M; the autotuner tries both). */
+constexpr int64_t XCD_ROWS_MIN_CELLS = (int64_t)1 << 27;
 
 /* Everything the fused LB kernel needs, passed by value. */
 struct LBArgs {
