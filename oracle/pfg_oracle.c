@@ -26,9 +26,22 @@
  *                               network::getSpringForce compiled from the
  *                               reference (oracle/_ref) and Newton's third law.
  *   bonds                       Philox4x32-10 known-answer vectors
- *                               (Random123); p = 1 - exp(-k dt) closed form.
- *   update / kick               closed form; kick against network::solverkick
- *                               compiled from the reference (oracle/_ref).
+ *                               (Random123); p = 1 - exp(-k dt) closed form;
+ *                               the bond spring as minus the gradient of its
+ *                               energy under rigid translation (finite
+ *                               differences), shares proportional to 1/r_k,
+ *                               all of it on a vertex the receptor sits on,
+ *                               equal shares at an equilateral centroid.
+ *   wall repulsion              minus the gradient of U(h) = -a ln(1 - e^{-kh})
+ *                               (finite differences), the a/h small-gap limit,
+ *                               zero above hc, clamped below hc/100, on the
+ *                               lowest element only.
+ *   update / kick               update: free markers in a uniform flow (an
+ *                               exact fixed point of the lattice) read the
+ *                               flow velocity and move by u dt per step,
+ *                               anchored ones stay; kick against
+ *                               network::solverkick compiled from the
+ *                               reference (oracle/_ref).
  *   bending chain               mean curvature of spheres and ellipsoids against
  *                               their closed forms; the ring Laplacian exact on
  *                               linear and quadratic functions over planar
