@@ -1,23 +1,31 @@
 #!/usr/bin/env python3
-"""Benchmark of the coupled platelet step on one MI355X.
+"""Benchmark of the coupled platelet-fibrin step on one MI355X.
 
-Workload: single platelet in shear, 256x128x128 D3Q19 Couette flow (wall
-velocity 0.05, tau 0.7), one 2048-element SEM platelet with 512 receptors over
-4096 wall ligands, Bell bonds, FP32 storage.  A step is one full coupled step:
-cell lists, membrane sums, bonds, forces, IB spreading, fused LB
+Default workload: the 1024x512x512 configuration (268M lattice cells) --
+D3Q19 Couette flow (wall velocity 0.05, tau 0.7), 2000 platelets of 1000 SEM
+elements with 64 receptors each over 65536 wall ligands, a fibrin network of
+80k nodes and 120k links, Bell bonds -- with the distributions stored as FP16
+deviations (f_i - w_i) x 2^10 and collided in FP32.  A step is one full
+coupled step: cell lists, membrane sums, bonds, forces, IB spreading, fused LB
 collide-stream, IB interpolation and marker update.  Inputs are resident in
 HBM before the timed region; the timed region is K hipGraph step launches
-between two device synchronisations.
+between two device synchronisations.  The same lattice then runs in FP32
+storage for the FP16/FP32 ratios.
 
-Prints ONE JSON line.  value = coupled-step MLUPS (lattice cells x steps / s).
-roofline: the fused LB kernel (the dominant kernel), algorithmic bytes
-152 B/cell (FP32 storage: 19 x 4 B read + 19 x 4 B write) over its mean
-duration, measured with HIP events on the solver's stream around each of K
-back-to-back launches of the same kernel on the same lattice.
+Prints ONE JSON line.  value = coupled-step MLUPS with FP16 storage (lattice
+cells x steps / s).  roofline: the fused LB kernel (the dominant kernel),
+algorithmic bytes 76 B/cell (19 x 2 B read + 19 x 2 B written) x 268435456
+cells per launch over its mean duration, measured with HIP events on the
+handle's stream around back-to-back launches of the same kernel.  traffic:
+HBM bytes per launch from the committed rocprofv3 PMC summary of this
+workload.  cpu_baseline and accuracy: the CPU leg on a 128^3 sub-box of the
+same physics.
 
-Other modes (for rocprofv3 runs):
-  --lb-only f32|f16|bf16   K fused LB steps only, no markers
-  --copy-probe             the library's float4 streaming copy (2 x 318 MB)
+Other modes:
+  --workload shear|thrombus|batch   the other configurations
+  --lb-only f32|f16|bf16            K fused LB steps only (no markers) on the
+                                    workload's lattice
+  --copy-probe                      the library's float4 streaming copy
 """
 import argparse
 import json
@@ -32,7 +40,8 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {0: 152, 1: 76, 2: 76}
 STORES = {"f32": 0, "f16": 1, "bf16": 2}
 METRIC = "D3Q19 MLUPS, % HBM peak; coupled steps/s FP16-store vs FP32 at stated tol"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lb_fused_shear256_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lb_fused_large1024_f16_traffic.txt")
+TRAFFIC_FILE_SHEAR = os.path.join(ROOT, "profiles", "lb_fused_shear256_traffic.txt")
 
 
 def log(msg):
@@ -79,11 +88,11 @@ def lb_kernel_us(pfg, scene, store, launches):
     return ms * 1e3
 
 
-def read_traffic():
-    """HBM bytes per LB launch from the committed rocprofv3 PMC summary."""
-    if not os.path.exists(TRAFFIC_FILE):
+def read_traffic(path):
+    """HBM bytes per LB launch from a committed rocprofv3 PMC summary."""
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         key, _, val = line.partition("=")
         if key.strip() == "traffic_bytes_per_launch":
             try:
@@ -94,8 +103,9 @@ def read_traffic():
 
 
 def oracle_threads():
-    """Host threads for the FP64 oracle: the process's CPU share, at most 16
-    (OMP_NUM_THREADS when it is set)."""
+    """Host threads for the CPU leg: OMP_NUM_THREADS when it is set (the
+    process's CPU share on a shared host), else every CPU of the process's
+    affinity mask."""
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
         return int(env)
@@ -103,7 +113,7 @@ def oracle_threads():
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, n)
 
 
 def load_oracle():
@@ -163,77 +173,75 @@ def cpu_leg_shear(pfg):
     return cpu, err[pfg.STORE_F16], err[pfg.STORE_BF16]
 
 
-def subbox_scene(pfg, n=128):
-    """The large configuration's physics on an n^3 sub-box with the same
-    per-cell parameters: tau 0.7, Couette with the shear rate of a 0.05 wall
-    velocity over 512 cells, FP16 deviation storage, the same densities of
-    platelets (2000 x 1000 elements per 1024x512x512), fibrin (80k nodes,
-    120k links) and wall ligands (65536 per 1024x512)."""
-    import numpy as np
-    frac = n ** 3 / (1024 * 512 * 512)
-    sc = pfg.Scene(pfg.SCENE_LARGE, nx=n, ny=n, nz=n,
-                   n_platelets=max(1, round(2000 * frac)),
-                   fibrin_nodes=max(8, round(80000 * frac)),
-                   fibrin_links=max(8, round(120000 * frac)),
-                   ligands=max(64, round(65536 * n * n / (1024 * 512))))
-    uw = 0.05 * n / 512
-    sc.params.wall_u_hi[2][0] = uw
-    sc.params.detach_rate = 0.05 / 512
-    sc.u0[0][:] = uw * (np.arange(sc.N) // (n * n) + 0.5) / n
-    return sc
-
-
-def cpu_leg_large(pfg, steps=200):
-    """CPU leg of the large workload.  Checker: velocity and density L2 and
-    platelet-centroid errors of the FP16-store device run against the FP64
-    oracle on a 128^3 sub-box of the same physics (`steps` coupled steps),
-    with an FP32-store device run of the same sub-box alongside: the Morse
-    cutoff makes element trajectories sensitive to rounding, so the FP32
-    figures show how much of the difference is not the storage format.
-    Baseline: the oracle's rate on that same run."""
+def cpu_leg_large(pfg, horizons=(50, 200)):
+    """CPU leg of the large workload, on a 128^3 sub-box of its physics
+    (16 platelets, fibrin and wall ligands at its densities, hard Morse
+    cutoff).  Checker: velocity and density L2 and platelet-centroid errors
+    of FP16-store and FP32-store device runs against the FP64 oracle after
+    each horizon, beside the distance between two FP64 runs whose initial
+    positions differ by 1e-6 (how far the trajectory itself departs: the
+    hard cutoff switches a pair's force a step apart in two runs that
+    differ slightly).  Baseline: the oracle's rate on the same run."""
     import numpy as np
     po, threads = load_oracle()
-    sc = subbox_scene(pfg)
-    runs = {}
+    sc = pfg.Scene.large_subbox(128)
+    npl = int(sc.plt_S0.size)
+    body = sc.body
+    dev = {}
     for store in (pfg.STORE_F16, pfg.STORE_F32):
         sc.params.store = store
         with pfg.Solver(sc.params) as s:
             s.load(sc)
-            s.step(steps)
-            runs[store] = (s.fields(), s.markers())
+            done = 0
+            for h in horizons:
+                s.step(h - done)
+                done = h
+                dev[(store, h)] = (s.fields(), s.markers())
     sc.params.store = pfg.STORE_F16
+    L = po.lattice(sc.params)
     o = po.Oracle(sc.params, scene=sc)
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
-    orho, ox, oy, oz = po.fields(po.lattice(sc.params), o.f())
-    OP = np.stack([o.marker(k) for k in ("x", "y", "z")], 1)
+    op = po.Oracle(sc.params, scene=sc)
+    rng = np.random.default_rng(7)
+    for k in ("x", "y", "z"):
+        m = op.marker(k)
+        m[:] = m + 1e-6 * rng.standard_normal(m.size)
+    acc = {"case": f"128^3 sub-box of the large configuration (its hard Morse cutoff), "
+                   f"{npl} platelets, {sc.n_markers} markers; device FP16 and FP32 "
+                   f"storage against the FP64 oracle; 'floor': FP64 against FP64 with "
+                   f"the initial positions perturbed by 1e-6", "tolerance_rel_l2_u": 1e-3}
+    t_oracle, done = 0.0, 0
+    for h in horizons:
+        t0 = time.perf_counter()
+        o.step(h - done)
+        t_oracle += time.perf_counter() - t0
+        op.step(h - done)
+        done = h
+        orho, ox, oy, oz = po.fields(L, o.f())
+        den = np.sqrt((ox ** 2 + oy ** 2 + oz ** 2).sum())
+        OP = np.stack([o.marker(k) for k in ("x", "y", "z")], 1)
+        ocen = np.array([OP[body == b].mean(0) for b in range(npl)])
+        entry = {}
+        _, px, py, pz = po.fields(L, op.f())
+        entry["floor_rel_l2_u"] = float(np.sqrt(((px - ox) ** 2 + (py - oy) ** 2 +
+                                                 (pz - oz) ** 2).sum()) / den)
+        for store, tag in ((pfg.STORE_F16, "f16"), (pfg.STORE_F32, "f32")):
+            (rho, ux, uy, uz), mk = dev[(store, h)]
+            P = np.stack([mk["x"], mk["y"], mk["z"]], 1).astype(np.float64)
+            cen = np.array([P[body == b].mean(0) for b in range(npl)])
+            num = np.sqrt(((ux - ox) ** 2 + (uy - oy) ** 2 + (uz - oz) ** 2).sum())
+            entry[tag] = {"rel_l2_u": float(num / den),
+                          "rel_l2_rho": float(np.sqrt(((rho - orho) ** 2).sum() /
+                                                      (orho ** 2).sum())),
+                          "centroid_err_max": float(np.linalg.norm(cen - ocen, axis=1).max())}
+        acc[f"steps_{h}"] = entry
+        log(f"sub-box {h} steps: {entry}")
+    acc["oracle_near_cutoff_pair_visits"] = o.counters()["near_cutoff"]
     o.close()
-    body = sc.body
-    npl = int(body.max()) + 1 if (body >= 0).any() else 0
-    ocen = np.array([OP[body == b].mean(0) for b in range(npl)])
-    X0 = np.stack([sc.x, sc.y, sc.z], 1).astype(np.float64)
-    disp = np.array([np.linalg.norm(ocen[b] - X0[body == b].mean(0)) for b in range(npl)])
-    den = np.sqrt((ox ** 2 + oy ** 2 + oz ** 2).sum())
-    acc = {"case": f"128^3 sub-box of the large configuration, {npl} platelets, "
-                   f"{sc.n_markers} markers, {steps} coupled steps, device vs FP64 oracle "
-                   f"(FP32-store run of the same sub-box alongside)",
-           "centroid_disp_mean": float(disp.mean()) if npl else None}
-    for store, tag in ((pfg.STORE_F16, "f16"), (pfg.STORE_F32, "f32")):
-        (rho, ux, uy, uz), mk = runs[store]
-        P = np.stack([mk["x"], mk["y"], mk["z"]], 1).astype(np.float64)
-        cen = np.array([P[body == b].mean(0) for b in range(npl)])
-        dcen = np.linalg.norm(cen - ocen, axis=1)
-        num = np.sqrt(((ux - ox) ** 2 + (uy - oy) ** 2 + (uz - oz) ** 2).sum())
-        acc[tag] = {"rel_l2_u": float(num / den),
-                    "rel_l2_rho": float(np.sqrt(((rho - orho) ** 2).sum() / (orho ** 2).sum())),
-                    "max_abs_rho": float(np.abs(rho - orho).max()),
-                    "centroid_err_max": float(dcen.max()) if npl else None,
-                    "centroid_err_mean": float(dcen.mean()) if npl else None}
-    cpu = {"value": round(sc.N * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads,
+    op.close()
+    cpu = {"value": round(sc.N * done / t_oracle / 1e6, 3), "unit": "MLUPS", "cores": threads,
            "kind": "port",
-           "sample": f"{steps} coupled steps of the 128^3 sub-box of the large workload, FP64 "
-                     f"oracle, {threads} OpenMP threads ({dt:.1f} s)"}
+           "sample": f"{done} coupled steps of the 128^3 sub-box of the large workload, FP64 "
+                     f"oracle, {threads} OpenMP threads ({t_oracle:.1f} s)"}
     return cpu, acc
 
 
@@ -242,16 +250,115 @@ WORKLOADS = {
                  "thrombus in a tube: 512x256x256 lattice, solid-flag tube of radius 120 "
                  "along x, body force, 500 platelets x 1000 elements, fibrin 20k nodes / "
                  "30k links, FP32 storage"),
-    "large": ("SCENE_LARGE",
-              "large shear: 1024x512x512 lattice, Couette, 2000 platelets x 1000 "
-              "elements, fibrin 80k nodes / 120k links, FP16 storage"),
 }
 
+LARGE_DESC = ("1024x512x512 D3Q19 Couette (wall velocity 0.05, tau 0.7), 2000 platelets x "
+              "1000 SEM elements (64 receptors each), 65536 wall ligands, fibrin 80k nodes / "
+              "120k links, Bell bonds; FP16 deviation storage (x 2^10), FP32 collision")
 
-def other_workload(pfg, name, K, W, cpu_leg=True):
-    """Coupled steps and the LB kernel on a full-size configuration other than
-    the headline one (same JSON schema; value = coupled MLUPS over all
-    lattice cells)."""
+
+def large_leg(pfg, scene, store, K, W):
+    """One storage format on the 1024x512x512 lattice, one handle: autotuned
+    coupled steps (K timed after W warm-up), then LB-only steps on the same
+    lattice, then the mean LB-kernel duration (HIP events on the handle's
+    stream around back-to-back launches)."""
+    p = scene.params
+    p.store = store
+    t0 = time.perf_counter()
+    s = pfg.Solver(p)
+    s.load(scene)
+    log(f"store {store}: handle loaded in {time.perf_counter() - t0:.1f} s, "
+        f"{s.device_bytes() / 1e9:.1f} GB on the device")
+    tag = {0: "f32", 1: "f16", 2: "bf16"}[store]
+    if not NO_TUNE:
+        TUNING[tag] = s.autotune(16)
+        log(f"autotune {tag}: {TUNING[tag]}")
+    s.step(W)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(K)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    info = s.step(0)
+    dev_gb = s.device_bytes() / 1e9
+    s.set_fluid_only(True)
+    kl = max(8, K // 2)
+    s.step(8)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(kl)
+    s.synchronize()
+    dtl = time.perf_counter() - t0
+    lb_ms = s.time_lb(max(10, min(K, 60)))
+    s.close()
+    N = scene.N
+    bpc = BYTES_PER_CELL[store]
+    gbs = bpc * N / (lb_ms * 1e-3) / 1e9
+    out = {"coupled_ms_per_step": round(dt * 1e3 / K, 4),
+           "coupled_mlups": round(N * K / dt / 1e6, 1),
+           "coupled_steps_per_s": round(K / dt, 2),
+           "lb_only_ms_per_step": round(dtl * 1e3 / kl, 4),
+           "lb_only_mlups": round(N * kl / dtl / 1e6, 1),
+           "lb_kernel_ms": round(lb_ms, 4),
+           "lb_kernel_gbs": round(gbs, 1),
+           "lb_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4),
+           "lb_kernel_mlups": round(N / (lb_ms * 1e-3) / 1e6, 1),
+           "bytes_per_cell": bpc,
+           "bonds_at_end": info["total_bonds"],
+           "bond_events": info["bond_events"],
+           "device_gb": round(dev_gb, 2)}
+    log(f"{tag}: {out}")
+    return out
+
+
+def large_workload(pfg, K, W, cpu_leg=True, fp32_leg=True):
+    """The 1024x512x512 configuration, FP16 storage (the headline) and the
+    same lattice in FP32 storage."""
+    t0 = time.perf_counter()
+    scene = pfg.Scene(pfg.SCENE_LARGE)
+    log(f"large: scene built in {time.perf_counter() - t0:.1f} s: {scene.N} cells, "
+        f"{scene.n_markers} markers; K={K} W={W}")
+    r16 = large_leg(pfg, scene, pfg.STORE_F16, K, W)
+    r32 = large_leg(pfg, scene, pfg.STORE_F32, K, W) if fp32_leg else None
+    cpu, acc = None, None
+    if cpu_leg:
+        log("large: CPU leg (128^3 sub-box: CPU baseline and errors against FP64) ...")
+        cpu, acc = cpu_leg_large(pfg)
+    ratios = None
+    if r32:
+        ratios = {"coupled": round(r32["coupled_ms_per_step"] / r16["coupled_ms_per_step"], 3),
+                  "lb_only": round(r32["lb_only_ms_per_step"] / r16["lb_only_ms_per_step"], 3),
+                  "lb_kernel": round(r32["lb_kernel_ms"] / r16["lb_kernel_ms"], 3)}
+    traffic = read_traffic(TRAFFIC_FILE)
+    return {
+        "metric": METRIC, "value": r16["coupled_mlups"], "unit": "MLUPS", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": r16["coupled_ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene)",
+        "config": {"workload": LARGE_DESC, "cells": scene.N, "markers": scene.n_markers,
+                   "store": "f16", "arithmetic": "f32 (FP64 conservation and membrane sums)"},
+        "coupled_steps_per_s": r16["coupled_steps_per_s"],
+        "f16": r16,
+        "f32": r32,
+        "speedup_f16_over_f32": ratios,
+        "roofline": {"bound": "hbm", "achieved": r16["lb_kernel_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": r16["lb_kernel_roofline_frac"],
+                     "traffic": traffic, "kernel": "lb_fused_kernel<F16>",
+                     "bytes_per_cell": BYTES_PER_CELL[1], "cells_per_launch": scene.N,
+                     "lb_kernel_us": round(r16["lb_kernel_ms"] * 1e3, 1)},
+        "roofline_f32": None if r32 is None else {
+            "achieved": r32["lb_kernel_gbs"], "frac": r32["lb_kernel_roofline_frac"],
+            "kernel": "lb_fused_kernel<F32>", "bytes_per_cell": BYTES_PER_CELL[0],
+            "target_frac": 0.70},
+        "tuning": TUNING,
+        "accuracy_vs_fp64": acc,
+        "cpu_baseline": cpu,
+    }
+
+
+def other_workload(pfg, name, K, W):
+    """Coupled steps and the LB kernel on another full-size configuration
+    (same JSON schema; value = coupled MLUPS over all lattice cells)."""
     attr, desc = WORKLOADS[name]
     t0 = time.perf_counter()
     scene = pfg.Scene(getattr(pfg, attr))
@@ -262,21 +369,17 @@ def other_workload(pfg, name, K, W, cpu_leg=True):
     info = s.step(0)
     dev_gb = s.device_bytes() / 1e9
     s.close()
-    fluid = int((scene.flags == 0).sum()) if scene.flags is not None else scene.N
+    fluid = scene.fluid_cells
     bpc = BYTES_PER_CELL[store]
     lb_us = lb_kernel_us(pfg, scene, store, min(K, 200))
     # solid cells are skipped by the kernel: algorithmic bytes count fluid cells
     achieved = bpc * fluid / (lb_us * 1e-6) / 1e9
     mlups = scene.N * K / dt / 1e6
     dtype = {0: "f32", 1: "f16", 2: "bf16"}[store]
-    cpu, acc = None, None
-    if cpu_leg and name == "large":
-        log("large: 128^3 sub-box against the FP64 oracle ...")
-        cpu, acc = cpu_leg_large(pfg)
     return {
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": round(dt * 1e3 / K, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded scene)",
         "config": {"workload": desc, "cells": scene.N, "fluid_cells": fluid,
                    "markers": scene.n_markers, "store": dtype},
@@ -289,8 +392,7 @@ def other_workload(pfg, name, K, W, cpu_leg=True):
                      "traffic": None, "kernel": f"lb_fused_kernel<{dtype.upper()}>",
                      "bytes_per_cell": bpc, "lb_kernel_us": round(lb_us, 2)},
         "tuning": TUNING.get(name),
-        "accuracy_vs_fp64": acc,
-        "cpu_baseline": cpu,
+        "cpu_baseline": None,
     }
 
 
@@ -319,7 +421,7 @@ def batch_scenes(pfg, n=256, seed=1101):
                            receptors=-1, ligands=512, seed=seed + k)
         uw = 0.05 * (rate / 1000.0) * (nz / 96.0)
         sc.params.wall_u_hi[2][0] = uw
-        sc.u0[0][:] = uw * (np.arange(sc.N) // (nx * ny) + 0.5) / nz
+        sc.u0_z[0][:] = uw * (np.arange(nz) + 0.5) / nz
         scenes.append(sc)
     return scenes
 
@@ -381,18 +483,101 @@ def batch_workload(pfg, K, W, store):
     }
 
 
+def shear_workload(pfg, K, W, cpu_leg=True, fp16_leg=True):
+    """The 256x128x128 single platelet in shear (one 2048-element platelet,
+    512 receptors, 4096 wall ligands), FP32 storage, with the FP16 figures of
+    the same workload alongside."""
+    scene = pfg.Scene(pfg.SCENE_SHEAR)
+    N = scene.N
+    log(f"workload 256x128x128 shear, {scene.n_markers} elements, "
+        f"{scene.rec_tri.size} receptors, {scene.lig_x.size} ligands; K={K} W={W}")
+    s32, dt32 = timed_run(pfg, scene, pfg.STORE_F32, K, W, tune="f32")
+    info = s32.step(0)
+    s32.close()
+    ms_step = dt32 * 1e3 / K
+    mlups = N * K / dt32 / 1e6
+    launches = min(K, 500)
+    lb32_us = lb_kernel_us(pfg, scene, pfg.STORE_F32, launches)
+    achieved = BYTES_PER_CELL[0] * N / (lb32_us * 1e-6) / 1e9
+    sl, dtl = timed_run(pfg, scene, pfg.STORE_F32, K, W, fluid_only=True)
+    sl.close()
+    res16 = {}
+    if fp16_leg:
+        s16, dt16 = timed_run(pfg, scene, pfg.STORE_F16, K, W, tune="f16")
+        s16.close()
+        sl16, dtl16 = timed_run(pfg, scene, pfg.STORE_F16, K, W, fluid_only=True)
+        sl16.close()
+        lb16_us = lb_kernel_us(pfg, scene, pfg.STORE_F16, launches)
+        res16 = {"coupled_mlups": round(N * K / dt16 / 1e6, 1),
+                 "coupled_steps_per_s": round(K / dt16, 1),
+                 "lb_only_mlups": round(N * K / dtl16 / 1e6, 1),
+                 "lb_kernel_us": round(lb16_us, 2),
+                 "lb_kernel_gbs": round(BYTES_PER_CELL[1] * N / (lb16_us * 1e-6) / 1e9, 1),
+                 "speedup_coupled_vs_fp32": round(dt32 / dt16, 3),
+                 "speedup_lb_only_vs_fp32": round(dtl / dtl16, 3),
+                 "speedup_lb_kernel_vs_fp32": round(lb32_us / lb16_us, 3)}
+    cpu = None
+    if cpu_leg:
+        log("cpu leg: FP64 oracle baseline and FP16 accuracy ...")
+        cpu, err16, errb16 = cpu_leg_shear(pfg)
+        if res16:
+            res16["rel_l2_u_vs_fp64"] = err16
+            res16["rel_l2_u_bf16_store_vs_fp64"] = errb16
+            res16["rel_l2_u_case"] = "64x32x32 channel, tau 0.8, F 1e-6, 300 steps; tol 1e-3"
+    return {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(ms_step, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene)",
+        "config": {"workload": "single platelet in shear, 256x128x128 D3Q19 Couette, "
+                               "2048 elements, 512 receptors, 4096 ligands, FP32 storage",
+                   "cells": N, "store": "f32"},
+        "coupled_steps_per_s": round(K / dt32, 1),
+        "lb_only_mlups_f32": round(N * K / dtl / 1e6, 1),
+        "bonds_at_end": info["total_bonds"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic(TRAFFIC_FILE_SHEAR),
+                     "kernel": "lb_fused_kernel<F32>", "bytes_per_cell": 152,
+                     "lb_kernel_us": round(lb32_us, 2)},
+        "fp16": res16,
+        "tuning": TUNING,
+        "cpu_baseline": cpu,
+    }
+
+
+def lb_only(pfg, workload, store, K, W):
+    """K fused LB steps on the workload's lattice without markers."""
+    if workload == "shear":
+        scene = pfg.Scene(pfg.SCENE_SHEAR, n_platelets=-1, receptors=-1, ligands=-1)
+    elif workload == "large":
+        scene = pfg.Scene(pfg.SCENE_LARGE, n_platelets=-1, receptors=-1, ligands=-1,
+                          fibrin_nodes=-1)
+    else:
+        raise SystemExit(f"--lb-only runs on the shear or large lattice, not {workload}")
+    s, dt = timed_run(pfg, scene, store, K, W, fluid_only=True)
+    ms = s.time_lb(max(1, min(K, 60)))
+    s.close()
+    return {"lb_only": store, "workload": workload, "cells": scene.N, "steps": K,
+            "ms_per_step": round(dt / K * 1e3, 4), "mlups": round(scene.N * K / dt / 1e6, 1),
+            "lb_kernel_ms": round(ms, 4),
+            "lb_kernel_gbs": round(BYTES_PER_CELL[store] * scene.N / (ms * 1e-3) / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true",
-                    help="skip the CPU leg (FP64 oracle baseline and accuracy checks)")
-    ap.add_argument("--no-fp16", action="store_true")
+                    help="skip the CPU leg (CPU baseline and accuracy against FP64)")
+    ap.add_argument("--no-f32", action="store_true",
+                    help="large workload: skip the FP32-store leg")
+    ap.add_argument("--no-fp16", action="store_true", help="shear workload: FP32 only")
     ap.add_argument("--lb-only", choices=sorted(STORES), default=None)
     ap.add_argument("--copy-probe", action="store_true")
-    ap.add_argument("--workload", choices=["shear", "batch"] + sorted(WORKLOADS),
-                    default="shear", help="shear (default) is the headline configuration")
+    ap.add_argument("--workload", choices=["large", "shear", "batch"] + sorted(WORKLOADS),
+                    default="large", help="large (default) is the headline configuration")
     ap.add_argument("--store", choices=sorted(STORES), default="f32",
                     help="storage of the batch workload")
     ap.add_argument("--no-tune", action="store_true",
@@ -419,98 +604,18 @@ def main():
         return
 
     K, W = args.steps, args.warmup
-    if args.workload == "batch":
-        print(json.dumps(batch_workload(pfg, K, W, STORES[args.store])), flush=True)
-        return
-    if args.workload != "shear":
-        print(json.dumps(other_workload(pfg, args.workload, K, W, not args.no_cpu)),
-              flush=True)
-        return
-    scene = pfg.Scene(pfg.SCENE_SHEAR)
-    N = scene.N
-
     if args.lb_only:
-        st = STORES[args.lb_only]
-        s, dt = timed_run(pfg, scene, st, K, W, fluid_only=True)
-        s.close()
-        print(json.dumps({"lb_only": args.lb_only, "steps": K, "us_per_step": dt / K * 1e6,
-                          "mlups": N * K / dt / 1e6}), flush=True)
-        return
-
-    log(f"workload 256x128x128 shear, {scene.n_markers} elements, "
-        f"{scene.rec_tri.size} receptors, {scene.lig_x.size} ligands; K={K} W={W}")
-    s32, dt32 = timed_run(pfg, scene, pfg.STORE_F32, K, W, tune="f32")
-    info = s32.step(0)
-    s32.close()
-    ms_step = dt32 * 1e3 / K
-    mlups = N * K / dt32 / 1e6
-    log(f"fp32 coupled: {ms_step:.4f} ms/step, {mlups:.0f} MLUPS")
-
-    launches = min(K, 500)
-    lb32_us = lb_kernel_us(pfg, scene, pfg.STORE_F32, launches)
-    achieved = BYTES_PER_CELL[0] * N / (lb32_us * 1e-6) / 1e9
-    log(f"fp32 LB kernel: {lb32_us:.1f} us/launch, {achieved:.0f} GB/s")
-
-    sl, dtl = timed_run(pfg, scene, pfg.STORE_F32, K, W, fluid_only=True)
-    sl.close()
-    lb32 = N * K / dtl / 1e6
-    res16 = {}
-    if not args.no_fp16:
-        s16, dt16 = timed_run(pfg, scene, pfg.STORE_F16, K, W, tune="f16")
-        s16.close()
-        sl16, dtl16 = timed_run(pfg, scene, pfg.STORE_F16, K, W, fluid_only=True)
-        sl16.close()
-        lb16_us = lb_kernel_us(pfg, scene, pfg.STORE_F16, launches)
-        res16 = {"coupled_mlups": round(N * K / dt16 / 1e6, 1),
-                 "coupled_steps_per_s": round(K / dt16, 1),
-                 "lb_only_mlups": round(N * K / dtl16 / 1e6, 1),
-                 "lb_kernel_us": round(lb16_us, 2),
-                 "lb_kernel_gbs": round(BYTES_PER_CELL[1] * N / (lb16_us * 1e-6) / 1e9, 1),
-                 "speedup_coupled_vs_fp32": round(dt32 / dt16, 3),
-                 "speedup_lb_only_vs_fp32": round(dtl / dtl16, 3),
-                 "speedup_lb_kernel_vs_fp32": round(lb32_us / lb16_us, 3),
-                 "rel_l2_u_vs_fp64": None,
-                 "rel_l2_u_case": "64x32x32 channel, tau 0.8, F 1e-6, 300 steps; tol 1e-3"}
-        log(f"fp16: {res16}")
-
-    cpu = None
-    if not args.no_cpu:
-        log("cpu leg: FP64 oracle baseline and FP16 accuracy ...")
-        cpu, err16, errb16 = cpu_leg_shear(pfg)
-        if res16:
-            res16["rel_l2_u_vs_fp64"] = err16
-            res16["rel_l2_u_bf16_store_vs_fp64"] = errb16
-
-    out = {
-        "metric": METRIC,
-        "value": round(mlups, 1),
-        "unit": "MLUPS",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": round(ms_step, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded scene)",
-        "config": {"workload": "single platelet in shear, 256x128x128 D3Q19 Couette, "
-                               "2048 elements, 512 receptors, 4096 ligands, FP32 storage",
-                   "cells": N, "store": "f32"},
-        "coupled_steps_per_s": round(K / dt32, 1),
-        "lb_only_mlups_f32": round(lb32, 1),
-        "bonds_at_end": info["total_bonds"],
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(),
-                     "kernel": "lb_fused_kernel<F32>", "bytes_per_cell": 152,
-                     "lb_kernel_us": round(lb32_us, 2),
-                     "mlups_kernel": round(N / (lb32_us * 1e-6) / 1e6, 1)},
-        "fp16": res16,
-        "tuning": TUNING,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+        print(json.dumps(lb_only(pfg, args.workload, STORES[args.lb_only], K, W)), flush=True)
+    elif args.workload == "large":
+        print(json.dumps(large_workload(pfg, K, W, not args.no_cpu, not args.no_f32)),
+              flush=True)
+    elif args.workload == "shear":
+        print(json.dumps(shear_workload(pfg, K, W, not args.no_cpu, not args.no_fp16)),
+              flush=True)
+    elif args.workload == "batch":
+        print(json.dumps(batch_workload(pfg, K, W, STORES[args.store])), flush=True)
+    else:
+        print(json.dumps(other_workload(pfg, args.workload, K, W)), flush=True)
 
 
 if __name__ == "__main__":

@@ -91,6 +91,41 @@ typedef enum pfg_fiber_move {
                             x += v dt (network.cpp:839-881); not fluid-coupled    */
 } pfg_fiber_move;
 
+/* How platelet SEM elements move. */
+typedef enum pfg_sem_integrator {
+  PFG_SEM_ADVECT = 0,    /* with the interpolated fluid velocity, x += V dt
+                            (sem_update_position_kernel, sem_kernel.cu:2087-2148) */
+  PFG_SEM_BBK = 1        /* Langevin leapfrog with the BBK exact-friction half
+                            kicks of verlet_1 / verlet_3 (sem_kernel.cu.orig:
+                            224-388, 519-568): per step, with F the force of
+                            this step, two half kicks
+                              v <- e^{-g dt/2} v + F (1 - e^{-g dt/2})/(g m)
+                                   + sqrt(T/m (1 - e^{-g dt})) xi
+                            (the first completes the previous step, the second
+                            starts this one; xi standard normals from the
+                            counter RNG), then the drift
+                              x += dt ((1 - w) v + w u),
+                            u the interpolated fluid velocity, w =
+                            sem_fluid_weight (verlet_1's coupled drift is
+                            w = 1/2, sem_kernel.cu.orig:377-379; w = 0: free
+                            Langevin dynamics).  The stored velocity is the
+                            half-step one; loaded velocities are taken as the
+                            velocity half a step before the first step.      */
+} pfg_sem_integrator;
+
+/* Receptor-ligand rate law. */
+typedef enum pfg_bond_law {
+  PFG_BOND_BELL = 0,     /* formation at k_on within reach; rupture at
+                            k_off0 e^{bell_xb_kT max(tension, 0)}              */
+  PFG_BOND_REFERENCE = 1 /* the force-dependent rates of
+                            sem_platelet_wall_kernel (sem_kernel.cu:1852-1863):
+                            with w = |bond_l0 - r| and f = bond_k w,
+                            formation k_on e^{f (x_b - w/2) / kT} toward the
+                            candidate ligand at distance r, rupture
+                            k_off0 e^{x_b f / kT}, x_b / kT = bell_xb_kT,
+                            kT = bell_kT                                      */
+} pfg_bond_law;
+
 /* Marker kinds. */
 enum {
   PFG_MARKER_ELEMENT = 0, /* platelet SEM element                                 */
@@ -172,6 +207,15 @@ typedef struct pfg_params {
                                   / (2 gamma)) per velocity component and step
                                   (sem_kernel.cu.orig:251-266, unit mass), from
                                   the counter RNG; 0: off                       */
+  /* ---- SEM element integration ---- */
+  int32_t sem_integrator;      /* pfg_sem_integrator                              */
+  float sem_mass;              /* element mass (PFG_SEM_BBK)                       */
+  float sem_gamma;             /* Langevin friction rate g (PFG_SEM_BBK)           */
+  float sem_temp;              /* k_B T of the Langevin noise (PFG_SEM_BBK), 0: off */
+  float sem_fluid_weight;      /* w of the BBK drift                              */
+  /* ---- bond law ---- */
+  int32_t bond_law;            /* pfg_bond_law                                     */
+  float bell_kT;               /* kT of PFG_BOND_REFERENCE                         */
 } pfg_params;
 
 /* Host description of the immersed structures; all arrays caller-owned and only
@@ -215,6 +259,13 @@ typedef struct pfg_step_info {
   int64_t detach_step;             /* step index at which it fired, else -1         */
   int64_t near_threshold;          /* bond decisions within eps of their threshold  */
   int64_t bond_events;             /* formations + ruptures so far                  */
+  int64_t near_cutoff;             /* Morse pair visits (each ordered pair (i, j))
+                                      whose distance lay within 1e-4 of the
+                                      cutoff, so far: rounding can switch them     */
+  int64_t contact_overflow;        /* contact candidates dropped because an element
+                                      already had 4 other platelets within
+                                      contact_hc, so far (0 means the closest
+                                      pairs are exact)                            */
 } pfg_step_info;
 
 /* Stages of one coupled step, in execution order (pfg_run_stage). */
@@ -266,6 +317,12 @@ pfg_status pfg_device_bytes(const pfg_handle *h, size_t *bytes);
 pfg_status pfg_init_equilibrium(pfg_handle *h, const float *rho,
                                 const float *ux, const float *uy,
                                 const float *uz);
+/* f = f_eq(1, u(z)) for a velocity that varies along z only, one value per
+ * z-plane in each array (nz values; NULL means 0): the reference's linear
+ * shear start u_y = shear rate * k * dx / UMAX (gpuSEM_LB.c:1234-1258)
+ * without a full-lattice host array. */
+pfg_status pfg_init_equilibrium_z(pfg_handle *h, const float *ux_z,
+                                  const float *uy_z, const float *uz_z);
 pfg_status pfg_upload_f(pfg_handle *h, const float *f);      /* [19][N] */
 pfg_status pfg_download_f(pfg_handle *h, float *f);          /* [19][N] */
 /* Solid-cell flags, 1 byte per cell, nonzero = solid (needs use_flags). */
@@ -361,6 +418,8 @@ pfg_status pfg_batch_size(const pfg_handle *h, int *n_lattices);
 pfg_status pfg_batch_init_equilibrium(pfg_handle *h, int k, const float *rho,
                                       const float *ux, const float *uy,
                                       const float *uz);
+pfg_status pfg_batch_init_equilibrium_z(pfg_handle *h, int k, const float *ux_z,
+                                        const float *uy_z, const float *uz_z);
 pfg_status pfg_batch_upload_flags(pfg_handle *h, int k, const uint8_t *flags);
 /* models[0..n_lattices-1], positions in each lattice's own coordinates */
 pfg_status pfg_batch_load_models(pfg_handle *h, const pfg_model *models);

@@ -70,6 +70,10 @@ pfg_status pfg_scene_model(const pfg_scene *s, pfg_model *out);
 const uint8_t *pfg_scene_flags(const pfg_scene *s);
 /* Initial velocity field [nz][ny][nx] per component (equilibrium init). */
 pfg_status pfg_scene_velocity(const pfg_scene *s, float *ux, float *uy, float *uz);
+/* The same initial velocity as one value per z-plane (it varies along z
+ * only): nz values per component. */
+pfg_status pfg_scene_velocity_z(const pfg_scene *s, float *ux_z, float *uy_z,
+                                float *uz_z);
 /* Fluid-cell count (cells not flagged solid). */
 int64_t pfg_scene_fluid_cells(const pfg_scene *s);
 

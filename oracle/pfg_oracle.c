@@ -520,6 +520,9 @@ struct orc_state {
   int nb[3];
   int *bin_start, *items;
   int64_t step, total_bonds, detached, detach_step, near, events;
+  int64_t near_cut; /* Morse pair visits within 1e-4 of the cutoff */
+  int64_t *plt_near_cut; /* the same per platelet */
+  double *uib; /* 3n: fluid velocity at the markers (PFG_SEM_BBK) */
   /* bending chain: 6-node triangles (3 vertices, 3 edge-midpoint nodes
    * numbered n + edge), triangles per node, ordered vertex rings */
   int n_enode, bend_ok;
@@ -668,6 +671,7 @@ orc_state *orc_create(const pfg_params *p, const pfg_model *m, const uint8_t *fl
   s->n = n;
   for (int k = 0; k < 9; ++k) s->mk[k] = (double *)xcalloc((size_t)n, sizeof(double));
   s->kind = (int *)xcalloc((size_t)n, sizeof(int));
+  s->uib = (double *)xcalloc((size_t)3 * n + 1, sizeof(double));
   s->body = (int *)xcalloc((size_t)n, sizeof(int));
   s->coupled = (int *)xcalloc((size_t)n, sizeof(int));
   for (int i = 0; i < n; ++i) {
@@ -702,6 +706,7 @@ orc_state *orc_create(const pfg_params *p, const pfg_model *m, const uint8_t *fl
     s->tri_S0[t] = m->tri_S0[t];
   }
   s->plt_S0 = (double *)xcalloc((size_t)s->n_plt, sizeof(double));
+  s->plt_near_cut = (int64_t *)xcalloc((size_t)s->n_plt + 1, sizeof(int64_t));
   s->plt_V0 = (double *)xcalloc((size_t)s->n_plt, sizeof(double));
   s->plt_S = (double *)xcalloc((size_t)s->n_plt, sizeof(double));
   s->plt_V = (double *)xcalloc((size_t)s->n_plt, sizeof(double));
@@ -775,6 +780,7 @@ void orc_destroy(orc_state *s) {
   free(s->u);
   for (int k = 0; k < 9; ++k) free(s->mk[k]);
   free(s->kind);
+  free(s->uib);
   free(s->body);
   free(s->coupled);
   free(s->plt_elem_start);
@@ -784,6 +790,7 @@ void orc_destroy(orc_state *s) {
   free(s->tri_S);
   free(s->tri_n);
   free(s->plt_S0);
+  free(s->plt_near_cut);
   free(s->plt_V0);
   free(s->plt_S);
   free(s->plt_V);
@@ -1043,18 +1050,30 @@ static void stage_forces(orc_state *s) {
     }
   }
   for (int i = 0; i < s->n; ++i) FX[i] = FY[i] = FZ[i] = 0;
-  /* Morse over all pairs of elements of one platelet within the cutoff */
+  /* Morse over all pairs of elements of one platelet within the cutoff.
+   * The cutoff decision is the device's: the squared distance of the
+   * FP32-rounded positions in FP32, in the same order of operations, against
+   * the FP32 squared cutoff, so that both take the same pairs while their
+   * positions agree to FP32; pairs within 1e-4 of the cutoff, which a
+   * rounding difference can switch, are counted.  The force is FP64. */
   if (p->morse_u0 != 0) {
     const double U0 = p->morse_u0, rho = p->morse_rho, re = p->morse_req;
-    const double cut2 = (double)p->morse_cutoff * p->morse_cutoff;
+    const float cut2f = p->morse_cutoff * p->morse_cutoff;
+    const float eps2f = 2.0f * p->morse_cutoff * 1e-4f;
     for (int q = 0; q < s->n_plt; ++q)
       for (int i = s->plt_elem_start[q]; i < s->plt_elem_start[q + 1]; ++i)
         for (int j = s->plt_elem_start[q]; j < s->plt_elem_start[q + 1]; ++j) {
           if (j == i) continue;
+          const float dxf = (float)X[j] - (float)X[i], dyf = (float)Y[j] - (float)Y[i],
+                      dzf = (float)Z[j] - (float)Z[i];
+          const float r2f = fmaf(dzf, dzf, fmaf(dyf, dyf, dxf * dxf));
+          if (fabsf(r2f - cut2f) < eps2f) {
+            s->near_cut++;
+            s->plt_near_cut[q]++;
+          }
+          if (r2f >= cut2f || r2f == 0.f) continue;
           const double dx = X[j] - X[i], dy = Y[j] - Y[i], dz = Z[j] - Z[i];
-          const double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 >= cut2 || r2 == 0) continue;
-          const double r = sqrt(r2);
+          const double r = sqrt(dx * dx + dy * dy + dz * dz);
           /* U = U0 (e^{-2 rho (r - re)} - 2 e^{-rho (r - re)}) times the
            * smoothstep S that falls to 0 over [rc - taper, rc];
            * F_i = -d(U S)/dr (x_i - x_j)/r */
@@ -1508,10 +1527,28 @@ static void stage_lb(orc_state *s) {
   s->f2 = t;
 }
 
+static int bbk_element(const orc_state *s, int i) {
+  return s->kind[i] == PFG_MARKER_ELEMENT && s->p.sem_integrator == PFG_SEM_BBK;
+}
+
 static void stage_interp(orc_state *s) {
-  orc_ib_interp(&s->L, s->n, s->mk[0], s->mk[1], s->mk[2], s->u, s->coupled, s->mk[3],
-                s->mk[4], s->mk[5]);
-  for (int i = 0; i < s->n; ++i)
+  const int n = s->n;
+  if (s->p.sem_integrator == PFG_SEM_BBK) {
+    /* elements keep their own velocity; the fluid's goes to uib */
+    double *v = (double *)xcalloc((size_t)3 * n + 1, sizeof(double));
+    orc_ib_interp(&s->L, n, s->mk[0], s->mk[1], s->mk[2], s->u, s->coupled, v, v + n,
+                  v + 2 * n);
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        if (bbk_element(s, i)) s->uib[3 * i + a] = v[(size_t)a * n + i];
+        else s->mk[3 + a][i] = v[(size_t)a * n + i];
+      }
+    free(v);
+  } else {
+    orc_ib_interp(&s->L, n, s->mk[0], s->mk[1], s->mk[2], s->u, s->coupled, s->mk[3],
+                  s->mk[4], s->mk[5]);
+  }
+  for (int i = 0; i < n; ++i)
     if (s->kind[i] == PFG_MARKER_FIXED) s->mk[3][i] = s->mk[4][i] = s->mk[5][i] = 0;
 }
 
@@ -1536,6 +1573,31 @@ static void stage_update(orc_state *s) {
       for (int a = 0; a < 3; ++a) {
         s->mk[3 + a][i] = vdt * s->mk[3 + a][i] + s->mk[6 + a][i] * fdt + sig * z[a];
         s->mk[a][i] += s->mk[3 + a][i] * dt;
+      }
+      continue;
+    }
+    if (bbk_element(s, i)) {
+      /* Langevin leapfrog (verlet_1 / verlet_3, sem_kernel.cu.orig:251-266,
+       * 547-560): two exact Ornstein-Uhlenbeck half steps of length dt/2
+       * with this step's force F,
+       *   v <- e^{-g dt/2} v + F (1 - e^{-g dt/2}) / (g m) + sqrt(T/m (1 - e^{-g dt})) xi,
+       * then x += dt ((1 - w) v + w u) with u the fluid velocity here */
+      const double g = p->sem_gamma, m = p->sem_mass, h = 0.5 * dt;
+      const double vdt = exp(-g * h);
+      const double fdt = g > 0 ? (1.0 - vdt) / g : h;
+      double za[3] = {0, 0, 0}, zb[3] = {0, 0, 0}, sig = 0;
+      if (p->sem_temp > 0 && g > 0) {
+        sig = sqrt((double)p->sem_temp / m * (1.0 - exp(-g * dt)));
+        pfg_normal3(p->seed, s->step - 1, (uint32_t)i, PFG_RNG_BBK_A, za);
+        pfg_normal3(p->seed, s->step - 1, (uint32_t)i, PFG_RNG_BBK_B, zb);
+      }
+      const double w = p->sem_fluid_weight;
+      for (int a = 0; a < 3; ++a) {
+        double v = s->mk[3 + a][i];
+        v = vdt * v + fdt * s->mk[6 + a][i] / m + sig * za[a];
+        v = vdt * v + fdt * s->mk[6 + a][i] / m + sig * zb[a];
+        s->mk[3 + a][i] = v;
+        s->mk[a][i] += dt * ((1.0 - w) * v + w * s->uib[3 * i + a]);
       }
       continue;
     }
@@ -1585,6 +1647,7 @@ double *orc_membrane(orc_state *s, int which) {
   return NULL;
 }
 int *orc_lowest(orc_state *s) { return s->lowest; }
+int64_t *orc_plt_near_cutoff(orc_state *s) { return s->plt_near_cut; }
 int *orc_rec_bond(orc_state *s) { return s->rec_bond; }
 int *orc_lig_owner(orc_state *s) { return s->lig_owner; }
 double *orc_rec_force(orc_state *s) { return s->rec_force; }
@@ -1593,10 +1656,11 @@ int *orc_cells(orc_state *s, int which, int *nb3) {
     for (int k = 0; k < 3; ++k) nb3[k] = s->nb[k];
   return which == 0 ? s->bin_start : s->items;
 }
-void orc_counters(orc_state *s, int64_t out[5]) {
+void orc_counters(orc_state *s, int64_t out[6]) {
   out[0] = s->total_bonds;
   out[1] = s->detached;
   out[2] = s->detach_step;
   out[3] = s->near;
   out[4] = s->events;
+  out[5] = s->near_cut;
 }

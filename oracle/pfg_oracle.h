@@ -107,12 +107,15 @@ double *orc_vel_field(orc_state *s);  /* 3*N velocity used for IB           */
 double *orc_marker(orc_state *s, int which); /* 0..8: x y z vx vy vz fx fy fz */
 double *orc_membrane(orc_state *s, int which); /* 0 S_all 1 V 2 tri_S 3 tri_n */
 int *orc_lowest(orc_state *s);
+/* Morse pair visits within 1e-4 of the cutoff so far, per platelet */
+int64_t *orc_plt_near_cutoff(orc_state *s);
 int *orc_rec_bond(orc_state *s);
 int *orc_lig_owner(orc_state *s);
 double *orc_rec_force(orc_state *s);
 int *orc_cells(orc_state *s, int which, int *nb3); /* 0 bin_start 1 items */
-/* counters: [0] bonds [1] detached [2] detach step [3] near-threshold [4] events */
-void orc_counters(orc_state *s, int64_t out[5]);
+/* counters: [0] bonds [1] detached [2] detach step [3] near-threshold
+ * [4] events [5] Morse pair visits within 1e-4 of the cutoff */
+void orc_counters(orc_state *s, int64_t out[6]);
 /* Bending chain on the current positions: mean curvature km (n), bending
  * force fb (3n, with the kbend of the parameters) and node normals (3n) of
  * the vertices; 0 when the meshes are not closed (then nothing is written). */

@@ -73,6 +73,7 @@ def load():
         "orc_marker": (f64p, [C.c_void_p, C.c_int]),
         "orc_membrane": (f64p, [C.c_void_p, C.c_int]),
         "orc_lowest": (i32p, [C.c_void_p]),
+        "orc_plt_near_cutoff": (C.POINTER(C.c_int64), [C.c_void_p]),
         "orc_rec_bond": (i32p, [C.c_void_p]),
         "orc_lig_owner": (i32p, [C.c_void_p]),
         "orc_rec_force": (f64p, [C.c_void_p]),
@@ -266,6 +267,13 @@ class Oracle:
             return None
         return out, area
 
+    def platelets_near_cutoff(self):
+        """Per platelet: Morse pair visits whose distance lay within 1e-4 of
+        the cutoff so far (a rounding difference can switch those pairs)."""
+        if self.n_plt == 0:
+            return np.zeros(0, np.int64)
+        return self._arr(self._lib.orc_plt_near_cutoff(self._h), self.n_plt).copy()
+
     def lowest(self):
         return self._arr(self._lib.orc_lowest(self._h), self.n_plt).copy()
 
@@ -294,10 +302,10 @@ class Oracle:
         return nb, start, items
 
     def counters(self):
-        out = (C.c_int64 * 5)()
+        out = (C.c_int64 * 6)()
         self._lib.orc_counters(self._h, out)
-        return dict(zip(("total_bonds", "detached", "detach_step", "near_threshold", "bond_events"),
-                        list(out)))
+        return dict(zip(("total_bonds", "detached", "detach_step", "near_threshold", "bond_events",
+                         "near_cutoff"), list(out)))
 
     def set_order(self, order):
         self._lib.orc_set_order(self._h, int(order))

@@ -15,6 +15,7 @@ import os
 import numpy as np
 
 from . import _ffi
+from ._ffi import (BOND_BELL, BOND_REFERENCE, SEM_ADVECT, SEM_BBK)  # noqa: F401
 from ._ffi import (FIBER_IB, FIBER_KICK, FIBER_LINEAR, FIBER_WLC, MARKER_ELEMENT,
                    MARKER_FIBER, MARKER_FIXED, SCENE_BATCH, SCENE_CHANNEL, SCENE_LARGE,
                    SCENE_SHEAR, SCENE_THROMBUS, STAGE_BONDS, STAGE_CELLS, STAGE_CLEAR,
@@ -30,7 +31,8 @@ __all__ = [
     "FIBER_IB", "FIBER_KICK", "SCENE_CHANNEL", "SCENE_SHEAR", "SCENE_THROMBUS",
     "SCENE_LARGE", "SCENE_BATCH", "STAGE_CELLS", "STAGE_MEMBRANE", "STAGE_BONDS",
     "STAGE_FORCES", "STAGE_SPREAD", "STAGE_LB", "STAGE_INTERP", "STAGE_UPDATE",
-    "STAGE_CLEAR", "read_neu", "read_fibrin",
+    "STAGE_CLEAR", "read_neu", "read_fibrin", "SEM_ADVECT", "SEM_BBK", "BOND_BELL",
+    "BOND_REFERENCE",
 ]
 
 Q = 19
@@ -159,18 +161,45 @@ class Scene:
         fp = L.pfg_scene_flags(h)
         self.flags = (np.ctypeslib.as_array(fp, shape=(self.N,)).copy() if fp else None)
         self.fluid_cells = int(L.pfg_scene_fluid_cells(h))
-        ux = np.zeros(self.N, np.float32)
-        uy = np.zeros(self.N, np.float32)
-        uz = np.zeros(self.N, np.float32)
-        _ffi.check("pfg_scene_velocity", L.pfg_scene_velocity(
-            h, _ptr(ux, C.c_float), _ptr(uy, C.c_float), _ptr(uz, C.c_float)))
-        self.u0 = (ux, uy, uz)
+        uz_ = [np.zeros(p.nz, np.float32) for _ in range(3)]
+        _ffi.check("pfg_scene_velocity_z", L.pfg_scene_velocity_z(
+            h, *[_ptr(a, C.c_float) for a in uz_]))
+        # initial flow: one value per z-plane (the scenes' flow varies along z
+        # only); change it here before loading
+        self.u0_z = tuple(uz_)
         L.pfg_scene_destroy(h)
         self._h = None
 
     @property
+    def u0(self):
+        """Initial velocity per cell ([nz][ny][nx] flattened, per component),
+        expanded from u0_z (a fresh copy on every access)."""
+        p = self.params
+        return tuple(np.repeat(a, p.nx * p.ny) for a in self.u0_z)
+
+    @property
     def n_markers(self):
         return int(self.x.size)
+
+    @classmethod
+    def large_subbox(cls, n=128, seed=0):
+        """The 1024x512x512 configuration's physics on an n^3 sub-box with the
+        same per-cell parameters: tau 0.7, Couette at the shear rate of a 0.05
+        wall velocity over 512 cells, FP16 deviation storage, and the same
+        densities of platelets (2000 x 1000 elements per 1024x512x512),
+        fibrin (80k nodes, 120k links) and wall ligands (65536 per
+        1024x512); the hard Morse cutoff of the configuration."""
+        frac = n ** 3 / (1024 * 512 * 512)
+        sc = cls(SCENE_LARGE, seed=seed, nx=n, ny=n, nz=n,
+                 n_platelets=max(1, round(2000 * frac)),
+                 fibrin_nodes=max(8, round(80000 * frac)),
+                 fibrin_links=max(8, round(120000 * frac)),
+                 ligands=max(64, round(65536 * n * n / (1024 * 512))))
+        uw = 0.05 * n / 512
+        sc.params.wall_u_hi[2][0] = uw
+        sc.params.detach_rate = 0.05 / 512
+        sc.u0_z[0][:] = uw * (np.arange(n) + 0.5) / n
+        return sc
 
     def model(self):
         """A Model struct pointing at this scene's numpy arrays (keep self alive)."""
@@ -249,6 +278,14 @@ class Solver:
         arrs = [None if a is None else f32(a).ravel() for a in (rho, ux, uy, uz)]
         self._c("pfg_init_equilibrium", *[_ptr(a, C.c_float) for a in arrs])
 
+    def init_equilibrium_z(self, ux_z=None, uy_z=None, uz_z=None):
+        """f = f_eq(1, u(z)) from one velocity value per z-plane."""
+        arrs = [None if a is None else f32(a).ravel() for a in (ux_z, uy_z, uz_z)]
+        for a in arrs:
+            if a is not None and a.size != self.params.nz:
+                raise ValueError("one value per z-plane expected")
+        self._c("pfg_init_equilibrium_z", *[_ptr(a, C.c_float) for a in arrs])
+
     def upload_flags(self, flags):
         a = np.ascontiguousarray(flags, dtype=np.uint8).ravel()
         self._c("pfg_upload_flags", _ptr(a, C.c_uint8))
@@ -290,7 +327,7 @@ class Solver:
             s = scene_or_model
             if s.flags is not None and self.params.use_flags:
                 self.upload_flags(s.flags)
-            self.init_equilibrium(None, *s.u0)
+            self.init_equilibrium_z(*s.u0_z)
 
     def set_markers(self, x, y, z):
         a = [f32(v) for v in (x, y, z)]
@@ -444,8 +481,8 @@ class Batch(Solver):
             if s.flags is not None and self.params_list[k].use_flags:
                 a = np.ascontiguousarray(s.flags, dtype=np.uint8).ravel()
                 self._c("pfg_batch_upload_flags", int(k), _ptr(a, C.c_uint8))
-            arrs = [None] + [f32(v).ravel() for v in s.u0]
-            self._c("pfg_batch_init_equilibrium", int(k), *[_ptr(a, C.c_float) for a in arrs])
+            arrs = [f32(v).ravel() for v in s.u0_z]
+            self._c("pfg_batch_init_equilibrium_z", int(k), *[_ptr(a, C.c_float) for a in arrs])
 
     def init_lattice(self, k, rho=None, ux=None, uy=None, uz=None):
         arrs = [None if a is None else f32(a).ravel() for a in (rho, ux, uy, uz)]

@@ -18,6 +18,8 @@ STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
 WALL_PERIODIC, WALL_BOUNCE, WALL_VELOCITY = 0, 1, 2
 FIBER_WLC, FIBER_LINEAR = 0, 1
 FIBER_IB, FIBER_KICK = 0, 1
+SEM_ADVECT, SEM_BBK = 0, 1
+BOND_BELL, BOND_REFERENCE = 0, 1
 MARKER_ELEMENT, MARKER_FIBER, MARKER_FIXED = 0, 1, 2
 (STAGE_CELLS, STAGE_MEMBRANE, STAGE_BONDS, STAGE_FORCES, STAGE_SPREAD, STAGE_LB,
  STAGE_INTERP, STAGE_UPDATE, STAGE_CLEAR) = range(9)
@@ -50,6 +52,9 @@ class Params(C.Structure):
         ("sem", C.c_float * 20), ("kbend", C.c_float),
         ("contact_a", C.c_float), ("contact_kappa", C.c_float), ("contact_hc", C.c_float),
         ("kick_temp", C.c_float),
+        ("sem_integrator", C.c_int32), ("sem_mass", C.c_float), ("sem_gamma", C.c_float),
+        ("sem_temp", C.c_float), ("sem_fluid_weight", C.c_float),
+        ("bond_law", C.c_int32), ("bell_kT", C.c_float),
     ]
 
 
@@ -71,7 +76,8 @@ class Model(C.Structure):
 class StepInfo(C.Structure):
     _fields_ = [("step", C.c_int64), ("total_bonds", C.c_int32), ("detached", C.c_int32),
                 ("detach_step", C.c_int64), ("near_threshold", C.c_int64),
-                ("bond_events", C.c_int64)]
+                ("bond_events", C.c_int64), ("near_cutoff", C.c_int64),
+                ("contact_overflow", C.c_int64)]
 
 
 class Tuning(C.Structure):
@@ -106,6 +112,7 @@ SIGNATURES = {
     "pfg_get_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
     "pfg_device_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "pfg_init_equilibrium": (C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p]),
+    "pfg_init_equilibrium_z": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
     "pfg_upload_f": (C.c_int, [C.c_void_p, f32p]),
     "pfg_download_f": (C.c_int, [C.c_void_p, f32p]),
     "pfg_upload_flags": (C.c_int, [C.c_void_p, u8p]),
@@ -135,6 +142,7 @@ SIGNATURES = {
                                    C.POINTER(C.c_void_p)]),
     "pfg_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "pfg_batch_init_equilibrium": (C.c_int, [C.c_void_p, C.c_int, f32p, f32p, f32p, f32p]),
+    "pfg_batch_init_equilibrium_z": (C.c_int, [C.c_void_p, C.c_int, f32p, f32p, f32p]),
     "pfg_batch_upload_flags": (C.c_int, [C.c_void_p, C.c_int, u8p]),
     "pfg_batch_load_models": (C.c_int, [C.c_void_p, C.POINTER(Model)]),
     "pfg_batch_download_fields": (C.c_int, [C.c_void_p, C.c_int, f32p, f32p, f32p, f32p]),
@@ -154,6 +162,7 @@ SIGNATURES = {
     "pfg_scene_model": (C.c_int, [C.c_void_p, C.POINTER(Model)]),
     "pfg_scene_flags": (u8p, [C.c_void_p]),
     "pfg_scene_velocity": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
+    "pfg_scene_velocity_z": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
     "pfg_scene_fluid_cells": (C.c_int64, [C.c_void_p]),
     "pfg_geom_ellipsoid": (C.c_int, [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                                      C.c_float, C.c_float, C.c_float, C.c_uint64, f32p,

@@ -81,7 +81,7 @@ void free_markers(Markers &m) {
   dfree(m.minst);
   dfree(m.x); dfree(m.y); dfree(m.z);
   dfree(m.vx); dfree(m.vy); dfree(m.vz);
-  dfree(m.fx); dfree(m.fy); dfree(m.fz);
+  dfree(m.fx); dfree(m.fy); dfree(m.fz); dfree(m.uib);
   dfree(m.kind); dfree(m.body);
   dfree(m.plt_elem_start); dfree(m.plt_tri_start);
   dfree(m.tri); dfree(m.tri_S0); dfree(m.tri_S); dfree(m.tri_V);
@@ -202,6 +202,12 @@ void pfg_params_default(pfg_params *p) {
   p->fiber_xmax = 0.9f;
   p->fiber_gamma = 1.0f;
   p->bin_cells = 3;
+  p->sem_integrator = PFG_SEM_ADVECT;
+  p->sem_mass = 1.0f;
+  p->sem_gamma = 1.0f;
+  p->sem_fluid_weight = 0.5f;
+  p->bond_law = PFG_BOND_BELL;
+  p->bell_kT = 1.0f;
 }
 
 /* hostSEMparameters index map (sem_kernel.cu:58-76, filled at
@@ -298,6 +304,13 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
       !(p.contact_hc > 0.f && p.contact_hc <= (float)p.bin_cells && p.contact_kappa > 0.f))
     return PFG_ERR_ARG;
   if (p.kick_temp < 0.f) return PFG_ERR_ARG;
+  if (p.sem_integrator != PFG_SEM_ADVECT && p.sem_integrator != PFG_SEM_BBK) return PFG_ERR_ARG;
+  if (p.sem_integrator == PFG_SEM_BBK &&
+      !(p.sem_mass > 0.f && p.sem_gamma >= 0.f && p.sem_temp >= 0.f &&
+        p.sem_fluid_weight >= 0.f && p.sem_fluid_weight <= 1.f))
+    return PFG_ERR_ARG;
+  if (p.bond_law != PFG_BOND_BELL && p.bond_law != PFG_BOND_REFERENCE) return PFG_ERR_ARG;
+  if (p.bond_law == PFG_BOND_REFERENCE && !(p.bell_kT > 0.f)) return PFG_ERR_ARG;
   int64_t ncells = 0;
   for (int k = 0; k < K; ++k) {
     const pfg_status st = check_lattice_params(ps[k]);
@@ -392,8 +405,8 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
     /* z-cells per lane of the LB kernel */
     h->lb_cpt = 1;
     h->lb_xcd_rows = (K == 1 && h->N >= XCD_ROWS_MIN_CELLS) ? 1 : 0;
-    if (dalloc(h, &h->m.counters, 8) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
-    if (hipMemset(h->m.counters, 0, sizeof(int64_t) * 8) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
+    if (dalloc(h, &h->m.counters, N_COUNTERS) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
+    if (hipMemset(h->m.counters, 0, sizeof(int64_t) * N_COUNTERS) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     if (dalloc(h, &h->m.ticket, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
     if (hipMemset(h->m.ticket, 0, sizeof(int32_t)) != hipSuccess) { st = PFG_ERR_DEVICE; break; }
     if (dalloc(h, &h->m.detach_step, 1) != hipSuccess) { st = PFG_ERR_ALLOC; break; }
@@ -459,6 +472,25 @@ pfg_status init_eq_impl(pfg_handle *h, int k, const float *rho, const float *ux,
   }
   for (int c = 0; c < 4; ++c)
     if (dev[c]) hipFree(dev[c]);
+  return st;
+}
+
+/* f = f_eq(1, u(z)) on lattice k from one velocity value per z-plane. */
+pfg_status init_eq_z_impl(pfg_handle *h, int k, const float *ux, const float *uy,
+                          const float *uz) {
+  hipSetDevice(h->device);
+  const int nz = h->inst[(size_t)k].p.nz;
+  const float *src[3] = {ux, uy, uz};
+  float *dev = nullptr;
+  pfg_status st = from_hip(hipMalloc(&dev, sizeof(float) * 3 * (size_t)nz));
+  for (int c = 0; c < 3 && st == PFG_OK; ++c)
+    if (src[c]) st = from_hip(upload(h, dev + (size_t)c * nz, src[c], (size_t)nz));
+  if (st == PFG_OK) {
+    st = from_hip(launch_init_eq(h, k, nullptr, ux ? dev : nullptr, uy ? dev + nz : nullptr,
+                                 uz ? dev + 2 * (size_t)nz : nullptr, true));
+    if (st == PFG_OK) st = from_hip(hipStreamSynchronize(h->stream));
+  }
+  if (dev) hipFree(dev);
   return st;
 }
 
@@ -555,6 +587,13 @@ pfg_status pfg_init_equilibrium(pfg_handle *h, const float *rho, const float *ux
   if (!h) return PFG_ERR_ARG;
   if (h->K != 1) return PFG_ERR_STATE;
   return init_eq_impl(h, 0, rho, ux, uy, uz);
+}
+
+pfg_status pfg_init_equilibrium_z(pfg_handle *h, const float *ux_z, const float *uy_z,
+                                  const float *uz_z) {
+  if (!h) return PFG_ERR_ARG;
+  if (h->K != 1) return PFG_ERR_STATE;
+  return init_eq_z_impl(h, 0, ux_z, uy_z, uz_z);
 }
 
 pfg_status pfg_upload_f(pfg_handle *h, const float *f) {
@@ -924,6 +963,8 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   ALLOC_FILL(m.vy, 0.0f, n);
   ALLOC_FILL(m.vz, 0.0f, n);
   ALLOC_FILL(m.fx, 0.0f, n);
+  if (h->p.sem_integrator == PFG_SEM_BBK)
+    ALLOC_FILL(m.uib, make_float4(0.f, 0.f, 0.f, 0.f), n);
   ALLOC_FILL(m.fy, 0.0f, n);
   ALLOC_FILL(m.fz, 0.0f, n);
   ALLOC_UP(m.kind, M.kind, n);
@@ -1015,8 +1056,8 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
     return from_hip(e);
   }
   /* fresh counters and step index */
-  const int64_t zeros[8] = {0, 0, 0, 0, 0, 0, 0, 0}, minus1 = -1;
-  PFG_TRY(upload(h, m.counters, zeros, 8));
+  const int64_t zeros[N_COUNTERS] = {}, minus1 = -1;
+  PFG_TRY(upload(h, m.counters, zeros, N_COUNTERS));
   PFG_TRY(upload(h, m.detach_step, &minus1, 1));
   PFG_TRY(upload(h, m.d_step, zeros, 1));
   /* the step index starts again: no tile carries a stamp or a force */
@@ -1062,6 +1103,12 @@ pfg_status pfg_batch_init_equilibrium(pfg_handle *h, int k, const float *rho, co
                                       const float *uy, const float *uz) {
   if (!h || k < 0 || k >= h->K) return PFG_ERR_ARG;
   return init_eq_impl(h, k, rho, ux, uy, uz);
+}
+
+pfg_status pfg_batch_init_equilibrium_z(pfg_handle *h, int k, const float *ux_z,
+                                        const float *uy_z, const float *uz_z) {
+  if (!h || k < 0 || k >= h->K) return PFG_ERR_ARG;
+  return init_eq_z_impl(h, k, ux_z, uy_z, uz_z);
 }
 
 pfg_status pfg_batch_upload_flags(pfg_handle *h, int k, const uint8_t *flags) {
@@ -1456,9 +1503,9 @@ static hipError_t build_graphs(pfg_handle *h) {
 
 static pfg_status fill_info(pfg_handle *h, pfg_step_info *info) {
   if (!info) return PFG_OK;
-  int64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t c[N_COUNTERS] = {};
   int64_t ds = -1;
-  PFG_TRY(download(h, c, h->m.counters, 8));
+  PFG_TRY(download(h, c, h->m.counters, N_COUNTERS));
   PFG_TRY(download(h, &ds, h->m.detach_step, 1));
   info->step = h->h_step;
   info->total_bonds = (int32_t)c[0];
@@ -1466,6 +1513,8 @@ static pfg_status fill_info(pfg_handle *h, pfg_step_info *info) {
   info->detach_step = ds;
   info->near_threshold = c[2];
   info->bond_events = c[4];
+  info->near_cutoff = c[8];
+  info->contact_overflow = c[9];
   return PFG_OK;
 }
 
@@ -1636,7 +1685,7 @@ struct Snapshot {
   size_t f_bytes = 0;
   float *mk = nullptr;   /* x, y, z, vx, vy, vz */
   int32_t *bonds = nullptr; /* rec_bond, lig_owner, lig_claim */
-  int64_t *ctr = nullptr;   /* counters[8], detach_step, d_step */
+  int64_t *ctr = nullptr;   /* counters[N_COUNTERS], detach_step, d_step */
   int cur = 0;
   int64_t h_step = 0;
 };
@@ -1674,9 +1723,9 @@ hipError_t snapshot_copy(pfg_handle *h, Snapshot &S, bool save) {
     if (e == hipSuccess) e = cp(m.lig_owner, S.bonds + nr, 4 * nl);
     if (e == hipSuccess) e = cp(m.lig_claim, S.bonds + nr + nl, 4 * nl);
   }
-  if (e == hipSuccess) e = cp(h->m.counters, S.ctr, 8 * sizeof(int64_t));
-  if (e == hipSuccess) e = cp(h->m.detach_step, S.ctr + 8, sizeof(int64_t));
-  if (e == hipSuccess) e = cp(h->m.d_step, S.ctr + 9, sizeof(int64_t));
+  if (e == hipSuccess) e = cp(h->m.counters, S.ctr, N_COUNTERS * sizeof(int64_t));
+  if (e == hipSuccess) e = cp(h->m.detach_step, S.ctr + N_COUNTERS, sizeof(int64_t));
+  if (e == hipSuccess) e = cp(h->m.d_step, S.ctr + N_COUNTERS + 1, sizeof(int64_t));
   if (!save) {
     /* no tile or row carries a stamp or a force of the steps being undone */
     if (e == hipSuccess) e = hipMemsetAsync(h->tile_stamp, 0xff, 4 * (size_t)h->ntiles, h->stream);
@@ -1706,7 +1755,7 @@ hipError_t snapshot_take(pfg_handle *h, Snapshot &S) {
   hipError_t e = hipMalloc(&S.f, S.f_bytes);
   if (e == hipSuccess) e = hipMalloc((void **)&S.mk, 6 * 4 * (n > 0 ? n : 1));
   if (e == hipSuccess) e = hipMalloc((void **)&S.bonds, 4 * (nb > 0 ? nb : 1));
-  if (e == hipSuccess) e = hipMalloc((void **)&S.ctr, 10 * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc((void **)&S.ctr, (N_COUNTERS + 2) * sizeof(int64_t));
   if (e == hipSuccess) e = snapshot_copy(h, S, true);
   return e;
 }

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <new>
 #include <sstream>
 #include <string>
@@ -173,8 +174,21 @@ pfg_status pfg_geom_ellipsoid(int n, float a, float b, float c, float cx, float 
     const double ph = golden * i;
     P[(size_t)i] = {r * std::cos(ph), r * std::sin(ph), zz};
   }
+  /* the hull of the unit-sphere points depends on n only: computed once per
+   * n and reused (a large scene builds thousands of equal platelets) */
+  static std::mutex hull_mu;
+  static std::map<int, std::vector<int>> hulls;
   std::vector<int> T;
-  if (!convex_hull(P, T)) return PFG_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> lock(hull_mu);
+    auto it = hulls.find(n);
+    if (it != hulls.end()) {
+      T = it->second;
+    } else {
+      if (!convex_hull(P, T)) return PFG_ERR_ARG;
+      hulls[n] = T;
+    }
+  }
   if ((int)T.size() != 3 * (2 * n - 4)) return PFG_ERR_ARG;
   for (int i = 0; i < n; ++i) {
     const double jx = jitter * (2.0 * urand(seed, TAG_JITTER, (uint32_t)i, 0) - 1.0);
@@ -967,6 +981,16 @@ pfg_status pfg_scene_velocity(const pfg_scene *s, float *ux, float *uy, float *u
       if (uy) uy[c] = 0.f;
       if (uz) uz[c] = 0.f;
     }
+  }
+  return PFG_OK;
+}
+
+pfg_status pfg_scene_velocity_z(const pfg_scene *s, float *ux, float *uy, float *uz) {
+  if (!s) return PFG_ERR_ARG;
+  for (int k = 0; k < s->p.nz; ++k) {
+    if (ux) ux[k] = s->couette_u * (k + 0.5f) / (float)s->p.nz;
+    if (uy) uy[k] = 0.f;
+    if (uz) uz[k] = 0.f;
   }
   return PFG_OK;
 }

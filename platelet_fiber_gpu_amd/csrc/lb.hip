@@ -404,20 +404,24 @@ __device__ inline void cell_xyz(const LBArgs &a, int64_t idx, int &x, int &y, in
   z = (int)(idx / ((int64_t)a.nx * a.ny));
 }
 
+/* per_z: the inputs hold one value per z-plane (a velocity that varies
+ * along z only, like the reference's linear shear start) instead of one per
+ * cell. */
 template <int S>
 __global__ void init_eq_kernel(LBArgs a, void *dst0, void *dst1, const float *rho_in,
                                const float *ux_in, const float *uy_in, const float *uz_in,
-                               int64_t n) {
+                               int64_t n, int per_z) {
   using ST = Store<S>;
   using T = typename ST::T;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   int x, y, z;
   cell_xyz(a, idx, x, y, z);
-  const float rho = rho_in ? rho_in[idx] : 1.0f;
-  const float ux = ux_in ? ux_in[idx] : 0.f;
-  const float uy = uy_in ? uy_in[idx] : 0.f;
-  const float uz = uz_in ? uz_in[idx] : 0.f;
+  const int64_t src = per_z ? (int64_t)z : idx;
+  const float rho = rho_in ? rho_in[src] : 1.0f;
+  const float ux = ux_in ? ux_in[src] : 0.f;
+  const float uy = uy_in ? uy_in[src] : 0.f;
+  const float uz = uz_in ? uz_in[src] : 0.f;
   const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
   const float drho = rho - 1.0f;
   const float sc = ST::scaled ? a.scale : 1.0f;
@@ -685,7 +689,7 @@ hipError_t launch_bbmask(pfg_handle *h, int k) {
 }
 
 hipError_t launch_init_eq(pfg_handle *h, int k, const float *rho, const float *ux,
-                          const float *uy, const float *uz) {
+                          const float *uy, const float *uz, bool per_z) {
   LBArgs a;
   make_lb_args(h, k, 0, &a);
   const int64_t n = h->inst[(size_t)k].N;
@@ -694,15 +698,15 @@ hipError_t launch_init_eq(pfg_handle *h, int k, const float *rho, const float *u
   switch (h->p.store) {
     case PFG_STORE_F16:
       hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F16>, grid, block, 0, h->stream, a, d0, d1,
-                         rho, ux, uy, uz, n);
+                         rho, ux, uy, uz, n, per_z ? 1 : 0);
       break;
     case PFG_STORE_BF16:
       hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_BF16>, grid, block, 0, h->stream, a, d0, d1,
-                         rho, ux, uy, uz, n);
+                         rho, ux, uy, uz, n, per_z ? 1 : 0);
       break;
     default:
       hipLaunchKernelGGL(init_eq_kernel<PFG_STORE_F32>, grid, block, 0, h->stream, a, d0, d1,
-                         rho, ux, uy, uz, n);
+                         rho, ux, uy, uz, n, per_z ? 1 : 0);
   }
   return hipGetLastError();
 }

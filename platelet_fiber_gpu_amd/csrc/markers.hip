@@ -58,6 +58,7 @@ struct MArgs {
   int64_t plane;          /* stride between the components of the force field */
   /* parameters */
   float morse_u0, morse_rho, morse_req, morse_cut2, morse_cut, morse_taper;
+  float morse_eps2;       /* |r^2 - cutoff^2| below this: near the cutoff      */
   float k_spring, ks, kv;
   float wall_a, wall_kappa, wall_hc, wall_z;
   float k_on, k_off0, bond_k, bond_l0, bond_reach, bell_xb_kT, detach_rate;
@@ -69,6 +70,10 @@ struct MArgs {
   float kbend;
   float contact_a, contact_kappa, contact_hc;
   float kick_temp;
+  int sem_integrator;
+  float sem_mass, sem_gamma, sem_temp, sem_w;
+  int bond_law;
+  float bell_kT;
   /* role split of the fused launches */
   int roles;
   int halt; /* 1: after the detach test fires, leave bonds, force field and
@@ -1186,7 +1191,10 @@ __device__ inline void morse_pair(const MArgs &a, float4 p, float tag, float xi,
                                   float zi, float &fx, float &fy, float &fz) {
   if (__float_as_int(p.w) != __float_as_int(tag)) return;
   const float dx = p.x - xi, dy = p.y - yi, dz = p.z - zi;
-  const float r2 = dx * dx + dy * dy + dz * dz;
+  /* fixed evaluation order, so that the cutoff decision is reproducible
+   * from the same FP32 positions elsewhere */
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  if (fabsf(r2 - a.morse_cut2) < a.morse_eps2) bump(a.m.counters, 8);
   if (r2 >= a.morse_cut2 || r2 == 0.f) return;
   const float r = sqrtf(r2);
   const float e = expf(-a.morse_rho * (r - a.morse_req));
@@ -1415,12 +1423,28 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
  * platelet, then lowest partner, on ties) repels with
  * a kappa e^{-kappa h} / (1 - e^{-kappa h}), h = max(distance, hc / 100),
  * here only below the cutoff hc and found on the cell lists.  First launch:
- * each element, for each other platelet with an element within hc (the
- * first four such platelets), proposes the distance to its nearest element
- * there; the key (distance bits, element of the lower platelet) goes to the
- * pair's slot by atomicMin (distances >= 0 order as their bits), and the
- * first proposal of a pair appends the pair to the list. */
+ * each element, for each other platelet with an element within hc,
+ * proposes the distance to its nearest element there; the key (distance
+ * bits, element of the lower platelet) goes to the pair's slot by atomicMin
+ * (distances >= 0 order as their bits), and the first proposal of a pair
+ * appends the pair to the list.  An element reduces its candidates over up
+ * to four other platelets in registers; a candidate of a fifth is proposed
+ * on its own (any candidate pair is a valid proposal, so the result is the
+ * same) and counted in counters[9]. */
 constexpr uint64_t CONTACT_EMPTY = ~(uint64_t)0;
+
+__device__ inline void contact_propose(const MArgs &a, int p, int q, int i, int j, float d2) {
+  const int lo = min(p, q), hi = max(p, q);
+  const uint32_t elem_lo = (uint32_t)(p < q ? i : j);
+  const uint64_t key = ((uint64_t)__float_as_uint(sqrtf(d2)) << 32) | elem_lo;
+  const int64_t pair = (int64_t)lo * a.m.n_plt + hi;
+  const uint64_t old = atomicMin((unsigned long long *)&a.m.contact_slot[pair],
+                                 (unsigned long long)key);
+  if (old == CONTACT_EMPTY) {
+    const int idx = (int)atomicAdd((unsigned long long *)&a.m.counters[7], 1ull);
+    a.m.contact_list[idx] = (int32_t)pair;
+  }
+}
 
 __global__ __launch_bounds__(256) void contact_pairs_kernel(MArgs a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -1442,6 +1466,7 @@ __global__ __launch_bounds__(256) void contact_pairs_kernel(MArgs a) {
       const float d2 = dx * dx + dy * dy + dz * dz;
       if (d2 >= hc2) continue;
       const int j = a.m.bin_items[c];
+      bool held = false;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (qb[k] == tag) {
@@ -1449,31 +1474,26 @@ __global__ __launch_bounds__(256) void contact_pairs_kernel(MArgs a) {
             qd[k] = d2;
             qj[k] = j;
           }
+          held = true;
           break;
         }
         if (qb[k] < 0) {
           qb[k] = tag;
           qd[k] = d2;
           qj[k] = j;
+          held = true;
           break;
         }
+      }
+      if (!held) { /* a fifth platelet: propose this pair on its own */
+        contact_propose(a, p, tag, i, j, d2);
+        bump(a.m.counters, 9);
       }
     }
   });
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (qb[k] < 0) continue;
-    const int q = qb[k], lo = min(p, q), hi = max(p, q);
-    const uint32_t elem_lo = (uint32_t)(p < q ? i : qj[k]);
-    const uint64_t key = ((uint64_t)__float_as_uint(sqrtf(qd[k])) << 32) | elem_lo;
-    const int64_t pair = (int64_t)lo * a.m.n_plt + hi;
-    const uint64_t old = atomicMin((unsigned long long *)&a.m.contact_slot[pair],
-                                   (unsigned long long)key);
-    if (old == CONTACT_EMPTY) {
-      const int idx = (int)atomicAdd((unsigned long long *)&a.m.counters[7], 1ull);
-      a.m.contact_list[idx] = (int32_t)pair;
-    }
-  }
+  for (int k = 0; k < 4; ++k)
+    if (qb[k] >= 0) contact_propose(a, p, qb[k], i, qj[k], qd[k]);
 }
 
 /* Second launch, after the force pass: each listed pair takes its winning
@@ -1793,9 +1813,50 @@ __device__ void interp_marker(const MArgs &a, int i, int j, float *v) {
   v[2] = uz;
 }
 
+/* Langevin leapfrog of a platelet element (the BBK half kicks of verlet_1 /
+ * verlet_3, sem_kernel.cu.orig:251-266, 547-560): two exact
+ * Ornstein-Uhlenbeck half steps with this step's force, then the drift
+ * with the weighted fluid velocity u.  The velocity is kept at half steps. */
+__device__ void bbk_element(const MArgs &a, int i, float ux, float uy, float uz) {
+  const float dt = a.marker_dt, g = a.sem_gamma, m = a.sem_mass;
+  const float vdt = expf(-0.5f * g * dt);
+  const float fdt = g > 0.f ? (1.0f - vdt) / g : 0.5f * dt;
+  const float fm = fdt / m;
+  float v[3] = {a.m.vx[i], a.m.vy[i], a.m.vz[i]};
+  const float F[3] = {a.m.fx[i], a.m.fy[i], a.m.fz[i]};
+  double za[3] = {0.0, 0.0, 0.0}, zb[3] = {0.0, 0.0, 0.0};
+  float sig = 0.f;
+  if (a.sem_temp > 0.f && g > 0.f) {
+    sig = (float)sqrt((double)a.sem_temp / m * (1.0 - exp(-(double)g * dt)));
+    const int64_t step = a.m.counters[6];
+    pfg_normal3(a.seed, step, (uint32_t)i, PFG_RNG_BBK_A, za);
+    pfg_normal3(a.seed, step, (uint32_t)i, PFG_RNG_BBK_B, zb);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    v[k] = fmaf(vdt, v[k], fm * F[k]) + sig * (float)za[k];
+    v[k] = fmaf(vdt, v[k], fm * F[k]) + sig * (float)zb[k];
+  }
+  const float w = a.sem_w, wv = 1.0f - w;
+  a.m.vx[i] = v[0];
+  a.m.vy[i] = v[1];
+  a.m.vz[i] = v[2];
+  a.m.x[i] += dt * (wv * v[0] + w * ux);
+  a.m.y[i] += dt * (wv * v[1] + w * uy);
+  a.m.z[i] += dt * (wv * v[2] + w * uz);
+}
+
+__device__ inline bool bbk_kind(const MArgs &a, int kind) {
+  return kind == PFG_MARKER_ELEMENT && a.sem_integrator == PFG_SEM_BBK;
+}
+
 __device__ void update_marker(const MArgs &a, int i, float vx, float vy, float vz) {
   const int kind = a.m.kind[i];
   if (kind == PFG_MARKER_FIXED) return;
+  if (bbk_kind(a, kind)) {
+    bbk_element(a, i, vx, vy, vz);
+    return;
+  }
   const float dt = a.marker_dt;
   if (kind == PFG_MARKER_FIBER && a.fiber_move == PFG_FIBER_KICK) {
     /* v <- e^{-g dt} v + F (1 - e^{-g dt})/g */
@@ -1835,17 +1896,27 @@ __global__ __launch_bounds__(256) void move_kernel(MArgs a) {
   if (i >= a.m.n) return;
   const int kind = a.m.kind[i];
   float v[3] = {0.f, 0.f, 0.f};
+  const bool bbk = bbk_kind(a, kind);
   if (a.roles & ROLE_A) {
     if (ib_coupled(kind, a.fiber_move)) interp_marker(a, i, lane, v);
-    if (lane == 0 && (ib_coupled(kind, a.fiber_move) || kind == PFG_MARKER_FIXED)) {
+    if (lane == 0 && bbk) { /* the element keeps its own velocity */
+      a.m.uib[i] = make_float4(v[0], v[1], v[2], 0.f);
+    } else if (lane == 0 && (ib_coupled(kind, a.fiber_move) || kind == PFG_MARKER_FIXED)) {
       a.m.vx[i] = v[0];
       a.m.vy[i] = v[1];
       a.m.vz[i] = v[2];
     }
   } else if (lane == 0) {
-    v[0] = a.m.vx[i];
-    v[1] = a.m.vy[i];
-    v[2] = a.m.vz[i];
+    if (bbk) {
+      const float4 u = a.m.uib[i];
+      v[0] = u.x;
+      v[1] = u.y;
+      v[2] = u.z;
+    } else {
+      v[0] = a.m.vx[i];
+      v[1] = a.m.vy[i];
+      v[2] = a.m.vz[i];
+    }
   }
   if ((a.roles & ROLE_B) && lane == 0) update_marker(a, i, v[0], v[1], v[2]);
 }
@@ -1887,6 +1958,7 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->morse_cut2 = p.morse_cutoff * p.morse_cutoff;
   a->morse_cut = p.morse_cutoff;
   a->morse_taper = p.morse_taper;
+  a->morse_eps2 = 2.0f * p.morse_cutoff * MORSE_EPS_R;
   a->k_spring = p.k_spring;
   a->ks = p.ks;
   a->kv = p.kv;
@@ -1916,6 +1988,13 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->contact_kappa = p.contact_kappa;
   a->contact_hc = p.contact_hc;
   a->kick_temp = p.kick_temp;
+  a->sem_integrator = p.sem_integrator;
+  a->sem_mass = p.sem_mass;
+  a->sem_gamma = p.sem_gamma;
+  a->sem_temp = p.sem_temp;
+  a->sem_w = p.sem_fluid_weight;
+  a->bond_law = p.bond_law;
+  a->bell_kT = p.bell_kT;
   a->roles = 0;
   a->halt = h->halt_on_detach ? 1 : 0;
   a->mark_rows = h->overlap && h->row_stamp != nullptr ? 1 : 0;

@@ -42,6 +42,11 @@ constexpr int SPAN_COUNT_MAX = 32768;
  * contiguous range of rows by default (faster at 268 M cells, slower at
  * 4 M; the autotuner tries both). */
 constexpr int64_t XCD_ROWS_MIN_CELLS = (int64_t)1 << 27;
+/* Device counters of a handle (Markers::counters). */
+constexpr int N_COUNTERS = 16;
+/* Morse pairs whose distance lies within this of the cutoff are counted: a
+ * rounding difference can switch them. */
+constexpr float MORSE_EPS_R = 1e-4f;
 
 /* Everything the fused LB kernel needs, passed by value. */
 struct LBArgs {
@@ -105,6 +110,9 @@ struct Markers {
   float4 *pos4 = nullptr;        /* x, y, z, Morse tag per marker, this step */
   float *vx = nullptr, *vy = nullptr, *vz = nullptr;
   float *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  /* PFG_SEM_BBK: the fluid velocity interpolated at each marker (the
+   * element's own velocity stays in vx, vy, vz) */
+  float4 *uib = nullptr;
   int32_t *kind = nullptr, *body = nullptr;
   /* platelets */
   int n_plt = 0;
@@ -181,9 +189,11 @@ struct Markers {
    * pairs proposed this step */
   uint64_t *contact_slot = nullptr;
   int32_t *contact_list = nullptr;
-  /* counters: [0] total bonds, [1] detached, [2] near-threshold lo, [3] hi,
-   * [4] events lo, [5] events hi, [6] the step index recorded by the first
-   * launch of the step, [7] platelet pairs in contact this step */
+  /* counters (N_COUNTERS): [0] total bonds, [1] detached, [2] near-threshold
+   * lo, [3] hi, [4] events lo, [5] events hi, [6] the step index recorded by
+   * the first launch of the step, [7] platelet pairs in contact this step,
+   * [8] Morse pair visits within MORSE_EPS_R of the cutoff, [9] contact
+   * candidates beyond the per-element platelet slots */
   int64_t *counters = nullptr;
   int32_t *ticket = nullptr;      /* workgroups of the gather launch done       */
   int64_t *detach_step = nullptr;
@@ -277,8 +287,9 @@ hipError_t launch_lb_part(pfg_handle *h, int part, hipStream_t s);
 hipError_t launch_bbmask(pfg_handle *h, int k);
 hipError_t launch_lb_fields(pfg_handle *h, int k, float *rho, float *ux, float *uy,
                             float *uz);
+/* per_z: the inputs hold one value per z-plane of lattice k */
 hipError_t launch_init_eq(pfg_handle *h, int k, const float *rho, const float *ux,
-                          const float *uy, const float *uz);
+                          const float *uy, const float *uz, bool per_z = false);
 hipError_t launch_convert_f(pfg_handle *h, float *f_out_or_null,
                             const float *f_in_or_null);
 hipError_t launch_conservation(pfg_handle *h, int k, double out[4]);

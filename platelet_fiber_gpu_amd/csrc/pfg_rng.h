@@ -70,7 +70,14 @@ PFG_HD float pfg_u01_from_u32(uint32_t r) {
 }
 
 /* Streams drawn per (step, item). */
-enum { PFG_RNG_BOND_OFF = 0, PFG_RNG_BOND_ON = 1, PFG_RNG_DETACH = 2, PFG_RNG_KICK = 3 };
+enum {
+  PFG_RNG_BOND_OFF = 0,
+  PFG_RNG_BOND_ON = 1,
+  PFG_RNG_DETACH = 2,
+  PFG_RNG_KICK = 3,
+  PFG_RNG_BBK_A = 4, /* first half kick of a step (completes the previous one) */
+  PFG_RNG_BBK_B = 5  /* second half kick of a step                           */
+};
 
 /* One uniform for (seed, step, item, stream). */
 PFG_HD float pfg_uniform(uint64_t seed, int64_t step, uint32_t item,

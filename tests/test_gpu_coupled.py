@@ -26,6 +26,67 @@ def rel_l2(a, b):
     return np.sqrt(((a - b) ** 2).sum()) / max(np.sqrt((b ** 2).sum()), 1e-300)
 
 
+def receptor_platelets(sc):
+    """Platelet of each receptor (through its triangle)."""
+    return np.searchsorted(sc.plt_tri_start, sc.rec_tri, side="right") - 1
+
+
+def excluded_platelets(sc, o, gpu_bonds=None):
+    """Platelets whose elements are not compared one by one: those with a
+    receptor whose bond differs between the runs (a decision within epsilon
+    of its threshold, counted) and those with a Morse pair that came within
+    1e-4 of the cutoff (a rounding difference can switch its force)."""
+    npl = sc.plt_S0.size
+    out = np.zeros(npl, bool)
+    near = o.platelets_near_cutoff()
+    if near.size:
+        out |= near > 0
+    if gpu_bonds is not None and sc.rec_tri.size:
+        bad = gpu_bonds != o.bonds()[0]
+        out[np.unique(receptor_platelets(sc)[bad])] = True
+    return out
+
+
+def compared_markers(sc, excl):
+    """Markers held to the tight tolerance: fibrin nodes and the elements of
+    the platelets not excluded."""
+    body = sc.body
+    keep = np.ones(body.size, bool)
+    el = body >= 0
+    keep[el] = ~excl[body[el]]
+    return keep
+
+
+def check_positions(pos, opos, keep, tight=1e-4, loose=3e-3):
+    """Every marker is compared: those in `keep` within `tight`, the others
+    (platelets with a counted near-threshold event, where one switched pair
+    force moves the platelet by up to ~1e-3 within a few steps) within
+    `loose`.  Returns the two maxima."""
+    err = np.abs(pos - opos).max(axis=0)
+    e_keep = float(err[keep].max()) if keep.any() else 0.0
+    e_rest = float(err[~keep].max()) if (~keep).any() else 0.0
+    assert e_keep < tight, e_keep
+    assert e_rest < loose, e_rest
+    return e_keep, e_rest
+
+
+def cells_away_from(sc, pts, radius):
+    """Boolean mask over the lattice cells farther than `radius` (per axis,
+    periodic where the lattice is) from every point of pts (n x 3)."""
+    p = sc.params
+    dims = (p.nx, p.ny, p.nz)
+    near = np.zeros((p.nz, p.ny, p.nx), bool)
+    for q in pts:
+        idx = []
+        for a in range(3):
+            c = int(np.floor(q[a]))
+            r = np.arange(c - radius, c + radius + 2)
+            r = r % dims[a] if p.wall[a] == 0 else r[(r >= 0) & (r < dims[a])]
+            idx.append(r)
+        near[np.ix_(idx[2], idx[1], idx[0])] = True
+    return ~near.ravel()
+
+
 def test_channel_platelet_short_horizon(gpu, po):
     """64x32x32 channel with the 256-element Morse platelet, 50 steps: the
     FP32 trajectory follows the FP64 one element by element."""
@@ -59,12 +120,47 @@ def test_headline_config_full_size_short_horizon(gpu, po):
     ob = o.bonds()[0]
     mism = int((g["bonds"] != ob).sum())
     near = o.counters()["near_threshold"]
+    excl = excluded_platelets(sc, o, g["bonds"])
+    keep = compared_markers(sc, excl)
     print(f"headline config, 5 steps: rel L2 u {err_u:.2e}, max position error {err_x:.2e}, "
-          f"bonds {int((ob >= 0).sum())}, mismatches {mism} (near threshold {near})")
+          f"bonds {int((ob >= 0).sum())}, mismatches {mism} (near threshold {near}), "
+          f"platelets with a near-threshold event {int(excl.sum())} "
+          f"(near-cutoff pair visits {o.counters()['near_cutoff']}, device "
+          f"{g['info']['near_cutoff']})")
     assert err_u < 1e-5
     assert mism <= near
-    if mism == 0:  # coordinates near 100: a few FP32 ulps
-        assert err_x < 1e-4
+    check_positions(pos, opos, keep)  # coordinates near 100: a few FP32 ulps
+
+
+def test_headline_config_variant_fast_binding_full_size(gpu, po):
+    """Variant of the headline configuration with a binding rate of 0.5 per
+    step instead of 1e-3 (everything else as configured: 256x128x128
+    Couette, one 2048-element platelet with 512 receptors half a cell above
+    4096 wall ligands, FP32 storage), so that bonds form within the short
+    horizon: 6 coupled steps against the FP64 oracle, every bond (exact
+    except decisions within epsilon of their threshold, counted), the bond
+    forces through the element positions, and the flow."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_SHEAR)
+    sc.params.k_on = 0.5
+    g, o = run_both(pfg, po, sc, 6)
+    ob = o.bonds()[0]
+    nb = int((ob >= 0).sum())
+    mism = int((g["bonds"] != ob).sum())
+    near = o.counters()["near_threshold"]
+    ofx = po.fields(po.lattice(sc.params), o.f())
+    err_u = rel_l2(np.stack(g["u"]), np.stack(ofx[1:]))
+    pos = np.stack([g["mk"][k] for k in ("x", "y", "z")])
+    opos = np.stack([o.marker(k) for k in ("x", "y", "z")])
+    keep = compared_markers(sc, excluded_platelets(sc, o, g["bonds"]))
+    assert nb >= 5
+    assert mism <= near
+    assert err_u < 1e-5
+    e_keep, e_rest = check_positions(pos, opos, keep)
+    print(f"fast-binding headline variant, 6 steps: bonds {nb} (events "
+          f"{o.counters()['bond_events']}), mismatches {mism} (near threshold {near}), "
+          f"rel L2 u {err_u:.2e}, position errors {e_keep:.2e} (tight set, "
+          f"{int(keep.sum())} of {keep.size}) / {e_rest:.2e}")
 
 
 def test_channel_platelet_config(gpu, po):
@@ -132,8 +228,7 @@ def test_shear_with_bonds_short(gpu, po):
     assert mism <= near
     pos = np.stack([g["mk"][k] for k in ("x", "y", "z")])
     opos = np.stack([o.marker(k) for k in ("x", "y", "z")])
-    if mism == 0:
-        assert np.abs(pos - opos).max() < 1e-4
+    check_positions(pos, opos, compared_markers(sc, excluded_platelets(sc, o, g["bonds"])))
 
 
 @pytest.mark.parametrize("staging", [1, 2])
@@ -157,36 +252,44 @@ def test_thrombus_small_flags_fibrin(gpu, po, staging):
 
 
 def test_thrombus_config_full_size_two_steps(gpu, po):
-    """The tube configuration at full size (512x256x256 flag tube of radius
-    120, body force, 500 platelets x 1000 elements, fibrin 20k nodes / 30k
-    links, FP32 storage), 2 coupled steps against the FP64 oracle: the
-    fluid-cell deviations and every marker.  The Morse force is tapered to
-    zero at its cutoff here: among 500 000 elements some pair always sits
-    within rounding of a hard cutoff, and its force switching on in one
-    run and off in the other moves that platelet by ~1e-4 within two
-    steps (measured), which would hide the arithmetic compared here."""
+    """The tube configuration at full size with its own physics (512x256x256
+    flag tube of radius 120, body force, 500 platelets x 1000 elements with
+    the hard Morse cutoff, fibrin 20k nodes / 30k links, FP32 storage), 2
+    coupled steps against the FP64 oracle.  Both take the Morse cutoff
+    decision from the same FP32 arithmetic, so they switch the same pairs
+    while the positions agree; a pair within 1e-4 of the cutoff can still
+    switch on a rounding difference: the oracle counts those per platelet,
+    and such platelets (and those with a bond decision that differs) are
+    left out of the element-wise comparison, as are the fluid cells within
+    four cells of them.  Everything else is compared: every other element
+    and fibrin node, every other fluid cell, every bond."""
     pfg = gpu
     sc = pfg.Scene(pfg.SCENE_THROMBUS)
     assert sc.N == 512 * 256 * 256 and sc.n_markers == 520000
-    sc.params.morse_taper = 0.3
+    assert sc.params.morse_taper == 0.0
     g, o = run_both(pfg, po, sc, 2)
-    fluid = sc.flags == 0
-    w = np.array([po.load().orc_w(d) for d in range(Q)])[:, None]
-    d = 0.0
-    for q in range(Q):  # one direction at a time keeps the host memory low
-        d = max(d, float(np.abs((g["f"][q] - w[q]) - (o.f()[q] - w[q]))[fluid].max()))
     pos = np.stack([g["mk"][k] for k in ("x", "y", "z")])
     opos = np.stack([o.marker(k) for k in ("x", "y", "z")])
-    err_x = np.abs(pos - opos).max()
     ob = o.bonds()[0]
     mism = int((g["bonds"] != ob).sum())
     near = o.counters()["near_threshold"]
-    print(f"tube config, 2 steps: max f-dev error {d:.2e}, max position error {err_x:.2e}, "
-          f"bonds {int((ob >= 0).sum())}, mismatches {mism} (near threshold {near})")
+    excl = excluded_platelets(sc, o, g["bonds"])
+    keep = compared_markers(sc, excl)
+    away = (sc.flags == 0) & cells_away_from(sc, opos[:, ~keep].T, 4)
+    w = np.array([po.load().orc_w(d) for d in range(Q)])[:, None]
+    d = 0.0
+    for q in range(Q):  # one direction at a time keeps the host memory low
+        d = max(d, float(np.abs((g["f"][q] - w[q]) - (o.f()[q] - w[q]))[away].max()))
+    print(f"tube config, 2 steps: max f-dev error {d:.2e} over {int(away.sum())} fluid "
+          f"cells, bonds {int((ob >= 0).sum())}, mismatches {mism} (near threshold {near}), "
+          f"platelets with a near-threshold event {int(excl.sum())} of {excl.size} "
+          f"(near-cutoff pair visits: oracle {o.counters()['near_cutoff']}, device "
+          f"{g['info']['near_cutoff']})")
     assert mism <= near
-    if mism == 0:
-        assert d < 1e-5
-        assert err_x < 1e-4
+    assert excl.sum() <= 0.2 * excl.size
+    assert d < 1e-5
+    e_keep, e_rest = check_positions(pos, opos, keep)
+    print(f"  position errors {e_keep:.2e} ({int(keep.sum())} markers) / {e_rest:.2e}")
 
 
 def test_thrombus_fibrin_from_file(gpu, po):
@@ -207,44 +310,89 @@ def test_thrombus_fibrin_from_file(gpu, po):
     assert err < 1e-4
 
 
-def test_large_physics_subbox_fp16(gpu, po):
-    """The large configuration's physics (Couette at the shear rate of 0.05
-    over 512 cells, tau 0.7, FP16 deviation storage, platelets, fibrin and
-    wall ligands at its densities) on a 64^3 sub-box, 100 steps against the
-    FP64 oracle: velocity within the FP16 tolerance 1e-3 (relative L2),
-    density, and platelet centroids.  The Morse force is tapered to zero at
-    its cutoff here: with the hard cutoff a pair that crosses it on rounding
-    switches its force on or off, and the FP32 run alone then differs from
-    the FP64 one by about 5e-4 in velocity, which would hide the storage
-    error this test measures (an FP16-rounded oracle run gives 4e-4 with the
-    hard cutoff)."""
+def test_large_config_subbox_128_fp16_hard_cutoff(gpu, po):
+    """The 1024x512x512 configuration's own physics (hard Morse cutoff, FP16
+    deviation storage) on a 128^3 sub-box at its densities (16 platelets,
+    fibrin, wall ligands), 50 coupled steps against the FP64 oracle:
+    velocity relative L2 within the FP16 tolerance 1e-3, density, platelet
+    centroids, and the same with FP32 storage.  50 steps: with the hard
+    cutoff a pair that crosses it a step apart in two runs moves its
+    platelet, and such differences grow along the trajectory -- an FP64 run
+    of this scene with its positions perturbed by 1e-6 departs from the
+    unperturbed one by ~1e-4 in velocity within 50 steps and ~5e-3 within
+    75; measured here, the FP16 and FP32 stores give 6.6e-4 and 5.9e-4 at
+    50 steps and both ~2.5e-3 at 200, so beyond ~100 steps the comparison
+    measures the trajectory, not the storage.  Near-cutoff pair visits are
+    counted on both sides."""
     pfg = gpu
-    n = 64
-    sc = pfg.Scene(pfg.SCENE_LARGE, nx=n, ny=n, nz=n, n_platelets=2, fibrin_nodes=80,
-                   fibrin_links=120, ligands=512)
-    assert sc.params.store == pfg.STORE_F16
-    sc.params.morse_taper = 0.4
-    uw = 0.05 * n / 512
-    sc.params.wall_u_hi[2][0] = uw
-    sc.u0[0][:] = uw * (np.arange(sc.N) // (n * n) + 0.5) / n
-    g, o = run_both(pfg, po, sc, 100)
+    sc = pfg.Scene.large_subbox(128)
+    assert sc.params.store == pfg.STORE_F16 and sc.params.morse_taper == 0.0
+    npl = int(sc.plt_S0.size)
+    steps = 50
+    dev = {}
+    for store in (pfg.STORE_F16, pfg.STORE_F32):
+        sc.params.store = store
+        with pfg.Solver(sc.params) as s:
+            s.load(sc)
+            info = s.step(steps)
+            dev[store] = (s.fields(), s.markers(), info)
+    sc.params.store = pfg.STORE_F16
+    o = po.Oracle(sc.params, scene=sc)
+    o.step(steps)
     orho, ox, oy, oz = po.fields(po.lattice(sc.params), o.f())
-    err_u = rel_l2(np.stack(g["u"]), np.stack([ox, oy, oz]))
-    err_rho = np.abs(g["rho"] - orho).max()
-    body = sc.body
-    pos = np.stack([g["mk"][k] for k in ("x", "y", "z")], 1)
     opos = np.stack([o.marker(k) for k in ("x", "y", "z")], 1)
-    x0 = np.stack([sc.x, sc.y, sc.z], 1).astype(np.float64)
-    dcen, disp = [], []
-    for b in range(2):
-        sel = body == b
-        dcen.append(np.linalg.norm(pos[sel].mean(0) - opos[sel].mean(0)))
-        disp.append(np.linalg.norm(opos[sel].mean(0) - x0[sel].mean(0)))
-    print(f"64^3 large-physics sub-box FP16, 100 steps: rel L2 u {err_u:.2e}, max |drho| "
-          f"{err_rho:.2e}, centroid errors {dcen} (displacements {disp})")
-    assert err_u < 1e-3
-    assert err_rho < 5e-4
-    assert max(dcen) < 1e-2 * max(max(disp), 0.1)
+    body = sc.body
+    ocen = np.array([opos[body == b].mean(0) for b in range(npl)])
+    res = {}
+    for store, tag in ((pfg.STORE_F16, "FP16"), (pfg.STORE_F32, "FP32")):
+        (rho, ux, uy, uz), mk, info = dev[store]
+        pos = np.stack([mk[k] for k in ("x", "y", "z")], 1).astype(np.float64)
+        cen = np.array([pos[body == b].mean(0) for b in range(npl)])
+        res[tag] = (rel_l2(np.stack([ux, uy, uz]), np.stack([ox, oy, oz])),
+                    rel_l2(rho, orho), float(np.linalg.norm(cen - ocen, axis=1).max()),
+                    info["near_cutoff"])
+        print(f"128^3 sub-box, {steps} steps, {tag} store: rel L2 u {res[tag][0]:.2e}, "
+              f"rho {res[tag][1]:.2e}, centroid max {res[tag][2]:.2e}, device near-cutoff "
+              f"pair visits {res[tag][3]}")
+    print(f"  oracle near-cutoff pair visits {o.counters()['near_cutoff']} over "
+          f"{int((o.platelets_near_cutoff() > 0).sum())} of {npl} platelets")
+    for tag in ("FP16", "FP32"):
+        err_u, err_rho, err_cen, _ = res[tag]
+        assert err_u < 1e-3, tag
+        assert err_rho < 2e-5, tag
+        assert err_cen < 5e-4, tag
+
+
+def test_large_config_full_size_conservation_and_profile(gpu):
+    """The 1024x512x512 configuration at full size on the device (268M cells,
+    FP16 storage, 2000 platelets x 1000 elements, fibrin 80k / 120k, 65536
+    wall ligands), 20 coupled steps: FP64 totals of mass (rounding to FP16
+    is unbiased: the relative drift stays near 1e-9), the Couette profile of
+    the plane-averaged velocity, and every marker finite and inside the
+    lattice."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_LARGE)
+    p = sc.params
+    assert sc.N == 1024 * 512 * 512 and p.store == pfg.STORE_F16
+    with pfg.Solver(p) as s:
+        s.load(sc)
+        c0 = s.conservation()
+        info = s.step(20)
+        c1 = s.conservation()
+        _, ux, _, _ = s.fields()
+        mk = s.markers()
+    drift = abs(c1[0] - c0[0]) / c0[0]
+    prof = ux.reshape(p.nz, p.ny * p.nx).mean(axis=1, dtype=np.float64)
+    exact = 0.05 * (np.arange(p.nz) + 0.5) / p.nz
+    dev_prof = np.abs(prof - exact).max()
+    pos = np.stack([mk[k] for k in ("x", "y", "z")])
+    print(f"1024x512x512 FP16, 20 steps: relative mass drift {drift:.2e}, Couette profile "
+          f"max deviation {dev_prof:.2e} (wall velocity 0.05), bonds {info['total_bonds']}, "
+          f"near-cutoff pair visits {info['near_cutoff']}")
+    assert drift < 1e-7
+    assert dev_prof < 1e-3
+    assert np.isfinite(pos).all()
+    assert pos[2].min() > -0.5 and pos[2].max() < p.nz - 0.5
 
 
 def test_forces_with_spreading_equals_separate_passes(gpu):
@@ -302,10 +450,11 @@ def test_autotune_restores_the_state(gpu):
     # a bond decision within epsilon of its threshold may flip with the
     # summation order; those are counted
     assert mism <= max(info["near_threshold"], ref_info["near_threshold"])
-    if mism == 0:
-        err = max(np.abs(got[k] - ref[k]).max() for k in ("x", "y", "z"))
-        assert err < 1e-4
-        assert np.abs(u - ref_u).max() < 1e-6
+    # one platelet: a flipped bond moves it (its bond force switches), so the
+    # tolerance widens only then
+    err = max(np.abs(got[k] - ref[k]).max() for k in ("x", "y", "z"))
+    assert err < (1e-4 if mism == 0 else 3e-3)
+    assert np.abs(u - ref_u).max() < (1e-6 if mism == 0 else 1e-4)
 
 
 @pytest.mark.parametrize("graph", [True, False])

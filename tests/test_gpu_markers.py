@@ -251,3 +251,87 @@ def test_kick_noise_parity(gpu, po):
     ov = np.stack([o.marker(k) for k in ("vx", "vy", "vz")], 1)[fib]
     assert np.abs(ov).max() > 0
     assert np.abs(v - ov).max() < 1e-4 * np.abs(ov).max()
+
+
+@pytest.mark.parametrize("staged", [True, False])
+def test_bbk_elements_parity(gpu, po, staged):
+    """Platelet elements on the Langevin leapfrog (BBK half kicks with
+    thermal noise from the counter RNG, drift with half the fluid velocity,
+    as verlet_1 couples them): 12 steps against the oracle, stage by stage
+    (the fluid velocity kept apart from the elements' own) or as whole
+    graph steps.  Morse is off so that no pair switches at its cutoff."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_CHANNEL)
+    p = sc.params
+    p.morse_u0 = 0.0
+    p.k_spring = 0.3
+    p.sem_integrator, p.sem_mass, p.sem_gamma = pfg.SEM_BBK, 1.5, 0.4
+    p.sem_temp, p.sem_fluid_weight = 1e-5, 0.5
+    steps = 12
+    s, o = pair(pfg, po, sc, p)
+    with s:
+        if staged:
+            for _ in range(steps):
+                for st in (pfg.STAGE_CELLS, pfg.STAGE_MEMBRANE, pfg.STAGE_BONDS,
+                           pfg.STAGE_FORCES, pfg.STAGE_SPREAD, pfg.STAGE_LB,
+                           pfg.STAGE_INTERP, pfg.STAGE_UPDATE):
+                    s.stage(st)
+                    o.stage(st)
+        else:
+            s.step(steps)
+            o.step(steps)
+        mk = s.markers()
+    pos = np.stack([mk[k] for k in ("x", "y", "z")], 1)
+    opos = np.stack([o.marker(k) for k in ("x", "y", "z")], 1)
+    v = np.stack([mk[k] for k in ("vx", "vy", "vz")], 1)
+    ov = np.stack([o.marker(k) for k in ("vx", "vy", "vz")], 1)
+    x0 = np.stack([sc.x, sc.y, sc.z], 1)
+    print(f"BBK {steps} steps ({'stages' if staged else 'graph'}): max position error "
+          f"{np.abs(pos - opos).max():.2e} (moved {np.abs(opos - x0).max():.2e}), max "
+          f"velocity error {np.abs(v - ov).max():.2e} of {np.abs(ov).max():.2e}")
+    assert np.abs(ov).max() > 1e-4  # thermal and spring velocities, not the fluid's
+    assert np.abs(pos - opos).max() < 2e-5
+    assert np.abs(v - ov).max() < 1e-4 * np.abs(ov).max()
+
+
+def test_contact_dense_cluster_parity(gpu, po):
+    """Seven platelets packed around a central one, each within the contact
+    cutoff of it: an element of the centre can have more than four other
+    platelets within reach, and every closest pair must still be found (the
+    oracle by brute force over all pairs)."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=64, ny=64, nz=64, n_platelets=7, elements=120,
+                   receptors=-1, ligands=-1, fibrin_nodes=-1)
+    p = sc.params
+    p.morse_u0 = 0.0
+    p.contact_a, p.contact_kappa, p.contact_hc = 0.01, 6.0, 2.0
+    P = np.stack([sc.x, sc.y, sc.z], 1).astype(np.float64)
+    starts = sc.plt_elem_start
+    centre = np.array([32.0, 32.0, 32.0])
+    dirs = [np.zeros(3)] + [np.array(d, float) for d in
+                            ([1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1],
+                             [0.6, 0.6, -0.5])]
+    out = []
+    for q in range(7):
+        A = P[starts[q]:starts[q + 1]]
+        A = (A - A.mean(0)) * 0.35  # small platelets: all within reach of the centre
+        out.append(A + centre + 2.2 * dirs[q])
+    Q = np.concatenate(out)
+    x, y, z = (Q[:, k].astype(np.float32) for k in range(3))
+    s, o = pair(pfg, po, sc, p)
+    with s:
+        set_pos(s, o, x, y, z)
+        for st in (pfg.STAGE_CELLS, pfg.STAGE_MEMBRANE, pfg.STAGE_BONDS, pfg.STAGE_FORCES):
+            s.stage(st)
+            o.stage(st)
+        mk = s.markers()
+        info = s.step(0)
+    F = np.stack([mk["fx"], mk["fy"], mk["fz"]], 1)
+    oF = np.stack([o.marker(k) for k in ("fx", "fy", "fz")], 1)
+    pairs = int(np.count_nonzero(np.abs(oF).sum(1)))
+    print(f"dense contact: {pairs} elements with a contact force, candidates proposed "
+          f"beyond four platelets {info['contact_overflow']}")
+    assert pairs >= 12
+    assert info["contact_overflow"] > 0  # the case the fifth-platelet path covers
+    assert np.flatnonzero(np.abs(F).sum(1)).tolist() == np.flatnonzero(np.abs(oF).sum(1)).tolist()
+    assert np.abs(F - oF).max() < 1e-5 * np.abs(oF).max()
