@@ -950,7 +950,11 @@ static void stage_bonds(orc_state *s) {
       ligand_vec(s, b, rp, d);
       const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
       const double tension = (double)p->bond_k * (dist - (double)p->bond_l0);
-      const double koff = (double)p->k_off0 * exp((double)p->bell_xb_kT * (tension > 0 ? tension : 0));
+      /* Bell on the pulling force, or the reference's law on the force
+       * magnitude k |l0 - r| (sem_kernel.cu:1852-1863) */
+      const double f = p->bond_law == PFG_BOND_REFERENCE ? fabs(tension)
+                                                          : (tension > 0 ? tension : 0);
+      const double koff = (double)p->k_off0 * exp((double)p->bell_xb_kT * f);
       const double p_off = 1.0 - exp(-koff);
       const double u = pfg_uniform(p->seed, s->step, (uint32_t)r, PFG_RNG_BOND_OFF);
       if (fabs(u - p_off) < eps_p) s->near++;
@@ -976,7 +980,14 @@ static void stage_bonds(orc_state *s) {
     }
     if (near_reach) s->near++;
     if (best < 0) continue;
-    const double p_on = 1.0 - exp(-(double)p->k_on);
+    double kon = p->k_on;
+    if (p->bond_law == PFG_BOND_REFERENCE) {
+      /* Kf = Kf0 e^{f (x_b - w/2) / kT}, w = |l0 - r|, f = k w
+       * (sem_kernel.cu:1852-1858) */
+      const double w = fabs((double)p->bond_l0 - sqrt(best_d2)), f = (double)p->bond_k * w;
+      kon = (double)p->k_on * exp((double)p->bell_xb_kT * f - 0.5 * f * w / (double)p->bell_kT);
+    }
+    const double p_on = 1.0 - exp(-kon);
     const double u = pfg_uniform(p->seed, s->step, (uint32_t)r, PFG_RNG_BOND_ON);
     if (fabs(u - p_on) < eps_p) s->near++;
     if (u < p_on) {

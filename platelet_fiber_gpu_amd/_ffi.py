@@ -174,6 +174,30 @@ SIGNATURES = {
                                        f32p, i32p, i32p]),
     "pfg_geom_fibrin": (C.c_int, [C.c_int, f32p, f32p, C.c_float, C.c_int, C.c_uint64,
                                   f32p, f32p, f32p, i32p, f32p, i32p]),
+    # the reference's own entry points (include/pfg_ref.h)
+    "pfg_ref_last_status": (C.c_int, []),
+    "pfg_ref_handle": (C.c_void_p, [C.c_void_p]),
+    "fluid_allocGPUKernel": (C.c_void_p, [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int,
+                                          C.c_int]),
+    "fluid_initGPUKernel": (None, [C.c_void_p, C.c_void_p, C.c_int, f32p,
+                                   C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)] +
+                            [C.c_void_p] * 9),
+    "fluid_invokeGPUKernel": (None, [C.c_void_p, C.c_void_p, C.c_void_p, f64p, C.c_void_p,
+                                     C.c_int]),
+    "fluid_releaseGPUKernel": (None, [C.c_void_p, C.c_void_p]),
+    "sem_allocGPUKernel": (C.c_void_p, [C.c_void_p] + [C.c_int] * 5 +
+                           [C.c_float, C.c_double, f32p]),
+    "sem_initGPUKernel": (None, [C.c_void_p, C.c_void_p, C.c_int, i32p, C.c_int, C.c_int] +
+                          [C.c_void_p] * 23 + [C.c_double, C.c_void_p, C.c_void_p]),
+    "sem_invokeGPUKernel_Force": (None, [C.c_void_p, C.c_void_p, C.c_int, i32p, i32p,
+                                         C.c_void_p, C.c_double]),
+    "sem_copyGPUKernel": (None, [C.c_void_p, C.c_void_p] + [C.c_void_p] * 10 + [C.c_int]),
+    "sem_releaseGPUKernel": (None, [C.c_void_p, C.c_void_p]),
+    "fiber_allocGPUKernel": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                          f64p]),
+    "fiber_init_GPUKernel": (None, [C.c_void_p, C.c_void_p] + [C.c_void_p] * 19),
+    "fiber_copy_GPUKernel": (None, [C.c_void_p, C.c_void_p] + [C.c_void_p] * 12),
+    "fiber_release_GPUKernel": (None, [C.c_void_p, C.c_void_p]),
 }
 
 _lib = None

@@ -213,9 +213,26 @@ void pfg_params_default(pfg_params *p) {
 /* hostSEMparameters index map (sem_kernel.cu:58-76, filled at
  * gpuSEM_LB.c:1206-1233): [0] receptors [1] seed [2] k_spring [3] Kf0 [4] Kr0
  * [5] shear rate [6] rho0 [7] L0 [8] T0 [9] M0 [10] F0 [11] UMAX [12] nu
- * [13] tau [14] P0 [15] C [16] UNIT_FACTOR [17] Ks [18] Kv [19] Kbend. */
-pfg_status pfg_params_from_sem(const float *s, float dx, float dt, int lx, int ly,
-                               int lz, pfg_params *p) {
+ * [13] tau [14] P0 [15] C [16] UNIT_FACTOR [17] Ks [18] Kv [19] Kbend.
+ *
+ * The device works in lattice units (length dx, time dt) with forces in
+ * units of F0, as the reference's SEM kernels divide every force by F0
+ * (sem_kernel.cu:383-389, 428, 1632-1637, 1720-1724).  A stiffness per
+ * micrometre becomes one per lattice length (x dx), an area gradient
+ * carries one more dx, a volume gradient two; the bending Laplacian of the
+ * mean curvature three fewer.  The constants the reference keeps in its
+ * kernel source (the vWF bond: BONDTHRESHOLD 0.078 um, k_receptvWF
+ * 10 nN/um, the Bell length 0.00071 and KbT 4.05 of sem_kernel.cu:79-85,
+ * 1852-1863; the wall repulsion 5e-4 * 2000 e^{-2000 h}/(1 - e^{-2000 h})
+ * nN below 0.02 um, sem_kernel.cu:424-427) are mapped the same way. */
+constexpr double REF_BOND_THRESHOLD_UM = 0.078;
+constexpr double REF_K_RECEPT_VWF = 10.0;
+constexpr double REF_BELL_XB = 0.00071;
+constexpr double REF_KBT = 4.05;
+constexpr double REF_WALL_A = 5e-4, REF_WALL_KAPPA = 2000.0, REF_WALL_HC_UM = 0.02;
+
+pfg_status pfg_params_from_sem(const float *s, float dx, float dt, int lx, int ly, int lz,
+                               pfg_params *p) {
   if (!s || !p || dx <= 0.f || dt <= 0.f || lx <= 0 || ly <= 0 || lz <= 0)
     return PFG_ERR_ARG;
   for (int i = 0; i < 20; ++i) p->sem[i] = s[i];
@@ -223,20 +240,34 @@ pfg_status pfg_params_from_sem(const float *s, float dx, float dt, int lx, int l
   p->ny = ly;
   p->nz = lz;
   p->seed = (uint64_t)s[1];
-  p->k_spring = s[2];
-  p->k_on = s[3] * dt;  /* rates per second -> per step */
-  p->k_off0 = s[4] * dt;
   p->tau = s[13];
-  p->ks = s[17];
-  p->kv = s[18];
-  /* marker force F/F0 -> lattice force density: F0 * UNIT_FACTOR * dt /
-   * (rho0 * UMAX) as in the Guo term (LB_kernel.cu:1298), in units of C */
-  const float F0 = s[10], rho0 = s[6], umax = s[11], C = s[15], uf = s[16];
-  if (rho0 > 0.f && umax > 0.f && C > 0.f)
-    p->ib_force_scale = F0 * uf * dt / (rho0 * umax) / C;
-  if (s[8] > 0.f) p->marker_dt = dt / s[8] / dx;
-  p->detach_rate = s[5] * dt;
-  p->kbend = F0 > 0.f ? s[19] / F0 : 0.f; /* Kbend ... / F0 (sem_kernel.cu:1630-1635) */
+  const double F0 = s[10] > 0.f ? s[10] : 1.0, rho0 = s[6], umax = s[11], C = s[15],
+               uf = s[16], L0 = s[7] > 0.f ? s[7] : 1.0, T0 = s[8], h = dx;
+  /* marker force (F0) -> lattice force density: F0 UNIT_FACTOR dt /
+   * (rho0 UMAX) in units of C, as the Guo half force of the moments
+   * (LB_kernel.cu:1158-1163) */
+  if (rho0 > 0.0 && umax > 0.0 && C > 0.0) p->ib_force_scale = (float)(F0 * uf * dt / (rho0 * umax) / C);
+  /* x += V dt / T0 with V in UMAX (sem_kernel.cu:2111-2137): a lattice
+   * velocity u = V / C moves C dt L0 / (T0 dx) lattice lengths per step
+   * (1 for the reference's own block) */
+  p->marker_dt = (T0 > 0.0 && C > 0.0) ? (float)(C * dt * L0 / (T0 * h)) : 1.0f;
+  p->k_spring = (float)(s[2] * h / F0);
+  p->ks = (float)(s[17] * h / F0);
+  p->kv = (float)(s[18] * h * h / F0);
+  p->kbend = (float)(s[19] / F0 / (h * h * h));
+  /* rates per second -> per step; Kf0 is set from the shear rate by the
+   * reference's host before the block is handed over (gpuSEM_LB.c:1209) */
+  p->k_on = (float)(s[3] * dt);
+  p->k_off0 = (float)(s[4] * dt);
+  p->detach_rate = (float)(s[5] * dt); /* gama = SHEAR_RATE dt, gpuSEM_LB.c:1433 */
+  p->bond_law = PFG_BOND_REFERENCE;
+  p->bond_k = (float)(REF_K_RECEPT_VWF * h / F0);
+  p->bond_l0 = (float)(REF_BOND_THRESHOLD_UM / h);
+  p->bell_xb_kT = (float)(REF_BELL_XB / REF_KBT * F0);
+  p->bell_kT = (float)(REF_KBT / F0 / h);
+  p->wall_rep_kappa = (float)(REF_WALL_KAPPA * h);
+  p->wall_rep_a = (float)(REF_WALL_A / F0 / h);
+  p->wall_rep_hc = (float)(REF_WALL_HC_UM / h);
   return PFG_OK;
 }
 

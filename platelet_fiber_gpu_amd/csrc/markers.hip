@@ -988,8 +988,11 @@ __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
       ligand_vec(a, L, b, rp, d);
       const float dist = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
       const float tension = a.bond_k * (dist - a.bond_l0);
-      /* Bell: k_off = k_off0 exp(x_b f / kBT) for a pulling bond */
-      const float koff = a.k_off0 * expf(a.bell_xb_kT * fmaxf(tension, 0.0f));
+      /* Bell: k_off = k_off0 exp(x_b f / kBT) for a pulling bond; the
+       * reference's law takes the force magnitude k |l0 - r|
+       * (sem_kernel.cu:1852-1863) */
+      const float f = a.bond_law == PFG_BOND_REFERENCE ? fabsf(tension) : fmaxf(tension, 0.0f);
+      const float koff = a.k_off0 * expf(a.bell_xb_kT * f);
       const float p_off = 1.0f - expf(-koff);
       const float u = pfg_uniform(a.seed, step, (uint32_t)r, PFG_RNG_BOND_OFF);
       if (fabsf(u - p_off) < BOND_EPS_P) bump(a.m.counters, 2);
@@ -1051,7 +1054,13 @@ __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
   a.m.rec_cand[r] = -1;
   if (any_near) bump(a.m.counters, 2);
   if (best == INT_MAX) return;
-  const float p_on = 1.0f - expf(-a.k_on);
+  float kon = a.k_on;
+  if (a.bond_law == PFG_BOND_REFERENCE) {
+    /* k_on e^{f (x_b - w/2) / kT}, w = |l0 - r|, f = k w (sem_kernel.cu:1852-1858) */
+    const float w = fabsf(a.bond_l0 - sqrtf(best_d2)), f = a.bond_k * w;
+    kon = a.k_on * expf(a.bell_xb_kT * f - 0.5f * f * w / a.bell_kT);
+  }
+  const float p_on = 1.0f - expf(-kon);
   const float u = pfg_uniform(a.seed, step, (uint32_t)r, PFG_RNG_BOND_ON);
   if (fabsf(u - p_on) < BOND_EPS_P) bump(a.m.counters, 2);
   if (u < p_on) {
