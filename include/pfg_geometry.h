@@ -74,6 +74,8 @@ pfg_status pfg_scene_velocity(const pfg_scene *s, float *ux, float *uy, float *u
  * only): nz values per component. */
 pfg_status pfg_scene_velocity_z(const pfg_scene *s, float *ux_z, float *uy_z,
                                 float *uz_z);
+/* The parameters the scene was built with (as pfg_scene_build returned). */
+pfg_status pfg_scene_params(const pfg_scene *s, pfg_params *out);
 /* Fluid-cell count (cells not flagged solid). */
 int64_t pfg_scene_fluid_cells(const pfg_scene *s);
 

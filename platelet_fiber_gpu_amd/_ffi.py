@@ -163,6 +163,7 @@ SIGNATURES = {
     "pfg_scene_flags": (u8p, [C.c_void_p]),
     "pfg_scene_velocity": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
     "pfg_scene_velocity_z": (C.c_int, [C.c_void_p, f32p, f32p, f32p]),
+    "pfg_scene_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
     "pfg_scene_fluid_cells": (C.c_int64, [C.c_void_p]),
     "pfg_geom_ellipsoid": (C.c_int, [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
                                      C.c_float, C.c_float, C.c_float, C.c_uint64, f32p,

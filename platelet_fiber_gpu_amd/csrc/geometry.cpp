@@ -995,6 +995,12 @@ pfg_status pfg_scene_velocity_z(const pfg_scene *s, float *ux, float *uy, float 
   return PFG_OK;
 }
 
+pfg_status pfg_scene_params(const pfg_scene *s, pfg_params *out) {
+  if (!s || !out) return PFG_ERR_ARG;
+  *out = s->p;
+  return PFG_OK;
+}
+
 int64_t pfg_scene_fluid_cells(const pfg_scene *s) {
   if (!s) return 0;
   if (s->flags.empty()) return s->N;

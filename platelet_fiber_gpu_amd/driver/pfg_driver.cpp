@@ -22,12 +22,23 @@
  * Positions, velocities and forces are written in lattice units; the pause
  * time is steps x DELTA_T.
  *
+ * Physics: the file's parameter block (with its derived entries) sets the
+ * run's physics through pfg_params_from_sem -- tau, the IB force scale, the
+ * marker step, the spring, area, volume and bending stiffnesses, the bond
+ * rates and the reference's bond and wall-repulsion constants -- and the
+ * shear: the top wall moves at SHEAR_RATE across the lattice depth, the flow
+ * starting from that linear profile.  The platelets carry no Morse term, as
+ * in the reference (NO_NB, sem_kernel.cu:78).  --scene-physics keeps the
+ * scene's own lattice-unit physics instead (with --rates-from-file taking
+ * only the rates from the file).
+ *
  * Usage: pfg_driver [-d device] [-p parameter file] [-o output dir]
  *                   [--scene shear|channel|thrombus|large] [--lattice nx ny nz]
  *                   [--store f32|f16|bf16] [--seeds first last] [--max-steps n]
  *                   [--mesh file.neu] [--mesh-scale s] [--no-mesh]
  *                   [--fibrin network.txt] [--fibrin-scale s]
- *                   [--rates-from-file] [--scene-counts] [--parse-only]
+ *                   [--rates-from-file] [--scene-physics] [--scene-counts]
+ *                   [--parse-only]
  * --fibrin places the network of a fibrin text file (the format
  * init_fibrin_network reads, fiber.cpp:145-217) in the scene's fibrin box.
  * Exit status: 0 done, 1 parameter or usage error, 2 I/O error, 11 device
@@ -149,6 +160,7 @@ struct Options {
   int store = -1;
   long long max_steps = -1;
   bool rates_from_file = false;
+  bool scene_physics = false; /* --scene-physics: keep the scene's physics */
   bool scene_counts = false;
   bool parse_only = false;
   std::string mesh;      /* --mesh */
@@ -166,7 +178,8 @@ int usage(const char *msg) {
                "       [--store f32|f16|bf16] [--seeds first last] [--max-steps n]\n"
                "       [--mesh file.neu] [--mesh-scale s] [--no-mesh]\n"
                "       [--fibrin network.txt] [--fibrin-scale s]\n"
-               "       [--rates-from-file] [--scene-counts] [--parse-only]\n");
+               "       [--rates-from-file] [--scene-physics] [--scene-counts]\n"
+               "       [--parse-only]\n");
   return 1;
 }
 
@@ -258,6 +271,8 @@ bool parse_args(int argc, char **argv, Options *o, std::string *err) {
       }
     } else if (a == "--rates-from-file") {
       o->rates_from_file = true;
+    } else if (a == "--scene-physics") {
+      o->scene_physics = true;
     } else if (a == "--scene-counts") {
       o->scene_counts = true;
     } else if (a == "--parse-only") {
@@ -417,12 +432,57 @@ int build_scene(const Options &o, ParFile *pf, long long seed, SceneGuard *sg, p
   fill_derived(pf, p->nz, mean_edge_length(*model) * pf->dx);
   for (int k = 0; k < 20; ++k) p->sem[k] = pf->host[k];
   p->seed = (uint64_t)seed;
-  if (o.rates_from_file) {
+  if (!o.scene_physics) {
+    pfg_params q = *p;
+    if (pfg_params_from_sem(pf->host, pf->dx, pf->dt, p->nx, p->ny, p->nz, &q) != PFG_OK)
+      return 1;
+    p->tau = q.tau;
+    p->ib_force_scale = q.ib_force_scale;
+    p->marker_dt = q.marker_dt;
+    p->k_spring = q.k_spring;
+    p->ks = q.ks;
+    p->kv = q.kv;
+    p->kbend = q.kbend;
+    p->k_on = q.k_on;
+    p->k_off0 = q.k_off0;
+    p->detach_rate = q.detach_rate;
+    p->bond_law = q.bond_law;
+    p->bond_k = q.bond_k;
+    p->bond_l0 = q.bond_l0;
+    p->bell_xb_kT = q.bell_xb_kT;
+    p->bell_kT = q.bell_kT;
+    p->wall_rep_a = q.wall_rep_a;
+    p->wall_rep_kappa = q.wall_rep_kappa;
+    p->wall_rep_hc = q.wall_rep_hc;
+    p->morse_u0 = 0.f; /* NO_NB: the reference's platelets have no Morse term */
+    /* the shear: SHEAR_RATE x dt per lattice length, over the nz cells
+     * between the two z walls, along the scene's flow direction */
+    if (p->wall[2] == PFG_WALL_BOUNCE) {
+      const float u_top = (float)((double)pf->host[5] * pf->dt * p->nz);
+      for (int a = 0; a < 3; ++a)
+        if (p->wall_u_hi[2][a] != 0.f) p->wall_u_hi[2][a] = u_top;
+    }
+  } else if (o.rates_from_file) {
     p->k_on = pf->host[3] * pf->dt;
     p->k_off0 = pf->host[4] * pf->dt;
     p->detach_rate = pf->host[5] * pf->dt; /* gama = SHEAR_RATE * dt */
   }
   return 0;
+}
+
+/* The initial flow per z-plane: the scene's profile, rescaled to the top
+ * wall velocity of the run (a linear Couette profile in the shear scenes). */
+pfg_status initial_flow(const pfg_scene *s, const pfg_params &p, std::vector<float> *u) {
+  for (int a = 0; a < 3; ++a) u[a].assign((size_t)p.nz, 0.f);
+  pfg_status st = pfg_scene_velocity_z(s, u[0].data(), u[1].data(), u[2].data());
+  if (st != PFG_OK) return st;
+  /* the scene's flow was built for its own wall velocity */
+  pfg_params sp;
+  if ((st = pfg_scene_params(s, &sp)) != PFG_OK) return st;
+  for (int a = 0; a < 3; ++a)
+    if (sp.wall_u_hi[2][a] != 0.f)
+      for (float &v : u[a]) v *= p.wall_u_hi[2][a] / sp.wall_u_hi[2][a];
+  return PFG_OK;
 }
 
 int run_seed(const Options &o, ParFile pf, long long seed, long long max_steps) {
@@ -441,12 +501,11 @@ int run_seed(const Options &o, ParFile pf, long long seed, long long max_steps) 
     return st == PFG_ERR_DEVICE ? 11 : 3;
   }
   pfg_handle *h = hg.h;
-  const int64_t N = (int64_t)p.nx * p.ny * p.nz;
-  std::vector<float> ux(N), uy(N), uz(N);
+  std::vector<float> u0[3];
   const uint8_t *flags = pfg_scene_flags(sg.s);
   if ((flags && p.use_flags && (st = pfg_upload_flags(h, flags)) != PFG_OK) ||
-      (st = pfg_scene_velocity(sg.s, ux.data(), uy.data(), uz.data())) != PFG_OK ||
-      (st = pfg_init_equilibrium(h, nullptr, ux.data(), uy.data(), uz.data())) != PFG_OK ||
+      (st = initial_flow(sg.s, p, u0)) != PFG_OK ||
+      (st = pfg_init_equilibrium_z(h, u0[0].data(), u0[1].data(), u0[2].data())) != PFG_OK ||
       (st = pfg_load_model(h, &model)) != PFG_OK || (st = pfg_set_halt_on_detach(h, 1)) != PFG_OK) {
     std::fprintf(stderr, "pfg_driver: set-up: %s\n", pfg_status_string(st));
     return 3;
@@ -526,10 +585,17 @@ int main(int argc, char **argv) {
     std::printf("\"max_steps\": %lld, \"out_freq\": %d, \"seeds\": [%lld, %lld], "
                 "\"lattice\": [%d, %d, %d], \"platelets\": %d, \"markers\": %d, "
                 "\"receptors\": %d, \"tau\": %.9g, \"k_on\": %.9g, \"k_off0\": %.9g, "
-                "\"detach_rate\": %.9g, \"seed\": %llu, \"sem\": [",
+                "\"detach_rate\": %.9g, \"seed\": %llu, ",
                 max_steps, pf.out_freq, first, last, p.nx, p.ny, p.nz, model.n_platelets,
                 model.n_markers, model.n_receptors, p.tau, p.k_on, p.k_off0, p.detach_rate,
                 (unsigned long long)p.seed);
+    std::printf("\"physics\": \"%s\", \"ib_force_scale\": %.9g, \"marker_dt\": %.9g, "
+                "\"k_spring\": %.9g, \"ks\": %.9g, \"kv\": %.9g, \"kbend\": %.9g, "
+                "\"bond_law\": %d, \"bond_k\": %.9g, \"bond_l0\": %.9g, "
+                "\"wall_rep_a\": %.9g, \"morse_u0\": %.9g, \"wall_u_top\": %.9g, \"sem\": [",
+                o.scene_physics ? "scene" : "file", p.ib_force_scale, p.marker_dt, p.k_spring,
+                p.ks, p.kv, p.kbend, p.bond_law, p.bond_k, p.bond_l0, p.wall_rep_a, p.morse_u0,
+                p.wall_u_hi[2][0] + p.wall_u_hi[2][1]);
     for (int k = 0; k < 20; ++k) std::printf("%s%.9g", k ? ", " : "", p.sem[k]);
     std::printf("]}\n");
     return 0;

@@ -139,3 +139,46 @@ def test_fibrin_network_from_file(tmp_path):
     assert json.loads(r.stdout)["links"] == 1422
     r = run("-p", FIXTURE, "--parse-only", "--fibrin", str(tmp_path / "none.txt"))
     assert r.returncode == 2
+
+
+def test_physics_follows_the_parameter_file(tmp_path):
+    """The run's physics comes from the file's block (pfg_params_from_sem):
+    a changed parameter.dat changes the handle's parameters accordingly --
+    the spring stiffness with K_SPRING, tau with INIT_RHO (nu = 1.2e6 /
+    rho0, gpuSEM_LB.c:1220-1221), the top wall and the detach hazard with
+    SHEAR_RATE, the IB force scale with both -- while --scene-physics keeps
+    the scene's lattice-unit physics."""
+    base = json.loads(run("-p", FIXTURE, "--parse-only").stdout)
+    v = read_fixture()
+    dx, dt = float(v["DELTA_X"]), float(v["DELTA_T"])
+    assert base["physics"] == "file"
+    assert base["tau"] == pytest.approx(base["sem"][13], rel=1e-6)
+    # a marker at lattice velocity u moves u lattice lengths per step
+    assert base["marker_dt"] == pytest.approx(1.0, rel=1e-6)
+    assert base["k_spring"] == pytest.approx(float(v["K_SPRING"]) * dx, rel=1e-6)
+    assert base["wall_u_top"] == pytest.approx(float(v["SHEAR_RATE"]) * dt * 128, rel=1e-6)
+    assert base["morse_u0"] == 0.0 and base["bond_law"] == 1
+
+    v2 = dict(v)
+    v2["K_SPRING"] = "2.5e3"
+    v2["INIT_RHO"] = "2.0"
+    v2["SHEAR_RATE"] = "6e2"
+    par = tmp_path / "parameter.dat"
+    write_par(par, v2)
+    # the mesh path of the file resolves relative to the file: keep the
+    # fixture's mesh
+    mesh = os.path.join(ROOT, "tests", "fixtures", "Platelet_Mesh_958.neu")
+    new = json.loads(run("-p", str(par), "--parse-only", "--mesh", mesh).stdout)
+    assert new["k_spring"] == pytest.approx(2.5e3 * dx, rel=1e-6)
+    nu = 1.2e6 / 2.0
+    assert new["tau"] == pytest.approx(3 * nu * dt / dx ** 2 + 0.5, rel=1e-5)
+    assert new["tau"] != pytest.approx(base["tau"], rel=1e-3)
+    assert new["wall_u_top"] == pytest.approx(600 * dt * 128, rel=1e-6)
+    assert new["detach_rate"] == pytest.approx(600 * dt, rel=1e-6)
+    # F0 UNIT_FACTOR dt / (rho0 UMAX C) = F0 UF dt^2 / (rho0 dx): rho0 only
+    assert new["ib_force_scale"] == pytest.approx(base["ib_force_scale"] * 1.0239 / 2.0,
+                                                  rel=1e-5)
+    # the scene's own physics on request
+    sc = json.loads(run("-p", FIXTURE, "--parse-only", "--scene-physics").stdout)
+    assert sc["physics"] == "scene"
+    assert sc["tau"] == pytest.approx(0.7) and sc["wall_u_top"] == pytest.approx(0.05)

@@ -32,8 +32,10 @@ def small_vals(**kw):
 
 
 def test_driver_frames_match_the_binding(gpu, tmp_path):
+    """With the scene's own physics (--scene-physics), the driver's frames
+    are the binding's run of the same scene."""
     pfg = gpu
-    out = driver(tmp_path, small_vals())
+    out = driver(tmp_path, small_vals(), "--scene-physics")
     od = tmp_path / "Output"
     n, frames = 200, 1 + 3  # initial frame + 30 steps / OUT_FREQ 10
     pos = np.fromfile(od / "cellsPos.data", dtype=np.float64)
@@ -72,7 +74,7 @@ def test_driver_stops_at_detach(gpu, tmp_path):
     hazard 0.3 per step and almost no bond formation, so the run ends at
     detach well inside 400 steps and appends the pause time."""
     vals = small_vals(DELTA_T="0.01", SIM_TIME="4", SHEAR_RATE="30")
-    out = driver(tmp_path, vals, "--rates-from-file")
+    out = driver(tmp_path, vals, "--scene-physics", "--rates-from-file")
     od = tmp_path / "Output"
     line = open(od / "cellsPauseTime.dat").read().strip().split("\n")[-1]
     t_pause, seed = line.split(", ")
@@ -82,3 +84,58 @@ def test_driver_stops_at_detach(gpu, tmp_path):
     assert f"detached after {steps} steps" in out
     rows = [l.split() for l in open(od / "cellsPosASCII.dat").read().split("\n")[1:] if l.strip()]
     assert int(rows[-1][0]) == steps  # the last frame is the state at detach
+
+
+def test_driver_file_physics_matches_the_binding(gpu, tmp_path):
+    """With the physics of the parameter file (the default), the driver's
+    frames are the binding's run of the same scene with the parameters
+    mapped from the file's block as the driver documents (pfg_params_from_sem
+    for the physics, the top wall at SHEAR_RATE x DELTA_T per lattice length,
+    no Morse, the linear shear start)."""
+    import ctypes as C
+    import json
+    pfg = gpu
+    vals = small_vals()
+    par = tmp_path / "parameter.dat"
+    write_par(par, vals)
+    mesh = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fixtures",
+                        "Platelet_Mesh_958.neu")
+    info = json.loads(subprocess.run([DRIVER, "-p", str(par), "--lattice", "48", "32", "24",
+                                      "--no-mesh", "--parse-only"], capture_output=True,
+                                     text=True, timeout=120).stdout)
+    assert info["physics"] == "file" and mesh
+    driver(tmp_path, vals, "--no-mesh")
+    od = tmp_path / "Output"
+    n, frames = 200, 4
+    pos = np.fromfile(od / "cellsPos.data", dtype=np.float64).reshape(frames, n, 4)
+    sc = pfg.Scene(pfg.SCENE_SHEAR, nx=48, ny=32, nz=24, n_platelets=1, elements=200,
+                   receptors=int(1584 / 396 / 2) * 396, seed=1018)
+    p = sc.params
+    q = pfg.Params()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(p))
+    sem = np.array(info["sem"], np.float32)
+    dx, dt = float(vals["DELTA_X"]), float(vals["DELTA_T"])
+    assert pfg.lib().pfg_params_from_sem(sem.ctypes.data_as(C.POINTER(C.c_float)), dx, dt,
+                                        p.nx, p.ny, p.nz, C.byref(q)) == 0
+    for k in ("tau", "ib_force_scale", "marker_dt", "k_spring", "ks", "kv", "kbend", "k_on",
+              "k_off0", "detach_rate", "bond_law", "bond_k", "bond_l0", "bell_xb_kT", "bell_kT",
+              "wall_rep_a", "wall_rep_kappa", "wall_rep_hc"):
+        setattr(p, k, getattr(q, k))
+    p.morse_u0 = 0.0
+    p.seed = 1018
+    u_top = float(vals["SHEAR_RATE"]) * dt * p.nz
+    scale = u_top / p.wall_u_hi[2][0]
+    p.wall_u_hi[2][0] = u_top
+    sc.u0_z[0][:] *= scale
+    assert np.isclose(p.tau, info["tau"], rtol=1e-6)
+    with pfg.Solver(p) as s:
+        s.load(sc)
+        for k in range(1, frames):
+            s.step(10)
+            mk = s.markers()
+            got = np.stack([mk["x"], mk["y"], mk["z"]], 1)
+            err = np.abs(got - pos[k, :, :3]).max()
+            assert err < 1e-5, (k, err)
+    # the platelet rides the shear: it moved along x, by about u(z) per step
+    assert np.isfinite(pos).all()
+    assert pos[-1, :, 0].mean() > pos[0, :, 0].mean()
