@@ -13,6 +13,7 @@ from platelet_fiber_gpu_amd._ffi import Model, Params
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "_build", "liboracle.so")
+PORT_LIB = os.path.join(_HERE, "_build", "libcpuport.so")
 Q = 19
 
 f64p = C.POINTER(C.c_double)
@@ -319,3 +320,75 @@ class Oracle:
 
     def stage(self, s):
         self._lib.orc_stage(self._h, int(s))
+
+
+_port = None
+
+
+def load_port():
+    """The FP32 OpenMP CPU port (the CPU baseline), built by the Makefile."""
+    global _port
+    if _port is not None:
+        return _port
+    if not os.path.exists(PORT_LIB):
+        build()
+    L = C.CDLL(PORT_LIB)
+    f32 = C.POINTER(C.c_float)
+    sig = {
+        "cport_create": (C.c_void_p, [C.POINTER(Params), C.POINTER(Model), f32, f32, f32, C.c_int]),
+        "cport_destroy": (None, [C.c_void_p]),
+        "cport_step": (None, [C.c_void_p, C.c_int]),
+        "cport_markers": (None, [C.c_void_p, f32, f32, f32]),
+        "cport_velocity": (None, [C.c_void_p, f32, f32, f32]),
+        "cport_bonds": (C.c_int64, [C.c_void_p]),
+    }
+    for k, (r, a) in sig.items():
+        fn = getattr(L, k)
+        fn.restype = r
+        fn.argtypes = a
+    _port = L
+    return L
+
+
+class CpuPort:
+    """The coupled step in FP32 on the host's cores (OpenMP): the CPU
+    baseline.  Raises ValueError when the scene uses a term the port does
+    not carry."""
+
+    def __init__(self, params, scene, threads=0):
+        self._lib = load_port()
+        m = scene.model()
+        self._scene = scene
+        u = [np.ascontiguousarray(a, dtype=np.float32) for a in scene.u0_z]
+        self._keep = u
+        f32 = C.POINTER(C.c_float)
+        self._h = self._lib.cport_create(C.byref(params), C.byref(m),
+                                         *[a.ctypes.data_as(f32) for a in u], int(threads))
+        if not self._h:
+            raise ValueError("the scene uses a term the CPU port does not carry")
+        self.n = m.n_markers
+        self.N = params.nx * params.ny * params.nz
+
+    def step(self, n=1):
+        self._lib.cport_step(self._h, int(n))
+
+    def markers(self):
+        out = [np.empty(self.n, np.float32) for _ in range(3)]
+        self._lib.cport_markers(self._h, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in out])
+        return out
+
+    def velocity(self):
+        out = [np.empty(self.N, np.float32) for _ in range(3)]
+        self._lib.cport_velocity(self._h, *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in out])
+        return out
+
+    def close(self):
+        if self._h:
+            self._lib.cport_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
