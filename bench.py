@@ -440,9 +440,10 @@ def batch_scenes(pfg, n=256, seed=1101):
     return scenes
 
 
-def batch_workload(pfg, K, W, store):
+def batch_workload(pfg, K, W, store, cpu_leg=True):
     """256 lattices stepped as one batch, against the same 256 lattices as
-    separate handles stepped one after the other."""
+    separate handles, each created, stepped and destroyed in turn (one
+    handle alive at a time)."""
     t0 = time.perf_counter()
     scenes = batch_scenes(pfg)
     for sc in scenes:
@@ -463,22 +464,49 @@ def batch_workload(pfg, K, W, store):
     b.synchronize()
     dt_batch = time.perf_counter() - t0
     dev_gb = b.device_bytes() / 1e9
+    persistent = int((TUNING.get("batch") or {}).get("lb_persistent", 0))
+    b.set_fluid_only(True)
+    lb_ms_by = {}
+    for form in (0, 1):  # one workgroup per x-block row; the persistent queue launch
+        b.set_lb_persistent(form)
+        b.step(8)
+        lb_ms_by[form] = b.time_lb(max(10, min(K, 60)))
     b.close()
-    solvers = []
-    for sc in scenes:
-        s = pfg.Solver(sc.params)
-        s.load(sc)
-        s.step(W)
-        solvers.append(s)
+    lb_ms = lb_ms_by[persistent]
+    bpc = BYTES_PER_CELL[store]
+    gbs = bpc * ncells / (lb_ms * 1e-3) / 1e9
     ks = max(1, K // 4)
-    t0 = time.perf_counter()
-    for s in solvers:
-        s.step(ks)
-    for s in solvers:
-        s.synchronize()
-    dt_sep = (time.perf_counter() - t0) * K / ks
-    for s in solvers:
-        s.close()
+    dt_sep = 0.0
+    for sc in scenes:
+        with pfg.Solver(sc.params) as s:
+            s.load(sc)
+            s.step(W)
+            s.synchronize()
+            t1 = time.perf_counter()
+            s.step(ks)
+            s.synchronize()
+            dt_sep += time.perf_counter() - t1
+    dt_sep *= K / ks
+    cpu = None
+    if cpu_leg:
+        po, threads = load_oracle()
+        sample = scenes[:16]
+        cells, t_cpu = 0, 0.0
+        for sc in sample:
+            sc.params.store = pfg.STORE_F32
+            port = po.CpuPort(sc.params, sc, threads)
+            port.step(1)
+            t1 = time.perf_counter()
+            port.step(20)
+            t_cpu += time.perf_counter() - t1
+            cells += sc.N * 20
+            port.close()
+            sc.params.store = store
+        cpu = {"value": round(cells / t_cpu / 1e6, 3), "unit": "MLUPS", "cores": threads,
+               "kind": "port",
+               "sample": f"20 coupled steps of each of the first 16 lattices of the batch "
+                         f"({t_cpu:.1f} s), FP32 OpenMP port of the same algorithm on "
+                         f"{threads} host threads, one lattice after the other"}
     dtype = {0: "f32", 1: "f16", 2: "bf16"}[store]
     return {
         "metric": METRIC, "value": round(ncells * K / dt_batch / 1e6, 1), "unit": "MLUPS",
@@ -493,7 +521,14 @@ def batch_workload(pfg, K, W, store):
         "speedup_vs_separate": round(dt_sep / dt_batch, 2),
         "device_gb": round(dev_gb, 2),
         "tuning": TUNING.get("batch"),
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": ("lb_batch_queue_kernel" if persistent else "lb_batch_kernel")
+                     + f"<{dtype.upper()}>", "bytes_per_cell": bpc,
+                     "cells_per_launch": ncells, "lb_kernel_us": round(lb_ms * 1e3, 1),
+                     "lb_kernel_us_per_row_launch": round(lb_ms_by[0] * 1e3, 1),
+                     "lb_kernel_us_persistent": round(lb_ms_by[1] * 1e3, 1)},
+        "cpu_baseline": cpu,
     }
 
 
@@ -627,7 +662,8 @@ def main():
         print(json.dumps(shear_workload(pfg, K, W, not args.no_cpu, not args.no_fp16)),
               flush=True)
     elif args.workload == "batch":
-        print(json.dumps(batch_workload(pfg, K, W, STORES[args.store])), flush=True)
+        print(json.dumps(batch_workload(pfg, K, W, STORES[args.store], not args.no_cpu)),
+              flush=True)
     else:
         print(json.dumps(other_workload(pfg, args.workload, K, W)), flush=True)
 

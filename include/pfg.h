@@ -451,6 +451,14 @@ pfg_status pfg_copy_probe(int device, size_t bytes, int reps, double *gbs);
  * 4; a wavefront updates one 64-cell x-block of that many rows).  Applies to
  * batches too; the overlap mode of a batch runs with 1 only. */
 pfg_status pfg_set_lb_tuning(pfg_handle *h, int cells_per_lane);
+/* Batches: enable = 1 runs the batched LB update as one persistent launch of
+ * the workgroups the CUs hold at once, which take the x-block-row tasks of
+ * all lattices from a device work queue (in the order of the workgroup
+ * table) until it is empty; 0 (default) launches one workgroup per task.
+ * Same results either way.  The queue resets itself at the end of each
+ * launch, so captured steps carry no clearing node.  PFG_ERR_STATE when
+ * enabled on a single-lattice handle. */
+pfg_status pfg_set_lb_persistent(pfg_handle *h, int enable);
 /* LB-kernel workgroup order: 0 as the hardware deals workgroups round-robin
  * to the 8 XCDs, 1 each XCD a contiguous range of rows, so that
  * neighbouring rows share its L2.  Same results either way.  Default: 1
@@ -468,11 +476,12 @@ typedef struct pfg_tuning {
   double ms_per_step;    /* the chosen shapes                                */
   double ms_default;     /* the automatic shapes                             */
   int shapes_timed;
+  int lb_persistent;     /* pfg_set_lb_persistent (batches)                  */
 } pfg_tuning;
 /* Times `steps` coupled steps of the handle's own lattices and markers for
  * each launch shape, one knob at a time from the automatic choices (LB cells
  * per lane, LB workgroup order, lanes per marker in the force pass, the
- * spreading form, the LB overlap), keeps the fastest value of each and sets it on the handle.  The
+ * spreading form, the LB overlap, the persistent batched launch), keeps the fastest value of each and sets it on the handle.  The
  * state is saved on the device before and restored after (the handle
  * continues from where it was; the trial steps leave no trace), at the cost
  * of one extra copy of the populations and of the marker state while it

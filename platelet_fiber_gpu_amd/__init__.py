@@ -484,6 +484,11 @@ class Batch(Solver):
             arrs = [f32(v).ravel() for v in s.u0_z]
             self._c("pfg_batch_init_equilibrium_z", int(k), *[_ptr(a, C.c_float) for a in arrs])
 
+    def set_lb_persistent(self, on):
+        """One persistent launch of resident workgroups taking every lattice's
+        x-block rows from a device work queue (pfg_set_lb_persistent)."""
+        self._c("pfg_set_lb_persistent", int(bool(on)))
+
     def init_lattice(self, k, rho=None, ux=None, uy=None, uz=None):
         arrs = [None if a is None else f32(a).ravel() for a in (rho, ux, uy, uz)]
         self._c("pfg_batch_init_equilibrium", int(k), *[_ptr(a, C.c_float) for a in arrs])

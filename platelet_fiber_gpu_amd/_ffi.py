@@ -85,7 +85,7 @@ class Tuning(C.Structure):
                 ("forces_lanes", C.c_int32),
                 ("ib_staging", C.c_int32), ("overlap", C.c_int32),
                 ("ms_per_step", C.c_double), ("ms_default", C.c_double),
-                ("shapes_timed", C.c_int32)]
+                ("shapes_timed", C.c_int32), ("lb_persistent", C.c_int32)]
 
 
 class SceneOpts(C.Structure):
@@ -155,6 +155,7 @@ SIGNATURES = {
     "pfg_set_lb_tuning": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tuning)]),
     "pfg_set_lb_xcd_rows": (C.c_int, [C.c_void_p, C.c_int]),
+    "pfg_set_lb_persistent": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_scene_opts_init": (None, [C.POINTER(SceneOpts), C.c_int]),
     "pfg_scene_build": (C.c_int, [C.POINTER(SceneOpts), C.POINTER(Params),
                                   C.POINTER(C.c_void_p)]),

@@ -462,6 +462,8 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
       if (e == hipSuccess) e = dalloc(h, &h->d_tile0, tile0.size());
       if (e == hipSuccess) e = upload(h, h->d_tile0, tile0.data(), tile0.size());
       for (int c = 0; c < 2 && e == hipSuccess; ++c) e = dalloc(h, &h->d_lbargs[c], (size_t)K);
+      if (e == hipSuccess) e = dalloc(h, &h->lb_queue, 2);
+      if (e == hipSuccess) e = hipMemset(h->lb_queue, 0, 2 * sizeof(uint32_t));
       if (e == hipSuccess) e = dalloc(h, &h->d_lats, (size_t)K);
       if (e == hipSuccess) e = upload_batch_args(h);
       if (e != hipSuccess) {
@@ -588,6 +590,7 @@ pfg_status pfg_destroy(pfg_handle *h) {
   dfree(h->bbmask);
   dfree(h->red);
   dfree(h->lb_block_inst);
+  dfree(h->lb_queue);
   dfree(h->d_lats);
   dfree(h->d_tile0);
   dfree(h->row_stamp);
@@ -1792,7 +1795,7 @@ hipError_t snapshot_take(pfg_handle *h, Snapshot &S) {
 }
 
 struct Shape {
-  int cpt, lanes, staging, overlap, xcd;
+  int cpt, lanes, staging, overlap, xcd, persistent;
 };
 
 hipError_t apply_shape(pfg_handle *h, const Shape &s) {
@@ -1801,6 +1804,7 @@ hipError_t apply_shape(pfg_handle *h, const Shape &s) {
   h->forces_lanes = s.lanes;
   h->ib_staging = s.staging;
   h->overlap = s.overlap != 0;
+  h->lb_persistent = h->K > 1 ? s.persistent : 0;
   if (h->lb_cpt == s.cpt) return hipSuccess;
   h->lb_cpt = s.cpt;
   if (h->K == 1) return hipSuccess;
@@ -1845,7 +1849,7 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   hipSetDevice(h->device);
   const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
   const Shape start{h->lb_cpt, h->forces_lanes, h->ib_staging, h->overlap ? 1 : 0,
-                    h->lb_xcd_rows};
+                    h->lb_xcd_rows, h->lb_persistent};
   Snapshot S;
   hipError_t e = snapshot_take(h, S);
   if (e != hipSuccess) {
@@ -1856,8 +1860,8 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
    * fastest value before the next is tried (values 0 are the automatic
    * choices); a value replaces the best only when it is 1 % faster */
   const int cpts[3] = {1, 2, 4}, lanes[7] = {64, 32, 16, 8, 4, 2, 1}, stag[2] = {1, 2},
-            ovl[2] = {0, 1}, xcds[2] = {0, 1};
-  Shape best{1, 0, 0, 0, (h->K == 1 && h->N >= XCD_ROWS_MIN_CELLS) ? 1 : 0};
+            ovl[2] = {0, 1}, xcds[2] = {0, 1}, pers[2] = {0, 1};
+  Shape best{1, 0, 0, 0, (h->K == 1 && h->N >= XCD_ROWS_MIN_CELLS) ? 1 : 0, 0};
   const int warm = GRAPH_CHUNK;
   pfg_status st = PFG_OK;
   double best_ms = 0.0;
@@ -1866,18 +1870,23 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   st = e == hipSuccess ? time_shape(h, S, steps, warm, &best_ms) : from_hip(e);
   out->ms_default = best_ms;
   tried = 1;
-  /* knobs: 0 LB cells per lane, 1 LB row order (single lattice), 2 force
-   * lanes, 3 spreading form, 4 overlap */
-  for (int knob = 0; knob < 5 && st == PFG_OK; ++knob) {
+  /* knobs: 0 LB cells per lane, 1 LB row order (single lattice), 5 the
+   * persistent batched launch (batches), 2 force lanes, 3 spreading form,
+   * 4 overlap */
+  const int order[6] = {0, 1, 5, 2, 3, 4};
+  for (int ki = 0; ki < 6 && st == PFG_OK; ++ki) {
+    const int knob = order[ki];
     const int *vals = knob == 0 ? cpts : knob == 1 ? xcds : knob == 2 ? lanes
-                                       : knob == 3 ? stag : ovl;
+                                       : knob == 3 ? stag : knob == 4 ? ovl : pers;
     const int nv = knob == 0 ? 3 : knob == 2 ? 7 : 2;
     if (knob == 1 && h->K > 1) continue; /* the batched launch keeps its table order */
-    if (knob > 1 && !markers) break;
+    if (knob == 5 && h->K == 1) continue;
+    if ((knob >= 2 && knob <= 4) && !markers) break;
     for (int v = 0; v < nv && st == PFG_OK; ++v) {
       Shape s = best;
       int *slot = knob == 0 ? &s.cpt : knob == 1 ? &s.xcd : knob == 2 ? &s.lanes
-                                     : knob == 3 ? &s.staging : &s.overlap;
+                                     : knob == 3 ? &s.staging : knob == 4 ? &s.overlap
+                                                              : &s.persistent;
       if (*slot == vals[v]) continue;
       *slot = vals[v];
       double ms = 0.0;
@@ -1905,8 +1914,17 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   out->forces_lanes = best.lanes;
   out->ib_staging = best.staging;
   out->overlap = best.overlap;
+  out->lb_persistent = best.persistent;
   out->ms_per_step = best_ms;
   out->shapes_timed = tried;
+  return PFG_OK;
+}
+
+pfg_status pfg_set_lb_persistent(pfg_handle *h, int enable) {
+  if (!h) return PFG_ERR_ARG;
+  if (h->K == 1) return enable ? PFG_ERR_STATE : PFG_OK;
+  h->lb_persistent = enable != 0;
+  drop_graph(h);
   return PFG_OK;
 }
 

@@ -368,6 +368,37 @@ __global__ __launch_bounds__(256) void lb_batch_kernel(const LBArgs *args,
                           nullptr);
 }
 
+/* A batch of lattices in one persistent launch: a grid of resident
+ * workgroups takes the table's workgroup tasks (each one x-block row of
+ * four wavefronts, of any lattice) from a work queue in task order, until the
+ * queue runs past the end.  queue[0] is the next task, queue[1] counts the
+ * workgroups that are done; the last one resets both for the next launch
+ * (every other workgroup has taken its last task by then), so a captured
+ * step needs no clearing node. */
+template <int S, int CPT, int MODE>
+__global__ __launch_bounds__(256) void lb_batch_queue_kernel(const LBArgs *args,
+                                                             const int32_t *block_inst,
+                                                             uint32_t ntasks, uint32_t *queue) {
+  __shared__ uint32_t next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const uint32_t blk = next;
+    __syncthreads();
+    if (blk >= ntasks) break;
+    const LBArgs &a = args[block_inst[blk]];
+    lb_update<S, CPT, MODE>(a, (uint32_t)(blk - a.block0), nullptr, nullptr, nullptr, nullptr);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+      queue[0] = 0u;
+      queue[1] = 0u;
+      __threadfence();
+    }
+  }
+}
+
 /* ---------------- auxiliary kernels ---------------- */
 
 /* Per-cell bounce mask from the solid flags (built once per flag upload). */
@@ -618,10 +649,38 @@ static hipError_t dispatch_lb(pfg_handle *h, int k, int mode, float *rho, float 
   }
 }
 
+/* Resident workgroups of the persistent batched launch: what the CUs hold at
+ * once (occupancy x CUs), at most the tasks there are. */
+template <class KER>
+static unsigned queue_grid(pfg_handle *h, KER kernel) {
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 4;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) !=
+          hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  const int64_t g = std::min<int64_t>((int64_t)per_cu * cus, h->lb_blocks);
+  return (unsigned)std::max<int64_t>(g, 1);
+}
+
 template <int S, int CPT, int MODE>
 static void launch_batch(pfg_handle *h, hipStream_t st) {
   const dim3 grid((unsigned)h->lb_blocks), block(256);
   const LBArgs *args = h->d_lbargs[h->cur];
+  if (h->lb_persistent && MODE == LB_MODE_STEP && h->lb_queue) {
+    if (h->p.wall[2] == PFG_WALL_VELOCITY) {
+      auto k = lb_batch_queue_kernel<S, CPT, MODE | LB_ZVEL>;
+      hipLaunchKernelGGL(k, dim3(queue_grid(h, k)), block, 0, st, args, h->lb_block_inst,
+                         (uint32_t)h->lb_blocks, h->lb_queue);
+    } else {
+      auto k = lb_batch_queue_kernel<S, CPT, MODE>;
+      hipLaunchKernelGGL(k, dim3(queue_grid(h, k)), block, 0, st, args, h->lb_block_inst,
+                         (uint32_t)h->lb_blocks, h->lb_queue);
+    }
+    return;
+  }
   if (h->p.wall[2] == PFG_WALL_VELOCITY) /* the lattices of a batch share the walls */
     hipLaunchKernelGGL((lb_batch_kernel<S, CPT, MODE | LB_ZVEL>), grid, block, 0, st, args,
                        h->lb_block_inst);

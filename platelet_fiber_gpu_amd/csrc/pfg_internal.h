@@ -266,6 +266,10 @@ struct pfg_handle {
   pfg::LBArgs *d_lbargs[2] = {nullptr, nullptr}; /* per starting copy of f, [K] */
   int32_t *lb_block_inst = nullptr;              /* batched LB: lattice per block */
   int64_t lb_blocks = 0;
+  /* batched LB: 1 = one persistent launch of resident workgroups taking the
+   * table's tasks from a work queue (lb_queue: next task, workgroups done) */
+  int lb_persistent = 0;
+  uint32_t *lb_queue = nullptr;
   pfg::MLat *d_lats = nullptr;                   /* [K] */
   int64_t *d_tile0 = nullptr;                    /* [K+1] first tile per lattice */
   std::vector<int32_t> marker0;                  /* [K+1] first marker per lattice */

@@ -24,18 +24,22 @@ def fluid_set(pfg, store):
             lattice_params(pfg, 130, 9, 5, (P, B, B), g=(1e-5, 0, 0), tau=1.1, store=store)]
 
 
+@pytest.mark.parametrize("persistent", [0, 1])
 @pytest.mark.parametrize("cpt", [1, 2, 4])
 @pytest.mark.parametrize("store", [0, 1])
-def test_batch_fluid_equals_single(gpu, store, cpt):
+def test_batch_fluid_equals_single(gpu, store, cpt, persistent):
     """Three lattices of different shapes, walls, tau and forcing: the batched
     LB launch gives bitwise the fields of three separate handles, with one,
-    two or four z-cells per lane (the shape changes only the launch)."""
+    two or four z-cells per lane, as one workgroup per x-block row or as the
+    persistent launch that takes the rows from a work queue (25 launches in
+    a row: the queue resets itself after each)."""
     pfg = gpu
     ps = fluid_set(pfg, store)
     steps = 25
     with pfg.Batch(ps) as b:
         b.set_fluid_only(True)
         b.set_lb_tuning(cpt)
+        b.set_lb_persistent(persistent)
         b.step(steps)
         got = [b.fields_of(k) for k in range(len(ps))]
         cons = [b.conservation_of(k) for k in range(len(ps))]
@@ -59,6 +63,41 @@ def test_batch_single_lattice_calls_are_refused(gpu):
         assert e.value.code == 5  # PFG_ERR_STATE
         with pytest.raises(pfg.PfgError):
             b.init_equilibrium()
+    with pfg.Solver(ps[0]) as s:
+        with pytest.raises(pfg.PfgError) as e:
+            s._c("pfg_set_lb_persistent", 1)
+        assert e.value.code == 5
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batch_persistent_coupled_equals_per_row_launch(gpu, graph):
+    """Coupled shear lattices, eagerly and as captured steps replayed with
+    host reads in between: the persistent queue launch gives the markers and
+    fields of the one-workgroup-per-row launch, up to the FP32 order of the
+    force-field atomics."""
+    pfg = gpu
+    scenes = [pfg.Scene(pfg.SCENE_BATCH, nx=48, ny=32, nz=24, elements=128, receptors=-1,
+                        ligands=-1, seed=11),
+              pfg.Scene(pfg.SCENE_BATCH, nx=40, ny=40, nz=28, n_platelets=2, elements=96,
+                        receptors=-1, ligands=-1, seed=12)]
+    runs = []
+    for persistent in (0, 1):
+        with pfg.Batch([s.params for s in scenes]) as b:
+            b.set_graph(graph)
+            b.set_lb_persistent(persistent)
+            b.load_scenes(scenes)
+            got = []
+            for _ in range(4):
+                b.step(3)
+                got.append([(b.markers_of(k), b.fields_of(k)) for k in range(len(scenes))])
+            runs.append(got)
+    for r in range(4):
+        for k in range(len(scenes)):
+            for name in ("x", "y", "z", "vx", "vy", "vz"):
+                d = np.abs(runs[0][r][k][0][name] - runs[1][r][k][0][name]).max()
+                assert d < 1e-5, (r, k, name)
+            for a, w in zip(runs[0][r][k][1], runs[1][r][k][1]):
+                assert np.abs(a - w).max() < 1e-6, (r, k)
 
 
 @pytest.mark.parametrize("staging", [1, 2])
