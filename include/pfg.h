@@ -334,6 +334,19 @@ pfg_status pfg_download_fields(pfg_handle *h, float *rho, float *ux, float *uy,
 /* FP64 totals over fluid cells: out[0] = sum rho, out[1..3] = sum rho u. */
 pfg_status pfg_conservation(pfg_handle *h, double out[4]);
 
+/* No counterpart in the reference.  pfg_download_fields for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc): the same values, staged through a fixed device array of
+ * PFG_FIELDS_SCRATCH_FLOATS floats (32 MiB, one per device, held until the
+ * process ends) a window of whole z-planes at a time; nothing is allocated
+ * or freed.  Limit: nx * ny * (number of non-NULL outputs) must not exceed
+ * PFG_FIELDS_SCRATCH_FLOATS, else PFG_ERR_ARG.  All calls on one device
+ * share the array: calls that overlap on one device (from several host
+ * threads) race and must be serialised by the caller. */
+#define PFG_FIELDS_SCRATCH_FLOATS (8 * 1024 * 1024)
+pfg_status pfg_download_fields_noalloc(pfg_handle *h, float *rho, float *ux, float *uy,
+                                       float *uz);
+
 /* ---- immersed structures ---- */
 pfg_status pfg_load_model(pfg_handle *h, const pfg_model *m);
 pfg_status pfg_upload_markers(pfg_handle *h, const float *x, const float *y,

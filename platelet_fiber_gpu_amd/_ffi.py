@@ -156,6 +156,7 @@ SIGNATURES = {
     "pfg_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tuning)]),
     "pfg_set_lb_xcd_rows": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_lb_persistent": (C.c_int, [C.c_void_p, C.c_int]),
+    "pfg_download_fields_noalloc": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_float)] * 4),
     "pfg_scene_opts_init": (None, [C.POINTER(SceneOpts), C.c_int]),
     "pfg_scene_build": (C.c_int, [C.POINTER(SceneOpts), C.POINTER(Params),
                                   C.POINTER(C.c_void_p)]),

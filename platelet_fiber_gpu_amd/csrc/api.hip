@@ -353,6 +353,8 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PFG_ERR_DEVICE;
   if (device < 0 || device >= ndev) return PFG_ERR_DEVICE;
   if (hipSetDevice(device) != hipSuccess) return PFG_ERR_DEVICE;
+  /* device address of the fixed staging array, once per device */
+  if (fields_scratch_setup(device, nullptr) != hipSuccess) return PFG_ERR_DEVICE;
 
   pfg_handle *h = new (std::nothrow) pfg_handle();
   if (!h) return PFG_ERR_ALLOC;
@@ -655,6 +657,24 @@ pfg_status pfg_download_fields(pfg_handle *h, float *rho, float *ux, float *uy,
   if (!h) return PFG_ERR_ARG;
   if (h->K != 1) return PFG_ERR_STATE;
   return fields_impl(h, 0, rho, ux, uy, uz);
+}
+
+pfg_status pfg_download_fields_noalloc(pfg_handle *h, float *rho, float *ux, float *uy,
+                                       float *uz) {
+  if (!h) return PFG_ERR_ARG;
+  if (h->K != 1) return PFG_ERR_STATE;
+  float *const host[4] = {rho, ux, uy, uz};
+  int64_t nf = 0;
+  for (int c = 0; c < 4; ++c) nf += host[c] != nullptr;
+  if ((int64_t)h->p.nx * h->p.ny * nf > FIELDS_SCRATCH_FLOATS) {
+    fprintf(stderr,
+            "pfg_download_fields_noalloc: a z-plane of %d x %d cells x %d fields exceeds the "
+            "%lld-float staging array\n",
+            h->p.nx, h->p.ny, (int)nf, (long long)FIELDS_SCRATCH_FLOATS);
+    return PFG_ERR_ARG;
+  }
+  hipSetDevice(h->device);
+  return from_hip(download_fields_noalloc(h, host));
 }
 
 pfg_status pfg_conservation(pfg_handle *h, double out[4]) {

@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "pfg_internal.h"
@@ -397,6 +398,109 @@ __global__ __launch_bounds__(256) void lb_batch_queue_kernel(const LBArgs *args,
       __threadfence();
     }
   }
+}
+
+/* ---------------- fields through a fixed device array ---------------- */
+
+/* Staging of pfg_download_fields_noalloc: the fields of a window of whole
+ * z-planes, [field][cell of the window].  One copy per device, held until the
+ * process ends; calls on one device share it. */
+static __device__ __align__(256) float scratch[FIELDS_SCRATCH_FLOATS];
+
+/* The fields pass over the x-block rows [t0, t1) (the rows of z-planes
+ * [z0, z1)), one wavefront per row; o* point at cell 0 of the window's first
+ * plane minus z0 planes, so that each cell lands at its offset in the window. */
+template <int S>
+__global__ __launch_bounds__(256) void lb_fields_window_kernel(LBArgs a, uint32_t blk0,
+                                                               uint32_t t0, uint32_t t1,
+                                                               float *orho, float *oux,
+                                                               float *ouy, float *ouz) {
+  const uint32_t blk = blk0 + blockIdx.x;
+  const uint32_t task = blk * 4u + (threadIdx.x >> 6);
+  if (task < t0 || task >= t1) return;
+  lb_update<S, 1, LB_MODE_FIELDS>(a, blk, orho, oux, ouy, ouz);
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(float *p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+/* Device address of `scratch` per device, looked up once with the device
+ * current (the array's name in host code is a host-side stand-in). */
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[FIELDS_SCRATCH_MAX_DEVICES];
+
+hipError_t fields_scratch_setup(int device, float **out) {
+  if (device < 0 || device >= FIELDS_SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = (float *)ptr;
+  }
+  if (out) *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+template <int S>
+static hipError_t fields_window_s(pfg_handle *h, const LBArgs &a, int z0, int z1,
+                                  float *const win[4]) {
+  const int64_t plane = (int64_t)a.nx * a.ny;
+  const uint32_t rows = (uint32_t)a.nxb * (uint32_t)a.ny;
+  const uint32_t t0 = (uint32_t)z0 * rows, t1 = (uint32_t)z1 * rows;
+  const uint32_t blk0 = t0 / 4u, blk1 = (t1 + 3u) / 4u;
+  float *o[4];
+  for (int c = 0; c < 4; ++c) o[c] = win[c] ? win[c] - (int64_t)z0 * plane : nullptr;
+  hipLaunchKernelGGL(lb_fields_window_kernel<S>, dim3(blk1 - blk0), dim3(256), 0, h->stream, a,
+                     blk0, t0, t1, o[0], o[1], o[2], o[3]);
+  return hipGetLastError();
+}
+
+hipError_t download_fields_noalloc(pfg_handle *h, float *const host[4]) {
+  float *base = nullptr;
+  hipError_t e = fields_scratch_setup(h->device, &base);
+  if (e != hipSuccess) return e;
+  LBArgs a;
+  make_lb_args(h, 0, h->cur, &a);
+  int nf = 0;
+  for (int c = 0; c < 4; ++c) nf += host[c] != nullptr;
+  if (nf == 0) return hipSuccess;
+  const int64_t plane = (int64_t)a.nx * a.ny;
+  const int64_t zmax = FIELDS_SCRATCH_FLOATS / (plane * nf); /* planes per window */
+  if (zmax < 1) return hipErrorInvalidValue;
+  for (int z0 = 0; z0 < a.nz && e == hipSuccess; z0 += (int)zmax) {
+    const int z1 = (int)std::min<int64_t>(a.nz, z0 + zmax);
+    const int64_t n = plane * (z1 - z0);
+    float *win[4] = {nullptr, nullptr, nullptr, nullptr};
+    int slot = 0;
+    for (int c = 0; c < 4; ++c)
+      if (host[c]) win[c] = base + n * slot++;
+    /* solid cells are not written by the pass: they read 0 */
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n * nf + 255) / 256)), dim3(256), 0,
+                       h->stream, base, n * nf);
+    e = hipGetLastError();
+    if (e != hipSuccess) break;
+    switch (h->p.store) {
+      case PFG_STORE_F16: e = fields_window_s<PFG_STORE_F16>(h, a, z0, z1, win); break;
+      case PFG_STORE_BF16: e = fields_window_s<PFG_STORE_BF16>(h, a, z0, z1, win); break;
+      default: e = fields_window_s<PFG_STORE_F32>(h, a, z0, z1, win);
+    }
+    for (int c = 0; c < 4 && e == hipSuccess; ++c)
+      if (host[c])
+        e = hipMemcpyAsync(host[c] + (int64_t)z0 * plane, win[c], sizeof(float) * n,
+                           hipMemcpyDeviceToHost, h->stream);
+    /* the next window reuses the array */
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  }
+  return e;
 }
 
 /* ---------------- auxiliary kernels ---------------- */

@@ -297,6 +297,15 @@ hipError_t launch_init_eq(pfg_handle *h, int k, const float *rho, const float *u
 hipError_t launch_convert_f(pfg_handle *h, float *f_out_or_null,
                             const float *f_in_or_null);
 hipError_t launch_conservation(pfg_handle *h, int k, double out[4]);
+/* pfg_download_fields_noalloc: staging in a fixed device array of
+ * FIELDS_SCRATCH_FLOATS floats per device (no allocation) */
+constexpr int64_t FIELDS_SCRATCH_FLOATS = PFG_FIELDS_SCRATCH_FLOATS;
+constexpr int FIELDS_SCRATCH_MAX_DEVICES = 64;
+/* the array's device address on `device` (looked up on the first call) */
+hipError_t fields_scratch_setup(int device, float **out_or_null);
+/* rho, ux, uy, uz of a single lattice to host (any may be NULL), window of
+ * whole z-planes at a time through the fixed array */
+hipError_t download_fields_noalloc(pfg_handle *h, float *const host[4]);
 
 /* markers.hip: fused marker-side launches, each with the roles it runs */
 void fill_mlat(const pfg_handle *h, int k, MLat *L);
