@@ -32,6 +32,18 @@ def test_reference_driver_links(pfg):
         assert "REF_ABI skip" in r.stdout
 
 
+def test_reference_host_declarations_link(pfg, tmp_path):
+    """The reference host's own declarations of the entry points (gsl_rng
+    opaque, no pfg header) compile without warnings and link against
+    libpfg.so; every name resolves."""
+    src = os.path.join(ROOT, "tests", "ref_abi", "ref_prototypes.c")
+    exe = str(tmp_path / "ref_prototypes")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", src, "-o", exe, "-L", PKG,
+                    "-lpfg", f"-Wl,-rpath,{PKG}", "-Wl,-rpath-link,/opt/rocm/lib"], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "resolved 13" in r.stdout, r.stdout + r.stderr
+
+
 @pytest.mark.gpu
 def test_reference_driver_matches_pfg(gpu):
     with tempfile.TemporaryDirectory() as d:
