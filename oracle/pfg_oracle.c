@@ -27,6 +27,12 @@
  *                               reference (oracle/_ref) and Newton's third law.
  *   bonds                       Philox4x32-10 known-answer vectors
  *                               (Random123); p = 1 - exp(-k dt) closed form;
+ *                               the reference law's rates (sem_kernel.cu:
+ *                               1852-1863) in detailed balance with the bond's
+ *                               spring energy, k_on/k_off = (Kf0/Kr0) e^{-E/kT},
+ *                               and Bell's off rate in the force k|l0 - r| for
+ *                               stretched and compressed bonds (rates measured
+ *                               from the decisions);
  *                               the bond spring as minus the gradient of its
  *                               energy under rigid translation (finite
  *                               differences), shares proportional to 1/r_k,
@@ -36,6 +42,11 @@
  *                               (finite differences), the a/h small-gap limit,
  *                               zero above hc, clamped below hc/100, on the
  *                               lowest element only.
+ *   Langevin leapfrog (BBK)     free decay of the velocity against the closed
+ *                               form e^{-g t}; without friction the discrete
+ *                               harmonic oscillator's exact recursion;
+ *                               equipartition <m v^2> = T; with drift weight 1
+ *                               the element follows the fluid exactly.
  *   update / kick               update: free markers in a uniform flow (an
  *                               exact fixed point of the lattice) read the
  *                               flow velocity and move by u dt per step,

@@ -182,3 +182,69 @@ def test_marker_update_in_uniform_flow(pfg, po):
     assert np.allclose(V[:3], u0[None, :], rtol=0, atol=1e-14)
     assert np.allclose(X[:3], X0[:3] + n * dt * u0[None, :], rtol=0, atol=1e-12)
     assert np.all(X[3] == X0[3]) and np.all(V[3] == 0)
+
+
+def _bond_rates(pfg, po, law, r, steps=50, n_side=20):
+    """Formation and rupture rates per step of a receptor with one ligand at
+    distance r: n_side^2 receptors (one per triangle, on its first vertex),
+    each with its own ligand straight below and no other within reach,
+    decided afresh at each of `steps` steps (bonds reset before each), so
+    the fractions estimate p = 1 - e^{-k} over n_side^2 * steps draws."""
+    p = pfg.default_params(nx=64, ny=64, nz=8)
+    p.morse_u0 = 0.0
+    p.k_on, p.k_off0 = 0.3, 0.2
+    p.bond_k, p.bond_l0, p.bond_reach = 1.0, 0.3, 1.4
+    p.bond_law, p.bell_xb_kT, p.bell_kT = law, 0.5, 0.5
+    n = n_side * n_side
+    g = np.arange(n)
+    cx, cy = 1.0 + 3.0 * (g % n_side), 1.0 + 3.0 * (g // n_side)
+    v = np.zeros((3 * n, 3))
+    v[0::3] = np.stack([cx, cy, np.full(n, 4.0)], 1)
+    v[1::3] = np.stack([cx + 0.8, cy, np.full(n, 4.0)], 1)
+    v[2::3] = np.stack([cx, cy + 0.8, np.full(n, 4.0)], 1)
+    tri = np.arange(3 * n, dtype=np.int32)
+    S = np.full(n, 0.32)
+    M = ModelArrays(v[:, 0], v[:, 1], v[:, 2], plt_elem_start=[0, 3 * n],
+                    plt_tri_start=[0, n], tri=tri, tri_S0=S, plt_S0=[S.sum()], plt_V0=[-1.0],
+                    rec_tri=g, rec_r1=np.zeros(n), rec_r2=np.zeros(n),
+                    lig_x=cx, lig_y=cy, lig_z=np.full(n, 4.0 - r))
+    o = po.Oracle(p, model=M.model)
+    formed = ruptured = 0
+    for s in range(steps):
+        for bonded in (False, True):
+            o.set_step(1000 + s)
+            o.set_bonds(g.astype(np.int32) if bonded else np.full(n, -1, np.int32))
+            for st in (0, 1, 2):
+                o.stage(st)
+            rb = o.bonds()[0]
+            if bonded:
+                ruptured += int((rb < 0).sum())
+            else:
+                formed += int((rb == g).sum())
+                assert np.all((rb == g) | (rb < 0))  # only its own ligand is in reach
+    o.close()
+    w = abs(float(np.float32(4.0)) - float(np.float32(4.0 - r)) - p.bond_l0)
+    k_on = -np.log1p(-formed / (n * steps))
+    k_off = -np.log1p(-ruptured / (n * steps))
+    return w, k_on, k_off, p
+
+
+def test_reference_bond_law_detailed_balance(pfg, po):
+    """The reference's force-dependent rates (sem_kernel.cu:1852-1863) obey
+    detailed balance with the bond's spring energy E = k w^2 / 2, w = |l0 - r|:
+    k_on / k_off = (Kf0 / Kr0) e^{-E / kT}, and the off rate follows Bell's
+    law in the bond force f = k w: k_off = Kr0 e^{x_b f / kT}, for stretched
+    and compressed bonds alike.  Under the Bell law option the on rate does
+    not depend on the distance and a compressed bond feels no force.  Each
+    rate is estimated from 20 000 draws (log-rate errors ~0.02)."""
+    for r in (0.05, 0.7, 1.1):  # compressed (w = 0.25), stretched (w = 0.4, 0.8)
+        w, kon, koff, p = _bond_rates(pfg, po, pfg.BOND_REFERENCE, r)
+        E = 0.5 * p.bond_k * w * w
+        lhs = np.log(kon / koff) - np.log(p.k_on / p.k_off0)
+        assert abs(lhs + E / p.bell_kT) < 0.08, (r, lhs, -E / p.bell_kT)
+        assert abs(np.log(koff / p.k_off0) - p.bell_xb_kT * p.bond_k * w) < 0.08, (r, koff)
+    for r in (0.05, 1.1):
+        w, kon, koff, p = _bond_rates(pfg, po, pfg.BOND_BELL, r)
+        assert abs(np.log(kon / p.k_on)) < 0.08, (r, kon)
+        f = p.bond_k * w if r > p.bond_l0 else 0.0
+        assert abs(np.log(koff / p.k_off0) - p.bell_xb_kT * f) < 0.08, (r, koff)
