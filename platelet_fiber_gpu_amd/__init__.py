@@ -396,7 +396,8 @@ class Solver:
         self._c("pfg_set_fluid_only", int(bool(on)))
 
     def set_ib_staging(self, mode):
-        """IB spreading form: 0 automatic, 1 LDS-staged chunks, 2 direct."""
+        """IB spreading form: 0 automatic, 1 LDS-staged chunks, 2 direct,
+        3 tile gather (no device-wide atomics)."""
         self._c("pfg_set_ib_staging", int(mode))
 
     def set_forces_lanes(self, lanes):

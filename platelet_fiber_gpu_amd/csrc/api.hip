@@ -103,6 +103,7 @@ void free_markers(Markers &m) {
   dfree(m.bin_count); dfree(m.bin_start); dfree(m.bin_items);
   dfree(m.marker_bin); dfree(m.marker_slot); dfree(m.scan_partials);
   dfree(m.chunk_start); dfree(m.chunk_items); dfree(m.bin_scatter); dfree(m.sorted_pos);
+  dfree(m.tile_list); dfree(m.tile_count); dfree(m.spread_rec);
   Markers fresh;
   fresh.counters = m.counters;
   fresh.ticket = m.ticket;
@@ -464,8 +465,8 @@ pfg_status create_impl(const pfg_params *ps, int K, int device, pfg_handle **out
       if (e == hipSuccess) e = dalloc(h, &h->d_tile0, tile0.size());
       if (e == hipSuccess) e = upload(h, h->d_tile0, tile0.data(), tile0.size());
       for (int c = 0; c < 2 && e == hipSuccess; ++c) e = dalloc(h, &h->d_lbargs[c], (size_t)K);
-      if (e == hipSuccess) e = dalloc(h, &h->lb_queue, 2);
-      if (e == hipSuccess) e = hipMemset(h->lb_queue, 0, 2 * sizeof(uint32_t));
+      if (e == hipSuccess) e = dalloc(h, &h->lb_queue, LB_QUEUES + 1);
+      if (e == hipSuccess) e = hipMemset(h->lb_queue, 0, (LB_QUEUES + 1) * sizeof(uint32_t));
       if (e == hipSuccess) e = dalloc(h, &h->d_lats, (size_t)K);
       if (e == hipSuccess) e = upload_batch_args(h);
       if (e != hipSuccess) {
@@ -552,7 +553,9 @@ pfg_status fields_impl(pfg_handle *h, int k, float *rho, float *ux, float *uy, f
       st = PFG_ERR_ALLOC;
       break;
     }
-    hipMemsetAsync(dev[c], 0, sizeof(float) * n, h->stream);
+    /* solid cells are not written by the fields pass: they read 0 */
+    st = from_hip(hipMemsetAsync(dev[c], 0, sizeof(float) * n, h->stream));
+    if (st != PFG_OK) break;
   }
   if (st == PFG_OK) st = from_hip(launch_lb_fields(h, k, dev[0], dev[1], dev[2], dev[3]));
   for (int c = 0; c < 4 && st == PFG_OK; ++c)
@@ -1103,6 +1106,15 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   if (minst) ALLOC_UP(m.minst, minst, n);
   ALLOC_FILL(m.bin_scatter, 0, n);
   ALLOC_FILL(m.sorted_pos, make_float4(0.f, 0.f, 0.f, 0.f), n);
+  ALLOC_FILL(m.tile_list, 0, h->ntiles > 0 ? h->ntiles : 1);
+  ALLOC_FILL(m.tile_count, 0, 2);
+  if (e == hipSuccess) e = dalloc(h, &m.spread_rec, n > 0 ? (size_t)n : 1);
+  m.tile_gather_ok = 1;
+  for (const Inst &I : h->inst) {
+    const int ext[3] = {TILE_X, TILE_Y, TILE_Z}, len[3] = {I.p.nx, I.p.ny, I.p.nz};
+    for (int ax = 0; ax < 3; ++ax)
+      if (I.p.wall[ax] == PFG_WALL_PERIODIC && len[ax] < 2 * ext[ax]) m.tile_gather_ok = 0;
+  }
 #undef ALLOC_UP
 #undef ALLOC_FILL
   if (e != hipSuccess) {
@@ -1615,7 +1627,7 @@ pfg_status pfg_set_overlap(pfg_handle *h, int enable) {
 }
 
 pfg_status pfg_set_ib_staging(pfg_handle *h, int mode) {
-  if (!h || mode < 0 || mode > 2) return PFG_ERR_ARG;
+  if (!h || mode < 0 || mode > 3) return PFG_ERR_ARG;
   h->ib_staging = mode;
   drop_graph(h);
   return PFG_OK;
@@ -1652,6 +1664,14 @@ pfg_status pfg_set_step(pfg_handle *h, int64_t step) {
   if (!h || step < 0) return PFG_ERR_ARG;
   hipSetDevice(h->device);
   PFG_TRY(upload(h, h->m.d_step, &step, 1));
+  /* stamps of other steps could equal the new index: no tile or row keeps
+   * one (the tile-gather spreading lists a tile when its stamp changes) */
+  PFG_TRY(hipMemset(h->tile_stamp, 0xff, sizeof(int32_t) * (size_t)h->ntiles));
+  if (h->row_stamp) {
+    size_t rows = 0;
+    for (const Inst &I : h->inst) rows += (size_t)I.nxb * I.p.ny * I.p.nz;
+    PFG_TRY(hipMemset(h->row_stamp, 0xff, sizeof(int32_t) * rows));
+  }
   h->h_step = step;
   return PFG_OK;
 }
@@ -1879,7 +1899,7 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   /* knobs, tried one at a time from the automatic shape, each kept at its
    * fastest value before the next is tried (values 0 are the automatic
    * choices); a value replaces the best only when it is 1 % faster */
-  const int cpts[3] = {1, 2, 4}, lanes[7] = {64, 32, 16, 8, 4, 2, 1}, stag[2] = {1, 2},
+  const int cpts[3] = {1, 2, 4}, lanes[7] = {64, 32, 16, 8, 4, 2, 1}, stag[3] = {1, 2, 3},
             ovl[2] = {0, 1}, xcds[2] = {0, 1}, pers[2] = {0, 1};
   Shape best{1, 0, 0, 0, (h->K == 1 && h->N >= XCD_ROWS_MIN_CELLS) ? 1 : 0, 0};
   const int warm = GRAPH_CHUNK;
@@ -1890,7 +1910,7 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   st = e == hipSuccess ? time_shape(h, S, steps, warm, &best_ms) : from_hip(e);
   out->ms_default = best_ms;
   tried = 1;
-  /* knobs: 0 LB cells per lane, 1 LB row order (single lattice), 5 the
+  /* knobs: 0 LB cells per lane, 1 LB row order, 5 the
    * persistent batched launch (batches), 2 force lanes, 3 spreading form,
    * 4 overlap */
   const int order[6] = {0, 1, 5, 2, 3, 4};
@@ -1898,8 +1918,7 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
     const int knob = order[ki];
     const int *vals = knob == 0 ? cpts : knob == 1 ? xcds : knob == 2 ? lanes
                                        : knob == 3 ? stag : knob == 4 ? ovl : pers;
-    const int nv = knob == 0 ? 3 : knob == 2 ? 7 : 2;
-    if (knob == 1 && h->K > 1) continue; /* the batched launch keeps its table order */
+    const int nv = knob == 0 ? 3 : knob == 2 ? 7 : knob == 3 ? (h->m.tile_gather_ok ? 3 : 2) : 2;
     if (knob == 5 && h->K == 1) continue;
     if ((knob >= 2 && knob <= 4) && !markers) break;
     for (int v = 0; v < nv && st == PFG_OK; ++v) {

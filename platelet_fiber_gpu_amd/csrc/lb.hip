@@ -160,10 +160,12 @@ __device__ __forceinline__ void velocity_face(const LBArgs &a, int x, int y, int
 }
 
 /* One wavefront per (x-block, y, CPT consecutive z); workgroup `blk` of the
- * lattice described by `a` (in memory that the whole workgroup reads). */
+ * lattice described by `a` (in memory that the whole workgroup reads), the
+ * wavefront's place in it `wave` (default: its place in the launch's
+ * workgroup). */
 template <int S, int CPT, int MODE>
 __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *orho,
-                                          float *oux, float *ouy, float *ouz) {
+                                          float *oux, float *ouy, float *ouz, int wave = -1) {
   constexpr int M = MODE & 3;
   constexpr bool ZV = (MODE & LB_ZVEL) != 0;
   using ST = Store<S>;
@@ -175,7 +177,8 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
   const int lane = threadIdx.x & (XBLOCK - 1);
   /* wavefront-uniform task decomposition in 32-bit arithmetic (x-blocks per
    * lattice stay below 2^31) */
-  const uint32_t task = blk * 4u + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t task =
+      blk * 4u + __builtin_amdgcn_readfirstlane(wave < 0 ? (int)(threadIdx.x >> 6) : wave);
   const uint32_t rows = (uint32_t)a.nxb * (uint32_t)a.ny;
   const uint32_t zc = task / rows;
   if (zc * CPT >= (uint32_t)a.nz) return;
@@ -223,6 +226,24 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
     const int zp = z1 ? (a.periodic[2] ? 0 : z) : z + 1;
     const bool ozm = z0 && !a.periodic[2], ozp = z1 && !a.periodic[2];
     const uint64_t bown = (uint32_t)xb + (uint32_t)a.nxb * (uint32_t)(y + a.ny * z);
+
+    /* the IB force of the tiles this step's stencils meet, read (and
+     * consumed: the next step's spreading adds into zeros) before the
+     * populations, so that its latency overlaps theirs */
+    bool flagged = false;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    if (M == LB_MODE_STEP || M == LB_MODE_REGION) {
+      const int tile = (x >> TILE_SHIFT_X) + a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
+      flagged = a.tile_stamp[tile] == step32;
+      if (flagged) {
+        ax = a.aux[idx];
+        ay = a.aux[a.plane + idx];
+        az = a.aux[2 * a.plane + idx];
+        a.aux[idx] = 0.f;
+        a.aux[a.plane + idx] = 0.f;
+        a.aux[2 * a.plane + idx] = 0.f;
+      }
+    }
 
     float dev[Q];
 #pragma unroll
@@ -284,18 +305,10 @@ __device__ __forceinline__ void lb_update(const LBArgs &a, uint32_t blk, float *
     }
     const float rho = 1.0f + drho;
     float Fx = a.g[0], Fy = a.g[1], Fz = a.g[2];
-    bool flagged = false;
-    if (M == LB_MODE_STEP || M == LB_MODE_REGION) {
-      const int tile = (x >> TILE_SHIFT_X) + a.tnx * ((y >> TILE_SHIFT_Y) + a.tny * (z >> TILE_SHIFT_Z));
-      flagged = a.tile_stamp[tile] == step32;
-      if (flagged) {
-        Fx += a.aux[idx];
-        Fy += a.aux[a.plane + idx];
-        Fz += a.aux[2 * a.plane + idx];
-        a.aux[idx] = 0.f; /* consumed: the next step's spreading adds into zeros */
-        a.aux[a.plane + idx] = 0.f;
-        a.aux[2 * a.plane + idx] = 0.f;
-      }
+    if (flagged) {
+      Fx += ax;
+      Fy += ay;
+      Fz += az;
     }
     const float inv_rho = __builtin_amdgcn_rcpf(rho); /* hardware reciprocal, 1 ulp */
     const float ux = (jx + 0.5f * Fx) * inv_rho;
@@ -360,41 +373,63 @@ __global__ __launch_bounds__(256) void lb_fused_kernel(LBArgs a, float *orho, fl
 }
 
 /* A batch of lattices in one launch: workgroup -> lattice through a table,
- * each lattice's arguments read from device memory. */
+ * each lattice's arguments read from device memory.  xcd_rows: each XCD
+ * takes a contiguous range of the table (as in the single-lattice kernel). */
 template <int S, int CPT, int MODE>
 __global__ __launch_bounds__(256) void lb_batch_kernel(const LBArgs *args,
-                                                       const int32_t *block_inst) {
-  const LBArgs &a = args[block_inst[blockIdx.x]];
-  lb_update<S, CPT, MODE>(a, (uint32_t)(blockIdx.x - a.block0), nullptr, nullptr, nullptr,
-                          nullptr);
+                                                       const int32_t *block_inst,
+                                                       int xcd_rows) {
+  uint32_t blk = blockIdx.x;
+  if (xcd_rows) {
+    const uint32_t q = gridDim.x / 8u;
+    if (blk < 8u * q) blk = (blk % 8u) * q + blk / 8u;
+  }
+  const LBArgs &a = args[block_inst[blk]];
+  lb_update<S, CPT, MODE>(a, (uint32_t)(blk - a.block0), nullptr, nullptr, nullptr, nullptr);
 }
 
 /* A batch of lattices in one persistent launch: a grid of resident
- * workgroups takes the table's workgroup tasks (each one x-block row of
- * four wavefronts, of any lattice) from a work queue in task order, until the
- * queue runs past the end.  queue[0] is the next task, queue[1] counts the
- * workgroups that are done; the last one resets both for the next launch
- * (every other workgroup has taken its last task by then), so a captured
- * step needs no clearing node. */
+ * workgroups whose wavefronts take the table's x-block rows (of any
+ * lattice; row r is wavefront r % 4 of workgroup task r / 4) from a device
+ * work queue until it is empty.  The rows are cut into LB_QUEUES contiguous
+ * ranges, one per XCD (the hardware deals workgroup w to XCD w % 8), each
+ * with its own counter, so neighbouring rows stay in one L2 and the counters
+ * are not one hot spot; a grab takes LB_QUEUE_GRAB consecutive rows, and a
+ * wavefront whose range is used up moves on to the next range.  Wavefronts
+ * run on their own (no barrier).  queue[LB_QUEUES] counts the wavefronts
+ * that are done; the last one resets every counter for the next launch (all
+ * others have made their last grab by then), so a captured step needs no
+ * clearing node. */
 template <int S, int CPT, int MODE>
 __global__ __launch_bounds__(256) void lb_batch_queue_kernel(const LBArgs *args,
                                                              const int32_t *block_inst,
                                                              uint32_t ntasks, uint32_t *queue) {
-  __shared__ uint32_t next;
-  for (;;) {
-    if (threadIdx.x == 0) next = atomicAdd(&queue[0], 1u);
-    __syncthreads();
-    const uint32_t blk = next;
-    __syncthreads();
-    if (blk >= ntasks) break;
-    const LBArgs &a = args[block_inst[blk]];
-    lb_update<S, CPT, MODE>(a, (uint32_t)(blk - a.block0), nullptr, nullptr, nullptr, nullptr);
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nrows = ntasks * 4u;
+  const uint32_t r0 = blockIdx.x % LB_QUEUES;
+  for (uint32_t k = 0; k < LB_QUEUES; ++k) {
+    const uint32_t r = (r0 + k) % LB_QUEUES;
+    const uint32_t lo = (uint32_t)((uint64_t)nrows * r / LB_QUEUES);
+    const uint32_t hi = (uint32_t)((uint64_t)nrows * (r + 1) / LB_QUEUES);
+    for (;;) {
+      uint32_t got = 0;
+      if (lane == 0) got = atomicAdd(&queue[r], (uint32_t)LB_QUEUE_GRAB);
+      got = __builtin_amdgcn_readfirstlane(__shfl(got, 0, 64));
+      if (got >= hi - lo) break;
+      const uint32_t t0 = lo + got;
+      const uint32_t t1 = t0 + LB_QUEUE_GRAB < hi ? t0 + LB_QUEUE_GRAB : hi;
+      for (uint32_t row = t0; row < t1; ++row) {
+        const uint32_t blk = row >> 2;
+        const LBArgs &a = args[block_inst[blk]];
+        lb_update<S, CPT, MODE>(a, (uint32_t)(blk - a.block0), nullptr, nullptr, nullptr,
+                                nullptr, (int)(row & 3u));
+      }
+    }
   }
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     __threadfence();
-    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
-      queue[0] = 0u;
-      queue[1] = 0u;
+    if (atomicAdd(&queue[LB_QUEUES], 1u) == gridDim.x * 4u - 1u) {
+      for (int r = 0; r <= LB_QUEUES; ++r) queue[r] = 0u;
       __threadfence();
     }
   }
@@ -785,12 +820,13 @@ static void launch_batch(pfg_handle *h, hipStream_t st) {
     }
     return;
   }
+  const int xcd = h->lb_xcd_rows;
   if (h->p.wall[2] == PFG_WALL_VELOCITY) /* the lattices of a batch share the walls */
     hipLaunchKernelGGL((lb_batch_kernel<S, CPT, MODE | LB_ZVEL>), grid, block, 0, st, args,
-                       h->lb_block_inst);
+                       h->lb_block_inst, xcd);
   else
     hipLaunchKernelGGL((lb_batch_kernel<S, CPT, MODE>), grid, block, 0, st, args,
-                       h->lb_block_inst);
+                       h->lb_block_inst, xcd);
 }
 
 /* The workgroup table follows h->lb_cpt; the row-split passes take one

@@ -238,8 +238,6 @@ __device__ inline int stencil_node(float X, int k, int n, int periodic, float *w
 
 /* ---------------- K1: tile clear + bin counting ---------------- */
 
-/* Block `block` scans 256 tile flags; the flagged tiles (16x4x4 = 256 cells
- * each) are zeroed by the whole block and unflagged. */
 /* Lattice of global tile g of a batch (tile0s ascending). */
 __device__ inline int lat_of_tile(const MArgs &a, int64_t g) {
   int lo = 0, hi = a.K - 1;
@@ -252,7 +250,8 @@ __device__ inline int lat_of_tile(const MArgs &a, int64_t g) {
 }
 
 /* Zero the force of the tiles that this step's spreading touched and take
- * their stamps back (a spread that no LB update will consume). */
+ * their stamps back (a spread that no LB update will consume): block `block`
+ * scans 256 tile stamps and zeroes the stamped tiles together. */
 __device__ void clear_tiles(const MArgs &a, int block) {
   __shared__ int64_t list[256];
   __shared__ int count;
@@ -264,11 +263,15 @@ __device__ void clear_tiles(const MArgs &a, int block) {
   const bool mine = my < a.ntiles && stamps[my] == step;
   if (mine) list[atomicAdd(&count, 1)] = my;
   __syncthreads();
-  const int t = threadIdx.x;
+  /* thread t takes cell t % TILE_CELLS of tile k = t / TILE_CELLS (mod the
+   * tiles per pass) */
+  constexpr int PER = 256 / TILE_CELLS;
+  static_assert(PER * TILE_CELLS == 256, "whole tiles per pass");
+  const int t = threadIdx.x % TILE_CELLS;
   const int lx = t & (TILE_X - 1);
   const int ly = (t >> TILE_SHIFT_X) & (TILE_Y - 1);
   const int lz = t >> (TILE_SHIFT_X + TILE_SHIFT_Y);
-  for (int k = 0; k < count; ++k) {
+  for (int k = (int)threadIdx.x / TILE_CELLS; k < count; k += PER) {
     const MLat L = a.lats ? a.lats[lat_of_tile(a, list[k])] : a.lat;
     const int64_t tile = list[k] - L.tile0;
     const int tx = (int)(tile % L.tnx);
@@ -1781,6 +1784,285 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   }
 }
 
+/* Tile-gather spreading.  A first pass, over the markers in cell-list
+ * order, writes each IB-coupled marker's record (first stencil node, the
+ * 3 x 4 kernel weights and the force) into its cell-list slot and stamps the
+ * 4x4x4 tiles its 4x4x4 stencil meets with the step, listing every tile
+ * once (the lane that changes a stamp lists the tile).  Then a wavefront
+ * per listed tile finds the markers whose stencils can reach it on the cell
+ * lists (the bins of the cells within two of the tile), stages 64 of their
+ * records in LDS at a time and adds them, one marker after the other, to
+ * its copy of the tile in LDS, lane l taking stencil node (l & 3,
+ * (l >> 2) & 3, l >> 4) (a marker's 64 nodes are distinct, and a
+ * wavefront's LDS adds complete in issue order), then writes the tile with
+ * plain stores.  No device-wide atomics, and the sums follow the cell-list
+ * order, so the field is the same from run to run.
+ * Each contribution is the direct form's w_x w_y w_z s F bit for bit.
+ * Needs every periodic axis at least twice the tile's extent along it (a
+ * node then lies in a tile in one periodic image only). */
+
+/* Distinct tiles (at most two) of the stencil nodes o..o+3 along one axis;
+ * nodes outside a wall-bounded axis are dropped. */
+__device__ inline int stencil_tiles(int o, int n, int periodic, int shift, int t[2]) {
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = axis_index(o + k, n, periodic);
+    if (i < 0) continue;
+    const int tt = i >> shift;
+    if (cnt == 0 || (tt != t[cnt - 1] && (cnt < 2 || tt != t[0]))) {
+      if (cnt < 2) t[cnt++] = tt;
+    }
+  }
+  return cnt;
+}
+
+__global__ __launch_bounds__(256) void spread_prep_kernel(MArgs a) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int tx[2] = {0, 0}, ty[2] = {0, 0}, tz[2] = {0, 0};
+  int ntx = 0, nty = 0, ntz = 0;
+  int64_t tile0 = 0;
+  int tnx = 1, tny = 1;
+  if (s < a.m.n && !halted(a)) {
+    const int i = a.m.bin_items[s];
+    SpreadRec r;
+    r.o = make_int4(0, 0, 0, 0);
+    if (ib_coupled(a.m.kind[i], a.fiber_move)) {
+      const MLat L = lat_of(a, i);
+      const float W[3] = {wrap_coord(a.m.x[i], L.nx, L.periodic[0]),
+                          wrap_coord(a.m.y[i], L.ny, L.periodic[1]),
+                          wrap_coord(a.m.z[i], L.nz, L.periodic[2])};
+      const int ox = (int)floorf(W[0]) - 1, oy = (int)floorf(W[1]) - 1,
+                oz = (int)floorf(W[2]) - 1;
+      r.o = make_int4(ox, oy, oz, 1);
+      r.wx = make_float4(stencil_weight(W[0], 0), stencil_weight(W[0], 1),
+                         stencil_weight(W[0], 2), stencil_weight(W[0], 3));
+      r.wy = make_float4(stencil_weight(W[1], 0), stencil_weight(W[1], 1),
+                         stencil_weight(W[1], 2), stencil_weight(W[1], 3));
+      r.wz = make_float4(stencil_weight(W[2], 0), stencil_weight(W[2], 1),
+                         stencil_weight(W[2], 2), stencil_weight(W[2], 3));
+      r.f = make_float4(a.m.fx[i], a.m.fy[i], a.m.fz[i], 0.f);
+      ntx = stencil_tiles(ox, L.nx, L.periodic[0], TILE_SHIFT_X, tx);
+      nty = stencil_tiles(oy, L.ny, L.periodic[1], TILE_SHIFT_Y, ty);
+      ntz = stencil_tiles(oz, L.nz, L.periodic[2], TILE_SHIFT_Z, tz);
+      tile0 = L.tile0;
+      tnx = L.tnx;
+      tny = L.tny;
+    }
+    a.m.spread_rec[s] = r;
+  }
+  /* the wavefront's distinct tiles, one per lane (neighbouring slots hold
+   * neighbouring markers, so a wavefront meets few), then each stamped and
+   * listed by its lane */
+  const int32_t step = (int32_t)*a.m.d_step;
+  int32_t *stamps = a.lats ? a.lats[0].tile_stamp : a.lat.tile_stamp; /* all lattices' tiles */
+  int64_t mine = -1;
+  int nd = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int q = c & 1, b = (c >> 1) & 1, z = c >> 2;
+    int64_t t = (q < ntx && b < nty && z < ntz)
+                    ? tile0 + tx[q] + (int64_t)tnx * (ty[b] + (int64_t)tny * tz[z])
+                    : -1;
+    for (;;) {
+      const uint64_t todo = __ballot(t >= 0);
+      if (todo == 0) break;
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const int64_t g = __shfl(t, leader, 64);
+      if (t == g) t = -1;
+      if (__ballot(mine == g) == 0) { /* not collected yet */
+        if (lane == nd) mine = g;
+        if (++nd == 64) { /* full: stamp these before collecting more */
+          if (atomicExch(&stamps[mine], step) != step)
+            a.m.tile_list[atomicAdd(&a.m.tile_count[0], 1)] = (int32_t)mine;
+          mine = -1;
+          nd = 0;
+        }
+      }
+    }
+  }
+  if (mine >= 0 && atomicExch(&stamps[mine], step) != step)
+    a.m.tile_list[atomicAdd(&a.m.tile_count[0], 1)] = (int32_t)mine;
+}
+
+/* Cell ranges [c0, c1] (at most two, wrapped into [0, n) on a periodic axis,
+ * clamped on a wall-bounded one, where markers outside sit in the edge
+ * cells' bins) -> bin ranges. */
+__device__ inline int bin_ranges(int c0, int c1, int n, int periodic, int nb, int lo[2],
+                                 int hi[2]) {
+  int r = 0;
+  if (!periodic) {
+    c0 = max(c0, 0);
+    c1 = min(c1, n - 1);
+    if (c0 > c1) return 0;
+    lo[0] = c0 * nb / n;
+    hi[0] = c1 * nb / n;
+    return 1;
+  }
+  if (c1 - c0 + 1 >= n) {
+    lo[0] = 0;
+    hi[0] = nb - 1;
+    return 1;
+  }
+  int a0 = c0 % n;
+  if (a0 < 0) a0 += n;
+  const int a1 = a0 + (c1 - c0);
+  if (a1 < n) {
+    lo[r] = a0 * nb / n;
+    hi[r++] = a1 * nb / n;
+  } else {
+    lo[r] = a0 * nb / n;
+    hi[r++] = nb - 1;
+    lo[r] = 0;
+    hi[r++] = (a1 - n) * nb / n;
+  }
+  return r;
+}
+
+/* Offset of a marker's first stencil node from the tile origin t0 along one
+ * axis, in the periodic image nearest to the tile. */
+__device__ inline int tile_offset(int o, int t0, int n, int periodic) {
+  int r = o - t0;
+  if (periodic) {
+    if (r > n / 2) r -= n;
+    else if (r < -(n / 2)) r += n;
+  }
+  return r;
+}
+
+__device__ inline float pick4(const float4 &v, int k) {
+  return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+constexpr int SPREAD_TILE_SEGS = 128; /* bin-row segments per tile: <= 7 x 7 x 2 */
+/* a wavefront's copy of the tile in LDS: node (x, y, z) at x + 36 y + 144 z,
+ * so the 32 nodes one half-wavefront adds (4 x 4 x 2 of a stencil) fall in
+ * 32 distinct banks */
+constexpr int ACC_SY = 36, ACC_SZ = 144, ACC_N = 560;
+static_assert(TILE_X <= 16 && TILE_Y <= 4 && TILE_Z <= 4 &&
+                  (TILE_X - 1) + ACC_SY * (TILE_Y - 1) + ACC_SZ * (TILE_Z - 1) < ACC_N,
+              "the padded tile fits its copy");
+
+/* One wavefront per listed tile (a grid of wavefronts taking the list in
+ * turn). */
+__global__ __launch_bounds__(256) void spread_tiles_kernel(MArgs a) {
+  __shared__ float sacc[4][3 * ACC_N];
+  __shared__ SpreadRec sstage[4][64];
+  __shared__ int sfirst[4][SPREAD_TILE_SEGS + 1]; /* candidates before segment s */
+  __shared__ int sslot[4][SPREAD_TILE_SEGS];      /* first cell-list slot of segment s */
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float *acc = sacc[wave];
+  SpreadRec *stage = sstage[wave];
+  int *seg_first = sfirst[wave], *seg_slot = sslot[wave];
+  const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
+  const int count = a.m.tile_count[0];
+  const int nwaves = (int)gridDim.x * 4;
+  const int32_t *bstart = a.m.bin_start;
+  for (int k = (int)blockIdx.x * 4 + wave; k < count; k += nwaves) {
+    const int64_t g = a.m.tile_list[k];
+    const MLat L = a.lats ? a.lats[lat_of_tile(a, g)] : a.lat;
+    const int64_t tile = g - L.tile0;
+    const int x0 = (int)(tile % L.tnx) * TILE_X;
+    const int y0 = (int)((tile / L.tnx) % L.tny) * TILE_Y;
+    const int z0 = (int)(tile / ((int64_t)L.tnx * L.tny)) * TILE_Z;
+    for (int q = lane; q < 3 * ACC_N; q += 64) acc[q] = 0.f;
+    /* the markers whose cell c has a stencil node c-1..c+2 in the tile:
+     * c in [t0 - 2, t0 + extent] along each axis */
+    int xl[2], xh[2], yl[2], yh[2], zl[2], zh[2];
+    const int ex = min(TILE_X, L.nx - x0), ey = min(TILE_Y, L.ny - y0),
+              ez = min(TILE_Z, L.nz - z0);
+    const int nxr = bin_ranges(x0 - 2, x0 + ex, L.nx, L.periodic[0], L.nb[0], xl, xh);
+    const int nyr = bin_ranges(y0 - 2, y0 + ey, L.ny, L.periodic[1], L.nb[1], yl, yh);
+    const int nzr = bin_ranges(z0 - 2, z0 + ez, L.nz, L.periodic[2], L.nb[2], zl, zh);
+    /* the bin-row segments (x-bin range of one (y, z) bin row), two per
+     * lane, and their candidate counts scanned into seg_first */
+    const int ly0 = yh[0] - yl[0] + 1, lz0 = zh[0] - zl[0] + 1;
+    const int nyb = ly0 + (nyr > 1 ? yh[1] - yl[1] + 1 : 0);
+    const int nzb = lz0 + (nzr > 1 ? zh[1] - zl[1] + 1 : 0);
+    const int ns = min(nxr * nyb * nzb, SPREAD_TILE_SEGS);
+    int carry = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int sg = lane + 64 * h;
+      int cnt = 0, s0 = 0;
+      if (sg < ns) {
+        const int rx = sg % nxr, iy = (sg / nxr) % nyb, iz = sg / (nxr * nyb);
+        const int by = iy < ly0 ? yl[0] + iy : yl[1] + (iy - ly0);
+        const int bz = iz < lz0 ? zl[0] + iz : zl[1] + (iz - lz0);
+        const int row = L.bin0 + L.nb[0] * (by + L.nb[1] * bz);
+        s0 = bstart[row + xl[rx]];
+        cnt = bstart[row + xh[rx] + 1] - s0;
+      }
+      int v = cnt; /* inclusive scan across the wavefront */
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+      }
+      if (sg < ns) {
+        seg_first[sg + 1] = carry + v;
+        seg_slot[sg] = s0;
+      }
+      carry += __shfl(v, 63, 64);
+    }
+    if (lane == 0) seg_first[0] = 0;
+    __builtin_amdgcn_wave_barrier();
+    const int total = carry;
+    for (int base = 0; base < total; base += 64) {
+      const int c = base + lane;
+      if (c < total) {
+        int lo = 0, hi = ns - 1; /* the segment holding candidate c */
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (seg_first[mid] <= c) lo = mid;
+          else hi = mid - 1;
+        }
+        stage[lane] = a.m.spread_rec[seg_slot[lo] + (c - seg_first[lo])];
+      }
+      __builtin_amdgcn_wave_barrier(); /* the records, in issue order */
+      const int n = min(64, total - base);
+      for (int j = 0; j < n; ++j) {
+        const SpreadRec r = stage[j];
+        if (!r.o.w) continue; /* not IB-coupled */
+        const int px = tile_offset(r.o.x, x0, L.nx, L.periodic[0]) + kx;
+        const int py = tile_offset(r.o.y, y0, L.ny, L.periodic[1]) + ky;
+        const int pz = tile_offset(r.o.z, z0, L.nz, L.periodic[2]) + kz;
+        if (px < 0 || px >= ex || py < 0 || py >= ey || pz < 0 || pz >= ez) continue;
+        const float w = pick4(r.wx, kx) * pick4(r.wy, ky) * pick4(r.wz, kz) * a.force_scale;
+        const int node = px + ACC_SY * py + ACC_SZ * pz;
+        /* LDS adds of one wavefront complete in issue order: the sums follow
+         * the candidates' (cell-list) order */
+        atomicAdd(&acc[node], w * r.f.x);
+        atomicAdd(&acc[ACC_N + node], w * r.f.y);
+        atomicAdd(&acc[2 * ACC_N + node], w * r.f.z);
+      }
+      __builtin_amdgcn_wave_barrier(); /* read before the next group overwrites */
+    }
+    for (int q = lane; q < TILE_CELLS; q += 64) {
+      const int lx = q & (TILE_X - 1), ly = (q >> TILE_SHIFT_X) & (TILE_Y - 1),
+                lz = q >> (TILE_SHIFT_X + TILE_SHIFT_Y);
+      if (lx < ex && ly < ey && lz < ez) {
+        const int64_t cell = (x0 + lx) + (int64_t)L.nx * ((y0 + ly) + (int64_t)L.ny * (z0 + lz));
+        const int node = lx + ACC_SY * ly + ACC_SZ * lz;
+        L.aux[cell] = acc[node];
+        L.aux[a.plane + cell] = acc[ACC_N + node];
+        L.aux[2 * a.plane + cell] = acc[2 * ACC_N + node];
+      }
+    }
+    __builtin_amdgcn_wave_barrier(); /* the copy is read before it is cleared again */
+  }
+  /* the last wavefront to finish empties the list for the next spread */
+  if (lane == 0) {
+    __threadfence();
+    if (atomicAdd(&a.m.tile_count[1], 1) == nwaves - 1) {
+      a.m.tile_count[0] = 0;
+      a.m.tile_count[1] = 0;
+      __threadfence();
+    }
+  }
+}
+
 /* Interpolated velocity of marker i by a group of 16 lanes: lane j takes
  * node (x, y) = (j & 3, j >> 2) of the four z-layers of the stencil, so one
  * load instruction of the wavefront covers 4 markers x 4 rows of 4
@@ -2095,9 +2377,14 @@ hipError_t launch_resolve(pfg_handle *h) {
   return hipGetLastError();
 }
 
-static bool staged_spreading(const pfg_handle *h) {
-  return h->ib_staging == 1 || (h->ib_staging == 0 && h->m.n >= IB_STAGED_MIN);
+/* spreading form: 1 staged chunks, 2 direct, 3 tile gather */
+static int spread_form(const pfg_handle *h) {
+  if (h->ib_staging == 3 && !h->m.tile_gather_ok) return 1;
+  if (h->ib_staging) return h->ib_staging;
+  return h->m.n >= IB_STAGED_MIN ? 1 : 2;
 }
+
+static bool staged_spreading(const pfg_handle *h) { return spread_form(h) != 2; }
 
 static int forces_lanes(const pfg_handle *h) {
   /* few markers: a wavefront each, for parallelism; many: 4 lanes each in
@@ -2153,9 +2440,18 @@ hipError_t launch_spread(pfg_handle *h) {
   MArgs a;
   make_margs(h, &a);
   /* staged through LDS when many markers share the lattice */
-  const bool staged = staged_spreading(h);
-  if (!staged) {
+  const int form = spread_form(h);
+  if (form == 2) {
     hipLaunchKernelGGL(spread_kernel, dim3(blocks_for(h->m.n, 4)), dim3(256), 0, h->stream, a);
+    return hipGetLastError();
+  }
+  if (form == 3) {
+    /* marker records, stamps and the tile list, then a workgroup per listed
+     * tile (a grid of SPREAD_TILE_GRID workgroups taking the list in turn) */
+    hipLaunchKernelGGL(spread_prep_kernel, dim3(blocks_for(h->m.n, 256)), dim3(256), 0,
+                       h->stream, a);
+    const int64_t grid = std::min<int64_t>(SPREAD_TILE_GRID, std::max<int64_t>(h->ntiles, 1));
+    hipLaunchKernelGGL(spread_tiles_kernel, dim3((unsigned)grid), dim3(256), 0, h->stream, a);
     return hipGetLastError();
   }
   /* one workgroup per chunk, then one per two markers listed for direct

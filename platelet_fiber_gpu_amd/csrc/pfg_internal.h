@@ -25,6 +25,9 @@ constexpr int GRAPH_CHUNK = 8;
  * chunk is summed node by node. */
 constexpr int SCAN_SPAN = 8192; /* bins per block of the bin-count scan */
 constexpr int IB_CHUNK = 128;
+/* workgroups of the tile-gather spreading launch (each takes listed tiles in
+ * turn) */
+constexpr int SPREAD_TILE_GRID = 4096;
 constexpr int IB_BOX_CELLS = 1024;
 /* Marker count from which spreading is staged through LDS by default. */
 constexpr int IB_STAGED_MIN = 32768;
@@ -100,6 +103,13 @@ struct MLat {
   int64_t tile0;          /* first global tile of the lattice                 */
   int nxb;                /* x-blocks per row                                 */
   int32_t *row_stamp;     /* the lattice's x-block rows (nullptr: unused)     */
+};
+
+/* Tile-gather spreading: a marker's first stencil node (w = 1: IB-coupled),
+ * its kernel weights along x, y, z and its force, per cell-list slot. */
+struct SpreadRec {
+  int4 o;
+  float4 wx, wy, wz, f;
 };
 
 /* Marker-side device arrays. */
@@ -184,6 +194,14 @@ struct Markers {
   int n_chunks = 0, n_direct = 0;
   int32_t *chunk_start = nullptr;
   int32_t *chunk_items = nullptr;
+  /* tile-gather spreading: the tiles this step's stencils meet, listed once
+   * each (global tile index); tile_count[0] tiles listed, [1] workgroups of
+   * the gather launch done (the last one empties the list).  Usable when
+   * every periodic axis is at least twice the tile's extent along it */
+  int32_t *tile_list = nullptr;
+  int32_t *tile_count = nullptr;
+  SpreadRec *spread_rec = nullptr; /* per cell-list slot: the marker's record */
+  int tile_gather_ok = 0;
   /* platelet-platelet contact: per platelet pair (lo * n_plt + hi) the best
    * proposal (distance bits << 32 | element of the lower platelet), and the
    * pairs proposed this step */
@@ -228,7 +246,7 @@ struct pfg_handle {
   void *f[2] = {nullptr, nullptr};
   int cur = 0;
   /* IB coupling fields: spreading adds the force of a step into aux and
-   * stamps the 16x4x4 tiles it touches with the step; the LB update reads
+   * stamps the 4x4x4 tiles it touches with the step; the LB update reads
    * the force of stamped tiles, zeroes it there for the next step and
    * writes the velocity the interpolation reads into vel.  The force is
    * kept as three planes, so the three atomic adds of a node go to three
@@ -269,7 +287,7 @@ struct pfg_handle {
   /* batched LB: 1 = one persistent launch of resident workgroups taking the
    * table's tasks from a work queue (lb_queue: next task, workgroups done) */
   int lb_persistent = 0;
-  uint32_t *lb_queue = nullptr;
+  uint32_t *lb_queue = nullptr; /* LB_QUEUES range counters, then the done count */
   pfg::MLat *d_lats = nullptr;                   /* [K] */
   int64_t *d_tile0 = nullptr;                    /* [K+1] first tile per lattice */
   std::vector<int32_t> marker0;                  /* [K+1] first marker per lattice */
@@ -297,6 +315,10 @@ hipError_t launch_init_eq(pfg_handle *h, int k, const float *rho, const float *u
 hipError_t launch_convert_f(pfg_handle *h, float *f_out_or_null,
                             const float *f_in_or_null);
 hipError_t launch_conservation(pfg_handle *h, int k, double out[4]);
+/* persistent batched LB launch: row ranges (one per XCD) and x-block rows per
+ * grab */
+constexpr int LB_QUEUES = 8;
+constexpr int LB_QUEUE_GRAB = 16;
 /* pfg_download_fields_noalloc: staging in a fixed device array of
  * FIELDS_SCRATCH_FLOATS floats per device (no allocation) */
 constexpr int64_t FIELDS_SCRATCH_FLOATS = PFG_FIELDS_SCRATCH_FLOATS;

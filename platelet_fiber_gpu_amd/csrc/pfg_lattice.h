@@ -35,9 +35,15 @@ PFG_HDI constexpr float weight(int d) {
 /* Cells per x-block of the blocked population layout (one wavefront). */
 constexpr int XBLOCK = 64;
 
-/* Force-field tile (cells) used to skip untouched IB force/velocity data. */
-constexpr int TILE_X = 16, TILE_Y = 4, TILE_Z = 4;
-constexpr int TILE_SHIFT_X = 4, TILE_SHIFT_Y = 2, TILE_SHIFT_Z = 2;
+/* Force-field tile (cells) used to skip untouched IB force/velocity data:
+ * the LB update reads the force and writes the velocity only in the tiles
+ * this step's stencils meet.  4 x 4 x 4 cells: a 4 x 4 x 4 stencil meets
+ * 1-8 of them, and the stamped cells stay close to the cells the stencils
+ * cover (15 % of the 1024x512x512 lattice with 16 x 4 x 4 tiles, 10 % with
+ * these). */
+constexpr int TILE_X = 4, TILE_Y = 4, TILE_Z = 4;
+constexpr int TILE_SHIFT_X = 2, TILE_SHIFT_Y = 2, TILE_SHIFT_Z = 2;
+constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
 
 /* 4-point cosine delta of LB_kernel.cu:1477-1478: phi(r) = (1 + cos(pi r/2))/4
  * for |r| < 2, r in lattice units. */

@@ -118,11 +118,12 @@ def test_forces(gpu, po, what, lanes):
     assert np.abs(F - oF).max() < 2e-5 * scale, (np.abs(F - oF).max(), scale)
 
 
-@pytest.mark.parametrize("staging", [1, 2])
+@pytest.mark.parametrize("staging", [1, 2, 3])
 def test_spread_interp_parity(gpu, po, staging):
-    """Both spreading forms (1: LDS-staged chunks, 2: direct atomics) against
-    the oracle's stencil form; a seventh of the markers sit one periodic
-    image away, so staged chunks straddle the periodic boundary."""
+    """The three spreading forms (1: LDS-staged chunks, 2: direct atomics,
+    3: tile gather over the cell lists) against the oracle's stencil form; a
+    seventh of the markers sit one periodic image away, so staged chunks and
+    gathered tiles straddle the periodic boundary."""
     pfg = gpu
     sc = scene_small(pfg, receptors=-1, ligands=-1)
     rng = np.random.default_rng(5)
@@ -137,6 +138,8 @@ def test_spread_interp_parity(gpu, po, staging):
         s.set_marker_forces(*f)
         for k, name in enumerate(("fx", "fy", "fz")):
             o.marker(name)[:] = f[k]
+        if staging == 3:
+            s.stage(pfg.STAGE_CELLS)  # the gather finds the markers on the cell lists
         s.stage(pfg.STAGE_SPREAD)
         o.stage(pfg.STAGE_SPREAD)
         Fg = s.aux()
@@ -337,3 +340,42 @@ def test_contact_dense_cluster_parity(gpu, po):
     assert info["contact_overflow"] > 0  # the case the fifth-platelet path covers
     assert np.flatnonzero(np.abs(F).sum(1)).tolist() == np.flatnonzero(np.abs(oF).sum(1)).tolist()
     assert np.abs(F - oF).max() < 1e-5 * np.abs(oF).max()
+
+
+@pytest.mark.parametrize("name", ["thrombus_small", "tube_full"])
+def test_tile_gather_follows_the_staged_form(gpu, name):
+    """Coupled steps with the tile-gather spreading (form 3) against the
+    LDS-staged chunks (form 1) on the same scene: platelets with receptors
+    and bonds and a fibrin network (and, at full size, the 512x256x256 flag
+    tube with 500 platelets): the same markers and fields up to the FP32
+    order of the force sums.  The gather sums in cell-list order, so two
+    runs of form 3 are bitwise equal."""
+    pfg = gpu
+    if name == "thrombus_small":
+        sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=96, ny=64, nz=64, n_platelets=6, elements=300,
+                       receptors=16, ligands=400, fibrin_nodes=800, fibrin_links=1200)
+        sc.params.morse_taper = 0.2
+        steps = 20
+    else:
+        sc = pfg.Scene(pfg.SCENE_THROMBUS)
+        sc.params.morse_taper = 0.2  # no pair switches at the cutoff on a rounding difference
+        steps = 2
+    runs = []
+    for form in (1, 3, 3):
+        with pfg.Solver(sc.params) as s:
+            s.set_ib_staging(form)
+            s.load(sc)
+            s.step(steps)
+            mk = s.markers()
+            runs.append((np.stack([mk[k] for k in ("x", "y", "z")]), s.fields()))
+    dx = float(np.abs(runs[0][0] - runs[1][0]).max())
+    du = max(float(np.abs(a - b).max()) for a, b in zip(runs[0][1], runs[1][1]))
+    print(f"{name}: {steps} steps, max position difference {dx:.2e}, max field difference "
+          f"{du:.2e}")
+    # positions: two FP32 ulps of the largest coordinate; fields: the force
+    # sums' FP32 order carried through the steps
+    xmax = max(sc.params.nx, sc.params.ny, sc.params.nz)
+    assert dx <= 2 * np.spacing(np.float32(xmax))
+    assert du < 5e-6
+    assert np.array_equal(runs[1][0], runs[2][0])
+    assert all(np.array_equal(a, b) for a, b in zip(runs[1][1], runs[2][1]))
