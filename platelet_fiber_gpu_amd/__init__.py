@@ -476,6 +476,9 @@ class Batch(Solver):
         self.n_markers_k = [m.n_markers for m in models]
         self.n_markers = sum(self.n_markers_k)
         self.n_rec = sum(m.n_receptors for m in models)
+        self.n_rec_k = [m.n_receptors for m in models]
+        self.n_wall_lig_k = [m.n_wall_ligands for m in models]
+        self.n_lig = sum(self.n_wall_lig_k) + (self.n_markers if models[0].fibrin_ligands else 0)
         self.n_tri = sum(m.n_tri for m in models)
         self.n_plt = sum(m.n_platelets for m in models)
         for k, s in enumerate(scenes):

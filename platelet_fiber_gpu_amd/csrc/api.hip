@@ -1275,6 +1275,17 @@ pfg_status pfg_batch_load_models(pfg_handle *h, const pfg_model *models) {
     p0 += M.n_platelets;
     t0 += M.n_tri;
   }
+  /* each lattice's receptors and markers keep their own indices in the
+   * lattice's random draws */
+  {
+    int32_t r0 = 0, mk0 = 0;
+    for (int k = 0; k < K; ++k) {
+      h->inst[(size_t)k].rec0 = r0;
+      h->inst[(size_t)k].marker0 = mk0;
+      r0 += models[k].n_receptors;
+      mk0 += models[k].n_markers;
+    }
+  }
   pfg_model all{};
   all.n_markers = (int32_t)n;
   all.x = x.data();

@@ -997,7 +997,7 @@ __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
       const float f = a.bond_law == PFG_BOND_REFERENCE ? fabsf(tension) : fmaxf(tension, 0.0f);
       const float koff = a.k_off0 * expf(a.bell_xb_kT * f);
       const float p_off = 1.0f - expf(-koff);
-      const float u = pfg_uniform(a.seed, step, (uint32_t)r, PFG_RNG_BOND_OFF);
+      const float u = pfg_uniform(L.seed, step, (uint32_t)(r - L.rec0), PFG_RNG_BOND_OFF);
       if (fabsf(u - p_off) < BOND_EPS_P) bump(a.m.counters, 2);
       a.m.rec_release[r] = u < p_off ? 1 : 0;
     }
@@ -1064,7 +1064,7 @@ __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
     kon = a.k_on * expf(a.bell_xb_kT * f - 0.5f * f * w / a.bell_kT);
   }
   const float p_on = 1.0f - expf(-kon);
-  const float u = pfg_uniform(a.seed, step, (uint32_t)r, PFG_RNG_BOND_ON);
+  const float u = pfg_uniform(L.seed, step, (uint32_t)(r - L.rec0), PFG_RNG_BOND_ON);
   if (fabsf(u - p_on) < BOND_EPS_P) bump(a.m.counters, 2);
   if (u < p_on) {
     a.m.rec_cand[r] = best;
@@ -2120,8 +2120,9 @@ __device__ void bbk_element(const MArgs &a, int i, float ux, float uy, float uz)
   if (a.sem_temp > 0.f && g > 0.f) {
     sig = (float)sqrt((double)a.sem_temp / m * (1.0 - exp(-(double)g * dt)));
     const int64_t step = a.m.counters[6];
-    pfg_normal3(a.seed, step, (uint32_t)i, PFG_RNG_BBK_A, za);
-    pfg_normal3(a.seed, step, (uint32_t)i, PFG_RNG_BBK_B, zb);
+    const MLat L = lat_of(a, i); /* the lattice's own key */
+    pfg_normal3(L.seed, step, (uint32_t)(i - L.marker0), PFG_RNG_BBK_A, za);
+    pfg_normal3(L.seed, step, (uint32_t)(i - L.marker0), PFG_RNG_BBK_B, zb);
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -2163,7 +2164,8 @@ __device__ void update_marker(const MArgs &a, int i, float vx, float vy, float v
       const double g = a.fiber_gamma, sig = sqrt((double)a.kick_temp *
                                                  (1.0 - exp(-2.0 * g * (double)dt)));
       double z[3];
-      pfg_normal3(a.seed, a.m.counters[6], (uint32_t)i, PFG_RNG_KICK, z);
+      const MLat L = lat_of(a, i); /* the lattice's own key */
+      pfg_normal3(L.seed, a.m.counters[6], (uint32_t)(i - L.marker0), PFG_RNG_KICK, z);
       vx += (float)(sig * z[0]);
       vy += (float)(sig * z[1]);
       vz += (float)(sig * z[2]);
@@ -2233,6 +2235,9 @@ void fill_mlat(const pfg_handle *h, int k, MLat *L) {
   L->tile0 = I.tile0;
   L->nxb = I.nxb;
   L->row_stamp = h->row_stamp ? h->row_stamp + I.fblk0 : nullptr;
+  L->seed = I.p.seed;
+  L->rec0 = I.rec0;
+  L->marker0 = I.marker0;
 }
 
 static void make_margs(const pfg_handle *h, MArgs *a) {

@@ -103,6 +103,12 @@ struct MLat {
   int64_t tile0;          /* first global tile of the lattice                 */
   int nxb;                /* x-blocks per row                                 */
   int32_t *row_stamp;     /* the lattice's x-block rows (nullptr: unused)     */
+  /* counter-RNG key of the lattice's own draws (bonds, noise): its seed and
+   * lattice-local receptor and marker indices, so that a lattice of a batch
+   * draws what it draws in a handle of its own */
+  uint64_t seed;
+  int32_t rec0;           /* first receptor of the lattice                    */
+  int32_t marker0;        /* first marker of the lattice                      */
 };
 
 /* Tile-gather spreading: a marker's first stencil node (w = 1: IB-coupled),
@@ -230,6 +236,7 @@ struct Inst {
   int nb[3] = {1, 1, 1};
   int64_t nbins = 0, bin0 = 0;
   int64_t lb_blocks = 0, lb_block0 = 0; /* workgroups in the batched LB launch */
+  int32_t rec0 = 0, marker0 = 0;  /* first receptor and marker of the lattice */
 };
 
 }  // namespace pfg

@@ -191,3 +191,46 @@ def test_full_batch_workload_equals_separate_handles(gpu):
           f"max field difference {worst_u:.2e}")
     assert worst_x < 3e-5  # coordinates up to 96: a few FP32 ulps
     assert worst_u < 1e-6
+
+
+def test_batch_bonds_follow_single_handles(gpu):
+    """Shear lattices with receptors over wall ligands and fast binding: each
+    lattice of the batch keys its bond draws by its own seed and its own
+    receptor indices, so it forms and breaks the bonds it does in a handle
+    of its own (decisions within the near-threshold band counted on both
+    sides), and its markers follow."""
+    pfg = gpu
+    specs = [(48, 32, 24, 11), (56, 40, 24, 12), (40, 32, 28, 13)]
+    scenes = []
+    for nx, ny, nz, seed in specs:
+        sc = pfg.Scene(pfg.SCENE_SHEAR, nx=nx, ny=ny, nz=nz, elements=256, receptors=256,
+                       ligands=2000, seed=seed)
+        sc.params.k_on, sc.params.k_off0, sc.params.bond_reach = 0.3, 0.1, 1.5
+        scenes.append(sc)
+    steps = 8
+    with pfg.Batch([s.params for s in scenes]) as b:
+        b.load_scenes(scenes)
+        info_b = b.step(steps)
+        rb_all = b.bonds()[0]
+        got = [b.markers_of(k) for k in range(len(scenes))]
+        nrec, nlig = b.n_rec_k, b.n_wall_lig_k
+    r0 = np.concatenate([[0], np.cumsum(nrec)])
+    l0 = np.concatenate([[0], np.cumsum(nlig)])
+    mism = near = formed = 0
+    for k, sc in enumerate(scenes):
+        with pfg.Solver(sc.params) as s:
+            s.load(sc)
+            info = s.step(steps)
+            rb = s.bonds()[0]
+            mk = s.markers()
+        mine = rb_all[r0[k]:r0[k + 1]]
+        mine = np.where(mine >= 0, mine - l0[k], -1)
+        mism += int((mine != rb).sum())
+        near += info["near_threshold"]
+        formed += int((rb >= 0).sum())
+        for name in ("x", "y", "z"):
+            assert np.abs(got[k][name] - mk[name]).max() < 1e-4, (k, name)
+    print(f"batch bonds: {formed} bonds held in the separate runs, {mism} differ "
+          f"(near threshold: separate {near}, batch {info_b['near_threshold']})")
+    assert formed > 20
+    assert mism <= near + info_b["near_threshold"]
