@@ -9,7 +9,9 @@
  * z = depth-1, linear shear u_y = SHEAR_RATE k dx / UMAX at equilibrium,
  * the derived parameter block hostSEMparameters[0..19]; the platelet is an
  * icosahedron of 12 elements with springs on its edges and 3 receptors per
- * triangle.
+ * triangle; a fibrin chain of six nodes (the first anchored) goes in through
+ * fiber_allocGPUKernel / fiber_init_GPUKernel and comes back through
+ * fiber_copy_GPUKernel.
  *
  * Prints one line "REF_ABI ok ..." and exits 0, or "REF_ABI FAIL ..." and 1.
  * Exit 77: no HIP device (the program then only shows that it links).
@@ -29,6 +31,7 @@
 #define NT 20
 #define NREC 3
 #define STEPS 24
+#define NF 6 /* fibrin chain: nodes (the first anchored) and NF - 1 links */
 
 static const double DX = 0.2, DT = 1e-8, RATE = 1000.0;
 
@@ -149,7 +152,31 @@ int main(void) {
   double randNum[2 * NREC * NT];
   for (int k = 0; k < 2 * NREC * NT; ++k) randNum[k] = -1.0;
 
-  /* ---- the reference's calls (gpuSEM_LB.c:1293-1304) ---- */
+  /* a fibrin chain along x above the platelet's side, node 0 anchored, links
+   * stretched by 10 % (the network arrays of fiber.cpp:145-217) */
+  int ntype[NF], nconn[NF], natl[2 * NF], ladj[2 * (NF - 1)];
+  double nx0[NF], ny0[NF], nz0[NF], nzero[NF], ll0[NF - 1], lth[NF - 1];
+  for (int i = 0; i < NF; ++i) {
+    ntype[i] = i == 0 ? 1 : 0;
+    nconn[i] = (i == 0 || i == NF - 1) ? 1 : 2;
+    natl[2 * i] = i > 0 ? i - 1 : -1;
+    natl[2 * i + 1] = i < NF - 1 ? i : -1;
+    nx0[i] = 1.0 + 0.55 * i;
+    ny0[i] = 3.9;
+    nz0[i] = 1.7;
+    nzero[i] = 0.0;
+  }
+  for (int l = 0; l < NF - 1; ++l) {
+    ladj[2 * l] = l;
+    ladj[2 * l + 1] = l + 1;
+    ll0[l] = 0.5;
+    lth[l] = 0.05;
+  }
+  /* ---- the reference's calls (gpuSEM_LB.c:1279-1304; the fibrin network's
+   * copies to the device as semLB.cu:609-664 takes them) ---- */
+  void *FIBgrid = fiber_allocGPUKernel(NULL, NF, NF - 1, 2, DT, NULL);
+  fiber_init_GPUKernel(NULL, FIBgrid, ntype, nconn, natl, ladj, ll0, ll0, lth, nx0, nzero, nx0,
+                       nzero, ny0, ny0, nzero, nzero, nz0, nz0, nzero, nzero);
   void *LBgrids = fluid_allocGPUKernel(NULL, (float)DT, (float)DX, LX, LY, LZ);
   fluid_initGPUKernel(NULL, LBgrids, 1, s, (void **)fIN, (void **)fOUT, ux, uy, uz, rho, obst,
                       Fx, Fy, Fz, vwf);
@@ -173,6 +200,9 @@ int main(void) {
   double ox[NE], oy[NE], ory[NE], oz[NE], ovx[NE], ovy[NE], ovz[NE], ofx[NE], ofy[NE],
       ofz[NE];
   sem_copyGPUKernel(NULL, SEMgrids, ox, oy, ory, oz, ovx, ovy, ovz, ofx, ofy, ofz, 1);
+  double fbx[NF], fby[NF], fbz[NF], fbvx[NF], fbvy[NF], fbvz[NF], fbfx[NF], fbfy[NF], fbfz[NF];
+  fiber_copy_GPUKernel(NULL, FIBgrid, fbx, fbvx, NULL, fbfx, fby, NULL, fbvy, fbfy, fbz, NULL,
+                       fbvz, fbfz);
   if (pfg_ref_last_status() != PFG_OK || done) {
     printf("REF_ABI FAIL run: %s done %d\n", pfg_status_string(pfg_ref_last_status()), done);
     return 1;
@@ -202,8 +232,22 @@ int main(void) {
     }
   }
   if (pfg_upload_f(h, f) != PFG_OK) return 1;
-  float mx[NE], my[NE], mz[NE];
-  int32_t kind[NE], body[NE], pes[2] = {0, NE}, pts[2] = {0, NT}, tri[3 * NT], edge[2 * 30];
+  float mx[NE + NF], my[NE + NF], mz[NE + NF];
+  int32_t kind[NE + NF], body[NE + NF], pes[2] = {0, NE}, pts[2] = {0, NT}, tri[3 * NT],
+      edge[2 * 30], link[2 * (NF - 1)];
+  float link_l0[NF - 1];
+  for (int i = 0; i < NF; ++i) {
+    mx[NE + i] = (float)(nx0[i] / DX);
+    my[NE + i] = (float)(ny0[i] / DX);
+    mz[NE + i] = (float)(nz0[i] / DX - 1.0);
+    kind[NE + i] = ntype[i] == 1 ? PFG_MARKER_FIXED : PFG_MARKER_FIBER;
+    body[NE + i] = -1;
+  }
+  for (int l = 0; l < NF - 1; ++l) {
+    link[2 * l] = NE + ladj[2 * l];
+    link[2 * l + 1] = NE + ladj[2 * l + 1];
+    link_l0[l] = (float)(ll0[l] / DX);
+  }
   float el0[30];
   double tS0[NT], pS0 = S0_all / (DX * DX), pV0 = V0 / (DX * DX * DX);
   int ne = 0;
@@ -235,7 +279,7 @@ int main(void) {
   }
   pfg_model m;
   memset(&m, 0, sizeof(m));
-  m.n_markers = NE;
+  m.n_markers = NE + NF;
   m.x = mx;
   m.y = my;
   m.z = mz;
@@ -260,10 +304,13 @@ int main(void) {
   m.lig_x = lx;
   m.lig_y = ly;
   m.lig_z = lz;
+  m.n_links = NF - 1;
+  m.link = link;
+  m.link_l0 = link_l0;
   if (pfg_load_model(h, &m) != PFG_OK) return 1;
   pfg_step_info info;
   if (pfg_step(h, totalT, &info) != PFG_OK) return 1;
-  float dxp[NE], dyp[NE], dzp[NE], dvx[NE], dvy[NE], dvz[NE];
+  float dxp[NE + NF], dyp[NE + NF], dzp[NE + NF], dvx[NE + NF], dvy[NE + NF], dvz[NE + NF];
   if (pfg_download_markers(h, dxp, dyp, dzp, dvx, dvy, dvz, NULL, NULL, NULL) != PFG_OK)
     return 1;
   float *fuy = malloc(sizeof(float) * n);
@@ -285,6 +332,22 @@ int main(void) {
     const double va[3] = {ovx[i], ovy[i], ovz[i]}, vb[3] = {dvx[i] * C, dvy[i] * C, dvz[i] * C};
     for (int k = 0; k < 3; ++k) dvel = fmax(dvel, fabs(va[k] - vb[k]));
   }
+  /* the fibrin chain: positions and velocities through fiber_copy_GPUKernel
+   * against the direct run; the anchored node stays, the others move */
+  double dfib = 0.0, fmoved = 0.0;
+  for (int i = 0; i < NF; ++i) {
+    const double a[3] = {fbx[i], fby[i], fbz[i]};
+    const double b[3] = {dxp[NE + i] * DX, dyp[NE + i] * DX, (dzp[NE + i] + 1.0) * DX};
+    const double x0[3] = {nx0[i], ny0[i], nz0[i]};
+    const double va[3] = {fbvx[i], fbvy[i], fbvz[i]};
+    const double vb[3] = {dvx[NE + i] * C, dvy[NE + i] * C, dvz[NE + i] * C};
+    for (int k = 0; k < 3; ++k) {
+      dfib = fmax(dfib, fabs(a[k] - b[k]));
+      dfib = fmax(dfib, fabs(va[k] - vb[k]));
+      if (i == 0) dfib = fmax(dfib, fabs(a[k] - x0[k]) > 1e-6 ? 1.0 : 0.0);
+      else fmoved = fmax(fmoved, fabs(a[k] - x0[k]));
+    }
+  }
   /* the downloaded flow against the direct run, and the linear shear away
    * from the platelet (x within 4 nodes of the far end) */
   double dflow = 0.0, dlin = 0.0;
@@ -298,12 +361,14 @@ int main(void) {
    * with the FP32 dx of fluid_allocGPUKernel (LB_kernel.cu:647) */
   const double u_plate = RATE * (LZ - 1) * (double)(float)DX / umax;
   const int ok = totalT == STEPS && dpos < 1e-6 && dvel < 1e-6 && dflow < 1e-6 && dlin < 1e-3 &&
+                 dfib < 1e-6 && fmoved > 1e-5 &&
                  moved > 1e-4 && fabs(uy[(LZ - 1) * plane] - u_plate) < 1e-12 && uy[0] == 0.0;
   printf("REF_ABI %s steps %d moved %.3e um, |dpos| %.2e um, |dvel| %.2e, |dflow| %.2e, "
-         "linear shear %.2e (UMAX units)\n",
-         ok ? "ok" : "FAIL", totalT, moved, dpos, dvel, dflow, dlin);
+         "linear shear %.2e (UMAX units), fibrin moved %.3e um, |dfibrin| %.2e\n",
+         ok ? "ok" : "FAIL", totalT, moved, dpos, dvel, dflow, dlin, fmoved, dfib);
   pfg_destroy(h);
   sem_releaseGPUKernel(NULL, SEMgrids);
+  fiber_release_GPUKernel(NULL, FIBgrid);
   fluid_releaseGPUKernel(NULL, LBgrids);
   return ok ? 0 : 1;
 }
