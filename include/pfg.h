@@ -390,8 +390,9 @@ pfg_status pfg_set_fluid_only(pfg_handle *h, int fluid_only);
  * neither the lattice, the markers nor the bonds, and pfg_step_info.step
  * reports the steps that ran.  Default 0: stepping continues after detach. */
 pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable);
-/* IB spreading form (LB_kernel.cu:1419-1520 restated three ways): 0 automatic
- * (LDS-staged from 32768 markers up), 1 always staged (a workgroup per run
+/* IB spreading form (LB_kernel.cu:1419-1520 restated four ways): 0 automatic
+ * (4 from 32768 markers up when the model has compact platelets, else 1
+ * there; 2 below), 1 always staged (a workgroup per run
  * of up to 128 markers of one platelet accumulates them in LDS first),
  * 2 always direct (FP32 atomics per marker and node), 3 tile gather (a
  * wavefront per 4x4x4 tile that this step's stencils meet sums the
@@ -399,7 +400,14 @@ pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable);
  * the tile; no device-wide atomics; needs the cell lists of the current
  * positions, i.e. PFG_STAGE_CELLS before PFG_STAGE_SPREAD when stepping by
  * stages, and falls back to 1 when a periodic axis is shorter than twice
- * the tile's extent along it).  Same result up to FP32 summation order. */
+ * the tile's extent along it), 4 by body (a workgroup per platelet sums
+ * all its stencils in an LDS box and stores the nodes of the 4x4x4 tiles
+ * no other marker's stencil meets this step, which the force pass claims
+ * for it, with plain writes, the others with FP32 atomics; the markers
+ * outside such platelets spread as in 1; needs the claims of the current
+ * positions, i.e. PFG_STAGE_FORCES before PFG_STAGE_SPREAD when stepping
+ * by stages, and without them adds every node with atomics).  Same result
+ * up to FP32 summation order. */
 pfg_status pfg_set_ib_staging(pfg_handle *h, int mode);
 /* Lanes per marker in the SEM/fibrin force pass (sem_Force_kernel,
  * sem_kernel.cu:320-513, with the Morse pairs over the cell lists): 0
@@ -490,7 +498,7 @@ typedef struct pfg_tuning {
   int lb_cells_per_lane; /* pfg_set_lb_tuning                                */
   int lb_xcd_rows;       /* pfg_set_lb_xcd_rows (0 for batches)              */
   int forces_lanes;      /* pfg_set_forces_lanes: 0 automatic, or 1 .. 64     */
-  int ib_staging;        /* pfg_set_ib_staging: 0 automatic, 1, 2, 3         */
+  int ib_staging;        /* pfg_set_ib_staging: 0 automatic, 1, 2, 3, 4      */
   int overlap;           /* pfg_set_overlap                                  */
   double ms_per_step;    /* the chosen shapes                                */
   double ms_default;     /* the automatic shapes                             */

@@ -104,6 +104,9 @@ void free_markers(Markers &m) {
   dfree(m.marker_bin); dfree(m.marker_slot); dfree(m.scan_partials);
   dfree(m.chunk_start); dfree(m.chunk_items); dfree(m.bin_scatter); dfree(m.sorted_pos);
   dfree(m.tile_list); dfree(m.tile_count); dfree(m.spread_rec);
+  dfree(m.body_range); dfree(m.rchunk_start); dfree(m.rchunk_items); dfree(m.tile_owner);
+  dfree(m.body_spread); dfree(m.body_fallback); dfree(m.chunk_body); dfree(m.chunk_sum);
+  dfree(m.chunk_info);
   Markers fresh;
   fresh.counters = m.counters;
   fresh.ticket = m.ticket;
@@ -775,12 +778,19 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   int chunk_len = 16;
   while (chunk_len < IB_CHUNK && (int64_t)chunk_len * 2 * 2048 <= n) chunk_len *= 2;
   std::vector<int32_t> chunk_start, chunk_items, direct_items;
+  /* body spreading: the groups that are compact platelets (elements only, at
+   * most BODY_MAX_MARKERS, stencils spanning at most BODY_BOX_DIM nodes
+   * along each axis at load) are bodies; their leaves are cut further until
+   * every chunk's box fits IB_BOX_CELLS, so that no marker of a body is
+   * listed for direct spreading; the chunks and direct markers of every
+   * other group are listed again for the chunk launch */
+  std::vector<int32_t> body_range, chunk_body, rchunk_start, rchunk_items, rdirect_items;
   {
     const float *P[3] = {M.x, M.y, M.z};
     std::vector<int32_t> idx((size_t)n);
     for (int i = 0; i < n; ++i) idx[(size_t)i] = i;
-    auto leaf = [&](int32_t *b, int cnt) {
-      std::sort(b, b + cnt);
+    bool body = false; /* the group being cut is a body */
+    auto box_volume = [&](const int32_t *b, int cnt) {
       int64_t vol = 1;
       for (int k = 0; k < 3; ++k) {
         float lo = P[k][b[0]], hi = lo;
@@ -790,23 +800,43 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
         }
         vol *= (int64_t)(std::floor(hi) - std::floor(lo)) + 4;
       }
-      if (vol <= IB_BOX_CELLS) {
+      return vol;
+    };
+    auto leaf = [&](int32_t *b, int cnt) {
+      std::sort(b, b + cnt);
+      if (box_volume(b, cnt) <= IB_BOX_CELLS) {
         chunk_start.push_back((int32_t)chunk_items.size());
         chunk_items.insert(chunk_items.end(), b, b + cnt);
+        chunk_body.push_back(body ? (int32_t)(body_range.size() / 4) : -1);
       } else {
         direct_items.insert(direct_items.end(), b, b + cnt);
+        if (!body) rdirect_items.insert(rdirect_items.end(), b, b + cnt);
       }
     };
     std::vector<std::pair<int32_t *, int>> todo;
     for (int g0 = 0; g0 < n;) {
       int g1 = g0 + 1;
       while (g1 < n && M.body[g1] == M.body[g0] && !(minst && minst[g1] != minst[g0])) ++g1;
+      body = g1 - g0 <= BODY_MAX_MARKERS;
+      for (int k = 0; k < 3 && body; ++k) {
+        float lo = P[k][g0], hi = lo;
+        for (int i = g0; i < g1; ++i) {
+          body = body && M.kind[i] == PFG_MARKER_ELEMENT;
+          lo = std::min(lo, P[k][i]);
+          hi = std::max(hi, P[k][i]);
+        }
+        body = body && (int64_t)(std::floor(hi) - std::floor(lo)) + 4 <= BODY_BOX_DIM;
+      }
+      const size_t c0 = chunk_start.size();
       todo.push_back({idx.data() + g0, g1 - g0});
       while (!todo.empty()) {
         int32_t *b = todo.back().first;
         const int cnt = todo.back().second;
         todo.pop_back();
-        if (cnt <= chunk_len) {
+        /* a body's leaf is cut in halves until its box fits */
+        const bool split = cnt > chunk_len ||
+                           (body && cnt > 1 && box_volume(b, cnt) > IB_BOX_CELLS);
+        if (!split) {
           leaf(b, cnt);
           continue;
         }
@@ -823,13 +853,32 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
             axis = k;
           }
         }
-        const int left = ((cnt + chunk_len - 1) / chunk_len / 2) * chunk_len;
+        const int left = cnt > chunk_len ? ((cnt + chunk_len - 1) / chunk_len / 2) * chunk_len
+                                         : cnt / 2;
         const float *c = P[axis];
         std::nth_element(b, b + left, b + cnt, [c](int32_t u, int32_t v) {
           return c[u] < c[v] || (c[u] == c[v] && u < v);
         });
         todo.push_back({b + left, cnt - left}); /* the left half is cut first */
         todo.push_back({b, left});
+      }
+      const size_t c1 = chunk_start.size();
+      if (body && c1 - c0 > (size_t)BODY_MAX_CHUNKS) { /* too many chunks: not a body */
+        body = false;
+        for (size_t c = c0; c < c1; ++c) chunk_body[c] = -1;
+      }
+      if (body) {
+        body_range.push_back(g0);
+        body_range.push_back(g1);
+        body_range.push_back((int32_t)c0);
+        body_range.push_back((int32_t)c1);
+      } else {
+        for (size_t c = c0; c < c1; ++c) {
+          const int32_t end_c = c + 1 < c1 ? chunk_start[c + 1] : (int32_t)chunk_items.size();
+          rchunk_start.push_back((int32_t)rchunk_items.size());
+          rchunk_items.insert(rchunk_items.end(), chunk_items.begin() + chunk_start[c],
+                              chunk_items.begin() + end_c);
+        }
       }
       g0 = g1;
     }
@@ -838,6 +887,12 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   m.n_direct = (int)direct_items.size();
   chunk_start.push_back((int32_t)chunk_items.size());
   chunk_items.insert(chunk_items.end(), direct_items.begin(), direct_items.end());
+  m.n_bodies = (int)(body_range.size() / 4);
+  m.n_rchunks = (int)rchunk_start.size();
+  m.n_rdirect = (int)rdirect_items.size();
+  m.n_ritems = (int)(rchunk_items.size() + rdirect_items.size());
+  rchunk_start.push_back((int32_t)rchunk_items.size());
+  rchunk_items.insert(rchunk_items.end(), rdirect_items.begin(), rdirect_items.end());
 
   /* ---- host-side CSR construction ---- */
   std::vector<std::pair<int, int>> kv;
@@ -1103,6 +1158,18 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   ALLOC_FILL(m.scan_partials, 0, m.scan_blocks);
   ALLOC_UP(m.chunk_start, chunk_start.data(), chunk_start.size());
   ALLOC_UP(m.chunk_items, chunk_items.data(), chunk_items.size());
+  if (!body_range.empty()) {
+    ALLOC_UP(m.body_range, body_range.data(), body_range.size());
+    ALLOC_UP(m.chunk_body, chunk_body.data(), chunk_body.size());
+    if (e == hipSuccess) e = dalloc(h, &m.chunk_sum, (size_t)3 * IB_BOX_CELLS * m.n_chunks);
+    if (e == hipSuccess) e = dalloc(h, &m.chunk_info, (size_t)2 * m.n_chunks);
+  }
+  ALLOC_UP(m.rchunk_start, rchunk_start.data(), rchunk_start.size());
+  if (!rchunk_items.empty()) ALLOC_UP(m.rchunk_items, rchunk_items.data(), rchunk_items.size());
+  /* no tile is claimed, and no body spread, before the first step */
+  ALLOC_FILL(m.tile_owner, ~(uint64_t)0, h->ntiles > 0 ? h->ntiles : 1);
+  ALLOC_FILL(m.body_spread, (int32_t)-1, m.n_bodies > 0 ? m.n_bodies : 1);
+  ALLOC_FILL(m.body_fallback, (int32_t)-1, m.n_bodies > 0 ? m.n_bodies : 1);
   if (minst) ALLOC_UP(m.minst, minst, n);
   ALLOC_FILL(m.bin_scatter, 0, n);
   ALLOC_FILL(m.sorted_pos, make_float4(0.f, 0.f, 0.f, 0.f), n);
@@ -1334,6 +1401,9 @@ pfg_status pfg_upload_markers(pfg_handle *h, const float *x, const float *y,
   PFG_TRY(upload(h, h->m.x, x, n));
   PFG_TRY(upload(h, h->m.y, y, n));
   PFG_TRY(upload(h, h->m.z, z, n));
+  /* tile claims made from the old positions no longer hold */
+  if (h->m.tile_owner)
+    PFG_TRY(hipMemset(h->m.tile_owner, 0xff, sizeof(uint64_t) * (size_t)h->ntiles));
   return PFG_OK;
 }
 
@@ -1638,7 +1708,7 @@ pfg_status pfg_set_overlap(pfg_handle *h, int enable) {
 }
 
 pfg_status pfg_set_ib_staging(pfg_handle *h, int mode) {
-  if (!h || mode < 0 || mode > 3) return PFG_ERR_ARG;
+  if (!h || mode < 0 || mode > 4) return PFG_ERR_ARG;
   h->ib_staging = mode;
   drop_graph(h);
   return PFG_OK;
@@ -1675,9 +1745,16 @@ pfg_status pfg_set_step(pfg_handle *h, int64_t step) {
   if (!h || step < 0) return PFG_ERR_ARG;
   hipSetDevice(h->device);
   PFG_TRY(upload(h, h->m.d_step, &step, 1));
-  /* stamps of other steps could equal the new index: no tile or row keeps
-   * one (the tile-gather spreading lists a tile when its stamp changes) */
+  /* stamps and claims of other steps could equal the new index: no tile or
+   * row keeps one (the tile-gather spreading lists a tile when its stamp
+   * changes; body spreading stores into a tile its body claimed alone) */
   PFG_TRY(hipMemset(h->tile_stamp, 0xff, sizeof(int32_t) * (size_t)h->ntiles));
+  if (h->m.tile_owner)
+    PFG_TRY(hipMemset(h->m.tile_owner, 0xff, sizeof(uint64_t) * (size_t)h->ntiles));
+  if (h->m.body_spread)
+    PFG_TRY(hipMemset(h->m.body_spread, 0xff, sizeof(int32_t) * (size_t)std::max(1, h->m.n_bodies)));
+  if (h->m.body_fallback)
+    PFG_TRY(hipMemset(h->m.body_fallback, 0xff, sizeof(int32_t) * (size_t)std::max(1, h->m.n_bodies)));
   if (h->row_stamp) {
     size_t rows = 0;
     for (const Inst &I : h->inst) rows += (size_t)I.nxb * I.p.ny * I.p.nz;
@@ -1814,6 +1891,12 @@ hipError_t snapshot_copy(pfg_handle *h, Snapshot &S, bool save) {
   if (!save) {
     /* no tile or row carries a stamp or a force of the steps being undone */
     if (e == hipSuccess) e = hipMemsetAsync(h->tile_stamp, 0xff, 4 * (size_t)h->ntiles, h->stream);
+    if (e == hipSuccess && h->m.tile_owner)
+      e = hipMemsetAsync(h->m.tile_owner, 0xff, 8 * (size_t)h->ntiles, h->stream);
+    if (e == hipSuccess && h->m.body_spread)
+      e = hipMemsetAsync(h->m.body_spread, 0xff, 4 * (size_t)std::max(1, h->m.n_bodies), h->stream);
+    if (e == hipSuccess && h->m.body_fallback)
+      e = hipMemsetAsync(h->m.body_fallback, 0xff, 4 * (size_t)std::max(1, h->m.n_bodies), h->stream);
     if (e == hipSuccess && h->row_stamp) {
       size_t rows = 0;
       for (const Inst &I : h->inst) rows += (size_t)I.nxb * I.p.ny * I.p.nz;
@@ -1910,8 +1993,12 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   /* knobs, tried one at a time from the automatic shape, each kept at its
    * fastest value before the next is tried (values 0 are the automatic
    * choices); a value replaces the best only when it is 1 % faster */
-  const int cpts[3] = {1, 2, 4}, lanes[7] = {64, 32, 16, 8, 4, 2, 1}, stag[3] = {1, 2, 3},
-            ovl[2] = {0, 1}, xcds[2] = {0, 1}, pers[2] = {0, 1};
+  const int cpts[3] = {1, 2, 4}, lanes[7] = {64, 32, 16, 8, 4, 2, 1}, ovl[2] = {0, 1},
+            xcds[2] = {0, 1}, pers[2] = {0, 1};
+  /* spreading forms that apply to this model */
+  int stag[4] = {1, 2, 0, 0}, nstag = 2;
+  if (h->m.n_bodies > 0) stag[nstag++] = 4;
+  if (h->m.tile_gather_ok) stag[nstag++] = 3;
   Shape best{1, 0, 0, 0, (h->K == 1 && h->N >= XCD_ROWS_MIN_CELLS) ? 1 : 0, 0};
   const int warm = GRAPH_CHUNK;
   pfg_status st = PFG_OK;
@@ -1929,7 +2016,7 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
     const int knob = order[ki];
     const int *vals = knob == 0 ? cpts : knob == 1 ? xcds : knob == 2 ? lanes
                                        : knob == 3 ? stag : knob == 4 ? ovl : pers;
-    const int nv = knob == 0 ? 3 : knob == 2 ? 7 : knob == 3 ? (h->m.tile_gather_ok ? 3 : 2) : 2;
+    const int nv = knob == 0 ? 3 : knob == 2 ? 7 : knob == 3 ? nstag : 2;
     if (knob == 5 && h->K == 1) continue;
     if ((knob >= 2 && knob <= 4) && !markers) break;
     for (int v = 0; v < nv && st == PFG_OK; ++v) {

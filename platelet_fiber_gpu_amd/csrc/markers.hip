@@ -81,6 +81,7 @@ struct MArgs {
   int mark_rows; /* 1: stamp the x-block rows this step's IB stencils touch */
   int resolve_last; /* 1: the gather launch's last workgroup resolves the bonds */
   int count_spans;  /* 1: the counting launch adds the scan's span totals */
+  int spread_scratch; /* 1: the chunks of bodies write scratch boxes (form 4) */
   int blocks0, blocks1, blocks2, blocks3; /* blocks of role ranges 0..3 */
 };
 
@@ -101,6 +102,9 @@ constexpr float BOND_EPS_R = 1e-4f; /* |r - reach| below this: near-threshold  *
 __device__ inline bool halted(const MArgs &a) { return a.halt && a.m.counters[1] != 0; }
 
 __device__ void spread_direct(const MArgs &a, int i, int lane, float Fx, float Fy, float Fz);
+/* body spreading's tile claims: workgroup `block` of the claim blocks the
+ * force launch appends (blocks1 bodies, then blocks2 of 256 other markers) */
+__device__ void claim_role(const MArgs &a, int block);
 
 __device__ inline int cell_of(float v, int n, int periodic) {
   int c = (int)floorf(v);
@@ -1301,6 +1305,10 @@ __device__ inline float group_sum(float v) {
 template <int G, bool SPREAD = false>
 __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
   static_assert(!SPREAD || G == 64, "spreading takes a lane per stencil node");
+  if ((int)blockIdx.x >= a.blocks0 && a.blocks1 + a.blocks2 > 0) { /* whole workgroups */
+    claim_role(a, (int)blockIdx.x - a.blocks0);
+    return;
+  }
   const int lane = threadIdx.x & (G - 1);
   if (blockIdx.x == 0 && threadIdx.x == 0 && a.m.n_rec > 0 && a.detach_rate > 0.f) {
     /* detach test (semLB.cu:451-455): no bond held after this step's update */
@@ -1568,11 +1576,19 @@ __global__ __launch_bounds__(256) void contact_force_kernel(MArgs a) {
  * times over when many markers share the lattice.  A chunk whose box is
  * large (markers scattered through the lattice) is spread directly. */
 
-/* Stamp the tile of node (x, y, z) with this step: the LB update reads the
- * force of stamped tiles only. */
-__device__ inline void flag_tile(const MLat &L, int x, int y, int z, int32_t step) {
-  const int tile = (x >> TILE_SHIFT_X) + L.tnx * ((y >> TILE_SHIFT_Y) + L.tny * (z >> TILE_SHIFT_Z));
-  if (L.tile_stamp[tile] != step) L.tile_stamp[tile] = step;
+/* Stamp the tiles of a marker's stencil with this step (the LB update reads
+ * the force of stamped tiles only): lane (kx, ky, kz) holds node (x, y, z)
+ * and stores the stamp, without reading it, when its node starts a run of
+ * the stencil's nodes inside one tile along every axis (the stencil's first
+ * node, or a node at a tile boundary, which a wrap or a wall-bounded axis's
+ * first node also is), so at most eight lanes store and every tile the
+ * stencil meets is stamped. */
+__device__ inline void stamp_stencil_tile(const MLat &L, int kx, int ky, int kz, int x, int y,
+                                          int z, int32_t step) {
+  if ((kx == 0 || (x & (TILE_X - 1)) == 0) && (ky == 0 || (y & (TILE_Y - 1)) == 0) &&
+      (kz == 0 || (z & (TILE_Z - 1)) == 0))
+    L.tile_stamp[(x >> TILE_SHIFT_X) + L.tnx * ((y >> TILE_SHIFT_Y) + L.tny * (z >> TILE_SHIFT_Z))] =
+        step;
 }
 
 /* Direct spreading of marker i: lane = stencil node (consecutive lanes take
@@ -1588,7 +1604,7 @@ __device__ void spread_direct(const MArgs &a, int i, int lane, float Fx, float F
   const int iy = stencil_node(Y, (lane >> 2) & 3, L.ny, L.periodic[1], &wy);
   const int iz = stencil_node(Z, lane >> 4, L.nz, L.periodic[2], &wz);
   if (ix < 0 || iy < 0 || iz < 0) return;
-  flag_tile(L, ix, iy, iz, (int32_t)*a.m.d_step);
+  stamp_stencil_tile(L, lane & 3, (lane >> 2) & 3, lane >> 4, ix, iy, iz, (int32_t)*a.m.d_step);
   const float w = wx * wy * wz * a.force_scale;
   const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
   atomicAdd(L.aux + c, w * Fx);
@@ -1611,9 +1627,8 @@ __global__ __launch_bounds__(256) void spread_kernel(MArgs a) {
  * and the first stencil node in the periodic image nearest to the chunk's
  * reference point `ref` (the wrapped position of its first marker), so that
  * one platelet's stencils stay contiguous across a periodic boundary. */
-__device__ inline void stencil_origin(const MArgs &a, const MLat &L, int i, const float ref[3],
+__device__ inline void stencil_origin(const MLat &L, const float v[3], const float ref[3],
                                       float W[3], int o[3]) {
-  const float v[3] = {a.m.x[i], a.m.y[i], a.m.z[i]};
   const int n[3] = {L.nx, L.ny, L.nz};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -1641,9 +1656,10 @@ __device__ inline float stencil_weight(float W, int k) {
  * then second) and go to the lattice field with one device-wide atomic per
  * nonzero node and component.  A scattered chunk (fibrin nodes far apart)
  * adds each marker's 64 nodes with device-wide atomics, as the direct form
- * does; so do the markers listed for direct spreading, two per workgroup
- * after the chunks.  Contributions are the direct form's bit for bit; only
- * the order in which they are added differs. */
+ * does; so do the markers listed for direct spreading, IB_CHUNK per
+ * workgroup after the chunks, staged the same way (each with its own
+ * lattice).  Contributions are the direct form's bit for bit; only the
+ * order in which they are added differs. */
 __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   static_assert(IB_CHUNK <= 128, "a thread per marker in the staging pass");
   __shared__ int4 so[IB_CHUNK];      /* first stencil node; x = INT_MIN: not coupled */
@@ -1651,16 +1667,26 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   __shared__ float sf[IB_CHUNK][3];
   __shared__ float box[2][3 * IB_BOX_CELLS];
   __shared__ int sh[2][6];
+  __shared__ int sidx[IB_CHUNK];
   if (halted(a)) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  if ((int)blockIdx.x >= a.m.n_chunks) { /* the markers listed for direct spreading */
-    const int k = 2 * ((int)blockIdx.x - a.m.n_chunks) + wave;
-    if (k < a.m.n_direct) spread_direct(a, a.m.chunk_items[a.m.chunk_start[a.m.n_chunks] + k], lane);
-    return;
+  const bool chunk = (int)blockIdx.x < a.m.n_chunks;
+  int c0, cnt;
+  if (chunk) {
+    c0 = a.m.chunk_start[blockIdx.x];
+    cnt = a.m.chunk_start[blockIdx.x + 1] - c0;
+  } else { /* IB_CHUNK markers of the direct list (of any lattices) */
+    const int k0 = IB_CHUNK * ((int)blockIdx.x - a.m.n_chunks);
+    c0 = a.m.chunk_start[a.m.n_chunks] + k0;
+    cnt = min(IB_CHUNK, a.m.n_direct - k0);
   }
-  const int c0 = a.m.chunk_start[blockIdx.x], cnt = a.m.chunk_start[blockIdx.x + 1] - c0;
+  if (cnt <= 0) return;
   const int *items = a.m.chunk_items + c0;
-  const int m0 = items[0];
+  /* body spreading: a chunk of a body writes its box to its scratch box,
+   * its stencils placed in the periodic image nearest to the body's first
+   * marker (the frame of the body's box) */
+  const int cbody = chunk && a.spread_scratch ? a.m.chunk_body[blockIdx.x] : -1;
+  const int m0 = cbody >= 0 ? a.m.body_range[4 * cbody] : items[0];
   const MLat L = lat_of(a, m0); /* a chunk never spans two lattices */
   float ref[3];
   ref[0] = wrap_coord(a.m.x[m0], L.nx, L.periodic[0]);
@@ -1669,11 +1695,22 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
   if (t < cnt) {
     const int i = items[t];
+    sidx[t] = i;
     const bool has = ib_coupled(a.m.kind[i], a.fiber_move);
     int o[3] = {0, 0, 0};
     if (has) {
       float W[3];
-      stencil_origin(a, L, i, ref, W, o);
+      const float v[3] = {a.m.x[i], a.m.y[i], a.m.z[i]};
+      if (chunk) {
+        stencil_origin(L, v, ref, W, o);
+      } else { /* its own lattice and image */
+        const MLat Li = lat_of(a, i);
+        float own[3];
+        own[0] = wrap_coord(v[0], Li.nx, Li.periodic[0]);
+        own[1] = wrap_coord(v[1], Li.ny, Li.periodic[1]);
+        own[2] = wrap_coord(v[2], Li.nz, Li.periodic[2]);
+        stencil_origin(Li, v, own, W, o);
+      }
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         sw[t][k] = stencil_weight(W[0], k);
@@ -1712,24 +1749,35 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
     lo[k] = min(sh[0][k], sh[1][k]);
     hi[k] = max(sh[0][3 + k], sh[1][3 + k]);
   }
-  if (lo[0] > hi[0]) return; /* no IB-coupled marker in the chunk */
   const int dim0 = hi[0] - lo[0] + 1, dim1 = hi[1] - lo[1] + 1, dim2 = hi[2] - lo[2] + 1;
   const int64_t vol64 = (int64_t)dim0 * dim1 * dim2;
   const int32_t step = (int32_t)*a.m.d_step;
+  if (cbody >= 0 && t == 0) {
+    /* the scratch box written below, or none (extent 0): no coupled
+     * marker, or a box too large, spread with atomics (the body's nodes then
+     * all go with atomics) */
+    const bool boxed = lo[0] <= hi[0] && vol64 <= IB_BOX_CELLS;
+    a.m.chunk_info[2 * blockIdx.x] = make_int4(lo[0], lo[1], lo[2], 0);
+    a.m.chunk_info[2 * blockIdx.x + 1] =
+        boxed ? make_int4(dim0, dim1, dim2, (int)vol64) : make_int4(0, 0, 0, 0);
+    if (lo[0] <= hi[0] && !boxed) a.m.body_fallback[cbody] = step;
+  }
+  if (lo[0] > hi[0]) return; /* no IB-coupled marker in the chunk */
   const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
-  if (vol64 > IB_BOX_CELLS) { /* scattered markers: device-wide atomics per node */
+  if (!chunk || vol64 > IB_BOX_CELLS) { /* scattered markers: device-wide atomics per node */
     for (int j = wave; j < cnt; j += 2) {
       if (so[j].x == INT_MIN) continue;
-      const int ix = axis_index(so[j].x + kx, L.nx, L.periodic[0]);
-      const int iy = axis_index(so[j].y + ky, L.ny, L.periodic[1]);
-      const int iz = axis_index(so[j].z + kz, L.nz, L.periodic[2]);
+      const MLat Lj = chunk ? L : lat_of(a, sidx[j]);
+      const int ix = axis_index(so[j].x + kx, Lj.nx, Lj.periodic[0]);
+      const int iy = axis_index(so[j].y + ky, Lj.ny, Lj.periodic[1]);
+      const int iz = axis_index(so[j].z + kz, Lj.nz, Lj.periodic[2]);
       if (ix < 0 || iy < 0 || iz < 0) continue;
-      flag_tile(L, ix, iy, iz, step);
+      stamp_stencil_tile(Lj, kx, ky, kz, ix, iy, iz, step);
       const float w = sw[j][kx] * sw[j][4 + ky] * sw[j][8 + kz] * a.force_scale;
-      const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
-      atomicAdd(L.aux + c, w * sf[j][0]);
-      atomicAdd(L.aux + a.plane + c, w * sf[j][1]);
-      atomicAdd(L.aux + 2 * a.plane + c, w * sf[j][2]);
+      const int64_t c = ix + (int64_t)Lj.nx * (iy + (int64_t)Lj.ny * iz);
+      atomicAdd(Lj.aux + c, w * sf[j][0]);
+      atomicAdd(Lj.aux + a.plane + c, w * sf[j][1]);
+      atomicAdd(Lj.aux + 2 * a.plane + c, w * sf[j][2]);
     }
     return;
   }
@@ -1759,6 +1807,11 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
     mybox[2 * vol + c] += w * sf[j][2];
   }
   __syncthreads();
+  if (cbody >= 0) { /* the chunk's sums to its scratch box, zeros included */
+    float *out = a.m.chunk_sum + (int64_t)blockIdx.x * (3 * IB_BOX_CELLS);
+    for (int k = t; k < 3 * vol; k += 128) out[k] = box[0][k] + box[1][k];
+    return;
+  }
   /* Every tile the box meets is stamped (a superset of the stencils' tiles:
    * interpolation reads the velocity of all their nodes).  Along each axis
    * the box nodes inside one tile form runs that start at the box's first
@@ -1781,6 +1834,362 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
     if (fx != 0.f) atomicAdd(L.aux + c, fx);
     if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
     if (fz != 0.f) atomicAdd(L.aux + 2 * a.plane + c, fz);
+  }
+}
+
+/* ---------------- body spreading (form 4) ---------------- */
+
+/* Per tile, the step (high word) and the body that claimed the tile alone
+ * (low word), or OWNER_SHARED: another body, a marker outside the bodies,
+ * or a body that spreads with atomics meets the tile this step. */
+constexpr uint32_t OWNER_SHARED = 0xffffffffu;
+
+__device__ inline uint64_t owner_word(uint32_t step, uint32_t who) {
+  return ((uint64_t)step << 32) | who;
+}
+
+/* Distinct tiles (at most three: a periodic axis whose length is no
+ * multiple of the tile's extent wraps from a partial tile) of stencil nodes
+ * o..o+3 along one axis; nodes outside a wall-bounded axis are dropped. */
+__device__ inline int stencil_tiles3(int o, int n, int periodic, int shift, int t[3]) {
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i = axis_index(o + k, n, periodic);
+    if (i < 0) continue;
+    const int tt = i >> shift;
+    bool seen = false;
+    for (int q = 0; q < cnt; ++q) seen = seen || t[q] == tt;
+    if (!seen && cnt < 3) t[cnt++] = tt;
+  }
+  return cnt;
+}
+
+/* Mark every tile of the stencil with first node o shared this step. */
+__device__ inline void claim_stencil_shared(const MLat &L, const int o[3], uint32_t step) {
+  int tx[3], ty[3], tz[3];
+  const int nx = stencil_tiles3(o[0], L.nx, L.periodic[0], TILE_SHIFT_X, tx);
+  const int ny = stencil_tiles3(o[1], L.ny, L.periodic[1], TILE_SHIFT_Y, ty);
+  const int nz = stencil_tiles3(o[2], L.nz, L.periodic[2], TILE_SHIFT_Z, tz);
+  const uint64_t shared = owner_word(step, OWNER_SHARED);
+  for (int c = 0; c < nz; ++c)
+    for (int b = 0; b < ny; ++b)
+      for (int q = 0; q < nx; ++q)
+        atomicExch((unsigned long long *)(L.tile_owner + tx[q] + (int64_t)L.tnx * (ty[b] + (int64_t)L.tny * tz[c])),
+                   (unsigned long long)shared);
+}
+
+/* Claim tile gt for body g: it stays the body's only while no other
+ * contributor claims it this step. */
+__device__ inline void claim_tile(uint64_t *owner, uint32_t step, uint32_t g) {
+  const uint64_t mine = owner_word(step, g), shared = owner_word(step, OWNER_SHARED);
+  unsigned long long *p = (unsigned long long *)owner;
+  unsigned long long cur = atomicCAS(p, (unsigned long long)mine, (unsigned long long)mine);
+  while (cur != (unsigned long long)mine) {
+    const unsigned long long want = (cur >> 32) == step ? (unsigned long long)shared
+                                                       : (unsigned long long)mine;
+    if (cur == want) break; /* already shared */
+    const unsigned long long prev = atomicCAS(p, cur, want);
+    if (prev == cur) break;
+    cur = prev;
+  }
+}
+
+/* Stencil box of body g (markers i0 .. i0+cnt-1): the first stencil nodes
+ * of its IB-coupled markers in the periodic image nearest to its first
+ * marker, reduced over the workgroup of 256 lanes; lo > hi: none.  The
+ * claims and the spreading both compute it from the same positions. */
+__device__ void body_box(const MArgs &a, int i0, int cnt, const MLat &L, const float ref[3],
+                         int lo[3], int hi[3]) {
+  __shared__ int shb[4][6];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = INT_MAX;
+    hi[k] = INT_MIN;
+  }
+#pragma unroll 4
+  for (int j = t; j < cnt; j += 256) {
+    const int i = i0 + j;
+    if (!ib_coupled(a.m.kind[i], a.fiber_move)) continue;
+    const float v[3] = {a.m.x[i], a.m.y[i], a.m.z[i]};
+    float W[3];
+    int o[3];
+    stencil_origin(L, v, ref, W, o);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = min(lo[k], o[k]);
+      hi[k] = max(hi[k], o[k] + 3);
+    }
+  }
+#pragma unroll
+  for (int s2 = 32; s2 > 0; s2 >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = min(lo[k], __shfl_xor(lo[k], s2, 64));
+      hi[k] = max(hi[k], __shfl_xor(hi[k], s2, 64));
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      shb[wave][k] = lo[k];
+      shb[wave][3 + k] = hi[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = min(min(shb[0][k], shb[1][k]), min(shb[2][k], shb[3][k]));
+    hi[k] = max(max(shb[0][3 + k], shb[1][3 + k]), max(shb[2][3 + k], shb[3][3 + k]));
+  }
+  __syncthreads(); /* shb may be written again */
+}
+
+/* The tiles of a body box, as runs of box nodes along each axis: run[ax][b]
+ * is the run of box node b, tile[ax][r] the lattice tile coordinate of run
+ * r (-1: nodes outside a wall-bounded axis).  Consecutive nodes in one tile
+ * form a run.  Lanes 0, 64 and 128 build one axis each. */
+struct BoxRuns {
+  uint8_t run[3][BODY_BOX_DIM];
+  int16_t tile[3][BODY_BOX_DIM];
+  int nrun[3];
+};
+
+__device__ void box_runs(const MLat &L, const int lo[3], const int dim[3], BoxRuns &R) {
+  const int t = threadIdx.x;
+  if ((t & 63) == 0 && (t >> 6) < 3) {
+    const int ax = t >> 6;
+    const int n = ax == 0 ? L.nx : ax == 1 ? L.ny : L.nz;
+    const int shift = ax == 0 ? TILE_SHIFT_X : ax == 1 ? TILE_SHIFT_Y : TILE_SHIFT_Z;
+    int r = -1, prev = INT_MIN;
+    for (int b = 0; b < dim[ax]; ++b) {
+      const int g = axis_index(lo[ax] + b, n, L.periodic[ax]);
+      const int tt = g < 0 ? -1 : g >> shift;
+      if (b == 0 || tt != prev) {
+        ++r;
+        R.tile[ax][r] = (int16_t)tt;
+        prev = tt;
+      }
+      R.run[ax][b] = (uint8_t)r;
+    }
+    R.nrun[ax] = r + 1;
+  }
+  __syncthreads();
+}
+
+/* Whether a body's box can take tile ownership: at most BODY_BOX_DIM nodes
+ * along each axis, none longer than a periodic axis (each box node is then
+ * one lattice node). */
+__device__ inline bool body_box_fits(const MLat &L, const int dim[3]) {
+  const int n[3] = {L.nx, L.ny, L.nz};
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    ok = ok && dim[k] <= BODY_BOX_DIM && !(L.periodic[k] && dim[k] > n[k]);
+  return ok;
+}
+
+__device__ void claim_role(const MArgs &a, int block) {
+  __shared__ BoxRuns R;
+  const uint32_t step = (uint32_t)*a.m.d_step;
+  const int t = threadIdx.x;
+  if (block >= a.blocks1) { /* the markers outside the bodies: their tiles are shared */
+    const int k = 256 * (block - a.blocks1) + t;
+    const int nitems = a.m.rchunk_start[a.m.n_rchunks] + a.m.n_rdirect;
+    if (k >= nitems) return;
+    const int i = a.m.rchunk_items[k];
+    if (!ib_coupled(a.m.kind[i], a.fiber_move)) return;
+    const MLat L = lat_of(a, i);
+    const float v[3] = {a.m.x[i], a.m.y[i], a.m.z[i]};
+    float own[3], W[3];
+    int o[3];
+    own[0] = wrap_coord(v[0], L.nx, L.periodic[0]);
+    own[1] = wrap_coord(v[1], L.ny, L.periodic[1]);
+    own[2] = wrap_coord(v[2], L.nz, L.periodic[2]);
+    stencil_origin(L, v, own, W, o);
+    claim_stencil_shared(L, o, step);
+    return;
+  }
+  const int g = block;
+  const int i0 = a.m.body_range[4 * g], cnt = a.m.body_range[4 * g + 1] - i0;
+  const MLat L = lat_of(a, i0);
+  float ref[3];
+  ref[0] = wrap_coord(a.m.x[i0], L.nx, L.periodic[0]);
+  ref[1] = wrap_coord(a.m.y[i0], L.ny, L.periodic[1]);
+  ref[2] = wrap_coord(a.m.z[i0], L.nz, L.periodic[2]);
+  int lo[3], hi[3];
+  body_box(a, i0, cnt, L, ref, lo, hi);
+  if (lo[0] > hi[0]) return;
+  const int dim[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+  bool fits = body_box_fits(L, dim);
+  if (fits) box_runs(L, lo, dim, R);
+  const int nt = fits ? R.nrun[0] * R.nrun[1] * R.nrun[2] : 0;
+  fits = fits && nt <= BODY_BOX_TILES;
+  if (!fits) { /* spread with atomics: every tile of its stencils is shared */
+    for (int j = t; j < cnt; j += 256) {
+      const int i = i0 + j;
+      if (!ib_coupled(a.m.kind[i], a.fiber_move)) continue;
+      const float v[3] = {a.m.x[i], a.m.y[i], a.m.z[i]};
+      float W[3];
+      int o[3];
+      stencil_origin(L, v, ref, W, o);
+      claim_stencil_shared(L, o, step);
+    }
+    return;
+  }
+  for (int lt = t; lt < nt; lt += 256) {
+    const int rx = lt % R.nrun[0], r2 = lt / R.nrun[0];
+    const int ry = r2 % R.nrun[1], rz = r2 / R.nrun[1];
+    const int tx = R.tile[0][rx], ty = R.tile[1][ry], tz = R.tile[2][rz];
+    if (tx < 0 || ty < 0 || tz < 0) continue;
+    claim_tile(L.tile_owner + tx + (int64_t)L.tnx * (ty + (int64_t)L.tny * tz), step, (uint32_t)g);
+  }
+}
+
+/* Second pass of the body spreading: a workgroup of 256 lanes per body.
+ * The first pass (spread_chunk_kernel with spread_scratch) left each chunk
+ * of the body's sums in a scratch box (its extents in chunk_info); the
+ * body's box is their union (the box the force launch claimed tiles for).
+ * Each tile of it is stamped with the step; each node sums the chunks'
+ * boxes that hold it, in chunk order, and a nonzero sum goes to the
+ * lattice field with a plain store when the force launch left its tile
+ * claimed by this body alone (no other stencil meets it, so the node is
+ * this body's sum and was zero), else with a device-wide atomic.  A body
+ * spread once this step, or one of whose chunks spread with atomics, adds
+ * with atomics only.  Contributions are the direct form's bit for bit;
+ * only the order of the sums differs, and it is fixed. */
+__global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
+  __shared__ BoxRuns R;
+  __shared__ uint8_t owned[BODY_BOX_TILES];
+  __shared__ int4 cinfo[BODY_MAX_CHUNKS][2];
+  __shared__ int shb[4][6];
+  __shared__ int may_store;
+  if (halted(a)) return;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int g = blockIdx.x;
+  const int i0 = a.m.body_range[4 * g];
+  const int c0 = a.m.body_range[4 * g + 2], nc = a.m.body_range[4 * g + 3] - c0;
+  const MLat L = lat_of(a, i0);
+  const uint32_t step = (uint32_t)*a.m.d_step;
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  for (int q = t; q < nc; q += 256) {
+    const int4 l = a.m.chunk_info[2 * (c0 + q)], d = a.m.chunk_info[2 * (c0 + q) + 1];
+    cinfo[q][0] = l;
+    cinfo[q][1] = d;
+    if (d.w > 0) {
+      lo[0] = min(lo[0], l.x);
+      lo[1] = min(lo[1], l.y);
+      lo[2] = min(lo[2], l.z);
+      hi[0] = max(hi[0], l.x + d.x - 1);
+      hi[1] = max(hi[1], l.y + d.y - 1);
+      hi[2] = max(hi[2], l.z + d.z - 1);
+    }
+  }
+#pragma unroll
+  for (int s2 = 32; s2 > 0; s2 >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = min(lo[k], __shfl_xor(lo[k], s2, 64));
+      hi[k] = max(hi[k], __shfl_xor(hi[k], s2, 64));
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      shb[wave][k] = lo[k];
+      shb[wave][3 + k] = hi[k];
+    }
+  }
+  if (t == 0) { /* the first spreading of this body in this step stores */
+    may_store = a.m.body_spread[g] != (int32_t)step && a.m.body_fallback[g] != (int32_t)step;
+    a.m.body_spread[g] = (int32_t)step;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = min(min(shb[0][k], shb[1][k]), min(shb[2][k], shb[3][k]));
+    hi[k] = max(max(shb[0][3 + k], shb[1][3 + k]), max(shb[2][3 + k], shb[3][3 + k]));
+  }
+  if (lo[0] > hi[0]) return; /* no chunk wrote a box */
+  const int dim[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+  bool fits = body_box_fits(L, dim);
+  if (fits) box_runs(L, lo, dim, R);
+  const int nt = fits ? R.nrun[0] * R.nrun[1] * R.nrun[2] : 0;
+  fits = fits && nt <= BODY_BOX_TILES;
+  if (!fits) { /* chunk by chunk, every nonzero node with atomics */
+    for (int q = 0; q < nc; ++q) {
+      const int4 l = cinfo[q][0], d = cinfo[q][1];
+      const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (3 * IB_BOX_CELLS);
+      for (int k = t; k < d.w; k += 256) {
+        const int r = k / d.x, bx = k - r * d.x;
+        const int bz = r / d.y, by = r - bz * d.y;
+        const int gx = axis_index(l.x + bx, L.nx, L.periodic[0]);
+        const int gy = axis_index(l.y + by, L.ny, L.periodic[1]);
+        const int gz = axis_index(l.z + bz, L.nz, L.periodic[2]);
+        if (gx < 0 || gy < 0 || gz < 0) continue;
+        if ((bx == 0 || (gx & (TILE_X - 1)) == 0) && (by == 0 || (gy & (TILE_Y - 1)) == 0) &&
+            (bz == 0 || (gz & (TILE_Z - 1)) == 0))
+          L.tile_stamp[(gx >> TILE_SHIFT_X) + L.tnx * ((gy >> TILE_SHIFT_Y) + L.tny * (gz >> TILE_SHIFT_Z))] =
+              (int32_t)step;
+        const int64_t c = gx + (int64_t)L.nx * (gy + (int64_t)L.ny * gz);
+        const float fx = src[k], fy = src[d.w + k], fz = src[2 * d.w + k];
+        if (fx != 0.f) atomicAdd(L.aux + c, fx);
+        if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
+        if (fz != 0.f) atomicAdd(L.aux + 2 * a.plane + c, fz);
+      }
+    }
+    return;
+  }
+  const uint64_t mine = owner_word(step, (uint32_t)g);
+  for (int lt = t; lt < nt; lt += 256) { /* stamps and ownership of the box's tiles */
+    const int rx = lt % R.nrun[0], r2 = lt / R.nrun[0];
+    const int ry = r2 % R.nrun[1], rz = r2 / R.nrun[1];
+    const int tx = R.tile[0][rx], ty = R.tile[1][ry], tz = R.tile[2][rz];
+    bool own = false;
+    if (tx >= 0 && ty >= 0 && tz >= 0) {
+      const int64_t gt = tx + (int64_t)L.tnx * (ty + (int64_t)L.tny * tz);
+      L.tile_stamp[gt] = (int32_t)step;
+      own = may_store && L.tile_owner[gt] == mine;
+    }
+    owned[lt] = own ? 1 : 0;
+  }
+  __syncthreads();
+  const int vol = dim[0] * dim[1] * dim[2];
+  for (int k = t; k < vol; k += 256) {
+    const int r = k / dim[0], bx = k - r * dim[0];
+    const int bz = r / dim[1], by = r - bz * dim[1];
+    const int ux = lo[0] + bx, uy = lo[1] + by, uz = lo[2] + bz;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    for (int q = 0; q < nc; ++q) {
+      const int4 d = cinfo[q][1];
+      const int4 l = cinfo[q][0];
+      const int px = ux - l.x, py = uy - l.y, pz = uz - l.z;
+      if (d.w == 0 || (unsigned)px >= (unsigned)d.x || (unsigned)py >= (unsigned)d.y ||
+          (unsigned)pz >= (unsigned)d.z)
+        continue;
+      const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (3 * IB_BOX_CELLS);
+      const int p = px + d.x * (py + d.y * pz);
+      fx += src[p];
+      fy += src[d.w + p];
+      fz += src[2 * d.w + p];
+    }
+    if (fx == 0.f && fy == 0.f && fz == 0.f) continue;
+    const int gx = axis_index(ux, L.nx, L.periodic[0]);
+    const int gy = axis_index(uy, L.ny, L.periodic[1]);
+    const int gz = axis_index(uz, L.nz, L.periodic[2]);
+    if (gx < 0 || gy < 0 || gz < 0) continue;
+    const int64_t c = gx + (int64_t)L.nx * (gy + (int64_t)L.ny * gz);
+    const int lt = R.run[0][bx] + R.nrun[0] * (R.run[1][by] + R.nrun[1] * R.run[2][bz]);
+    if (owned[lt]) {
+      if (fx != 0.f) L.aux[c] = fx;
+      if (fy != 0.f) L.aux[a.plane + c] = fy;
+      if (fz != 0.f) L.aux[2 * a.plane + c] = fz;
+    } else {
+      if (fx != 0.f) atomicAdd(L.aux + c, fx);
+      if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
+      if (fz != 0.f) atomicAdd(L.aux + 2 * a.plane + c, fz);
+    }
   }
 }
 
@@ -2179,19 +2588,12 @@ __device__ void update_marker(const MArgs &a, int i, float vx, float vy, float v
   a.m.z[i] += vz * dt;
 }
 
-/* roles: A = interpolate V at the markers, B = move the markers and advance
- * the step counter.  Four markers per wavefront, 16 lanes each. */
-__global__ __launch_bounds__(256) void move_kernel(MArgs a) {
-  const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;
-  const int lane = threadIdx.x & 15;
-  if (halted(a)) return;
-  if ((a.roles & ROLE_B) && blockIdx.x == 0 && threadIdx.x == 0) *a.m.d_step += 1;
-  if (i >= a.m.n) return;
-  const int kind = a.m.kind[i];
-  float v[3] = {0.f, 0.f, 0.f};
+/* After the interpolation of marker i by its group of 16 lanes (v, ROLE_A):
+ * store the velocity, then (ROLE_B) move the marker; lane 0 of the group
+ * writes. */
+__device__ inline void finish_marker(const MArgs &a, int i, int lane, int kind, float v[3]) {
   const bool bbk = bbk_kind(a, kind);
   if (a.roles & ROLE_A) {
-    if (ib_coupled(kind, a.fiber_move)) interp_marker(a, i, lane, v);
     if (lane == 0 && bbk) { /* the element keeps its own velocity */
       a.m.uib[i] = make_float4(v[0], v[1], v[2], 0.f);
     } else if (lane == 0 && (ib_coupled(kind, a.fiber_move) || kind == PFG_MARKER_FIXED)) {
@@ -2214,6 +2616,235 @@ __global__ __launch_bounds__(256) void move_kernel(MArgs a) {
   if ((a.roles & ROLE_B) && lane == 0) update_marker(a, i, v[0], v[1], v[2]);
 }
 
+/* roles: A = interpolate V at the markers, B = move the markers and advance
+ * the step counter.  Four markers per wavefront, 16 lanes each. */
+__global__ __launch_bounds__(256) void move_kernel(MArgs a) {
+  const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int lane = threadIdx.x & 15;
+  if (halted(a)) return;
+  if ((a.roles & ROLE_B) && blockIdx.x == 0 && threadIdx.x == 0) *a.m.d_step += 1;
+  if (i >= a.m.n) return;
+  const int kind = a.m.kind[i];
+  float v[3] = {0.f, 0.f, 0.f};
+  if ((a.roles & ROLE_A) && ib_coupled(kind, a.fiber_move)) interp_marker(a, i, lane, v);
+  finish_marker(a, i, lane, kind, v);
+}
+
+/* Interpolation of marker i by its group of 16 lanes from its staged first
+ * stencil node o and kernel weights w (x 0..3, y 4..7, z 8..11), reading
+ * the field: the nodes and weights of interp_marker, summed in its order. */
+__device__ inline void interp_staged(const MLat &L, int4 o, const float *w, int j, float v[3]) {
+  const int jx = j & 3, jy = j >> 2;
+  const int ix = axis_index(o.x + jx, L.nx, L.periodic[0]);
+  const int iy = axis_index(o.y + jy, L.ny, L.periodic[1]);
+  float ux = 0.f, uy = 0.f, uz = 0.f;
+  if (ix >= 0 && iy >= 0) {
+    const int64_t col = ix + (int64_t)L.nx * iy;
+    const int64_t layer = (int64_t)L.nx * L.ny;
+    const float wxy = w[jx] * w[4 + jy];
+#pragma unroll
+    for (int kz = 0; kz < 4; ++kz) {
+      const int iz = axis_index(o.z + kz, L.nz, L.periodic[2]);
+      if (iz < 0) continue;
+      const float ww = wxy * w[8 + kz];
+      const float4 u = L.vel[col + layer * iz];
+      ux += ww * u.x;
+      uy += ww * u.y;
+      uz += ww * u.z;
+    }
+  }
+#pragma unroll
+  for (int s = 8; s > 0; s >>= 1) {
+    ux += __shfl_xor(ux, s, 16);
+    uy += __shfl_xor(uy, s, 16);
+    uz += __shfl_xor(uz, s, 16);
+  }
+  v[0] = ux;
+  v[1] = uy;
+  v[2] = uz;
+}
+
+/* Chunk-staged interpolation (ROLE_A, always) and move (ROLE_B): a
+ * workgroup of 128 lanes per IB chunk, then one per IB_CHUNK markers listed
+ * for direct spreading.  Every global read of a marker (index, kind,
+ * position) is made once, by a lane per marker, into LDS together with the
+ * marker's first stencil node and kernel weights (the staged spreading's),
+ * so that the loop over the markers reads only LDS and writes.  When a
+ * chunk's stencils span at most IB_BOX_CELLS nodes, the velocity of that
+ * box of nodes is read once into LDS, row by row (nodes outside a
+ * wall-bounded axis read as zero), and each group of 16 lanes interpolates
+ * one marker from it: lane j takes column (j & 3, j >> 2) over the four
+ * z-layers and the group sums as interp_marker sums, so the velocities
+ * equal the per-marker form's bit for bit while each node is read from the
+ * field once per chunk.  Scattered chunks and the listed markers read their
+ * stencils from the field with the staged weights.  Without the Langevin
+ * leapfrog and the fibrin kick, the move x += v dt uses the staged
+ * position; otherwise update_marker reads what it needs. */
+__global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
+  __shared__ int4 so[IB_CHUNK];      /* first stencil node (x = INT_MIN: not coupled);
+                                        .w: its box node */
+  __shared__ float sw[IB_CHUNK][12]; /* weights along x (0..3), y (4..7), z (8..11) */
+  __shared__ float4 sp[IB_CHUNK];    /* position; .w: marker index (int bits) */
+  __shared__ int skind[IB_CHUNK];
+  __shared__ float4 box[IB_BOX_CELLS];
+  __shared__ int sh[2][6];
+  if (halted(a)) return;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int grp = t >> 4, j16 = t & 15;
+  if ((a.roles & ROLE_B) && blockIdx.x == 0 && t == 0) *a.m.d_step += 1;
+  const bool chunk = (int)blockIdx.x < a.m.n_chunks;
+  int c0, cnt;
+  if (chunk) {
+    c0 = a.m.chunk_start[blockIdx.x];
+    cnt = a.m.chunk_start[blockIdx.x + 1] - c0;
+  } else { /* IB_CHUNK markers of the direct list (of any lattices) */
+    const int k0 = IB_CHUNK * ((int)blockIdx.x - a.m.n_chunks);
+    c0 = a.m.chunk_start[a.m.n_chunks] + k0;
+    cnt = min(IB_CHUNK, a.m.n_direct - k0);
+  }
+  if (cnt <= 0) return;
+  const int *items = a.m.chunk_items + c0;
+  const int m0 = items[0];
+  const MLat L0 = lat_of(a, m0); /* a chunk never spans two lattices */
+  float ref[3];
+  ref[0] = wrap_coord(a.m.x[m0], L0.nx, L0.periodic[0]);
+  ref[1] = wrap_coord(a.m.y[m0], L0.ny, L0.periodic[1]);
+  ref[2] = wrap_coord(a.m.z[m0], L0.nz, L0.periodic[2]);
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  if (t < cnt) {
+    const int i = items[t];
+    const int kind = a.m.kind[i];
+    const float v[3] = {a.m.x[i], a.m.y[i], a.m.z[i]};
+    sp[t] = make_float4(v[0], v[1], v[2], __int_as_float(i));
+    skind[t] = kind;
+    const bool has = ib_coupled(kind, a.fiber_move);
+    int o[3] = {0, 0, 0};
+    if (has) {
+      const MLat L = chunk ? L0 : lat_of(a, i);
+      float W[3];
+      if (chunk) {
+        stencil_origin(L, v, ref, W, o);
+      } else { /* its own image */
+        float own[3];
+        own[0] = wrap_coord(v[0], L.nx, L.periodic[0]);
+        own[1] = wrap_coord(v[1], L.ny, L.periodic[1]);
+        own[2] = wrap_coord(v[2], L.nz, L.periodic[2]);
+        stencil_origin(L, v, own, W, o);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        sw[t][k] = stencil_weight(W[0], k);
+        sw[t][4 + k] = stencil_weight(W[1], k);
+        sw[t][8 + k] = stencil_weight(W[2], k);
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = o[k];
+        hi[k] = o[k] + 3;
+      }
+    }
+    so[t] = make_int4(has ? o[0] : INT_MIN, o[1], o[2], -1);
+  }
+#pragma unroll
+  for (int s2 = 32; s2 > 0; s2 >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = min(lo[k], __shfl_xor(lo[k], s2, 64));
+      hi[k] = max(hi[k], __shfl_xor(hi[k], s2, 64));
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      sh[wave][k] = lo[k];
+      sh[wave][3 + k] = hi[k];
+    }
+  }
+  __syncthreads(); /* the staged markers and both halves' bounds */
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = min(sh[0][k], sh[1][k]);
+    hi[k] = max(sh[0][3 + k], sh[1][3 + k]);
+  }
+  const int64_t dim0 = (int64_t)hi[0] - lo[0] + 1, dim1 = (int64_t)hi[1] - lo[1] + 1,
+                dim2 = (int64_t)hi[2] - lo[2] + 1;
+  /* a chunk with IB-coupled markers (lo <= hi) whose box fits LDS */
+  const bool boxed = chunk && lo[0] <= hi[0] && dim0 * dim1 * dim2 <= IB_BOX_CELLS;
+  if (boxed) {
+    const int d0 = (int)dim0, d1 = (int)dim1, vol = (int)(dim0 * dim1 * dim2);
+    for (int k = t; k < vol; k += 128) {
+      const int r = k / d0, bx = k - r * d0;
+      const int bz = r / d1, by = r - bz * d1;
+      const int gx = axis_index(lo[0] + bx, L0.nx, L0.periodic[0]);
+      const int gy = axis_index(lo[1] + by, L0.ny, L0.periodic[1]);
+      const int gz = axis_index(lo[2] + bz, L0.nz, L0.periodic[2]);
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (gx >= 0 && gy >= 0 && gz >= 0) v = L0.vel[gx + (int64_t)L0.nx * (gy + (int64_t)L0.ny * gz)];
+      box[k] = v;
+    }
+    if (t < cnt && so[t].x != INT_MIN) {
+      const int4 oj = so[t];
+      so[t].w = (oj.x - lo[0]) + d0 * ((oj.y - lo[1]) + d1 * (oj.z - lo[2]));
+    }
+    __syncthreads(); /* the box and the box nodes */
+  }
+  const bool fast = a.sem_integrator != PFG_SEM_BBK && a.fiber_move != PFG_FIBER_KICK;
+  const int jx = j16 & 3, jy = j16 >> 2;
+  const int col = boxed ? jx + (int)dim0 * jy : 0, layer = boxed ? (int)(dim0 * dim1) : 0;
+  for (int j = grp; j < cnt; j += 8) {
+    const float4 p = sp[j];
+    const int i = __float_as_int(p.w);
+    const int kind = skind[j];
+    const bool coupled = ib_coupled(kind, a.fiber_move);
+    float v[3] = {0.f, 0.f, 0.f};
+    if (coupled) {
+      if (boxed) {
+        const int base = so[j].w + col;
+        const float wxy = sw[j][jx] * sw[j][4 + jy];
+        float ux = 0.f, uy = 0.f, uz = 0.f;
+#pragma unroll
+        for (int kz = 0; kz < 4; ++kz) {
+          const float w = wxy * sw[j][8 + kz];
+          const float4 u = box[base + layer * kz];
+          ux += w * u.x;
+          uy += w * u.y;
+          uz += w * u.z;
+        }
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+          ux += __shfl_xor(ux, o, 16);
+          uy += __shfl_xor(uy, o, 16);
+          uz += __shfl_xor(uz, o, 16);
+        }
+        v[0] = ux;
+        v[1] = uy;
+        v[2] = uz;
+      } else {
+        interp_staged(chunk ? L0 : lat_of(a, i), so[j], sw[j], j16, v);
+      }
+    }
+    if (!fast) {
+      finish_marker(a, i, j16, kind, v);
+    } else if (j16 == 0) { /* finish_marker's stores, update_marker's Euler move */
+      if (coupled || kind == PFG_MARKER_FIXED) {
+        a.m.vx[i] = v[0];
+        a.m.vy[i] = v[1];
+        a.m.vz[i] = v[2];
+      }
+      if ((a.roles & ROLE_B) && kind != PFG_MARKER_FIXED) {
+        float x = p.x, y = p.y, z = p.z;
+        const float dt = a.marker_dt;
+        x += v[0] * dt;
+        y += v[1] * dt;
+        z += v[2] * dt;
+        a.m.x[i] = x;
+        a.m.y[i] = y;
+        a.m.z[i] = z;
+      }
+    }
+  }
+}
+
 /* ---------------- launchers ---------------- */
 
 void fill_mlat(const pfg_handle *h, int k, MLat *L) {
@@ -2221,6 +2852,7 @@ void fill_mlat(const pfg_handle *h, int k, MLat *L) {
   L->aux = h->aux + I.cell0;
   L->vel = h->vel + I.cell0;
   L->tile_stamp = h->tile_stamp + I.tile0;
+  L->tile_owner = h->m.tile_owner ? h->m.tile_owner + I.tile0 : nullptr;
   L->tnx = I.tnx;
   L->tny = I.tny;
   L->tnz = I.tnz;
@@ -2296,6 +2928,7 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->mark_rows = h->overlap && h->row_stamp != nullptr ? 1 : 0;
   a->resolve_last = 0;
   a->count_spans = 0;
+  a->spread_scratch = 0;
   a->blocks0 = a->blocks1 = a->blocks2 = a->blocks3 = 0;
 }
 
@@ -2382,11 +3015,12 @@ hipError_t launch_resolve(pfg_handle *h) {
   return hipGetLastError();
 }
 
-/* spreading form: 1 staged chunks, 2 direct, 3 tile gather */
+/* spreading form: 1 staged chunks, 2 direct, 3 tile gather, 4 by body */
 static int spread_form(const pfg_handle *h) {
   if (h->ib_staging == 3 && !h->m.tile_gather_ok) return 1;
+  if (h->ib_staging == 4 && h->m.n_bodies == 0) return 1;
   if (h->ib_staging) return h->ib_staging;
-  return h->m.n >= IB_STAGED_MIN ? 1 : 2;
+  return h->m.n >= IB_STAGED_MIN ? (h->m.n_bodies > 0 ? 4 : 1) : 2;
 }
 
 static bool staged_spreading(const pfg_handle *h) { return spread_form(h) != 2; }
@@ -2414,7 +3048,14 @@ hipError_t launch_forces(pfg_handle *h, bool spread) {
   MArgs a;
   make_margs(h, &a);
   const int G = forces_lanes(h);
-  const unsigned grid = (unsigned)blocks_for((int64_t)h->m.n * G, 256);
+  unsigned grid = (unsigned)blocks_for((int64_t)h->m.n * G, 256);
+  if (!spread && spread_form(h) == 4) {
+    /* the body spreading's tile claims, in workgroups after the markers' */
+    a.blocks0 = (int)grid;
+    a.blocks1 = h->m.n_bodies;
+    a.blocks2 = blocks_for(h->m.n_ritems, 256);
+    grid += (unsigned)(a.blocks1 + a.blocks2);
+  }
   if (spread) {
     if (!forces_can_spread(h)) return hipErrorInvalidValue;
     hipLaunchKernelGGL((forces_kernel<64, true>), dim3(grid), dim3(256), 0, h->stream, a);
@@ -2459,9 +3100,20 @@ hipError_t launch_spread(pfg_handle *h) {
     hipLaunchKernelGGL(spread_tiles_kernel, dim3((unsigned)grid), dim3(256), 0, h->stream, a);
     return hipGetLastError();
   }
-  /* one workgroup per chunk, then one per two markers listed for direct
-   * spreading, in a single launch */
-  hipLaunchKernelGGL(spread_chunk_kernel, dim3(h->m.n_chunks + (h->m.n_direct + 1) / 2),
+  if (form == 4) {
+    /* the chunk launch (chunks of bodies to their scratch boxes, every other
+     * marker as in form 1), then a workgroup per body adds its boxes */
+    a.spread_scratch = 1;
+    hipLaunchKernelGGL(spread_chunk_kernel,
+                       dim3(h->m.n_chunks + (h->m.n_direct + IB_CHUNK - 1) / IB_CHUNK),
+                       dim3(128), 0, h->stream, a);
+    hipLaunchKernelGGL(spread_body_kernel, dim3(h->m.n_bodies), dim3(256), 0, h->stream, a);
+    return hipGetLastError();
+  }
+  /* one workgroup per chunk, then one per IB_CHUNK markers listed for
+   * direct spreading, in a single launch */
+  hipLaunchKernelGGL(spread_chunk_kernel,
+                     dim3(h->m.n_chunks + (h->m.n_direct + IB_CHUNK - 1) / IB_CHUNK),
                      dim3(128), 0, h->stream, a);
   return hipGetLastError();
 }
@@ -2471,6 +3123,13 @@ hipError_t launch_move(pfg_handle *h, bool interp, bool update) {
   MArgs a;
   make_margs(h, &a);
   a.roles = (interp ? ROLE_A : 0) | (update ? ROLE_B : 0);
+  if (interp && staged_spreading(h)) {
+    /* interpolation staged by chunk, as the spreading is */
+    hipLaunchKernelGGL(move_chunk_kernel,
+                       dim3(h->m.n_chunks + (h->m.n_direct + IB_CHUNK - 1) / IB_CHUNK),
+                       dim3(128), 0, h->stream, a);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(move_kernel, dim3(blocks_for(h->m.n, 16)), dim3(256), 0, h->stream, a);
   return hipGetLastError();
 }

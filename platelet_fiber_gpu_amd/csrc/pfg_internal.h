@@ -29,6 +29,16 @@ constexpr int IB_CHUNK = 128;
  * turn) */
 constexpr int SPREAD_TILE_GRID = 4096;
 constexpr int IB_BOX_CELLS = 1024;
+/* Body spreading (spreading form 4): the chunks of a compact body (a
+ * platelet of at most BODY_MAX_MARKERS markers, its stencils at most
+ * BODY_BOX_DIM nodes along each axis) write their LDS sums to a scratch box
+ * each; a workgroup per body adds its chunks' boxes over the body's box
+ * (at most BODY_BOX_TILES tiles for the tile claims, at most
+ * BODY_MAX_CHUNKS chunks) */
+constexpr int BODY_MAX_MARKERS = 4096;
+constexpr int BODY_BOX_DIM = 256;
+constexpr int BODY_BOX_TILES = 1024;
+constexpr int BODY_MAX_CHUNKS = 256;
 /* Marker count from which spreading is staged through LDS by default. */
 constexpr int IB_STAGED_MIN = 32768;
 /* Marker count from which the forces take 4 lanes per marker in bin order
@@ -95,6 +105,7 @@ struct MLat {
   float *aux;             /* the lattice's cells of the [3][plane] force      */
   const float4 *vel;      /* the lattice's cells of the velocity field        */
   int32_t *tile_stamp;    /* the lattice's tiles                              */
+  uint64_t *tile_owner;   /* the lattice's tiles (body spreading claims)       */
   int tnx, tny, tnz;
   int nx, ny, nz;
   int periodic[3];
@@ -206,6 +217,27 @@ struct Markers {
    * every periodic axis is at least twice the tile's extent along it */
   int32_t *tile_list = nullptr;
   int32_t *tile_count = nullptr;
+  /* body spreading: bodies (compact platelets: body_range[4g .. 4g+3] =
+   * their markers m0..m1-1 and chunks c0..c1-1; every marker of a body is
+   * in one of its chunks), chunk_body (the body of each chunk, -1: none),
+   * the chunks' scratch boxes (chunk_sum: 3 x IB_BOX_CELLS floats per chunk;
+   * chunk_info: the box's first node and extents per chunk, extent 0: not
+   * written this step), and the chunks and direct list of every marker
+   * outside the bodies (the chunk lists' format); per tile of all
+   * lattices, the step and the body that claimed it alone this step
+   * (OWNER_SHARED: more than one contributor), written by the force launch
+   * and read by spreading */
+  int n_bodies = 0;
+  int32_t *body_range = nullptr;
+  int32_t *chunk_body = nullptr;
+  float *chunk_sum = nullptr;
+  int4 *chunk_info = nullptr;
+  int n_rchunks = 0, n_rdirect = 0, n_ritems = 0;
+  int32_t *rchunk_start = nullptr, *rchunk_items = nullptr;
+  uint64_t *tile_owner = nullptr;
+  int32_t *body_spread = nullptr;   /* per body: the last step it spread in  */
+  int32_t *body_fallback = nullptr; /* per body: the last step one of its
+                                       chunks spread with atomics            */
   SpreadRec *spread_rec = nullptr; /* per cell-list slot: the marker's record */
   int tile_gather_ok = 0;
   /* platelet-platelet contact: per platelet pair (lo * n_plt + hi) the best

@@ -100,7 +100,7 @@ def test_batch_persistent_coupled_equals_per_row_launch(gpu, graph):
                 assert np.abs(a - w).max() < 1e-6, (r, k)
 
 
-@pytest.mark.parametrize("staging", [1, 2, 3])
+@pytest.mark.parametrize("staging", [1, 2, 3, 4])
 def test_batch_coupled_equals_single(gpu, staging):
     """Shear lattices of different sizes with one or two platelets each (no
     receptors: the bond RNG is keyed by the batch-wide receptor index): each

@@ -118,12 +118,15 @@ def test_forces(gpu, po, what, lanes):
     assert np.abs(F - oF).max() < 2e-5 * scale, (np.abs(F - oF).max(), scale)
 
 
-@pytest.mark.parametrize("staging", [1, 2, 3])
+@pytest.mark.parametrize("staging", [1, 2, 3, 4, -4])
 def test_spread_interp_parity(gpu, po, staging):
-    """The three spreading forms (1: LDS-staged chunks, 2: direct atomics,
-    3: tile gather over the cell lists) against the oracle's stencil form; a
-    seventh of the markers sit one periodic image away, so staged chunks and
-    gathered tiles straddle the periodic boundary."""
+    """The four spreading forms (1: LDS-staged chunks, 2: direct atomics,
+    3: tile gather over the cell lists, 4: a workgroup per platelet storing
+    the tiles the force pass claimed for it alone; -4: form 4 without the
+    claims, so every node is added with atomics) against the oracle's
+    stencil form; a seventh of the markers sit one periodic image away, so
+    staged chunks, platelet boxes and gathered tiles straddle the periodic
+    boundary.  Forms 1, 3 and 4 also stage the interpolation by chunk."""
     pfg = gpu
     sc = scene_small(pfg, receptors=-1, ligands=-1)
     rng = np.random.default_rng(5)
@@ -131,10 +134,12 @@ def test_spread_interp_parity(gpu, po, staging):
     f = (1e-3 * rng.standard_normal((3, n))).astype(np.float32)
     s, o = pair(pfg, po, sc)
     with s:
-        s.set_ib_staging(staging)
+        s.set_ib_staging(abs(staging))
         x, y, z = perturbed(sc, 0.1)
         x[::7] += sc.params.nx  # wrap on the periodic axis
         set_pos(s, o, x, y, z)
+        if staging == 4:
+            s.stage(pfg.STAGE_FORCES)  # the tile claims of these positions
         s.set_marker_forces(*f)
         for k, name in enumerate(("fx", "fy", "fz")):
             o.marker(name)[:] = f[k]
@@ -340,6 +345,41 @@ def test_contact_dense_cluster_parity(gpu, po):
     assert info["contact_overflow"] > 0  # the case the fifth-platelet path covers
     assert np.flatnonzero(np.abs(F).sum(1)).tolist() == np.flatnonzero(np.abs(oF).sum(1)).tolist()
     assert np.abs(F - oF).max() < 1e-5 * np.abs(oF).max()
+
+
+@pytest.mark.parametrize("name", ["thrombus_small", "tube_full"])
+def test_body_spreading_follows_the_staged_form(gpu, name):
+    """Coupled steps with the body spreading (form 4: a workgroup per
+    platelet, plain stores into the tiles the force pass claimed for it
+    alone, atomics elsewhere; the fibrin nodes as in form 1) against the
+    LDS-staged chunks (form 1) on the same scene, with platelets, receptors,
+    bonds and a fibrin network (at full size the 512x256x256 flag tube with
+    500 platelets and 20000 fibrin nodes): the same markers and fields up to
+    the FP32 order of the force sums."""
+    pfg = gpu
+    if name == "thrombus_small":
+        sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=96, ny=64, nz=64, n_platelets=6, elements=300,
+                       receptors=16, ligands=400, fibrin_nodes=800, fibrin_links=1200)
+        steps = 20
+    else:
+        sc = pfg.Scene(pfg.SCENE_THROMBUS)
+        steps = 2
+    sc.params.morse_taper = 0.2  # no pair switches at the cutoff on a rounding difference
+    runs = []
+    for form in (1, 4):
+        with pfg.Solver(sc.params) as s:
+            s.set_ib_staging(form)
+            s.load(sc)
+            s.step(steps)
+            mk = s.markers()
+            runs.append((np.stack([mk[k] for k in ("x", "y", "z")]), s.fields()))
+    dx = float(np.abs(runs[0][0] - runs[1][0]).max())
+    du = max(float(np.abs(a - b).max()) for a, b in zip(runs[0][1], runs[1][1]))
+    print(f"{name}: {steps} steps, max position difference {dx:.2e}, max field difference "
+          f"{du:.2e}")
+    xmax = max(sc.params.nx, sc.params.ny, sc.params.nz)
+    assert dx <= 2 * np.spacing(np.float32(xmax))
+    assert du < 5e-6
 
 
 @pytest.mark.parametrize("name", ["thrombus_small", "tube_full"])
