@@ -415,6 +415,16 @@ pfg_status pfg_set_ib_staging(pfg_handle *h, int mode);
  * fewer than 64 the markers are taken in cell-list order.  Same result up
  * to FP32 summation order. */
 pfg_status pfg_set_forces_lanes(pfg_handle *h, int lanes);
+/* Force pass by body (sem_Force_kernel's Morse term, sem_kernel.cu:320-513,
+ * acts between the elements of one platelet): 0 automatic (on from 32768
+ * markers), 1 on, 2 off.  On: a workgroup per platelet of at most 2048
+ * elements sorts its positions into a cell list of its own in LDS and sums
+ * each element's Morse pairs there, then its other terms; the other
+ * markers take the per-marker pass (pfg_set_forces_lanes).  It applies
+ * when each platelet's elements are one run of indices; otherwise the
+ * per-marker pass takes every marker.  Same forces up to FP32 summation
+ * order. */
+pfg_status pfg_set_forces_by_body(pfg_handle *h, int mode);
 /* enable = 1: within a step, the LB update of the lattice rows that no IB
  * stencil touches runs on a second HIP stream concurrently with the marker
  * pipeline, and the touched rows update after the spreading (semLB.cu:303-438
@@ -504,11 +514,13 @@ typedef struct pfg_tuning {
   double ms_default;     /* the automatic shapes                             */
   int shapes_timed;
   int lb_persistent;     /* pfg_set_lb_persistent (batches)                  */
+  int forces_by_body;    /* pfg_set_forces_by_body: 0 automatic, 1, 2        */
 } pfg_tuning;
 /* Times `steps` coupled steps of the handle's own lattices and markers for
  * each launch shape, one knob at a time from the automatic choices (LB cells
- * per lane, LB workgroup order, lanes per marker in the force pass, the
- * spreading form, the LB overlap, the persistent batched launch), keeps the fastest value of each and sets it on the handle.  The
+ * per lane, LB workgroup order, lanes per marker in the force pass, the force
+ * pass by body, the spreading form, the LB overlap, the persistent batched
+ * launch), keeps the fastest value of each and sets it on the handle.  The
  * state is saved on the device before and restored after (the handle
  * continues from where it was; the trial steps leave no trace), at the cost
  * of one extra copy of the populations and of the marker state while it

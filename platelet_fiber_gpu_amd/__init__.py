@@ -397,12 +397,18 @@ class Solver:
 
     def set_ib_staging(self, mode):
         """IB spreading form: 0 automatic, 1 LDS-staged chunks, 2 direct,
-        3 tile gather (no device-wide atomics)."""
+        3 tile gather (no device-wide atomics), 4 by platelet (plain stores
+        into the tiles a platelet meets alone)."""
         self._c("pfg_set_ib_staging", int(mode))
 
     def set_forces_lanes(self, lanes):
         """Lanes per marker in the force pass: 0 automatic, or 1, 4, 16, 64."""
         self._c("pfg_set_forces_lanes", int(lanes))
+
+    def set_forces_by_body(self, mode):
+        """Force pass by platelet (Morse pairs from an LDS cell list of each
+        platelet): 0 automatic, 1 on, 2 off."""
+        self._c("pfg_set_forces_by_body", int(mode))
 
     def set_overlap(self, on):
         """LB rows away from IB stencils update concurrently with the marker

@@ -85,7 +85,8 @@ class Tuning(C.Structure):
                 ("forces_lanes", C.c_int32),
                 ("ib_staging", C.c_int32), ("overlap", C.c_int32),
                 ("ms_per_step", C.c_double), ("ms_default", C.c_double),
-                ("shapes_timed", C.c_int32), ("lb_persistent", C.c_int32)]
+                ("shapes_timed", C.c_int32), ("lb_persistent", C.c_int32),
+                ("forces_by_body", C.c_int32)]
 
 
 class SceneOpts(C.Structure):
@@ -137,6 +138,7 @@ SIGNATURES = {
     "pfg_set_halt_on_detach": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_ib_staging": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_forces_lanes": (C.c_int, [C.c_void_p, C.c_int]),
+    "pfg_set_forces_by_body": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_set_overlap": (C.c_int, [C.c_void_p, C.c_int]),
     "pfg_batch_create": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int,
                                    C.POINTER(C.c_void_p)]),

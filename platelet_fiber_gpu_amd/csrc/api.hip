@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <set>
 #include <vector>
 
 #include "pfg_internal.h"
@@ -106,6 +107,7 @@ void free_markers(Markers &m) {
   dfree(m.tile_list); dfree(m.tile_count); dfree(m.spread_rec);
   dfree(m.body_range); dfree(m.rchunk_start); dfree(m.rchunk_items); dfree(m.tile_owner);
   dfree(m.body_spread); dfree(m.body_fallback); dfree(m.chunk_body); dfree(m.chunk_sum);
+  dfree(m.fbody_range); dfree(m.frest_items);
   dfree(m.chunk_info);
   Markers fresh;
   fresh.counters = m.counters;
@@ -894,6 +896,39 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   rchunk_start.push_back((int32_t)rchunk_items.size());
   rchunk_items.insert(rchunk_items.end(), rdirect_items.begin(), rdirect_items.end());
 
+  /* ---- force pass by body: the bodies of at most FB_MAX markers, when no
+   * body's elements are split over two runs of indices (Morse pairs act
+   * between the elements of one body, and a workgroup sees one run) ---- */
+  std::vector<int32_t> fbody_range, frest_items;
+  {
+    std::set<std::pair<int, int>> seen;
+    bool ok = true;
+    for (int g0 = 0; g0 < n && ok;) {
+      int g1 = g0 + 1;
+      while (g1 < n && M.body[g1] == M.body[g0] && !(minst && minst[g1] != minst[g0])) ++g1;
+      bool elements = false;
+      for (int i = g0; i < g1; ++i) elements = elements || M.kind[i] == PFG_MARKER_ELEMENT;
+      if (elements) ok = seen.insert({minst ? minst[g0] : 0, M.body[g0]}).second;
+      g0 = g1;
+    }
+    std::vector<char> in_body((size_t)n, 0);
+    for (size_t g = 0; ok && g < body_range.size(); g += 4) {
+      const int m0 = body_range[g], m1 = body_range[g + 1];
+      if (m1 - m0 > FB_MAX) continue;
+      fbody_range.push_back(m0);
+      fbody_range.push_back(m1);
+      for (int i = m0; i < m1; ++i) in_body[(size_t)i] = 1;
+    }
+    if (!fbody_range.empty())
+      for (int k = 0; k < n; ++k) {
+        /* the other markers in the spreading's spatial order */
+        const int i = chunk_items[(size_t)k];
+        if (!in_body[(size_t)i]) frest_items.push_back(i);
+      }
+  }
+  m.n_fbodies = (int)(fbody_range.size() / 2);
+  m.n_frest = (int)frest_items.size();
+
   /* ---- host-side CSR construction ---- */
   std::vector<std::pair<int, int>> kv;
   std::vector<int32_t> m2t_start, m2t, m2e_start, m2e_nbr, m2r_start, m2r, m2l_start,
@@ -1158,6 +1193,10 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   ALLOC_FILL(m.scan_partials, 0, m.scan_blocks);
   ALLOC_UP(m.chunk_start, chunk_start.data(), chunk_start.size());
   ALLOC_UP(m.chunk_items, chunk_items.data(), chunk_items.size());
+  if (!fbody_range.empty()) {
+    ALLOC_UP(m.fbody_range, fbody_range.data(), fbody_range.size());
+    if (!frest_items.empty()) ALLOC_UP(m.frest_items, frest_items.data(), frest_items.size());
+  }
   if (!body_range.empty()) {
     ALLOC_UP(m.body_range, body_range.data(), body_range.size());
     ALLOC_UP(m.chunk_body, chunk_body.data(), chunk_body.size());
@@ -1723,6 +1762,13 @@ pfg_status pfg_set_forces_lanes(pfg_handle *h, int lanes) {
   return PFG_OK;
 }
 
+pfg_status pfg_set_forces_by_body(pfg_handle *h, int mode) {
+  if (!h || mode < 0 || mode > 2) return PFG_ERR_ARG;
+  h->forces_body = mode;
+  drop_graph(h);
+  return PFG_OK;
+}
+
 pfg_status pfg_set_halt_on_detach(pfg_handle *h, int enable) {
   if (!h) return PFG_ERR_ARG;
   h->halt_on_detach = enable != 0;
@@ -1929,13 +1975,14 @@ hipError_t snapshot_take(pfg_handle *h, Snapshot &S) {
 }
 
 struct Shape {
-  int cpt, lanes, staging, overlap, xcd, persistent;
+  int cpt, lanes, staging, overlap, xcd, persistent, fbody;
 };
 
 hipError_t apply_shape(pfg_handle *h, const Shape &s) {
   drop_graph(h);
   h->lb_xcd_rows = s.xcd;
   h->forces_lanes = s.lanes;
+  h->forces_body = s.fbody;
   h->ib_staging = s.staging;
   h->overlap = s.overlap != 0;
   h->lb_persistent = h->K > 1 ? s.persistent : 0;
@@ -1983,7 +2030,7 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   hipSetDevice(h->device);
   const bool markers = h->model_loaded && h->m.n > 0 && !h->fluid_only;
   const Shape start{h->lb_cpt, h->forces_lanes, h->ib_staging, h->overlap ? 1 : 0,
-                    h->lb_xcd_rows, h->lb_persistent};
+                    h->lb_xcd_rows, h->lb_persistent, h->forces_body};
   Snapshot S;
   hipError_t e = snapshot_take(h, S);
   if (e != hipSuccess) {
@@ -1994,12 +2041,12 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
    * fastest value before the next is tried (values 0 are the automatic
    * choices); a value replaces the best only when it is 1 % faster */
   const int cpts[3] = {1, 2, 4}, lanes[7] = {64, 32, 16, 8, 4, 2, 1}, ovl[2] = {0, 1},
-            xcds[2] = {0, 1}, pers[2] = {0, 1};
+            xcds[2] = {0, 1}, pers[2] = {0, 1}, fbod[2] = {1, 2};
   /* spreading forms that apply to this model */
   int stag[4] = {1, 2, 0, 0}, nstag = 2;
   if (h->m.n_bodies > 0) stag[nstag++] = 4;
   if (h->m.tile_gather_ok) stag[nstag++] = 3;
-  Shape best{1, 0, 0, 0, (h->K == 1 && h->N >= XCD_ROWS_MIN_CELLS) ? 1 : 0, 0};
+  Shape best{1, 0, 0, 0, (h->K == 1 && h->N >= XCD_ROWS_MIN_CELLS) ? 1 : 0, 0, 0};
   const int warm = GRAPH_CHUNK;
   pfg_status st = PFG_OK;
   double best_ms = 0.0;
@@ -2009,21 +2056,23 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   out->ms_default = best_ms;
   tried = 1;
   /* knobs: 0 LB cells per lane, 1 LB row order, 5 the
-   * persistent batched launch (batches), 2 force lanes, 3 spreading form,
-   * 4 overlap */
-  const int order[6] = {0, 1, 5, 2, 3, 4};
-  for (int ki = 0; ki < 6 && st == PFG_OK; ++ki) {
+   * persistent batched launch (batches), 6 the force pass by body, 2 force
+   * lanes, 3 spreading form, 4 overlap */
+  const int order[7] = {0, 1, 5, 6, 2, 3, 4};
+  for (int ki = 0; ki < 7 && st == PFG_OK; ++ki) {
     const int knob = order[ki];
     const int *vals = knob == 0 ? cpts : knob == 1 ? xcds : knob == 2 ? lanes
-                                       : knob == 3 ? stag : knob == 4 ? ovl : pers;
+                                       : knob == 3 ? stag : knob == 4 ? ovl
+                                                          : knob == 5 ? pers : fbod;
     const int nv = knob == 0 ? 3 : knob == 2 ? 7 : knob == 3 ? nstag : 2;
     if (knob == 5 && h->K == 1) continue;
+    if (knob == 6 && (h->m.n_fbodies == 0 || !markers)) continue;
     if ((knob >= 2 && knob <= 4) && !markers) break;
     for (int v = 0; v < nv && st == PFG_OK; ++v) {
       Shape s = best;
       int *slot = knob == 0 ? &s.cpt : knob == 1 ? &s.xcd : knob == 2 ? &s.lanes
                                      : knob == 3 ? &s.staging : knob == 4 ? &s.overlap
-                                                              : &s.persistent;
+                                     : knob == 5 ? &s.persistent : &s.fbody;
       if (*slot == vals[v]) continue;
       *slot = vals[v];
       double ms = 0.0;
@@ -2052,6 +2101,7 @@ pfg_status pfg_autotune(pfg_handle *h, int steps, pfg_tuning *out) {
   out->ib_staging = best.staging;
   out->overlap = best.overlap;
   out->lb_persistent = best.persistent;
+  out->forces_by_body = best.fbody;
   out->ms_per_step = best_ms;
   out->shapes_timed = tried;
   return PFG_OK;

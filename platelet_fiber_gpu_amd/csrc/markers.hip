@@ -82,6 +82,8 @@ struct MArgs {
   int resolve_last; /* 1: the gather launch's last workgroup resolves the bonds */
   int count_spans;  /* 1: the counting launch adds the scan's span totals */
   int spread_scratch; /* 1: the chunks of bodies write scratch boxes (form 4) */
+  const int32_t *force_list; /* force_count >= 0: the force pass takes these */
+  int force_count;           /* markers only; -1: every marker                */
   int blocks0, blocks1, blocks2, blocks3; /* blocks of role ranges 0..3 */
 };
 
@@ -222,22 +224,57 @@ __device__ inline float wrap_coord(float v, int n, int periodic) {
   return w;
 }
 
-/* Lattice index along an axis, or -1 when outside a wall-bounded axis. */
+/* Lattice index along an axis, or -1 when outside a wall-bounded axis.
+ * Stencil and box nodes lie within one period of the lattice, where one
+ * add or subtract wraps them; the remainder handles any other index. */
 __device__ inline int axis_index(int i, int n, int periodic) {
   if (periodic) {
-    i %= n;
-    return i < 0 ? i + n : i;
+    if (i < 0) i += n;
+    else if (i >= n) i -= n;
+    if ((unsigned)i >= (unsigned)n) {
+      i %= n;
+      if (i < 0) i += n;
+    }
+    return i;
   }
   return (i < 0 || i >= n) ? -1 : i;
+}
+
+/* k / d for 0 <= k < 1024 and 1 <= d <= 1024, by the FP32 reciprocal
+ * inv = 1/d: (k + 1/2) / d lies at least 1/(2d) >= 2^-11 from an integer,
+ * and the product (below 1025) carries an error below 2^-13, so its floor
+ * is the quotient. */
+static_assert(IB_BOX_CELLS <= 1024, "div_small covers box nodes below 1024");
+__device__ inline int div_small(int k, float inv) { return (int)(((float)k + 0.5f) * inv); }
+
+/* The 4-point cosine kernel phi(r) = (1 + cos(pi r / 2)) / 4 at the four
+ * nodes floor(W) - 1 + k (k = 0..3) of coordinate W: with d = W - floor(W),
+ * r = d + 1 - k and cos(pi r / 2) is -sin, cos, sin, -cos of pi d / 2, so
+ * one sine-cosine pair gives all four weights. */
+__device__ inline void stencil_weights(float W, float w[4]) {
+  const float d = W - floorf(W);
+  float sn, cs;
+  sincospif(0.5f * d, &sn, &cs);
+  w[0] = 0.25f * (1.0f - sn);
+  w[1] = 0.25f * (1.0f + cs);
+  w[2] = 0.25f * (1.0f + sn);
+  w[3] = 0.25f * (1.0f - cs);
+}
+
+/* Weight k of stencil_weights, for a lane that needs one of them. */
+__device__ inline float stencil_weight(float W, int k) {
+  const float d = W - floorf(W);
+  float sn, cs;
+  sincospif(0.5f * d, &sn, &cs);
+  const float v = (k & 1) ? cs : sn;
+  return 0.25f * ((k == 0 || k == 3) ? 1.0f - v : 1.0f + v);
 }
 
 /* Stencil node k (0..3) along one axis for wrapped coordinate X: lattice index
  * (or -1) and weight phi(X - i). */
 __device__ inline int stencil_node(float X, int k, int n, int periodic, float *w) {
-  const float fl = floorf(X);
-  const float r = X - fl + 1.0f - (float)k; /* X - (fl - 1 + k) */
-  *w = 0.25f * (1.0f + cospif(0.5f * r));
-  return axis_index((int)fl - 1 + k, n, periodic);
+  *w = stencil_weight(X, k);
+  return axis_index((int)floorf(X) - 1 + k, n, periodic);
 }
 
 /* ---------------- K1: tile clear + bin counting ---------------- */
@@ -1295,6 +1332,59 @@ __device__ inline float group_sum(float v) {
   return v;
 }
 
+/* The element's force terms other than Morse, lane `lane` of G taking every
+ * G-th entry of its CSR lists: area and volume constraints, mesh-edge
+ * springs, receptor bond shares, and (lane 0) the wall repulsion on the
+ * platelet's lowest element. */
+template <int G>
+__device__ inline void element_terms(const MArgs &a, int i, int lane, float xi, float yi,
+                                     float zi, int body, float &fx, float &fy, float &fz) {
+  const int t0 = a.m.m2t_start[i], nt = (a.ks != 0.f || a.kv != 0.f) ? a.m.m2t_start[i + 1] - t0 : 0;
+  const int e0 = a.m.m2e_start[i], ne = a.k_spring != 0.f ? a.m.m2e_start[i + 1] - e0 : 0;
+  const int r0 = a.m.m2r_start[i], nr = a.m.m2r_start[i + 1] - r0;
+  float g_area = 0.f, cvol = 0.f;
+  if (nt > 0) {
+    const double Sall = a.m.plt_S[body], S0all = a.m.plt_S0[body];
+    const double V = a.m.plt_V[body], V0 = a.m.plt_V0[body];
+    g_area = (float)((Sall - S0all) / S0all);
+    cvol = -a.kv * (float)((V - V0) / V0) / 6.0f;
+  }
+  for (int k = lane; k < nt + ne + nr; k += G) {
+    if (k < nt) {
+      area_volume_term(a, a.m.m2t[t0 + k], g_area, cvol, fx, fy, fz);
+    } else if (k < nt + ne) {
+      /* mesh-edge Hookean spring k (r - L)/r (x_j - x_i) (sem_kernel.cu:374-384) */
+      const int q = e0 + k - nt;
+      const float4 pj = a.m.pos4[a.m.m2e_nbr[q]];
+      const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+      const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (r > 0.f) {
+        const float c = a.k_spring * (r - a.m.m2e_l0[q]) / r;
+        fx += c * dx;
+        fy += c * dy;
+        fz += c * dz;
+      }
+    } else {
+      /* receptor bond force, inverse-distance share of this vertex */
+      const int e = a.m.m2r[r0 + k - nt - ne];
+      const int r = e / 3, slot = e % 3;
+      const float w = a.m.rec_w[3 * r + slot];
+      fx += w * a.m.rec_force[3 * r];
+      fy += w * a.m.rec_force[3 * r + 1];
+      fz += w * a.m.rec_force[3 * r + 2];
+    }
+  }
+  /* wall repulsion on the lowest element (sem_kernel.cu:409-430) */
+  if (lane == 0 && a.wall_a != 0.f && a.m.plt_lowest[body] == i) {
+    float h = zi - a.wall_z;
+    if (h <= a.wall_hc) {
+      h = fmaxf(h, 0.01f * a.wall_hc);
+      const float e = expf(-a.wall_kappa * h);
+      fz += a.wall_a * a.wall_kappa * e / (1.0f - e);
+    }
+  }
+}
+
 /* A group of G lanes per marker (G = 64: one wavefront, markers in index
  * order, the Morse candidates of all neighbour ranges flattened over the
  * lanes; G < 64: markers in bin-sorted order, so that the groups of a
@@ -1323,8 +1413,9 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
   }
   __shared__ int sh_off[4][NBR_RANGES], sh_end[4][NBR_RANGES];
   const int k = (blockIdx.x * 256 + threadIdx.x) / G;
-  if (k >= a.m.n) return;
-  const int i = G == 64 ? k : a.m.bin_items[k];
+  /* every marker (in index or bin order), or those of a list */
+  if (k >= (a.force_count < 0 ? a.m.n : a.force_count)) return;
+  const int i = a.force_count >= 0 ? a.force_list[k] : (G == 64 ? k : a.m.bin_items[k]);
   const int kind = a.m.kind[i];
   const float4 pi = a.m.pos4[i];
   const float xi = pi.x, yi = pi.y, zi = pi.z;
@@ -1354,50 +1445,7 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
         });
       }
     }
-    const int t0 = a.m.m2t_start[i], nt = (a.ks != 0.f || a.kv != 0.f) ? a.m.m2t_start[i + 1] - t0 : 0;
-    const int e0 = a.m.m2e_start[i], ne = a.k_spring != 0.f ? a.m.m2e_start[i + 1] - e0 : 0;
-    const int r0 = a.m.m2r_start[i], nr = a.m.m2r_start[i + 1] - r0;
-    float g_area = 0.f, cvol = 0.f;
-    if (nt > 0) {
-      const double Sall = a.m.plt_S[body], S0all = a.m.plt_S0[body];
-      const double V = a.m.plt_V[body], V0 = a.m.plt_V0[body];
-      g_area = (float)((Sall - S0all) / S0all);
-      cvol = -a.kv * (float)((V - V0) / V0) / 6.0f;
-    }
-    for (int k = lane; k < nt + ne + nr; k += G) {
-      if (k < nt) {
-        area_volume_term(a, a.m.m2t[t0 + k], g_area, cvol, fx, fy, fz);
-      } else if (k < nt + ne) {
-        /* mesh-edge Hookean spring k (r - L)/r (x_j - x_i) (sem_kernel.cu:374-384) */
-        const int q = e0 + k - nt;
-        const float4 pj = a.m.pos4[a.m.m2e_nbr[q]];
-        const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        if (r > 0.f) {
-          const float c = a.k_spring * (r - a.m.m2e_l0[q]) / r;
-          fx += c * dx;
-          fy += c * dy;
-          fz += c * dz;
-        }
-      } else {
-        /* receptor bond force, inverse-distance share of this vertex */
-        const int e = a.m.m2r[r0 + k - nt - ne];
-        const int r = e / 3, slot = e % 3;
-        const float w = a.m.rec_w[3 * r + slot];
-        fx += w * a.m.rec_force[3 * r];
-        fy += w * a.m.rec_force[3 * r + 1];
-        fz += w * a.m.rec_force[3 * r + 2];
-      }
-    }
-    /* wall repulsion on the lowest element (sem_kernel.cu:409-430) */
-    if (lane == 0 && a.wall_a != 0.f && a.m.plt_lowest[body] == i) {
-      float h = zi - a.wall_z;
-      if (h <= a.wall_hc) {
-        h = fmaxf(h, 0.01f * a.wall_hc);
-        const float e = expf(-a.wall_kappa * h);
-        fz += a.wall_a * a.wall_kappa * e / (1.0f - e);
-      }
-    }
+    element_terms<G>(a, i, lane, xi, yi, zi, body, fx, fy, fz);
   } else {
     /* fibrin links in node-gather form (fiber.cpp:647-706) */
     const float sgn = a.fiber_ref_sign ? -1.0f : 1.0f;
@@ -1434,6 +1482,157 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
   /* the wavefront holds the marker's whole force: spread it at once, a
    * lane per stencil node (the direct form, without another launch) */
   if constexpr (SPREAD) spread_direct(a, i, lane, fx, fy, fz);
+}
+
+/* Force pass by body: a workgroup of BODY_THREADS lanes per body (a platelet of at
+ * most FB_MAX elements, whose Morse partners are its own elements).  Its
+ * packed positions go to LDS sorted into a cell list of the body's own box,
+ * with cells no smaller than the cutoff (so the 27 cells around an element
+ * hold every partner within it; the cell size doubles until the cells fit
+ * FB_BINS), each cell's elements in index order.  A lane per element then
+ * sums the Morse pairs of those cells from LDS (the pair test and FP32
+ * distance of the per-marker form, so the same cutoff decisions), adds the
+ * element's other terms and writes its force. */
+__global__ __launch_bounds__(BODY_THREADS) void forces_body_kernel(MArgs a) {
+  constexpr int NT = BODY_THREADS, NW = NT / 64, PER = (FB_MAX + NT - 1) / NT;
+  __shared__ float4 sp[FB_MAX];          /* positions in cell order          */
+  __shared__ int16_t sidx[FB_MAX];       /* their index within the body      */
+  __shared__ int cend[FB_BINS + 1];      /* counts, then ends of the cells   */
+  __shared__ float shf[NW][6];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int g = blockIdx.x;
+  const int i0 = a.m.fbody_range[2 * g], cnt = a.m.fbody_range[2 * g + 1] - i0;
+  float4 pr[PER];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int j = t + NT * k;
+    pr[k] = j < cnt ? a.m.pos4[i0 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (j < cnt) {
+      lo[0] = fminf(lo[0], pr[k].x);
+      lo[1] = fminf(lo[1], pr[k].y);
+      lo[2] = fminf(lo[2], pr[k].z);
+      hi[0] = fmaxf(hi[0], pr[k].x);
+      hi[1] = fmaxf(hi[1], pr[k].y);
+      hi[2] = fmaxf(hi[2], pr[k].z);
+    }
+  }
+#pragma unroll
+  for (int s2 = 32; s2 > 0; s2 >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fminf(lo[k], __shfl_xor(lo[k], s2, 64));
+      hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], s2, 64));
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      shf[wave][k] = lo[k];
+      shf[wave][3 + k] = hi[k];
+    }
+  }
+  for (int b = t; b <= FB_BINS; b += NT) cend[b] = 0;
+  __syncthreads();
+  if (cnt <= 0) return;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = shf[0][k];
+    hi[k] = shf[0][3 + k];
+    for (int w = 1; w < NW; ++w) {
+      lo[k] = fminf(lo[k], shf[w][k]);
+      hi[k] = fmaxf(hi[k], shf[w][3 + k]);
+    }
+  }
+  /* cells no smaller than the cutoff, doubled until they fit */
+  float cs = fmaxf(a.morse_cut, 1e-3f);
+  int d[3];
+  for (;;) {
+    int64_t nb = 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      d[k] = (int)fminf((hi[k] - lo[k]) / cs, 1e6f) + 1;
+      nb *= d[k];
+    }
+    if (nb <= FB_BINS) break;
+    cs *= 2.0f;
+  }
+  const float inv = 1.0f / cs;
+  const int nbins = d[0] * d[1] * d[2];
+  int cell[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int bx = min(max((int)((pr[k].x - lo[0]) * inv), 0), d[0] - 1);
+    const int by = min(max((int)((pr[k].y - lo[1]) * inv), 0), d[1] - 1);
+    const int bz = min(max((int)((pr[k].z - lo[2]) * inv), 0), d[2] - 1);
+    cell[k] = bx + d[0] * (by + d[1] * bz);
+    if (t + NT * k < cnt) atomicAdd(&cend[cell[k] + 1], 1);
+  }
+  __syncthreads();
+  if (t < 64) { /* inclusive scan: cend[b + 1] = end of cell b = start of b + 1 */
+    int run = 0;
+    for (int b0 = 1; b0 <= nbins; b0 += 64) {
+      const int b = b0 + lane;
+      const int v = b <= nbins ? cend[b] : 0;
+      const int incl = wave_incl_scan(v);
+      if (b <= nbins) cend[b] = run + incl;
+      run += __shfl(incl, 63, 64);
+    }
+  }
+  __syncthreads();
+  /* place: the fill pointer of cell b runs from its start (cend[b]) to its
+   * end, after which cend[b] is the end of cell b and cell b starts at
+   * cend[b - 1] (0 for the first) */
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int j = t + NT * k;
+    if (j < cnt) {
+      const int slot = atomicAdd(&cend[cell[k]], 1);
+      sp[slot] = pr[k];
+      sidx[slot] = (int16_t)j;
+    }
+  }
+  __syncthreads();
+  /* each cell's elements in index order (insertion sort, a lane per cell) */
+  for (int b = t; b < nbins; b += NT) {
+    const int s0 = b == 0 ? 0 : cend[b - 1], s1 = cend[b];
+    for (int q = s0 + 1; q < s1; ++q) {
+      const int16_t key = sidx[q];
+      const float4 kp = sp[q];
+      int r = q - 1;
+      while (r >= s0 && sidx[r] > key) {
+        sidx[r + 1] = sidx[r];
+        sp[r + 1] = sp[r];
+        --r;
+      }
+      sidx[r + 1] = key;
+      sp[r + 1] = kp;
+    }
+  }
+  __syncthreads();
+  for (int q = t; q < cnt; q += NT) {
+    const float4 pi = sp[q];
+    const int i = i0 + sidx[q];
+    const float xi = pi.x, yi = pi.y, zi = pi.z;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (a.morse_u0 != 0.f) {
+      const int bx = min(max((int)((xi - lo[0]) * inv), 0), d[0] - 1);
+      const int by = min(max((int)((yi - lo[1]) * inv), 0), d[1] - 1);
+      const int bz = min(max((int)((zi - lo[2]) * inv), 0), d[2] - 1);
+      for (int z = max(bz - 1, 0); z <= min(bz + 1, d[2] - 1); ++z)
+        for (int y = max(by - 1, 0); y <= min(by + 1, d[1] - 1); ++y) {
+          /* the three cells along x are one contiguous run of slots */
+          const int row = d[0] * (y + d[1] * z);
+          const int c0 = row + max(bx - 1, 0), c1 = row + min(bx + 1, d[0] - 1);
+          const int s0 = c0 == 0 ? 0 : cend[c0 - 1], s1 = cend[c1];
+          for (int c = s0; c < s1; ++c) morse_pair(a, sp[c], pi.w, xi, yi, zi, fx, fy, fz);
+        }
+    }
+    element_terms<1>(a, i, 0, xi, yi, zi, a.m.body[i], fx, fy, fz);
+    a.m.fx[i] = fx;
+    a.m.fy[i] = fy;
+    a.m.fz[i] = fz;
+  }
 }
 
 /* ---------------- platelet-platelet contact ---------------- */
@@ -1638,10 +1837,6 @@ __device__ inline void stencil_origin(const MLat &L, const float v[3], const flo
   }
 }
 
-__device__ inline float stencil_weight(float W, int k) {
-  const float r = W - floorf(W) + 1.0f - (float)k; /* W - (floor(W) - 1 + k) */
-  return 0.25f * (1.0f + cospif(0.5f * r));
-}
 
 /* Two wavefronts per chunk (at most IB_CHUNK markers of one body).  The
  * workgroup first stages the chunk in LDS: per marker its first stencil
@@ -1711,12 +1906,9 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
         own[2] = wrap_coord(v[2], Li.nz, Li.periodic[2]);
         stencil_origin(Li, v, own, W, o);
       }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        sw[t][k] = stencil_weight(W[0], k);
-        sw[t][4 + k] = stencil_weight(W[1], k);
-        sw[t][8 + k] = stencil_weight(W[2], k);
-      }
+      stencil_weights(W[0], &sw[t][0]);
+      stencil_weights(W[1], &sw[t][4]);
+      stencil_weights(W[2], &sw[t][8]);
       sf[t][0] = a.m.fx[i];
       sf[t][1] = a.m.fy[i];
       sf[t][2] = a.m.fz[i];
@@ -1812,6 +2004,7 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
     for (int k = t; k < 3 * vol; k += 128) out[k] = box[0][k] + box[1][k];
     return;
   }
+  const float inv0 = 1.0f / (float)dim0, inv1 = 1.0f / (float)dim1; /* vol <= IB_BOX_CELLS */
   /* Every tile the box meets is stamped (a superset of the stencils' tiles:
    * interpolation reads the velocity of all their nodes).  Along each axis
    * the box nodes inside one tile form runs that start at the box's first
@@ -1819,8 +2012,8 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
    * axis resumes at node 0), so the corner nodes of those runs reach every
    * tile while only a few lanes read and write the stamps. */
   for (int k = t; k < vol; k += 128) {
-    const int r = k / dim0, bx = k - r * dim0;
-    const int bz = r / dim1, by = r - bz * dim1;
+    const int r = div_small(k, inv0), bx = k - r * dim0;
+    const int bz = div_small(r, inv1), by = r - bz * dim1;
     const int gx = axis_index(lo[0] + bx, L.nx, L.periodic[0]);
     const int gy = axis_index(lo[1] + by, L.ny, L.periodic[1]);
     const int gz = axis_index(lo[2] + bz, L.nz, L.periodic[2]);
@@ -2063,7 +2256,8 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
   __shared__ BoxRuns R;
   __shared__ uint8_t owned[BODY_BOX_TILES];
   __shared__ int4 cinfo[BODY_MAX_CHUNKS][2];
-  __shared__ int shb[4][6];
+  constexpr int NT = 256, NW = NT / 64;
+  __shared__ int shb[NW][6];
   __shared__ int may_store;
   if (halted(a)) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
@@ -2073,7 +2267,7 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
   const MLat L = lat_of(a, i0);
   const uint32_t step = (uint32_t)*a.m.d_step;
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-  for (int q = t; q < nc; q += 256) {
+  for (int q = t; q < nc; q += NT) {
     const int4 l = a.m.chunk_info[2 * (c0 + q)], d = a.m.chunk_info[2 * (c0 + q) + 1];
     cinfo[q][0] = l;
     cinfo[q][1] = d;
@@ -2108,8 +2302,12 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    lo[k] = min(min(shb[0][k], shb[1][k]), min(shb[2][k], shb[3][k]));
-    hi[k] = max(max(shb[0][3 + k], shb[1][3 + k]), max(shb[2][3 + k], shb[3][3 + k]));
+    lo[k] = shb[0][k];
+    hi[k] = shb[0][3 + k];
+    for (int w = 1; w < NW; ++w) {
+      lo[k] = min(lo[k], shb[w][k]);
+      hi[k] = max(hi[k], shb[w][3 + k]);
+    }
   }
   if (lo[0] > hi[0]) return; /* no chunk wrote a box */
   const int dim[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
@@ -2121,7 +2319,7 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
     for (int q = 0; q < nc; ++q) {
       const int4 l = cinfo[q][0], d = cinfo[q][1];
       const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (3 * IB_BOX_CELLS);
-      for (int k = t; k < d.w; k += 256) {
+      for (int k = t; k < d.w; k += NT) {
         const int r = k / d.x, bx = k - r * d.x;
         const int bz = r / d.y, by = r - bz * d.y;
         const int gx = axis_index(l.x + bx, L.nx, L.periodic[0]);
@@ -2142,7 +2340,7 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
     return;
   }
   const uint64_t mine = owner_word(step, (uint32_t)g);
-  for (int lt = t; lt < nt; lt += 256) { /* stamps and ownership of the box's tiles */
+  for (int lt = t; lt < nt; lt += NT) { /* stamps and ownership of the box's tiles */
     const int rx = lt % R.nrun[0], r2 = lt / R.nrun[0];
     const int ry = r2 % R.nrun[1], rz = r2 / R.nrun[1];
     const int tx = R.tile[0][rx], ty = R.tile[1][ry], tz = R.tile[2][rz];
@@ -2156,23 +2354,38 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
   }
   __syncthreads();
   const int vol = dim[0] * dim[1] * dim[2];
-  for (int k = t; k < vol; k += 256) {
+  for (int k = t; k < vol; k += NT) {
     const int r = k / dim[0], bx = k - r * dim[0];
     const int bz = r / dim[1], by = r - bz * dim[1];
     const int ux = lo[0] + bx, uy = lo[1] + by, uz = lo[2] + bz;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    for (int q = 0; q < nc; ++q) {
-      const int4 d = cinfo[q][1];
-      const int4 l = cinfo[q][0];
-      const int px = ux - l.x, py = uy - l.y, pz = uz - l.z;
-      if (d.w == 0 || (unsigned)px >= (unsigned)d.x || (unsigned)py >= (unsigned)d.y ||
-          (unsigned)pz >= (unsigned)d.z)
-        continue;
-      const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (3 * IB_BOX_CELLS);
-      const int p = px + d.x * (py + d.y * pz);
-      fx += src[p];
-      fy += src[d.w + p];
-      fz += src[2 * d.w + p];
+    /* eight chunks at a time: their loads are issued together, then summed
+     * in chunk order (a chunk whose box does not hold the node adds 0) */
+    for (int q0 = 0; q0 < nc; q0 += 8) {
+      float vx[8], vy[8], vz[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        vx[u] = vy[u] = vz[u] = 0.f;
+        const int q = q0 + u;
+        if (q >= nc) continue;
+        const int4 d = cinfo[q][1];
+        const int4 l = cinfo[q][0];
+        const int px = ux - l.x, py = uy - l.y, pz = uz - l.z;
+        if (d.w == 0 || (unsigned)px >= (unsigned)d.x || (unsigned)py >= (unsigned)d.y ||
+            (unsigned)pz >= (unsigned)d.z)
+          continue;
+        const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (3 * IB_BOX_CELLS);
+        const int p = px + d.x * (py + d.y * pz);
+        vx[u] = src[p];
+        vy[u] = src[d.w + p];
+        vz[u] = src[2 * d.w + p];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        fx += vx[u];
+        fy += vy[u];
+        fz += vz[u];
+      }
     }
     if (fx == 0.f && fy == 0.f && fz == 0.f) continue;
     const int gx = axis_index(ux, L.nx, L.periodic[0]);
@@ -2245,12 +2458,13 @@ __global__ __launch_bounds__(256) void spread_prep_kernel(MArgs a) {
       const int ox = (int)floorf(W[0]) - 1, oy = (int)floorf(W[1]) - 1,
                 oz = (int)floorf(W[2]) - 1;
       r.o = make_int4(ox, oy, oz, 1);
-      r.wx = make_float4(stencil_weight(W[0], 0), stencil_weight(W[0], 1),
-                         stencil_weight(W[0], 2), stencil_weight(W[0], 3));
-      r.wy = make_float4(stencil_weight(W[1], 0), stencil_weight(W[1], 1),
-                         stencil_weight(W[1], 2), stencil_weight(W[1], 3));
-      r.wz = make_float4(stencil_weight(W[2], 0), stencil_weight(W[2], 1),
-                         stencil_weight(W[2], 2), stencil_weight(W[2], 3));
+      float w4[4];
+      stencil_weights(W[0], w4);
+      r.wx = make_float4(w4[0], w4[1], w4[2], w4[3]);
+      stencil_weights(W[1], w4);
+      r.wy = make_float4(w4[0], w4[1], w4[2], w4[3]);
+      stencil_weights(W[2], w4);
+      r.wz = make_float4(w4[0], w4[1], w4[2], w4[3]);
       r.f = make_float4(a.m.fx[i], a.m.fy[i], a.m.fz[i], 0.f);
       ntx = stencil_tiles(ox, L.nx, L.periodic[0], TILE_SHIFT_X, tx);
       nty = stencil_tiles(oy, L.ny, L.periodic[1], TILE_SHIFT_Y, ty);
@@ -2489,12 +2703,14 @@ __device__ void interp_marker(const MArgs &a, int i, int j, float *v) {
     const int64_t col = ix + (int64_t)L.nx * iy;
     const int64_t layer = (int64_t)L.nx * L.ny;
     const float wxy = wx * wy;
+    float wz4[4];
+    stencil_weights(Z, wz4);
+    const int oz = (int)floorf(Z) - 1;
 #pragma unroll
     for (int kz = 0; kz < 4; ++kz) {
-      float wz;
-      const int iz = stencil_node(Z, kz, L.nz, L.periodic[2], &wz);
+      const int iz = axis_index(oz + kz, L.nz, L.periodic[2]);
       if (iz < 0) continue;
-      const float w = wxy * wz;
+      const float w = wxy * wz4[kz];
       const int64_t c = col + layer * iz;
       const float4 v = L.vel[c];
       ux += w * v.x;
@@ -2731,12 +2947,9 @@ __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
         own[2] = wrap_coord(v[2], L.nz, L.periodic[2]);
         stencil_origin(L, v, own, W, o);
       }
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        sw[t][k] = stencil_weight(W[0], k);
-        sw[t][4 + k] = stencil_weight(W[1], k);
-        sw[t][8 + k] = stencil_weight(W[2], k);
-      }
+      stencil_weights(W[0], &sw[t][0]);
+      stencil_weights(W[1], &sw[t][4]);
+      stencil_weights(W[2], &sw[t][8]);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         lo[k] = o[k];
@@ -2772,9 +2985,10 @@ __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
   const bool boxed = chunk && lo[0] <= hi[0] && dim0 * dim1 * dim2 <= IB_BOX_CELLS;
   if (boxed) {
     const int d0 = (int)dim0, d1 = (int)dim1, vol = (int)(dim0 * dim1 * dim2);
+    const float inv0 = 1.0f / (float)d0, inv1 = 1.0f / (float)d1; /* vol <= IB_BOX_CELLS */
     for (int k = t; k < vol; k += 128) {
-      const int r = k / d0, bx = k - r * d0;
-      const int bz = r / d1, by = r - bz * d1;
+      const int r = div_small(k, inv0), bx = k - r * d0;
+      const int bz = div_small(r, inv1), by = r - bz * d1;
       const int gx = axis_index(lo[0] + bx, L0.nx, L0.periodic[0]);
       const int gy = axis_index(lo[1] + by, L0.ny, L0.periodic[1]);
       const int gz = axis_index(lo[2] + bz, L0.nz, L0.periodic[2]);
@@ -2929,6 +3143,8 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->resolve_last = 0;
   a->count_spans = 0;
   a->spread_scratch = 0;
+  a->force_list = nullptr;
+  a->force_count = -1;
   a->blocks0 = a->blocks1 = a->blocks2 = a->blocks3 = 0;
 }
 
@@ -3031,6 +3247,13 @@ static int forces_lanes(const pfg_handle *h) {
   return h->forces_lanes ? h->forces_lanes : (h->m.n >= FORCES_SORTED_MIN ? 4 : 64);
 }
 
+/* the force pass by body: automatic with many markers, when the model has
+ * such bodies */
+static bool forces_by_body(const pfg_handle *h) {
+  if (h->m.n_fbodies == 0 || h->forces_body == 2) return false;
+  return h->forces_body == 1 || h->m.n >= FORCES_SORTED_MIN;
+}
+
 static bool contact_on(const pfg_handle *h) {
   return h->m.contact_slot && h->p.contact_a != 0.f && h->m.n_plt >= 2;
 }
@@ -3048,7 +3271,14 @@ hipError_t launch_forces(pfg_handle *h, bool spread) {
   MArgs a;
   make_margs(h, &a);
   const int G = forces_lanes(h);
-  unsigned grid = (unsigned)blocks_for((int64_t)h->m.n * G, 256);
+  const bool by_body = !spread && forces_by_body(h);
+  if (by_body) { /* the per-marker pass over the markers outside those bodies */
+    a.force_list = h->m.frest_items;
+    a.force_count = h->m.n_frest;
+  }
+  /* at least one workgroup: the first one makes the detach test */
+  unsigned grid = (unsigned)std::max(
+      1, blocks_for((int64_t)(by_body ? h->m.n_frest : h->m.n) * G, 256));
   if (!spread && spread_form(h) == 4) {
     /* the body spreading's tile claims, in workgroups after the markers' */
     a.blocks0 = (int)grid;
@@ -3070,6 +3300,9 @@ hipError_t launch_forces(pfg_handle *h, bool spread) {
     case 32: hipLaunchKernelGGL(forces_kernel<32>, dim3(grid), dim3(256), 0, h->stream, a); break;
     default: hipLaunchKernelGGL(forces_kernel<64>, dim3(grid), dim3(256), 0, h->stream, a); break;
   }
+  if (by_body)
+    hipLaunchKernelGGL(forces_body_kernel, dim3(h->m.n_fbodies), dim3(BODY_THREADS), 0,
+                       h->stream, a);
   if (h->m.bend)
     hipLaunchKernelGGL(bend_force_kernel, dim3(blocks_for((int64_t)h->m.n * 4, 256)),
                        dim3(256), 0, h->stream, a);

@@ -39,6 +39,13 @@ constexpr int BODY_MAX_MARKERS = 4096;
 constexpr int BODY_BOX_DIM = 256;
 constexpr int BODY_BOX_TILES = 1024;
 constexpr int BODY_MAX_CHUNKS = 256;
+/* Force pass by body: a workgroup per body of at most FB_MAX markers keeps
+ * their positions and a cell list of at most FB_BINS bins in LDS for the
+ * Morse pairs (which act within a body only) */
+constexpr int FB_MAX = 1024;
+constexpr int FB_BINS = 2048;
+/* lanes of the per-body workgroups (force pass, second spreading pass) */
+constexpr int BODY_THREADS = 1024;
 /* Marker count from which spreading is staged through LDS by default. */
 constexpr int IB_STAGED_MIN = 32768;
 /* Marker count from which the forces take 4 lanes per marker in bin order
@@ -238,6 +245,11 @@ struct Markers {
   int32_t *body_spread = nullptr;   /* per body: the last step it spread in  */
   int32_t *body_fallback = nullptr; /* per body: the last step one of its
                                        chunks spread with atomics            */
+  /* force pass by body: bodies of at most FB_MAX markers (marker ranges
+   * fbody_range[2g], fbody_range[2g+1]) when every body's elements form one
+   * run of indices, and every other marker (frest_items) */
+  int n_fbodies = 0, n_frest = 0;
+  int32_t *fbody_range = nullptr, *frest_items = nullptr;
   SpreadRec *spread_rec = nullptr; /* per cell-list slot: the marker's record */
   int tile_gather_ok = 0;
   /* platelet-platelet contact: per platelet pair (lo * n_plt + hi) the best
@@ -308,6 +320,7 @@ struct pfg_handle {
   bool halt_on_detach = false; /* steps after the detach test fired change nothing */
   int ib_staging = 0;          /* spreading: 0 auto, 1 LDS-staged chunks, 2 direct */
   int forces_lanes = 0;        /* lanes per marker in the forces: 0 auto, 1, 4, 16, 64 */
+  int forces_body = 0;         /* force pass by body: 0 auto, 1 on, 2 off     */
   /* LB rows away from IB stencils update on stream2 while the marker
    * pipeline runs; the rows the stencils touch update after spreading.  Off
    * by default: on MI355X the LB launch saturates HBM and the CU slots, and

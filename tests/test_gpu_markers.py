@@ -75,13 +75,14 @@ def test_membrane_sums_fp64(gpu, po):
     assert np.allclose(tS, otS, rtol=1e-12)
 
 
-@pytest.mark.parametrize("lanes", [0, 8, 4, 1])
+@pytest.mark.parametrize("lanes", [0, 8, 4, 1, -1])
 @pytest.mark.parametrize("what", ["morse", "springs_area_volume", "wall", "fibrin_wlc",
                                   "fibrin_linear_refsign", "bending", "bending_mesh"])
 def test_forces(gpu, po, what, lanes):
     """Marker forces against the oracle with a wavefront per marker (0: the
-    automatic choice at this size) and with 8, 4 or 1 lanes per marker in
-    cell-list order."""
+    automatic choice at this size), with 8, 4 or 1 lanes per marker in
+    cell-list order, and (-1) by platelet: each platelet's Morse pairs from
+    a cell list of its own in LDS, the other markers with 4 lanes each."""
     pfg = gpu
     if what.startswith("fibrin"):
         sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=64, ny=48, nz=48, n_platelets=2, elements=150,
@@ -105,7 +106,12 @@ def test_forces(gpu, po, what, lanes):
         p.kbend = 2.0
     s, o = pair(pfg, po, sc, p)
     with s:
-        s.set_forces_lanes(lanes)
+        if lanes < 0:
+            s.set_forces_by_body(1)
+            s.set_forces_lanes(4)
+        else:
+            s.set_forces_by_body(2)
+            s.set_forces_lanes(lanes)
         set_pos(s, o, *perturbed(sc, 0.1))
         for st in (pfg.STAGE_CELLS, pfg.STAGE_MEMBRANE, pfg.STAGE_BONDS, pfg.STAGE_FORCES):
             s.stage(st)
