@@ -354,6 +354,48 @@ def test_contact_dense_cluster_parity(gpu, po):
 
 
 @pytest.mark.parametrize("name", ["thrombus_small", "tube_full"])
+def test_force_pass_by_body_follows_the_per_marker_pass(gpu, name):
+    """Coupled steps with the force pass by platelet (Morse pairs from each
+    platelet's neighbour lists, rebuilt when an element has moved half the
+    skin away from its platelet's mean motion) against the per-marker pass
+    over the global cell lists, on a scene with platelets, bonds and a
+    fibrin network: the same markers and fields up to the FP32 order of the
+    force sums, with the hard Morse cutoff; the near-cutoff pair visits the
+    two passes count agree (both visit every pair within the cutoff from
+    both ends; the trajectories differ in the last bits, which may move a
+    pair across the edge of that window).  Midway the positions are
+    uploaded again, which makes every platelet build its lists again."""
+    pfg = gpu
+    if name == "thrombus_small":
+        sc = pfg.Scene(pfg.SCENE_THROMBUS, nx=96, ny=64, nz=64, n_platelets=6, elements=300,
+                       receptors=16, ligands=400, fibrin_nodes=800, fibrin_links=1200)
+        steps = 30
+    else:
+        sc = pfg.Scene(pfg.SCENE_THROMBUS)
+        steps = 4
+    runs = []
+    for mode in (2, 1):
+        with pfg.Solver(sc.params) as s:
+            s.set_forces_by_body(mode)
+            s.load(sc)
+            s.step(steps // 2)
+            mk = s.markers()
+            s.set_markers(mk["x"], mk["y"], mk["z"])
+            info = s.step(steps - steps // 2)
+            mk = s.markers()
+            runs.append((np.stack([mk[k] for k in ("x", "y", "z")]), s.fields(), info))
+    dx = float(np.abs(runs[0][0] - runs[1][0]).max())
+    du = max(float(np.abs(a - b).max()) for a, b in zip(runs[0][1], runs[1][1]))
+    print(f"{name}: {steps} steps, max position difference {dx:.2e}, max field difference "
+          f"{du:.2e}, near-cutoff visits {runs[0][2]['near_cutoff']} / {runs[1][2]['near_cutoff']}")
+    xmax = max(sc.params.nx, sc.params.ny, sc.params.nz)
+    assert dx <= 4 * np.spacing(np.float32(xmax))
+    assert du < 5e-6
+    n0, n1 = runs[0][2]["near_cutoff"], runs[1][2]["near_cutoff"]
+    assert abs(n0 - n1) <= max(2, n0 // 100)
+
+
+@pytest.mark.parametrize("name", ["thrombus_small", "tube_full"])
 def test_body_spreading_follows_the_staged_form(gpu, name):
     """Coupled steps with the body spreading (form 4: a workgroup per
     platelet, plain stores into the tiles the force pass claimed for it
