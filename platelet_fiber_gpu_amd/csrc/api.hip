@@ -103,11 +103,11 @@ void free_markers(Markers &m) {
   dfree(m.m2l_start); dfree(m.m2l_nbr); dfree(m.m2l_l0);
   dfree(m.bin_count); dfree(m.bin_start); dfree(m.bin_items);
   dfree(m.marker_bin); dfree(m.marker_slot); dfree(m.scan_partials);
-  dfree(m.chunk_start); dfree(m.chunk_items); dfree(m.bin_scatter); dfree(m.sorted_pos);
+  dfree(m.chunk_start); dfree(m.chunk_items); dfree(m.chunk_ref); dfree(m.bin_scatter); dfree(m.sorted_pos);
   dfree(m.tile_list); dfree(m.tile_count); dfree(m.spread_rec);
   dfree(m.body_range); dfree(m.rchunk_start); dfree(m.rchunk_items); dfree(m.tile_owner);
   dfree(m.body_spread); dfree(m.body_fallback); dfree(m.chunk_body); dfree(m.chunk_sum);
-  dfree(m.fbody_range); dfree(m.frest_items);
+  dfree(m.fbody_range); dfree(m.frest_items); dfree(m.fb_ref); dfree(m.fb_nlen); dfree(m.fb_nbr);
   dfree(m.chunk_info);
   Markers fresh;
   fresh.counters = m.counters;
@@ -890,6 +890,10 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   chunk_start.push_back((int32_t)chunk_items.size());
   chunk_items.insert(chunk_items.end(), direct_items.begin(), direct_items.end());
   m.n_bodies = (int)(body_range.size() / 4);
+  std::vector<int32_t> chunk_ref((size_t)std::max(1, m.n_chunks));
+  for (int c = 0; c < m.n_chunks; ++c)
+    chunk_ref[(size_t)c] = chunk_body[(size_t)c] >= 0 ? body_range[4 * (size_t)chunk_body[(size_t)c]]
+                                                      : chunk_items[(size_t)chunk_start[(size_t)c]];
   m.n_rchunks = (int)rchunk_start.size();
   m.n_rdirect = (int)rdirect_items.size();
   m.n_ritems = (int)(rchunk_items.size() + rdirect_items.size());
@@ -1193,9 +1197,14 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   ALLOC_FILL(m.scan_partials, 0, m.scan_blocks);
   ALLOC_UP(m.chunk_start, chunk_start.data(), chunk_start.size());
   ALLOC_UP(m.chunk_items, chunk_items.data(), chunk_items.size());
+  ALLOC_UP(m.chunk_ref, chunk_ref.data(), chunk_ref.size());
   if (!fbody_range.empty()) {
     ALLOC_UP(m.fbody_range, fbody_range.data(), fbody_range.size());
     if (!frest_items.empty()) ALLOC_UP(m.frest_items, frest_items.data(), frest_items.size());
+    /* no list is built yet */
+    ALLOC_FILL(m.fb_ref, make_float4(NAN, NAN, NAN, NAN), n);
+    if (e == hipSuccess) e = dalloc(h, &m.fb_nlen, (size_t)n);
+    if (e == hipSuccess) e = dalloc(h, &m.fb_nbr, (size_t)FB_NBR_MAX * n);
   }
   if (!body_range.empty()) {
     ALLOC_UP(m.body_range, body_range.data(), body_range.size());
@@ -1440,9 +1449,11 @@ pfg_status pfg_upload_markers(pfg_handle *h, const float *x, const float *y,
   PFG_TRY(upload(h, h->m.x, x, n));
   PFG_TRY(upload(h, h->m.y, y, n));
   PFG_TRY(upload(h, h->m.z, z, n));
-  /* tile claims made from the old positions no longer hold */
+  /* tile claims and neighbour lists made from the old positions no longer
+   * hold (all-ones words are NaN references: the lists are built again) */
   if (h->m.tile_owner)
     PFG_TRY(hipMemset(h->m.tile_owner, 0xff, sizeof(uint64_t) * (size_t)h->ntiles));
+  if (h->m.fb_ref) PFG_TRY(hipMemset(h->m.fb_ref, 0xff, sizeof(float4) * n));
   return PFG_OK;
 }
 

@@ -1339,9 +1339,12 @@ __device__ inline float group_sum(float v) {
 template <int G>
 __device__ inline void element_terms(const MArgs &a, int i, int lane, float xi, float yi,
                                      float zi, int body, float &fx, float &fy, float &fz) {
-  const int t0 = a.m.m2t_start[i], nt = (a.ks != 0.f || a.kv != 0.f) ? a.m.m2t_start[i + 1] - t0 : 0;
-  const int e0 = a.m.m2e_start[i], ne = a.k_spring != 0.f ? a.m.m2e_start[i + 1] - e0 : 0;
-  const int r0 = a.m.m2r_start[i], nr = a.m.m2r_start[i + 1] - r0;
+  /* the lists of terms whose coefficients are zero are not read */
+  const bool area_vol = a.ks != 0.f || a.kv != 0.f, springs = a.k_spring != 0.f;
+  const int t0 = area_vol ? a.m.m2t_start[i] : 0, nt = area_vol ? a.m.m2t_start[i + 1] - t0 : 0;
+  const int e0 = springs ? a.m.m2e_start[i] : 0, ne = springs ? a.m.m2e_start[i + 1] - e0 : 0;
+  const int r0 = a.m.n_rec > 0 ? a.m.m2r_start[i] : 0;
+  const int nr = a.m.n_rec > 0 ? a.m.m2r_start[i + 1] - r0 : 0;
   float g_area = 0.f, cvol = 0.f;
   if (nt > 0) {
     const double Sall = a.m.plt_S[body], S0all = a.m.plt_S0[body];
@@ -1484,39 +1487,68 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
   if constexpr (SPREAD) spread_direct(a, i, lane, fx, fy, fz);
 }
 
-/* Force pass by body: a workgroup of BODY_THREADS lanes per body (a platelet of at
- * most FB_MAX elements, whose Morse partners are its own elements).  Its
- * packed positions go to LDS sorted into a cell list of the body's own box,
- * with cells no smaller than the cutoff (so the 27 cells around an element
- * hold every partner within it; the cell size doubles until the cells fit
- * FB_BINS), each cell's elements in index order.  A lane per element then
- * sums the Morse pairs of those cells from LDS (the pair test and FP32
- * distance of the per-marker form, so the same cutoff decisions), adds the
- * element's other terms and writes its force. */
+/* Force pass by body: a workgroup of BODY_THREADS lanes per body (a
+ * platelet of at most FB_MAX elements, whose Morse partners are its own
+ * elements), a lane per element, with a neighbour list per element kept in
+ * device memory between steps.
+ * - The list of element i holds every element j of the body with
+ *   |x_j - x_i| < R = cutoff + skin at the positions of its last build
+ *   (fb_ref).  It still holds every pair now within the cutoff while no
+ *   element has moved more than skin / 2 away from the body's mean
+ *   displacement since (the distance of a pair changes by at most the
+ *   difference of its two displacements); otherwise the body rebuilds it
+ *   (so does a body whose positions were uploaded or whose list is new:
+ *   fb_ref is NaN then).
+ * - A build sorts the positions into a cell list of the body's own box in
+ *   LDS, cells no smaller than R (the cell size doubles until the cells fit
+ *   FB_BINS), each cell's elements in index order, and lists each element's
+ *   partners in cell order (deterministic).  A list longer than FB_NBR_MAX
+ *   leaves the body without lists: it sums from the cell list this step and
+ *   builds again the next.
+ * - Each element then sums the Morse pairs of its list from the positions
+ *   in LDS (the pair test and FP32 distance of the per-marker form, so the
+ *   same cutoff decisions), adds its other terms and writes its force. */
 __global__ __launch_bounds__(BODY_THREADS) void forces_body_kernel(MArgs a) {
   constexpr int NT = BODY_THREADS, NW = NT / 64, PER = (FB_MAX + NT - 1) / NT;
-  __shared__ float4 sp[FB_MAX];          /* positions in cell order          */
-  __shared__ int16_t sidx[FB_MAX];       /* their index within the body      */
+  __shared__ float4 sp[FB_MAX];          /* positions, index order           */
+  __shared__ int16_t sidx[FB_MAX];       /* elements in cell order           */
   __shared__ int cend[FB_BINS + 1];      /* counts, then ends of the cells   */
   __shared__ float shf[NW][6];
+  __shared__ int sflag[2];               /* [0] rebuild, [1] a list overflowed */
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int g = blockIdx.x;
   const int i0 = a.m.fbody_range[2 * g], cnt = a.m.fbody_range[2 * g + 1] - i0;
+  const int64_t n = a.m.n;
+  const float cut = a.morse_cut, skin = FB_SKIN * a.morse_cut, R = cut + skin;
+  const bool morse = a.morse_u0 != 0.f && cut > 0.f;
   float4 pr[PER];
+  float dsp[3] = {0.f, 0.f, 0.f};
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  bool bad = false; /* a reference position that is not a number */
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int j = t + NT * k;
-    pr[k] = j < cnt ? a.m.pos4[i0 + j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    pr[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (j < cnt) {
+      pr[k] = a.m.pos4[i0 + j];
+      sp[j] = pr[k];
       lo[0] = fminf(lo[0], pr[k].x);
       lo[1] = fminf(lo[1], pr[k].y);
       lo[2] = fminf(lo[2], pr[k].z);
       hi[0] = fmaxf(hi[0], pr[k].x);
       hi[1] = fmaxf(hi[1], pr[k].y);
       hi[2] = fmaxf(hi[2], pr[k].z);
+      if (morse) {
+        const float4 r0 = a.m.fb_ref[i0 + j];
+        dsp[0] += pr[k].x - r0.x;
+        dsp[1] += pr[k].y - r0.y;
+        dsp[2] += pr[k].z - r0.z;
+        bad = bad || !(r0.x == r0.x);
+      }
     }
   }
+  /* the box, and the sum of the displacements (fixed order: lanes, then
+   * wavefronts) */
 #pragma unroll
   for (int s2 = 32; s2 > 0; s2 >>= 1) {
 #pragma unroll
@@ -1533,6 +1565,10 @@ __global__ __launch_bounds__(BODY_THREADS) void forces_body_kernel(MArgs a) {
     }
   }
   for (int b = t; b <= FB_BINS; b += NT) cend[b] = 0;
+  if (t == 0) {
+    sflag[0] = 0;
+    sflag[1] = 0;
+  }
   __syncthreads();
   if (cnt <= 0) return;
 #pragma unroll
@@ -1544,89 +1580,172 @@ __global__ __launch_bounds__(BODY_THREADS) void forces_body_kernel(MArgs a) {
       hi[k] = fmaxf(hi[k], shf[w][3 + k]);
     }
   }
-  /* cells no smaller than the cutoff, doubled until they fit */
-  float cs = fmaxf(a.morse_cut, 1e-3f);
-  int d[3];
-  for (;;) {
-    int64_t nb = 1;
+  __syncthreads(); /* shf is used again */
+  bool rebuild = false;
+  if (morse) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dsp[k] = wave_sum(dsp[k]);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) shf[wave][k] = dsp[k];
+    }
+    if (__any(bad) && lane == 0) sflag[0] = 1;
+    __syncthreads();
+    float mean[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      d[k] = (int)fminf((hi[k] - lo[k]) / cs, 1e6f) + 1;
-      nb *= d[k];
+      float v = 0.f;
+      for (int w = 0; w < NW; ++w) v += shf[w][k];
+      mean[k] = v / (float)cnt;
     }
-    if (nb <= FB_BINS) break;
-    cs *= 2.0f;
-  }
-  const float inv = 1.0f / cs;
-  const int nbins = d[0] * d[1] * d[2];
-  int cell[PER];
+    /* the largest displacement away from the mean, against skin / 2 */
+    bool far = false;
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int bx = min(max((int)((pr[k].x - lo[0]) * inv), 0), d[0] - 1);
-    const int by = min(max((int)((pr[k].y - lo[1]) * inv), 0), d[1] - 1);
-    const int bz = min(max((int)((pr[k].z - lo[2]) * inv), 0), d[2] - 1);
-    cell[k] = bx + d[0] * (by + d[1] * bz);
-    if (t + NT * k < cnt) atomicAdd(&cend[cell[k] + 1], 1);
-  }
-  __syncthreads();
-  if (t < 64) { /* inclusive scan: cend[b + 1] = end of cell b = start of b + 1 */
-    int run = 0;
-    for (int b0 = 1; b0 <= nbins; b0 += 64) {
-      const int b = b0 + lane;
-      const int v = b <= nbins ? cend[b] : 0;
-      const int incl = wave_incl_scan(v);
-      if (b <= nbins) cend[b] = run + incl;
-      run += __shfl(incl, 63, 64);
+    for (int k = 0; k < PER; ++k) {
+      const int j = t + NT * k;
+      if (j >= cnt) continue;
+      const float4 r0 = a.m.fb_ref[i0 + j];
+      const float ex = pr[k].x - r0.x - mean[0], ey = pr[k].y - r0.y - mean[1],
+                  ez = pr[k].z - r0.z - mean[2];
+      /* skin / 2 less 2 %, a margin over FP32 rounding in these distances */
+      far = far || !(ex * ex + ey * ey + ez * ez <= 0.2401f * skin * skin);
     }
+    if (__any(far) && lane == 0) sflag[0] = 1;
+    __syncthreads();
+    rebuild = sflag[0] != 0;
   }
-  __syncthreads();
-  /* place: the fill pointer of cell b runs from its start (cend[b]) to its
-   * end, after which cend[b] is the end of cell b and cell b starts at
-   * cend[b - 1] (0 for the first) */
+  /* a cell list: for a build, or to sum from it when a list overflows */
+  float cs = 0.f, inv = 0.f;
+  int d[3] = {1, 1, 1};
+  if (rebuild) {
+    cs = fmaxf(R, 1e-3f);
+    for (;;) {
+      int64_t nb = 1;
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int j = t + NT * k;
-    if (j < cnt) {
-      const int slot = atomicAdd(&cend[cell[k]], 1);
-      sp[slot] = pr[k];
-      sidx[slot] = (int16_t)j;
-    }
-  }
-  __syncthreads();
-  /* each cell's elements in index order (insertion sort, a lane per cell) */
-  for (int b = t; b < nbins; b += NT) {
-    const int s0 = b == 0 ? 0 : cend[b - 1], s1 = cend[b];
-    for (int q = s0 + 1; q < s1; ++q) {
-      const int16_t key = sidx[q];
-      const float4 kp = sp[q];
-      int r = q - 1;
-      while (r >= s0 && sidx[r] > key) {
-        sidx[r + 1] = sidx[r];
-        sp[r + 1] = sp[r];
-        --r;
+      for (int k = 0; k < 3; ++k) {
+        d[k] = (int)fminf((hi[k] - lo[k]) / cs, 1e6f) + 1;
+        nb *= d[k];
       }
-      sidx[r + 1] = key;
-      sp[r + 1] = kp;
+      if (nb <= FB_BINS) break;
+      cs *= 2.0f;
     }
-  }
-  __syncthreads();
-  for (int q = t; q < cnt; q += NT) {
-    const float4 pi = sp[q];
-    const int i = i0 + sidx[q];
-    const float xi = pi.x, yi = pi.y, zi = pi.z;
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (a.morse_u0 != 0.f) {
-      const int bx = min(max((int)((xi - lo[0]) * inv), 0), d[0] - 1);
-      const int by = min(max((int)((yi - lo[1]) * inv), 0), d[1] - 1);
-      const int bz = min(max((int)((zi - lo[2]) * inv), 0), d[2] - 1);
+    inv = 1.0f / cs;
+    const int nbins = d[0] * d[1] * d[2];
+    int cell[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int bx = min(max((int)((pr[k].x - lo[0]) * inv), 0), d[0] - 1);
+      const int by = min(max((int)((pr[k].y - lo[1]) * inv), 0), d[1] - 1);
+      const int bz = min(max((int)((pr[k].z - lo[2]) * inv), 0), d[2] - 1);
+      cell[k] = bx + d[0] * (by + d[1] * bz);
+      if (t + NT * k < cnt) atomicAdd(&cend[cell[k] + 1], 1);
+    }
+    __syncthreads();
+    if (t < 64) { /* inclusive scan: cend[b + 1] = end of cell b = start of b + 1 */
+      int run = 0;
+      for (int b0 = 1; b0 <= nbins; b0 += 64) {
+        const int b = b0 + lane;
+        const int v = b <= nbins ? cend[b] : 0;
+        const int incl = wave_incl_scan(v);
+        if (b <= nbins) cend[b] = run + incl;
+        run += __shfl(incl, 63, 64);
+      }
+    }
+    __syncthreads();
+    /* place: cell b's fill pointer runs from its start to its end, after
+     * which cend[b] ends cell b and cell b starts at cend[b - 1] */
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int j = t + NT * k;
+      if (j < cnt) sidx[atomicAdd(&cend[cell[k]], 1)] = (int16_t)j;
+    }
+    __syncthreads();
+    for (int b = t; b < nbins; b += NT) { /* each cell in index order */
+      const int s0 = b == 0 ? 0 : cend[b - 1], s1 = cend[b];
+      for (int q = s0 + 1; q < s1; ++q) {
+        const int16_t key = sidx[q];
+        int r = q - 1;
+        while (r >= s0 && sidx[r] > key) {
+          sidx[r + 1] = sidx[r];
+          --r;
+        }
+        sidx[r + 1] = key;
+      }
+    }
+    __syncthreads();
+    /* the lists, and the new reference positions */
+    const float R2 = R * R;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int j = t + NT * k;
+      if (j >= cnt) continue;
+      const float4 p = pr[k];
+      const int bx = min(max((int)((p.x - lo[0]) * inv), 0), d[0] - 1);
+      const int by = min(max((int)((p.y - lo[1]) * inv), 0), d[1] - 1);
+      const int bz = min(max((int)((p.z - lo[2]) * inv), 0), d[2] - 1);
+      int len = 0;
       for (int z = max(bz - 1, 0); z <= min(bz + 1, d[2] - 1); ++z)
         for (int y = max(by - 1, 0); y <= min(by + 1, d[1] - 1); ++y) {
-          /* the three cells along x are one contiguous run of slots */
           const int row = d[0] * (y + d[1] * z);
           const int c0 = row + max(bx - 1, 0), c1 = row + min(bx + 1, d[0] - 1);
           const int s0 = c0 == 0 ? 0 : cend[c0 - 1], s1 = cend[c1];
-          for (int c = s0; c < s1; ++c) morse_pair(a, sp[c], pi.w, xi, yi, zi, fx, fy, fz);
+          for (int c = s0; c < s1; ++c) {
+            const int q = sidx[c];
+            if (q == j) continue;
+            const float4 o = sp[q];
+            const float dx = o.x - p.x, dy = o.y - p.y, dz = o.z - p.z;
+            if (fmaf(dz, dz, fmaf(dy, dy, dx * dx)) >= R2) continue;
+            if (len < FB_NBR_MAX) a.m.fb_nbr[(int64_t)len * n + i0 + j] = (int16_t)q;
+            ++len;
+          }
         }
+      a.m.fb_nlen[i0 + j] = (int16_t)min(len, FB_NBR_MAX);
+      if (len > FB_NBR_MAX) sflag[1] = 1;
+    }
+    __syncthreads();
+    const bool overflow = sflag[1] != 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) { /* an overflowed body builds again next step */
+      const int j = t + NT * k;
+      if (j < cnt)
+        a.m.fb_ref[i0 + j] = overflow ? make_float4(NAN, NAN, NAN, NAN) : pr[k];
+    }
+  }
+  const bool from_cells = rebuild && sflag[1] != 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int j = t + NT * k;
+    if (j >= cnt) continue;
+    const int i = i0 + j;
+    const float4 pi = pr[k];
+    const float xi = pi.x, yi = pi.y, zi = pi.z;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (morse) {
+      if (!from_cells) {
+        /* eight list entries are read at a time, then their pairs summed in
+         * list order */
+        const int len = a.m.fb_nlen[i];
+        for (int q0 = 0; q0 < len; q0 += 8) {
+          int16_t nb[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            nb[u] = q0 + u < len ? a.m.fb_nbr[(int64_t)(q0 + u) * n + i] : (int16_t)-1;
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (nb[u] >= 0) morse_pair(a, sp[nb[u]], pi.w, xi, yi, zi, fx, fy, fz);
+        }
+      } else {
+        const int bx = min(max((int)((xi - lo[0]) * inv), 0), d[0] - 1);
+        const int by = min(max((int)((yi - lo[1]) * inv), 0), d[1] - 1);
+        const int bz = min(max((int)((zi - lo[2]) * inv), 0), d[2] - 1);
+        for (int z = max(bz - 1, 0); z <= min(bz + 1, d[2] - 1); ++z)
+          for (int y = max(by - 1, 0); y <= min(by + 1, d[1] - 1); ++y) {
+            const int row = d[0] * (y + d[1] * z);
+            const int c0 = row + max(bx - 1, 0), c1 = row + min(bx + 1, d[0] - 1);
+            const int s0 = c0 == 0 ? 0 : cend[c0 - 1], s1 = cend[c1];
+            for (int c = s0; c < s1; ++c) morse_pair(a, sp[sidx[c]], pi.w, xi, yi, zi, fx, fy, fz);
+          }
+      }
     }
     element_terms<1>(a, i, 0, xi, yi, zi, a.m.body[i], fx, fy, fz);
     a.m.fx[i] = fx;
@@ -1877,11 +1996,12 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   }
   if (cnt <= 0) return;
   const int *items = a.m.chunk_items + c0;
-  /* body spreading: a chunk of a body writes its box to its scratch box,
-   * its stencils placed in the periodic image nearest to the body's first
-   * marker (the frame of the body's box) */
+  /* body spreading: a chunk of a body writes its box to its scratch box;
+   * a chunk's stencils are placed in the periodic image nearest to its
+   * reference marker (for a chunk of a body the body's first marker, the
+   * frame of the body's box) */
   const int cbody = chunk && a.spread_scratch ? a.m.chunk_body[blockIdx.x] : -1;
-  const int m0 = cbody >= 0 ? a.m.body_range[4 * cbody] : items[0];
+  const int m0 = chunk ? a.m.chunk_ref[blockIdx.x] : items[0];
   const MLat L = lat_of(a, m0); /* a chunk never spans two lattices */
   float ref[3];
   ref[0] = wrap_coord(a.m.x[m0], L.nx, L.periodic[0]);
@@ -2896,6 +3016,7 @@ __device__ inline void interp_staged(const MLat &L, int4 o, const float *w, int 
  * stencils from the field with the staged weights.  Without the Langevin
  * leapfrog and the fibrin kick, the move x += v dt uses the staged
  * position; otherwise update_marker reads what it needs. */
+template <bool FAST>
 __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
   __shared__ int4 so[IB_CHUNK];      /* first stencil node (x = INT_MIN: not coupled);
                                         .w: its box node */
@@ -2920,7 +3041,7 @@ __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
   }
   if (cnt <= 0) return;
   const int *items = a.m.chunk_items + c0;
-  const int m0 = items[0];
+  const int m0 = chunk ? a.m.chunk_ref[blockIdx.x] : items[0];
   const MLat L0 = lat_of(a, m0); /* a chunk never spans two lattices */
   float ref[3];
   ref[0] = wrap_coord(a.m.x[m0], L0.nx, L0.periodic[0]);
@@ -3002,7 +3123,7 @@ __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
     }
     __syncthreads(); /* the box and the box nodes */
   }
-  const bool fast = a.sem_integrator != PFG_SEM_BBK && a.fiber_move != PFG_FIBER_KICK;
+  /* FAST: launched without the Langevin leapfrog and the fibrin kick */
   const int jx = j16 & 3, jy = j16 >> 2;
   const int col = boxed ? jx + (int)dim0 * jy : 0, layer = boxed ? (int)(dim0 * dim1) : 0;
   for (int j = grp; j < cnt; j += 8) {
@@ -3037,7 +3158,7 @@ __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
         interp_staged(chunk ? L0 : lat_of(a, i), so[j], sw[j], j16, v);
       }
     }
-    if (!fast) {
+    if constexpr (!FAST) {
       finish_marker(a, i, j16, kind, v);
     } else if (j16 == 0) { /* finish_marker's stores, update_marker's Euler move */
       if (coupled || kind == PFG_MARKER_FIXED) {
@@ -3358,9 +3479,11 @@ hipError_t launch_move(pfg_handle *h, bool interp, bool update) {
   a.roles = (interp ? ROLE_A : 0) | (update ? ROLE_B : 0);
   if (interp && staged_spreading(h)) {
     /* interpolation staged by chunk, as the spreading is */
-    hipLaunchKernelGGL(move_chunk_kernel,
-                       dim3(h->m.n_chunks + (h->m.n_direct + IB_CHUNK - 1) / IB_CHUNK),
-                       dim3(128), 0, h->stream, a);
+    const dim3 grid(h->m.n_chunks + (h->m.n_direct + IB_CHUNK - 1) / IB_CHUNK);
+    if (h->p.sem_integrator != PFG_SEM_BBK && h->p.fiber_move != PFG_FIBER_KICK)
+      hipLaunchKernelGGL(move_chunk_kernel<true>, grid, dim3(128), 0, h->stream, a);
+    else
+      hipLaunchKernelGGL(move_chunk_kernel<false>, grid, dim3(128), 0, h->stream, a);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(move_kernel, dim3(blocks_for(h->m.n, 16)), dim3(256), 0, h->stream, a);

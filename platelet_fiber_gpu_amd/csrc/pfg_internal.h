@@ -44,6 +44,10 @@ constexpr int BODY_MAX_CHUNKS = 256;
  * Morse pairs (which act within a body only) */
 constexpr int FB_MAX = 1024;
 constexpr int FB_BINS = 2048;
+/* its neighbour lists: at most FB_NBR_MAX partners per element within the
+ * cutoff plus a skin of FB_SKIN cutoffs */
+constexpr int FB_NBR_MAX = 64;
+constexpr float FB_SKIN = 0.25f;
 /* lanes of the per-body workgroups (force pass, second spreading pass) */
 constexpr int BODY_THREADS = 1024;
 /* Marker count from which spreading is staged through LDS by default. */
@@ -218,6 +222,9 @@ struct Markers {
   int n_chunks = 0, n_direct = 0;
   int32_t *chunk_start = nullptr;
   int32_t *chunk_items = nullptr;
+  /* per chunk the marker whose periodic image places its stencils: the
+   * first marker of its platelet (body spreading), else its own first */
+  int32_t *chunk_ref = nullptr;
   /* tile-gather spreading: the tiles this step's stencils meet, listed once
    * each (global tile index); tile_count[0] tiles listed, [1] workgroups of
    * the gather launch done (the last one empties the list).  Usable when
@@ -250,6 +257,12 @@ struct Markers {
    * run of indices, and every other marker (frest_items) */
   int n_fbodies = 0, n_frest = 0;
   int32_t *fbody_range = nullptr, *frest_items = nullptr;
+  /* per marker: the positions of its body's last list build (NaN: build),
+   * its list length and its partners (body-local indices, partner q of
+   * marker i at fb_nbr[q * n + i]) */
+  float4 *fb_ref = nullptr;
+  int16_t *fb_nlen = nullptr;
+  int16_t *fb_nbr = nullptr;
   SpreadRec *spread_rec = nullptr; /* per cell-list slot: the marker's record */
   int tile_gather_ok = 0;
   /* platelet-platelet contact: per platelet pair (lo * n_plt + hi) the best
