@@ -368,7 +368,11 @@ pfg_status pfg_download_membrane(pfg_handle *h, double *S_all, double *V,
 pfg_status pfg_download_bonds(pfg_handle *h, int32_t *rec_bond,
                               int32_t *lig_owner, float *rec_force);
 pfg_status pfg_upload_bonds(pfg_handle *h, const int32_t *rec_bond);
-/* Cell list of the last CELLS stage: bin_start[n_bins+1], bin_items[n_markers]. */
+/* Cell list of the last CELLS stage: bin_start[n_bins+1], bin_items[n_markers].
+ * With the force pass by body on (pfg_set_forces_by_body), every element in
+ * such a body and no contact search or tile-gather spreading, nothing reads
+ * the elements' entries, and the lists hold the other markers only
+ * (bin_start[n_bins] is then their number). */
 pfg_status pfg_cell_grid(const pfg_handle *h, int32_t dims[3]);
 pfg_status pfg_download_cells(pfg_handle *h, int32_t *bin_start,
                               int32_t *bin_items);

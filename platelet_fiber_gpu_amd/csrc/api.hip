@@ -932,6 +932,8 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   }
   m.n_fbodies = (int)(fbody_range.size() / 2);
   m.n_frest = (int)frest_items.size();
+  m.fb_all_elements = m.n_fbodies > 0;
+  for (int32_t i : frest_items) m.fb_all_elements = m.fb_all_elements && M.kind[i] != PFG_MARKER_ELEMENT;
 
   /* ---- host-side CSR construction ---- */
   std::vector<std::pair<int, int>> kv;
@@ -1209,7 +1211,7 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   if (!body_range.empty()) {
     ALLOC_UP(m.body_range, body_range.data(), body_range.size());
     ALLOC_UP(m.chunk_body, chunk_body.data(), chunk_body.size());
-    if (e == hipSuccess) e = dalloc(h, &m.chunk_sum, (size_t)3 * IB_BOX_CELLS * m.n_chunks);
+    if (e == hipSuccess) e = dalloc(h, &m.chunk_sum, (size_t)6 * IB_BOX_CELLS * m.n_chunks);
     if (e == hipSuccess) e = dalloc(h, &m.chunk_info, (size_t)2 * m.n_chunks);
   }
   ALLOC_UP(m.rchunk_start, rchunk_start.data(), rchunk_start.size());
@@ -1648,7 +1650,10 @@ static hipError_t enqueue_step(pfg_handle *h, hipEvent_t lb0, hipEvent_t lb1) {
     if (e == hipSuccess) e = launch_scan(h);
     if (e == hipSuccess) e = launch_place(h, true, true);
     bool resolved = false;
-    if (e == hipSuccess) e = launch_gather(h, true, true, true, &resolved);
+    /* the bin-sorted order serves the Morse gathers, the contact search and
+     * the tile gather, none of which runs when the lists leave the elements
+     * out */
+    if (e == hipSuccess) e = launch_gather(h, bin_elements(h), true, true, &resolved);
     if (e == hipSuccess && !resolved) e = launch_resolve(h);
     const bool fused = forces_can_spread(h);
     if (e == hipSuccess) e = launch_forces(h, fused);

@@ -81,6 +81,9 @@ struct MArgs {
   int mark_rows; /* 1: stamp the x-block rows this step's IB stencils touch */
   int resolve_last; /* 1: the gather launch's last workgroup resolves the bonds */
   int count_spans;  /* 1: the counting launch adds the scan's span totals */
+  int bin_elements; /* 0: the cell lists hold the markers other than elements
+                       (every element's Morse partners come from its
+                       platelet's own lists and nothing else reads them) */
   int spread_scratch; /* 1: the chunks of bodies write scratch boxes (form 4) */
   const int32_t *force_list; /* force_count >= 0: the force pass takes these */
   int force_count;           /* markers only; -1: every marker                */
@@ -240,11 +243,12 @@ __device__ inline int axis_index(int i, int n, int periodic) {
   return (i < 0 || i >= n) ? -1 : i;
 }
 
-/* k / d for 0 <= k < 1024 and 1 <= d <= 1024, by the FP32 reciprocal
- * inv = 1/d: (k + 1/2) / d lies at least 1/(2d) >= 2^-11 from an integer,
- * and the product (below 1025) carries an error below 2^-13, so its floor
- * is the quotient. */
-static_assert(IB_BOX_CELLS <= 1024, "div_small covers box nodes below 1024");
+/* k / d for 0 <= k < 2048 and 1 <= d <= 512 (a box of at most 2048 nodes is
+ * at least 4 nodes along every axis), by the FP32 reciprocal inv = 1/d:
+ * (k + 1/2) / d lies at least 1/(2d) >= 2^-10 from an integer, and the
+ * product (below 2049) carries an error below 2^-12, so its floor is the
+ * quotient. */
+static_assert(IB_BOX_CELLS <= 1024, "div_small covers boxes of at most 2048 nodes");
 __device__ inline int div_small(int k, float inv) { return (int)(((float)k + 0.5f) * inv); }
 
 /* The 4-point cosine kernel phi(r) = (1 + cos(pi r / 2)) / 4 at the four
@@ -383,13 +387,16 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
   }
   if (i >= a.m.n) return;
   const MLat L = lat_of(a, i);
-  const float4 p = make_float4(a.m.x[i], a.m.y[i], a.m.z[i], morse_tag(a, i));
+  const int kind_i = a.m.kind[i];
+  const float4 p = make_float4(a.m.x[i], a.m.y[i], a.m.z[i],
+                               __int_as_float(kind_i == PFG_MARKER_ELEMENT ? a.m.body[i] : -1));
   a.m.pos4[i] = p;
+  const bool binned = a.bin_elements || kind_i != PFG_MARKER_ELEMENT;
   int b3[3];
   bin3_of(L, p.x, p.y, p.z, b3);
   const int b = L.bin0 + b3[0] + L.nb[0] * (b3[1] + L.nb[1] * b3[2]);
-  a.m.marker_bin[i] = b;
-  a.m.marker_slot[i] = atomicAdd(&a.m.bin_count[b], 1);
+  a.m.marker_bin[i] = binned ? b : -1;
+  if (binned) a.m.marker_slot[i] = atomicAdd(&a.m.bin_count[b], 1);
   if (a.count_spans) {
     /* the scan's per-span totals, one atomic per span a wavefront touches
      * (its markers are consecutive, so mostly one); the scan then needs no
@@ -397,7 +404,7 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
      * With many markers the waves meet on few spans and these atomics
      * serialise: the reduction launch is cheaper then. */
     const int span = b / SCAN_SPAN, lane = threadIdx.x & 63;
-    uint64_t todo = __ballot(1);
+    uint64_t todo = __ballot(binned);
     while (todo) {
       const int leader = __ffsll((unsigned long long)todo) - 1;
       const int ls = __shfl(span, leader, 64);
@@ -536,6 +543,8 @@ __device__ void triangle_geometry(const MArgs &a, int t) {
    * local area strain; the volume gradient at every vertex of the triangle
    * is (x1 - x0) x (x2 - x0) (the reference's expansion at
    * sem_kernel.cu:1723-1731 reduces to it) */
+  /* the gradients below feed the area and volume forces only */
+  if (a.ks == 0.f && a.kv == 0.f) return;
   const double lf = (St - a.m.tri_S0[t]) / a.m.tri_S0[t];
   const double L[3][3] = {{x1 - x2, y1 - y2, z1 - z2},
                           {x2 - x0, y2 - y0, z2 - z0},
@@ -788,7 +797,7 @@ __global__ __launch_bounds__(256) void place_kernel(MArgs a) {
       const int i = blk * 256 + threadIdx.x;
       if (i < a.m.n) {
         const int b = a.m.marker_bin[i];
-        a.m.bin_scatter[a.m.bin_start[b] + a.m.marker_slot[i]] = i;
+        if (b >= 0) a.m.bin_scatter[a.m.bin_start[b] + a.m.marker_slot[i]] = i;
       }
       /* the scan has read the span totals: clear them for the next count */
       for (int64_t k = i; k < a.m.scan_blocks; k += (int64_t)a.blocks0 * 256)
@@ -1124,8 +1133,10 @@ __device__ void gather_body(const MArgs &a) {
     if (blk < a.blocks0) {
       const int i = blk * 256 + threadIdx.x;
       if (i < a.m.n) {
-        /* rank of i among its bin's markers by index */
+        /* rank of i among its bin's markers by index (an element left out
+         * of the lists has none) */
         const int b = a.m.marker_bin[i];
+        if (b < 0) return;
         const int s = a.m.bin_start[b], e = a.m.bin_start[b + 1];
         int k = s;
         for (int q = s; q < e; ++q) k += a.m.bin_scatter[q] < i;
@@ -1923,6 +1934,7 @@ __device__ void spread_direct(const MArgs &a, int i, int lane, float Fx, float F
   const int iz = stencil_node(Z, lane >> 4, L.nz, L.periodic[2], &wz);
   if (ix < 0 || iy < 0 || iz < 0) return;
   stamp_stencil_tile(L, lane & 3, (lane >> 2) & 3, lane >> 4, ix, iy, iz, (int32_t)*a.m.d_step);
+  if (Fx == 0.f && Fy == 0.f && Fz == 0.f) return; /* adds nothing; the tiles are stamped */
   const float w = wx * wy * wz * a.force_scale;
   const int64_t c = ix + (int64_t)L.nx * (iy + (int64_t)L.ny * iz);
   atomicAdd(L.aux + c, w * Fx);
@@ -1968,7 +1980,9 @@ __device__ inline void stencil_origin(const MLat &L, const float v[3], const flo
  * the read-add-write of each node needs no atomic and the sums follow
  * marker order whatever the timing.  The two boxes are added (first half,
  * then second) and go to the lattice field with one device-wide atomic per
- * nonzero node and component.  A scattered chunk (fibrin nodes far apart)
+ * nonzero node and component.  A box of up to twice IB_BOX_CELLS nodes (a
+ * chunk that rotated or deformed since load) is summed by wavefront 0
+ * alone, marker after marker, in the two boxes' memory taken as one.  A scattered chunk (fibrin nodes far apart)
  * adds each marker's 64 nodes with device-wide atomics, as the direct form
  * does; so do the markers listed for direct spreading, IB_CHUNK per
  * workgroup after the chunks, staged the same way (each with its own
@@ -1979,7 +1993,7 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   __shared__ int4 so[IB_CHUNK];      /* first stencil node; x = INT_MIN: not coupled */
   __shared__ float sw[IB_CHUNK][12]; /* weights along x (0..3), y (4..7), z (8..11) */
   __shared__ float sf[IB_CHUNK][3];
-  __shared__ float box[2][3 * IB_BOX_CELLS];
+  __shared__ float box[6 * IB_BOX_CELLS]; /* a box per wavefront, or one wide box */
   __shared__ int sh[2][6];
   __shared__ int sidx[IB_CHUNK];
   if (halted(a)) return;
@@ -2068,7 +2082,7 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
     /* the scratch box written below, or none (extent 0): no coupled
      * marker, or a box too large, spread with atomics (the body's nodes then
      * all go with atomics) */
-    const bool boxed = lo[0] <= hi[0] && vol64 <= IB_BOX_CELLS;
+    const bool boxed = lo[0] <= hi[0] && vol64 <= 2 * IB_BOX_CELLS;
     a.m.chunk_info[2 * blockIdx.x] = make_int4(lo[0], lo[1], lo[2], 0);
     a.m.chunk_info[2 * blockIdx.x + 1] =
         boxed ? make_int4(dim0, dim1, dim2, (int)vol64) : make_int4(0, 0, 0, 0);
@@ -2076,7 +2090,11 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   }
   if (lo[0] > hi[0]) return; /* no IB-coupled marker in the chunk */
   const int kx = lane & 3, ky = (lane >> 2) & 3, kz = lane >> 4;
-  if (!chunk || vol64 > IB_BOX_CELLS) { /* scattered markers: device-wide atomics per node */
+  /* a box of IB_BOX_CELLS to 2 IB_BOX_CELLS nodes (a chunk that rotated or
+   * deformed since load) is summed by wavefront 0 alone in the two
+   * wavefronts' boxes taken as one */
+  const bool wide = vol64 > IB_BOX_CELLS;
+  if (!chunk || vol64 > 2 * IB_BOX_CELLS) { /* scattered markers: device-wide atomics per node */
     for (int j = wave; j < cnt; j += 2) {
       if (so[j].x == INT_MIN) continue;
       const MLat Lj = chunk ? L : lat_of(a, sidx[j]);
@@ -2085,6 +2103,9 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
       const int iz = axis_index(so[j].z + kz, Lj.nz, Lj.periodic[2]);
       if (ix < 0 || iy < 0 || iz < 0) continue;
       stamp_stencil_tile(Lj, kx, ky, kz, ix, iy, iz, step);
+      /* a marker without force adds nothing (its tiles are stamped still:
+       * the interpolation reads the velocity there) */
+      if (sf[j][0] == 0.f && sf[j][1] == 0.f && sf[j][2] == 0.f) continue;
       const float w = sw[j][kx] * sw[j][4 + ky] * sw[j][8 + kz] * a.force_scale;
       const int64_t c = ix + (int64_t)Lj.nx * (iy + (int64_t)Lj.ny * iz);
       atomicAdd(Lj.aux + c, w * sf[j][0]);
@@ -2095,8 +2116,8 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   }
   const int vol = (int)vol64;
   for (int k = t; k < 3 * vol; k += 128) {
-    box[0][k] = 0.f;
-    box[1][k] = 0.f;
+    box[k] = 0.f;
+    if (!wide) box[3 * IB_BOX_CELLS + k] = 0.f;
   }
   /* each marker's first box node (-1: not coupled), and each lane's offset
    * of its stencil node from it */
@@ -2106,10 +2127,10 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   }
   const int off = kx + dim0 * (ky + dim1 * kz);
   __syncthreads();
-  float *mybox = box[wave];
-  const int j1 = min(cnt, 64 * (wave + 1));
+  float *mybox = box + (wide ? 0 : wave * 3 * IB_BOX_CELLS);
+  const int j0 = wide ? 0 : 64 * wave, j1 = wide ? (wave == 0 ? cnt : 0) : min(cnt, 64 * (wave + 1));
 #pragma unroll 4
-  for (int j = 64 * wave; j < j1; ++j) {
+  for (int j = j0; j < j1; ++j) {
     const int base = so[j].w;
     if (base < 0) continue;
     const float w = sw[j][kx] * sw[j][4 + ky] * sw[j][8 + kz] * a.force_scale;
@@ -2120,11 +2141,11 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   }
   __syncthreads();
   if (cbody >= 0) { /* the chunk's sums to its scratch box, zeros included */
-    float *out = a.m.chunk_sum + (int64_t)blockIdx.x * (3 * IB_BOX_CELLS);
-    for (int k = t; k < 3 * vol; k += 128) out[k] = box[0][k] + box[1][k];
+    float *out = a.m.chunk_sum + (int64_t)blockIdx.x * (6 * IB_BOX_CELLS);
+    for (int k = t; k < 3 * vol; k += 128) out[k] = wide ? box[k] : box[k] + box[3 * IB_BOX_CELLS + k];
     return;
   }
-  const float inv0 = 1.0f / (float)dim0, inv1 = 1.0f / (float)dim1; /* vol <= IB_BOX_CELLS */
+  const float inv0 = 1.0f / (float)dim0, inv1 = 1.0f / (float)dim1; /* vol <= 2 IB_BOX_CELLS */
   /* Every tile the box meets is stamped (a superset of the stencils' tiles:
    * interpolation reads the velocity of all their nodes).  Along each axis
    * the box nodes inside one tile form runs that start at the box's first
@@ -2142,8 +2163,10 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
         (bz == 0 || (gz & (TILE_Z - 1)) == 0)) /* few per chunk: stored without a read */
       L.tile_stamp[(gx >> TILE_SHIFT_X) + L.tnx * ((gy >> TILE_SHIFT_Y) + L.tny * (gz >> TILE_SHIFT_Z))] = step;
     const int64_t c = gx + (int64_t)L.nx * (gy + (int64_t)L.ny * gz);
-    const float fx = box[0][k] + box[1][k], fy = box[0][vol + k] + box[1][vol + k],
-                fz = box[0][2 * vol + k] + box[1][2 * vol + k];
+    const float *b1 = box + 3 * IB_BOX_CELLS;
+    const float fx = wide ? box[k] : box[k] + b1[k];
+    const float fy = wide ? box[vol + k] : box[vol + k] + b1[vol + k];
+    const float fz = wide ? box[2 * vol + k] : box[2 * vol + k] + b1[2 * vol + k];
     if (fx != 0.f) atomicAdd(L.aux + c, fx);
     if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
     if (fz != 0.f) atomicAdd(L.aux + 2 * a.plane + c, fz);
@@ -2438,7 +2461,7 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
   if (!fits) { /* chunk by chunk, every nonzero node with atomics */
     for (int q = 0; q < nc; ++q) {
       const int4 l = cinfo[q][0], d = cinfo[q][1];
-      const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (3 * IB_BOX_CELLS);
+      const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (6 * IB_BOX_CELLS);
       for (int k = t; k < d.w; k += NT) {
         const int r = k / d.x, bx = k - r * d.x;
         const int bz = r / d.y, by = r - bz * d.y;
@@ -2494,7 +2517,7 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
         if (d.w == 0 || (unsigned)px >= (unsigned)d.x || (unsigned)py >= (unsigned)d.y ||
             (unsigned)pz >= (unsigned)d.z)
           continue;
-        const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (3 * IB_BOX_CELLS);
+        const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (6 * IB_BOX_CELLS);
         const int p = px + d.x * (py + d.y * pz);
         vx[u] = src[p];
         vy[u] = src[d.w + p];
@@ -3264,6 +3287,7 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->resolve_last = 0;
   a->count_spans = 0;
   a->spread_scratch = 0;
+  a->bin_elements = bin_elements(h) ? 1 : 0;
   a->force_list = nullptr;
   a->force_count = -1;
   a->blocks0 = a->blocks1 = a->blocks2 = a->blocks3 = 0;
@@ -3377,6 +3401,16 @@ static bool forces_by_body(const pfg_handle *h) {
 
 static bool contact_on(const pfg_handle *h) {
   return h->m.contact_slot && h->p.contact_a != 0.f && h->m.n_plt >= 2;
+}
+
+/* Whether the cell lists hold the elements: not when the step's force pass
+ * takes every element's Morse partners from its platelet's own lists (the
+ * pass by body, which the step runs unless the force pass spreads too) and
+ * no contact search or tile gather reads the lists (then only the fibrin
+ * ligand search does). */
+bool bin_elements(const pfg_handle *h) {
+  const bool body_pass = forces_by_body(h) && !forces_can_spread(h);
+  return !(body_pass && h->m.fb_all_elements && !contact_on(h) && spread_form(h) != 3);
 }
 
 /* The force pass can spread each marker's force itself when it has a

@@ -234,7 +234,7 @@ struct Markers {
   /* body spreading: bodies (compact platelets: body_range[4g .. 4g+3] =
    * their markers m0..m1-1 and chunks c0..c1-1; every marker of a body is
    * in one of its chunks), chunk_body (the body of each chunk, -1: none),
-   * the chunks' scratch boxes (chunk_sum: 3 x IB_BOX_CELLS floats per chunk;
+   * the chunks' scratch boxes (chunk_sum: 6 x IB_BOX_CELLS floats per chunk;
    * chunk_info: the box's first node and extents per chunk, extent 0: not
    * written this step), and the chunks and direct list of every marker
    * outside the bodies (the chunk lists' format); per tile of all
@@ -256,6 +256,7 @@ struct Markers {
    * fbody_range[2g], fbody_range[2g+1]) when every body's elements form one
    * run of indices, and every other marker (frest_items) */
   int n_fbodies = 0, n_frest = 0;
+  int fb_all_elements = 0;       /* every element is in one of those bodies  */
   int32_t *fbody_range = nullptr, *frest_items = nullptr;
   /* per marker: the positions of its body's last list build (NaN: build),
    * its list length and its partners (body-local indices, partner q of
@@ -406,6 +407,9 @@ hipError_t launch_gather(pfg_handle *h, bool sort, bool sums, bool decide,
 hipError_t launch_resolve(pfg_handle *h);
 /* spread = true: the force pass also spreads (only when forces_can_spread) */
 hipError_t launch_forces(pfg_handle *h, bool spread = false);
+/* whether the step's cell lists hold the elements (else only the other
+ * markers, and the step needs no bin-sorted order) */
+bool bin_elements(const pfg_handle *h);
 bool forces_can_spread(const pfg_handle *h);
 hipError_t launch_spread(pfg_handle *h);
 hipError_t launch_move(pfg_handle *h, bool interp, bool update);

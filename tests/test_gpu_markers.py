@@ -353,6 +353,67 @@ def test_contact_dense_cluster_parity(gpu, po):
     assert np.abs(F - oF).max() < 1e-5 * np.abs(oF).max()
 
 
+def test_spreading_forms_agree_on_rotated_platelets(gpu):
+    """The 512x256x256 tube's 500 platelets each turned by 45 degrees about x
+    around their centroids, so that the axis-aligned boxes of their chunks
+    (cut at load from the flat orientation) grow past the LDS box: chunks up
+    to twice its nodes are summed by one wavefront in both boxes' memory,
+    larger ones spread with atomics.  One spreading stage of the same forces
+    with the staged chunks (form 1) and by platelet (form 4, with the force
+    launch's tile claims) against the direct form (2): the same field up to
+    the FP32 order of the sums."""
+    pfg = gpu
+    sc = pfg.Scene(pfg.SCENE_THROMBUS)
+    x, y, z = sc.x.copy(), sc.y.copy(), sc.z.copy()
+    c, sn = np.cos(np.pi / 4), np.sin(np.pi / 4)
+    st = sc.plt_elem_start
+    for p in range(len(st) - 1):
+        a, b = st[p], st[p + 1]
+        cy, cz = y[a:b].mean(), z[a:b].mean()
+        dy, dz = y[a:b] - cy, z[a:b] - cz
+        y[a:b] = (cy + c * dy - sn * dz).astype(np.float32)
+        z[a:b] = (cz + sn * dy + c * dz).astype(np.float32)
+    rng = np.random.default_rng(17)
+    f = (1e-3 * rng.standard_normal((3, sc.n_markers))).astype(np.float32)
+    fields = {}
+    for form in (2, 1, 4):
+        with pfg.Solver(sc.params) as s:
+            s.set_ib_staging(form)
+            s.load(sc)
+            s.set_markers(x, y, z)
+            s.stage(pfg.STAGE_FORCES)  # the tile claims of these positions
+            s.set_marker_forces(*f)
+            s.stage(pfg.STAGE_SPREAD)
+            fields[form] = s.aux()
+    scale = np.abs(fields[2]).max()
+    assert scale > 0
+    for form in (1, 4):
+        err = float(np.abs(fields[form] - fields[2]).max())
+        print(f"form {form}: max difference {err:.2e} of {scale:.2e}")
+        assert err < 1e-5 * scale
+
+
+def test_force_pass_by_body_request_with_spreading_force_pass(gpu):
+    """The force pass by body requested where the step's force pass spreads
+    each marker's force itself (direct spreading, a wavefront per marker):
+    that pass sums the Morse pairs from the cell lists, which must then hold
+    the elements; the steps follow the same run without the request."""
+    pfg = gpu
+    sc = scene_small(pfg, receptors=-1, ligands=-1)
+    sc.params.morse_u0 = 1.0
+    runs = []
+    for mode in (2, 1):
+        with pfg.Solver(sc.params) as s:
+            s.set_ib_staging(2)
+            s.set_forces_lanes(64)
+            s.set_forces_by_body(mode)
+            s.load(sc)
+            s.step(10)
+            mk = s.markers()
+            runs.append(np.stack([mk[k] for k in ("x", "y", "z")]))
+    assert np.abs(runs[0] - runs[1]).max() <= 2 * np.spacing(np.float32(sc.params.nx))
+
+
 @pytest.mark.parametrize("name", ["thrombus_small", "tube_full"])
 def test_force_pass_by_body_follows_the_per_marker_pass(gpu, name):
     """Coupled steps with the force pass by platelet (Morse pairs from each
