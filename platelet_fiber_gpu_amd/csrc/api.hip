@@ -887,6 +887,7 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   }
   m.n_chunks = (int)chunk_start.size();
   m.n_direct = (int)direct_items.size();
+  m.n_chunk_items = (int)chunk_items.size();
   chunk_start.push_back((int32_t)chunk_items.size());
   chunk_items.insert(chunk_items.end(), direct_items.begin(), direct_items.end());
   m.n_bodies = (int)(body_range.size() / 4);
@@ -1056,7 +1057,7 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
   }
   /* cell-list geometry: bins of bin_cells cells per lattice, the lattices'
    * bins one after the other */
-  int64_t nbins = 0;
+  int64_t nbins = 0, nbinsc = 0;
   for (Inst &I : h->inst) {
     I.nb[0] = std::max(1, I.p.nx / p.bin_cells);
     I.nb[1] = std::max(1, I.p.ny / p.bin_cells);
@@ -1064,11 +1065,17 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
     I.nbins = (int64_t)I.nb[0] * I.nb[1] * I.nb[2];
     I.bin0 = nbins;
     nbins += I.nbins;
+    for (int k = 0; k < 3; ++k) I.nbc[k] = std::max(1, I.nb[k] / COARSE_BINS);
+    I.nbinsc = (int64_t)I.nbc[0] * I.nbc[1] * I.nbc[2];
+    I.binc0 = nbinsc;
+    nbinsc += I.nbinsc;
   }
   if (nbins > (int64_t)INT_MAX / 2) return PFG_ERR_ARG;
   for (int k = 0; k < 3; ++k) m.nb[k] = h->inst[0].nb[k];
   m.nbins = nbins;
   m.scan_blocks = (m.nbins + SCAN_SPAN - 1) / SCAN_SPAN;
+  m.nbinsc = nbinsc; /* <= nbins: the arrays of the full grid hold it */
+  m.scan_blocks_c = (m.nbinsc + SCAN_SPAN - 1) / SCAN_SPAN;
   /* static ligands on the same bin grid, ascending ligand id per bin */
   std::vector<int32_t> lg_start, lg_items;
   std::vector<float> lig_z_host((size_t)M.n_wall_ligands, p.wall_z);
@@ -1569,7 +1576,9 @@ pfg_status pfg_upload_bonds(pfg_handle *h, const int32_t *rec_bond) {
 pfg_status pfg_cell_grid(const pfg_handle *h, int32_t dims[3]) {
   if (!h || !dims) return PFG_ERR_ARG;
   if (!h->model_loaded) return PFG_ERR_STATE;
-  for (int k = 0; k < 3; ++k) dims[k] = h->m.nb[k];
+  /* the coarse grid when the lists leave the elements out */
+  const bool full = bin_elements(h);
+  for (int k = 0; k < 3; ++k) dims[k] = full ? h->m.nb[k] : h->inst[0].nbc[k];
   return PFG_OK;
 }
 
@@ -1577,7 +1586,8 @@ pfg_status pfg_download_cells(pfg_handle *h, int32_t *bin_start, int32_t *bin_it
   if (!h) return PFG_ERR_ARG;
   if (!h->model_loaded) return PFG_ERR_STATE;
   hipSetDevice(h->device);
-  PFG_TRY(download(h, bin_start, h->m.bin_start, (size_t)h->m.nbins + 1));
+  const int64_t nb = bin_elements(h) ? h->m.nbins : h->m.nbinsc;
+  PFG_TRY(download(h, bin_start, h->m.bin_start, (size_t)nb + 1));
   PFG_TRY(download(h, bin_items, h->m.bin_items, (size_t)h->m.n));
   return PFG_OK;
 }

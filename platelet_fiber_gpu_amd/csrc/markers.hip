@@ -87,6 +87,8 @@ struct MArgs {
   int spread_scratch; /* 1: the chunks of bodies write scratch boxes (form 4) */
   const int32_t *force_list; /* force_count >= 0: the force pass takes these */
   int force_count;           /* markers only; -1: every marker                */
+  const int32_t *move_list;  /* move_count >= 0: move_kernel takes these       */
+  int move_count;            /* markers only (and leaves the step counter)    */
   int blocks0, blocks1, blocks2, blocks3; /* blocks of role ranges 0..3 */
 };
 
@@ -131,6 +133,16 @@ __device__ inline void bin3_of(const MLat &L, float x, float y, float z, int b[3
   b[0] = cell_of(x, L.nx, L.periodic[0]) * L.nb[0] / L.nx;
   b[1] = cell_of(y, L.ny, L.periodic[1]) * L.nb[1] / L.ny;
   b[2] = cell_of(z, L.nz, L.periodic[2]) * L.nb[2] / L.nz;
+}
+
+/* The lattice with the coarse cell-list grid in place of the full one (the
+ * lists of the markers other than elements). */
+__device__ inline MLat coarse_grid(const MLat &L) {
+  MLat c = L;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c.nb[k] = L.nbc[k];
+  c.bin0 = L.binc0;
+  return c;
 }
 
 __device__ inline float min_image(float d, int n, int periodic) {
@@ -392,9 +404,10 @@ __global__ __launch_bounds__(256) void begin_kernel(MArgs a) {
                                __int_as_float(kind_i == PFG_MARKER_ELEMENT ? a.m.body[i] : -1));
   a.m.pos4[i] = p;
   const bool binned = a.bin_elements || kind_i != PFG_MARKER_ELEMENT;
+  const MLat G = a.bin_elements ? L : coarse_grid(L); /* the lists' grid */
   int b3[3];
-  bin3_of(L, p.x, p.y, p.z, b3);
-  const int b = L.bin0 + b3[0] + L.nb[0] * (b3[1] + L.nb[1] * b3[2]);
+  bin3_of(G, p.x, p.y, p.z, b3);
+  const int b = G.bin0 + b3[0] + G.nb[0] * (b3[1] + G.nb[1] * b3[2]);
   a.m.marker_bin[i] = binned ? b : -1;
   if (binned) a.m.marker_slot[i] = atomicAdd(&a.m.bin_count[b], 1);
   if (a.count_spans) {
@@ -1075,9 +1088,13 @@ __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
   }
   if (a.m.fibrin_lig) {
     /* markers other than elements are ligands (index n_wall_lig + marker);
-     * the scatter order is final before this launch and order-free here */
+     * the scatter order is final before this launch and order-free here;
+     * the lists' grid is the coarse one when they leave the elements out */
+    const MLat G = a.bin_elements ? L : coarse_grid(L);
+    int g3[3];
+    bin3_of(G, rp[0], rp[1], rp[2], g3);
     int s = 0, e = 0;
-    const int nr = nbr_range(L, a.m.bin_start, b3, lane, &s, &e);
+    const int nr = nbr_range(G, a.m.bin_start, g3, lane, &s, &e);
     const int total = flatten_ranges(nr, s, e, off, ends);
     int q = 0;
     for (int c = lane; c < total; c += 64) {
@@ -2978,11 +2995,15 @@ __device__ inline void finish_marker(const MArgs &a, int i, int lane, int kind, 
 /* roles: A = interpolate V at the markers, B = move the markers and advance
  * the step counter.  Four markers per wavefront, 16 lanes each. */
 __global__ __launch_bounds__(256) void move_kernel(MArgs a) {
-  const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int k = (blockIdx.x * 256 + threadIdx.x) >> 4;
   const int lane = threadIdx.x & 15;
   if (halted(a)) return;
-  if ((a.roles & ROLE_B) && blockIdx.x == 0 && threadIdx.x == 0) *a.m.d_step += 1;
-  if (i >= a.m.n) return;
+  /* every marker, advancing the step counter, or the markers of a list
+   * (the chunk launch of the same step advances it) */
+  if ((a.roles & ROLE_B) && a.move_count < 0 && blockIdx.x == 0 && threadIdx.x == 0)
+    *a.m.d_step += 1;
+  if (k >= (a.move_count < 0 ? a.m.n : a.move_count)) return;
+  const int i = a.move_count < 0 ? k : a.move_list[k];
   const int kind = a.m.kind[i];
   float v[3] = {0.f, 0.f, 0.f};
   if ((a.roles & ROLE_A) && ib_coupled(kind, a.fiber_move)) interp_marker(a, i, lane, v);
@@ -3220,8 +3241,10 @@ void fill_mlat(const pfg_handle *h, int k, MLat *L) {
   for (int ax = 0; ax < 3; ++ax) {
     L->periodic[ax] = I.p.wall[ax] == PFG_WALL_PERIODIC;
     L->nb[ax] = I.nb[ax];
+    L->nbc[ax] = I.nbc[ax];
   }
   L->bin0 = (int)I.bin0;
+  L->binc0 = (int)I.binc0;
   L->tile0 = I.tile0;
   L->nxb = I.nxb;
   L->row_stamp = h->row_stamp ? h->row_stamp + I.fblk0 : nullptr;
@@ -3290,6 +3313,8 @@ static void make_margs(const pfg_handle *h, MArgs *a) {
   a->bin_elements = bin_elements(h) ? 1 : 0;
   a->force_list = nullptr;
   a->force_count = -1;
+  a->move_list = nullptr;
+  a->move_count = -1;
   a->blocks0 = a->blocks1 = a->blocks2 = a->blocks3 = 0;
 }
 
@@ -3319,11 +3344,13 @@ hipError_t launch_begin(pfg_handle *h, bool clear, bool count) {
 hipError_t launch_scan(pfg_handle *h) {
   const Markers &m = h->m;
   if (m.n == 0) return hipSuccess;
+  const bool full = bin_elements(h); /* else the coarse grid */
+  const int64_t blocks = full ? m.scan_blocks : m.scan_blocks_c;
   if (!count_spans(h))
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
-                       h->stream, m.bin_count, m.scan_partials);
-  hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)m.scan_blocks), dim3(256), 0,
-                     h->stream, m.bin_count, m.nbins, m.scan_partials, m.bin_start);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, h->stream,
+                       m.bin_count, m.scan_partials);
+  hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)blocks), dim3(256), 0, h->stream,
+                     m.bin_count, full ? m.nbins : m.nbinsc, m.scan_partials, m.bin_start);
   return hipGetLastError();
 }
 
@@ -3511,13 +3538,22 @@ hipError_t launch_move(pfg_handle *h, bool interp, bool update) {
   MArgs a;
   make_margs(h, &a);
   a.roles = (interp ? ROLE_A : 0) | (update ? ROLE_B : 0);
-  if (interp && staged_spreading(h)) {
-    /* interpolation staged by chunk, as the spreading is */
-    const dim3 grid(h->m.n_chunks + (h->m.n_direct + IB_CHUNK - 1) / IB_CHUNK);
+  if (interp && staged_spreading(h) && h->m.n_chunks > 0) {
+    /* interpolation staged by chunk, as the spreading is (this launch
+     * advances the step counter); the markers listed for direct spreading,
+     * scattered, take a group of lanes each in a launch of their own, where
+     * the many groups hide their loads' latency */
+    const dim3 grid(h->m.n_chunks);
     if (h->p.sem_integrator != PFG_SEM_BBK && h->p.fiber_move != PFG_FIBER_KICK)
       hipLaunchKernelGGL(move_chunk_kernel<true>, grid, dim3(128), 0, h->stream, a);
     else
       hipLaunchKernelGGL(move_chunk_kernel<false>, grid, dim3(128), 0, h->stream, a);
+    if (h->m.n_direct > 0) {
+      a.move_list = h->m.chunk_items + h->m.n_chunk_items;
+      a.move_count = h->m.n_direct;
+      hipLaunchKernelGGL(move_kernel, dim3(blocks_for(h->m.n_direct, 16)), dim3(256), 0,
+                         h->stream, a);
+    }
     return hipGetLastError();
   }
   hipLaunchKernelGGL(move_kernel, dim3(blocks_for(h->m.n, 16)), dim3(256), 0, h->stream, a);

@@ -24,6 +24,10 @@ constexpr int GRAPH_CHUNK = 8;
  * in LDS; when their stencils span at most IB_BOX_CELLS lattice nodes the
  * chunk is summed node by node. */
 constexpr int SCAN_SPAN = 8192; /* bins per block of the bin-count scan */
+/* When the cell lists leave the elements out (only the fibrin ligand search
+ * reads them), their bins are COARSE_BINS bins of the full grid along each
+ * axis: a scan over 64 times fewer bins for a few more candidates. */
+constexpr int COARSE_BINS = 4;
 constexpr int IB_CHUNK = 128;
 /* workgroups of the tile-gather spreading launch (each takes listed tiles in
  * turn) */
@@ -122,6 +126,8 @@ struct MLat {
   int periodic[3];
   int nb[3];              /* cell-list bins per axis                          */
   int bin0;               /* first global bin of the lattice                  */
+  int nbc[3];             /* coarse bins per axis (lists without elements)    */
+  int binc0;              /* first global coarse bin of the lattice           */
   int64_t tile0;          /* first global tile of the lattice                 */
   int nxb;                /* x-blocks per row                                 */
   int32_t *row_stamp;     /* the lattice's x-block rows (nullptr: unused)     */
@@ -208,6 +214,7 @@ struct Markers {
   /* cell list */
   int nb[3] = {1, 1, 1};
   int64_t nbins = 0;
+  int64_t nbinsc = 0, scan_blocks_c = 0; /* the coarse grid                  */
   int32_t *bin_count = nullptr, *bin_start = nullptr, *bin_items = nullptr,
           *marker_bin = nullptr, *marker_slot = nullptr;
   int32_t *bin_scatter = nullptr; /* items in scatter (atomic slot) order    */
@@ -220,6 +227,7 @@ struct Markers {
    * at load time; the n_direct markers chunk_items[chunk_start[n_chunks] ..]
    * were too far apart and are spread directly */
   int n_chunks = 0, n_direct = 0;
+  int n_chunk_items = 0;         /* markers in the chunks (the direct list follows) */
   int32_t *chunk_start = nullptr;
   int32_t *chunk_items = nullptr;
   /* per chunk the marker whose periodic image places its stencils: the
@@ -293,6 +301,8 @@ struct Inst {
   int64_t ntiles = 0, tile0 = 0;
   int nb[3] = {1, 1, 1};
   int64_t nbins = 0, bin0 = 0;
+  int nbc[3] = {1, 1, 1};
+  int64_t nbinsc = 0, binc0 = 0;
   int64_t lb_blocks = 0, lb_block0 = 0; /* workgroups in the batched LB launch */
   int32_t rec0 = 0, marker0 = 0;  /* first receptor and marker of the lattice */
 };
