@@ -108,6 +108,7 @@ void free_markers(Markers &m) {
   dfree(m.body_range); dfree(m.rchunk_start); dfree(m.rchunk_items); dfree(m.tile_owner);
   dfree(m.body_spread); dfree(m.body_fallback); dfree(m.chunk_body); dfree(m.chunk_sum);
   dfree(m.fbody_range); dfree(m.frest_items); dfree(m.fb_ref); dfree(m.fb_nlen); dfree(m.fb_nbr);
+  dfree(m.fb_done);
   dfree(m.chunk_info);
   Markers fresh;
   fresh.counters = m.counters;
@@ -1214,6 +1215,7 @@ pfg_status load_impl(pfg_handle *h, const pfg_model &M, const int32_t *minst,
     ALLOC_FILL(m.fb_ref, make_float4(NAN, NAN, NAN, NAN), n);
     if (e == hipSuccess) e = dalloc(h, &m.fb_nlen, (size_t)n);
     if (e == hipSuccess) e = dalloc(h, &m.fb_nbr, (size_t)FB_NBR_MAX * n);
+    ALLOC_FILL(m.fb_done, (int32_t)-1, m.n_fbodies);
   }
   if (!body_range.empty()) {
     ALLOC_UP(m.body_range, body_range.data(), body_range.size());
@@ -1827,6 +1829,7 @@ pfg_status pfg_set_step(pfg_handle *h, int64_t step) {
     PFG_TRY(hipMemset(h->m.body_spread, 0xff, sizeof(int32_t) * (size_t)std::max(1, h->m.n_bodies)));
   if (h->m.body_fallback)
     PFG_TRY(hipMemset(h->m.body_fallback, 0xff, sizeof(int32_t) * (size_t)std::max(1, h->m.n_bodies)));
+  if (h->m.fb_done) PFG_TRY(hipMemset(h->m.fb_done, 0xff, sizeof(int32_t) * (size_t)h->m.n_fbodies));
   if (h->row_stamp) {
     size_t rows = 0;
     for (const Inst &I : h->inst) rows += (size_t)I.nxb * I.p.ny * I.p.nz;
@@ -1969,6 +1972,8 @@ hipError_t snapshot_copy(pfg_handle *h, Snapshot &S, bool save) {
       e = hipMemsetAsync(h->m.body_spread, 0xff, 4 * (size_t)std::max(1, h->m.n_bodies), h->stream);
     if (e == hipSuccess && h->m.body_fallback)
       e = hipMemsetAsync(h->m.body_fallback, 0xff, 4 * (size_t)std::max(1, h->m.n_bodies), h->stream);
+    if (e == hipSuccess && h->m.fb_done)
+      e = hipMemsetAsync(h->m.fb_done, 0xff, 4 * (size_t)h->m.n_fbodies, h->stream);
     if (e == hipSuccess && h->row_stamp) {
       size_t rows = 0;
       for (const Inst &I : h->inst) rows += (size_t)I.nxb * I.p.ny * I.p.nz;

@@ -1536,7 +1536,7 @@ __global__ __launch_bounds__(256) void forces_kernel(MArgs a) {
  * - Each element then sums the Morse pairs of its list from the positions
  *   in LDS (the pair test and FP32 distance of the per-marker form, so the
  *   same cutoff decisions), adds its other terms and writes its force. */
-__global__ __launch_bounds__(BODY_THREADS) void forces_body_kernel(MArgs a) {
+__global__ __launch_bounds__(BODY_THREADS) void forces_body_build_kernel(MArgs a) {
   constexpr int NT = BODY_THREADS, NW = NT / 64, PER = (FB_MAX + NT - 1) / NT;
   __shared__ float4 sp[FB_MAX];          /* positions, index order           */
   __shared__ int16_t sidx[FB_MAX];       /* elements in cell order           */
@@ -1739,7 +1739,10 @@ __global__ __launch_bounds__(BODY_THREADS) void forces_body_kernel(MArgs a) {
         a.m.fb_ref[i0 + j] = overflow ? make_float4(NAN, NAN, NAN, NAN) : pr[k];
     }
   }
-  const bool from_cells = rebuild && sflag[1] != 0;
+  /* a body whose lists overflowed sums its forces here, from the cell list,
+   * and the list pass leaves it alone this step */
+  if (!(rebuild && sflag[1] != 0)) return;
+  if (t == 0) a.m.fb_done[g] = (int32_t)*a.m.d_step;
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int j = t + NT * k;
@@ -1749,20 +1752,7 @@ __global__ __launch_bounds__(BODY_THREADS) void forces_body_kernel(MArgs a) {
     const float xi = pi.x, yi = pi.y, zi = pi.z;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     if (morse) {
-      if (!from_cells) {
-        /* eight list entries are read at a time, then their pairs summed in
-         * list order */
-        const int len = a.m.fb_nlen[i];
-        for (int q0 = 0; q0 < len; q0 += 8) {
-          int16_t nb[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u)
-            nb[u] = q0 + u < len ? a.m.fb_nbr[(int64_t)(q0 + u) * n + i] : (int16_t)-1;
-#pragma unroll
-          for (int u = 0; u < 8; ++u)
-            if (nb[u] >= 0) morse_pair(a, sp[nb[u]], pi.w, xi, yi, zi, fx, fy, fz);
-        }
-      } else {
+      {
         const int bx = min(max((int)((xi - lo[0]) * inv), 0), d[0] - 1);
         const int by = min(max((int)((yi - lo[1]) * inv), 0), d[1] - 1);
         const int bz = min(max((int)((zi - lo[2]) * inv), 0), d[2] - 1);
@@ -1773,6 +1763,45 @@ __global__ __launch_bounds__(BODY_THREADS) void forces_body_kernel(MArgs a) {
             const int s0 = c0 == 0 ? 0 : cend[c0 - 1], s1 = cend[c1];
             for (int c = s0; c < s1; ++c) morse_pair(a, sp[sidx[c]], pi.w, xi, yi, zi, fx, fy, fz);
           }
+      }
+    }
+    element_terms<1>(a, i, 0, xi, yi, zi, a.m.body[i], fx, fy, fz);
+    a.m.fx[i] = fx;
+    a.m.fy[i] = fy;
+    a.m.fz[i] = fz;
+  }
+}
+
+/* The list pass of the force pass by body: a workgroup of 256 lanes per
+ * body, after forces_body_build_kernel has rebuilt the lists it had to.
+ * The body's positions go to LDS; each element sums the Morse pairs of
+ * its list (eight list entries read at a time, the pairs summed in list
+ * order), adds its other terms and writes its force.  A body whose lists
+ * overflowed this step was summed by the build kernel. */
+__global__ __launch_bounds__(256) void forces_body_kernel(MArgs a) {
+  __shared__ float4 sp[FB_MAX];
+  const int t = threadIdx.x, g = blockIdx.x;
+  if (a.m.fb_done[g] == (int32_t)*a.m.d_step) return;
+  const int i0 = a.m.fbody_range[2 * g], cnt = a.m.fbody_range[2 * g + 1] - i0;
+  const int64_t n = a.m.n;
+  const bool morse = a.morse_u0 != 0.f && a.morse_cut > 0.f;
+  for (int j = t; j < cnt; j += 256) sp[j] = a.m.pos4[i0 + j];
+  __syncthreads();
+  for (int j = t; j < cnt; j += 256) {
+    const int i = i0 + j;
+    const float4 pi = sp[j];
+    const float xi = pi.x, yi = pi.y, zi = pi.z;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (morse) {
+      const int len = a.m.fb_nlen[i];
+      for (int q0 = 0; q0 < len; q0 += 8) {
+        int16_t nb[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          nb[u] = q0 + u < len ? a.m.fb_nbr[(int64_t)(q0 + u) * n + i] : (int16_t)-1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (nb[u] >= 0) morse_pair(a, sp[nb[u]], pi.w, xi, yi, zi, fx, fy, fz);
       }
     }
     element_terms<1>(a, i, 0, xi, yi, zi, a.m.body[i], fx, fy, fz);
@@ -2002,7 +2031,7 @@ __device__ inline void stencil_origin(const MLat &L, const float v[3], const flo
  * alone, marker after marker, in the two boxes' memory taken as one.  A scattered chunk (fibrin nodes far apart)
  * adds each marker's 64 nodes with device-wide atomics, as the direct form
  * does; so do the markers listed for direct spreading, IB_CHUNK per
- * workgroup after the chunks, staged the same way (each with its own
+ * workgroup before the chunks, staged the same way (each with its own
  * lattice).  Contributions are the direct form's bit for bit; only the
  * order in which they are added differs. */
 __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
@@ -2015,13 +2044,17 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   __shared__ int sidx[IB_CHUNK];
   if (halted(a)) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-  const bool chunk = (int)blockIdx.x < a.m.n_chunks;
+  /* the workgroups of the direct list come first, so that their atomics
+   * run beside the chunks' summation instead of after it */
+  const int ndirect = (a.m.n_direct + IB_CHUNK - 1) / IB_CHUNK;
+  const bool chunk = (int)blockIdx.x >= ndirect;
+  const int cid = (int)blockIdx.x - ndirect; /* the chunk */
   int c0, cnt;
   if (chunk) {
-    c0 = a.m.chunk_start[blockIdx.x];
-    cnt = a.m.chunk_start[blockIdx.x + 1] - c0;
+    c0 = a.m.chunk_start[cid];
+    cnt = a.m.chunk_start[cid + 1] - c0;
   } else { /* IB_CHUNK markers of the direct list (of any lattices) */
-    const int k0 = IB_CHUNK * ((int)blockIdx.x - a.m.n_chunks);
+    const int k0 = IB_CHUNK * (int)blockIdx.x;
     c0 = a.m.chunk_start[a.m.n_chunks] + k0;
     cnt = min(IB_CHUNK, a.m.n_direct - k0);
   }
@@ -2031,8 +2064,8 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
    * a chunk's stencils are placed in the periodic image nearest to its
    * reference marker (for a chunk of a body the body's first marker, the
    * frame of the body's box) */
-  const int cbody = chunk && a.spread_scratch ? a.m.chunk_body[blockIdx.x] : -1;
-  const int m0 = chunk ? a.m.chunk_ref[blockIdx.x] : items[0];
+  const int cbody = chunk && a.spread_scratch ? a.m.chunk_body[cid] : -1;
+  const int m0 = chunk ? a.m.chunk_ref[cid] : items[0];
   const MLat L = lat_of(a, m0); /* a chunk never spans two lattices */
   float ref[3];
   ref[0] = wrap_coord(a.m.x[m0], L.nx, L.periodic[0]);
@@ -2100,8 +2133,8 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
      * marker, or a box too large, spread with atomics (the body's nodes then
      * all go with atomics) */
     const bool boxed = lo[0] <= hi[0] && vol64 <= 2 * IB_BOX_CELLS;
-    a.m.chunk_info[2 * blockIdx.x] = make_int4(lo[0], lo[1], lo[2], 0);
-    a.m.chunk_info[2 * blockIdx.x + 1] =
+    a.m.chunk_info[2 * cid] = make_int4(lo[0], lo[1], lo[2], 0);
+    a.m.chunk_info[2 * cid + 1] =
         boxed ? make_int4(dim0, dim1, dim2, (int)vol64) : make_int4(0, 0, 0, 0);
     if (lo[0] <= hi[0] && !boxed) a.m.body_fallback[cbody] = step;
   }
@@ -2158,7 +2191,7 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   }
   __syncthreads();
   if (cbody >= 0) { /* the chunk's sums to its scratch box, zeros included */
-    float *out = a.m.chunk_sum + (int64_t)blockIdx.x * (6 * IB_BOX_CELLS);
+    float *out = a.m.chunk_sum + (int64_t)cid * (6 * IB_BOX_CELLS);
     for (int k = t; k < 3 * vol; k += 128) out[k] = wide ? box[k] : box[k] + box[3 * IB_BOX_CELLS + k];
     return;
   }
@@ -2306,6 +2339,7 @@ __device__ void body_box(const MArgs &a, int i0, int cnt, const MLat &L, const f
 struct BoxRuns {
   uint8_t run[3][BODY_BOX_DIM];
   int16_t tile[3][BODY_BOX_DIM];
+  int16_t first[3][BODY_BOX_DIM + 1]; /* first box node of each run, then dim */
   int nrun[3];
 };
 
@@ -2322,11 +2356,13 @@ __device__ void box_runs(const MLat &L, const int lo[3], const int dim[3], BoxRu
       if (b == 0 || tt != prev) {
         ++r;
         R.tile[ax][r] = (int16_t)tt;
+        R.first[ax][r] = (int16_t)b;
         prev = tt;
       }
       R.run[ax][b] = (uint8_t)r;
     }
     R.nrun[ax] = r + 1;
+    R.first[ax][r + 1] = (int16_t)dim[ax];
   }
   __syncthreads();
 }
@@ -2500,15 +2536,31 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
     return;
   }
   const uint64_t mine = owner_word(step, (uint32_t)g);
-  for (int lt = t; lt < nt; lt += NT) { /* stamps and ownership of the box's tiles */
+  /* stamps and ownership of the box's tiles: a tile is stamped when one of
+   * the chunks' boxes (which hold all stencils) meets it, so the corners of
+   * the platelet's box that no stencil reaches stay out of the LB update's
+   * IB work */
+  for (int lt = t; lt < nt; lt += NT) {
     const int rx = lt % R.nrun[0], r2 = lt / R.nrun[0];
     const int ry = r2 % R.nrun[1], rz = r2 / R.nrun[1];
     const int tx = R.tile[0][rx], ty = R.tile[1][ry], tz = R.tile[2][rz];
     bool own = false;
     if (tx >= 0 && ty >= 0 && tz >= 0) {
-      const int64_t gt = tx + (int64_t)L.tnx * (ty + (int64_t)L.tny * tz);
-      L.tile_stamp[gt] = (int32_t)step;
-      own = may_store && L.tile_owner[gt] == mine;
+      /* the tile's box nodes, in the frame of the chunks' boxes */
+      const int x0 = lo[0] + R.first[0][rx], x1 = lo[0] + R.first[0][rx + 1] - 1;
+      const int y0 = lo[1] + R.first[1][ry], y1 = lo[1] + R.first[1][ry + 1] - 1;
+      const int z0 = lo[2] + R.first[2][rz], z1 = lo[2] + R.first[2][rz + 1] - 1;
+      bool met = false;
+      for (int q = 0; q < nc && !met; ++q) {
+        const int4 l = cinfo[q][0], d = cinfo[q][1];
+        met = d.w > 0 && x0 < l.x + d.x && l.x <= x1 && y0 < l.y + d.y && l.y <= y1 &&
+              z0 < l.z + d.z && l.z <= z1;
+      }
+      if (met) {
+        const int64_t gt = tx + (int64_t)L.tnx * (ty + (int64_t)L.tny * tz);
+        L.tile_stamp[gt] = (int32_t)step;
+        own = may_store && L.tile_owner[gt] == mine;
+      }
     }
     owned[lt] = own ? 1 : 0;
   }
@@ -3482,9 +3534,11 @@ hipError_t launch_forces(pfg_handle *h, bool spread) {
     case 32: hipLaunchKernelGGL(forces_kernel<32>, dim3(grid), dim3(256), 0, h->stream, a); break;
     default: hipLaunchKernelGGL(forces_kernel<64>, dim3(grid), dim3(256), 0, h->stream, a); break;
   }
-  if (by_body)
-    hipLaunchKernelGGL(forces_body_kernel, dim3(h->m.n_fbodies), dim3(BODY_THREADS), 0,
+  if (by_body) { /* lists rebuilt where needed, then the list pass */
+    hipLaunchKernelGGL(forces_body_build_kernel, dim3(h->m.n_fbodies), dim3(BODY_THREADS), 0,
                        h->stream, a);
+    hipLaunchKernelGGL(forces_body_kernel, dim3(h->m.n_fbodies), dim3(256), 0, h->stream, a);
+  }
   if (h->m.bend)
     hipLaunchKernelGGL(bend_force_kernel, dim3(blocks_for((int64_t)h->m.n * 4, 256)),
                        dim3(256), 0, h->stream, a);
@@ -3525,8 +3579,8 @@ hipError_t launch_spread(pfg_handle *h) {
     hipLaunchKernelGGL(spread_body_kernel, dim3(h->m.n_bodies), dim3(256), 0, h->stream, a);
     return hipGetLastError();
   }
-  /* one workgroup per chunk, then one per IB_CHUNK markers listed for
-   * direct spreading, in a single launch */
+  /* one workgroup per IB_CHUNK markers listed for direct spreading, then
+   * one per chunk, in a single launch */
   hipLaunchKernelGGL(spread_chunk_kernel,
                      dim3(h->m.n_chunks + (h->m.n_direct + IB_CHUNK - 1) / IB_CHUNK),
                      dim3(128), 0, h->stream, a);

@@ -272,6 +272,8 @@ struct Markers {
   float4 *fb_ref = nullptr;
   int16_t *fb_nlen = nullptr;
   int16_t *fb_nbr = nullptr;
+  int32_t *fb_done = nullptr;    /* per body: the step whose forces the build
+                                    kernel summed (its lists overflowed)     */
   SpreadRec *spread_rec = nullptr; /* per cell-list slot: the marker's record */
   int tile_gather_ok = 0;
   /* platelet-platelet contact: per platelet pair (lo * n_plt + hi) the best
