@@ -1352,11 +1352,38 @@ __device__ inline void for_nbr_ranges(const MLat &L, const int32_t *starts, cons
   }
 }
 
-/* Sum over an aligned group of G lanes. */
+/* v of lane (i XOR s), read with a DPP lane move instead of a trip through
+ * the LDS crossbar.  Groups of 4 lanes: the quad permutation is the XOR
+ * itself.  Groups of 16 (one DPP row), with s taken in the order 8, 4, 2, 1
+ * of a butterfly sum: rotating the row by s gives lane (i - s) mod 16, which
+ * is lane i XOR s for s = 8 and, for the smaller s, a lane holding the same
+ * partial sum as lane i XOR s (after the steps above s the partial sums
+ * repeat with period 2s, each computed as a commutative pair), so the sums
+ * are bit for bit the XOR butterfly's. */
+template <int G, int S>
+__device__ inline float dpp_partner(float v) {
+  static_assert((G == 4 && (S == 2 || S == 1)) || (G == 16 && S >= 1 && S <= 8), "DPP group");
+  constexpr int ctrl = G == 4 ? (S == 2 ? 0x4E /* quad_perm 2,3,0,1 */ : 0xB1 /* 1,0,3,2 */)
+                              : 0x120 + S /* row_ror:S */;
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+}
+
+/* Sum over an aligned group of G lanes (XOR butterfly, largest stride
+ * first); every lane of the group gets the sum. */
 template <int G>
 __device__ inline float group_sum(float v) {
+  if constexpr (G == 4) {
+    v += dpp_partner<4, 2>(v);
+    v += dpp_partner<4, 1>(v);
+  } else if constexpr (G == 16) {
+    v += dpp_partner<16, 8>(v);
+    v += dpp_partner<16, 4>(v);
+    v += dpp_partner<16, 2>(v);
+    v += dpp_partner<16, 1>(v);
+  } else {
 #pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  }
   return v;
 }
 
@@ -2028,24 +2055,29 @@ __device__ inline void stencil_origin(const MLat &L, const float v[3], const flo
  * then second) and go to the lattice field with one device-wide atomic per
  * nonzero node and component.  A box of up to twice IB_BOX_CELLS nodes (a
  * chunk that rotated or deformed since load) is summed by wavefront 0
- * alone, marker after marker, in the two boxes' memory taken as one.  A scattered chunk (fibrin nodes far apart)
- * adds each marker's 64 nodes with device-wide atomics, as the direct form
- * does; so do the markers listed for direct spreading, IB_CHUNK per
- * workgroup before the chunks, staged the same way (each with its own
- * lattice).  Contributions are the direct form's bit for bit; only the
+ * alone, marker after marker, in the two boxes' memory taken as one.  A
+ * scattered chunk (fibrin nodes far apart) adds each marker's 64 nodes with
+ * device-wide atomics, as the direct form does; so do the markers listed
+ * for direct spreading, IB_CHUNK per workgroup before the chunks', staged
+ * the same way (each with its own lattice).  Contributions are the direct form's bit for bit; only the
  * order in which they are added differs. */
 __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
   static_assert(IB_CHUNK <= 128, "a thread per marker in the staging pass");
   __shared__ int4 so[IB_CHUNK];      /* first stencil node; x = INT_MIN: not coupled */
   __shared__ float sw[IB_CHUNK][12]; /* weights along x (0..3), y (4..7), z (8..11) */
   __shared__ float sf[IB_CHUNK][3];
-  __shared__ float box[6 * IB_BOX_CELLS]; /* a box per wavefront, or one wide box */
+  /* a box per wavefront (wavefront w's at w * IB_BOX_CELLS), or one wide
+   * box; the three components in arrays of their own, so that a marker's
+   * three read-add-writes are independent */
+  __shared__ float bxs[2 * IB_BOX_CELLS], bys[2 * IB_BOX_CELLS], bzs[2 * IB_BOX_CELLS];
   __shared__ int sh[2][6];
   __shared__ int sidx[IB_CHUNK];
   if (halted(a)) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   /* the workgroups of the direct list come first, so that their atomics
-   * run beside the chunks' summation instead of after it */
+   * run beside the chunks' summation instead of after it (spreading them
+   * evenly among the chunks' workgroups measured no faster: the atomics and
+   * the chunks' summation do not overlap either way) */
   const int ndirect = (a.m.n_direct + IB_CHUNK - 1) / IB_CHUNK;
   const bool chunk = (int)blockIdx.x >= ndirect;
   const int cid = (int)blockIdx.x - ndirect; /* the chunk */
@@ -2101,6 +2133,10 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
         lo[k] = o[k];
         hi[k] = o[k] + 3;
       }
+    } else { /* summed into the box as zeros, without a branch */
+#pragma unroll
+      for (int k = 0; k < 12; ++k) sw[t][k] = 0.f;
+      sf[t][0] = sf[t][1] = sf[t][2] = 0.f;
     }
     so[t] = make_int4(has ? o[0] : INT_MIN, o[1], o[2], 0);
   }
@@ -2165,34 +2201,42 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
     return;
   }
   const int vol = (int)vol64;
-  for (int k = t; k < 3 * vol; k += 128) {
-    box[k] = 0.f;
-    if (!wide) box[3 * IB_BOX_CELLS + k] = 0.f;
+  for (int k = t; k < vol; k += 128) {
+    bxs[k] = 0.f;
+    bys[k] = 0.f;
+    bzs[k] = 0.f;
+    if (!wide) {
+      bxs[IB_BOX_CELLS + k] = 0.f;
+      bys[IB_BOX_CELLS + k] = 0.f;
+      bzs[IB_BOX_CELLS + k] = 0.f;
+    }
   }
-  /* each marker's first box node (-1: not coupled), and each lane's offset
-   * of its stencil node from it */
+  /* each marker's first box node (0 for a marker not coupled: its weights
+   * and force are zero), and each lane's offset of its stencil node from it */
   if (t < cnt) {
     const int4 oj = so[t];
-    so[t].w = oj.x == INT_MIN ? -1 : (oj.x - lo[0]) + dim0 * ((oj.y - lo[1]) + dim1 * (oj.z - lo[2]));
+    so[t].w = oj.x == INT_MIN ? 0 : (oj.x - lo[0]) + dim0 * ((oj.y - lo[1]) + dim1 * (oj.z - lo[2]));
   }
   const int off = kx + dim0 * (ky + dim1 * kz);
   __syncthreads();
-  float *mybox = box + (wide ? 0 : wave * 3 * IB_BOX_CELLS);
+  const int boff = wide ? 0 : wave * IB_BOX_CELLS;
   const int j0 = wide ? 0 : 64 * wave, j1 = wide ? (wave == 0 ? cnt : 0) : min(cnt, 64 * (wave + 1));
 #pragma unroll 4
   for (int j = j0; j < j1; ++j) {
-    const int base = so[j].w;
-    if (base < 0) continue;
     const float w = sw[j][kx] * sw[j][4 + ky] * sw[j][8 + kz] * a.force_scale;
-    const int c = base + off;
-    mybox[c] += w * sf[j][0];
-    mybox[vol + c] += w * sf[j][1];
-    mybox[2 * vol + c] += w * sf[j][2];
+    const int c = boff + so[j].w + off;
+    bxs[c] += w * sf[j][0];
+    bys[c] += w * sf[j][1];
+    bzs[c] += w * sf[j][2];
   }
   __syncthreads();
-  if (cbody >= 0) { /* the chunk's sums to its scratch box, zeros included */
+  if (cbody >= 0) { /* the chunk's sums to its scratch box (x, y, z planes of vol), zeros included */
     float *out = a.m.chunk_sum + (int64_t)cid * (6 * IB_BOX_CELLS);
-    for (int k = t; k < 3 * vol; k += 128) out[k] = wide ? box[k] : box[k] + box[3 * IB_BOX_CELLS + k];
+    for (int k = t; k < vol; k += 128) {
+      out[k] = wide ? bxs[k] : bxs[k] + bxs[IB_BOX_CELLS + k];
+      out[vol + k] = wide ? bys[k] : bys[k] + bys[IB_BOX_CELLS + k];
+      out[2 * vol + k] = wide ? bzs[k] : bzs[k] + bzs[IB_BOX_CELLS + k];
+    }
     return;
   }
   const float inv0 = 1.0f / (float)dim0, inv1 = 1.0f / (float)dim1; /* vol <= 2 IB_BOX_CELLS */
@@ -2213,10 +2257,9 @@ __global__ __launch_bounds__(128) void spread_chunk_kernel(MArgs a) {
         (bz == 0 || (gz & (TILE_Z - 1)) == 0)) /* few per chunk: stored without a read */
       L.tile_stamp[(gx >> TILE_SHIFT_X) + L.tnx * ((gy >> TILE_SHIFT_Y) + L.tny * (gz >> TILE_SHIFT_Z))] = step;
     const int64_t c = gx + (int64_t)L.nx * (gy + (int64_t)L.ny * gz);
-    const float *b1 = box + 3 * IB_BOX_CELLS;
-    const float fx = wide ? box[k] : box[k] + b1[k];
-    const float fy = wide ? box[vol + k] : box[vol + k] + b1[vol + k];
-    const float fz = wide ? box[2 * vol + k] : box[2 * vol + k] + b1[2 * vol + k];
+    const float fx = wide ? bxs[k] : bxs[k] + bxs[IB_BOX_CELLS + k];
+    const float fy = wide ? bys[k] : bys[k] + bys[IB_BOX_CELLS + k];
+    const float fz = wide ? bzs[k] : bzs[k] + bzs[IB_BOX_CELLS + k];
     if (fx != 0.f) atomicAdd(L.aux + c, fx);
     if (fy != 0.f) atomicAdd(L.aux + a.plane + c, fy);
     if (fz != 0.f) atomicAdd(L.aux + 2 * a.plane + c, fz);
@@ -2451,6 +2494,8 @@ __device__ void claim_role(const MArgs &a, int block) {
 __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
   __shared__ BoxRuns R;
   __shared__ uint8_t owned[BODY_BOX_TILES];
+  /* per tile of the box, the chunks (of the first 32) whose boxes meet it */
+  __shared__ uint32_t tmask[BODY_BOX_TILES];
   __shared__ int4 cinfo[BODY_MAX_CHUNKS][2];
   constexpr int NT = 256, NW = NT / 64;
   __shared__ int shb[NW][6];
@@ -2545,16 +2590,19 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
     const int ry = r2 % R.nrun[1], rz = r2 / R.nrun[1];
     const int tx = R.tile[0][rx], ty = R.tile[1][ry], tz = R.tile[2][rz];
     bool own = false;
+    uint32_t mask = 0;
     if (tx >= 0 && ty >= 0 && tz >= 0) {
       /* the tile's box nodes, in the frame of the chunks' boxes */
       const int x0 = lo[0] + R.first[0][rx], x1 = lo[0] + R.first[0][rx + 1] - 1;
       const int y0 = lo[1] + R.first[1][ry], y1 = lo[1] + R.first[1][ry + 1] - 1;
       const int z0 = lo[2] + R.first[2][rz], z1 = lo[2] + R.first[2][rz + 1] - 1;
       bool met = false;
-      for (int q = 0; q < nc && !met; ++q) {
+      for (int q = 0; q < nc; ++q) {
         const int4 l = cinfo[q][0], d = cinfo[q][1];
-        met = d.w > 0 && x0 < l.x + d.x && l.x <= x1 && y0 < l.y + d.y && l.y <= y1 &&
-              z0 < l.z + d.z && l.z <= z1;
+        const bool m = d.w > 0 && x0 < l.x + d.x && l.x <= x1 && y0 < l.y + d.y &&
+                       l.y <= y1 && z0 < l.z + d.z && l.z <= z1;
+        met = met || m;
+        if (m && q < 32) mask |= 1u << q;
       }
       if (met) {
         const int64_t gt = tx + (int64_t)L.tnx * (ty + (int64_t)L.tny * tz);
@@ -2563,17 +2611,61 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
       }
     }
     owned[lt] = own ? 1 : 0;
+    tmask[lt] = mask;
   }
   __syncthreads();
   const int vol = dim[0] * dim[1] * dim[2];
+  /* node k = bx + dim0 (by + dim1 bz), stepped by NT without divisions */
+  const int sx = NT % dim[0], sq = NT / dim[0], sy = sq % dim[1], sz = sq / dim[1];
+  int nbx = t % dim[0], nby = (t / dim[0]) % dim[1], nbz = t / dim[0] / dim[1];
   for (int k = t; k < vol; k += NT) {
-    const int r = k / dim[0], bx = k - r * dim[0];
-    const int bz = r / dim[1], by = r - bz * dim[1];
+    const int bx = nbx, by = nby, bz = nbz;
+    nbx += sx;
+    nby += sy;
+    nbz += sz;
+    if (nbx >= dim[0]) {
+      nbx -= dim[0];
+      ++nby;
+    }
+    if (nby >= dim[1]) {
+      nby -= dim[1];
+      ++nbz;
+    }
     const int ux = lo[0] + bx, uy = lo[1] + by, uz = lo[2] + bz;
+    const int lt = R.run[0][bx] + R.nrun[0] * (R.run[1][by] + R.nrun[1] * R.run[2][bz]);
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    /* eight chunks at a time: their loads are issued together, then summed
-     * in chunk order (a chunk whose box does not hold the node adds 0) */
-    for (int q0 = 0; q0 < nc; q0 += 8) {
+    /* the chunks whose boxes meet the node's tile (all chunks when there are
+     * more than 32), four or eight at a time: their loads are issued
+     * together, then summed in chunk order; a chunk whose box does not hold
+     * the node adds nothing (+0, which leaves a sum begun at +0 unchanged) */
+    for (uint32_t mask = nc <= 32 ? tmask[lt] : 0u; mask != 0u;) {
+      float vx[4], vy[4], vz[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        vx[u] = vy[u] = vz[u] = 0.f;
+        if (mask == 0u) continue;
+        const int q = __builtin_ctz(mask);
+        mask &= mask - 1u;
+        const int4 d = cinfo[q][1];
+        const int4 l = cinfo[q][0];
+        const int px = ux - l.x, py = uy - l.y, pz = uz - l.z;
+        if ((unsigned)px >= (unsigned)d.x || (unsigned)py >= (unsigned)d.y ||
+            (unsigned)pz >= (unsigned)d.z)
+          continue;
+        const float *src = a.m.chunk_sum + (int64_t)(c0 + q) * (6 * IB_BOX_CELLS);
+        const int p = px + d.x * (py + d.y * pz);
+        vx[u] = src[p];
+        vy[u] = src[d.w + p];
+        vz[u] = src[2 * d.w + p];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        fx += vx[u];
+        fy += vy[u];
+        fz += vz[u];
+      }
+    }
+    for (int q0 = 0; q0 < (nc > 32 ? nc : 0); q0 += 8) {
       float vx[8], vy[8], vz[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -2605,7 +2697,6 @@ __global__ __launch_bounds__(256) void spread_body_kernel(MArgs a) {
     const int gz = axis_index(uz, L.nz, L.periodic[2]);
     if (gx < 0 || gy < 0 || gz < 0) continue;
     const int64_t c = gx + (int64_t)L.nx * (gy + (int64_t)L.ny * gz);
-    const int lt = R.run[0][bx] + R.nrun[0] * (R.run[1][by] + R.nrun[1] * R.run[2][bz]);
     if (owned[lt]) {
       if (fx != 0.f) L.aux[c] = fx;
       if (fy != 0.f) L.aux[a.plane + c] = fy;
@@ -2930,12 +3021,9 @@ __device__ void interp_marker(const MArgs &a, int i, int j, float *v) {
       uz += w * v.z;
     }
   }
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) {
-    ux += __shfl_xor(ux, o, 16);
-    uy += __shfl_xor(uy, o, 16);
-    uz += __shfl_xor(uz, o, 16);
-  }
+  ux = group_sum<16>(ux);
+  uy = group_sum<16>(uy);
+  uz = group_sum<16>(uz);
   v[0] = ux;
   v[1] = uy;
   v[2] = uz;
@@ -3085,20 +3173,19 @@ __device__ inline void interp_staged(const MLat &L, int4 o, const float *w, int 
       uz += ww * u.z;
     }
   }
-#pragma unroll
-  for (int s = 8; s > 0; s >>= 1) {
-    ux += __shfl_xor(ux, s, 16);
-    uy += __shfl_xor(uy, s, 16);
-    uz += __shfl_xor(uz, s, 16);
-  }
+  ux = group_sum<16>(ux);
+  uy = group_sum<16>(uy);
+  uz = group_sum<16>(uz);
   v[0] = ux;
   v[1] = uy;
   v[2] = uz;
 }
 
 /* Chunk-staged interpolation (ROLE_A, always) and move (ROLE_B): a
- * workgroup of 128 lanes per IB chunk, then one per IB_CHUNK markers listed
- * for direct spreading.  Every global read of a marker (index, kind,
+ * workgroup of MC_THREADS lanes per IB chunk, then one per IB_CHUNK markers
+ * listed for direct spreading (the first IB_CHUNK lanes stage the markers;
+ * every lane loads the velocity box and interpolates, a marker per group of
+ * 16 lanes).  Every global read of a marker (index, kind,
  * position) is made once, by a lane per marker, into LDS together with the
  * marker's first stencil node and kernel weights (the staged spreading's),
  * so that the loop over the markers reads only LDS and writes.  When a
@@ -3112,15 +3199,17 @@ __device__ inline void interp_staged(const MLat &L, int4 o, const float *w, int 
  * stencils from the field with the staged weights.  Without the Langevin
  * leapfrog and the fibrin kick, the move x += v dt uses the staged
  * position; otherwise update_marker reads what it needs. */
+constexpr int MC_THREADS = 256;
+static_assert(IB_CHUNK <= MC_THREADS && IB_CHUNK <= 128, "the markers fill the first two wavefronts");
 template <bool FAST>
-__global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
+__global__ __launch_bounds__(MC_THREADS, 4) void move_chunk_kernel(MArgs a) {
   __shared__ int4 so[IB_CHUNK];      /* first stencil node (x = INT_MIN: not coupled);
                                         .w: its box node */
   __shared__ float sw[IB_CHUNK][12]; /* weights along x (0..3), y (4..7), z (8..11) */
   __shared__ float4 sp[IB_CHUNK];    /* position; .w: marker index (int bits) */
   __shared__ int skind[IB_CHUNK];
   __shared__ float4 box[IB_BOX_CELLS];
-  __shared__ int sh[2][6];
+  __shared__ int sh[MC_THREADS / 64][6];
   if (halted(a)) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int grp = t >> 4, j16 = t & 15;
@@ -3192,7 +3281,7 @@ __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
   }
   __syncthreads(); /* the staged markers and both halves' bounds */
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 3; ++k) { /* the markers are in the first two wavefronts */
     lo[k] = min(sh[0][k], sh[1][k]);
     hi[k] = max(sh[0][3 + k], sh[1][3 + k]);
   }
@@ -3203,7 +3292,7 @@ __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
   if (boxed) {
     const int d0 = (int)dim0, d1 = (int)dim1, vol = (int)(dim0 * dim1 * dim2);
     const float inv0 = 1.0f / (float)d0, inv1 = 1.0f / (float)d1; /* vol <= IB_BOX_CELLS */
-    for (int k = t; k < vol; k += 128) {
+    for (int k = t; k < vol; k += MC_THREADS) {
       const int r = div_small(k, inv0), bx = k - r * d0;
       const int bz = div_small(r, inv1), by = r - bz * d1;
       const int gx = axis_index(lo[0] + bx, L0.nx, L0.periodic[0]);
@@ -3222,7 +3311,7 @@ __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
   /* FAST: launched without the Langevin leapfrog and the fibrin kick */
   const int jx = j16 & 3, jy = j16 >> 2;
   const int col = boxed ? jx + (int)dim0 * jy : 0, layer = boxed ? (int)(dim0 * dim1) : 0;
-  for (int j = grp; j < cnt; j += 8) {
+  for (int j = grp; j < cnt; j += MC_THREADS / 16) {
     const float4 p = sp[j];
     const int i = __float_as_int(p.w);
     const int kind = skind[j];
@@ -3241,12 +3330,9 @@ __global__ __launch_bounds__(128) void move_chunk_kernel(MArgs a) {
           uy += w * u.y;
           uz += w * u.z;
         }
-#pragma unroll
-        for (int o = 8; o > 0; o >>= 1) {
-          ux += __shfl_xor(ux, o, 16);
-          uy += __shfl_xor(uy, o, 16);
-          uz += __shfl_xor(uz, o, 16);
-        }
+        ux = group_sum<16>(ux);
+        uy = group_sum<16>(uy);
+        uz = group_sum<16>(uz);
         v[0] = ux;
         v[1] = uy;
         v[2] = uz;
@@ -3599,9 +3685,9 @@ hipError_t launch_move(pfg_handle *h, bool interp, bool update) {
      * the many groups hide their loads' latency */
     const dim3 grid(h->m.n_chunks);
     if (h->p.sem_integrator != PFG_SEM_BBK && h->p.fiber_move != PFG_FIBER_KICK)
-      hipLaunchKernelGGL(move_chunk_kernel<true>, grid, dim3(128), 0, h->stream, a);
+      hipLaunchKernelGGL(move_chunk_kernel<true>, grid, dim3(MC_THREADS), 0, h->stream, a);
     else
-      hipLaunchKernelGGL(move_chunk_kernel<false>, grid, dim3(128), 0, h->stream, a);
+      hipLaunchKernelGGL(move_chunk_kernel<false>, grid, dim3(MC_THREADS), 0, h->stream, a);
     if (h->m.n_direct > 0) {
       a.move_list = h->m.chunk_items + h->m.n_chunk_items;
       a.move_count = h->m.n_direct;
