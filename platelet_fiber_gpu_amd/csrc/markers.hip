@@ -1036,12 +1036,91 @@ __device__ inline void ligand_candidate(const MArgs &a, int l, float dx, float d
   }
 }
 
+/* v of lane (i XOR s), read with a DPP lane move instead of a trip through
+ * the LDS crossbar.  Groups of 4 lanes: the quad permutation is the XOR
+ * itself.  Groups of 16 (one DPP row), with s taken in the order 8, 4, 2, 1
+ * of a butterfly sum: rotating the row by s gives lane (i - s) mod 16, which
+ * is lane i XOR s for s = 8 and, for the smaller s, a lane holding the same
+ * partial sum as lane i XOR s (after the steps above s the partial sums
+ * repeat with period 2s, each computed as a commutative pair), so the sums
+ * are bit for bit the XOR butterfly's. */
+template <int G, int S>
+__device__ inline float dpp_partner(float v) {
+  static_assert((G == 4 && (S == 2 || S == 1)) || (G == 16 && S >= 1 && S <= 8), "DPP group");
+  constexpr int ctrl = G == 4 ? (S == 2 ? 0x4E /* quad_perm 2,3,0,1 */ : 0xB1 /* 1,0,3,2 */)
+                              : 0x120 + S /* row_ror:S */;
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+}
+
+/* Lanes per receptor in the bond decisions, receptors per workgroup of 256. */
+constexpr int REC_LANES = 16, REC_PER_BLOCK = 256 / REC_LANES;
+
+/* flatten_ranges for a group of REC_LANES lanes: lane l holds ranges l and
+ * l + 16 of nbr_range (at most NBR_RANGES = 18); the group's offsets and
+ * running ends go to its off/ends; returns the group's candidate count. */
+__device__ inline int group_flatten_ranges(int nr, int s0, int e0, int s1, int e1, int *off,
+                                           int *ends) {
+  static_assert(REC_LANES == 16 && NBR_RANGES <= 2 * REC_LANES, "two ranges per lane");
+  const int gl = threadIdx.x & (REC_LANES - 1);
+  const int len0 = gl < nr ? e0 - s0 : 0;
+  int end0 = len0;
+#pragma unroll
+  for (int d = 1; d < REC_LANES; d <<= 1) {
+    const int v = __shfl_up(end0, d, REC_LANES);
+    if (gl >= d) end0 += v;
+  }
+  const int tot0 = __shfl(end0, REC_LANES - 1, REC_LANES);
+  const int len1 = gl + REC_LANES < nr ? e1 - s1 : 0;
+  const int l16 = __shfl(len1, 0, REC_LANES);
+  const int end1 = tot0 + (gl == 0 ? len1 : l16 + len1); /* ranges 16 and 17 */
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (gl < nr) {
+    off[gl] = s0 - (end0 - len0);
+    ends[gl] = end0;
+  }
+  if (gl + REC_LANES < nr) {
+    off[gl + REC_LANES] = s1 - (end1 - len1);
+    ends[gl + REC_LANES] = end1;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return nr > REC_LANES ? __shfl(end1, nr - REC_LANES - 1, REC_LANES) : tot0;
+}
+
+/* The neighbour ranges of bin b3 for a group of REC_LANES lanes (lane l
+ * takes ranges l and l + 16), flattened into off/ends. */
+__device__ inline int group_ranges(const MLat &L, const int32_t *starts, const int b3[3],
+                                   int *off, int *ends) {
+  const int gl = threadIdx.x & (REC_LANES - 1);
+  int s0 = 0, e0 = 0, s1 = 0, e1 = 0;
+  const int nr = nbr_range(L, starts, b3, gl, &s0, &e0);
+  if (nr > REC_LANES) nbr_range(L, starts, b3, gl + REC_LANES, &s1, &e1);
+  return group_flatten_ranges(nr, s0, e0, s1, e1, off, ends);
+}
+
+/* (d2, id) of lane (i XOR S) within a row of 16 lanes, by row rotation (the
+ * arg-min butterfly's pairs repeat with period 2S after the steps above S,
+ * as in group_sum). */
+template <int S>
+__device__ inline void argmin_partner16(float &d2, int &id) {
+  const float od = dpp_partner<16, S>(d2);
+  const int ol = __builtin_amdgcn_mov_dpp(id, 0x120 + S, 0xF, 0xF, false);
+  if (od < d2 || (od == d2 && ol < id)) {
+    d2 = od;
+    id = ol;
+  }
+}
+
 /* Ruptures of held bonds and formation candidates of free receptors, from the
- * state at the start of the step; one wavefront per receptor, the lanes over
- * the ligands of the neighbouring bins (wall ligands from their bin-ordered
- * packed positions, fibrin ligands from the scatter order of the markers). */
+ * state at the start of the step; a group of REC_LANES lanes per receptor,
+ * the lanes over the ligands of the neighbouring bins (wall ligands from
+ * their bin-ordered packed positions, fibrin ligands from the scatter order
+ * of the markers).  The decision depends on no lane order: the arg-min is
+ * over (distance^2, ligand id). */
 __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & (REC_LANES - 1);
   const int64_t step = *a.m.d_step;
   const float4 rq = a.m.rec_pos[r];
   const float rp[3] = {rq.x, rq.y, rq.z};
@@ -1073,11 +1152,9 @@ __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
   int b3[3];
   bin3_of(L, rp[0], rp[1], rp[2], b3);
   if (a.m.n_wall_lig > 0) {
-    int s = 0, e = 0;
-    const int nr = nbr_range(L, a.m.lg_start, b3, lane, &s, &e);
-    const int total = flatten_ranges(nr, s, e, off, ends);
+    const int total = group_ranges(L, a.m.lg_start, b3, off, ends);
     int q = 0;
-    for (int c = lane; c < total; c += 64) {
+    for (int c = lane; c < total; c += REC_LANES) {
       while (c >= ends[q]) ++q;
       const float4 g = a.m.lg_pos[c + off[q]];
       ligand_candidate(a, __float_as_int(g.w), min_image(g.x - rp[0], L.nx, L.periodic[0]),
@@ -1093,11 +1170,9 @@ __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
     const MLat G = a.bin_elements ? L : coarse_grid(L);
     int g3[3];
     bin3_of(G, rp[0], rp[1], rp[2], g3);
-    int s = 0, e = 0;
-    const int nr = nbr_range(G, a.m.bin_start, g3, lane, &s, &e);
-    const int total = flatten_ranges(nr, s, e, off, ends);
+    const int total = group_ranges(G, a.m.bin_start, g3, off, ends);
     int q = 0;
-    for (int c = lane; c < total; c += 64) {
+    for (int c = lane; c < total; c += REC_LANES) {
       while (c >= ends[q]) ++q;
       const int mk = a.m.bin_scatter[c + off[q]];
       const float4 g = a.m.pos4[mk];
@@ -1108,17 +1183,13 @@ __device__ void bond_decide(const MArgs &a, int r, int *off, int *ends) {
                        near_reach);
     }
   }
-  /* wavefront arg-min on (distance^2, ligand id) */
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const float od = __shfl_xor(best_d2, o, 64);
-    const int ol = __shfl_xor(best, o, 64);
-    if (od < best_d2 || (od == best_d2 && ol < best)) {
-      best_d2 = od;
-      best = ol;
-    }
-  }
-  const bool any_near = __any(near_reach);
+  /* the group's arg-min on (distance^2, ligand id) */
+  argmin_partner16<8>(best_d2, best);
+  argmin_partner16<4>(best_d2, best);
+  argmin_partner16<2>(best_d2, best);
+  argmin_partner16<1>(best_d2, best);
+  const int sh = threadIdx.x & 63 & ~(REC_LANES - 1);
+  const bool any_near = ((__ballot(near_reach) >> sh) & ((1ull << REC_LANES) - 1)) != 0;
   if (lane != 0) return;
   a.m.rec_release[r] = 0;
   a.m.rec_cand[r] = -1;
@@ -1172,9 +1243,9 @@ __device__ void gather_body(const MArgs &a) {
     blk -= a.blocks1;
   }
   if (a.roles & ROLE_C) {
-    __shared__ int sh_off[4][NBR_RANGES], sh_end[4][NBR_RANGES];
-    const int w = threadIdx.x >> 6;
-    const int r = blk * 4 + w;
+    __shared__ int sh_off[REC_PER_BLOCK][NBR_RANGES], sh_end[REC_PER_BLOCK][NBR_RANGES];
+    const int w = threadIdx.x / REC_LANES;
+    const int r = blk * REC_PER_BLOCK + w;
     if (r < a.m.n_rec && !halted(a)) bond_decide(a, r, sh_off[w], sh_end[w]);
   }
 }
@@ -1350,22 +1421,6 @@ __device__ inline void for_nbr_ranges(const MLat &L, const int32_t *starts, cons
       if (runs == 2) f(starts[row + f1], starts[row + l1 + 1]);
     }
   }
-}
-
-/* v of lane (i XOR s), read with a DPP lane move instead of a trip through
- * the LDS crossbar.  Groups of 4 lanes: the quad permutation is the XOR
- * itself.  Groups of 16 (one DPP row), with s taken in the order 8, 4, 2, 1
- * of a butterfly sum: rotating the row by s gives lane (i - s) mod 16, which
- * is lane i XOR s for s = 8 and, for the smaller s, a lane holding the same
- * partial sum as lane i XOR s (after the steps above s the partial sums
- * repeat with period 2s, each computed as a commutative pair), so the sums
- * are bit for bit the XOR butterfly's. */
-template <int G, int S>
-__device__ inline float dpp_partner(float v) {
-  static_assert((G == 4 && (S == 2 || S == 1)) || (G == 16 && S >= 1 && S <= 8), "DPP group");
-  constexpr int ctrl = G == 4 ? (S == 2 ? 0x4E /* quad_perm 2,3,0,1 */ : 0xB1 /* 1,0,3,2 */)
-                              : 0x120 + S /* row_ror:S */;
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
 }
 
 /* Sum over an aligned group of G lanes (XOR butterfly, largest stride
@@ -3525,7 +3580,7 @@ hipError_t launch_gather(pfg_handle *h, bool sort, bool sums, bool decide, bool 
   if (resolved) *resolved = a.resolve_last != 0;
   a.blocks0 = sort ? blocks_for(h->m.n, 256) : 0;
   a.blocks1 = sums ? h->m.n_plt : 0;
-  a.blocks2 = decide ? blocks_for(h->m.n_rec, 4) : 0;
+  a.blocks2 = decide ? blocks_for(h->m.n_rec, REC_PER_BLOCK) : 0;
   const int grid = a.blocks0 + a.blocks1 + a.blocks2;
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(gather_kernel, dim3(grid), dim3(256), 0, h->stream, a);
