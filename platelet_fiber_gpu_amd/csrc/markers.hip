@@ -1631,47 +1631,24 @@ __global__ __launch_bounds__(BODY_THREADS) void forces_body_build_kernel(MArgs a
   const int64_t n = a.m.n;
   const float cut = a.morse_cut, skin = FB_SKIN * a.morse_cut, R = cut + skin;
   const bool morse = a.morse_u0 != 0.f && cut > 0.f;
-  float4 pr[PER];
+  float4 pr[PER], rr[PER];
   float dsp[3] = {0.f, 0.f, 0.f};
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   bool bad = false; /* a reference position that is not a number */
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int j = t + NT * k;
     pr[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    rr[k] = pr[k];
     if (j < cnt) {
       pr[k] = a.m.pos4[i0 + j];
       sp[j] = pr[k];
-      lo[0] = fminf(lo[0], pr[k].x);
-      lo[1] = fminf(lo[1], pr[k].y);
-      lo[2] = fminf(lo[2], pr[k].z);
-      hi[0] = fmaxf(hi[0], pr[k].x);
-      hi[1] = fmaxf(hi[1], pr[k].y);
-      hi[2] = fmaxf(hi[2], pr[k].z);
       if (morse) {
-        const float4 r0 = a.m.fb_ref[i0 + j];
-        dsp[0] += pr[k].x - r0.x;
-        dsp[1] += pr[k].y - r0.y;
-        dsp[2] += pr[k].z - r0.z;
-        bad = bad || !(r0.x == r0.x);
+        rr[k] = a.m.fb_ref[i0 + j];
+        dsp[0] += pr[k].x - rr[k].x;
+        dsp[1] += pr[k].y - rr[k].y;
+        dsp[2] += pr[k].z - rr[k].z;
+        bad = bad || !(rr[k].x == rr[k].x);
       }
-    }
-  }
-  /* the box, and the sum of the displacements (fixed order: lanes, then
-   * wavefronts) */
-#pragma unroll
-  for (int s2 = 32; s2 > 0; s2 >>= 1) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = fminf(lo[k], __shfl_xor(lo[k], s2, 64));
-      hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], s2, 64));
-    }
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      shf[wave][k] = lo[k];
-      shf[wave][3 + k] = hi[k];
     }
   }
   for (int b = t; b <= FB_BINS; b += NT) cend[b] = 0;
@@ -1681,16 +1658,7 @@ __global__ __launch_bounds__(BODY_THREADS) void forces_body_build_kernel(MArgs a
   }
   __syncthreads();
   if (cnt <= 0) return;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    lo[k] = shf[0][k];
-    hi[k] = shf[0][3 + k];
-    for (int w = 1; w < NW; ++w) {
-      lo[k] = fminf(lo[k], shf[w][k]);
-      hi[k] = fmaxf(hi[k], shf[w][3 + k]);
-    }
-  }
-  __syncthreads(); /* shf is used again */
+  /* the sum of the displacements (fixed order: lanes, then wavefronts) */
   bool rebuild = false;
   if (morse) {
 #pragma unroll
@@ -1714,15 +1682,53 @@ __global__ __launch_bounds__(BODY_THREADS) void forces_body_build_kernel(MArgs a
     for (int k = 0; k < PER; ++k) {
       const int j = t + NT * k;
       if (j >= cnt) continue;
-      const float4 r0 = a.m.fb_ref[i0 + j];
-      const float ex = pr[k].x - r0.x - mean[0], ey = pr[k].y - r0.y - mean[1],
-                  ez = pr[k].z - r0.z - mean[2];
+      const float ex = pr[k].x - rr[k].x - mean[0], ey = pr[k].y - rr[k].y - mean[1],
+                  ez = pr[k].z - rr[k].z - mean[2];
       /* skin / 2 less 2 %, a margin over FP32 rounding in these distances */
       far = far || !(ex * ex + ey * ey + ez * ez <= 0.2401f * skin * skin);
     }
     if (__any(far) && lane == 0) sflag[0] = 1;
     __syncthreads();
     rebuild = sflag[0] != 0;
+  }
+  /* the platelet's box, for a build only (the decision is the workgroup's) */
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (rebuild) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      if (t + NT * k >= cnt) continue;
+      lo[0] = fminf(lo[0], pr[k].x);
+      lo[1] = fminf(lo[1], pr[k].y);
+      lo[2] = fminf(lo[2], pr[k].z);
+      hi[0] = fmaxf(hi[0], pr[k].x);
+      hi[1] = fmaxf(hi[1], pr[k].y);
+      hi[2] = fmaxf(hi[2], pr[k].z);
+    }
+#pragma unroll
+    for (int s2 = 32; s2 > 0; s2 >>= 1) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = fminf(lo[k], __shfl_xor(lo[k], s2, 64));
+        hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], s2, 64));
+      }
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        shf[wave][k] = lo[k];
+        shf[wave][3 + k] = hi[k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = shf[0][k];
+      hi[k] = shf[0][3 + k];
+      for (int w = 1; w < NW; ++w) {
+        lo[k] = fminf(lo[k], shf[w][k]);
+        hi[k] = fmaxf(hi[k], shf[w][3 + k]);
+      }
+    }
   }
   /* a cell list: for a build, or to sum from it when a list overflows */
   float cs = 0.f, inv = 0.f;
